@@ -1,0 +1,126 @@
+"""ctypes binding of the C ABI in ``include/gpumatfun.h``.
+
+The shared library ``libgpumatfun_amd.so`` is built in-tree by
+``gpumatfun_amd/build.py``.  There is no fallback: if the library is missing
+or cannot be loaded, every entry point raises.
+"""
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpumatfun_amd.so")
+
+GM_OK, GM_EINVAL, GM_ENOMEM, GM_EDEVICE, GM_EBREAKDOWN, GM_EBADHANDLE, GM_ESTATE = range(7)
+GM_SPARSE_CSC, GM_SPARSE_CSR = 0, 1
+GM_FLAG_NO_CLAMP = 1
+GM_FLAG_AUTOTUNE = 2
+
+STATUS_NAMES = {0: "GM_OK", 1: "GM_EINVAL", 2: "GM_ENOMEM", 3: "GM_EDEVICE",
+                4: "GM_EBREAKDOWN", 5: "GM_EBADHANDLE", 6: "GM_ESTATE"}
+
+# Every symbol declared in include/gpumatfun.h.
+EXPORTS = (
+    "gm_last_error", "gm_lanczos_create", "gm_lanczos_delete", "gm_build_subspace",
+    "gm_lin_sys_solves", "gm_apply_V", "gm_update_U", "gm_update_U_monodomain",
+    "gm_get_U", "gm_get_xsol", "gm_get_aabb", "gm_set_y", "gm_get_y", "gm_get_state",
+    "gm_set_state", "gm_step", "gm_synchronize", "gm_get_info", "gm_get_stream",
+)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class GmCreateDesc(C.Structure):
+    """Mirror of ``gm_create_desc`` (field order and types must match)."""
+    _fields_ = [
+        ("n", C.c_int32),
+        ("U", _dp),
+        ("sparse_format", C.c_int32),
+        ("nnz", C.c_int64),
+        ("A_values", _dp),
+        ("A_ptr", _ip),
+        ("A_ind", _ip),
+        ("num_eigs", C.c_int32),
+        ("Q", _dp),
+        ("lambda_", _dp),
+        ("subspace_size", C.c_int32),
+        ("gamma_size", C.c_int32),
+        ("gamma", _dp),
+        ("contour_size", C.c_int32),
+        ("gamma_s_real", _dp),
+        ("gamma_s_imag", _dp),
+        ("exp_lambda", _dp),
+        ("phi_lambda", _dp),
+        ("dt", C.c_double),
+        ("ode", _dp),
+        ("aode", C.c_double), ("bode", C.c_double), ("code", C.c_double),
+        ("vrest", C.c_double), ("vamp", C.c_double), ("vth", C.c_double),
+        ("vpeak", C.c_double), ("c1ion", C.c_double), ("c2ion", C.c_double),
+        ("mass", _dp),
+        ("wts1", _dp),
+        ("wts2", _dp),
+        ("shifts", _dp),
+        ("valvec", _dp),
+        ("flags", C.c_uint32),
+        ("device", C.c_int32),
+    ]
+
+
+class GpuMatfunError(RuntimeError):
+    """An entry point returned a non-OK gm_status."""
+
+    def __init__(self, status, message):
+        self.status = status
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+
+
+_lib = None
+
+
+def load():
+    """Load the shared library (once); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run gpumatfun_amd/build.py (hipcc, gfx950) first")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "gm_last_error": (C.c_char_p, []),
+        "gm_lanczos_create": (C.c_int, [C.POINTER(GmCreateDesc), C.POINTER(vp)]),
+        "gm_lanczos_delete": (C.c_int, [vp]),
+        "gm_build_subspace": (C.c_int, [vp]),
+        "gm_lin_sys_solves": (C.c_int, [vp]),
+        "gm_apply_V": (C.c_int, [vp]),
+        "gm_update_U": (C.c_int, [vp]),
+        "gm_update_U_monodomain": (C.c_int, [vp]),
+        "gm_get_U": (C.c_int, [vp, _dp]),
+        "gm_get_xsol": (C.c_int, [vp, _dp]),
+        "gm_get_aabb": (C.c_int, [vp, _dp]),
+        "gm_set_y": (C.c_int, [vp, _dp, _dp]),
+        "gm_get_y": (C.c_int, [vp, _dp, _dp]),
+        "gm_get_state": (C.c_int, [vp, _dp, _dp, _dp]),
+        "gm_set_state": (C.c_int, [vp, _dp, _dp, _dp]),
+        "gm_step": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+        "gm_synchronize": (C.c_int, [vp]),
+        "gm_get_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32)]),
+        "gm_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != GM_OK:
+        msg = load().gm_last_error().decode("utf-8", "replace")
+        raise GpuMatfunError(status, msg)

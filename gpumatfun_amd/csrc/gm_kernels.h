@@ -1,0 +1,691 @@
+// gm_kernels.h -- gfx950 device kernels of the f(A)b engine (FP64 throughout).
+//
+// Layout in HBM (chosen for 64-wide wavefronts, not the reference's):
+//   * the two columns of every N x 2 quantity (exp path, phi path) are
+//     interleaved as double2, so a CSR gather fetches both columns of x[col]
+//     with one 16-byte load;
+//   * the Lanczos basis is V[k][i] (double2), k = 0..m-1, each slice
+//     contiguous in i: the three-term recurrence and the V*C recombination
+//     stream it with fully coalesced loads;
+//   * A is CSR (int32 row pointers and columns, FP64 values);
+//   * Q is stored row-major (N x K) so a thread reads one row's K entries
+//     contiguously in both deflation passes;
+//   * aabb, Yreal, Yimag keep the reference's column-major layout
+//     (lanczos.h:51-55) so the getters return them unchanged.
+//
+// Reductions (dot products, norms, Q^T U) are deterministic: each block
+// reduces its rows in a fixed tree, writes one partial, and the last block to
+// finish (atomic ticket) sums the partials in block order.  Results do not
+// depend on scheduling, so repeated runs are bit-identical.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gm {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxDegree = 16;        // gamma_size <= 17
+constexpr int kDeflTile = 8;          // deflation vectors per register tile
+
+__device__ __forceinline__ double2 d2(double a, double b) { return make_double2(a, b); }
+
+// Load that bypasses the non-coherent per-CU cache: used only by the last
+// block to read partials that other CUs wrote during the same kernel.
+__device__ __forceinline__ double ld_coherent(const double* p) {
+  unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double(static_cast<long long>(v));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block sum of a double2; result valid in thread 0.  Fixed order.
+__device__ __forceinline__ double2 block_sum2(double2 v) {
+  __shared__ double2 sh[kWaves];
+  v.x = wave_sum(v.x);
+  v.y = wave_sum(v.y);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double2 r = d2(0.0, 0.0);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) { r.x += sh[i].x; r.y += sh[i].y; }
+  }
+  return r;
+}
+
+// Grid-wide deterministic sum of one double2 per block.  Returns true in the
+// last block to arrive; there *out (shared) holds the total in every thread.
+__device__ __forceinline__ bool grid_sum2(double2 blockval /* valid in thread 0 */,
+                                          double* partials, unsigned* counter,
+                                          double2* out) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = blockval.x;
+    partials[2 * blockIdx.x + 1] = blockval.y;
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double2 acc = d2(0.0, 0.0);
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
+    acc.x += ld_coherent(partials + 2 * i);
+    acc.y += ld_coherent(partials + 2 * i + 1);
+  }
+  acc = block_sum2(acc);
+  __shared__ double2 tot;
+  if (threadIdx.x == 0) {
+    tot = acc;
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  *out = tot;
+  return true;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that the
+// blocks resident on one XCD own one contiguous range of rows (their x
+// gathers then share that XCD's L2).  Bijective on [0, G).
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+  const int q = G >> 3, r = G & 7;
+  const int x = b & 7, idx = b >> 3;
+  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+__device__ __forceinline__ double safe_inv(double b) {
+  return (b != 0.0 && isfinite(b)) ? 1.0 / b : 0.0;
+}
+__device__ __forceinline__ double safe_div(double a, double b) {
+  return (b != 0.0 && isfinite(b)) ? a / b : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMM with R = 2 interleaved columns and fused Lanczos / Horner epilogues.
+// LANES consecutive lanes share one row (coalesced val/col loads, butterfly
+// reduction inside the lane group); each block owns a contiguous row chunk.
+// ---------------------------------------------------------------------------
+enum SpmmMode : int {
+  kFirst = 0,   // v_k = w/beta_k (stored); y = A(g0 v_k) + g1 v_k
+  kMid = 1,     // y = A x + g_i v_k
+  kLast = 2,    // y = A x - beta_k v_{k-1};  alpha_k = y . v_k
+  kOnly = 3,    // (degree 0) v_k = w/beta_k; y = A(g0 v_k) - beta_k v_{k-1}; alpha_k
+  kHorner = 4   // y = A x + z  (optionally y += previous y)
+};
+
+struct SpmmArgs {
+  const int* __restrict__ rowptr;
+  const int* __restrict__ col;
+  const double* __restrict__ val;
+  int nrows;
+  int chunk;                   // rows per block
+  const double2* x;            // gather source
+  double2* y;                  // output (row aligned)
+  double2* vk;                 // V[k]: written by kFirst/kOnly, read by kMid/kLast
+  const double2* vkm1;         // V[k-1] or null (k == 0)
+  const double2* z;            // kHorner addend
+  const double* aabb;          // (m+1) x 4 column-major
+  int ld;                      // m + 1
+  int k;                       // iteration
+  double g0, gi;               // gamma_0, gamma_i
+  int accumulate;              // kHorner: y += result instead of y = result
+  double* partials;
+  unsigned* counter;
+  unsigned* status;            // bit 0: breakdown seen
+};
+
+template <int LANES, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
+  constexpr int GROUPS = kBlock / LANES;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int row0 = b * p.chunk;
+  const int row1 = min(row0 + p.chunk, p.nrows);
+  const int sub = threadIdx.x % LANES;
+  const int grp = threadIdx.x / LANES;
+
+  double2 inv = d2(1.0, 1.0), bet = d2(0.0, 0.0);
+  if (MODE == kFirst || MODE == kOnly) {
+    const double b0 = p.aabb[p.k + 2 * p.ld], b1 = p.aabb[p.k + 3 * p.ld];
+    inv = d2(safe_inv(b0), safe_inv(b1));
+    bet = d2(b0, b1);
+  }
+  double2 bprev = d2(0.0, 0.0);
+  if ((MODE == kLast || MODE == kOnly) && p.k > 0)
+    bprev = d2(p.aabb[p.k + 2 * p.ld], p.aabb[p.k + 3 * p.ld]);
+  const double2 sx = (MODE == kFirst || MODE == kOnly) ? d2(p.g0 * inv.x, p.g0 * inv.y)
+                                                        : d2(1.0, 1.0);
+  double2 dot = d2(0.0, 0.0);
+
+  for (int r = row0 + grp; r < row1; r += GROUPS) {
+    const int j0 = p.rowptr[r], j1 = p.rowptr[r + 1];
+    double2 acc = d2(0.0, 0.0);
+    for (int j = j0 + sub; j < j1; j += LANES) {
+      const double a = p.val[j];
+      const double2 xv = p.x[p.col[j]];
+      acc.x = fma(a, xv.x, acc.x);
+      acc.y = fma(a, xv.y, acc.y);
+    }
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) {
+      acc.x += __shfl_xor(acc.x, off, LANES);
+      acc.y += __shfl_xor(acc.y, off, LANES);
+    }
+    if (sub == 0) {
+      if (MODE == kFirst) {
+        const double2 w = p.x[r];
+        const double2 v = d2(safe_div(w.x, bet.x), safe_div(w.y, bet.y));
+        p.vk[r] = v;
+        p.y[r] = d2(fma(p.gi, v.x, acc.x * sx.x), fma(p.gi, v.y, acc.y * sx.y));
+      } else if (MODE == kMid) {
+        const double2 v = p.vk[r];
+        p.y[r] = d2(fma(p.gi, v.x, acc.x), fma(p.gi, v.y, acc.y));
+      } else if (MODE == kLast || MODE == kOnly) {
+        double2 v;
+        if (MODE == kOnly) {
+          const double2 w = p.x[r];
+          v = d2(safe_div(w.x, bet.x), safe_div(w.y, bet.y));
+          p.vk[r] = v;
+          acc.x *= sx.x;
+          acc.y *= sx.y;
+        } else {
+          v = p.vk[r];
+        }
+        double2 yv = acc;
+        if (p.k > 0) {
+          const double2 vp = p.vkm1[r];
+          yv.x = fma(-bprev.x, vp.x, yv.x);
+          yv.y = fma(-bprev.y, vp.y, yv.y);
+        }
+        p.y[r] = yv;
+        dot.x = fma(yv.x, v.x, dot.x);
+        dot.y = fma(yv.y, v.y, dot.y);
+      } else {  // kHorner
+        const double2 zz = p.z[r];
+        double2 yv = d2(acc.x + zz.x, acc.y + zz.y);
+        if (p.accumulate) {
+          const double2 old = p.y[r];
+          yv.x += old.x;
+          yv.y += old.y;
+        }
+        p.y[r] = yv;
+      }
+    }
+  }
+
+  if (MODE == kLast || MODE == kOnly) {
+    const double2 bs = block_sum2(dot);
+    double2 tot;
+    if (grid_sum2(bs, p.partials, p.counter, &tot)) {
+      if (threadIdx.x == 0) {
+        double* a = const_cast<double*>(p.aabb);
+        a[p.k] = tot.x;            // alpha_1(k)
+        a[p.k + p.ld] = tot.y;     // alpha_2(k)
+      }
+    }
+  }
+}
+
+// w <- w - alpha_k v_k (in place), beta_{k+1} = ||w||.   build_subspace.cu:63-76
+struct FinishArgs {
+  double2* w;
+  const double2* vk;
+  double* aabb;
+  int ld, k, nrows, chunk;
+  double* partials;
+  unsigned* counter;
+  unsigned* status;
+};
+
+__global__ __launch_bounds__(kBlock) void k_lanczos_finish(FinishArgs p) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int row0 = b * p.chunk;
+  const int row1 = min(row0 + p.chunk, p.nrows);
+  const double2 al = d2(p.aabb[p.k], p.aabb[p.k + p.ld]);
+  double2 nrm = d2(0.0, 0.0);
+  for (int r = row0 + threadIdx.x; r < row1; r += kBlock) {
+    const double2 v = p.vk[r];
+    double2 w = p.w[r];
+    w.x = fma(-al.x, v.x, w.x);
+    w.y = fma(-al.y, v.y, w.y);
+    p.w[r] = w;
+    nrm.x = fma(w.x, w.x, nrm.x);
+    nrm.y = fma(w.y, w.y, nrm.y);
+  }
+  const double2 bs = block_sum2(nrm);
+  double2 tot;
+  if (grid_sum2(bs, p.partials, p.counter, &tot)) {
+    if (threadIdx.x == 0) {
+      const double b1 = sqrt(tot.x), b2 = sqrt(tot.y);
+      p.aabb[p.k + 1 + 2 * p.ld] = b1;
+      p.aabb[p.k + 1 + 3 * p.ld] = b2;
+      if (!isfinite(b1) || !isfinite(b2))
+        atomicOr(p.status, 4u);
+      else if (b1 == 0.0 || b2 == 0.0)
+        atomicOr(p.status, 1u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Shifted complex tridiagonal solves, one lane per system, LAPACK ZGTSV-style
+// partial pivoting (what the CPU path's sparse backslash runs,
+// nicksmethodcpu.m:19-20).  System s < n_s uses column 0 (exp), s >= n_s
+// column 1 (phi); (T - shift_s I) y = -beta_0 w_s e1 as in
+// lin_sys_solves.cu:5-21.  Workspaces are [row][system] so a wavefront's
+// accesses are contiguous.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return d2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return d2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
+// Smith's algorithm.
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double r = b.y / b.x, den = b.x + b.y * r;
+    return d2((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+  }
+  const double r = b.x / b.y, den = b.x * r + b.y;
+  return d2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+}
+
+struct SolveArgs {
+  const double* aabb;
+  int ld, m, ns;
+  const double* wts1;   // ns x 2
+  const double* wts2;
+  const double* shifts; // 4 m ns, complex interleaved, system s at s*m
+  double2* wD;
+  double2* wDU;
+  double2* wDU2;
+  double2* wX;
+  double* yr;           // m x 2ns column-major
+  double* yi;
+  unsigned* status;     // bit 1: singular pivot
+};
+
+__global__ __launch_bounds__(64) void k_shifted_solves(SolveArgs p) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int S = 2 * p.ns;
+  if (s >= S) return;
+  const int c = s < p.ns ? 0 : 1;
+  const int node = s - c * p.ns;
+  const double* w = c == 0 ? p.wts1 : p.wts2;
+  const double b0 = p.aabb[2 * p.ld + c * p.ld];
+  const double2 wt = d2(w[node], w[node + p.ns]);
+  const double* al = p.aabb + c * p.ld;
+  const double* be = p.aabb + (2 + c) * p.ld;
+  const double* sh = p.shifts + 2 * (size_t)s * p.m;
+  const int m = p.m;
+  auto diag = [&](int k) { return d2(al[k] - sh[2 * k], -sh[2 * k + 1]); };
+  auto off = [&](int k) { return d2(be[k + 1], 0.0); };  // T(k+1,k) = T(k,k+1)
+
+  double2 cur_d = diag(0);
+  double2 cur_x = d2(-(b0 * wt.x), -(b0 * wt.y));
+  double2 cur_du = m > 1 ? off(0) : d2(0.0, 0.0);
+  bool singular = false;
+  for (int k = 0; k < m - 1; ++k) {
+    const double2 dl = off(k);
+    double2 next_d = diag(k + 1);
+    double2 next_x = d2(0.0, 0.0);
+    double2 next_du = (k + 1 < m - 1) ? off(k + 1) : d2(0.0, 0.0);
+    double2 sD, sDU, sDU2 = d2(0.0, 0.0), sX;
+    if (dl.x == 0.0 && dl.y == 0.0) {
+      if (cur_d.x == 0.0 && cur_d.y == 0.0) singular = true;
+      sD = cur_d; sDU = cur_du; sX = cur_x;
+    } else if (cabs1(cur_d) >= cabs1(dl)) {
+      const double2 mult = cdiv(dl, cur_d);
+      next_d = csub(next_d, cmul(mult, cur_du));
+      next_x = csub(next_x, cmul(mult, cur_x));
+      sD = cur_d; sDU = cur_du; sX = cur_x;
+    } else {
+      const double2 mult = cdiv(cur_d, dl);
+      sD = dl;
+      const double2 temp = next_d;
+      next_d = csub(cur_du, cmul(mult, temp));
+      if (k < m - 2) {
+        sDU2 = next_du;
+        next_du = d2(-(mult.x * sDU2.x - mult.y * sDU2.y), -(mult.x * sDU2.y + mult.y * sDU2.x));
+      }
+      sDU = temp;
+      sX = next_x;
+      next_x = csub(cur_x, cmul(mult, next_x));
+    }
+    const size_t o = (size_t)k * S + s;
+    p.wD[o] = sD; p.wDU[o] = sDU; p.wDU2[o] = sDU2; p.wX[o] = sX;
+    cur_d = next_d; cur_x = next_x; cur_du = next_du;
+  }
+  if (cur_d.x == 0.0 && cur_d.y == 0.0) singular = true;
+  if (singular) {
+    atomicOr(p.status, 2u);
+    for (int k = 0; k < m; ++k) { p.yr[k + (size_t)m * s] = 0.0; p.yi[k + (size_t)m * s] = 0.0; }
+    return;
+  }
+  // back substitution
+  double2 x2 = cdiv(cur_x, cur_d);           // x_{m-1}
+  p.yr[(m - 1) + (size_t)m * s] = x2.x;
+  p.yi[(m - 1) + (size_t)m * s] = x2.y;
+  if (m > 1) {
+    const size_t o = (size_t)(m - 2) * S + s;
+    double2 x1 = cdiv(csub(p.wX[o], cmul(p.wDU[o], x2)), p.wD[o]);
+    p.yr[(m - 2) + (size_t)m * s] = x1.x;
+    p.yi[(m - 2) + (size_t)m * s] = x1.y;
+    for (int k = m - 3; k >= 0; --k) {
+      const size_t q = (size_t)k * S + s;
+      double2 t = csub(p.wX[q], cmul(p.wDU[q], x1));
+      t = csub(t, cmul(p.wDU2[q], x2));
+      const double2 xk = cdiv(t, p.wD[q]);
+      p.yr[k + (size_t)m * s] = xk.x;
+      p.yi[k + (size_t)m * s] = xk.y;
+      x2 = x1;
+      x1 = xk;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// apply_V, collapsed over the nodes:
+//   x_c = sum_s Im(q_s(A) V_c y_s) = sum_j A^(d-j) V_c c_j,
+//   c_j = sum_s (gsr_{j,s} Im y_s + gsi_{j,s} Re y_s)
+// so one pass over V (V*C, tall-skinny) plus d SpMMs replace the reference's
+// 2 n_s d SpMMs and 4 n_s gemv (apply_V.cu:5-95).
+// ---------------------------------------------------------------------------
+struct CollapseArgs {
+  const double* yr;    // m x 2ns
+  const double* yi;
+  const double* gsr;   // (d+1) x ns
+  const double* gsi;
+  int m, ns, nd;       // nd = d + 1
+  double2* C;          // [nd][m] double2 (x: exp, y: phi)
+};
+
+__global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs p) {
+  const int total = p.nd * p.m;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+    const int j = t / p.m, k = t - j * p.m;
+    double cx = 0.0, cy = 0.0;
+    for (int s = 0; s < p.ns; ++s) {
+      const double gr = p.gsr[j + (size_t)p.nd * s], gim = p.gsi[j + (size_t)p.nd * s];
+      const size_t e = k + (size_t)p.m * s, f = k + (size_t)p.m * (s + p.ns);
+      cx = fma(gr, p.yi[e], cx);
+      cx = fma(gim, p.yr[e], cx);
+      cy = fma(gr, p.yi[f], cy);
+      cy = fma(gim, p.yr[f], cy);
+    }
+    p.C[t] = d2(cx, cy);
+  }
+}
+
+// Z_j = V C_j for j < nd, one row per thread; C staged in LDS.
+struct VCArgs {
+  const double2* V;    // [m][N]
+  const double2* C;    // [nd][m]
+  int m, nd, nrows;
+  size_t ldv;          // N
+  double2* Z;          // [nd][N]; nd == 1 -> Z is the output X
+  int accumulate;      // nd == 1: X += V c_0
+};
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_vc(VCArgs p) {
+  extern __shared__ double2 sC[];
+  for (int t = threadIdx.x; t < ND * p.m; t += kBlock) sC[t] = p.C[t];
+  __syncthreads();
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= p.nrows) return;
+  double2 acc[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) acc[j] = d2(0.0, 0.0);
+  for (int k = 0; k < p.m; ++k) {
+    const double2 v = p.V[(size_t)k * p.ldv + r];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      const double2 c = sC[j * p.m + k];
+      acc[j].x = fma(v.x, c.x, acc[j].x);
+      acc[j].y = fma(v.y, c.y, acc[j].y);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    double2 o = acc[j];
+    if (ND == 1 && p.accumulate) {
+      const double2 old = p.Z[r];
+      o.x += old.x;
+      o.y += old.y;
+    }
+    p.Z[(size_t)j * p.ldv + r] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Time-step update and deflation.
+// ---------------------------------------------------------------------------
+enum UpdateMode : int { kUpdFisher = 0, kUpdMono = 1, kUpdPrepare = 2 };
+
+struct UpdateArgs {
+  const double2* X;
+  const double2* upd;
+  const double* mass;
+  double* ode;
+  double dt, aode_dt, bode_dt, code, vrest, vpeak, cion;
+  int clamp, x_zero;
+  double2* Usol;
+  double2* U;          // K == 0: U = Usol
+  const double* Q;     // row-major N x K
+  int K, nrows, chunk;
+  double* partials;    // K == 0: 2 per block; K > 0: 2K per block
+  unsigned* counter;
+  double* beta0;       // &aabb[2 ld] and &aabb[3 ld] -> beta0[0], beta0[ld]
+  int ld;
+  double* Zout;        // 2K (K > 0): Z = Q^T Usol, [k][c]
+  unsigned* status;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_update1(UpdateArgs p) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int row0 = b * p.chunk;
+  const int row1 = min(row0 + p.chunk, p.nrows);
+  // pass 1: the elementwise update (update_U.cu:8-16, update_U_monodomain.cu:8-25)
+  double2 nrm = d2(0.0, 0.0);
+  for (int r = row0 + threadIdx.x; r < row1; r += kBlock) {
+    double2 us;
+    if (MODE == kUpdPrepare) {
+      us = p.Usol[r];
+    } else {
+      const double2 x = p.x_zero ? d2(0.0, 0.0) : p.X[r];
+      const double2 up = p.upd[r];
+      double u = ((x.x + x.y) + up.x) + up.y;
+      double g;
+      if (MODE == kUpdFisher) {
+        if (p.clamp) u = u * (u >= 0.0 ? 1.0 : 0.0);
+        const double ms = p.mass[r];
+        const double ub = u / ms;
+        g = (ms * ub) * (1.0 - ub);
+      } else {
+        const double o = p.ode[r];
+        const double on = o + ((u * p.aode_dt + o * p.bode_dt) + p.code);
+        p.ode[r] = on;
+        const double src = ((u - p.vrest) * u) * (u - p.vpeak) * p.cion;
+        g = src - on;
+      }
+      us = d2(u, g * p.dt);
+      p.Usol[r] = us;
+    }
+    if (p.K == 0) {
+      p.U[r] = us;
+      nrm.x = fma(us.x, us.x, nrm.x);
+      nrm.y = fma(us.y, us.y, nrm.y);
+    }
+  }
+  if (p.K == 0) {
+    const double2 bs = block_sum2(nrm);
+    double2 tot;
+    if (grid_sum2(bs, p.partials, p.counter, &tot) && threadIdx.x == 0) {
+      p.beta0[0] = sqrt(tot.x);
+      p.beta0[p.ld] = sqrt(tot.y);
+      if (!isfinite(tot.x) || !isfinite(tot.y)) atomicOr(p.status, 4u);
+    }
+    return;
+  }
+  // pass 2: block partials of Z = Q^T Usol, kDeflTile vectors at a time.
+  __shared__ double red[kWaves][2 * kDeflTile];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k0 = 0; k0 < p.K; k0 += kDeflTile) {
+    double acc[2 * kDeflTile];
+#pragma unroll
+    for (int t = 0; t < 2 * kDeflTile; ++t) acc[t] = 0.0;
+    for (int r = row0 + threadIdx.x; r < row1; r += kBlock) {
+      const double2 us = p.Usol[r];
+      const double* q = p.Q + (size_t)r * p.K;
+#pragma unroll
+      for (int t = 0; t < kDeflTile; ++t) {
+        if (k0 + t < p.K) {
+          const double qv = q[k0 + t];
+          acc[2 * t] = fma(qv, us.x, acc[2 * t]);
+          acc[2 * t + 1] = fma(qv, us.y, acc[2 * t + 1]);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 2 * kDeflTile; ++t) acc[t] = wave_sum(acc[t]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+      for (int t = 0; t < 2 * kDeflTile; ++t) red[wv][t] = acc[t];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kDeflTile && k0 * 2 + (int)threadIdx.x < 2 * p.K) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < kWaves; ++i) s += red[i][threadIdx.x];
+      p.partials[(size_t)blockIdx.x * 2 * p.K + 2 * k0 + threadIdx.x] = s;
+    }
+  }
+  // last block sums the partials in block order
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned t = atomicAdd(p.counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const int NV = 2 * p.K;
+  const int nsub = NV >= kBlock ? 1 : kBlock / NV;
+  __shared__ double sub[kBlock];
+  for (int v0 = 0; v0 < NV; v0 += kBlock) {
+    const int v = v0 + (threadIdx.x % min(NV - v0, kBlock));
+    const int sb = threadIdx.x / min(NV - v0, kBlock);
+    const int ns = NV - v0 >= kBlock ? 1 : nsub;
+    double s = 0.0;
+    if (sb < ns)
+      for (int bb = sb; bb < (int)gridDim.x; bb += ns)
+        s += ld_coherent(p.partials + (size_t)bb * NV + v);
+    __syncthreads();
+    sub[threadIdx.x] = s;
+    __syncthreads();
+    const int width = min(NV - v0, kBlock);
+    if ((int)threadIdx.x < width) {
+      double t = 0.0;
+      for (int i = 0; i < ns; ++i) t += sub[i * width + threadIdx.x];
+      p.Zout[v0 + threadIdx.x] = t;
+    }
+  }
+  if (threadIdx.x == 0)
+    __hip_atomic_store(p.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// U = Usol - Q Z; update = Q diag(e^L, phi(L)) Z; beta0 = ||U||.
+// lanczos_impl.cu:84-89 / update_U.cu:22-27.
+struct Update2Args {
+  const double2* Usol;
+  double2* U;
+  double2* upd;
+  const double* Q;
+  const double* Z;       // 2K, [k][c]
+  const double* el;      // K
+  const double* pl;      // K
+  int K, nrows, chunk, ld;
+  double* partials;
+  unsigned* counter;
+  double* beta0;
+  unsigned* status;
+};
+
+__global__ __launch_bounds__(kBlock) void k_update2(Update2Args p) {
+  extern __shared__ double sz[];   // 4K: Z (2K) then f*Z (2K)
+  for (int t = threadIdx.x; t < p.K; t += kBlock) {
+    const double zx = p.Z[2 * t], zy = p.Z[2 * t + 1];
+    sz[2 * t] = zx;
+    sz[2 * t + 1] = zy;
+    sz[2 * p.K + 2 * t] = zx * p.el[t];
+    sz[2 * p.K + 2 * t + 1] = zy * p.pl[t];
+  }
+  __syncthreads();
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int row0 = b * p.chunk;
+  const int row1 = min(row0 + p.chunk, p.nrows);
+  double2 nrm = d2(0.0, 0.0);
+  for (int r = row0 + threadIdx.x; r < row1; r += kBlock) {
+    const double* q = p.Q + (size_t)r * p.K;
+    double cx = 0.0, cy = 0.0, ux = 0.0, uy = 0.0;
+    for (int k = 0; k < p.K; ++k) {
+      const double qv = q[k];
+      cx = fma(qv, sz[2 * k], cx);
+      cy = fma(qv, sz[2 * k + 1], cy);
+      ux = fma(qv, sz[2 * p.K + 2 * k], ux);
+      uy = fma(qv, sz[2 * p.K + 2 * k + 1], uy);
+    }
+    const double2 us = p.Usol[r];
+    const double2 u = d2(us.x - cx, us.y - cy);
+    p.U[r] = u;
+    p.upd[r] = d2(ux, uy);
+    nrm.x = fma(u.x, u.x, nrm.x);
+    nrm.y = fma(u.y, u.y, nrm.y);
+  }
+  const double2 bs = block_sum2(nrm);
+  double2 tot;
+  if (grid_sum2(bs, p.partials, p.counter, &tot) && threadIdx.x == 0) {
+    p.beta0[0] = sqrt(tot.x);
+    p.beta0[p.ld] = sqrt(tot.y);
+    if (!isfinite(tot.x) || !isfinite(tot.y)) atomicOr(p.status, 4u);
+  }
+}
+
+// de-interleave an N x 2 double2 array into column-major (for the getters)
+__global__ __launch_bounds__(kBlock) void k_deinterleave(const double2* src, double* dst, int n) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < n) {
+    const double2 v = src[r];
+    dst[r] = v.x;
+    dst[(size_t)n + r] = v.y;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero2(double2* p, int n) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < n) p[r] = d2(0.0, 0.0);
+}
+
+// Plain streaming copy (16-byte elements), the calibration point for the HBM
+// byte counters.
+__global__ __launch_bounds__(kBlock) void k_copy16(const double2* __restrict__ src,
+                                                   double2* __restrict__ dst, long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
+}  // namespace gm

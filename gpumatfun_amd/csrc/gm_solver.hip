@@ -1,0 +1,841 @@
+// gm_solver.hip -- solver state and the C ABI (include/gpumatfun.h).
+//
+// One gm_solver owns every device buffer of one problem (the reference's
+// lanczos_solver, lanczos.h:16-86) and one HIP stream.  The stepping entry
+// points only enqueue kernels; nothing inside a step waits on the host, so a
+// whole step can be captured into a HIP graph (gm_step).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/gpumatfun.h"
+#include "gm_kernels.h"
+
+using gm::kBlock;
+
+namespace {
+
+thread_local std::string g_last_error;
+std::mutex g_registry_mu;
+std::unordered_set<const void*> g_registry;
+
+gm_status fail(gm_status st, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return st;
+}
+
+#define GM_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? GM_ENOMEM : GM_EDEVICE, "%s: %s (%s:%d)", \
+                  #call, hipGetErrorString(e_), __FILE__, __LINE__);                   \
+  } while (0)
+
+#define GM_TRY(expr)             \
+  do {                           \
+    gm_status st_ = (expr);      \
+    if (st_ != GM_OK) return st_; \
+  } while (0)
+
+inline int cdiv_i(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+struct gm_solver {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int n = 0;
+  long long nnz = 0;
+  int m = 0, nd = 0, ns = 0, K = 0;
+  double dt = 0, aode = 0, bode = 0, code = 0, vrest = 0, vamp = 0, vth = 0, vpeak = 0,
+         c1ion = 0, c2ion = 0;
+  bool clamp = true;
+  std::vector<double> gamma;
+  // schedules
+  int lanes = 8, G_sp = 1, chunk_sp = 1, G_el = 1, chunk_el = 1, G_df = 1, chunk_df = 1;
+  // device buffers
+  int* rowptr = nullptr;
+  int* col = nullptr;
+  double* val = nullptr;
+  double2 *U = nullptr, *Usol = nullptr, *X = nullptr, *upd = nullptr;
+  double2 *W[2] = {nullptr, nullptr}, *T[2] = {nullptr, nullptr};
+  double2* V = nullptr;
+  double2* Z = nullptr;
+  double2* C = nullptr;
+  double *Q = nullptr, *el = nullptr, *pl = nullptr, *mass = nullptr, *ode = nullptr;
+  double *aabb = nullptr, *Yr = nullptr, *Yi = nullptr, *wts1 = nullptr, *wts2 = nullptr,
+         *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *Zdefl = nullptr;
+  double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  unsigned* status = nullptr;
+  double* stage = nullptr;  // n x 2 column-major staging for the getters
+  // host-side state machine
+  bool built = false, solved = false, x_zero = true;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+bool live(const gm_solver* s) {
+  std::lock_guard<std::mutex> lk(g_registry_mu);
+  return s != nullptr && g_registry.count(s) != 0;
+}
+
+#define GM_CHECK_HANDLE(s)                                                              \
+  do {                                                                                  \
+    if (!live(s)) return fail(GM_EBADHANDLE, "invalid, deleted or null solver handle"); \
+    GM_HIP(hipSetDevice((s)->dev));                                                     \
+  } while (0)
+
+template <typename T>
+gm_status dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  GM_HIP(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+  return GM_OK;
+}
+
+template <typename T>
+gm_status upload(T** p, const T* src, size_t count, hipStream_t st) {
+  GM_TRY(dalloc(p, count));
+  if (count && src) GM_HIP(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+  return GM_OK;
+}
+
+void free_all(gm_solver* s) {
+  for (auto& g : s->graph)
+    if (g) hipGraphExecDestroy(g);
+  void* bufs[] = {s->rowptr, s->col, s->val, s->U, s->Usol, s->X, s->upd, s->W[0], s->W[1],
+                  s->T[0], s->T[1], s->V, s->Z, s->C, s->Q, s->el, s->pl, s->mass, s->ode,
+                  s->aabb, s->Yr, s->Yi, s->wts1, s->wts2, s->shifts, s->gsr, s->gsi,
+                  s->Zdefl, s->wD, s->wDU, s->wDU2, s->wX, s->partials, s->counter,
+                  s->status, s->stage};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (s->stream) hipStreamDestroy(s->stream);
+}
+
+// ----------------------------------------------------------------- launches
+using gm::SpmmArgs;
+
+template <int MODE>
+gm_status launch_spmm_mode(gm_solver* s, const SpmmArgs& a) {
+  dim3 grid(s->G_sp), block(kBlock);
+  switch (s->lanes) {
+    case 1: hipLaunchKernelGGL((gm::k_spmm<1, MODE>), grid, block, 0, s->stream, a); break;
+    case 2: hipLaunchKernelGGL((gm::k_spmm<2, MODE>), grid, block, 0, s->stream, a); break;
+    case 4: hipLaunchKernelGGL((gm::k_spmm<4, MODE>), grid, block, 0, s->stream, a); break;
+    case 8: hipLaunchKernelGGL((gm::k_spmm<8, MODE>), grid, block, 0, s->stream, a); break;
+    case 16: hipLaunchKernelGGL((gm::k_spmm<16, MODE>), grid, block, 0, s->stream, a); break;
+    case 32: hipLaunchKernelGGL((gm::k_spmm<32, MODE>), grid, block, 0, s->stream, a); break;
+    default: hipLaunchKernelGGL((gm::k_spmm<64, MODE>), grid, block, 0, s->stream, a); break;
+  }
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+SpmmArgs spmm_base(gm_solver* s) {
+  SpmmArgs a{};
+  a.rowptr = s->rowptr;
+  a.col = s->col;
+  a.val = s->val;
+  a.nrows = s->n;
+  a.chunk = s->chunk_sp;
+  a.aabb = s->aabb;
+  a.ld = s->m + 1;
+  a.partials = s->partials;
+  a.counter = s->counter;
+  a.status = s->status;
+  return a;
+}
+
+void set_schedule(gm_solver* s, int lanes) {
+  s->lanes = lanes;
+  const int groups = kBlock / lanes;
+  s->G_sp = std::max(1, std::min(4096, cdiv_i(s->n, groups)));
+  s->chunk_sp = cdiv_i(s->n, s->G_sp);
+}
+
+// build_subspace.cu:5-91.  Iteration k reads w_k (U for k = 0) and writes
+// w_{k+1} into W[k & 1]; beta_k lives in aabb and is applied on the fly.
+gm_status enqueue_build(gm_solver* s) {
+  const int d = s->nd - 1;
+  const size_t N = (size_t)s->n;
+  for (int k = 0; k < s->m; ++k) {
+    const double2* wsrc = k == 0 ? s->U : s->W[(k - 1) & 1];
+    double2* wnext = s->W[k & 1];
+    double2* vk = s->V + (size_t)k * N;
+    const double2* vkm1 = k > 0 ? s->V + (size_t)(k - 1) * N : nullptr;
+    SpmmArgs a = spmm_base(s);
+    a.k = k;
+    a.vk = vk;
+    a.vkm1 = vkm1;
+    a.g0 = s->gamma[0];
+    if (d == 0) {
+      a.x = wsrc;
+      a.y = wnext;
+      GM_TRY(launch_spmm_mode<gm::kOnly>(s, a));
+    } else {
+      a.x = wsrc;
+      a.y = s->T[0];
+      a.gi = s->gamma[1];
+      GM_TRY(launch_spmm_mode<gm::kFirst>(s, a));
+      for (int i = 2; i <= d; ++i) {
+        a.x = s->T[(i - 2) & 1];
+        a.y = s->T[(i - 1) & 1];
+        a.gi = s->gamma[i];
+        GM_TRY(launch_spmm_mode<gm::kMid>(s, a));
+      }
+      a.x = s->T[(d - 1) & 1];
+      a.y = wnext;
+      GM_TRY(launch_spmm_mode<gm::kLast>(s, a));
+    }
+    gm::FinishArgs f{};
+    f.w = wnext;
+    f.vk = vk;
+    f.aabb = s->aabb;
+    f.ld = s->m + 1;
+    f.k = k;
+    f.nrows = s->n;
+    f.chunk = s->chunk_el;
+    f.partials = s->partials;
+    f.counter = s->counter;
+    f.status = s->status;
+    hipLaunchKernelGGL(gm::k_lanczos_finish, dim3(s->G_el), dim3(kBlock), 0, s->stream, f);
+    GM_HIP(hipGetLastError());
+  }
+  return GM_OK;
+}
+
+gm_status enqueue_solves(gm_solver* s) {
+  gm::SolveArgs a{};
+  a.aabb = s->aabb;
+  a.ld = s->m + 1;
+  a.m = s->m;
+  a.ns = s->ns;
+  a.wts1 = s->wts1;
+  a.wts2 = s->wts2;
+  a.shifts = s->shifts;
+  a.wD = s->wD;
+  a.wDU = s->wDU;
+  a.wDU2 = s->wDU2;
+  a.wX = s->wX;
+  a.yr = s->Yr;
+  a.yi = s->Yi;
+  a.status = s->status;
+  hipLaunchKernelGGL(gm::k_shifted_solves, dim3(cdiv_i(2 * s->ns, 64)), dim3(64), 0, s->stream, a);
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+template <int ND>
+void launch_vc(gm_solver* s, const gm::VCArgs& a) {
+  hipLaunchKernelGGL((gm::k_vc<ND>), dim3(cdiv_i(s->n, kBlock)), dim3(kBlock),
+                     ND * s->m * sizeof(double2), s->stream, a);
+}
+
+gm_status enqueue_apply(gm_solver* s) {
+  const int d = s->nd - 1;
+  const size_t N = (size_t)s->n;
+  gm::CollapseArgs c{};
+  c.yr = s->Yr;
+  c.yi = s->Yi;
+  c.gsr = s->gsr;
+  c.gsi = s->gsi;
+  c.m = s->m;
+  c.ns = s->ns;
+  c.nd = s->nd;
+  c.C = s->C;
+  hipLaunchKernelGGL(gm::k_collapse, dim3(std::min(64, cdiv_i(s->nd * s->m, kBlock))), dim3(kBlock),
+                     0, s->stream, c);
+  GM_HIP(hipGetLastError());
+  gm::VCArgs v{};
+  v.V = s->V;
+  v.C = s->C;
+  v.m = s->m;
+  v.nd = s->nd;
+  v.nrows = s->n;
+  v.ldv = N;
+  v.Z = d == 0 ? s->X : s->Z;
+  v.accumulate = (d == 0 && !s->x_zero) ? 1 : 0;
+  switch (s->nd) {
+#define GM_VC(ND) \
+  case ND: launch_vc<ND>(s, v); break;
+    GM_VC(1) GM_VC(2) GM_VC(3) GM_VC(4) GM_VC(5) GM_VC(6) GM_VC(7) GM_VC(8) GM_VC(9)
+    GM_VC(10) GM_VC(11) GM_VC(12) GM_VC(13) GM_VC(14) GM_VC(15) GM_VC(16) GM_VC(17)
+#undef GM_VC
+    default: return fail(GM_EINVAL, "gamma_size %d exceeds %d", s->nd, gm::kMaxDegree + 1);
+  }
+  GM_HIP(hipGetLastError());
+  // Horner over the collapsed coefficients: x = Z_0; x = A x + Z_j.
+  const double2* src = s->Z;
+  for (int j = 1; j <= d; ++j) {
+    SpmmArgs a = spmm_base(s);
+    double2* dst = (j == d) ? s->X : s->T[(j - 1) & 1];
+    a.x = src;
+    a.y = dst;
+    a.z = s->Z + (size_t)j * N;
+    a.accumulate = (j == d && !s->x_zero) ? 1 : 0;
+    GM_TRY(launch_spmm_mode<gm::kHorner>(s, a));
+    src = dst;
+  }
+  s->x_zero = false;
+  return GM_OK;
+}
+
+// update_U.cu:5-32 (mode kUpdFisher), update_U_monodomain.cu:5-43 (kUpdMono),
+// or the constructor's deflation of a given Usol (kUpdPrepare,
+// lanczos_impl.cu:79-89).
+gm_status enqueue_update(gm_solver* s, int mode) {
+  gm::UpdateArgs a{};
+  a.X = s->X;
+  a.upd = s->upd;
+  a.mass = s->mass;
+  a.ode = s->ode;
+  a.dt = s->dt;
+  a.aode_dt = s->dt * s->aode;
+  a.bode_dt = s->dt * s->bode;
+  a.code = s->code;
+  a.vrest = s->vrest;
+  a.vpeak = s->vpeak;
+  a.cion = s->c1ion / s->vamp / s->vamp;
+  a.clamp = s->clamp ? 1 : 0;
+  a.x_zero = s->x_zero ? 1 : 0;
+  a.Usol = s->Usol;
+  a.U = s->U;
+  a.Q = s->Q;
+  a.K = s->K;
+  a.nrows = s->n;
+  a.partials = s->partials;
+  a.counter = s->counter;
+  a.beta0 = s->aabb + 2 * (s->m + 1);
+  a.ld = s->m + 1;
+  a.Zout = s->Zdefl;
+  a.status = s->status;
+  const int G = s->K == 0 ? s->G_el : s->G_df;
+  a.chunk = s->K == 0 ? s->chunk_el : s->chunk_df;
+  switch (mode) {
+    case gm::kUpdFisher:
+      hipLaunchKernelGGL(gm::k_update1<gm::kUpdFisher>, dim3(G), dim3(kBlock), 0, s->stream, a);
+      break;
+    case gm::kUpdMono:
+      hipLaunchKernelGGL(gm::k_update1<gm::kUpdMono>, dim3(G), dim3(kBlock), 0, s->stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL(gm::k_update1<gm::kUpdPrepare>, dim3(G), dim3(kBlock), 0, s->stream, a);
+      break;
+  }
+  GM_HIP(hipGetLastError());
+  if (s->K > 0) {
+    gm::Update2Args b{};
+    b.Usol = s->Usol;
+    b.U = s->U;
+    b.upd = s->upd;
+    b.Q = s->Q;
+    b.Z = s->Zdefl;
+    b.el = s->el;
+    b.pl = s->pl;
+    b.K = s->K;
+    b.nrows = s->n;
+    b.chunk = s->chunk_el;
+    b.ld = s->m + 1;
+    b.partials = s->partials;
+    b.counter = s->counter;
+    b.beta0 = s->aabb + 2 * (s->m + 1);
+    b.status = s->status;
+    hipLaunchKernelGGL(gm::k_update2, dim3(s->G_el), dim3(kBlock), 4 * s->K * sizeof(double),
+                       s->stream, b);
+    GM_HIP(hipGetLastError());
+  } else {
+    // update = 0 when nothing is deflated (Q is N x 0)
+  }
+  if (mode != gm::kUpdPrepare) s->x_zero = true;
+  return GM_OK;
+}
+
+gm_status enqueue_step(gm_solver* s, int mono) {
+  GM_TRY(enqueue_build(s));
+  GM_TRY(enqueue_solves(s));
+  GM_TRY(enqueue_apply(s));
+  GM_TRY(enqueue_update(s, mono ? gm::kUpdMono : gm::kUpdFisher));
+  return GM_OK;
+}
+
+gm_status check_status(gm_solver* s) {
+  unsigned st = 0;
+  GM_HIP(hipMemcpyAsync(&st, s->status, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
+  GM_HIP(hipStreamSynchronize(s->stream));
+  if (st & 6u)
+    return fail(GM_EBREAKDOWN, "Lanczos breakdown: %s%s",
+                (st & 2u) ? "singular shifted tridiagonal system; " : "",
+                (st & 4u) ? "non-finite beta (NaN/Inf in the input?)" : "");
+  return GM_OK;
+}
+
+gm_status fetch_pair(gm_solver* s, const double2* src, double* dst) {
+  if (!dst) return fail(GM_EINVAL, "null destination pointer");
+  hipLaunchKernelGGL(gm::k_deinterleave, dim3(cdiv_i(s->n, kBlock)), dim3(kBlock), 0, s->stream,
+                     src, s->stage, s->n);
+  GM_HIP(hipGetLastError());
+  GM_HIP(hipMemcpyAsync(dst, s->stage, 2 * (size_t)s->n * sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  GM_HIP(hipStreamSynchronize(s->stream));
+  return GM_OK;
+}
+
+// ------------------------------------------------------------ validation
+gm_status validate(const gm_create_desc* d) {
+  const char* P = "MATLAB:lanczos_create:";
+  if (!d) return fail(GM_EINVAL, "%sinvalidNumInputs: null create descriptor", P);
+  if (d->n <= 0 || !d->U) return fail(GM_EINVAL, "%sbadU: Invalid U input argument.", P);
+  if (!d->A_values && d->nnz > 0)
+    return fail(GM_EINVAL, "%sinputNotSparse: Invalid sparse matrix input argument.", P);
+  if (!d->A_ptr || (d->nnz > 0 && !d->A_ind) || d->nnz < 0 || d->nnz > 0x7fffffffLL)
+    return fail(GM_EINVAL, "%sinputNotSparse: Invalid sparse matrix input argument.", P);
+  if (d->sparse_format != GM_SPARSE_CSC && d->sparse_format != GM_SPARSE_CSR)
+    return fail(GM_EINVAL, "%sinputNotSparse: unknown sparse format %d", P, d->sparse_format);
+  if (d->A_ptr[0] != 0 || d->A_ptr[d->n] != d->nnz)
+    return fail(GM_EINVAL, "%sinputNotSparse: pointer array must run from 0 to nnz", P);
+  for (int i = 0; i < d->n; ++i)
+    if (d->A_ptr[i + 1] < d->A_ptr[i])
+      return fail(GM_EINVAL, "%sinputNotSparse: pointer array not monotone at %d", P, i);
+  for (long long j = 0; j < d->nnz; ++j)
+    if (d->A_ind[j] < 0 || d->A_ind[j] >= d->n)
+      return fail(GM_EINVAL, "%sinputNotSquare: index %d out of range at %lld (matrix must be %d x %d)",
+                  P, d->A_ind[j], j, d->n, d->n);
+  if (d->num_eigs < 0 || (d->num_eigs > 0 && !d->Q))
+    return fail(GM_EINVAL, "%sbadQ: Invalid Q input argument.", P);
+  if (d->num_eigs > 0 && !d->lambda)
+    return fail(GM_EINVAL, "%sbad_lambda: Invalid lambda input argument.", P);
+  if (d->subspace_size < 1) return fail(GM_EINVAL, "%sbad_N: Invalid N input argument.", P);
+  if (d->gamma_size < 1 || !d->gamma)
+    return fail(GM_EINVAL, "%sbad_gamma: Invalid gamma input argument.", P);
+  if (d->gamma_size > gm::kMaxDegree + 1)
+    return fail(GM_EINVAL, "%sbad_gamma: polynomial degree %d exceeds %d", P, d->gamma_size - 1,
+                gm::kMaxDegree);
+  if (d->contour_size < 1 || !d->gamma_s_real)
+    return fail(GM_EINVAL, "%sbad_gamma_s_real: Invalid gamma_s_real input argument.", P);
+  if (!d->gamma_s_imag)
+    return fail(GM_EINVAL, "%sbad_gamma_s_imag: Invalid gamma_s_imag input argument.", P);
+  if (d->num_eigs > 0 && !d->exp_lambda)
+    return fail(GM_EINVAL, "%sbad_exp_lambda: Invalid exp_lambda input argument.", P);
+  if (d->num_eigs > 0 && !d->phi_lambda)
+    return fail(GM_EINVAL, "%sbad_phi_lambda: Invalid phi_lambda input argument.", P);
+  if (!std::isfinite(d->dt)) return fail(GM_EINVAL, "%sbad_dt: Invalid dt input argument.", P);
+  if (!d->wts1) return fail(GM_EINVAL, "%sbad_wts1: Invalid wts1 input argument.", P);
+  if (!d->wts2) return fail(GM_EINVAL, "%sbad_wts2: Invalid wts2 input argument.", P);
+  if (!d->shifts) return fail(GM_EINVAL, "%sbad_shifts: Invalid shifts input argument c.", P);
+  if (d->valvec && (d->valvec[0] != -1.0 || d->valvec[1] != 0.0 || d->valvec[2] != 1.0))
+    return fail(GM_EINVAL, "%sbad_valvec: valvec must be [-1 0 1]", P);
+  if ((long long)d->gamma_size * d->subspace_size * (long long)sizeof(double2) > 65536)
+    return fail(GM_EINVAL, "%sbad_N: (d+1)*m = %d exceeds the 4096 coefficients kept in LDS", P,
+                d->gamma_size * d->subspace_size);
+  if (d->mass)
+    for (int i = 0; i < d->n; ++i)
+      if (!(d->mass[i] != 0.0))
+        return fail(GM_EINVAL, "%sbad_mass: mass must be nonzero (entry %d)", P, i);
+  return GM_OK;
+}
+
+gm_status check_device(int dev) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return fail(GM_EDEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+  if (dev < 0 || dev >= count) return fail(GM_EDEVICE, "device %d out of range (%d)", dev, count);
+  hipDeviceProp_t prop;
+  GM_HIP(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(GM_EDEVICE, "device %d is %s; this library is built for gfx950 only", dev,
+                prop.gcnArchName);
+  return GM_OK;
+}
+
+gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
+  GM_TRY(check_device(d->device));
+  s->dev = d->device;
+  GM_HIP(hipSetDevice(s->dev));
+  GM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  hipStream_t st = s->stream;
+  s->n = d->n;
+  s->nnz = d->nnz;
+  s->m = d->subspace_size;
+  s->nd = d->gamma_size;
+  s->ns = d->contour_size;
+  s->K = d->num_eigs;
+  s->dt = d->dt;
+  s->aode = d->aode; s->bode = d->bode; s->code = d->code;
+  s->vrest = d->vrest; s->vamp = d->vamp; s->vth = d->vth; s->vpeak = d->vpeak;
+  s->c1ion = d->c1ion; s->c2ion = d->c2ion;
+  s->clamp = !(d->flags & GM_FLAG_NO_CLAMP);
+  s->gamma.assign(d->gamma, d->gamma + s->nd);
+  const size_t N = (size_t)s->n;
+
+  // A -> CSR (a CSC input is the CSR of A^T: transpose its index structure,
+  // which is what the reference gets by flipping the cuSPARSE op,
+  // cusparse_wrapper.cpp:83-84).
+  std::vector<int> rp(N + 1, 0), ci((size_t)s->nnz);
+  std::vector<double> vv((size_t)s->nnz);
+  if (d->sparse_format == GM_SPARSE_CSR) {
+    std::copy(d->A_ptr, d->A_ptr + N + 1, rp.begin());
+    std::copy(d->A_ind, d->A_ind + s->nnz, ci.begin());
+    std::copy(d->A_values, d->A_values + s->nnz, vv.begin());
+  } else {
+    for (long long j = 0; j < s->nnz; ++j) rp[d->A_ind[j] + 1]++;
+    for (size_t i = 0; i < N; ++i) rp[i + 1] += rp[i];
+    std::vector<int> pos(rp.begin(), rp.end() - 1);
+    for (int c = 0; c < s->n; ++c)
+      for (int j = d->A_ptr[c]; j < d->A_ptr[c + 1]; ++j) {
+        const int r = d->A_ind[j];
+        ci[pos[r]] = c;
+        vv[pos[r]] = d->A_values[j];
+        pos[r]++;
+      }
+  }
+  GM_TRY(upload(&s->rowptr, rp.data(), N + 1, st));
+  GM_TRY(upload(&s->col, ci.data(), (size_t)s->nnz, st));
+  GM_TRY(upload(&s->val, vv.data(), (size_t)s->nnz, st));
+
+  // U (n x 2 column-major) -> interleaved, column 1 scaled by dt
+  // (lanczos_impl.cu:79); Usol = U (lanczos_impl.cu:81-82).
+  std::vector<double2> u2(N);
+  for (size_t i = 0; i < N; ++i) u2[i] = make_double2(d->U[i], d->U[N + i] * s->dt);
+  GM_TRY(upload(&s->Usol, u2.data(), N, st));
+  GM_TRY(dalloc(&s->U, N));
+  GM_TRY(dalloc(&s->X, N));
+  GM_TRY(dalloc(&s->upd, N));
+  GM_HIP(hipMemsetAsync(s->upd, 0, N * sizeof(double2), st));
+  GM_HIP(hipMemsetAsync(s->X, 0, N * sizeof(double2), st));
+  for (int i = 0; i < 2; ++i) {
+    GM_TRY(dalloc(&s->W[i], N));
+    GM_TRY(dalloc(&s->T[i], N));
+  }
+  GM_TRY(dalloc(&s->V, N * s->m));
+  GM_TRY(dalloc(&s->Z, s->nd > 1 ? N * s->nd : 1));
+  GM_TRY(dalloc(&s->C, (size_t)s->nd * s->m));
+  // Q column-major n x K -> row-major
+  if (s->K > 0) {
+    std::vector<double> qr(N * s->K);
+    for (int k = 0; k < s->K; ++k)
+      for (size_t i = 0; i < N; ++i) qr[i * s->K + k] = d->Q[(size_t)k * N + i];
+    GM_TRY(upload(&s->Q, qr.data(), qr.size(), st));
+    GM_TRY(upload(&s->el, d->exp_lambda, (size_t)s->K, st));
+    GM_TRY(upload(&s->pl, d->phi_lambda, (size_t)s->K, st));
+  }
+  std::vector<double> ones;
+  if (!d->mass) ones.assign(N, 1.0);
+  GM_TRY(upload(&s->mass, d->mass ? d->mass : ones.data(), N, st));
+  std::vector<double> zeros;
+  if (!d->ode) zeros.assign(N, 0.0);
+  GM_TRY(upload(&s->ode, d->ode ? d->ode : zeros.data(), N, st));
+  const size_t ld = (size_t)s->m + 1;
+  GM_TRY(dalloc(&s->aabb, 4 * ld));
+  GM_HIP(hipMemsetAsync(s->aabb, 0, 4 * ld * sizeof(double), st));
+  const size_t ny = (size_t)s->m * 2 * s->ns;
+  GM_TRY(dalloc(&s->Yr, ny));
+  GM_TRY(dalloc(&s->Yi, ny));
+  GM_HIP(hipMemsetAsync(s->Yr, 0, ny * sizeof(double), st));
+  GM_HIP(hipMemsetAsync(s->Yi, 0, ny * sizeof(double), st));
+  GM_TRY(upload(&s->wts1, d->wts1, 2 * (size_t)s->ns, st));
+  GM_TRY(upload(&s->wts2, d->wts2, 2 * (size_t)s->ns, st));
+  GM_TRY(upload(&s->shifts, d->shifts, 4 * (size_t)s->m * s->ns, st));
+  GM_TRY(upload(&s->gsr, d->gamma_s_real, (size_t)s->nd * s->ns, st));
+  GM_TRY(upload(&s->gsi, d->gamma_s_imag, (size_t)s->nd * s->ns, st));
+  GM_TRY(dalloc(&s->Zdefl, 2 * (size_t)std::max(1, s->K)));
+  GM_TRY(dalloc(&s->wD, ny));
+  GM_TRY(dalloc(&s->wDU, ny));
+  GM_TRY(dalloc(&s->wDU2, ny));
+  GM_TRY(dalloc(&s->wX, ny));
+  GM_TRY(dalloc(&s->stage, 2 * N));
+
+  // schedules
+  const double avg = s->n ? (double)s->nnz / s->n : 1.0;
+  int lanes = 1;
+  while (lanes < 64 && lanes < avg) lanes <<= 1;
+  set_schedule(s, lanes);
+  s->G_el = std::max(1, std::min(4096, cdiv_i(s->n, kBlock)));
+  s->chunk_el = cdiv_i(s->n, s->G_el);
+  s->G_df = std::max(1, std::min(512, cdiv_i(s->n, kBlock)));
+  s->chunk_df = cdiv_i(s->n, s->G_df);
+  const size_t npart = std::max<size_t>(2 * 4096, (size_t)s->G_df * 2 * std::max(1, s->K));
+  GM_TRY(dalloc(&s->partials, npart));
+  GM_TRY(dalloc(&s->counter, 1));
+  GM_TRY(dalloc(&s->status, 1));
+  GM_HIP(hipMemsetAsync(s->counter, 0, sizeof(unsigned), st));
+  GM_HIP(hipMemsetAsync(s->status, 0, sizeof(unsigned), st));
+
+  // deflation of the initial vectors (lanczos_impl.cu:84-89) and beta_0
+  s->x_zero = true;
+  GM_TRY(enqueue_update(s, gm::kUpdPrepare));
+  GM_HIP(hipStreamSynchronize(st));
+
+  if (d->flags & GM_FLAG_AUTOTUNE) {
+    // Time the build-subspace SpMM for each lane width on this matrix and
+    // keep the fastest.  Each candidate runs a short Lanczos run; the solver
+    // state is rebuilt by the next build_subspace anyway.
+    const int cands[] = {1, 2, 4, 8, 16, 32};
+    float best = 1e30f;
+    int best_l = s->lanes;
+    hipEvent_t e0, e1;
+    GM_HIP(hipEventCreate(&e0));
+    GM_HIP(hipEventCreate(&e1));
+    for (int l : cands) {
+      if (l > 4 * avg + 4) continue;
+      set_schedule(s, l);
+      GM_TRY(enqueue_build(s));  // warm-up
+      GM_HIP(hipEventRecord(e0, st));
+      GM_TRY(enqueue_build(s));
+      GM_HIP(hipEventRecord(e1, st));
+      GM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) { best = ms; best_l = l; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    set_schedule(s, best_l);
+    GM_HIP(hipMemsetAsync(s->status, 0, sizeof(unsigned), st));
+    GM_HIP(hipStreamSynchronize(st));
+  }
+  return GM_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+const char* gm_last_error(void) { return g_last_error.c_str(); }
+
+gm_status gm_lanczos_create(const gm_create_desc* desc, gm_solver** out) {
+  g_last_error.clear();
+  if (!out) return fail(GM_EINVAL, "MATLAB:lanczos_create:maxlhs: One output argument is required.");
+  *out = nullptr;
+  GM_TRY(validate(desc));
+  gm_solver* s = new (std::nothrow) gm_solver();
+  if (!s) return fail(GM_ENOMEM, "host allocation of the solver failed");
+  gm_status st = create_impl(desc, s);
+  if (st != GM_OK) {
+    std::string keep = g_last_error;
+    free_all(s);
+    delete s;
+    g_last_error = keep;
+    return st;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_registry_mu);
+    g_registry.insert(s);
+  }
+  *out = s;
+  return GM_OK;
+}
+
+gm_status gm_lanczos_delete(gm_solver* s) {
+  {
+    std::lock_guard<std::mutex> lk(g_registry_mu);
+    if (!s || !g_registry.count(s))
+      return fail(GM_EBADHANDLE, "MATLAB:lanczos_delete:inputNotHandle: invalid or deleted handle");
+    g_registry.erase(s);
+  }
+  hipSetDevice(s->dev);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  free_all(s);
+  delete s;
+  return GM_OK;
+}
+
+gm_status gm_build_subspace(gm_solver* s) {
+  GM_CHECK_HANDLE(s);
+  GM_TRY(enqueue_build(s));
+  s->built = true;
+  s->solved = false;
+  return GM_OK;
+}
+
+gm_status gm_lin_sys_solves(gm_solver* s) {
+  GM_CHECK_HANDLE(s);
+  if (!s->built) return fail(GM_ESTATE, "lin_sys_solves called before build_subspace");
+  GM_TRY(enqueue_solves(s));
+  s->solved = true;
+  return GM_OK;
+}
+
+gm_status gm_apply_V(gm_solver* s) {
+  GM_CHECK_HANDLE(s);
+  if (!s->built) return fail(GM_ESTATE, "apply_V called before build_subspace");
+  return enqueue_apply(s);
+}
+
+gm_status gm_update_U(gm_solver* s) {
+  GM_CHECK_HANDLE(s);
+  return enqueue_update(s, gm::kUpdFisher);
+}
+
+gm_status gm_update_U_monodomain(gm_solver* s) {
+  GM_CHECK_HANDLE(s);
+  return enqueue_update(s, gm::kUpdMono);
+}
+
+gm_status gm_get_U(gm_solver* s, double* dst) {
+  GM_CHECK_HANDLE(s);
+  return fetch_pair(s, s->Usol, dst);
+}
+
+gm_status gm_get_xsol(gm_solver* s, double* dst) {
+  GM_CHECK_HANDLE(s);
+  if (s->x_zero) {
+    if (!dst) return fail(GM_EINVAL, "null destination pointer");
+    GM_HIP(hipStreamSynchronize(s->stream));
+    std::fill(dst, dst + 2 * (size_t)s->n, 0.0);
+    return GM_OK;
+  }
+  return fetch_pair(s, s->X, dst);
+}
+
+gm_status gm_get_aabb(gm_solver* s, double* dst) {
+  GM_CHECK_HANDLE(s);
+  if (!dst) return fail(GM_EINVAL, "null destination pointer");
+  GM_HIP(hipMemcpyAsync(dst, s->aabb, 4 * ((size_t)s->m + 1) * sizeof(double),
+                        hipMemcpyDeviceToHost, s->stream));
+  GM_HIP(hipStreamSynchronize(s->stream));
+  return GM_OK;
+}
+
+gm_status gm_set_y(gm_solver* s, const double* yreal, const double* yimag) {
+  GM_CHECK_HANDLE(s);
+  if (!yreal || !yimag) return fail(GM_EINVAL, "null y pointer");
+  const size_t ny = (size_t)s->m * 2 * s->ns;
+  GM_HIP(hipMemcpyAsync(s->Yr, yreal, ny * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  GM_HIP(hipMemcpyAsync(s->Yi, yimag, ny * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  GM_HIP(hipStreamSynchronize(s->stream));
+  s->solved = true;
+  return GM_OK;
+}
+
+gm_status gm_get_y(gm_solver* s, double* yreal, double* yimag) {
+  GM_CHECK_HANDLE(s);
+  if (!yreal || !yimag) return fail(GM_EINVAL, "null y pointer");
+  const size_t ny = (size_t)s->m * 2 * s->ns;
+  GM_HIP(hipMemcpyAsync(yreal, s->Yr, ny * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  GM_HIP(hipMemcpyAsync(yimag, s->Yi, ny * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  GM_HIP(hipStreamSynchronize(s->stream));
+  return GM_OK;
+}
+
+gm_status gm_get_state(gm_solver* s, double* u, double* dtg, double* ode) {
+  GM_CHECK_HANDLE(s);
+  if (u || dtg) {
+    std::vector<double> tmp(2 * (size_t)s->n);
+    GM_TRY(fetch_pair(s, s->Usol, tmp.data()));
+    if (u) std::copy(tmp.begin(), tmp.begin() + s->n, u);
+    if (dtg) std::copy(tmp.begin() + s->n, tmp.end(), dtg);
+  }
+  if (ode) {
+    GM_HIP(hipMemcpyAsync(ode, s->ode, (size_t)s->n * sizeof(double), hipMemcpyDeviceToHost,
+                          s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
+  }
+  return GM_OK;
+}
+
+gm_status gm_set_state(gm_solver* s, const double* u, const double* dtg, const double* ode) {
+  GM_CHECK_HANDLE(s);
+  const size_t N = (size_t)s->n;
+  if (u || dtg) {
+    std::vector<double> cur(2 * N);
+    GM_TRY(fetch_pair(s, s->Usol, cur.data()));
+    std::vector<double2> u2(N);
+    for (size_t i = 0; i < N; ++i)
+      u2[i] = make_double2(u ? u[i] : cur[i], dtg ? dtg[i] : cur[N + i]);
+    GM_HIP(hipMemcpyAsync(s->Usol, u2.data(), N * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
+  }
+  if (ode) {
+    GM_HIP(hipMemcpyAsync(s->ode, ode, N * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    GM_HIP(hipStreamSynchronize(s->stream));
+  }
+  s->x_zero = true;
+  GM_TRY(enqueue_update(s, gm::kUpdPrepare));
+  GM_HIP(hipStreamSynchronize(s->stream));
+  return GM_OK;
+}
+
+gm_status gm_step(gm_solver* s, int32_t nsteps, int32_t monodomain, int32_t use_graph) {
+  GM_CHECK_HANDLE(s);
+  if (nsteps < 0) return fail(GM_EINVAL, "nsteps must be >= 0");
+  const int mono = monodomain ? 1 : 0;
+  if (!s->x_zero) {
+    // a partial step is pending (apply_V without update_U): finish it eagerly
+    use_graph = 0;
+  }
+  if (!use_graph) {
+    for (int i = 0; i < nsteps; ++i) GM_TRY(enqueue_step(s, mono));
+  } else {
+    if (!s->graph[mono]) {
+      hipGraph_t g = nullptr;
+      GM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      gm_status st = enqueue_step(s, mono);
+      hipError_t e = hipStreamEndCapture(s->stream, &g);
+      if (st != GM_OK) {
+        if (g) hipGraphDestroy(g);
+        return st;
+      }
+      if (e != hipSuccess) return fail(GM_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&s->graph[mono], g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        s->graph[mono] = nullptr;
+        return fail(GM_EDEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+      }
+    }
+    for (int i = 0; i < nsteps; ++i) GM_HIP(hipGraphLaunch(s->graph[mono], s->stream));
+  }
+  if (nsteps > 0) {
+    s->built = true;
+    s->solved = true;
+    s->x_zero = true;
+  }
+  return GM_OK;
+}
+
+gm_status gm_synchronize(gm_solver* s) {
+  GM_CHECK_HANDLE(s);
+  return check_status(s);
+}
+
+gm_status gm_get_info(gm_solver* s, int64_t* n, int64_t* nnz, int32_t* m, int32_t* gamma_size,
+                      int32_t* contour_size, int32_t* num_eigs, int32_t* spmm_lanes) {
+  GM_CHECK_HANDLE(s);
+  if (n) *n = s->n;
+  if (nnz) *nnz = s->nnz;
+  if (m) *m = s->m;
+  if (gamma_size) *gamma_size = s->nd;
+  if (contour_size) *contour_size = s->ns;
+  if (num_eigs) *num_eigs = s->K;
+  if (spmm_lanes) *spmm_lanes = s->lanes;
+  return GM_OK;
+}
+
+gm_status gm_get_stream(gm_solver* s, void** stream) {
+  GM_CHECK_HANDLE(s);
+  if (!stream) return fail(GM_EINVAL, "null stream pointer");
+  *stream = reinterpret_cast<void*>(s->stream);
+  return GM_OK;
+}
+
+}  // extern "C"

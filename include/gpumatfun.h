@@ -1,0 +1,184 @@
+/*
+ * gpumatfun.h -- C ABI of the MI355X (gfx950) f(A)b engine.
+ *
+ * The entry points mirror the reference's nine MATLAB MEX gateways one for
+ * one; each declaration cites the gateway (and the solver method behind it)
+ * that it replaces.  A MEX shim, a ctypes binding or any FFI binds these
+ * symbols directly: plain pointers and sizes, no C++ or torch types.
+ *
+ * Conventions (all entry points):
+ *   - every function returns gm_status and never aborts; the text of the
+ *     last error of the calling thread is available from gm_last_error();
+ *   - dense arrays are column-major double precision, as MATLAB passes them;
+ *   - all host arrays given to gm_lanczos_create are deep-copied;
+ *   - a solver owns one HIP stream; the stepping entry points only enqueue
+ *     work on it, the getters synchronise it; one thread at a time per solver;
+ *   - handles are checked against the set of live solvers, so a deleted or
+ *     unknown handle gives GM_EBADHANDLE instead of undefined behaviour.
+ */
+#ifndef GPUMATFUN_H
+#define GPUMATFUN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gm_status {
+    GM_OK = 0,
+    GM_EINVAL = 1,       /* invalid argument (shape, type, value)            */
+    GM_ENOMEM = 2,       /* device or host allocation failed                 */
+    GM_EDEVICE = 3,      /* HIP runtime error or no usable gfx950 device     */
+    GM_EBREAKDOWN = 4,   /* Lanczos breakdown (beta = 0 or non-finite)       */
+    GM_EBADHANDLE = 5,   /* null, deleted or unknown solver handle           */
+    GM_ESTATE = 6        /* call out of order (e.g. apply_V before a build)  */
+} gm_status;
+
+typedef struct gm_solver gm_solver;
+
+/* Sparse input format of the matrix A. */
+enum {
+    GM_SPARSE_CSC = 0,   /* MATLAB layout: ptr = jc (n+1), ind = ir (nnz)    */
+    GM_SPARSE_CSR = 1
+};
+
+/* Flags of gm_create_desc.flags. */
+enum {
+    GM_FLAG_NO_CLAMP = 1u << 0,  /* update_U without the u >= 0 clamp of
+                                    update_U.cu:9 (the CPU path's semantics) */
+    GM_FLAG_AUTOTUNE = 1u << 1   /* time the SpMM row-to-lane schedules on A at
+                                    create and keep the fastest              */
+};
+
+/*
+ * The 26 positional inputs of lanczos_create (lanczos_create.cpp:7-32), in
+ * the same order, plus the sizes MATLAB carries inside its mxArrays.
+ */
+typedef struct gm_create_desc {
+    int32_t n;                     /* rows of A (= rows of U)               */
+    const double* U;               /* [0]  n x 2: [u, g(u)]                 */
+    int32_t sparse_format;         /* [1]  A: GM_SPARSE_CSC or _CSR         */
+    int64_t nnz;
+    const double* A_values;        /*      nnz                              */
+    const int32_t* A_ptr;          /*      n + 1                            */
+    const int32_t* A_ind;          /*      nnz                              */
+    int32_t num_eigs;              /*      K (columns of Q), may be 0       */
+    const double* Q;               /* [2]  n x K                            */
+    const double* lambda;          /* [3]  K (unused, as in the reference)  */
+    int32_t subspace_size;         /* [4]  m, the "N" argument              */
+    int32_t gamma_size;            /*      d + 1                            */
+    const double* gamma;           /* [5]  d + 1, highest power first       */
+    int32_t contour_size;          /*      n_s                              */
+    const double* gamma_s_real;    /* [6]  (d+1) x n_s                      */
+    const double* gamma_s_imag;    /* [7]  (d+1) x n_s                      */
+    const double* exp_lambda;      /* [8]  K                                */
+    const double* phi_lambda;      /* [9]  K                                */
+    double dt;                     /* [10]                                  */
+    const double* ode;             /* [11] n (may be NULL: zeros)           */
+    double aode, bode, code;       /* [12..14]                              */
+    double vrest, vamp, vth;       /* [15..17] (vth unused)                 */
+    double vpeak, c1ion, c2ion;    /* [18..20] (c2ion unused)               */
+    const double* mass;            /* [21] n (may be NULL: ones)            */
+    const double* wts1;            /* [22] n_s x 2 [Re Im], includes s      */
+    const double* wts2;            /* [23] n_s x 2                          */
+    const double* shifts;          /* [24] 4 * m * n_s, packed as in
+                                            problem_startup.m:140-144       */
+    const double* valvec;          /* [25] must be [-1, 0, 1] (may be NULL) */
+    uint32_t flags;                /* GM_FLAG_*                             */
+    int32_t device;                /* HIP device ordinal                    */
+} gm_create_desc;
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* gm_last_error(void);
+
+/* replaces lanczos_create.cpp:35-768 (mexFunction) and the constructor
+ * lanczos_impl.cu:5-91: upload, U(:,1) *= dt, deflation Z = Q'U,
+ * U -= QZ, update = Q diag(e^L, phi(L)) Z.                                 */
+gm_status gm_lanczos_create(const gm_create_desc* desc, gm_solver** out);
+
+/* replaces lanczos_delete.cpp:6-35.                                        */
+gm_status gm_lanczos_delete(gm_solver* s);
+
+/* replaces build_subspace.cpp:6-35 -> build_subspace.cu:5-91.            */
+gm_status gm_build_subspace(gm_solver* s);
+
+/* replaces lin_sys_solves.cpp:6-35 -> lin_sys_solves.cu:5-21
+ * (copyforsolve/linsyssolve/copyafter, lanczos_impl.cu:292-406).          */
+gm_status gm_lin_sys_solves(gm_solver* s);
+
+/* replaces apply_V.cpp:6-35 -> apply_V.cu:5-95.                           */
+gm_status gm_apply_V(gm_solver* s);
+
+/* replaces update_U.cpp:6-35 -> update_U.cu:5-32.                         */
+gm_status gm_update_U(gm_solver* s);
+
+/* replaces update_U_monodomain.cpp:6-35 -> update_U_monodomain.cu:5-43.   */
+gm_status gm_update_U_monodomain(gm_solver* s);
+
+/* replaces get_U.cpp:6-38 -> lanczos_impl.cu:502-506: dst is n x 2
+ * column-major host memory, receives Usol = [u, dt*g(u)].                  */
+gm_status gm_get_U(gm_solver* s, double* dst);
+
+/* replaces get_xsol.cpp:6-38 -> lanczos_impl.cu:508-512: dst is n x 2,
+ * receives X = [f1(A) ubar, f2(A) dt*gbar] (without the deflation update). */
+gm_status gm_get_xsol(gm_solver* s, double* dst);
+
+/* replaces get_aabb_mex, called at lanczosEngine.m:25-27 but absent from
+ * the reference tree: dst is (m+1) x 4 [alpha1 alpha2 beta1 beta2].       */
+gm_status gm_get_aabb(gm_solver* s, double* dst);
+
+/* replaces set_y_mex, called at lanczosEngine.m:29-31 but absent from the
+ * reference tree: overwrites Yreal/Yimag (each m x 2n_s).                  */
+gm_status gm_set_y(gm_solver* s, const double* yreal, const double* yimag);
+
+/* Reads Yreal/Yimag (m x 2n_s each), the output of lin_sys_solves.        */
+gm_status gm_get_y(gm_solver* s, double* yreal, double* yimag);
+
+/* State get/set for checkpoint/resume (no reference counterpart: the
+ * reference's get_U returns [u, dt*g] and create multiplies by dt again,
+ * lanczos_impl.cu:79, so it cannot round-trip).  u, g (without dt) and ode
+ * are n-vectors; any pointer may be NULL to skip it.  set_state recomputes
+ * the deflation split exactly as update_U does.                           */
+gm_status gm_get_state(gm_solver* s, double* u, double* g, double* ode);
+gm_status gm_set_state(gm_solver* s, const double* u, const double* g, const double* ode);
+
+/* One or more full time steps: build_subspace, lin_sys_solves, apply_V,
+ * update_U (or update_U_monodomain), the sequence of
+ * nicksmethodgpu_problem.m:7-16 / nicksmethodgpu_problem_monodomain.m:7-16.
+ * use_graph != 0 captures one step into a HIP graph on first use and
+ * replays it.  Asynchronous: returns once the work is enqueued.            */
+gm_status gm_step(gm_solver* s, int32_t nsteps, int32_t monodomain, int32_t use_graph);
+
+/* Waits for the solver's stream; returns GM_EBREAKDOWN if any Lanczos run
+ * since create met beta = 0 or a non-finite beta (the affected basis
+ * vectors are set to zero instead of propagating NaN).                     */
+gm_status gm_synchronize(gm_solver* s);
+
+/* Introspection: sizes and the chosen SpMM schedule (lanes per row).      */
+gm_status gm_get_info(gm_solver* s, int64_t* n, int64_t* nnz, int32_t* m,
+                      int32_t* gamma_size, int32_t* contour_size,
+                      int32_t* num_eigs, int32_t* spmm_lanes);
+
+/* The solver's HIP stream (as void*), for callers that time it with
+ * events of their own.                                                      */
+gm_status gm_get_stream(gm_solver* s, void** stream);
+
+/* Measurement helpers (no reference counterpart).  gm_profile_spmm launches
+ * one SpMM variant of the step `reps` times on the solver's stream between
+ * two HIP events and returns the average duration of one launch in ms:
+ * mode 0 = the Horner inner step of build_subspace (y = A x + g v_k, the
+ * most frequent kernel of a step with a polynomial preconditioner),
+ * mode 1 = the Lanczos step without preconditioner (y = A v - b v_{k-1}
+ * fused with the alpha dot product).  It overwrites only scratch buffers;
+ * call it between steps.  gm_profile_copy times a plain device-to-device
+ * copy kernel over `bytes` bytes (read once, written once), the reference
+ * point for the HBM counters.                                               */
+gm_status gm_profile_spmm(gm_solver* s, int32_t mode, int32_t reps, double* avg_ms);
+gm_status gm_profile_copy(gm_solver* s, int64_t bytes, int32_t reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUMATFUN_H */
