@@ -25,6 +25,7 @@ EXPORTS = (
     "gm_lin_sys_solves", "gm_apply_V", "gm_update_U", "gm_update_U_monodomain",
     "gm_get_U", "gm_get_xsol", "gm_get_aabb", "gm_set_y", "gm_get_y", "gm_get_state",
     "gm_set_state", "gm_step", "gm_synchronize", "gm_get_info", "gm_get_stream",
+    "gm_profile_spmm", "gm_profile_copy",
 )
 
 _dp = C.POINTER(C.c_double)
@@ -111,6 +112,8 @@ def load():
                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32)]),
         "gm_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
+        "gm_profile_spmm": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
+        "gm_profile_copy": (C.c_int, [vp, C.c_int64, C.c_int32, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -838,4 +838,79 @@ gm_status gm_get_stream(gm_solver* s, void** stream) {
   return GM_OK;
 }
 
+gm_status gm_profile_spmm(gm_solver* s, int32_t mode, int32_t reps, double* avg_ms) {
+  GM_CHECK_HANDLE(s);
+  if (!avg_ms || reps < 1) return fail(GM_EINVAL, "avg_ms must be non-null and reps >= 1");
+  if (mode != 0 && mode != 1) return fail(GM_EINVAL, "mode must be 0 or 1");
+  if (!s->built) return fail(GM_ESTATE, "profile_spmm needs a built subspace (run a step first)");
+  const size_t N = (size_t)s->n;
+  SpmmArgs a = spmm_base(s);
+  a.g0 = s->gamma[0];
+  a.gi = s->nd > 1 ? s->gamma[1] : 0.0;
+  // Inputs are V[0] (unit norm) and scratch T/W buffers; k = 1 so the fused
+  // beta/dot terms are live.  The alpha written to aabb row 1 is overwritten
+  // by the next build_subspace.
+  const int k = s->m > 1 ? 1 : 0;
+  a.k = k;
+  a.vk = s->V + (size_t)k * N;
+  a.vkm1 = k > 0 ? s->V : nullptr;
+  hipEvent_t e0, e1;
+  GM_HIP(hipEventCreate(&e0));
+  GM_HIP(hipEventCreate(&e1));
+  auto one = [&](int i) -> gm_status {
+    if (mode == 0) {
+      a.x = (i & 1) ? s->T[1] : s->V;
+      a.y = (i & 1) ? s->T[0] : s->T[1];
+      return launch_spmm_mode<gm::kMid>(s, a);
+    }
+    a.x = s->V + (size_t)k * N;
+    a.y = s->W[i & 1];
+    return launch_spmm_mode<gm::kLast>(s, a);
+  };
+  gm_status st = one(0);  // warm-up
+  if (st == GM_OK) {
+    GM_HIP(hipEventRecord(e0, s->stream));
+    for (int i = 0; i < reps && st == GM_OK; ++i) st = one(i);
+    GM_HIP(hipEventRecord(e1, s->stream));
+    GM_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = (double)ms / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
+
+gm_status gm_profile_copy(gm_solver* s, int64_t bytes, int32_t reps, double* avg_ms) {
+  GM_CHECK_HANDLE(s);
+  if (!avg_ms || reps < 1 || bytes < 16) return fail(GM_EINVAL, "bad profile_copy arguments");
+  const long long n = bytes / 16;
+  double2 *src = nullptr, *dst = nullptr;
+  GM_TRY(dalloc(&src, (size_t)n));
+  gm_status st = dalloc(&dst, (size_t)n);
+  if (st != GM_OK) {
+    (void)hipFree(src);
+    return st;
+  }
+  hipEvent_t e0, e1;
+  GM_HIP(hipEventCreate(&e0));
+  GM_HIP(hipEventCreate(&e1));
+  const int grid = (int)std::min<long long>(8192, (n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(gm::k_copy16, dim3(grid), dim3(kBlock), 0, s->stream, src, dst, n);
+  GM_HIP(hipEventRecord(e0, s->stream));
+  for (int i = 0; i < reps; ++i)
+    hipLaunchKernelGGL(gm::k_copy16, dim3(grid), dim3(kBlock), 0, s->stream, src, dst, n);
+  GM_HIP(hipEventRecord(e1, s->stream));
+  GM_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = (double)ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return GM_OK;
+}
+
 }  // extern "C"

@@ -249,3 +249,15 @@ class LanczosEngine:
         p = C.c_void_p()
         self._call(self._lib.gm_get_stream, C.byref(p))
         return p.value
+
+    def profile_spmm(self, mode=0, reps=50):
+        """Average duration (ms) of one SpMM launch, timed with HIP events on the solver stream."""
+        out = C.c_double()
+        self._call(self._lib.gm_profile_spmm, int(mode), int(reps), C.byref(out))
+        return out.value
+
+    def profile_copy(self, nbytes, reps=20):
+        """Average duration (ms) of a plain device copy of ``nbytes`` bytes."""
+        out = C.c_double()
+        self._call(self._lib.gm_profile_copy, int(nbytes), int(reps), C.byref(out))
+        return out.value

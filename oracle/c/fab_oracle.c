@@ -1,0 +1,333 @@
+/*
+ * fab_oracle.c -- C restatement of the reference's MATLAB CPU path, for
+ * TESTS AND THE CPU BASELINE ONLY (never linked into the product library).
+ *
+ *   lanczos            lanczos.m:1-54
+ *   jacobi_poly_eval   jacobi_poly_eval.m:1-19
+ *   shifted solves     nicksmethodcpu.m:17-21 (sparse backslash on a
+ *                      tridiagonal matrix = LAPACK ZGTSV: Gaussian elimination
+ *                      with partial pivoting)
+ *   recombination      nicksmethodcpu.m:33-43 (node by node, as written)
+ *   step loop          problem_startup_CPU_nonuniform.m:110-122 (with mass = 1
+ *                      it is problem_startup_CPU.m:119-129)
+ *
+ * Sparse products are row-parallel (OpenMP) with each row summed in storage
+ * order, so they are bit-identical for any thread count; dot products and
+ * norms are summed in fixed 4096-element chunks, then over chunks in order,
+ * so they are also independent of the thread count.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex zd;
+
+#define CHUNK 4096
+
+static double dot_fixed(long n, const double* a, const double* b) {
+  long nc = (n + CHUNK - 1) / CHUNK;
+  double* part = (double*)malloc(sizeof(double) * (nc > 0 ? nc : 1));
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    long lo = c * CHUNK, hi = lo + CHUNK < n ? lo + CHUNK : n;
+    double s = 0.0;
+    for (long i = lo; i < hi; ++i) s += a[i] * b[i];
+    part[c] = s;
+  }
+  double s = 0.0;
+  for (long c = 0; c < nc; ++c) s += part[c];
+  free(part);
+  return s;
+}
+
+/* y(:,c) = A x(:,c) for ncol columns, column-major with leading dim n. */
+static void spmm(long n, const int* rp, const int* ci, const double* va, int ncol,
+                 const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    for (int c = 0; c < ncol; ++c) {
+      const double* xc = x + (size_t)c * n;
+      double s = 0.0;
+      for (int j = rp[i]; j < rp[i + 1]; ++j) s += va[j] * xc[ci[j]];
+      y[(size_t)c * n + i] = s;
+    }
+  }
+}
+
+/* jacobi_poly_eval.m with z = 1 column (real and imaginary parts). */
+static void poly_eval(long n, const int* rp, const int* ci, const double* va, int nd,
+                      const double* gr, const double* gi, const double* br, const double* bi,
+                      double* xr, double* xi, double* work /* 4n */) {
+  for (long k = 0; k < n; ++k) {
+    xr[k] = br[k] * gr[0] - bi[k] * gi[0];
+    xi[k] = br[k] * gi[0] + bi[k] * gr[0];
+  }
+  double* in = work;
+  double* out = work + 2 * n;
+  for (int i = 1; i < nd; ++i) {
+    memcpy(in, xr, sizeof(double) * n);
+    memcpy(in + n, xi, sizeof(double) * n);
+    spmm(n, rp, ci, va, 2, in, out);
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < n; ++k) {
+      xr[k] = out[k] + br[k] * gr[i] - bi[k] * gi[i];
+      xi[k] = out[n + k] + br[k] * gi[i] + bi[k] * gr[i];
+    }
+  }
+}
+
+/* lanczos.m for two columns; V1, V2 n x m column-major; al (m), be (m+1). */
+static void lanczos2(long n, const int* rp, const int* ci, const double* va, int m, int nd,
+                     const double* gamma, const double* a, const double* b, double* V1,
+                     double* V2, double* al1, double* be1, double* al2, double* be2,
+                     double* work /* 8n */) {
+  double* w = work;               /* n x 2 */
+  double* t = work + 2 * n;       /* n x 2 */
+  double* u = work + 4 * n;       /* n x 2 */
+  double* zero = work + 6 * n;    /* 2n zeros */
+  memset(zero, 0, sizeof(double) * 2 * n);
+  be1[0] = sqrt(dot_fixed(n, a, a));
+  be2[0] = sqrt(dot_fixed(n, b, b));
+  for (long k = 0; k < n; ++k) {
+    V1[k] = a[k] / be1[0];
+    V2[k] = b[k] / be2[0];
+  }
+  for (int j = 0; j < m; ++j) {
+    const double* v1 = V1 + (size_t)j * n;
+    const double* v2 = V2 + (size_t)j * n;
+    /* w = p(A) [v1 v2]  (jacobi_poly_eval with zero imaginary parts) */
+    for (long k = 0; k < n; ++k) {
+      u[k] = v1[k] * gamma[0];
+      u[n + k] = v2[k] * gamma[0];
+    }
+    for (int i = 1; i < nd; ++i) {
+      spmm(n, rp, ci, va, 2, u, t);
+#pragma omp parallel for schedule(static)
+      for (long k = 0; k < n; ++k) {
+        u[k] = t[k] + v1[k] * gamma[i];
+        u[n + k] = t[n + k] + v2[k] * gamma[i];
+      }
+    }
+    spmm(n, rp, ci, va, 2, u, w);                                  /* lanczos.m:27 */
+    if (j != 0) {                                                  /* :30-32 */
+      const double* p1 = V1 + (size_t)(j - 1) * n;
+      const double* p2 = V2 + (size_t)(j - 1) * n;
+#pragma omp parallel for schedule(static)
+      for (long k = 0; k < n; ++k) {
+        w[k] = w[k] - be1[j] * p1[k];
+        w[n + k] = w[n + k] - be2[j] * p2[k];
+      }
+    }
+    al1[j] = dot_fixed(n, w, v1);                                  /* :35-36 */
+    al2[j] = dot_fixed(n, w + n, v2);
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < n; ++k) {                                 /* :38 */
+      w[k] = w[k] - al1[j] * v1[k];
+      w[n + k] = w[n + k] - al2[j] * v2[k];
+    }
+    be1[j + 1] = sqrt(dot_fixed(n, w, w));                         /* :40-41 */
+    be2[j + 1] = sqrt(dot_fixed(n, w + n, w + n));
+    if (j < m - 1) {                                               /* :44-47 */
+      double* n1 = V1 + (size_t)(j + 1) * n;
+      double* n2 = V2 + (size_t)(j + 1) * n;
+#pragma omp parallel for schedule(static)
+      for (long k = 0; k < n; ++k) {
+        n1[k] = w[k] / be1[j + 1];
+        n2[k] = w[n + k] / be2[j + 1];
+      }
+    }
+  }
+}
+
+static double cabs1(zd z) { return fabs(creal(z)) + fabs(cimag(z)); }
+
+/* ZGTSV: returns 0 or the 1-based index of a zero pivot. */
+static int zgtsv1(int n, zd* dl, zd* d, zd* du, zd* b) {
+  zd* du2 = (zd*)calloc(n > 2 ? n - 2 : 1, sizeof(zd));
+  int info = 0;
+  for (int k = 0; k < n - 1 && !info; ++k) {
+    if (dl[k] == 0) {
+      if (d[k] == 0) info = k + 1;
+    } else if (cabs1(d[k]) >= cabs1(dl[k])) {
+      zd mult = dl[k] / d[k];
+      d[k + 1] = d[k + 1] - mult * du[k];
+      b[k + 1] = b[k + 1] - mult * b[k];
+    } else {
+      zd mult = d[k] / dl[k];
+      d[k] = dl[k];
+      zd temp = d[k + 1];
+      d[k + 1] = du[k] - mult * temp;
+      if (k < n - 2) {
+        du2[k] = du[k + 1];
+        du[k + 1] = -mult * du2[k];
+      }
+      du[k] = temp;
+      temp = b[k];
+      b[k] = b[k + 1];
+      b[k + 1] = temp - mult * b[k + 1];
+    }
+  }
+  if (!info && d[n - 1] == 0) info = n;
+  if (!info) {
+    b[n - 1] = b[n - 1] / d[n - 1];
+    if (n > 1) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / d[n - 2];
+    for (int k = n - 3; k >= 0; --k)
+      b[k] = (b[k] - du[k] * b[k + 1] - du2[k] * b[k + 2]) / d[k];
+  }
+  free(du2);
+  return info;
+}
+
+/* y(:,j) = s*w_j*((etaxi_j I - H) \ beta0 e1), real/imag parts m x ns. */
+static int shifted(int m, int ns, const double* al, const double* be, const double* etaxi,
+                   const double* w, double s, double* yr, double* yi) {
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int j = 0; j < ns; ++j) {
+    zd* dl = (zd*)malloc(sizeof(zd) * m);
+    zd* d = (zd*)malloc(sizeof(zd) * m);
+    zd* du = (zd*)malloc(sizeof(zd) * m);
+    zd* x = (zd*)malloc(sizeof(zd) * m);
+    zd ex = etaxi[2 * j] + I * etaxi[2 * j + 1];
+    for (int k = 0; k < m; ++k) {
+      d[k] = ex - al[k];
+      x[k] = 0;
+    }
+    for (int k = 0; k + 1 < m; ++k) dl[k] = du[k] = -be[k + 1];
+    x[0] = be[0];
+    if (zgtsv1(m, dl, d, du, x)) bad = 1;
+    zd sc = s * (w[2 * j] + I * w[2 * j + 1]);
+    for (int k = 0; k < m; ++k) {
+      zd v = sc * x[k];
+      yr[(size_t)j * m + k] = creal(v);
+      yi[(size_t)j * m + k] = cimag(v);
+    }
+    free(dl); free(d); free(du); free(x);
+  }
+  return bad;
+}
+
+/* out = V y  (n x m times m) */
+static void gemv(long n, int m, const double* V, const double* y, double* out) {
+#pragma omp parallel for schedule(static)
+  for (long k = 0; k < n; ++k) {
+    double s = 0.0;
+    for (int j = 0; j < m; ++j) s += V[(size_t)j * n + k] * y[j];
+    out[k] = s;
+  }
+}
+
+/*
+ * nicksmethodcpu.m: v1 ~ f1(A) a, v2 ~ f2(A) b.
+ *   wts1/wts2/etaxi: complex interleaved (ns); gsr/gsi: nd x ns column-major.
+ */
+int fab_nicksmethodcpu(long n, const int* rp, const int* ci, const double* va, int m, int nd,
+                       const double* gamma, int ns, const double* gsr, const double* gsi,
+                       const double* wts1, const double* wts2, double s, const double* etaxi,
+                       const double* a, const double* b, double* v1, double* v2) {
+  double* V1 = (double*)malloc(sizeof(double) * (size_t)n * m);
+  double* V2 = (double*)malloc(sizeof(double) * (size_t)n * m);
+  double* work = (double*)malloc(sizeof(double) * 8 * (size_t)n);
+  double* al1 = (double*)malloc(sizeof(double) * m);
+  double* al2 = (double*)malloc(sizeof(double) * m);
+  double* be1 = (double*)malloc(sizeof(double) * (m + 1));
+  double* be2 = (double*)malloc(sizeof(double) * (m + 1));
+  double* y1r = (double*)malloc(sizeof(double) * (size_t)m * ns);
+  double* y1i = (double*)malloc(sizeof(double) * (size_t)m * ns);
+  double* y2r = (double*)malloc(sizeof(double) * (size_t)m * ns);
+  double* y2i = (double*)malloc(sizeof(double) * (size_t)m * ns);
+  double* Yr = (double*)malloc(sizeof(double) * (size_t)n);
+  double* Yi = (double*)malloc(sizeof(double) * (size_t)n);
+  double* xr = (double*)malloc(sizeof(double) * (size_t)n);
+  double* xi = (double*)malloc(sizeof(double) * (size_t)n);
+  lanczos2(n, rp, ci, va, m, nd, gamma, a, b, V1, V2, al1, be1, al2, be2, work);
+  int bad = shifted(m, ns, al1, be1, etaxi, wts1, s, y1r, y1i);
+  bad |= shifted(m, ns, al2, be2, etaxi, wts2, s, y2r, y2i);
+  memset(v1, 0, sizeof(double) * n);
+  memset(v2, 0, sizeof(double) * n);
+  for (int i = 0; i < ns; ++i) {                                  /* nicksmethodcpu.m:33 */
+    for (int col = 0; col < 2; ++col) {
+      const double* V = col == 0 ? V1 : V2;
+      const double* yr = (col == 0 ? y1r : y2r) + (size_t)i * m;
+      const double* yi = (col == 0 ? y1i : y2i) + (size_t)i * m;
+      double* v = col == 0 ? v1 : v2;
+      gemv(n, m, V, yr, Yr);                                      /* :34-37 */
+      gemv(n, m, V, yi, Yi);
+      poly_eval(n, rp, ci, va, nd, gsr + (size_t)i * nd, gsi + (size_t)i * nd, Yr, Yi, xr,
+                xi, work);                                        /* :39-40 */
+#pragma omp parallel for schedule(static)
+      for (long k = 0; k < n; ++k) v[k] = v[k] + xi[k];            /* :41-42 */
+    }
+  }
+  free(V1); free(V2); free(work); free(al1); free(al2); free(be1); free(be2);
+  free(y1r); free(y1i); free(y2r); free(y2i); free(Yr); free(Yi); free(xr); free(xi);
+  return bad;
+}
+
+/*
+ * `steps` time steps of problem_startup_CPU_nonuniform.m:110-122:
+ *   deflate (Q n x K column-major), f(A) products, u = x1 + dt x2 + up1 + dt up2,
+ *   g = mass ubar (1 - ubar), ubar = u / mass.
+ * u and gu are updated in place.
+ */
+int fab_cpu_steps(long n, const int* rp, const int* ci, const double* va, int K,
+                  const double* Q, const double* el, const double* pl, double dt, int m,
+                  int nd, const double* gamma, int ns, const double* gsr, const double* gsi,
+                  const double* wts1, const double* wts2, double s, const double* etaxi,
+                  const double* mass, int steps, double* u, double* gu) {
+  double* xb = (double*)malloc(sizeof(double) * n);
+  double* yb = (double*)malloc(sizeof(double) * n);
+  double* up1 = (double*)calloc(n, sizeof(double));
+  double* up2 = (double*)calloc(n, sizeof(double));
+  double* x1 = (double*)malloc(sizeof(double) * n);
+  double* x2 = (double*)malloc(sizeof(double) * n);
+  double* Qu = (double*)malloc(sizeof(double) * (K > 0 ? K : 1));
+  double* Qg = (double*)malloc(sizeof(double) * (K > 0 ? K : 1));
+  int bad = 0;
+  for (int st = 0; st < steps; ++st) {
+    for (int k = 0; k < K; ++k) {                                   /* Q'*u, Q'*gu */
+      Qu[k] = dot_fixed(n, Q + (size_t)k * n, u);
+      Qg[k] = dot_fixed(n, Q + (size_t)k * n, gu);
+    }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+      double cu = 0.0, cg = 0.0, e1 = 0.0, e2 = 0.0;
+      for (int k = 0; k < K; ++k) {
+        const double q = Q[(size_t)k * n + i];
+        cu += q * Qu[k];
+        cg += q * Qg[k];
+        e1 += q * (el[k] * Qu[k]);
+        e2 += q * (pl[k] * Qg[k]);
+      }
+      xb[i] = u[i] - cu;
+      yb[i] = gu[i] - cg;
+      up1[i] = e1;
+      up2[i] = e2;
+    }
+    bad |= fab_nicksmethodcpu(n, rp, ci, va, m, nd, gamma, ns, gsr, gsi, wts1, wts2, s, etaxi,
+                              xb, yb, x1, x2);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+      u[i] = x1[i] + dt * x2[i] + up1[i] + dt * up2[i];
+      const double ub = u[i] / mass[i];
+      gu[i] = mass[i] * ub * (1 - ub);
+    }
+  }
+  free(xb); free(yb); free(up1); free(up2); free(x1); free(x2); free(Qu); free(Qg);
+  return bad;
+}
+
+int fab_set_threads(int t) {
+#ifdef _OPENMP
+  if (t > 0) omp_set_num_threads(t);
+  return omp_get_max_threads();
+#else
+  (void)t;
+  return 1;
+#endif
+}

@@ -1,0 +1,94 @@
+"""ctypes wrapper of the C restatement ``oracle/c/fab_oracle.c`` (test infrastructure).
+
+``build()`` compiles it with gcc/OpenMP into ``oracle/_build/libfab_oracle.so``;
+``cpu_steps_c`` runs the step loop of ``problem_startup_CPU_nonuniform.m``
+(mass = 1 gives ``problem_startup_CPU.m``) on ``threads`` host threads.
+"""
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "c", "fab_oracle.c")
+OUT = os.path.join(HERE, "_build", "libfab_oracle.so")
+
+_lib = None
+
+
+def build():
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    tmp = OUT + ".tmp"
+    cmd = ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c11", "-o", tmp, SRC, "-lm"]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(OUT):
+            build()
+        lib = C.CDLL(OUT)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        lib.fab_cpu_steps.restype = C.c_int
+        lib.fab_cpu_steps.argtypes = [C.c_long, ip, ip, dp, C.c_int, dp, dp, dp, C.c_double,
+                                      C.c_int, C.c_int, dp, C.c_int, dp, dp, dp, dp, C.c_double,
+                                      dp, dp, C.c_int, dp, dp]
+        lib.fab_set_threads.restype = C.c_int
+        lib.fab_set_threads.argtypes = [C.c_int]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _cplx(z):
+    z = np.asarray(z, dtype=np.complex128).ravel()
+    out = np.empty(2 * z.size)
+    out[0::2] = z.real
+    out[1::2] = z.imag
+    return out
+
+
+def cpu_steps_c(setup, u0, gu0, steps, threads=0):
+    """Same contract as ``oracle.cpu_steps_nonuniform`` (mass from setup, default ones)."""
+    lib = load()
+    used = lib.fab_set_threads(int(threads))
+    A = setup["A"].tocsr()
+    A.sort_indices()
+    n = A.shape[0]
+    rp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+    ci = np.ascontiguousarray(A.indices, dtype=np.int32)
+    va = np.ascontiguousarray(A.data, dtype=np.float64)
+    Q = setup.get("Q")
+    K = 0 if Q is None else Q.shape[1]
+    Qf = np.ascontiguousarray(np.asarray(Q, dtype=np.float64).T.ravel()) if K else np.zeros(1)
+    el = np.ascontiguousarray(setup["exp_lambda"], dtype=np.float64) if K else np.zeros(1)
+    pl = np.ascontiguousarray(setup["phi_lambda"], dtype=np.float64) if K else np.zeros(1)
+    gamma = np.ascontiguousarray(setup["gamma"], dtype=np.float64)
+    nd = gamma.size
+    gsr = np.ascontiguousarray(np.asarray(setup["gamma_s_real"]).reshape(nd, -1).T.ravel())
+    gsi = np.ascontiguousarray(np.asarray(setup["gamma_s_imag"]).reshape(nd, -1).T.ravel())
+    ns = int(setup["n"])
+    w1 = _cplx(setup["wts1_cpu"][:ns])
+    w2 = _cplx(setup["wts2_cpu"][:ns])
+    ex = _cplx(setup["etaxi"][:ns])
+    mass = setup.get("mass")
+    mass = np.ones(n) if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+    u = np.array(u0, dtype=np.float64)
+    gu = np.array(gu0, dtype=np.float64)
+    bad = lib.fab_cpu_steps(n, rp.ctypes.data_as(C.POINTER(C.c_int)),
+                            ci.ctypes.data_as(C.POINTER(C.c_int)), _p(va), K, _p(Qf), _p(el),
+                            _p(pl), float(setup["dt"]), int(setup["m"]), nd, _p(gamma), ns,
+                            _p(gsr), _p(gsi), _p(w1), _p(w2), float(setup["s"]), _p(ex),
+                            _p(mass), int(steps), _p(u), _p(gu))
+    if bad:
+        raise ZeroDivisionError("singular shifted tridiagonal system in the C oracle")
+    return u, gu, used
