@@ -1,0 +1,125 @@
+"""The MEX gateways (gpumatfun_amd/mex/gm_mex.cpp) driven through the fake-MEX harness.
+
+CPU tests: argument checking with the reference's message identifiers
+(lanczos_create.cpp:40-733, get_U.cpp:12-26).  GPU tests: the step sequence of
+nicksmethodgpu.m:8-21 through the gateways, against the engine and the oracle.
+"""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel
+
+sys.path.insert(0, os.path.join(ROOT, "tests", "fake_mex"))
+import harness  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def mex():
+    harness.build()
+    return harness.Mex()
+
+
+def create_args(inp):
+    """The 26 arguments of lanczosEngine.m:9, valvec = -1:1."""
+    return [inp.U, inp.A.tocsc(), inp.Q, inp.lam, float(inp.m), inp.gamma, inp.gamma_s_real,
+            inp.gamma_s_imag, inp.exp_lambda, inp.phi_lambda, inp.dt, inp.ode, inp.aode,
+            inp.bode, inp.code, inp.vrest, inp.vamp, inp.vth, inp.vpeak, inp.c1ion, inp.c2ion,
+            inp.mass, inp.wts1, inp.wts2, inp.shifts, np.array([-1.0, 0.0, 1.0])]
+
+
+def small_inputs():
+    from cases import dirichlet_case
+    return dirichlet_case(side=12, m=10, n=8)
+
+
+@pytest.mark.parametrize("idx,bad,ident", [
+    (24, "cut", "MATLAB:lanczos_create:bad_shifts"),
+    (22, "cut", "MATLAB:lanczos_create:bad_wts1"),
+    (1, "dense", "MATLAB:lanczos_create:inputNotSparse"),
+    (4, "vector", "MATLAB:lanczos_create:bad_N"),
+    (11, "cut", "MATLAB:lanczos_create:bad_ode"),
+    (21, "cut", "MATLAB:lanczos_create:bad_mass"),
+    (6, "transpose", "MATLAB:lanczos_create:bad_gamma_s_real"),
+    (3, "extra", "MATLAB:lanczos_create:bad_lambda"),
+    (0, "complex", "MATLAB:lanczos_create:inputNotReal"),
+])
+def test_create_argument_errors(mex, idx, bad, ident):
+    args = create_args(small_inputs())
+    a = args[idx]
+    if bad == "cut":
+        args[idx] = np.asarray(a)[:-2] if np.ndim(a) == 1 else np.asarray(a)[:-1]
+    elif bad == "dense":
+        args[idx] = a.toarray()
+    elif bad == "vector":
+        args[idx] = np.array([10.0, 11.0])
+    elif bad == "transpose":
+        args[idx] = np.asarray(a).T
+    elif bad == "extra":
+        args[idx] = np.array([1.0])
+    elif bad == "complex":
+        args[idx] = np.asarray(a) + 0j
+    with pytest.raises(harness.MexError) as e:
+        mex("lanczos_create", *args)
+    assert e.value.ident == ident
+
+
+def test_create_arity(mex):
+    args = create_args(small_inputs())
+    with pytest.raises(harness.MexError) as e:
+        mex("lanczos_create", *args[:25])
+    assert e.value.ident == "MATLAB:lanczos_create:invalidNumInputs"
+    with pytest.raises(harness.MexError) as e:
+        mex("lanczos_create", *args, nlhs=0)
+    assert e.value.ident == "MATLAB:lanczos_create:maxlhs"
+
+
+def test_handle_gateways_check_their_argument(mex):
+    with pytest.raises(harness.MexError) as e:
+        mex("get_U_mex", np.array([1.0]))
+    assert e.value.ident == "MATLAB:get_U:inputNotHandle"
+    with pytest.raises(harness.MexError) as e:
+        mex("build_subspace_mex")
+    assert e.value.ident == "MATLAB:build_subspace:invalidNumInputs"
+    with pytest.raises(harness.MexError) as e:   # unknown handle -> library status
+        mex("build_subspace_mex", np.uint64(12345))
+    assert e.value.ident == "gpumatfun:GM_EBADHANDLE"
+
+
+@pytest.mark.gpu
+def test_mex_step_sequence_matches_engine_and_oracle(mex):
+    """nicksmethodgpu.m:8-21 through the gateways (reference semantics: clamp on)."""
+    import oracle
+    from cases import dirichlet_case
+    from gpumatfun_amd.engine import LanczosEngine
+    inp = dirichlet_case(side=32, m=24, n=16)
+    h = mex("lanczos_create", *create_args(inp))
+    eng = LanczosEngine.from_inputs(inp, clamp=True)
+    ref = oracle.GpuSemantics(inp.U, inp.A, inp.Q, inp.m, inp.gamma, inp.gamma_s_real,
+                              inp.gamma_s_imag, inp.exp_lambda, inp.phi_lambda, inp.dt, inp.mass,
+                              inp.wts1, inp.wts2, inp.shifts, clamp=True)
+    for _ in range(2):
+        mex("build_subspace_mex", h)
+        mex("lin_sys_solves_mex", h)
+        mex("apply_V_mex", h)
+        X = mex("get_xsol_mex", h)
+        mex("update_U_mex", h)
+        eng.step(1)
+        ref.build_subspace()
+        ref.lin_sys_solves()
+        ref.apply_V()
+        assert rel(X, ref.get_xsol()) < 1e-10
+        ref.update_U()
+    U = mex("get_U_mex", h)
+    assert np.array_equal(U, eng.get_U())
+    assert rel(U, ref.get_U()) < 1e-10
+    ab = mex("get_aabb_mex", h)
+    assert ab.shape == (inp.m + 1, 4)
+    mex("lanczos_delete", h)
+    with pytest.raises(harness.MexError) as e:
+        mex("get_U_mex", h)
+    assert e.value.ident == "gpumatfun:GM_EBADHANDLE"
+    eng.delete()
