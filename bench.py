@@ -1,20 +1,24 @@
 """Benchmark of the f(A)b engine on one MI355X.
 
-Default workload (``--config fem1m``): the plain method at full scale, FP64 --
-P1 FEM stiffness K and lumped mass M on a seeded Delaunay triangulation of
-the unit disc with 1,000,000 nodes (uniform random interior points plus
-4096 equispaced boundary points), A = M^{-1/2} K M^{-1/2} in CSR (symrcm
-order), f1 = exp(-tau kappa A^{alpha/2}), f2 = phi(...), alpha = 1.7,
-tau = 1e-3, kappa = 1, degree-8 Jacobi polynomial preconditioner, Lanczos
-m = 80, 64 contour shifts, 20 deflated eigenvectors.  One step is one time
-step of the exponential integrator = 2 f(A)b solves (exp and phi columns);
-inputs are resident in HBM when the timed region starts.
+Workloads (``--config``):
 
-``--config dirichlet128``: 2D 5-point Dirichlet Laplacian 128 x 128 scaled by
-1/h^2, b ~ N(0,1) seed 0, alpha = 1.5, m = 50, 32 shifts, no preconditioner,
-no deflation.
+* ``fem1m`` (default) -- the plain method at full scale, FP64: P1 FEM
+  stiffness K and lumped mass M on a seeded Delaunay triangulation of the unit
+  disc with 1,000,000 nodes (uniform random interior points plus 4096
+  equispaced boundary points), A = M^{-1/2} K M^{-1/2} in CSR (symrcm order),
+  f1 = exp(-tau kappa A^{alpha/2}), f2 = phi(...), alpha = 1.7, tau = 1e-3,
+  kappa = 1, degree-8 Jacobi polynomial preconditioner, Lanczos m = 80, 64
+  contour shifts, 20 deflated eigenvectors.
+* ``dirichlet128`` -- 2D 5-point Dirichlet Laplacian 128 x 128 scaled by 1/h^2,
+  b ~ N(0,1) seed 0, alpha = 1.5, m = 50, 32 shifts, no preconditioner, no deflation.
+* ``batch3d`` -- batched form: 3D 7-point Dirichlet Laplacian 128^3 (scaled by
+  1/h^2), 256 problems with u_p ~ N(0,1) seeds 0..255 (512 columns), 64
+  shifts, m = 50, alpha = 1.5, tau = 1e-3, no preconditioner; the plain path
+  on the same matrix is timed beside it.
 
-Prints one JSON line.
+One step is one time step of the exponential integrator = 2 f(A)b solves per
+problem; inputs are resident in HBM when the timed region starts.  Prints one
+JSON line.
 """
 
 import argparse
@@ -62,8 +66,9 @@ def build_fem1m(args):
                         Q=Q[:, :Kdefl], lam=lam[:Kdefl], mass=mass)
     cfg = {"workload": "fem_disc_1m_plain", "n": int(A.shape[0]), "nnz": int(A.nnz),
            "m": 80, "poly_degree": 8, "shifts": 64, "deflation": Kdefl, "alpha": 1.7,
-           "tau": 1e-3, "kappa": 1.0, "columns_per_step": 2, "mesh_seed": 0}
-    return inp, cfg, "nonuniform", time.time() - t0
+           "tau": 1e-3, "kappa": 1.0, "columns_per_step": 2, "mesh_seed": 0,
+           "lambda_max": lN, "l1": float(lam[Kdefl])}
+    return inp, cfg, time.time() - t0
 
 
 def build_dirichlet128(args):
@@ -77,34 +82,118 @@ def build_dirichlet128(args):
     cfg = {"workload": "dirichlet2d_128_plain", "n": int(A.shape[0]), "nnz": int(A.nnz),
            "m": 50, "poly_degree": 0, "shifts": 32, "deflation": 0, "alpha": 1.5,
            "tau": 1e-3, "kappa": 1.0, "columns_per_step": 2, "b_seed": 0}
-    return inp, cfg, "fisher", time.time() - t0
+    return inp, cfg, time.time() - t0
 
 
-def oracle_one_step(inp, threads):
-    """One time step of the oracle's C restatement (oracle/c/fab_oracle.c) on `threads`
-    host threads: the step loop of problem_startup_CPU_nonuniform.m (mass = 1 for the
-    Dirichlet workload, which is then problem_startup_CPU.m's loop)."""
-    from oracle import cfab
-    S = {"A": inp.A, "Q": inp.Q if inp.Q.shape[1] else None,
+def build_batch3d(args):
+    from gpumatfun_amd import problems, quadrature as q
+    t0 = time.time()
+    side = args.side3d
+    A = problems.laplacian_3d_dirichlet(side)
+    h = 1.0 / (side + 1)
+    l1d = 4.0 * np.sin(np.arange(1, side + 1) * np.pi * h / 2) ** 2 / h ** 2
+    lmin, lmax = 3 * l1d.min(), 3 * l1d.max()
+    N = A.shape[0]
+    P = args.problems
+    U = np.empty((N, 2 * P))
+    for p in range(P):
+        u = np.random.default_rng(p).standard_normal(N)
+        U[:, 2 * p] = u
+        U[:, 2 * p + 1] = u * (1 - u)
+    inp = q.make_inputs(A, U[:, 0], U[:, 1], m=50, n=64, eta=1e-3, alpha=1.5, dt=1e-3,
+                        l1=float(lmin), lN=float(lmax), degree=0)
+    cfg = {"workload": f"batch_laplace3d_{side}_{P}rhs", "n": int(N), "nnz": int(A.nnz),
+           "problems": P, "m": 50, "poly_degree": 0, "shifts": 64, "deflation": 0,
+           "alpha": 1.5, "tau": 1e-3, "kappa": 1.0, "columns_per_step": 2 * P,
+           "rhs_seeds": f"0..{P - 1}"}
+    return inp, U, cfg, time.time() - t0
+
+
+def oracle_setup_dict(inp, A=None, perm=None):
+    S = {"A": inp.A if A is None else A, "Q": inp.Q if inp.Q.shape[1] else None,
          "exp_lambda": inp.exp_lambda, "phi_lambda": inp.phi_lambda, "dt": inp.dt,
          "n": inp.extra["n"], "m": inp.m, "s": inp.extra["s"], "gamma": inp.gamma,
          "gamma_s_real": inp.gamma_s_real, "gamma_s_imag": inp.gamma_s_imag,
          "etaxi": inp.extra["etaxi"], "wts1_cpu": inp.extra["wts1_cpu"],
          "wts2_cpu": inp.extra["wts2_cpu"], "mass": inp.mass}
+    if perm is not None:
+        S["Q"] = None if S["Q"] is None else inp.Q[perm]
+        S["mass"] = inp.mass[perm]
+    return S
+
+
+def oracle_one_step(inp, u0, g0, threads, perm_seed=None):
+    """One time step of the oracle's C restatement (oracle/c/fab_oracle.c) on `threads`
+    host threads: the step loop of problem_startup_CPU_nonuniform.m (mass = 1 is
+    problem_startup_CPU.m's loop).  With perm_seed, the same problem with rows and
+    columns permuted (only the floating-point summation order changes)."""
+    import scipy.sparse as sp
+    from oracle import cfab
     cfab.load()
+    if perm_seed is None:
+        S = oracle_setup_dict(inp)
+        t0 = time.perf_counter()
+        u, gu, used = cfab.cpu_steps_c(S, u0, g0, 1, threads=threads)
+        return u, gu, time.perf_counter() - t0, used
+    N = inp.A.shape[0]
+    p = np.random.default_rng(perm_seed).permutation(N)
+    ip = np.argsort(p)
+    S = oracle_setup_dict(inp, A=sp.csr_matrix(inp.A[p][:, p]), perm=p)
     t0 = time.perf_counter()
-    u, gu, used = cfab.cpu_steps_c(S, inp.U[:, 0], inp.U[:, 1], 1, threads=threads)
-    dt = time.perf_counter() - t0
-    return u, gu, dt, used
+    u, gu, used = cfab.cpu_steps_c(S, u0[p], g0[p], 1, threads=threads)
+    return u[ip], gu[ip], time.perf_counter() - t0, used
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def cpu_leg(inp, u0, g0, U1, result):
+    """Parity of the GPU's first step against the oracle, the oracle's own rounding
+    floor (a row-permuted run), and the timed CPU baseline."""
+    threads = min(16, os.cpu_count() or 1)
+    log(f"CPU oracle (C restatement, {threads} threads): one step ...")
+    u, gu, cpu_s, used = oracle_one_step(inp, u0, g0, threads)
+    e_u, e_g = rel(U1[:, 0], u), rel(U1[:, 1], inp.dt * gu)
+    up, gp, _, _ = oracle_one_step(inp, u0, g0, threads, perm_seed=1)
+    floor = max(rel(up, u), rel(gp, gu))
+    result["rel_err_vs_oracle"] = {"u": e_u, "dt_g": e_g, "steps": 1,
+                                   "oracle_floor": floor, "tolerance": 1e-10}
+    reps = 1
+    if cpu_s < 5.0:   # small workloads: repeat the step to a ~10 s sample
+        reps = int(min(1000, max(2, np.ceil(10.0 / max(cpu_s, 1e-3)))))
+        cpu_s = sum(oracle_one_step(inp, u0, g0, threads)[2] for _ in range(reps)) / reps
+    result["cpu_baseline"] = {"value": round(2.0 / cpu_s, 6), "unit": "solves/s",
+                              "cores": used, "kind": "port",
+                              "sample": f"{reps} time step(s) (2 solves each) of one problem of "
+                                        "the same workload by the oracle's C/OpenMP restatement "
+                                        "(oracle/c/fab_oracle.c), mean per step",
+                              "seconds_per_step": round(cpu_s, 4), "host_cpus": os.cpu_count()}
+    log(f"oracle step {cpu_s:.2f}s, rel err u={e_u:.3e} dt*g={e_g:.3e}, floor {floor:.3e}")
+
+
+def timed(step_fn, sync_fn, steps, warm):
+    import torch
+    step_fn(warm)
+    sync_fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step_fn(steps)
+    sync_fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="fem1m", choices=["fem1m", "dirichlet128"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--config", default="fem1m", choices=["fem1m", "dirichlet128", "batch3d"])
     ap.add_argument("--nodes", type=int, default=1_000_000)
+    ap.add_argument("--side3d", type=int, default=128)
+    ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
@@ -124,30 +213,33 @@ def main():
         b.build()
     from gpumatfun_amd.engine import LanczosEngine
 
-    inp, cfg, kind, setup_s = (build_fem1m if args.config == "fem1m" else build_dirichlet128)(args)
+    use_graph = not args.no_graph
+    if args.config == "batch3d":
+        steps = args.steps or 3
+        warm = 1 if args.warmup is None else args.warmup
+    else:
+        steps = args.steps or 20
+        warm = 3 if args.warmup is None else args.warmup
+    if args.profile_only:
+        steps, warm = 2, 1
+
+    if args.config == "batch3d":
+        run_batch(args, steps, warm, use_graph)
+        return
+
+    inp, cfg, setup_s = (build_fem1m if args.config == "fem1m" else build_dirichlet128)(args)
     log(f"setup {setup_s:.1f}s")
     t0 = time.time()
     eng = LanczosEngine.from_inputs(inp, clamp=False, autotune=True)
     info = eng.info()
     log(f"create {time.time() - t0:.2f}s, lanes={info['spmm_lanes']}")
 
-    # parity leg: one step from the initial state against the oracle
+    # parity leg: one step from the initial state
     eng.step(1, use_graph=False)
     U1 = eng.get_U()
     eng.synchronize()
 
-    # timed region
-    use_graph = not args.no_graph
-    steps = 2 if args.profile_only else args.steps
-    warm = 1 if args.profile_only else args.warmup
-    eng.step(warm, use_graph=use_graph)
-    eng.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.step(steps, use_graph=use_graph)
-    eng.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed = timed(lambda k: eng.step(k, use_graph=use_graph), eng.synchronize, steps, warm)
     ms_per_step = 1000.0 * elapsed / steps
     value = 2.0 * steps / elapsed
     log(f"{steps} steps in {elapsed:.4f}s: {ms_per_step:.3f} ms/step, {value:.2f} solves/s")
@@ -159,9 +251,8 @@ def main():
     bytes_per = 12 * nnz + 4 * (n + 1) + (48 if mode == 0 else 64) * n
     achieved = bytes_per / (spmm_ms * 1e-3) / 1e9
     log(f"spmm mode {mode}: {spmm_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
-    copy_bytes = 2 * 16 * n * 8   # read + write of an 8-column-pair vector block
+    copy_bytes = 2 * 16 * n * 8   # read + write of 8 column pairs
     copy_ms = eng.profile_copy(copy_bytes // 2, reps=5 if args.profile_only else 50)
-    copy_gbs = copy_bytes / (copy_ms * 1e-3) / 1e9
 
     result = {
         "metric": "f(A)b solves/sec (FP64, plain path)",
@@ -182,35 +273,81 @@ def main():
         "setup_s": round(setup_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_spmm<%d,%s>" % (info["spmm_lanes"],
-                                                                   "kMid" if mode == 0 else "kLast"),
+                     "traffic": None,
+                     "kernel": "k_spmm<%d,%s>" % (info["spmm_lanes"], "kMid" if mode == 0 else "kLast"),
                      "avg_launch_us": round(spmm_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(bytes_per)},
-        "copy_kernel_gbs": round(copy_gbs, 1),
+        "copy_kernel_gbs": round(copy_bytes / (copy_ms * 1e-3) / 1e9, 1),
+        "cpu_baseline": None,
     }
-
     if not args.no_cpu and not args.profile_only:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"CPU oracle (C restatement, {threads} threads): one step ...")
-        u, gu, cpu_s, used = oracle_one_step(inp, threads)
-        e_u = float(np.linalg.norm(U1[:, 0] - u) / np.linalg.norm(u))
-        e_g = float(np.linalg.norm(U1[:, 1] - inp.dt * gu) / np.linalg.norm(inp.dt * gu))
-        result["rel_err_vs_oracle"] = {"u": e_u, "dt_g": e_g, "steps": 1}
-        reps = 1
-        if cpu_s < 5.0:   # small workloads: repeat the step to a ~10 s sample
-            reps = int(min(1000, max(2, np.ceil(10.0 / max(cpu_s, 1e-3)))))
-            cpu_s = sum(oracle_one_step(inp, threads)[2] for _ in range(reps)) / reps
-        result["cpu_baseline"] = {"value": round(2.0 / cpu_s, 6), "unit": "solves/s",
-                                  "cores": used, "kind": "port",
-                                  "sample": f"{reps} time step(s) (2 solves each) of the same "
-                                            "workload by the oracle's C/OpenMP restatement "
-                                            "(oracle/c/fab_oracle.c), mean per step",
-                                  "seconds_per_step": round(cpu_s, 4),
-                                  "host_cpus": os.cpu_count()}
-        log(f"oracle step {cpu_s:.1f}s, rel err u={e_u:.3e} dt*g={e_g:.3e}")
-    else:
-        result["cpu_baseline"] = None
+        cpu_leg(inp, inp.U[:, 0], inp.U[:, 1], U1, result)
     eng.delete()
+    print(json.dumps(result), flush=True)
+
+
+def run_batch(args, steps, warm, use_graph):
+    from gpumatfun_amd.batch import BatchEngine
+    from gpumatfun_amd.engine import LanczosEngine
+    inp, U, cfg, setup_s = build_batch3d(args)
+    log(f"setup {setup_s:.1f}s, U {U.shape}")
+    P = U.shape[1] // 2
+    t0 = time.time()
+    b = BatchEngine.from_inputs(inp, U, clamp=False, autotune=True)
+    info = b.info()
+    log(f"batch create {time.time() - t0:.1f}s, chunk columns {info['chunk_columns']}")
+    # parity leg: problem 0 after one step
+    b.step(1, use_graph=False)
+    b.synchronize()
+    U1 = b.get_U()[:, 0:2]
+    elapsed = timed(lambda k: b.step(k, use_graph=use_graph), b.synchronize, steps, warm)
+    value = 2.0 * P * steps / elapsed
+    log(f"batched: {steps} steps in {elapsed:.3f}s -> {value:.1f} solves/s")
+    b.delete()
+    del U
+
+    # the plain path on the same matrix (one problem, 2 columns per step)
+    e = LanczosEngine.from_inputs(inp, clamp=False, autotune=True)
+    einfo = e.info()
+    psteps = max(5, steps * 4)
+    pel = timed(lambda k: e.step(k, use_graph=use_graph), e.synchronize, psteps, 2)
+    plain = 2.0 * psteps / pel
+    mode = 1
+    spmm_ms = e.profile_spmm(mode=mode, reps=10 if args.profile_only else 100)
+    n, nnz = einfo["n"], einfo["nnz"]
+    bytes_per = 12 * nnz + 4 * (n + 1) + 64 * n
+    e.delete()
+    log(f"plain same matrix: {plain:.1f} solves/s; batched/plain = {value / plain:.2f}")
+    result = {
+        "metric": "f(A)b solves/sec (FP64, batched form)",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": round(1000.0 * elapsed / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded right-hand sides)",
+        "config": cfg,
+        "graph": use_graph,
+        "chunk_columns": info["chunk_columns"],
+        "setup_s": round(setup_s, 1),
+        "plain_same_matrix": {"value": round(plain, 3), "unit": "solves/s",
+                              "spmm_lanes": einfo["spmm_lanes"]},
+        "batched_vs_plain": round(value / plain, 3),
+        "roofline": {"bound": "hbm", "kernel": "plain k_spmm kLast on the same matrix",
+                     "achieved": round(bytes_per / (spmm_ms * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bytes_per / (spmm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_us": round(spmm_ms * 1e3, 3)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu and not args.profile_only:
+        r0 = np.random.default_rng(0).standard_normal(inp.A.shape[0])
+        cpu_leg(inp, r0, r0 * (1 - r0), U1, result)
     print(json.dumps(result), flush=True)
 
 

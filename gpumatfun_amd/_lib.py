@@ -26,6 +26,8 @@ EXPORTS = (
     "gm_get_U", "gm_get_xsol", "gm_get_aabb", "gm_set_y", "gm_get_y", "gm_get_state",
     "gm_set_state", "gm_step", "gm_synchronize", "gm_get_info", "gm_get_stream",
     "gm_profile_spmm", "gm_profile_copy",
+    "gm_batch_last_error", "gm_batch_create", "gm_batch_delete", "gm_batch_step",
+    "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
 )
 
 _dp = C.POINTER(C.c_double)
@@ -63,6 +65,34 @@ class GmCreateDesc(C.Structure):
         ("wts2", _dp),
         ("shifts", _dp),
         ("valvec", _dp),
+        ("flags", C.c_uint32),
+        ("device", C.c_int32),
+    ]
+
+
+class GmBatchDesc(C.Structure):
+    """Mirror of ``gm_batch_desc``."""
+    _fields_ = [
+        ("n", C.c_int32),
+        ("sparse_format", C.c_int32),
+        ("nnz", C.c_int64),
+        ("A_values", _dp),
+        ("A_ptr", _ip),
+        ("A_ind", _ip),
+        ("num_problems", C.c_int32),
+        ("U", _dp),
+        ("subspace_size", C.c_int32),
+        ("gamma_size", C.c_int32),
+        ("gamma", _dp),
+        ("contour_size", C.c_int32),
+        ("gamma_s_real", _dp),
+        ("gamma_s_imag", _dp),
+        ("wts1", _dp),
+        ("wts2", _dp),
+        ("etaxi", _dp),
+        ("dt", C.c_double),
+        ("mass", _dp),
+        ("chunk_columns", C.c_int32),
         ("flags", C.c_uint32),
         ("device", C.c_int32),
     ]
@@ -114,6 +144,14 @@ def load():
         "gm_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
         "gm_profile_spmm": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "gm_profile_copy": (C.c_int, [vp, C.c_int64, C.c_int32, _dp]),
+        "gm_batch_last_error": (C.c_char_p, []),
+        "gm_batch_create": (C.c_int, [C.POINTER(GmBatchDesc), C.POINTER(vp)]),
+        "gm_batch_delete": (C.c_int, [vp]),
+        "gm_batch_step": (C.c_int, [vp, C.c_int32, C.c_int32]),
+        "gm_batch_get_U": (C.c_int, [vp, _dp]),
+        "gm_batch_synchronize": (C.c_int, [vp]),
+        "gm_batch_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "gm_batch_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -123,7 +161,8 @@ def load():
     return lib
 
 
-def check(status):
+def check(status, batch=False):
     if status != GM_OK:
-        msg = load().gm_last_error().decode("utf-8", "replace")
+        lib = load()
+        msg = (lib.gm_batch_last_error() if batch else lib.gm_last_error()).decode("utf-8", "replace")
         raise GpuMatfunError(status, msg)

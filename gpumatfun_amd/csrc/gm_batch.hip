@@ -1,0 +1,1052 @@
+// gm_batch.hip -- batched form: many right-hand sides sharing one matrix.
+//
+// P independent problems share A (and the contour nodes, weights and
+// polynomial).  Problem p owns the column pair [u_p, dt g_p] -- the same two
+// columns as the plain path (exp and phi) -- so a batch of P problems is R = 2P
+// columns, each with its own Lanczos process (its own alpha, beta).
+//
+// Layout: every N x R quantity is row-major, columns contiguous
+// (X[i*ld + c]); a CSR gather of x[col] is then one coalesced R*8-byte load
+// across the lanes of a wavefront, and each matrix entry is read once for all
+// R columns.  Lanes map to columns: with R >= 64 a wavefront works on one row
+// at a time (64 columns per pass), with R < 64 it works on 64/R rows.
+// Columns are processed in chunks of Rc (a power of two); the Lanczos basis of
+// one chunk, m x N x Rc doubles, is what bounds Rc.
+//
+// Per step and chunk: build (m iterations of SpMM kernels + finish), shifted
+// solves (Rc x n_s systems, one lane each), collapse, V*C, Horner SpMMs, and
+// the Fisher-KPP update (update_U.cu:8-16, optional clamp) with the column
+// norms for the next step.  No deflation in the batched form.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/gpumatfun.h"
+#include "gm_device.h"
+
+namespace gmb {
+using namespace gm;
+
+constexpr int kMaxND = 17;
+
+enum Mode : int { kFirst = 0, kMid = 1, kLast = 2, kOnly = 3, kHorner = 4 };
+
+struct BSpmm {
+  const int* __restrict__ rowptr;
+  const int* __restrict__ col;
+  const double* __restrict__ val;
+  int nrows, chunk;
+  const double* x;  size_t ldx;     // gather source (+ column offset applied by caller)
+  double* y;        size_t ldy;     // output
+  double* vk;                       // V[k] (ld = R)
+  const double* vkm1;               // V[k-1] or null
+  const double* z;  size_t ldz;     // kHorner addend
+  const double* al;                 // alpha [k][R]
+  double* be;                       // beta  [k][R]
+  int k;
+  double g0, gi;
+  int accumulate;
+  double* alpha_out;                // kLast/kOnly: alpha[k][R]
+  double* partials;                 // [G][R]
+  unsigned* counter;
+};
+
+// Per-column block sums (fixed order) written to partials[block][R]; the grid
+// total is formed by k_reduce after the kernel.
+template <int R>
+__device__ void block_partial_cols(const double (&v)[(R < 64 ? 1 : R / 64)], double* partials) {
+  constexpr int CPW = R < 64 ? R : 64;
+  constexpr int RPW = 64 / CPW;
+  constexpr int CL = R / CPW;
+  __shared__ double sh[kBlock * CL];
+  for (int q = 0; q < CL; ++q) sh[threadIdx.x * CL + q] = v[q];
+  __syncthreads();
+  for (int c = threadIdx.x; c < R; c += kBlock) {
+    const int lanec = c % CPW, q = c / CPW;
+    double s = 0.0;
+    for (int w = 0; w < kWaves; ++w)
+      for (int sl = 0; sl < RPW; ++sl) s += sh[(w * 64 + sl * CPW + lanec) * CL + q];
+    partials[(size_t)blockIdx.x * R + c] = s;
+  }
+}
+
+template <int R, int MODE>
+__global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
+  constexpr int CPW = R < 64 ? R : 64;
+  constexpr int RPW = 64 / CPW;
+  constexpr int CL = R / CPW;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int row0 = b * p.chunk;
+  const int row1 = min(row0 + p.chunk, p.nrows);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int slot = lane / CPW, c0 = lane % CPW;
+  double bet[CL], inv[CL], bprev[CL], dot[CL];
+#pragma unroll
+  for (int q = 0; q < CL; ++q) {
+    const int c = c0 + q * CPW;
+    bet[q] = (MODE == kFirst || MODE == kOnly) ? p.be[(size_t)p.k * R + c] : 0.0;
+    inv[q] = safe_inv(bet[q]) * p.g0;
+    bprev[q] = ((MODE == kLast || MODE == kOnly) && p.k > 0) ? p.be[(size_t)p.k * R + c] : 0.0;
+    dot[q] = 0.0;
+  }
+  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
+    const int j0 = p.rowptr[r], j1 = p.rowptr[r + 1];
+    double acc[CL];
+#pragma unroll
+    for (int q = 0; q < CL; ++q) acc[q] = 0.0;
+    for (int j = j0; j < j1; ++j) {
+      const double a = p.val[j];
+      const double* xr = p.x + (size_t)p.col[j] * p.ldx + c0;
+#pragma unroll
+      for (int q = 0; q < CL; ++q) acc[q] = fma(a, xr[q * CPW], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const int c = c0 + q * CPW;
+      const size_t iy = (size_t)r * p.ldy + c;
+      const size_t iv = (size_t)r * R + c;
+      if (MODE == kFirst) {
+        const double v = safe_div(p.x[(size_t)r * p.ldx + c0 + q * CPW], bet[q]);
+        p.vk[iv] = v;
+        p.y[iy] = fma(p.gi, v, acc[q] * inv[q]);
+      } else if (MODE == kMid) {
+        p.y[iy] = fma(p.gi, p.vk[iv], acc[q]);
+      } else if (MODE == kLast || MODE == kOnly) {
+        double v, yv;
+        if (MODE == kOnly) {
+          v = safe_div(p.x[(size_t)r * p.ldx + c0 + q * CPW], bet[q]);
+          p.vk[iv] = v;
+          yv = acc[q] * inv[q];
+        } else {
+          v = p.vk[iv];
+          yv = acc[q];
+        }
+        if (p.k > 0) yv = fma(-bprev[q], p.vkm1[iv], yv);
+        p.y[iy] = yv;
+        dot[q] = fma(yv, v, dot[q]);
+      } else {
+        double yv = acc[q] + p.z[(size_t)r * p.ldz + c];
+        if (p.accumulate) yv += p.y[iy];
+        p.y[iy] = yv;
+      }
+    }
+  }
+  if (MODE == kLast || MODE == kOnly)
+    block_partial_cols<R>(dot, p.partials);
+}
+
+struct BFinish {
+  double* w;          // ld R
+  const double* vk;
+  const double* al;
+  double* be;
+  int k, nrows, chunk;
+  double* partials;
+  unsigned* counter;
+  unsigned* status;
+};
+
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
+  constexpr int CPW = R < 64 ? R : 64;
+  constexpr int RPW = 64 / CPW;
+  constexpr int CL = R / CPW;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int row0 = b * p.chunk;
+  const int row1 = min(row0 + p.chunk, p.nrows);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int slot = lane / CPW, c0 = lane % CPW;
+  double al[CL], nrm[CL];
+#pragma unroll
+  for (int q = 0; q < CL; ++q) {
+    al[q] = p.al[(size_t)p.k * R + c0 + q * CPW];
+    nrm[q] = 0.0;
+  }
+  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const size_t i = (size_t)r * R + c0 + q * CPW;
+      const double w = fma(-al[q], p.vk[i], p.w[i]);
+      p.w[i] = w;
+      nrm[q] = fma(w, w, nrm[q]);
+    }
+  }
+  block_partial_cols<R>(nrm, p.partials);
+}
+
+// Column norms of U (ld ldu, column offset applied) -> be[0][c].
+struct BNorm {
+  const double* u;
+  size_t ldu;
+  double* be;
+  int nrows, chunk;
+  double* partials;
+  unsigned* counter;
+  unsigned* status;
+};
+
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_bnorm(BNorm p) {
+  constexpr int CPW = R < 64 ? R : 64;
+  constexpr int RPW = 64 / CPW;
+  constexpr int CL = R / CPW;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int row0 = b * p.chunk;
+  const int row1 = min(row0 + p.chunk, p.nrows);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int slot = lane / CPW, c0 = lane % CPW;
+  double nrm[CL];
+#pragma unroll
+  for (int q = 0; q < CL; ++q) nrm[q] = 0.0;
+  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const double u = p.u[(size_t)r * p.ldu + c0 + q * CPW];
+      nrm[q] = fma(u, u, nrm[q]);
+    }
+  }
+  block_partial_cols<R>(nrm, p.partials);
+}
+
+// (T_c - etaxi_s I) y = -beta0_c w_{f(c),s} e1, one lane per (column, node);
+// column c uses weights wts1 (even c, exp) or wts2 (odd c, phi).  Pivoted like
+// k_shifted_solves; Y is [c][s][k].
+struct BSolve {
+  const double* al;   // [k][R]
+  const double* be;   // [k][R]
+  int R, m, ns;
+  const double* wts1, *wts2;   // ns x 2
+  const double* etaxi;         // 2 ns (re, im)
+  double2 *wD, *wDU, *wDU2, *wX;   // [k][R*ns]
+  double* yr;
+  double* yi;
+  unsigned* status;
+};
+
+__global__ __launch_bounds__(64) void k_bsolves(BSolve p) {
+  const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+  const int S = p.R * p.ns;
+  if (sys >= S) return;
+  const int c = sys / p.ns, s = sys - c * p.ns;
+  const double* w = (c & 1) ? p.wts2 : p.wts1;
+  const double2 wt = d2(w[s], w[s + p.ns]);
+  const double2 ex = d2(p.etaxi[2 * s], p.etaxi[2 * s + 1]);
+  const double b0 = p.be[c];
+  const int m = p.m;
+  auto diag = [&](int k) { return d2(p.al[(size_t)k * p.R + c] - ex.x, -ex.y); };
+  auto off = [&](int k) { return d2(p.be[(size_t)(k + 1) * p.R + c], 0.0); };
+  double2 cur_d = diag(0);
+  double2 cur_x = d2(-(b0 * wt.x), -(b0 * wt.y));
+  double2 cur_du = m > 1 ? off(0) : d2(0.0, 0.0);
+  bool singular = false;
+  for (int k = 0; k < m - 1; ++k) {
+    const double2 dl = off(k);
+    double2 next_d = diag(k + 1), next_x = d2(0.0, 0.0);
+    double2 next_du = (k + 1 < m - 1) ? off(k + 1) : d2(0.0, 0.0);
+    double2 sD, sDU, sDU2 = d2(0.0, 0.0), sX;
+    if (dl.x == 0.0 && dl.y == 0.0) {
+      if (cur_d.x == 0.0 && cur_d.y == 0.0) singular = true;
+      sD = cur_d; sDU = cur_du; sX = cur_x;
+    } else if (cabs1(cur_d) >= cabs1(dl)) {
+      const double2 mult = cdiv(dl, cur_d);
+      next_d = csub(next_d, cmul(mult, cur_du));
+      next_x = csub(next_x, cmul(mult, cur_x));
+      sD = cur_d; sDU = cur_du; sX = cur_x;
+    } else {
+      const double2 mult = cdiv(cur_d, dl);
+      sD = dl;
+      const double2 temp = next_d;
+      next_d = csub(cur_du, cmul(mult, temp));
+      if (k < m - 2) {
+        sDU2 = next_du;
+        next_du = d2(-(mult.x * sDU2.x - mult.y * sDU2.y), -(mult.x * sDU2.y + mult.y * sDU2.x));
+      }
+      sDU = temp;
+      sX = next_x;
+      next_x = csub(cur_x, cmul(mult, next_x));
+    }
+    const size_t o = (size_t)k * S + sys;
+    p.wD[o] = sD; p.wDU[o] = sDU; p.wDU2[o] = sDU2; p.wX[o] = sX;
+    cur_d = next_d; cur_x = next_x; cur_du = next_du;
+  }
+  if (cur_d.x == 0.0 && cur_d.y == 0.0) singular = true;
+  double* yr = p.yr + (size_t)sys * m;
+  double* yi = p.yi + (size_t)sys * m;
+  if (singular) {
+    atomicOr(p.status, 2u);
+    for (int k = 0; k < m; ++k) { yr[k] = 0.0; yi[k] = 0.0; }
+    return;
+  }
+  double2 x2 = cdiv(cur_x, cur_d);
+  yr[m - 1] = x2.x;
+  yi[m - 1] = x2.y;
+  if (m > 1) {
+    const size_t o = (size_t)(m - 2) * S + sys;
+    double2 x1 = cdiv(csub(p.wX[o], cmul(p.wDU[o], x2)), p.wD[o]);
+    yr[m - 2] = x1.x;
+    yi[m - 2] = x1.y;
+    for (int k = m - 3; k >= 0; --k) {
+      const size_t q = (size_t)k * S + sys;
+      double2 t = csub(p.wX[q], cmul(p.wDU[q], x1));
+      t = csub(t, cmul(p.wDU2[q], x2));
+      const double2 xk = cdiv(t, p.wD[q]);
+      yr[k] = xk.x;
+      yi[k] = xk.y;
+      x2 = x1;
+      x1 = xk;
+    }
+  }
+}
+
+// C[j][k][c] = sum_s (gsr_{j,s} Im y_{c,s,k} + gsi_{j,s} Re y_{c,s,k})
+struct BCollapse {
+  const double* yr;
+  const double* yi;
+  const double* gsr;
+  const double* gsi;
+  int R, m, ns, nd;
+  double* C;
+};
+
+__global__ __launch_bounds__(kBlock) void k_bcollapse(BCollapse p) {
+  const long long total = (long long)p.nd * p.m * p.R;
+  for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
+       t += (long long)gridDim.x * kBlock) {
+    const int c = (int)(t % p.R);
+    const long long jk = t / p.R;
+    const int k = (int)(jk % p.m), j = (int)(jk / p.m);
+    double acc = 0.0;
+    for (int s = 0; s < p.ns; ++s) {
+      const size_t e = ((size_t)c * p.ns + s) * p.m + k;
+      acc = fma(p.gsr[j + (size_t)p.nd * s], p.yi[e], acc);
+      acc = fma(p.gsi[j + (size_t)p.nd * s], p.yr[e], acc);
+    }
+    p.C[t] = acc;
+  }
+}
+
+// Z_j[i][c] = sum_k V[k][i][c] C[j][k][c]   (nd == 1: written to X with ld ldx)
+struct BVC {
+  const double* V;
+  const double* C;
+  int R, m, nd, nrows;
+  double* Z;          // [j][N][R] or X
+  size_t ldz;         // R, or the X row stride
+  int accumulate;
+};
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_bvc(BVC p) {
+  const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (long long)p.nrows * p.R) return;
+  const int c = (int)(e % p.R);
+  const long long i = e / p.R;
+  const size_t slice = (size_t)p.nrows * p.R;
+  double acc[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) acc[j] = 0.0;
+  for (int k = 0; k < p.m; ++k) {
+    const double v = p.V[(size_t)k * slice + e];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) acc[j] = fma(v, p.C[((size_t)j * p.m + k) * p.R + c], acc[j]);
+  }
+  if (ND == 1) {
+    const size_t o = (size_t)i * p.ldz + c;
+    p.Z[o] = p.accumulate ? p.Z[o] + acc[0] : acc[0];
+  } else {
+#pragma unroll
+    for (int j = 0; j < ND; ++j) p.Z[(size_t)j * slice + e] = acc[j];
+  }
+}
+
+// Fisher-KPP update per column pair (update_U.cu:8-16), column norms after.
+struct BUpdate {
+  const double* X;    // ld ldu (chunk columns at offset)
+  double* Usol;       // ld ldu
+  const double* mass;
+  double dt;
+  int clamp, x_zero, prepare;
+  size_t ldu;
+  int R, nrows, chunk;
+  double* be;         // be[0][c]
+  double* partials;
+  unsigned* counter;
+  unsigned* status;
+};
+
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_bupdate(BUpdate p) {
+  constexpr int CPW = R < 64 ? R : 64;
+  constexpr int RPW = 64 / CPW;
+  constexpr int CL = R / CPW;
+  static_assert(R % 2 == 0, "column pairs");
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int row0 = b * p.chunk;
+  const int row1 = min(row0 + p.chunk, p.nrows);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int slot = lane / CPW, c0 = lane % CPW;
+  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
+    const double ms = p.mass[r];
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const int c = c0 + q * CPW;
+      const size_t i = (size_t)r * p.ldu + c;
+      double val;
+      if (p.prepare) {
+        val = p.Usol[i];
+      } else {
+        // both columns of the pair are read by both lanes of the pair
+        const size_t i0 = i - (c & 1);
+        double u = p.x_zero ? 0.0 : (p.X[i0] + p.X[i0 + 1]);
+        if (p.clamp) u = u * (u >= 0.0 ? 1.0 : 0.0);
+        if (c & 1) {
+          const double ub = u / ms;
+          val = ((ms * ub) * (1.0 - ub)) * p.dt;
+        } else {
+          val = u;
+        }
+      }
+      if (!p.prepare) p.Usol[i] = val;   // X and Usol are distinct buffers
+    }
+  }
+  // the column norms of the next step's starting vectors are taken per chunk
+  // by k_bnorm
+}
+
+}  // namespace gmb
+
+// ====================================================================== host
+namespace {
+
+thread_local std::string g_berr;
+std::mutex g_bmu;
+std::unordered_set<const void*> g_breg;
+
+gm_status bfail(gm_status st, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_berr = buf;
+  return st;
+}
+
+#define B_HIP(call)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return bfail(e_ == hipErrorOutOfMemory ? GM_ENOMEM : GM_EDEVICE, "%s: %s (%s:%d)", \
+                   #call, hipGetErrorString(e_), __FILE__, __LINE__);                    \
+  } while (0)
+#define B_TRY(x)              \
+  do {                        \
+    gm_status s_ = (x);       \
+    if (s_ != GM_OK) return s_; \
+  } while (0)
+
+inline int cdivi(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+struct gm_batch {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int n = 0, P = 0, R = 0, Rc = 0, m = 0, nd = 0, ns = 0;
+  long long nnz = 0;
+  double dt = 0.0;
+  bool clamp = true, x_zero = true;
+  std::vector<double> gamma;
+  int *rowptr = nullptr, *col = nullptr;
+  double *val = nullptr, *Usol = nullptr, *X = nullptr, *W0 = nullptr, *W1 = nullptr, *T0 = nullptr,
+         *T1 = nullptr, *V = nullptr, *Z = nullptr, *C = nullptr, *al = nullptr, *be = nullptr,
+         *yr = nullptr, *yi = nullptr, *gsr = nullptr, *gsi = nullptr, *wts1 = nullptr,
+         *wts2 = nullptr, *etaxi = nullptr, *mass = nullptr, *partials = nullptr, *stage = nullptr;
+  double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
+  unsigned *counter = nullptr, *status = nullptr;
+  int G_sp = 1, chunk_sp = 1;
+  hipGraphExec_t graph = nullptr;
+};
+
+namespace {
+
+bool blive(const gm_batch* b) {
+  std::lock_guard<std::mutex> lk(g_bmu);
+  return b && g_breg.count(b);
+}
+
+template <typename T>
+gm_status balloc(T** p, size_t count) {
+  *p = nullptr;
+  B_HIP(hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)));
+  return GM_OK;
+}
+
+void bfree_all(gm_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  void* bufs[] = {b->rowptr, b->col, b->val, b->Usol, b->X, b->W0, b->W1, b->T0, b->T1, b->V,
+                  b->Z, b->C, b->al, b->be, b->yr, b->yi, b->gsr, b->gsi, b->wts1, b->wts2,
+                  b->etaxi, b->mass, b->partials, b->stage, b->wD, b->wDU, b->wDU2, b->wX,
+                  b->counter, b->status};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+gm_status breduce(gm_batch* b, int nv, double* out, int do_sqrt, int flag_zero) {
+  gm::ReduceArgs r{};
+  r.partials = b->partials;
+  r.G = b->G_sp;
+  r.nv = nv;
+  r.out = out;
+  r.stride = 1;
+  r.do_sqrt = do_sqrt;
+  r.flag_zero = flag_zero;
+  r.status = b->status;
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kBlock), 0, b->stream, r);
+  B_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+template <int R>
+struct Launch {
+  static gm_status spmm(gm_batch* b, int mode, const gmb::BSpmm& a) {
+    dim3 g(b->G_sp), t(gm::kBlock);
+    switch (mode) {
+      case gmb::kFirst: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kFirst>), g, t, 0, b->stream, a); break;
+      case gmb::kMid: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kMid>), g, t, 0, b->stream, a); break;
+      case gmb::kLast: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kLast>), g, t, 0, b->stream, a); break;
+      case gmb::kOnly: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kOnly>), g, t, 0, b->stream, a); break;
+      default: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kHorner>), g, t, 0, b->stream, a); break;
+    }
+    B_HIP(hipGetLastError());
+    return GM_OK;
+  }
+  static gm_status finish(gm_batch* b, const gmb::BFinish& a) {
+    hipLaunchKernelGGL((gmb::k_bfinish<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
+    B_HIP(hipGetLastError());
+    return GM_OK;
+  }
+  static gm_status update(gm_batch* b, const gmb::BUpdate& a) {
+    hipLaunchKernelGGL((gmb::k_bupdate<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
+    B_HIP(hipGetLastError());
+    return GM_OK;
+  }
+};
+
+template <int ND>
+void launch_bvc(gm_batch* b, const gmb::BVC& a) {
+  const long long e = (long long)b->n * b->Rc;
+  hipLaunchKernelGGL((gmb::k_bvc<ND>), dim3(cdivi(e, gm::kBlock)), dim3(gm::kBlock), 0, b->stream, a);
+}
+
+// One step of one chunk of Rc columns starting at column c0.
+template <int R>
+gm_status enqueue_chunk(gm_batch* b, int c0) {
+  const size_t N = (size_t)b->n;
+  const int d = b->nd - 1;
+  const size_t ldu = (size_t)b->R;
+  double* Wb[2] = {b->W0, b->W1};
+  double* Tb[2] = {b->T0, b->T1};
+  gmb::BSpmm base{};
+  base.rowptr = b->rowptr;
+  base.col = b->col;
+  base.val = b->val;
+  base.nrows = b->n;
+  base.chunk = b->chunk_sp;
+  base.al = b->al;
+  base.be = b->be;
+  base.alpha_out = b->al;
+  base.partials = b->partials;
+  base.counter = b->counter;
+  base.g0 = b->gamma[0];
+  // the chunk's starting vectors: Usol columns c0..c0+R-1 (row stride R_total)
+  {
+    gmb::BNorm nn{};
+    nn.u = b->Usol + c0;
+    nn.ldu = ldu;
+    nn.be = b->be;
+    nn.nrows = b->n;
+    nn.chunk = b->chunk_sp;
+    nn.partials = b->partials;
+    nn.counter = b->counter;
+    nn.status = b->status;
+    hipLaunchKernelGGL((gmb::k_bnorm<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, nn);
+    B_HIP(hipGetLastError());
+    B_TRY(breduce(b, R, b->be, 1, 0));                                   // beta_0
+  }
+  for (int k = 0; k < b->m; ++k) {
+    gmb::BSpmm a = base;
+    const double* wsrc = k == 0 ? b->Usol + c0 : Wb[(k - 1) & 1];
+    const size_t ldw = k == 0 ? ldu : (size_t)R;
+    double* wnext = Wb[k & 1];
+    a.k = k;
+    a.vk = b->V + (size_t)k * N * R;
+    a.vkm1 = k > 0 ? b->V + (size_t)(k - 1) * N * R : nullptr;
+    a.x = wsrc;
+    a.ldx = ldw;
+    if (d == 0) {
+      a.y = wnext;
+      a.ldy = R;
+      B_TRY(Launch<R>::spmm(b, gmb::kOnly, a));
+      B_TRY(breduce(b, R, b->al + (size_t)k * R, 0, 0));                // alpha_k
+    } else {
+      a.y = Tb[0];
+      a.ldy = R;
+      a.gi = b->gamma[1];
+      B_TRY(Launch<R>::spmm(b, gmb::kFirst, a));
+      for (int i = 2; i <= d; ++i) {
+        a.x = Tb[(i - 2) & 1];
+        a.ldx = R;
+        a.y = Tb[(i - 1) & 1];
+        a.gi = b->gamma[i];
+        B_TRY(Launch<R>::spmm(b, gmb::kMid, a));
+      }
+      a.x = Tb[(d - 1) & 1];
+      a.ldx = R;
+      a.y = wnext;
+      B_TRY(Launch<R>::spmm(b, gmb::kLast, a));
+      B_TRY(breduce(b, R, b->al + (size_t)k * R, 0, 0));                // alpha_k
+    }
+    gmb::BFinish f{};
+    f.w = wnext;
+    f.vk = a.vk;
+    f.al = b->al;
+    f.be = b->be;
+    f.k = k;
+    f.nrows = b->n;
+    f.chunk = b->chunk_sp;
+    f.partials = b->partials;
+    f.counter = b->counter;
+    f.status = b->status;
+    B_TRY(Launch<R>::finish(b, f));
+    B_TRY(breduce(b, R, b->be + (size_t)(k + 1) * R, 1, 1));          // beta_{k+1}
+  }
+  // shifted solves
+  gmb::BSolve s{};
+  s.al = b->al;
+  s.be = b->be;
+  s.R = R;
+  s.m = b->m;
+  s.ns = b->ns;
+  s.wts1 = b->wts1;
+  s.wts2 = b->wts2;
+  s.etaxi = b->etaxi;
+  s.wD = b->wD;
+  s.wDU = b->wDU;
+  s.wDU2 = b->wDU2;
+  s.wX = b->wX;
+  s.yr = b->yr;
+  s.yi = b->yi;
+  s.status = b->status;
+  hipLaunchKernelGGL(gmb::k_bsolves, dim3(cdivi((long long)R * b->ns, 64)), dim3(64), 0, b->stream, s);
+  B_HIP(hipGetLastError());
+  gmb::BCollapse cc{};
+  cc.yr = b->yr;
+  cc.yi = b->yi;
+  cc.gsr = b->gsr;
+  cc.gsi = b->gsi;
+  cc.R = R;
+  cc.m = b->m;
+  cc.ns = b->ns;
+  cc.nd = b->nd;
+  cc.C = b->C;
+  hipLaunchKernelGGL(gmb::k_bcollapse, dim3(std::min(1024, cdivi((long long)b->nd * b->m * R, gm::kBlock))),
+                     dim3(gm::kBlock), 0, b->stream, cc);
+  B_HIP(hipGetLastError());
+  gmb::BVC v{};
+  v.V = b->V;
+  v.C = b->C;
+  v.R = R;
+  v.m = b->m;
+  v.nd = b->nd;
+  v.nrows = b->n;
+  v.Z = d == 0 ? b->X + c0 : b->Z;
+  v.ldz = d == 0 ? ldu : (size_t)R;
+  v.accumulate = 0;
+  switch (b->nd) {
+#define B_VC(K) \
+  case K: launch_bvc<K>(b, v); break;
+    B_VC(1) B_VC(2) B_VC(3) B_VC(4) B_VC(5) B_VC(6) B_VC(7) B_VC(8) B_VC(9)
+    B_VC(10) B_VC(11) B_VC(12) B_VC(13) B_VC(14) B_VC(15) B_VC(16) B_VC(17)
+#undef B_VC
+    default: return bfail(GM_EINVAL, "gamma_size too large");
+  }
+  B_HIP(hipGetLastError());
+  const double* src = b->Z;
+  size_t lds = R;
+  for (int j = 1; j <= d; ++j) {
+    gmb::BSpmm a = base;
+    double* dst = (j == d) ? b->X + c0 : Tb[(j - 1) & 1];
+    a.x = src;
+    a.ldx = lds;
+    a.y = dst;
+    a.ldy = (j == d) ? ldu : (size_t)R;
+    a.z = b->Z + (size_t)j * N * R;
+    a.ldz = R;
+    a.accumulate = 0;
+    B_TRY(Launch<R>::spmm(b, gmb::kHorner, a));
+    src = dst;
+    lds = a.ldy;
+  }
+  return GM_OK;
+}
+
+template <int R>
+gm_status enqueue_update_chunk(gm_batch* b, int c0, int prepare) {
+  gmb::BUpdate u{};
+  u.X = b->X + c0;
+  u.Usol = b->Usol + c0;
+  u.mass = b->mass;
+  u.dt = b->dt;
+  u.clamp = b->clamp ? 1 : 0;
+  u.x_zero = 0;
+  u.prepare = prepare;
+  u.ldu = (size_t)b->R;
+  u.R = R;
+  u.nrows = b->n;
+  u.chunk = b->chunk_sp;
+  u.be = b->be;   // overwritten by the next chunk's norm kernel anyway
+  u.partials = b->partials;
+  u.counter = b->counter;
+  u.status = b->status;
+  return Launch<R>::update(b, u);
+}
+
+gm_status dispatch_chunk(gm_batch* b, int c0) {
+  switch (b->Rc) {
+    case 2: return enqueue_chunk<2>(b, c0);
+    case 4: return enqueue_chunk<4>(b, c0);
+    case 8: return enqueue_chunk<8>(b, c0);
+    case 16: return enqueue_chunk<16>(b, c0);
+    case 32: return enqueue_chunk<32>(b, c0);
+    case 64: return enqueue_chunk<64>(b, c0);
+    case 128: return enqueue_chunk<128>(b, c0);
+    case 256: return enqueue_chunk<256>(b, c0);
+  }
+  return bfail(GM_EINVAL, "unsupported chunk width %d", b->Rc);
+}
+
+gm_status dispatch_update(gm_batch* b, int c0, int prepare) {
+  switch (b->Rc) {
+    case 2: return enqueue_update_chunk<2>(b, c0, prepare);
+    case 4: return enqueue_update_chunk<4>(b, c0, prepare);
+    case 8: return enqueue_update_chunk<8>(b, c0, prepare);
+    case 16: return enqueue_update_chunk<16>(b, c0, prepare);
+    case 32: return enqueue_update_chunk<32>(b, c0, prepare);
+    case 64: return enqueue_update_chunk<64>(b, c0, prepare);
+    case 128: return enqueue_update_chunk<128>(b, c0, prepare);
+    case 256: return enqueue_update_chunk<256>(b, c0, prepare);
+  }
+  return bfail(GM_EINVAL, "unsupported chunk width %d", b->Rc);
+}
+
+// A whole step: every chunk's f(A) products, then every chunk's update.
+gm_status enqueue_bstep(gm_batch* b) {
+  for (int c0 = 0; c0 < b->R; c0 += b->Rc) B_TRY(dispatch_chunk(b, c0));
+  for (int c0 = 0; c0 < b->R; c0 += b->Rc) B_TRY(dispatch_update(b, c0, 0));
+  b->x_zero = true;
+  return GM_OK;
+}
+
+void set_bschedule(gm_batch* b) {
+  const int rows_per_pass = gm::kWaves * (b->Rc < 64 ? 64 / b->Rc : 1);
+  b->G_sp = std::max(1, std::min(4096, cdivi(b->n, rows_per_pass)));
+  b->chunk_sp = cdivi(b->n, b->G_sp);
+}
+
+gm_status alloc_chunk_buffers(gm_batch* b) {
+  const size_t N = (size_t)b->n, Rc = (size_t)b->Rc;
+  B_TRY(balloc(&b->W0, N * Rc));
+  B_TRY(balloc(&b->W1, N * Rc));
+  B_TRY(balloc(&b->T0, N * Rc));
+  B_TRY(balloc(&b->T1, N * Rc));
+  B_TRY(balloc(&b->V, N * Rc * b->m));
+  B_TRY(balloc(&b->Z, b->nd > 1 ? N * Rc * b->nd : 1));
+  B_TRY(balloc(&b->C, (size_t)b->nd * b->m * Rc));
+  B_TRY(balloc(&b->al, (size_t)b->m * Rc));
+  B_TRY(balloc(&b->be, (size_t)(b->m + 1) * Rc));
+  const size_t ny = (size_t)b->m * Rc * b->ns;
+  B_TRY(balloc(&b->yr, ny));
+  B_TRY(balloc(&b->yi, ny));
+  B_TRY(balloc(&b->wD, ny));
+  B_TRY(balloc(&b->wDU, ny));
+  B_TRY(balloc(&b->wDU2, ny));
+  B_TRY(balloc(&b->wX, ny));
+  B_TRY(balloc(&b->partials, 4096 * std::max<size_t>(Rc, 2)));
+  return GM_OK;
+}
+
+void free_chunk_buffers(gm_batch* b) {
+  double** bufs[] = {&b->W0, &b->W1, &b->T0, &b->T1, &b->V, &b->Z, &b->C, &b->al, &b->be,
+                     &b->yr, &b->yi, &b->partials};
+  for (double** p : bufs) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  double2** cb[] = {&b->wD, &b->wDU, &b->wDU2, &b->wX};
+  for (double2** p : cb) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gm_batch_last_error(void) { return g_berr.c_str(); }
+
+gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
+  g_berr.clear();
+  if (!out || !d) return bfail(GM_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->n <= 0 || d->num_problems <= 0 || !d->U || !d->A_ptr || !d->A_values || !d->A_ind)
+    return bfail(GM_EINVAL, "bad sizes or null arrays");
+  if (d->subspace_size < 1 || d->gamma_size < 1 || d->gamma_size > gmb::kMaxND || !d->gamma)
+    return bfail(GM_EINVAL, "bad subspace size or gamma");
+  if (d->contour_size < 1 || !d->gamma_s_real || !d->gamma_s_imag || !d->wts1 || !d->wts2 ||
+      !d->etaxi)
+    return bfail(GM_EINVAL, "bad contour data");
+  if (d->sparse_format != GM_SPARSE_CSR && d->sparse_format != GM_SPARSE_CSC)
+    return bfail(GM_EINVAL, "bad sparse format");
+  if (d->A_ptr[0] != 0 || d->A_ptr[d->n] != d->nnz) return bfail(GM_EINVAL, "bad pointer array");
+  for (int i = 0; i < d->n; ++i)
+    if (d->A_ptr[i + 1] < d->A_ptr[i]) return bfail(GM_EINVAL, "pointer array not monotone");
+  for (long long j = 0; j < d->nnz; ++j)
+    if (d->A_ind[j] < 0 || d->A_ind[j] >= d->n) return bfail(GM_EINVAL, "index out of range");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= d->device || d->device < 0)
+    return bfail(GM_EDEVICE, "no HIP device %d", d->device);
+  gm_batch* b = new gm_batch();
+  auto fail_clean = [&](gm_status st) {
+    std::string keep = g_berr;
+    bfree_all(b);
+    delete b;
+    g_berr = keep;
+    return st;
+  };
+#define B_CREATE(x)                       \
+  do {                                    \
+    gm_status s_ = (x);                   \
+    if (s_ != GM_OK) return fail_clean(s_); \
+  } while (0)
+  b->dev = d->device;
+  if (hipSetDevice(b->dev) != hipSuccess) return fail_clean(bfail(GM_EDEVICE, "hipSetDevice"));
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail_clean(bfail(GM_EDEVICE, "stream create"));
+  b->n = d->n;
+  b->nnz = d->nnz;
+  b->P = d->num_problems;
+  b->R = 2 * b->P;
+  b->m = d->subspace_size;
+  b->nd = d->gamma_size;
+  b->ns = d->contour_size;
+  b->dt = d->dt;
+  b->clamp = !(d->flags & GM_FLAG_NO_CLAMP);
+  b->gamma.assign(d->gamma, d->gamma + b->nd);
+  const size_t N = (size_t)b->n;
+  // chunk width: a power of two dividing R, at most 256 and at most chunk_columns
+  int rc = 2;
+  while (rc * 2 <= std::min(256, b->R) && b->R % (rc * 2) == 0) rc *= 2;
+  if (d->chunk_columns > 0)
+    while (rc > d->chunk_columns && rc > 2) rc /= 2;
+  if (b->R % rc != 0) return fail_clean(bfail(GM_EINVAL, "chunk width must divide 2*num_problems"));
+  b->Rc = rc;
+  // A -> CSR
+  std::vector<int> rp(N + 1, 0), ci((size_t)b->nnz);
+  std::vector<double> vv((size_t)b->nnz);
+  if (d->sparse_format == GM_SPARSE_CSR) {
+    std::copy(d->A_ptr, d->A_ptr + N + 1, rp.begin());
+    std::copy(d->A_ind, d->A_ind + b->nnz, ci.begin());
+    std::copy(d->A_values, d->A_values + b->nnz, vv.begin());
+  } else {
+    for (long long j = 0; j < b->nnz; ++j) rp[d->A_ind[j] + 1]++;
+    for (size_t i = 0; i < N; ++i) rp[i + 1] += rp[i];
+    std::vector<int> pos(rp.begin(), rp.end() - 1);
+    for (int c = 0; c < b->n; ++c)
+      for (int j = d->A_ptr[c]; j < d->A_ptr[c + 1]; ++j) {
+        const int r = d->A_ind[j];
+        ci[pos[r]] = c;
+        vv[pos[r]] = d->A_values[j];
+        pos[r]++;
+      }
+  }
+  hipStream_t st = b->stream;
+  auto up = [&](double** p, const double* src, size_t cnt) -> gm_status {
+    B_TRY(balloc(p, cnt));
+    B_HIP(hipMemcpyAsync(*p, src, cnt * sizeof(double), hipMemcpyHostToDevice, st));
+    return GM_OK;
+  };
+  B_CREATE(balloc(&b->rowptr, N + 1));
+  B_CREATE(balloc(&b->col, (size_t)b->nnz));
+  if (hipMemcpyAsync(b->rowptr, rp.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(b->col, ci.data(), ci.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+    return fail_clean(bfail(GM_EDEVICE, "upload A"));
+  B_CREATE(up(&b->val, vv.data(), vv.size()));
+  // U (n x 2P column-major) -> row-major Usol with dt on the g columns
+  std::vector<double> u((size_t)N * b->R);
+  for (int c = 0; c < b->R; ++c)
+    for (size_t i = 0; i < N; ++i)
+      u[i * b->R + c] = d->U[(size_t)c * N + i] * ((c & 1) ? b->dt : 1.0);
+  B_CREATE(up(&b->Usol, u.data(), u.size()));
+  B_CREATE(balloc(&b->X, N * b->R));
+  B_CREATE(up(&b->gsr, d->gamma_s_real, (size_t)b->nd * b->ns));
+  B_CREATE(up(&b->gsi, d->gamma_s_imag, (size_t)b->nd * b->ns));
+  B_CREATE(up(&b->wts1, d->wts1, 2 * (size_t)b->ns));
+  B_CREATE(up(&b->wts2, d->wts2, 2 * (size_t)b->ns));
+  B_CREATE(up(&b->etaxi, d->etaxi, 2 * (size_t)b->ns));
+  std::vector<double> ones;
+  if (!d->mass) ones.assign(N, 1.0);
+  B_CREATE(up(&b->mass, d->mass ? d->mass : ones.data(), N));
+  B_CREATE(balloc(&b->stage, N * b->R));
+  B_CREATE(balloc(&b->counter, 1));
+  B_CREATE(balloc(&b->status, 1));
+  if (hipMemsetAsync(b->counter, 0, sizeof(unsigned), st) != hipSuccess ||
+      hipMemsetAsync(b->status, 0, sizeof(unsigned), st) != hipSuccess)
+    return fail_clean(bfail(GM_EDEVICE, "memset"));
+  // chunk buffers: halve the chunk until they fit
+  for (;;) {
+    set_bschedule(b);
+    gm_status s = alloc_chunk_buffers(b);
+    if (s == GM_OK) break;
+    free_chunk_buffers(b);
+    (void)hipGetLastError();
+    if (s != GM_ENOMEM || b->Rc <= 2) return fail_clean(s);
+    b->Rc /= 2;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return fail_clean(bfail(GM_EDEVICE, "sync"));
+  if (d->flags & GM_FLAG_AUTOTUNE) {
+    // Time one chunk of the step for each chunk width that the allocated
+    // buffers hold (every width <= Rc) and keep the cheapest per column.  The
+    // runs overwrite only X and scratch; Usol is untouched.
+    const int rc_max = b->Rc;
+    double best = 1e300;
+    int best_rc = rc_max;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+      return fail_clean(bfail(GM_EDEVICE, "event create"));
+    for (int rc = rc_max; rc >= 8 || rc == rc_max; rc /= 2) {
+      b->Rc = rc;
+      set_bschedule(b);
+      gm_status s1 = dispatch_chunk(b, 0);                       // warm-up
+      (void)hipEventRecord(e0, st);
+      gm_status s2 = dispatch_chunk(b, 0);
+      (void)hipEventRecord(e1, st);
+      if (s1 != GM_OK || s2 != GM_OK || hipEventSynchronize(e1) != hipSuccess)
+        return fail_clean(s1 != GM_OK ? s1 : (s2 != GM_OK ? s2 : bfail(GM_EDEVICE, "autotune")));
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      const double per_col = (double)ms / rc;
+      if (per_col < best) {
+        best = per_col;
+        best_rc = rc;
+      }
+      if (rc <= 2) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->Rc = best_rc;
+    set_bschedule(b);
+    if (hipMemsetAsync(b->status, 0, sizeof(unsigned), st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail_clean(bfail(GM_EDEVICE, "autotune sync"));
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_bmu);
+    g_breg.insert(b);
+  }
+  *out = b;
+  return GM_OK;
+#undef B_CREATE
+}
+
+gm_status gm_batch_delete(gm_batch* b) {
+  {
+    std::lock_guard<std::mutex> lk(g_bmu);
+    if (!b || !g_breg.count(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
+    g_breg.erase(b);
+  }
+  (void)hipSetDevice(b->dev);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  bfree_all(b);
+  delete b;
+  return GM_OK;
+}
+
+gm_status gm_batch_step(gm_batch* b, int32_t nsteps, int32_t use_graph) {
+  if (!blive(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
+  B_HIP(hipSetDevice(b->dev));
+  if (nsteps < 0) return bfail(GM_EINVAL, "nsteps < 0");
+  if (!use_graph) {
+    for (int i = 0; i < nsteps; ++i) B_TRY(enqueue_bstep(b));
+    return GM_OK;
+  }
+  if (!b->graph) {
+    hipGraph_t g = nullptr;
+    B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    gm_status s = enqueue_bstep(b);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (s != GM_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    if (e != hipSuccess) return bfail(GM_EDEVICE, "capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      b->graph = nullptr;
+      return bfail(GM_EDEVICE, "instantiate: %s", hipGetErrorString(e));
+    }
+  }
+  for (int i = 0; i < nsteps; ++i) B_HIP(hipGraphLaunch(b->graph, b->stream));
+  return GM_OK;
+}
+
+gm_status gm_batch_get_U(gm_batch* b, double* dst) {
+  if (!blive(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
+  if (!dst) return bfail(GM_EINVAL, "null destination");
+  B_HIP(hipSetDevice(b->dev));
+  const size_t N = (size_t)b->n;
+  std::vector<double> tmp(N * b->R);
+  B_HIP(hipMemcpyAsync(tmp.data(), b->Usol, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       b->stream));
+  B_HIP(hipStreamSynchronize(b->stream));
+  for (int c = 0; c < b->R; ++c)
+    for (size_t i = 0; i < N; ++i) dst[(size_t)c * N + i] = tmp[i * b->R + c];
+  return GM_OK;
+}
+
+gm_status gm_batch_synchronize(gm_batch* b) {
+  if (!blive(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
+  B_HIP(hipSetDevice(b->dev));
+  unsigned stv = 0;
+  B_HIP(hipMemcpyAsync(&stv, b->status, sizeof(unsigned), hipMemcpyDeviceToHost, b->stream));
+  B_HIP(hipStreamSynchronize(b->stream));
+  if (stv & 6u) return bfail(GM_EBREAKDOWN, "batched Lanczos breakdown (status %u)", stv);
+  return GM_OK;
+}
+
+gm_status gm_batch_info(gm_batch* b, int32_t* num_problems, int32_t* chunk_columns) {
+  if (!blive(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
+  if (num_problems) *num_problems = b->P;
+  if (chunk_columns) *chunk_columns = b->Rc;
+  return GM_OK;
+}
+
+gm_status gm_batch_get_stream(gm_batch* b, void** stream) {
+  if (!blive(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
+  if (!stream) return bfail(GM_EINVAL, "null pointer");
+  *stream = b->stream;
+  return GM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// gm_device.h -- device helpers shared by the kernels of the f(A)b engine.
+//
+// Deterministic reductions: each block reduces in a fixed tree, writes one
+// partial, and the last block to finish (atomic ticket) sums the partials in
+// block order, so results do not depend on scheduling.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gm {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ double2 d2(double a, double b) { return make_double2(a, b); }
+
+// Load that bypasses the non-coherent per-CU cache: used only by the last
+// block to read partials that other CUs wrote during the same kernel.
+__device__ __forceinline__ double ld_coherent(const double* p) {
+  unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double(static_cast<long long>(v));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block sum of a double2; result valid in thread 0.  Fixed order.
+__device__ __forceinline__ double2 block_sum2(double2 v) {
+  __shared__ double2 sh[kWaves];
+  v.x = wave_sum(v.x);
+  v.y = wave_sum(v.y);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double2 r = d2(0.0, 0.0);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) { r.x += sh[i].x; r.y += sh[i].y; }
+  }
+  return r;
+}
+
+// Block sum of a double2 written by thread 0 to partials[2b], partials[2b+1].
+// The grid-wide total is formed by k_reduce after the kernel ends: a kernel
+// boundary is the cheapest cross-XCD synchronisation (an in-kernel ticket
+// needs a device-scope release fence per block, i.e. an L2 write-back on
+// every XCD).
+__device__ __forceinline__ void block_partial2(double2 v, double* partials) {
+  const double2 bs = block_sum2(v);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = bs.x;
+    partials[2 * blockIdx.x + 1] = bs.y;
+  }
+}
+
+// Sums nv values over G blocks of partials ([G][nv]) in a fixed order and
+// writes out[v * stride] (square-rooted when do_sqrt).  One block.  Status
+// bits: 4 = non-finite result, 1 = exact zero (when flag_zero).
+struct ReduceArgs {
+  const double* partials;
+  int G, nv;
+  double* out;
+  int stride;
+  int do_sqrt, flag_zero;
+  unsigned* status;
+};
+
+// (a template so that both translation units may instantiate it)
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs p) {
+  __shared__ double sub[kBlock];
+  for (int v0 = 0; v0 < p.nv; v0 += kBlock) {
+    const int W = min(p.nv - v0, kBlock);
+    const int ns = kBlock / W;
+    const int v = v0 + (int)threadIdx.x % W, sb = (int)threadIdx.x / W;
+    double s = 0.0;
+    if (sb < ns)
+      for (int b = sb; b < p.G; b += ns) s += p.partials[(size_t)b * p.nv + v];
+    __syncthreads();
+    sub[threadIdx.x] = s;
+    __syncthreads();
+    if ((int)threadIdx.x < W) {
+      double t = 0.0;
+      for (int i = 0; i < ns; ++i) t += sub[i * W + threadIdx.x];
+      if (p.do_sqrt) {
+        t = sqrt(t);
+        if (p.status) {
+          if (!isfinite(t))
+            atomicOr(p.status, 4u);
+          else if (p.flag_zero && t == 0.0)
+            atomicOr(p.status, 1u);
+        }
+      }
+      p.out[(size_t)(v0 + threadIdx.x) * p.stride] = t;
+    }
+  }
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that the
+// blocks resident on one XCD own one contiguous range of rows (their x
+// gathers then share that XCD's L2).  Bijective on [0, G).
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+  const int q = G >> 3, r = G & 7;
+  const int x = b & 7, idx = b >> 3;
+  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+__device__ __forceinline__ double safe_inv(double b) {
+  return (b != 0.0 && isfinite(b)) ? 1.0 / b : 0.0;
+}
+__device__ __forceinline__ double safe_div(double a, double b) {
+  return (b != 0.0 && isfinite(b)) ? a / b : 0.0;
+}
+
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return d2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return d2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
+// Smith's algorithm.
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double r = b.y / b.x, den = b.x + b.y * r;
+    return d2((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+  }
+  const double r = b.x / b.y, den = b.x * r + b.y;
+  return d2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+}
+
+}  // namespace gm

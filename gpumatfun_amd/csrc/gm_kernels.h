@@ -18,96 +18,13 @@
 // finish (atomic ticket) sums the partials in block order.  Results do not
 // depend on scheduling, so repeated runs are bit-identical.
 #pragma once
-#include <hip/hip_runtime.h>
-#include <stdint.h>
+#include "gm_device.h"
+
 
 namespace gm {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
 constexpr int kMaxDegree = 16;        // gamma_size <= 17
 constexpr int kDeflTile = 8;          // deflation vectors per register tile
-
-__device__ __forceinline__ double2 d2(double a, double b) { return make_double2(a, b); }
-
-// Load that bypasses the non-coherent per-CU cache: used only by the last
-// block to read partials that other CUs wrote during the same kernel.
-__device__ __forceinline__ double ld_coherent(const double* p) {
-  unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double(static_cast<long long>(v));
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-// Block sum of a double2; result valid in thread 0.  Fixed order.
-__device__ __forceinline__ double2 block_sum2(double2 v) {
-  __shared__ double2 sh[kWaves];
-  v.x = wave_sum(v.x);
-  v.y = wave_sum(v.y);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double2 r = d2(0.0, 0.0);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < kWaves; ++i) { r.x += sh[i].x; r.y += sh[i].y; }
-  }
-  return r;
-}
-
-// Grid-wide deterministic sum of one double2 per block.  Returns true in the
-// last block to arrive; there *out (shared) holds the total in every thread.
-__device__ __forceinline__ bool grid_sum2(double2 blockval /* valid in thread 0 */,
-                                          double* partials, unsigned* counter,
-                                          double2* out) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = blockval.x;
-    partials[2 * blockIdx.x + 1] = blockval.y;
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double2 acc = d2(0.0, 0.0);
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
-    acc.x += ld_coherent(partials + 2 * i);
-    acc.y += ld_coherent(partials + 2 * i + 1);
-  }
-  acc = block_sum2(acc);
-  __shared__ double2 tot;
-  if (threadIdx.x == 0) {
-    tot = acc;
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  *out = tot;
-  return true;
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs; remap so that the
-// blocks resident on one XCD own one contiguous range of rows (their x
-// gathers then share that XCD's L2).  Bijective on [0, G).
-__device__ __forceinline__ int xcd_remap(int b, int G) {
-  const int q = G >> 3, r = G & 7;
-  const int x = b & 7, idx = b >> 3;
-  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
-}
-
-__device__ __forceinline__ double safe_inv(double b) {
-  return (b != 0.0 && isfinite(b)) ? 1.0 / b : 0.0;
-}
-__device__ __forceinline__ double safe_div(double a, double b) {
-  return (b != 0.0 && isfinite(b)) ? a / b : 0.0;
-}
 
 // ---------------------------------------------------------------------------
 // CSR SpMM with R = 2 interleaved columns and fused Lanczos / Horner epilogues.
@@ -221,17 +138,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
     }
   }
 
-  if (MODE == kLast || MODE == kOnly) {
-    const double2 bs = block_sum2(dot);
-    double2 tot;
-    if (grid_sum2(bs, p.partials, p.counter, &tot)) {
-      if (threadIdx.x == 0) {
-        double* a = const_cast<double*>(p.aabb);
-        a[p.k] = tot.x;            // alpha_1(k)
-        a[p.k + p.ld] = tot.y;     // alpha_2(k)
-      }
-    }
-  }
+  // alpha_k = sum of these partials, formed by k_reduce into aabb(k, 0:1)
+  if (MODE == kLast || MODE == kOnly) block_partial2(dot, p.partials);
 }
 
 // w <- w - alpha_k v_k (in place), beta_{k+1} = ||w||.   build_subspace.cu:63-76
@@ -260,19 +168,8 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_finish(FinishArgs p) {
     nrm.x = fma(w.x, w.x, nrm.x);
     nrm.y = fma(w.y, w.y, nrm.y);
   }
-  const double2 bs = block_sum2(nrm);
-  double2 tot;
-  if (grid_sum2(bs, p.partials, p.counter, &tot)) {
-    if (threadIdx.x == 0) {
-      const double b1 = sqrt(tot.x), b2 = sqrt(tot.y);
-      p.aabb[p.k + 1 + 2 * p.ld] = b1;
-      p.aabb[p.k + 1 + 3 * p.ld] = b2;
-      if (!isfinite(b1) || !isfinite(b2))
-        atomicOr(p.status, 4u);
-      else if (b1 == 0.0 || b2 == 0.0)
-        atomicOr(p.status, 1u);
-    }
-  }
+  // beta_{k+1} = sqrt(sum of these partials), formed by k_reduce into aabb(k+1, 2:3)
+  block_partial2(nrm, p.partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -283,20 +180,6 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_finish(FinishArgs p) {
 // lin_sys_solves.cu:5-21.  Workspaces are [row][system] so a wavefront's
 // accesses are contiguous.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return d2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return d2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
-// Smith's algorithm.
-__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
-  if (fabs(b.x) >= fabs(b.y)) {
-    const double r = b.y / b.x, den = b.x + b.y * r;
-    return d2((a.x + a.y * r) / den, (a.y - a.x * r) / den);
-  }
-  const double r = b.x / b.y, den = b.x * r + b.y;
-  return d2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
-}
 
 struct SolveArgs {
   const double* aabb;
@@ -528,13 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_update1(UpdateArgs p) {
     }
   }
   if (p.K == 0) {
-    const double2 bs = block_sum2(nrm);
-    double2 tot;
-    if (grid_sum2(bs, p.partials, p.counter, &tot) && threadIdx.x == 0) {
-      p.beta0[0] = sqrt(tot.x);
-      p.beta0[p.ld] = sqrt(tot.y);
-      if (!isfinite(tot.x) || !isfinite(tot.y)) atomicOr(p.status, 4u);
-    }
+    block_partial2(nrm, p.partials);   // beta_0 = sqrt(sum), formed by k_reduce
     return;
   }
   // pass 2: block partials of Z = Q^T Usol, kDeflTile vectors at a time.
@@ -571,40 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_update1(UpdateArgs p) {
       p.partials[(size_t)blockIdx.x * 2 * p.K + 2 * k0 + threadIdx.x] = s;
     }
   }
-  // last block sums the partials in block order
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned t = atomicAdd(p.counter, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const int NV = 2 * p.K;
-  const int nsub = NV >= kBlock ? 1 : kBlock / NV;
-  __shared__ double sub[kBlock];
-  for (int v0 = 0; v0 < NV; v0 += kBlock) {
-    const int v = v0 + (threadIdx.x % min(NV - v0, kBlock));
-    const int sb = threadIdx.x / min(NV - v0, kBlock);
-    const int ns = NV - v0 >= kBlock ? 1 : nsub;
-    double s = 0.0;
-    if (sb < ns)
-      for (int bb = sb; bb < (int)gridDim.x; bb += ns)
-        s += ld_coherent(p.partials + (size_t)bb * NV + v);
-    __syncthreads();
-    sub[threadIdx.x] = s;
-    __syncthreads();
-    const int width = min(NV - v0, kBlock);
-    if ((int)threadIdx.x < width) {
-      double t = 0.0;
-      for (int i = 0; i < ns; ++i) t += sub[i * width + threadIdx.x];
-      p.Zout[v0 + threadIdx.x] = t;
-    }
-  }
-  if (threadIdx.x == 0)
-    __hip_atomic_store(p.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // Z = Q^T Usol = sum of these [G][2K] partials, formed by k_reduce
 }
 
 // U = Usol - Q Z; update = Q diag(e^L, phi(L)) Z; beta0 = ||U||.
@@ -655,13 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_update2(Update2Args p) {
     nrm.x = fma(u.x, u.x, nrm.x);
     nrm.y = fma(u.y, u.y, nrm.y);
   }
-  const double2 bs = block_sum2(nrm);
-  double2 tot;
-  if (grid_sum2(bs, p.partials, p.counter, &tot) && threadIdx.x == 0) {
-    p.beta0[0] = sqrt(tot.x);
-    p.beta0[p.ld] = sqrt(tot.y);
-    if (!isfinite(tot.x) || !isfinite(tot.y)) atomicOr(p.status, 4u);
-  }
+  block_partial2(nrm, p.partials);   // beta_0 = sqrt(sum), formed by k_reduce
 }
 
 // de-interleave an N x 2 double2 array into column-major (for the getters)

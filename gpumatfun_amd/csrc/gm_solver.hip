@@ -149,6 +149,23 @@ gm_status launch_spmm_mode(gm_solver* s, const SpmmArgs& a) {
   return GM_OK;
 }
 
+// Sum [G][nv] block partials into out[v * stride] (k_reduce, one block).
+gm_status launch_reduce(gm_solver* s, int G, int nv, double* out, int stride, int do_sqrt,
+                        int flag_zero) {
+  gm::ReduceArgs r{};
+  r.partials = s->partials;
+  r.G = G;
+  r.nv = nv;
+  r.out = out;
+  r.stride = stride;
+  r.do_sqrt = do_sqrt;
+  r.flag_zero = flag_zero;
+  r.status = s->status;
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(kBlock), 0, s->stream, r);
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
 SpmmArgs spmm_base(gm_solver* s) {
   SpmmArgs a{};
   a.rowptr = s->rowptr;
@@ -190,6 +207,7 @@ gm_status enqueue_build(gm_solver* s) {
       a.x = wsrc;
       a.y = wnext;
       GM_TRY(launch_spmm_mode<gm::kOnly>(s, a));
+      GM_TRY(launch_reduce(s, s->G_sp, 2, s->aabb + k, s->m + 1, 0, 0));   // alpha_k
     } else {
       a.x = wsrc;
       a.y = s->T[0];
@@ -204,6 +222,7 @@ gm_status enqueue_build(gm_solver* s) {
       a.x = s->T[(d - 1) & 1];
       a.y = wnext;
       GM_TRY(launch_spmm_mode<gm::kLast>(s, a));
+      GM_TRY(launch_reduce(s, s->G_sp, 2, s->aabb + k, s->m + 1, 0, 0));   // alpha_k
     }
     gm::FinishArgs f{};
     f.w = wnext;
@@ -218,6 +237,8 @@ gm_status enqueue_build(gm_solver* s) {
     f.status = s->status;
     hipLaunchKernelGGL(gm::k_lanczos_finish, dim3(s->G_el), dim3(kBlock), 0, s->stream, f);
     GM_HIP(hipGetLastError());
+    // beta_{k+1} = ||w||, flagged when zero or non-finite
+    GM_TRY(launch_reduce(s, s->G_el, 2, s->aabb + (k + 1) + 2 * (s->m + 1), s->m + 1, 1, 1));
   }
   return GM_OK;
 }
@@ -341,6 +362,12 @@ gm_status enqueue_update(gm_solver* s, int mode) {
       break;
   }
   GM_HIP(hipGetLastError());
+  double* beta0 = s->aabb + 2 * (s->m + 1);
+  if (s->K == 0) {
+    GM_TRY(launch_reduce(s, G, 2, beta0, s->m + 1, 1, 0));              // beta_0
+  } else {
+    GM_TRY(launch_reduce(s, G, 2 * s->K, s->Zdefl, 1, 0, 0));           // Z = Q^T Usol
+  }
   if (s->K > 0) {
     gm::Update2Args b{};
     b.Usol = s->Usol;
@@ -361,6 +388,7 @@ gm_status enqueue_update(gm_solver* s, int mode) {
     hipLaunchKernelGGL(gm::k_update2, dim3(s->G_el), dim3(kBlock), 4 * s->K * sizeof(double),
                        s->stream, b);
     GM_HIP(hipGetLastError());
+    GM_TRY(launch_reduce(s, s->G_el, 2, beta0, s->m + 1, 1, 0));        // beta_0
   } else {
     // update = 0 when nothing is deflated (Q is N x 0)
   }

@@ -70,6 +70,124 @@ def laplacian_3d_dirichlet(n, scaled=True):
     return _csr(A)
 
 
+def gaussian_random_field(n, corr_len=0.1, seed=1):
+    """Zero-mean, unit-variance stationary GRF on the n x n interior grid of the unit square.
+
+    Squared-exponential covariance exp(-r^2 / (2 l^2)), synthesised spectrally:
+    white noise (``default_rng(seed).standard_normal``) on a periodic grid of
+    twice the size (to suppress wrap-around), filtered by the square root of the
+    covariance's spectral density, cropped and normalised to unit sample variance.
+    Returned in natural ordering (index i + n*j).
+    """
+    h = 1.0 / (n + 1)
+    P = 2 * n
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((P, P))
+    k = 2 * np.pi * np.fft.fftfreq(P, d=h)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    spec = np.exp(-0.25 * corr_len ** 2 * (kx ** 2 + ky ** 2))   # sqrt of the SE density
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * spec))[:n, :n]
+    f = (f - f.mean()) / f.std()
+    return f.reshape(-1, order="F")
+
+
+def varcoef_laplacian_2d(kappa, n):
+    """-div(kappa grad) on the n x n interior grid (Dirichlet), 5-point, scaled 1/h^2.
+
+    Edge conductance (kappa_i + kappa_k)/2 between grid neighbours; an edge to
+    the boundary uses kappa_i.  With this model the matrix depends on kappa as
+        A_ik = -(kappa_i + kappa_k) / (2 h^2),   row sum_k A_ik = nb_i kappa_i / h^2
+    (nb_i = number of boundary edges of node i), which is what the adjoint's
+    d A / d kappa uses.
+    """
+    kappa = np.asarray(kappa, dtype=np.float64).reshape(n, n, order="F")
+    h2 = float((n + 1) ** 2)
+    idx = np.arange(n * n).reshape(n, n, order="F")
+    rows, cols, vals = [], [], []
+    diag = np.zeros((n, n))
+    for axis in (0, 1):
+        a = [slice(None), slice(None)]
+        b = [slice(None), slice(None)]
+        a[axis] = slice(0, n - 1)
+        b[axis] = slice(1, n)
+        ke = 0.5 * (kappa[tuple(a)] + kappa[tuple(b)]) * h2
+        ia = idx[tuple(a)].ravel(order="F")
+        ib = idx[tuple(b)].ravel(order="F")
+        kv = ke.ravel(order="F")
+        rows += [ia, ib]
+        cols += [ib, ia]
+        vals += [-kv, -kv]
+        diag[tuple(a)] += ke
+        diag[tuple(b)] += ke
+        # boundary edges (both ends of the axis)
+        lo = [slice(None), slice(None)]
+        hi = [slice(None), slice(None)]
+        lo[axis] = 0
+        hi[axis] = n - 1
+        diag[tuple(lo)] += kappa[tuple(lo)] * h2
+        diag[tuple(hi)] += kappa[tuple(hi)] * h2
+    rows.append(idx.ravel(order="F"))
+    cols.append(idx.ravel(order="F"))
+    vals.append(diag.ravel(order="F"))
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(n * n, n * n))
+    return _csr(A)
+
+
+def anisotropic_9pt(n, ratio=9.0, theta_max=np.pi / 3, length=1.0):
+    """-div(D grad) on an n x n Dirichlet grid with a rotating fibre tensor, 9-point stencil.
+
+    D(x) = R(theta) diag(ratio, 1) R(theta)^T with theta = theta_max * x / length
+    (fibre angle rotating linearly across x).  Assembled as bilinear (Q1)
+    finite elements on the uniform grid with D evaluated at cell centres, which
+    gives a symmetric positive definite 9-point matrix for any SPD D; scaled so
+    that it is the stiffness matrix divided by h^2 (a lumped-mass-free
+    finite-difference analogue).  Returns A (CSR).
+    """
+    h = length / (n + 1)
+    # Q1 element stiffness of a constant tensor [[a, b], [b, c]] on a unit square
+    # cell is a*Ka + b*Kb + c*Kc; local nodes (0,0), (1,0), (0,1), (1,1);
+    # 2x2 Gauss quadrature (exact for the bilinear gradients).
+    gp = np.array([-1, 1]) / np.sqrt(3.0)
+    Ka = np.zeros((4, 4))
+    Kb = np.zeros((4, 4))
+    Kc = np.zeros((4, 4))
+    for s in gp:
+        for t in gp:
+            xi, et = 0.5 * (s + 1), 0.5 * (t + 1)
+            dx = np.array([-(1 - et), (1 - et), -et, et])
+            dy = np.array([-(1 - xi), -xi, (1 - xi), xi])
+            Ka += 0.25 * np.outer(dx, dx)
+            Kb += 0.25 * (np.outer(dx, dy) + np.outer(dy, dx))
+            Kc += 0.25 * np.outer(dy, dy)
+    N1 = n + 2                                   # node grid including the boundary
+    ci, cj = np.meshgrid(np.arange(N1 - 1), np.arange(N1 - 1), indexing="ij")
+    ci = ci.ravel()
+    cj = cj.ravel()
+    th = theta_max * (ci + 0.5) * h / length
+    c2, s2 = np.cos(th) ** 2, np.sin(th) ** 2
+    a = ratio * c2 + s2
+    b = (ratio - 1.0) * np.cos(th) * np.sin(th)
+    c = ratio * s2 + c2
+    Ke = a[:, None, None] * Ka + b[:, None, None] * Kb + c[:, None, None] * Kc
+    lx = np.stack([ci, ci + 1, ci, ci + 1], 1)
+    ly = np.stack([cj, cj, cj + 1, cj + 1], 1)
+    inside = (lx >= 1) & (lx <= n) & (ly >= 1) & (ly <= n)
+    gidx = np.where(inside, (lx - 1) + n * (ly - 1), -1)
+    R = np.repeat(gidx[:, :, None], 4, axis=2)
+    Cc = np.repeat(gidx[:, None, :], 4, axis=1)
+    keep = (R >= 0) & (Cc >= 0)
+    A = sp.csr_matrix((Ke[keep] / h ** 2, (R[keep], Cc[keep])), shape=(n * n, n * n))
+    A = 0.5 * (A + A.T)
+    return _csr(A)
+
+
+def uneven_batch_sides(count=64, lo=32, hi=448, seed=0):
+    """Side lengths drawn log-uniformly in [lo, hi] (integers), seeded."""
+    rng = np.random.default_rng(seed)
+    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(int)
+
+
 def p1_stiffness_lumped_mass(nodes, elements):
     """P1 finite-element stiffness K and lumped (row-sum) mass diagonal.
 

@@ -177,6 +177,55 @@ gm_status gm_get_stream(gm_solver* s, void** stream);
 gm_status gm_profile_spmm(gm_solver* s, int32_t mode, int32_t reps, double* avg_ms);
 gm_status gm_profile_copy(gm_solver* s, int64_t bytes, int32_t reps, double* avg_ms);
 
+/* ---------------------------------------------------------------------------
+ * Batched form (no reference counterpart): P independent problems that share
+ * A, the contour nodes/weights and the polynomial preconditioner, stepped
+ * together.  Problem p owns the column pair [u_p, g_p] (the plain path's exp
+ * and phi columns), so one step is 2P f(A)b solves; the engine processes
+ * columns in chunks whose Lanczos bases fit in device memory.  The step is
+ * the plain path's step without deflation (update_U.cu:8-16 per pair).
+ * ------------------------------------------------------------------------- */
+typedef struct gm_batch gm_batch;
+
+typedef struct gm_batch_desc {
+    int32_t n;
+    int32_t sparse_format;         /* GM_SPARSE_CSC or GM_SPARSE_CSR         */
+    int64_t nnz;
+    const double* A_values;
+    const int32_t* A_ptr;
+    const int32_t* A_ind;
+    int32_t num_problems;          /* P                                      */
+    const double* U;               /* n x 2P column-major [u_1 g_1 u_2 g_2 ...] */
+    int32_t subspace_size;         /* m                                      */
+    int32_t gamma_size;
+    const double* gamma;
+    int32_t contour_size;          /* n_s                                    */
+    const double* gamma_s_real;    /* (d+1) x n_s                            */
+    const double* gamma_s_imag;
+    const double* wts1;            /* n_s x 2 [Re Im], includes s            */
+    const double* wts2;
+    const double* etaxi;           /* n_s complex shifts, [re im] interleaved */
+    double dt;
+    const double* mass;            /* n (may be NULL: ones)                  */
+    int32_t chunk_columns;         /* columns per pass, power of two dividing
+                                      2P (0: largest that fits, <= 256)      */
+    uint32_t flags;                /* GM_FLAG_NO_CLAMP; GM_FLAG_AUTOTUNE times
+                                      each chunk width <= the allocated one and
+                                      keeps the cheapest per column          */
+    int32_t device;
+} gm_batch_desc;
+
+const char* gm_batch_last_error(void);
+gm_status gm_batch_create(const gm_batch_desc* desc, gm_batch** out);
+gm_status gm_batch_delete(gm_batch* b);
+/* nsteps time steps of all problems (asynchronous; HIP graph when use_graph). */
+gm_status gm_batch_step(gm_batch* b, int32_t nsteps, int32_t use_graph);
+/* dst: n x 2P column-major [u_p, dt g_p].                                  */
+gm_status gm_batch_get_U(gm_batch* b, double* dst);
+gm_status gm_batch_synchronize(gm_batch* b);
+gm_status gm_batch_info(gm_batch* b, int32_t* num_problems, int32_t* chunk_columns);
+gm_status gm_batch_get_stream(gm_batch* b, void** stream);
+
 #ifdef __cplusplus
 }
 #endif

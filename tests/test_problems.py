@@ -3,6 +3,7 @@
 import os
 
 import numpy as np
+import pytest
 import scipy.fft as sfft
 import scipy.sparse as sp
 
@@ -72,6 +73,42 @@ def test_disc_mesh_is_seeded_and_on_the_disc():
     r = np.hypot(n1[:, 0], n1[:, 1])
     assert r.max() <= 1.0 + 1e-15
     assert np.sum(np.isclose(r, 1.0)) == 128
+
+
+def test_varcoef_laplacian_constant_kappa_and_kappa_model():
+    n = 16
+    A1 = problems.varcoef_laplacian_2d(np.ones(n * n), n)
+    assert abs(A1 - problems.laplacian_2d_dirichlet(n)).max() == 0.0
+    kappa = np.exp(0.5 * problems.gaussian_random_field(n, 0.1, seed=1))
+    A = problems.varcoef_laplacian_2d(kappa, n)
+    assert abs(A - A.T).max() == 0.0
+    # linear in kappa: A(2 kappa) = 2 A(kappa); off-diagonals -(k_i + k_j)/(2h^2)
+    assert abs(problems.varcoef_laplacian_2d(2 * kappa, n) - 2 * A).max() < 1e-9
+    h2 = (n + 1) ** 2
+    assert A[0, 1] == pytest.approx(-(kappa[0] + kappa[1]) * h2 / 2, rel=1e-15)
+    assert np.all(np.linalg.eigvalsh(A.toarray()) > 0)
+
+
+def test_grf_is_seeded_unit_variance():
+    a = problems.gaussian_random_field(64, 0.1, seed=1)
+    b = problems.gaussian_random_field(64, 0.1, seed=1)
+    assert np.array_equal(a, b)
+    assert abs(a.mean()) < 1e-12 and abs(a.std() - 1) < 1e-12
+    # correlation length: neighbouring values are strongly correlated (h = 1/65 << 0.1)
+    g = a.reshape(64, 64, order="F")
+    assert np.corrcoef(g[:-1].ravel(), g[1:].ravel())[0, 1] > 0.9
+
+
+def test_anisotropic_9pt_spd_and_isotropic_limit():
+    A = problems.anisotropic_9pt(24, ratio=9.0)
+    assert abs(A - A.T).max() == 0.0
+    assert np.linalg.eigvalsh(A.toarray()).min() > 0
+    assert A.getnnz(axis=1).max() == 9
+    # ratio 1 (isotropic): the Q1 Laplacian annihilates linear functions in the interior
+    A1 = problems.anisotropic_9pt(10, ratio=1.0)
+    x = np.tile(np.arange(1, 11, dtype=float), 10)        # linear in x
+    r = (A1 @ x).reshape(10, 10, order="F")
+    assert np.abs(r[1:-1, 1:-1]).max() < 1e-9
 
 
 def test_make_inputs_shapes_follow_lanczos_create():
