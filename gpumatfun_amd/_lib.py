@@ -28,7 +28,10 @@ EXPORTS = (
     "gm_profile_spmm", "gm_profile_copy",
     "gm_batch_last_error", "gm_batch_create", "gm_batch_delete", "gm_batch_step",
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
+    "gm_fab_last_error", "gm_fab_create", "gm_fab_delete", "gm_fab_set_matrix",
+    "gm_fab_set_weights", "gm_fab_forward", "gm_fab_backward",
 )
+GM_FAB_RECOMPUTE = 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -98,6 +101,25 @@ class GmBatchDesc(C.Structure):
     ]
 
 
+class GmFabDesc(C.Structure):
+    """Mirror of ``gm_fab_desc``."""
+    _fields_ = [
+        ("n", C.c_int32),
+        ("sparse_format", C.c_int32),
+        ("nnz", C.c_int64),
+        ("A_values", _dp),
+        ("A_ptr", _ip),
+        ("A_ind", _ip),
+        ("subspace_size", C.c_int32),
+        ("contour_size", C.c_int32),
+        ("xi", _dp),
+        ("wts", _dp),
+        ("dwts", _dp),
+        ("flags", C.c_uint32),
+        ("device", C.c_int32),
+    ]
+
+
 class GpuMatfunError(RuntimeError):
     """An entry point returned a non-OK gm_status."""
 
@@ -152,6 +174,13 @@ def load():
         "gm_batch_synchronize": (C.c_int, [vp]),
         "gm_batch_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "gm_batch_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
+        "gm_fab_last_error": (C.c_char_p, []),
+        "gm_fab_create": (C.c_int, [C.POINTER(GmFabDesc), C.POINTER(vp)]),
+        "gm_fab_delete": (C.c_int, [vp]),
+        "gm_fab_set_matrix": (C.c_int, [vp, _dp]),
+        "gm_fab_set_weights": (C.c_int, [vp, _dp, _dp]),
+        "gm_fab_forward": (C.c_int, [vp, _dp, _dp]),
+        "gm_fab_backward": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -161,8 +190,13 @@ def load():
     return lib
 
 
-def check(status, batch=False):
+def check(status, batch=False, fab=False):
     if status != GM_OK:
         lib = load()
-        msg = (lib.gm_batch_last_error() if batch else lib.gm_last_error()).decode("utf-8", "replace")
-        raise GpuMatfunError(status, msg)
+        if fab:
+            raw = lib.gm_fab_last_error()
+        elif batch:
+            raw = lib.gm_batch_last_error()
+        else:
+            raw = lib.gm_last_error()
+        raise GpuMatfunError(status, raw.decode("utf-8", "replace"))

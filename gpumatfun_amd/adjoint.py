@@ -1,0 +1,183 @@
+"""Differentiable f(A)b: ``FabEngine`` (C ABI ``gm_fab_*``) and a ``torch.autograd.Function``.
+
+u = f(A(kappa)) b with f(x) = exp(-tau x^(alpha/2)), evaluated by Lanczos and
+the contour rule of ``problem_startup.m:95-103`` on a fixed contour; the
+backward pass returns dL/db, dL/dkappa and dL/dalpha for an upstream
+gradient dL/du (see ``include/gpumatfun.h``).  A(kappa) is the
+variable-coefficient Laplacian of ``problems.varcoef_laplacian_2d``.
+"""
+
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from . import problems, quadrature as q
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _cplx(z):
+    z = np.asarray(z, dtype=np.complex128).ravel()
+    out = np.empty(2 * z.size)
+    out[0::2] = z.real
+    out[1::2] = z.imag
+    return out
+
+
+def contour_weights(xi, dxidt, s, tau, alpha):
+    """c_j = s f(xi_j) dxi/dt_j and dc_j/dalpha for f = exp(-tau x^(alpha/2))."""
+    xi = np.asarray(xi, dtype=np.complex128)
+    p = np.power(xi, alpha / 2.0)
+    c = s * np.exp(-tau * p) * dxidt
+    dc = c * (-tau * p * np.log(xi) / 2.0)
+    return c, dc
+
+
+class FabEngine:
+    """Device-resident u = f(A) b with its adjoint (one matrix, one right-hand side)."""
+
+    def __init__(self, A, *, m, xi, wts, dwts=None, recompute=False, device=0):
+        lib = _lib.load()
+        self._lib = lib
+        self._h = C.c_void_p()
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        self.n = A.shape[0]
+        self.nnz = A.nnz
+        self._ptr = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        self._ind = np.ascontiguousarray(A.indices, dtype=np.int32)
+        val = np.ascontiguousarray(A.data, dtype=np.float64)
+        xi_i = _cplx(xi)
+        w_i = _cplx(wts)
+        dw_i = None if dwts is None else _cplx(dwts)
+        d = _lib.GmFabDesc()
+        d.n = self.n
+        d.sparse_format = _lib.GM_SPARSE_CSR
+        d.nnz = self.nnz
+        d.A_values = _p(val)
+        d.A_ptr = self._ptr.ctypes.data_as(C.POINTER(C.c_int32))
+        d.A_ind = self._ind.ctypes.data_as(C.POINTER(C.c_int32))
+        d.subspace_size = int(m)
+        d.contour_size = int(np.size(xi))
+        d.xi = _p(xi_i)
+        d.wts = _p(w_i)
+        d.dwts = _p(dw_i)
+        d.flags = _lib.GM_FAB_RECOMPUTE if recompute else 0
+        d.device = int(device)
+        _lib.check(lib.gm_fab_create(C.byref(d), C.byref(self._h)), fab=True)
+        self.ns = int(np.size(xi))
+
+    def _call(self, fn, *args):
+        if not self._h:
+            raise _lib.GpuMatfunError(_lib.GM_EBADHANDLE, "engine has been deleted")
+        _lib.check(fn(self._h, *args), fab=True)
+
+    def set_matrix(self, A):
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        if A.nnz != self.nnz or not np.array_equal(A.indptr, self._ptr):
+            raise ValueError("set_matrix needs the same sparsity pattern")
+        self._call(self._lib.gm_fab_set_matrix, _p(np.ascontiguousarray(A.data, dtype=np.float64)))
+
+    def set_weights(self, wts, dwts=None):
+        self._call(self._lib.gm_fab_set_weights, _p(_cplx(wts)),
+                   None if dwts is None else _p(_cplx(dwts)))
+
+    def forward(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        u = np.empty(self.n)
+        self._call(self._lib.gm_fab_forward, _p(b), _p(u))
+        return u
+
+    def backward(self, g, kappa=None, want_b=True, want_kappa=True, want_alpha=True):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        gb = np.empty(self.n) if want_b else None
+        gk = np.empty(self.n) if want_kappa else None
+        ga = np.zeros(1) if want_alpha else None
+        kap = None if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
+        self._call(self._lib.gm_fab_backward, _p(g), _p(kap), _p(gb), _p(gk), _p(ga))
+        return gb, gk, (None if ga is None else float(ga[0]))
+
+    def delete(self):
+        if self._h:
+            _lib.check(self._lib.gm_fab_delete(self._h), fab=True)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self._lib.gm_fab_delete(self._h)
+        except Exception:
+            pass
+
+
+class FractionalDiffusion:
+    """f(A(kappa)) b on an n x n grid with a fixed contour on [l1, lN].
+
+    ``apply(b, kappa, alpha)`` (torch tensors, float64) is differentiable in
+    all three arguments through :class:`FabFunction`.
+    """
+
+    def __init__(self, n, *, tau, l1, lN, ns=32, m=200, recompute=False, device=0):
+        self.n_side = n
+        self.tau = float(tau)
+        c = q.contour_nodes(l1, lN, ns, tau, 1.0)
+        self.xi, self.dxidt, self.s = c["xi"], c["dxidt"], c["s"]
+        self.m = m
+        self.recompute = recompute
+        self.device = device
+        self.engine = None
+
+    def weights(self, alpha):
+        return contour_weights(self.xi, self.dxidt, self.s, self.tau, alpha)
+
+    def matrix(self, kappa):
+        return problems.varcoef_laplacian_2d(kappa, self.n_side)
+
+    def ensure(self, A, alpha):
+        c, dc = self.weights(alpha)
+        if self.engine is None:
+            self.engine = FabEngine(A, m=self.m, xi=self.xi, wts=c, dwts=dc,
+                                    recompute=self.recompute, device=self.device)
+        else:
+            self.engine.set_matrix(A)
+            self.engine.set_weights(c, dc)
+        return self.engine
+
+    def apply(self, b, kappa, alpha):
+        return FabFunction.apply(b, kappa, alpha, self)
+
+
+try:
+    import torch
+
+    class FabFunction(torch.autograd.Function):
+        """u = f(A(kappa)) b; backward via the C ABI adjoint."""
+
+        @staticmethod
+        def forward(ctx, b, kappa, alpha, op):
+            kap = kappa.detach().cpu().double().numpy()
+            a = float(alpha.detach().cpu())
+            eng = op.ensure(op.matrix(kap), a)
+            u = eng.forward(b.detach().cpu().double().numpy())
+            ctx.op = op
+            ctx.save_for_backward(b, kappa, alpha)
+            return torch.from_numpy(u).to(b.device)
+
+        @staticmethod
+        def backward(ctx, grad_u):
+            b, kappa, alpha = ctx.saved_tensors
+            op = ctx.op
+            kap = kappa.detach().cpu().double().numpy()
+            eng = op.ensure(op.matrix(kap), float(alpha.detach().cpu()))
+            eng.forward(b.detach().cpu().double().numpy())   # state for this (b, kappa, alpha)
+            gb, gk, ga = eng.backward(grad_u.detach().cpu().double().numpy(), kap)
+            dev = b.device
+            return (torch.from_numpy(gb).to(dev), torch.from_numpy(gk).to(dev),
+                    torch.tensor(ga, dtype=torch.float64, device=dev), None)
+except ImportError:   # pragma: no cover - torch is part of the image
+    FabFunction = None

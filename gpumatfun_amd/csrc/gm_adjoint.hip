@@ -1,0 +1,617 @@
+// gm_adjoint.hip -- f(A)b with its adjoint (no reference counterpart).
+//
+// Forward: u = f(A) b by Lanczos on A (m steps, no preconditioner, no
+// deflation) and the contour rule u = Im sum_j c_j x_j, x_j = (xi_j - A)^{-1} b
+// ~= V ytilde_j, ytilde_j = beta0 (xi_j - T)^{-1} e1, c_j = s f(xi_j) dxi/dt_j.
+//
+// Backward, for an upstream gradient g = dL/du (A symmetric, so
+// (xi - A)^T = xi - A and no conjugate appears):
+//   dL/db       = f(A) g                      (the same machinery on g)
+//   dL/dkappa_i = Im sum_j c_j z_j^T (dA/dkappa_i) x_j,  z_j = (xi_j - A)^{-1} g
+//   dL/dalpha   = Im sum_j (dc_j/dalpha) g^T x_j
+// with dA/dkappa of the variable-coefficient model (edge conductance
+// (kappa_i + kappa_k)/2, boundary terms linear in kappa_i):
+//   z^T dA/dkappa_i x = sum_{k != i} w_ik (z_i - z_k)(x_i - x_k) + s_i z_i x_i,
+//   w_ik = -A_ik / (kappa_i + kappa_k),  s_i = (sum_k A_ik) / kappa_i.
+// The blocks X = V_b Ytilde_b and Z = V_g Ytilde_g (N x 2 n_s real columns)
+// are the one dense contraction here and run on FP64 MFMA
+// (v_mfma_f64_16x16x4f64).  With GM_FAB_RECOMPUTE the forward keeps only the
+// tridiagonal data and the backward rebuilds V_b (bit-identical: the
+// reductions are deterministic) instead of holding it between the passes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/gpumatfun.h"
+#include "gm_bkernels.h"
+
+namespace gma {
+using namespace gm;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// X[i][j] = sum_k V[k][i] Y[k][j], V column-major N x m (slices of N), Y row-major
+// m x ncol (ncol a multiple of 64), X row-major N x ncol.  One wavefront per
+// 16-row tile and 64 columns (4 MFMA 16x16 output tiles); K stepped by 4.
+// MFMA f64 16x16x4 operand layout: A lane l = (row l%16, k l/16), B lane l =
+// (k l/16, col l%16), D lane l holds rows 4(l/16)..4(l/16)+3 of column l%16.
+__global__ __launch_bounds__(256) void k_vy_mfma(const double* __restrict__ V, int N, int m,
+                                                 const double* __restrict__ Y, int ncol,
+                                                 double* __restrict__ X) {
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int i0 = tile * 16;
+  const int ct = blockIdx.y;                 // 64-column group
+  if (i0 >= N) return;
+  d4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+  const int ia = i0 + (lane & 15);
+  for (int k0 = 0; k0 < m; k0 += 4) {
+    const int k = k0 + (lane >> 4);
+    const double a = (k < m && ia < N) ? V[(size_t)k * N + ia] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int j = ct * 64 + t * 16 + (lane & 15);
+      const double bv = (k < m) ? Y[(size_t)k * ncol + j] : 0.0;
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = ct * 64 + t * 16 + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + 4 * (lane >> 4) + r;
+      if (i < N) X[(size_t)i * ncol + j] = acc[t][r];
+    }
+  }
+}
+
+// Ytilde in GEMM layout: Yg[k][2s + 0/1] = Re/Im ytilde_s[k]; padding columns 0.
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_pack_y(const double* yr, const double* yi, int m, int ns,
+                                                int ncol, double* Yg) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= m * ncol) return;
+  const int k = t / ncol, j = t - k * ncol;
+  const int s = j >> 1;
+  double v = 0.0;
+  if (s < ns) v = (j & 1) ? yi[(size_t)s * m + k] : yr[(size_t)s * m + k];
+  Yg[t] = v;
+}
+
+// dL/dkappa_i = Im sum_j c_j [sum_k w_ik (z_i - z_k)(x_i - x_k) + s_i z_i x_i]
+struct KGrad {
+  const int* rowptr;
+  const int* col;
+  const double* val;
+  const double* kappa;
+  const double* X;     // [N][ncol] (re, im interleaved per node)
+  const double* Z;
+  const double* c;     // ns complex
+  int N, ns, ncol;
+  double* grad;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_kappa_grad(KGrad p) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.N) return;
+  const double ki = p.kappa[i];
+  const double* xi = p.X + (size_t)i * p.ncol;
+  const double* zi = p.Z + (size_t)i * p.ncol;
+  double rowsum = 0.0;
+  // accumulate T_j = sum_k w_ik (z_i - z_k)(x_i - x_k) + s_i z_i x_i per node j,
+  // then grad = Im sum_j c_j T_j, in fixed order
+  double g = 0.0;
+  for (int e = p.rowptr[i]; e < p.rowptr[i + 1]; ++e) rowsum += p.val[e];
+  const double si = rowsum / ki;
+  for (int s = 0; s < p.ns; ++s) {
+    const double xr = xi[2 * s], xim = xi[2 * s + 1];
+    const double zr = zi[2 * s], zim = zi[2 * s + 1];
+    // self term s_i z x
+    double tr = si * (zr * xr - zim * xim);
+    double tim = si * (zr * xim + zim * xr);
+    for (int e = p.rowptr[i]; e < p.rowptr[i + 1]; ++e) {
+      const int k = p.col[e];
+      if (k == i) continue;
+      const double w = -p.val[e] / (ki + p.kappa[k]);
+      const double* xk = p.X + (size_t)k * p.ncol;
+      const double* zk = p.Z + (size_t)k * p.ncol;
+      const double dxr = xr - xk[2 * s], dxi = xim - xk[2 * s + 1];
+      const double dzr = zr - zk[2 * s], dzi = zim - zk[2 * s + 1];
+      tr = fma(w, dzr * dxr - dzi * dxi, tr);
+      tim = fma(w, dzr * dxi + dzi * dxr, tim);
+    }
+    const double cr = p.c[2 * s], ci = p.c[2 * s + 1];
+    g += cr * tim + ci * tr;      // Im(c * T)
+  }
+  p.grad[i] = g;
+}
+
+// Block partials of g^T X_j (2 ns real values per block), [G][2ns].
+struct GX {
+  const double* g;
+  const double* X;
+  int N, ncol, nv;   // nv = 2 ns
+  double* partials;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_gx(GX p) {
+  __shared__ double sh[256];
+  const int per = (p.N + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * per, r1 = min(p.N, r0 + per);
+  for (int v = 0; v < p.nv; ++v) {
+    double s = 0.0;
+    for (int i = r0 + threadIdx.x; i < r1; i += 256) s = fma(p.g[i], p.X[(size_t)i * p.ncol + v], s);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) p.partials[(size_t)blockIdx.x * p.nv + v] = sh[0];
+    __syncthreads();
+  }
+}
+
+}  // namespace gma
+
+namespace {
+
+thread_local std::string g_aerr;
+std::mutex g_amu;
+std::unordered_set<const void*> g_areg;
+
+gm_status afail(gm_status st, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_aerr = buf;
+  return st;
+}
+
+#define A_HIP(call)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return afail(e_ == hipErrorOutOfMemory ? GM_ENOMEM : GM_EDEVICE, "%s: %s (%s:%d)", \
+                   #call, hipGetErrorString(e_), __FILE__, __LINE__);                    \
+  } while (0)
+#define A_TRY(x)              \
+  do {                        \
+    gm_status s_ = (x);       \
+    if (s_ != GM_OK) return s_; \
+  } while (0)
+
+inline int adiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// One Krylov side (b or g): basis, tridiagonal data, ytilde.
+struct Side {
+  double *U = nullptr, *V = nullptr, *W0 = nullptr, *W1 = nullptr, *al = nullptr, *be = nullptr,
+         *yr = nullptr, *yi = nullptr, *Yg = nullptr, *Xc = nullptr, *out = nullptr;
+};
+
+}  // namespace
+
+struct gm_fab {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int n = 0, m = 0, ns = 0, ncol = 0;
+  long long nnz = 0;
+  bool recompute = false, have_forward = false;
+  int G = 1, chunk = 1;
+  int *rowptr = nullptr, *col = nullptr;
+  double *val = nullptr, *xi = nullptr, *ones = nullptr, *c = nullptr, *gsr = nullptr,
+         *gsi = nullptr, *C = nullptr, *partials = nullptr, *kappa = nullptr, *grad = nullptr,
+         *gxv = nullptr;
+  double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
+  unsigned* status = nullptr;
+  std::vector<double> dc;   // ns complex dc/dalpha (host)
+  Side sb, sg;
+};
+
+namespace {
+
+bool alive(const gm_fab* f) {
+  std::lock_guard<std::mutex> lk(g_amu);
+  return f && g_areg.count(f);
+}
+
+template <typename T>
+gm_status aalloc(T** p, size_t count) {
+  *p = nullptr;
+  A_HIP(hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)));
+  return GM_OK;
+}
+
+void afree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+void free_fab(gm_fab* f) {
+  void* bufs[] = {f->rowptr, f->col, f->val, f->xi, f->ones, f->c, f->gsr, f->gsi, f->C,
+                  f->partials, f->kappa, f->grad, f->gxv, f->wD, f->wDU, f->wDU2, f->wX,
+                  f->status};
+  for (void* p : bufs) afree(p);
+  for (Side* s : {&f->sb, &f->sg}) {
+    void* sbufs[] = {s->U, s->V, s->W0, s->W1, s->al, s->be, s->yr, s->yi, s->Yg, s->Xc, s->out};
+    for (void* p : sbufs) afree(p);
+  }
+  if (f->stream) (void)hipStreamDestroy(f->stream);
+}
+
+gm_status areduce(gm_fab* f, int G, int nv, double* out, int do_sqrt, int flag_zero) {
+  gm::ReduceArgs r{};
+  r.partials = f->partials;
+  r.G = G;
+  r.nv = nv;
+  r.out = out;
+  r.stride = 1;
+  r.do_sqrt = do_sqrt;
+  r.flag_zero = flag_zero;
+  r.status = f->status;
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kBlock), 0, f->stream, r);
+  A_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+// Lanczos on side.U, shifted solves with unit weights (ytilde), and
+// side.out = V * Im(sum_j c_j ytilde_j) = f(A) rhs.
+gm_status krylov(gm_fab* f, Side& s, bool want_out) {
+  const size_t N = (size_t)f->n;
+  gmb::BNorm nn{};
+  nn.u = s.U;
+  nn.ldu = 1;
+  nn.be = s.be;
+  nn.nrows = f->n;
+  nn.chunk = f->chunk;
+  nn.partials = f->partials;
+  nn.status = f->status;
+  hipLaunchKernelGGL((gmb::k_bnorm<1>), dim3(f->G), dim3(gm::kBlock), 0, f->stream, nn);
+  A_HIP(hipGetLastError());
+  A_TRY(areduce(f, f->G, 1, s.be, 1, 0));
+  gmb::BSpmm base{};
+  base.rowptr = f->rowptr;
+  base.col = f->col;
+  base.val = f->val;
+  base.nrows = f->n;
+  base.chunk = f->chunk;
+  base.al = s.al;
+  base.be = s.be;
+  base.partials = f->partials;
+  base.g0 = 1.0;
+  double* Wb[2] = {s.W0, s.W1};
+  for (int k = 0; k < f->m; ++k) {
+    gmb::BSpmm a = base;
+    a.k = k;
+    a.x = k == 0 ? s.U : Wb[(k - 1) & 1];
+    a.ldx = 1;
+    a.y = Wb[k & 1];
+    a.ldy = 1;
+    a.vk = s.V + (size_t)k * N;
+    a.vkm1 = k > 0 ? s.V + (size_t)(k - 1) * N : nullptr;
+    hipLaunchKernelGGL((gmb::k_bspmm<1, gmb::kOnly>), dim3(f->G), dim3(gm::kBlock), 0, f->stream, a);
+    A_HIP(hipGetLastError());
+    A_TRY(areduce(f, f->G, 1, s.al + k, 0, 0));
+    gmb::BFinish fn{};
+    fn.w = Wb[k & 1];
+    fn.vk = a.vk;
+    fn.al = s.al;
+    fn.be = s.be;
+    fn.k = k;
+    fn.nrows = f->n;
+    fn.chunk = f->chunk;
+    fn.partials = f->partials;
+    fn.status = f->status;
+    hipLaunchKernelGGL((gmb::k_bfinish<1>), dim3(f->G), dim3(gm::kBlock), 0, f->stream, fn);
+    A_HIP(hipGetLastError());
+    A_TRY(areduce(f, f->G, 1, s.be + k + 1, 1, 1));
+  }
+  gmb::BSolve sv{};
+  sv.al = s.al;
+  sv.be = s.be;
+  sv.R = 1;
+  sv.m = f->m;
+  sv.ns = f->ns;
+  sv.wts1 = f->ones;
+  sv.wts2 = f->ones;
+  sv.etaxi = f->xi;
+  sv.wD = f->wD;
+  sv.wDU = f->wDU;
+  sv.wDU2 = f->wDU2;
+  sv.wX = f->wX;
+  sv.yr = s.yr;
+  sv.yi = s.yi;
+  sv.status = f->status;
+  hipLaunchKernelGGL(gmb::k_bsolves<0>, dim3(adiv(f->ns, 64)), dim3(64), 0, f->stream, sv);
+  A_HIP(hipGetLastError());
+  if (want_out) {
+    gmb::BCollapse cc{};
+    cc.yr = s.yr;
+    cc.yi = s.yi;
+    cc.gsr = f->gsr;
+    cc.gsi = f->gsi;
+    cc.R = 1;
+    cc.m = f->m;
+    cc.ns = f->ns;
+    cc.nd = 1;
+    cc.C = f->C;
+    hipLaunchKernelGGL(gmb::k_bcollapse<0>, dim3(adiv(f->m, 256)), dim3(256), 0, f->stream, cc);
+    A_HIP(hipGetLastError());
+    gmb::BVC v{};
+    v.V = s.V;
+    v.C = f->C;
+    v.R = 1;
+    v.m = f->m;
+    v.nd = 1;
+    v.nrows = f->n;
+    v.Z = s.out;
+    v.ldz = 1;
+    v.accumulate = 0;
+    hipLaunchKernelGGL((gmb::k_bvc<1>), dim3(adiv(f->n, 256)), dim3(256), 0, f->stream, v);
+    A_HIP(hipGetLastError());
+  }
+  return GM_OK;
+}
+
+// side.Xc = V * Ytilde (N x ncol) on FP64 MFMA.
+gm_status resolvent_block(gm_fab* f, Side& s) {
+  hipLaunchKernelGGL(gma::k_pack_y<0>, dim3(adiv((long long)f->m * f->ncol, 256)), dim3(256), 0,
+                     f->stream, s.yr, s.yi, f->m, f->ns, f->ncol, s.Yg);
+  A_HIP(hipGetLastError());
+  dim3 grid(adiv(adiv(f->n, 16), 4), f->ncol / 64);
+  hipLaunchKernelGGL(gma::k_vy_mfma, grid, dim3(256), 0, f->stream, s.V, f->n, f->m, s.Yg,
+                     f->ncol, s.Xc);
+  A_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+gm_status alloc_side(gm_fab* f, Side& s) {
+  const size_t N = (size_t)f->n;
+  A_TRY(aalloc(&s.U, N));
+  A_TRY(aalloc(&s.V, N * f->m));
+  A_TRY(aalloc(&s.W0, N));
+  A_TRY(aalloc(&s.W1, N));
+  A_TRY(aalloc(&s.al, (size_t)f->m));
+  A_TRY(aalloc(&s.be, (size_t)f->m + 1));
+  A_TRY(aalloc(&s.yr, (size_t)f->m * f->ns));
+  A_TRY(aalloc(&s.yi, (size_t)f->m * f->ns));
+  A_TRY(aalloc(&s.Yg, (size_t)f->m * f->ncol));
+  A_TRY(aalloc(&s.Xc, N * f->ncol));
+  A_TRY(aalloc(&s.out, N));
+  return GM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gm_fab_last_error(void) { return g_aerr.c_str(); }
+
+gm_status gm_fab_create(const gm_fab_desc* d, gm_fab** out) {
+  g_aerr.clear();
+  if (!d || !out) return afail(GM_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->n <= 0 || !d->A_ptr || !d->A_ind || !d->A_values || d->subspace_size < 1 ||
+      d->contour_size < 1 || !d->xi || !d->wts)
+    return afail(GM_EINVAL, "bad sizes or null arrays");
+  if (d->sparse_format != GM_SPARSE_CSR)
+    return afail(GM_EINVAL, "the adjoint needs A in CSR with its diagonal stored");
+  if (d->A_ptr[0] != 0 || d->A_ptr[d->n] != d->nnz) return afail(GM_EINVAL, "bad pointer array");
+  for (int i = 0; i < d->n; ++i)
+    if (d->A_ptr[i + 1] < d->A_ptr[i]) return afail(GM_EINVAL, "pointer array not monotone");
+  for (long long j = 0; j < d->nnz; ++j)
+    if (d->A_ind[j] < 0 || d->A_ind[j] >= d->n) return afail(GM_EINVAL, "index out of range");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= d->device || d->device < 0)
+    return afail(GM_EDEVICE, "no HIP device %d", d->device);
+  gm_fab* f = new gm_fab();
+  auto bail = [&](gm_status st) {
+    std::string keep = g_aerr;
+    free_fab(f);
+    delete f;
+    g_aerr = keep;
+    return st;
+  };
+#define A_CREATE(x)                  \
+  do {                               \
+    gm_status s_ = (x);              \
+    if (s_ != GM_OK) return bail(s_); \
+  } while (0)
+  f->dev = d->device;
+  if (hipSetDevice(f->dev) != hipSuccess || hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(afail(GM_EDEVICE, "device/stream"));
+  f->n = d->n;
+  f->nnz = d->nnz;
+  f->m = d->subspace_size;
+  f->ns = d->contour_size;
+  f->ncol = adiv(2 * f->ns, 64) * 64;
+  f->recompute = (d->flags & GM_FAB_RECOMPUTE) != 0;
+  f->G = std::max(1, std::min(4096, adiv(f->n, 256)));
+  f->chunk = adiv(f->n, f->G);
+  hipStream_t st = f->stream;
+  auto up = [&](auto** p, const auto* src, size_t cnt) -> gm_status {
+    A_TRY(aalloc(p, cnt));
+    A_HIP(hipMemcpyAsync(*p, src, cnt * sizeof(**p), hipMemcpyHostToDevice, st));
+    return GM_OK;
+  };
+  A_CREATE(up(&f->rowptr, d->A_ptr, (size_t)f->n + 1));
+  A_CREATE(up(&f->col, d->A_ind, (size_t)f->nnz));
+  A_CREATE(up(&f->val, d->A_values, (size_t)f->nnz));
+  A_CREATE(up(&f->xi, d->xi, 2 * (size_t)f->ns));
+  std::vector<double> ones(2 * (size_t)f->ns, 0.0), gr(f->ns), gi(f->ns);
+  for (int s = 0; s < f->ns; ++s) {
+    ones[s] = 1.0;            // wts layout ns x 2 [Re Im]: Re = 1, Im = 0
+    gr[s] = d->wts[2 * s];
+    gi[s] = d->wts[2 * s + 1];
+  }
+  A_CREATE(up(&f->ones, ones.data(), ones.size()));
+  A_CREATE(up(&f->c, d->wts, 2 * (size_t)f->ns));
+  A_CREATE(up(&f->gsr, gr.data(), gr.size()));
+  A_CREATE(up(&f->gsi, gi.data(), gi.size()));
+  f->dc.assign(2 * (size_t)f->ns, 0.0);
+  if (d->dwts) std::copy(d->dwts, d->dwts + 2 * f->ns, f->dc.begin());
+  A_CREATE(aalloc(&f->C, (size_t)f->m));
+  A_CREATE(aalloc(&f->partials, (size_t)4096 * std::max(2, f->ncol)));
+  A_CREATE(aalloc(&f->kappa, (size_t)f->n));
+  A_CREATE(aalloc(&f->grad, (size_t)f->n));
+  A_CREATE(aalloc(&f->gxv, (size_t)f->ncol));
+  const size_t ny = (size_t)f->m * f->ns;
+  A_CREATE(aalloc(&f->wD, ny));
+  A_CREATE(aalloc(&f->wDU, ny));
+  A_CREATE(aalloc(&f->wDU2, ny));
+  A_CREATE(aalloc(&f->wX, ny));
+  A_CREATE(aalloc(&f->status, 1));
+  if (hipMemsetAsync(f->status, 0, sizeof(unsigned), st) != hipSuccess)
+    return bail(afail(GM_EDEVICE, "memset"));
+  A_CREATE(alloc_side(f, f->sb));
+  A_CREATE(alloc_side(f, f->sg));
+  if (hipStreamSynchronize(st) != hipSuccess) return bail(afail(GM_EDEVICE, "sync"));
+  {
+    std::lock_guard<std::mutex> lk(g_amu);
+    g_areg.insert(f);
+  }
+  *out = f;
+  return GM_OK;
+#undef A_CREATE
+}
+
+gm_status gm_fab_delete(gm_fab* f) {
+  {
+    std::lock_guard<std::mutex> lk(g_amu);
+    if (!f || !g_areg.count(f)) return afail(GM_EBADHANDLE, "invalid fab handle");
+    g_areg.erase(f);
+  }
+  (void)hipSetDevice(f->dev);
+  if (f->stream) (void)hipStreamSynchronize(f->stream);
+  free_fab(f);
+  delete f;
+  return GM_OK;
+}
+
+gm_status gm_fab_set_matrix(gm_fab* f, const double* A_values) {
+  if (!alive(f)) return afail(GM_EBADHANDLE, "invalid fab handle");
+  if (!A_values) return afail(GM_EINVAL, "null values");
+  A_HIP(hipSetDevice(f->dev));
+  A_HIP(hipMemcpyAsync(f->val, A_values, (size_t)f->nnz * sizeof(double), hipMemcpyHostToDevice,
+                       f->stream));
+  A_HIP(hipStreamSynchronize(f->stream));
+  f->have_forward = false;
+  return GM_OK;
+}
+
+gm_status gm_fab_set_weights(gm_fab* f, const double* wts, const double* dwts) {
+  if (!alive(f)) return afail(GM_EBADHANDLE, "invalid fab handle");
+  if (!wts) return afail(GM_EINVAL, "null weights");
+  A_HIP(hipSetDevice(f->dev));
+  std::vector<double> gr(f->ns), gi(f->ns);
+  for (int s = 0; s < f->ns; ++s) {
+    gr[s] = wts[2 * s];
+    gi[s] = wts[2 * s + 1];
+  }
+  A_HIP(hipMemcpyAsync(f->c, wts, 2 * (size_t)f->ns * sizeof(double), hipMemcpyHostToDevice, f->stream));
+  A_HIP(hipMemcpyAsync(f->gsr, gr.data(), f->ns * sizeof(double), hipMemcpyHostToDevice, f->stream));
+  A_HIP(hipMemcpyAsync(f->gsi, gi.data(), f->ns * sizeof(double), hipMemcpyHostToDevice, f->stream));
+  A_HIP(hipStreamSynchronize(f->stream));
+  if (dwts) std::copy(dwts, dwts + 2 * f->ns, f->dc.begin());
+  f->have_forward = false;
+  return GM_OK;
+}
+
+gm_status gm_fab_forward(gm_fab* f, const double* b, double* u) {
+  if (!alive(f)) return afail(GM_EBADHANDLE, "invalid fab handle");
+  if (!b || !u) return afail(GM_EINVAL, "null vector");
+  A_HIP(hipSetDevice(f->dev));
+  const size_t N = (size_t)f->n;
+  A_HIP(hipMemcpyAsync(f->sb.U, b, N * sizeof(double), hipMemcpyHostToDevice, f->stream));
+  A_TRY(krylov(f, f->sb, true));
+  A_HIP(hipMemcpyAsync(u, f->sb.out, N * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+  A_HIP(hipStreamSynchronize(f->stream));
+  f->have_forward = true;
+  unsigned stv = 0;
+  A_HIP(hipMemcpy(&stv, f->status, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (stv & 6u) return afail(GM_EBREAKDOWN, "Lanczos breakdown in the forward pass (%u)", stv);
+  return GM_OK;
+}
+
+gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, double* grad_b,
+                          double* grad_kappa, double* grad_alpha) {
+  if (!alive(f)) return afail(GM_EBADHANDLE, "invalid fab handle");
+  if (!f->have_forward) return afail(GM_ESTATE, "backward needs a forward pass first");
+  if (!g) return afail(GM_EINVAL, "null upstream gradient");
+  if (grad_kappa && !kappa) return afail(GM_EINVAL, "kappa is needed for the kappa gradient");
+  A_HIP(hipSetDevice(f->dev));
+  const size_t N = (size_t)f->n;
+  hipStream_t st = f->stream;
+  A_HIP(hipMemcpyAsync(f->sg.U, g, N * sizeof(double), hipMemcpyHostToDevice, st));
+  // z side: f(A) g is dL/db
+  A_TRY(krylov(f, f->sg, grad_b != nullptr));
+  if (grad_b)
+    A_HIP(hipMemcpyAsync(grad_b, f->sg.out, N * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (grad_kappa || grad_alpha) {
+    if (f->recompute) A_TRY(krylov(f, f->sb, false));   // rebuild V_b (deterministic)
+    A_TRY(resolvent_block(f, f->sb));
+  }
+  if (grad_kappa) {
+    A_TRY(resolvent_block(f, f->sg));
+    A_HIP(hipMemcpyAsync(f->kappa, kappa, N * sizeof(double), hipMemcpyHostToDevice, st));
+    gma::KGrad kg{};
+    kg.rowptr = f->rowptr;
+    kg.col = f->col;
+    kg.val = f->val;
+    kg.kappa = f->kappa;
+    kg.X = f->sb.Xc;
+    kg.Z = f->sg.Xc;
+    kg.c = f->c;
+    kg.N = f->n;
+    kg.ns = f->ns;
+    kg.ncol = f->ncol;
+    kg.grad = f->grad;
+    hipLaunchKernelGGL(gma::k_kappa_grad<0>, dim3(adiv(f->n, 256)), dim3(256), 0, st, kg);
+    A_HIP(hipGetLastError());
+    A_HIP(hipMemcpyAsync(grad_kappa, f->grad, N * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  if (grad_alpha) {
+    const int G = std::max(1, std::min(1024, adiv(f->n, 256)));
+    gma::GX gx{};
+    gx.g = f->sg.U;
+    gx.X = f->sb.Xc;
+    gx.N = f->n;
+    gx.ncol = f->ncol;
+    gx.nv = 2 * f->ns;
+    gx.partials = f->partials;
+    hipLaunchKernelGGL(gma::k_gx<0>, dim3(G), dim3(256), 0, st, gx);
+    A_HIP(hipGetLastError());
+    A_TRY(areduce(f, G, 2 * f->ns, f->gxv, 0, 0));
+    std::vector<double> gxv(2 * (size_t)f->ns);
+    A_HIP(hipMemcpyAsync(gxv.data(), f->gxv, gxv.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    A_HIP(hipStreamSynchronize(st));
+    double ga = 0.0;
+    for (int s = 0; s < f->ns; ++s) {
+      const double dr = f->dc[2 * s], di = f->dc[2 * s + 1];
+      const double xr = gxv[2 * s], xi = gxv[2 * s + 1];
+      ga += dr * xi + di * xr;       // Im(dc * g^T x)
+    }
+    *grad_alpha = ga;
+  }
+  A_HIP(hipStreamSynchronize(st));
+  unsigned stv = 0;
+  A_HIP(hipMemcpy(&stv, f->status, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (stv & 6u) return afail(GM_EBREAKDOWN, "Lanczos breakdown in the backward pass (%u)", stv);
+  return GM_OK;
+}
+
+}  // extern "C"

@@ -30,398 +30,9 @@
 #include <vector>
 
 #include "../../include/gpumatfun.h"
-#include "gm_device.h"
+#include "gm_bkernels.h"
 
-namespace gmb {
-using namespace gm;
 
-constexpr int kMaxND = 17;
-
-enum Mode : int { kFirst = 0, kMid = 1, kLast = 2, kOnly = 3, kHorner = 4 };
-
-struct BSpmm {
-  const int* __restrict__ rowptr;
-  const int* __restrict__ col;
-  const double* __restrict__ val;
-  int nrows, chunk;
-  const double* x;  size_t ldx;     // gather source (+ column offset applied by caller)
-  double* y;        size_t ldy;     // output
-  double* vk;                       // V[k] (ld = R)
-  const double* vkm1;               // V[k-1] or null
-  const double* z;  size_t ldz;     // kHorner addend
-  const double* al;                 // alpha [k][R]
-  double* be;                       // beta  [k][R]
-  int k;
-  double g0, gi;
-  int accumulate;
-  double* alpha_out;                // kLast/kOnly: alpha[k][R]
-  double* partials;                 // [G][R]
-  unsigned* counter;
-};
-
-// Per-column block sums (fixed order) written to partials[block][R]; the grid
-// total is formed by k_reduce after the kernel.
-template <int R>
-__device__ void block_partial_cols(const double (&v)[(R < 64 ? 1 : R / 64)], double* partials) {
-  constexpr int CPW = R < 64 ? R : 64;
-  constexpr int RPW = 64 / CPW;
-  constexpr int CL = R / CPW;
-  __shared__ double sh[kBlock * CL];
-  for (int q = 0; q < CL; ++q) sh[threadIdx.x * CL + q] = v[q];
-  __syncthreads();
-  for (int c = threadIdx.x; c < R; c += kBlock) {
-    const int lanec = c % CPW, q = c / CPW;
-    double s = 0.0;
-    for (int w = 0; w < kWaves; ++w)
-      for (int sl = 0; sl < RPW; ++sl) s += sh[(w * 64 + sl * CPW + lanec) * CL + q];
-    partials[(size_t)blockIdx.x * R + c] = s;
-  }
-}
-
-template <int R, int MODE>
-__global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
-  constexpr int CPW = R < 64 ? R : 64;
-  constexpr int RPW = 64 / CPW;
-  constexpr int CL = R / CPW;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int row0 = b * p.chunk;
-  const int row1 = min(row0 + p.chunk, p.nrows);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int slot = lane / CPW, c0 = lane % CPW;
-  double bet[CL], inv[CL], bprev[CL], dot[CL];
-#pragma unroll
-  for (int q = 0; q < CL; ++q) {
-    const int c = c0 + q * CPW;
-    bet[q] = (MODE == kFirst || MODE == kOnly) ? p.be[(size_t)p.k * R + c] : 0.0;
-    inv[q] = safe_inv(bet[q]) * p.g0;
-    bprev[q] = ((MODE == kLast || MODE == kOnly) && p.k > 0) ? p.be[(size_t)p.k * R + c] : 0.0;
-    dot[q] = 0.0;
-  }
-  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
-    const int j0 = p.rowptr[r], j1 = p.rowptr[r + 1];
-    double acc[CL];
-#pragma unroll
-    for (int q = 0; q < CL; ++q) acc[q] = 0.0;
-    for (int j = j0; j < j1; ++j) {
-      const double a = p.val[j];
-      const double* xr = p.x + (size_t)p.col[j] * p.ldx + c0;
-#pragma unroll
-      for (int q = 0; q < CL; ++q) acc[q] = fma(a, xr[q * CPW], acc[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < CL; ++q) {
-      const int c = c0 + q * CPW;
-      const size_t iy = (size_t)r * p.ldy + c;
-      const size_t iv = (size_t)r * R + c;
-      if (MODE == kFirst) {
-        const double v = safe_div(p.x[(size_t)r * p.ldx + c0 + q * CPW], bet[q]);
-        p.vk[iv] = v;
-        p.y[iy] = fma(p.gi, v, acc[q] * inv[q]);
-      } else if (MODE == kMid) {
-        p.y[iy] = fma(p.gi, p.vk[iv], acc[q]);
-      } else if (MODE == kLast || MODE == kOnly) {
-        double v, yv;
-        if (MODE == kOnly) {
-          v = safe_div(p.x[(size_t)r * p.ldx + c0 + q * CPW], bet[q]);
-          p.vk[iv] = v;
-          yv = acc[q] * inv[q];
-        } else {
-          v = p.vk[iv];
-          yv = acc[q];
-        }
-        if (p.k > 0) yv = fma(-bprev[q], p.vkm1[iv], yv);
-        p.y[iy] = yv;
-        dot[q] = fma(yv, v, dot[q]);
-      } else {
-        double yv = acc[q] + p.z[(size_t)r * p.ldz + c];
-        if (p.accumulate) yv += p.y[iy];
-        p.y[iy] = yv;
-      }
-    }
-  }
-  if (MODE == kLast || MODE == kOnly)
-    block_partial_cols<R>(dot, p.partials);
-}
-
-struct BFinish {
-  double* w;          // ld R
-  const double* vk;
-  const double* al;
-  double* be;
-  int k, nrows, chunk;
-  double* partials;
-  unsigned* counter;
-  unsigned* status;
-};
-
-template <int R>
-__global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
-  constexpr int CPW = R < 64 ? R : 64;
-  constexpr int RPW = 64 / CPW;
-  constexpr int CL = R / CPW;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int row0 = b * p.chunk;
-  const int row1 = min(row0 + p.chunk, p.nrows);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int slot = lane / CPW, c0 = lane % CPW;
-  double al[CL], nrm[CL];
-#pragma unroll
-  for (int q = 0; q < CL; ++q) {
-    al[q] = p.al[(size_t)p.k * R + c0 + q * CPW];
-    nrm[q] = 0.0;
-  }
-  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
-#pragma unroll
-    for (int q = 0; q < CL; ++q) {
-      const size_t i = (size_t)r * R + c0 + q * CPW;
-      const double w = fma(-al[q], p.vk[i], p.w[i]);
-      p.w[i] = w;
-      nrm[q] = fma(w, w, nrm[q]);
-    }
-  }
-  block_partial_cols<R>(nrm, p.partials);
-}
-
-// Column norms of U (ld ldu, column offset applied) -> be[0][c].
-struct BNorm {
-  const double* u;
-  size_t ldu;
-  double* be;
-  int nrows, chunk;
-  double* partials;
-  unsigned* counter;
-  unsigned* status;
-};
-
-template <int R>
-__global__ __launch_bounds__(kBlock) void k_bnorm(BNorm p) {
-  constexpr int CPW = R < 64 ? R : 64;
-  constexpr int RPW = 64 / CPW;
-  constexpr int CL = R / CPW;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int row0 = b * p.chunk;
-  const int row1 = min(row0 + p.chunk, p.nrows);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int slot = lane / CPW, c0 = lane % CPW;
-  double nrm[CL];
-#pragma unroll
-  for (int q = 0; q < CL; ++q) nrm[q] = 0.0;
-  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
-#pragma unroll
-    for (int q = 0; q < CL; ++q) {
-      const double u = p.u[(size_t)r * p.ldu + c0 + q * CPW];
-      nrm[q] = fma(u, u, nrm[q]);
-    }
-  }
-  block_partial_cols<R>(nrm, p.partials);
-}
-
-// (T_c - etaxi_s I) y = -beta0_c w_{f(c),s} e1, one lane per (column, node);
-// column c uses weights wts1 (even c, exp) or wts2 (odd c, phi).  Pivoted like
-// k_shifted_solves; Y is [c][s][k].
-struct BSolve {
-  const double* al;   // [k][R]
-  const double* be;   // [k][R]
-  int R, m, ns;
-  const double* wts1, *wts2;   // ns x 2
-  const double* etaxi;         // 2 ns (re, im)
-  double2 *wD, *wDU, *wDU2, *wX;   // [k][R*ns]
-  double* yr;
-  double* yi;
-  unsigned* status;
-};
-
-__global__ __launch_bounds__(64) void k_bsolves(BSolve p) {
-  const int sys = blockIdx.x * blockDim.x + threadIdx.x;
-  const int S = p.R * p.ns;
-  if (sys >= S) return;
-  const int c = sys / p.ns, s = sys - c * p.ns;
-  const double* w = (c & 1) ? p.wts2 : p.wts1;
-  const double2 wt = d2(w[s], w[s + p.ns]);
-  const double2 ex = d2(p.etaxi[2 * s], p.etaxi[2 * s + 1]);
-  const double b0 = p.be[c];
-  const int m = p.m;
-  auto diag = [&](int k) { return d2(p.al[(size_t)k * p.R + c] - ex.x, -ex.y); };
-  auto off = [&](int k) { return d2(p.be[(size_t)(k + 1) * p.R + c], 0.0); };
-  double2 cur_d = diag(0);
-  double2 cur_x = d2(-(b0 * wt.x), -(b0 * wt.y));
-  double2 cur_du = m > 1 ? off(0) : d2(0.0, 0.0);
-  bool singular = false;
-  for (int k = 0; k < m - 1; ++k) {
-    const double2 dl = off(k);
-    double2 next_d = diag(k + 1), next_x = d2(0.0, 0.0);
-    double2 next_du = (k + 1 < m - 1) ? off(k + 1) : d2(0.0, 0.0);
-    double2 sD, sDU, sDU2 = d2(0.0, 0.0), sX;
-    if (dl.x == 0.0 && dl.y == 0.0) {
-      if (cur_d.x == 0.0 && cur_d.y == 0.0) singular = true;
-      sD = cur_d; sDU = cur_du; sX = cur_x;
-    } else if (cabs1(cur_d) >= cabs1(dl)) {
-      const double2 mult = cdiv(dl, cur_d);
-      next_d = csub(next_d, cmul(mult, cur_du));
-      next_x = csub(next_x, cmul(mult, cur_x));
-      sD = cur_d; sDU = cur_du; sX = cur_x;
-    } else {
-      const double2 mult = cdiv(cur_d, dl);
-      sD = dl;
-      const double2 temp = next_d;
-      next_d = csub(cur_du, cmul(mult, temp));
-      if (k < m - 2) {
-        sDU2 = next_du;
-        next_du = d2(-(mult.x * sDU2.x - mult.y * sDU2.y), -(mult.x * sDU2.y + mult.y * sDU2.x));
-      }
-      sDU = temp;
-      sX = next_x;
-      next_x = csub(cur_x, cmul(mult, next_x));
-    }
-    const size_t o = (size_t)k * S + sys;
-    p.wD[o] = sD; p.wDU[o] = sDU; p.wDU2[o] = sDU2; p.wX[o] = sX;
-    cur_d = next_d; cur_x = next_x; cur_du = next_du;
-  }
-  if (cur_d.x == 0.0 && cur_d.y == 0.0) singular = true;
-  double* yr = p.yr + (size_t)sys * m;
-  double* yi = p.yi + (size_t)sys * m;
-  if (singular) {
-    atomicOr(p.status, 2u);
-    for (int k = 0; k < m; ++k) { yr[k] = 0.0; yi[k] = 0.0; }
-    return;
-  }
-  double2 x2 = cdiv(cur_x, cur_d);
-  yr[m - 1] = x2.x;
-  yi[m - 1] = x2.y;
-  if (m > 1) {
-    const size_t o = (size_t)(m - 2) * S + sys;
-    double2 x1 = cdiv(csub(p.wX[o], cmul(p.wDU[o], x2)), p.wD[o]);
-    yr[m - 2] = x1.x;
-    yi[m - 2] = x1.y;
-    for (int k = m - 3; k >= 0; --k) {
-      const size_t q = (size_t)k * S + sys;
-      double2 t = csub(p.wX[q], cmul(p.wDU[q], x1));
-      t = csub(t, cmul(p.wDU2[q], x2));
-      const double2 xk = cdiv(t, p.wD[q]);
-      yr[k] = xk.x;
-      yi[k] = xk.y;
-      x2 = x1;
-      x1 = xk;
-    }
-  }
-}
-
-// C[j][k][c] = sum_s (gsr_{j,s} Im y_{c,s,k} + gsi_{j,s} Re y_{c,s,k})
-struct BCollapse {
-  const double* yr;
-  const double* yi;
-  const double* gsr;
-  const double* gsi;
-  int R, m, ns, nd;
-  double* C;
-};
-
-__global__ __launch_bounds__(kBlock) void k_bcollapse(BCollapse p) {
-  const long long total = (long long)p.nd * p.m * p.R;
-  for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
-       t += (long long)gridDim.x * kBlock) {
-    const int c = (int)(t % p.R);
-    const long long jk = t / p.R;
-    const int k = (int)(jk % p.m), j = (int)(jk / p.m);
-    double acc = 0.0;
-    for (int s = 0; s < p.ns; ++s) {
-      const size_t e = ((size_t)c * p.ns + s) * p.m + k;
-      acc = fma(p.gsr[j + (size_t)p.nd * s], p.yi[e], acc);
-      acc = fma(p.gsi[j + (size_t)p.nd * s], p.yr[e], acc);
-    }
-    p.C[t] = acc;
-  }
-}
-
-// Z_j[i][c] = sum_k V[k][i][c] C[j][k][c]   (nd == 1: written to X with ld ldx)
-struct BVC {
-  const double* V;
-  const double* C;
-  int R, m, nd, nrows;
-  double* Z;          // [j][N][R] or X
-  size_t ldz;         // R, or the X row stride
-  int accumulate;
-};
-
-template <int ND>
-__global__ __launch_bounds__(kBlock) void k_bvc(BVC p) {
-  const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (e >= (long long)p.nrows * p.R) return;
-  const int c = (int)(e % p.R);
-  const long long i = e / p.R;
-  const size_t slice = (size_t)p.nrows * p.R;
-  double acc[ND];
-#pragma unroll
-  for (int j = 0; j < ND; ++j) acc[j] = 0.0;
-  for (int k = 0; k < p.m; ++k) {
-    const double v = p.V[(size_t)k * slice + e];
-#pragma unroll
-    for (int j = 0; j < ND; ++j) acc[j] = fma(v, p.C[((size_t)j * p.m + k) * p.R + c], acc[j]);
-  }
-  if (ND == 1) {
-    const size_t o = (size_t)i * p.ldz + c;
-    p.Z[o] = p.accumulate ? p.Z[o] + acc[0] : acc[0];
-  } else {
-#pragma unroll
-    for (int j = 0; j < ND; ++j) p.Z[(size_t)j * slice + e] = acc[j];
-  }
-}
-
-// Fisher-KPP update per column pair (update_U.cu:8-16), column norms after.
-struct BUpdate {
-  const double* X;    // ld ldu (chunk columns at offset)
-  double* Usol;       // ld ldu
-  const double* mass;
-  double dt;
-  int clamp, x_zero, prepare;
-  size_t ldu;
-  int R, nrows, chunk;
-  double* be;         // be[0][c]
-  double* partials;
-  unsigned* counter;
-  unsigned* status;
-};
-
-template <int R>
-__global__ __launch_bounds__(kBlock) void k_bupdate(BUpdate p) {
-  constexpr int CPW = R < 64 ? R : 64;
-  constexpr int RPW = 64 / CPW;
-  constexpr int CL = R / CPW;
-  static_assert(R % 2 == 0, "column pairs");
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int row0 = b * p.chunk;
-  const int row1 = min(row0 + p.chunk, p.nrows);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int slot = lane / CPW, c0 = lane % CPW;
-  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
-    const double ms = p.mass[r];
-#pragma unroll
-    for (int q = 0; q < CL; ++q) {
-      const int c = c0 + q * CPW;
-      const size_t i = (size_t)r * p.ldu + c;
-      double val;
-      if (p.prepare) {
-        val = p.Usol[i];
-      } else {
-        // both columns of the pair are read by both lanes of the pair
-        const size_t i0 = i - (c & 1);
-        double u = p.x_zero ? 0.0 : (p.X[i0] + p.X[i0 + 1]);
-        if (p.clamp) u = u * (u >= 0.0 ? 1.0 : 0.0);
-        if (c & 1) {
-          const double ub = u / ms;
-          val = ((ms * ub) * (1.0 - ub)) * p.dt;
-        } else {
-          val = u;
-        }
-      }
-      if (!p.prepare) p.Usol[i] = val;   // X and Usol are distinct buffers
-    }
-  }
-  // the column norms of the next step's starting vectors are taken per chunk
-  // by k_bnorm
-}
-
-}  // namespace gmb
 
 // ====================================================================== host
 namespace {
@@ -647,7 +258,7 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
   s.yr = b->yr;
   s.yi = b->yi;
   s.status = b->status;
-  hipLaunchKernelGGL(gmb::k_bsolves, dim3(cdivi((long long)R * b->ns, 64)), dim3(64), 0, b->stream, s);
+  hipLaunchKernelGGL(gmb::k_bsolves<0>, dim3(cdivi((long long)R * b->ns, 64)), dim3(64), 0, b->stream, s);
   B_HIP(hipGetLastError());
   gmb::BCollapse cc{};
   cc.yr = b->yr;
@@ -659,7 +270,7 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
   cc.ns = b->ns;
   cc.nd = b->nd;
   cc.C = b->C;
-  hipLaunchKernelGGL(gmb::k_bcollapse, dim3(std::min(1024, cdivi((long long)b->nd * b->m * R, gm::kBlock))),
+  hipLaunchKernelGGL(gmb::k_bcollapse<0>, dim3(std::min(1024, cdivi((long long)b->nd * b->m * R, gm::kBlock))),
                      dim3(gm::kBlock), 0, b->stream, cc);
   B_HIP(hipGetLastError());
   gmb::BVC v{};

@@ -226,6 +226,50 @@ gm_status gm_batch_synchronize(gm_batch* b);
 gm_status gm_batch_info(gm_batch* b, int32_t* num_problems, int32_t* chunk_columns);
 gm_status gm_batch_get_stream(gm_batch* b, void** stream);
 
+/* ---------------------------------------------------------------------------
+ * Differentiation (no reference counterpart): u = f(A) b for one f given by
+ * its contour weights c_j = s f(xi_j) dxi/dt_j (the quadrature of
+ * problem_startup.m:95-103, no preconditioner, no deflation), and the
+ * vector-Jacobian product for an upstream gradient g = dL/du:
+ *   dL/db = f(A) g,  dL/dkappa (variable-coefficient model A(kappa) with
+ *   off-diagonals -(kappa_i + kappa_k)/(2h^2) and boundary terms linear in
+ *   kappa_i),  dL/dalpha = Im sum_j (dc_j/dalpha) g^T (xi_j - A)^{-1} b.
+ * ------------------------------------------------------------------------- */
+typedef struct gm_fab gm_fab;
+
+enum { GM_FAB_RECOMPUTE = 1u << 0 };   /* rebuild the forward basis in backward
+                                          instead of keeping it                */
+
+typedef struct gm_fab_desc {
+    int32_t n;
+    int32_t sparse_format;         /* GM_SPARSE_CSR, diagonal stored         */
+    int64_t nnz;
+    const double* A_values;
+    const int32_t* A_ptr;
+    const int32_t* A_ind;
+    int32_t subspace_size;         /* m                                      */
+    int32_t contour_size;          /* n_s                                    */
+    const double* xi;              /* n_s complex nodes, [re im] interleaved */
+    const double* wts;             /* n_s complex c_j, interleaved           */
+    const double* dwts;            /* n_s complex dc_j/dalpha (may be NULL)  */
+    uint32_t flags;                /* GM_FAB_*                               */
+    int32_t device;
+} gm_fab_desc;
+
+const char* gm_fab_last_error(void);
+gm_status gm_fab_create(const gm_fab_desc* desc, gm_fab** out);
+gm_status gm_fab_delete(gm_fab* f);
+/* new values of A (same sparsity), e.g. after kappa changes               */
+gm_status gm_fab_set_matrix(gm_fab* f, const double* A_values);
+/* new weights (and their alpha derivatives), e.g. after alpha changes      */
+gm_status gm_fab_set_weights(gm_fab* f, const double* wts, const double* dwts);
+/* u = f(A) b, host vectors of length n                                     */
+gm_status gm_fab_forward(gm_fab* f, const double* b, double* u);
+/* after a forward: any output pointer may be NULL to skip that gradient;
+ * kappa (n) is needed for grad_kappa                                       */
+gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, double* grad_b,
+                          double* grad_kappa, double* grad_alpha);
+
 #ifdef __cplusplus
 }
 #endif

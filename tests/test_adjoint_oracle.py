@@ -1,0 +1,79 @@
+"""Pin the exact-resolvent adjoint oracle by central finite differences (no GPU)."""
+
+import numpy as np
+import pytest
+
+from oracle import adjoint, contour
+from conftest import rel
+from gpumatfun_amd import problems
+
+TAU, ALPHA = 1e-3, 1.5
+
+
+def setup(n=12, seed=1):
+    kappa = np.exp(0.5 * problems.gaussian_random_field(n, 0.1, seed=seed))
+    A = problems.varcoef_laplacian_2d(kappa, n)
+    w = np.linalg.eigvalsh(A.toarray())
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(n * n)
+    y = rng.standard_normal(n * n)
+    return kappa, A, w, b, y
+
+
+def weights(l1, lN, alpha, ns=40):
+    c = contour.contour_setup(l1, lN, ns, TAU, alpha)
+    xi = c["xi"]
+    cw = c["wts1"]                         # s * f(xi) * dxi/dt
+    dcw = cw * (-TAU * xi ** (alpha / 2) * np.log(xi) / 2)
+    return xi, cw, dcw
+
+
+def loss(A, b, y, xi, cw):
+    u, _ = adjoint.forward(A, b, xi, cw)
+    return float(np.sum((u - y) ** 2)), u
+
+
+def test_forward_matches_dense_fA_b():
+    kappa, A, w, b, y = setup()
+    xi, cw, _ = weights(w.min(), w.max(), ALPHA)
+    u, _ = adjoint.forward(A, b, xi, cw)
+    W, Q = np.linalg.eigh(A.toarray())
+    ref = Q @ (np.exp(-TAU * W ** (ALPHA / 2)) * (Q.T @ b))
+    assert rel(u, ref) < 1e-9
+
+
+def test_kappa_gradient_central_differences():
+    n = 12
+    kappa, A, w, b, y = setup(n)
+    # contour on bounds that stay valid under the small kappa perturbations
+    xi, cw, dcw = weights(0.9 * w.min(), 1.1 * w.max(), ALPHA)
+    L0, u = loss(A, b, y, xi, cw)
+    g = 2 * (u - y)
+    gb, gk, ga = adjoint.gradients(A, kappa, b, g, xi, cw, dcw)
+    rng = np.random.default_rng(3)
+    for i in rng.choice(n * n, 8, replace=False):
+        d = 1e-6 * kappa[i]
+        kp = kappa.copy(); kp[i] += d
+        km = kappa.copy(); km[i] -= d
+        Lp, _ = loss(problems.varcoef_laplacian_2d(kp, n), b, y, xi, cw)
+        Lm, _ = loss(problems.varcoef_laplacian_2d(km, n), b, y, xi, cw)
+        fd = (Lp - Lm) / (2 * d)
+        assert abs(fd - gk[i]) <= 1e-5 * max(abs(gk[i]), 1e-3 * np.abs(gk).max())
+
+
+def test_b_and_alpha_gradients_central_differences():
+    kappa, A, w, b, y = setup()
+    l1, lN = 0.9 * w.min(), 1.1 * w.max()
+    xi, cw, dcw = weights(l1, lN, ALPHA)
+    L0, u = loss(A, b, y, xi, cw)
+    g = 2 * (u - y)
+    gb, gk, ga = adjoint.gradients(A, kappa, b, g, xi, cw, dcw)
+    v = np.random.default_rng(5).standard_normal(b.size)
+    h = 1e-6
+    fd = (loss(A, b + h * v, y, xi, cw)[0] - loss(A, b - h * v, y, xi, cw)[0]) / (2 * h)
+    assert abs(fd - gb @ v) <= 1e-6 * abs(fd)
+    ha = 1e-6
+    xp, cp, _ = weights(l1, lN, ALPHA + ha)
+    xm, cm, _ = weights(l1, lN, ALPHA - ha)
+    fda = (loss(A, b, y, xp, cp)[0] - loss(A, b, y, xm, cm)[0]) / (2 * ha)
+    assert abs(fda - ga) <= 1e-5 * abs(fda)

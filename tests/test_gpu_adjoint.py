@@ -1,0 +1,148 @@
+"""The adjoint (gm_fab_*) against the CPU oracle and central finite differences.
+
+Tolerances:
+* same Krylov approximation on both sides (oracle/adjoint.krylov_gradients),
+  small well-conditioned grid: relative 2-norm <= 1e-10;
+* against exact resolvents (sparse LU), configs[3]-type problem: the Krylov
+  error of the m used (measured on the CPU: 7.5e-7 at m = 600 on the 256^2
+  grid), so <= 1e-5 for the forward and the gradients;
+* central differences, step 1e-6 on kappa_i: <= 1e-3 relative per entry, with
+  an absolute floor of 1e-4 * max|grad| for entries near zero.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import adjoint as oadj
+from conftest import rel
+from gpumatfun_amd import problems, quadrature as q
+from gpumatfun_amd.adjoint import FabEngine, FractionalDiffusion, contour_weights
+
+pytestmark = pytest.mark.gpu
+TAU, ALPHA = 1e-3, 1.5
+
+
+def problem(n, seed=1):
+    kappa = np.exp(0.5 * problems.gaussian_random_field(n, 0.1, seed=seed))
+    A = problems.varcoef_laplacian_2d(kappa, n)
+    return kappa, A
+
+
+def bounds(A):
+    import scipy.sparse.linalg as sla
+    if A.shape[0] <= 2000:
+        w = np.linalg.eigvalsh(A.toarray())
+        return w.min(), w.max()
+    lo = sla.eigsh(A, k=1, sigma=0, which="LM", return_eigenvectors=False)[0]
+    hi = sla.eigsh(A, k=1, which="LA", return_eigenvectors=False)[0]
+    return lo, hi
+
+
+def contour(A, ns=32, alpha=ALPHA):
+    lo, hi = bounds(A)
+    c = q.contour_nodes(0.95 * lo, 1.05 * hi, ns, TAU, 1.0)
+    cw, dc = contour_weights(c["xi"], c["dxidt"], c["s"], TAU, alpha)
+    return c, cw, dc
+
+
+def test_adjoint_matches_krylov_consistent_oracle():
+    n, m = 16, 24
+    kappa, A = problem(n)
+    c, cw, dc = contour(A, ns=24)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(n * n)
+    y = rng.standard_normal(n * n)
+    print(f"case adjoint n={n} m={m}", flush=True)
+    eng = FabEngine(A, m=m, xi=c["xi"], wts=cw, dwts=dc)
+    u = eng.forward(b)
+    X = oadj.krylov_resolvents(A, c["xi"], b, m)
+    assert rel(u, np.imag(X @ cw)) < 1e-10
+    g = 2 * (u - y)
+    gb, gk, ga = eng.backward(g, kappa)
+    rb, rk, ra = oadj.krylov_gradients(A, kappa, b, g, c["xi"], cw, dc, m)
+    print("rel", rel(gb, rb), rel(gk, rk), abs(ga - ra) / abs(ra), flush=True)
+    assert rel(gb, rb) < 1e-10
+    assert rel(gk, rk) < 1e-10
+    assert abs(ga - ra) <= 1e-10 * abs(ra)
+    eng.delete()
+
+
+def test_recompute_basis_is_bitwise_identical():
+    n, m = 16, 24
+    kappa, A = problem(n)
+    c, cw, dc = contour(A, ns=24)
+    b = np.random.default_rng(2).standard_normal(n * n)
+    g = np.random.default_rng(3).standard_normal(n * n)
+    e1 = FabEngine(A, m=m, xi=c["xi"], wts=cw, dwts=dc)
+    e2 = FabEngine(A, m=m, xi=c["xi"], wts=cw, dwts=dc, recompute=True)
+    e1.forward(b)
+    e2.forward(b)
+    r1 = e1.backward(g, kappa)
+    r2 = e2.backward(g, kappa)
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1]) and r1[2] == r2[2]
+    e1.delete()
+    e2.delete()
+
+
+def test_adjoint_against_exact_resolvents_128():
+    n, m = 128, 400
+    kappa, A = problem(n)
+    c, cw, dc = contour(A, ns=32)
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(n * n)
+    y = rng.standard_normal(n * n)
+    eng = FabEngine(A, m=m, xi=c["xi"], wts=cw, dwts=dc)
+    u = eng.forward(b)
+    ue, _ = oadj.forward(A, b, c["xi"], cw)
+    g = 2 * (u - y)
+    gb, gk, ga = eng.backward(g, kappa)
+    rb, rk, ra = oadj.gradients(A, kappa, b, g, c["xi"], cw, dc)
+    e = (rel(u, ue), rel(gb, rb), rel(gk, rk), abs(ga - ra) / abs(ra))
+    print("rel vs exact resolvents (u, b, kappa, alpha):", e, flush=True)
+    assert max(e) < 1e-5
+    eng.delete()
+
+
+def test_kappa_gradient_central_differences_cfg3():
+    """256^2, kappa = exp(0.5 GRF), L = ||f(A)b - y||^2, 64 random kappa
+    entries, central differences with step 1e-6."""
+    n, m = 256, 600
+    kappa, A = problem(n)
+    c, cw, dc = contour(A, ns=32)
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(n * n)
+    y = rng.standard_normal(n * n)
+    eng = FabEngine(A, m=m, xi=c["xi"], wts=cw, dwts=dc)
+    u = eng.forward(b)
+    g = 2 * (u - y)
+    gb, gk, ga = eng.backward(g, kappa)
+    gmax = np.abs(gk).max()
+    idx = rng.choice(n * n, 64, replace=False)
+    worst = 0.0
+    h = 1e-6
+    for i in idx:
+        kp = kappa.copy(); kp[i] += h
+        km = kappa.copy(); km[i] -= h
+        eng.set_matrix(problems.varcoef_laplacian_2d(kp, n))
+        Lp = np.sum((eng.forward(b) - y) ** 2)
+        eng.set_matrix(problems.varcoef_laplacian_2d(km, n))
+        Lm = np.sum((eng.forward(b) - y) ** 2)
+        fd = (Lp - Lm) / (2 * h)
+        err = abs(fd - gk[i]) / max(abs(gk[i]), 1e-1 * gmax)
+        worst = max(worst, err)
+    print(f"worst FD relative error over 64 entries: {worst:.3e} (max |grad| {gmax:.3e})", flush=True)
+    assert worst < 1e-3
+    eng.delete()
+
+
+def test_torch_autograd_function_gradcheck():
+    import torch
+    n = 6
+    kappa, A = problem(n)
+    lo, hi = bounds(A)
+    op = FractionalDiffusion(n, tau=TAU, l1=0.8 * lo, lN=1.3 * hi, ns=40, m=n * n)
+    b = torch.tensor(np.random.default_rng(0).standard_normal(n * n), requires_grad=True)
+    k = torch.tensor(kappa, requires_grad=True)
+    a = torch.tensor(ALPHA, dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda b_, k_, a_: op.apply(b_, k_, a_), (b, k, a),
+                                    eps=1e-6, atol=1e-6, rtol=1e-4)
