@@ -25,7 +25,7 @@ EXPORTS = (
     "gm_lin_sys_solves", "gm_apply_V", "gm_update_U", "gm_update_U_monodomain",
     "gm_get_U", "gm_get_xsol", "gm_get_aabb", "gm_set_y", "gm_get_y", "gm_get_state",
     "gm_set_state", "gm_step", "gm_synchronize", "gm_get_info", "gm_get_stream",
-    "gm_profile_spmm", "gm_profile_copy",
+    "gm_profile_spmm", "gm_profile_copy", "gm_set_ms_params", "gm_update_U_ms",
     "gm_batch_last_error", "gm_batch_create", "gm_batch_delete", "gm_batch_step",
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
     "gm_fab_last_error", "gm_fab_create", "gm_fab_delete", "gm_fab_set_matrix",
@@ -71,6 +71,12 @@ class GmCreateDesc(C.Structure):
         ("flags", C.c_uint32),
         ("device", C.c_int32),
     ]
+
+
+class GmMsParams(C.Structure):
+    """Mirror of ``gm_ms_params`` (Mitchell-Schaeffer time constants and gate threshold)."""
+    _fields_ = [("tau_in", C.c_double), ("tau_out", C.c_double), ("tau_open", C.c_double),
+                ("tau_close", C.c_double), ("v_gate", C.c_double)]
 
 
 class GmBatchDesc(C.Structure):
@@ -166,6 +172,8 @@ def load():
         "gm_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
         "gm_profile_spmm": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "gm_profile_copy": (C.c_int, [vp, C.c_int64, C.c_int32, _dp]),
+        "gm_set_ms_params": (C.c_int, [vp, C.POINTER(GmMsParams)]),
+        "gm_update_U_ms": (C.c_int, [vp]),
         "gm_batch_last_error": (C.c_char_p, []),
         "gm_batch_create": (C.c_int, [C.POINTER(GmBatchDesc), C.POINTER(vp)]),
         "gm_batch_delete": (C.c_int, [vp]),

@@ -352,7 +352,12 @@ __global__ __launch_bounds__(kBlock) void k_vc(VCArgs p) {
 // ---------------------------------------------------------------------------
 // Time-step update and deflation.
 // ---------------------------------------------------------------------------
-enum UpdateMode : int { kUpdFisher = 0, kUpdMono = 1, kUpdPrepare = 2 };
+// kUpdMS: Mitchell-Schaeffer ionic model (no reference counterpart; the
+// reference's monodomain reaction is FitzHugh-Nagumo).  The gate h lives in
+// the vector the FHN path uses for its recovery variable:
+//   h <- h + dt ((1 - h)/tau_open)  if u < v_gate,  h - dt h/tau_close otherwise
+//   g  = h u^2 (1 - u)/tau_in - u/tau_out   (with the updated h)
+enum UpdateMode : int { kUpdFisher = 0, kUpdMono = 1, kUpdPrepare = 2, kUpdMS = 3 };
 
 struct UpdateArgs {
   const double2* X;
@@ -360,6 +365,7 @@ struct UpdateArgs {
   const double* mass;
   double* ode;
   double dt, aode_dt, bode_dt, code, vrest, vpeak, cion;
+  double ms_tau_in, ms_tau_out, ms_tau_open, ms_tau_close, ms_v_gate;
   int clamp, x_zero;
   double2* Usol;
   double2* U;          // K == 0: U = Usol
@@ -394,12 +400,18 @@ __global__ __launch_bounds__(kBlock) void k_update1(UpdateArgs p) {
         const double ms = p.mass[r];
         const double ub = u / ms;
         g = (ms * ub) * (1.0 - ub);
-      } else {
+      } else if (MODE == kUpdMono) {
         const double o = p.ode[r];
         const double on = o + ((u * p.aode_dt + o * p.bode_dt) + p.code);
         p.ode[r] = on;
         const double src = ((u - p.vrest) * u) * (u - p.vpeak) * p.cion;
         g = src - on;
+      } else {   // kUpdMS
+        const double h = p.ode[r];
+        const double dh = u < p.ms_v_gate ? (1.0 - h) / p.ms_tau_open : -h / p.ms_tau_close;
+        const double hn = h + p.dt * dh;
+        p.ode[r] = hn;
+        g = (hn * (u * u) * (1.0 - u)) / p.ms_tau_in - u / p.ms_tau_out;
       }
       us = d2(u, g * p.dt);
       p.Usol[r] = us;

@@ -64,6 +64,9 @@ struct gm_solver {
   double dt = 0, aode = 0, bode = 0, code = 0, vrest = 0, vamp = 0, vth = 0, vpeak = 0,
          c1ion = 0, c2ion = 0;
   bool clamp = true;
+  // Mitchell-Schaeffer parameters (gm_set_ms_params); tau_in, tau_out, tau_open,
+  // tau_close, v_gate
+  double ms[5] = {0.3, 6.0, 120.0, 150.0, 0.13};
   std::vector<double> gamma;
   // schedules
   int lanes = 8, G_sp = 1, chunk_sp = 1, G_el = 1, chunk_el = 1, G_df = 1, chunk_df = 1;
@@ -86,7 +89,7 @@ struct gm_solver {
   double* stage = nullptr;  // n x 2 column-major staging for the getters
   // host-side state machine
   bool built = false, solved = false, x_zero = true;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -335,6 +338,11 @@ gm_status enqueue_update(gm_solver* s, int mode) {
   a.vrest = s->vrest;
   a.vpeak = s->vpeak;
   a.cion = s->c1ion / s->vamp / s->vamp;
+  a.ms_tau_in = s->ms[0];
+  a.ms_tau_out = s->ms[1];
+  a.ms_tau_open = s->ms[2];
+  a.ms_tau_close = s->ms[3];
+  a.ms_v_gate = s->ms[4];
   a.clamp = s->clamp ? 1 : 0;
   a.x_zero = s->x_zero ? 1 : 0;
   a.Usol = s->Usol;
@@ -356,6 +364,9 @@ gm_status enqueue_update(gm_solver* s, int mode) {
       break;
     case gm::kUpdMono:
       hipLaunchKernelGGL(gm::k_update1<gm::kUpdMono>, dim3(G), dim3(kBlock), 0, s->stream, a);
+      break;
+    case gm::kUpdMS:
+      hipLaunchKernelGGL(gm::k_update1<gm::kUpdMS>, dim3(G), dim3(kBlock), 0, s->stream, a);
       break;
     default:
       hipLaunchKernelGGL(gm::k_update1<gm::kUpdPrepare>, dim3(G), dim3(kBlock), 0, s->stream, a);
@@ -400,7 +411,7 @@ gm_status enqueue_step(gm_solver* s, int mono) {
   GM_TRY(enqueue_build(s));
   GM_TRY(enqueue_solves(s));
   GM_TRY(enqueue_apply(s));
-  GM_TRY(enqueue_update(s, mono ? gm::kUpdMono : gm::kUpdFisher));
+  GM_TRY(enqueue_update(s, mono == 2 ? gm::kUpdMS : (mono ? gm::kUpdMono : gm::kUpdFisher)));
   return GM_OK;
 }
 
@@ -719,6 +730,28 @@ gm_status gm_update_U_monodomain(gm_solver* s) {
   return enqueue_update(s, gm::kUpdMono);
 }
 
+gm_status gm_set_ms_params(gm_solver* s, const gm_ms_params* p) {
+  GM_CHECK_HANDLE(s);
+  if (!p || !(p->tau_in > 0) || !(p->tau_out > 0) || !(p->tau_open > 0) || !(p->tau_close > 0))
+    return fail(GM_EINVAL, "Mitchell-Schaeffer time constants must be positive");
+  s->ms[0] = p->tau_in;
+  s->ms[1] = p->tau_out;
+  s->ms[2] = p->tau_open;
+  s->ms[3] = p->tau_close;
+  s->ms[4] = p->v_gate;
+  for (auto& g : s->graph)   // captured steps hold the old parameters
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  return GM_OK;
+}
+
+gm_status gm_update_U_ms(gm_solver* s) {
+  GM_CHECK_HANDLE(s);
+  return enqueue_update(s, gm::kUpdMS);
+}
+
 gm_status gm_get_U(gm_solver* s, double* dst) {
   GM_CHECK_HANDLE(s);
   return fetch_pair(s, s->Usol, dst);
@@ -806,7 +839,8 @@ gm_status gm_set_state(gm_solver* s, const double* u, const double* dtg, const d
 gm_status gm_step(gm_solver* s, int32_t nsteps, int32_t monodomain, int32_t use_graph) {
   GM_CHECK_HANDLE(s);
   if (nsteps < 0) return fail(GM_EINVAL, "nsteps must be >= 0");
-  const int mono = monodomain ? 1 : 0;
+  if (monodomain < 0 || monodomain > 2) return fail(GM_EINVAL, "reaction model must be 0, 1 or 2");
+  const int mono = monodomain;
   if (!s->x_zero) {
     // a partial step is pending (apply_V without update_U): finish it eagerly
     use_graph = 0;

@@ -189,6 +189,15 @@ class LanczosEngine:
     def update_U_monodomain(self):
         self._call(self._lib.gm_update_U_monodomain)
 
+    def update_U_ms(self):
+        """Mitchell-Schaeffer reaction (gate kept in the ode vector)."""
+        self._call(self._lib.gm_update_U_ms)
+
+    def set_ms_params(self, tau_in=0.3, tau_out=6.0, tau_open=120.0, tau_close=150.0,
+                      v_gate=0.13):
+        p = _lib.GmMsParams(tau_in, tau_out, tau_open, tau_close, v_gate)
+        self._call(self._lib.gm_set_ms_params, C.byref(p))
+
     def _get_pair(self, fn):
         out = np.empty(2 * self.n)
         self._call(fn, _ptr(out))
@@ -232,7 +241,8 @@ class LanczosEngine:
         self._call(self._lib.gm_set_state, *[_ptr(a) for a in arrs])
 
     def step(self, nsteps=1, monodomain=False, use_graph=False):
-        self._call(self._lib.gm_step, int(nsteps), int(bool(monodomain)), int(bool(use_graph)))
+        """monodomain: False/0 Fisher-KPP, True/1 FitzHugh-Nagumo, 2 Mitchell-Schaeffer."""
+        self._call(self._lib.gm_step, int(nsteps), int(monodomain), int(bool(use_graph)))
 
     def synchronize(self):
         self._call(self._lib.gm_synchronize)

@@ -116,6 +116,19 @@ gm_status gm_update_U(gm_solver* s);
 /* replaces update_U_monodomain.cpp:6-35 -> update_U_monodomain.cu:5-43.   */
 gm_status gm_update_U_monodomain(gm_solver* s);
 
+/* Mitchell-Schaeffer reaction (no reference counterpart: the reference's
+ * monodomain reaction is FitzHugh-Nagumo, update_U_monodomain.cu:11-22).
+ * The gate h is kept in the ode vector; one update does
+ *   u  = X(:,1) + X(:,2) + update(:,1) + update(:,2)
+ *   h += dt*((1-h)/tau_open)  if u < v_gate,  h += dt*(-h/tau_close) otherwise
+ *   g  = h u^2 (1-u)/tau_in - u/tau_out,  U(:,2) = dt*g
+ * then the deflation split as in update_U.  Defaults: 0.3, 6, 120, 150, 0.13. */
+typedef struct gm_ms_params {
+    double tau_in, tau_out, tau_open, tau_close, v_gate;
+} gm_ms_params;
+gm_status gm_set_ms_params(gm_solver* s, const gm_ms_params* p);
+gm_status gm_update_U_ms(gm_solver* s);
+
 /* replaces get_U.cpp:6-38 -> lanczos_impl.cu:502-506: dst is n x 2
  * column-major host memory, receives Usol = [u, dt*g(u)].                  */
 gm_status gm_get_U(gm_solver* s, double* dst);
@@ -144,7 +157,8 @@ gm_status gm_get_state(gm_solver* s, double* u, double* g, double* ode);
 gm_status gm_set_state(gm_solver* s, const double* u, const double* g, const double* ode);
 
 /* One or more full time steps: build_subspace, lin_sys_solves, apply_V,
- * update_U (or update_U_monodomain), the sequence of
+ * then update_U (monodomain = 0), update_U_monodomain (1) or update_U_ms (2),
+ * the sequence of
  * nicksmethodgpu_problem.m:7-16 / nicksmethodgpu_problem_monodomain.m:7-16.
  * use_graph != 0 captures one step into a HIP graph on first use and
  * replays it.  Asynchronous: returns once the work is enqueued.            */

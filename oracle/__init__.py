@@ -37,6 +37,12 @@ nicksmethodcpu          dense f(A) b (eigh) on N <= 64 and the DST-I closed form
                         for the 2D Dirichlet Laplacian; ``test_fab_*``
 step loops              dense exponential-Euler recurrence on tiny N;
                         ``test_step_dense``
+ms_reaction             forward-Euler gate in closed form for constant u
+                        (h_n = 1 - (1-h0)(1-dt/tau_open)^n), resting state a
+                        fixed point, J sign/zeros at u = 0, 1;
+                        ``test_ms_reaction_*``
+adjoint.gradients       central differences in kappa, b and alpha;
+                        ``tests/test_adjoint_oracle.py``
 gpu_semantics pipeline  equal to the CPU step loop when the clamp is off and
                         u >= 0; its own clamp/deflation algebra checked densely;
                         ``test_gpu_semantics_*``
@@ -49,5 +55,6 @@ from .krylov import (  # noqa: F401
     lanczos, jacobi_poly_eval, tridiag_solve_pivoted, nicksmethodcpu,
 )
 from .stepping import (  # noqa: F401
-    cpu_steps, cpu_steps_nonuniform, cpu_steps_monodomain, GpuSemantics,
+    cpu_steps, cpu_steps_nonuniform, cpu_steps_monodomain, cpu_steps_ms, ms_reaction,
+    GpuSemantics,
 )

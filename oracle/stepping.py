@@ -88,6 +88,44 @@ def cpu_steps_monodomain(setup, u0, gu0, ode0, steps, aode, epsil, beta, gamma_f
     return u, gu, ode
 
 
+MS_DEFAULT = dict(tau_in=0.3, tau_out=6.0, tau_open=120.0, tau_close=150.0, v_gate=0.13)
+
+
+def ms_reaction(u, h, dt, p=MS_DEFAULT):
+    """Mitchell-Schaeffer gate update and current (Mitchell & Schaeffer 2003):
+
+        dh/dt = (1 - h)/tau_open  (u < v_gate),   -h/tau_close  (u >= v_gate)
+        J     = h u^2 (1 - u)/tau_in - u/tau_out
+
+    One forward-Euler step of h, then J with the updated h (the order the FHN
+    path uses, update_U_monodomain.cu:11-22).  No reference counterpart.
+    """
+    dh = np.where(u < p["v_gate"], (1.0 - h) / p["tau_open"], -h / p["tau_close"])
+    hn = h + dt * dh
+    return hn, (hn * (u * u) * (1.0 - u)) / p["tau_in"] - u / p["tau_out"]
+
+
+def cpu_steps_ms(setup, u0, h0, steps, p=MS_DEFAULT):
+    """Exponential-Euler steps with the Mitchell-Schaeffer reaction; returns (u, g, h).
+
+    The step loop of problem_startup_CPU_monodomain.m:128-141 with the FHN
+    source replaced by ``ms_reaction``; g(u0) uses h0 unchanged.
+    """
+    S = setup
+    dt = S["dt"]
+    u = np.array(u0, dtype=np.float64)
+    h = np.array(h0, dtype=np.float64)
+    gu = (h * (u * u) * (1.0 - u)) / p["tau_in"] - u / p["tau_out"]
+    for _ in range(steps):
+        xbar, ybar, up1, up2 = _deflate(S["Q"], S["exp_lambda"], S["phi_lambda"], u, gu)
+        x1, x2 = nicksmethodcpu(S["wts1_cpu"], S["wts2_cpu"], S["A"], xbar, ybar,
+                                S["n"], S["m"], S["s"], S["gamma"],
+                                S["gamma_s_real"], S["gamma_s_imag"], S["etaxi"])
+        u = x1 + dt * x2 + up1 + dt * up2
+        h, gu = ms_reaction(u, h, dt, p)
+    return u, gu, h
+
+
 class GpuSemantics:
     """The reference GPU solver's state machine, restated on the CPU.
 

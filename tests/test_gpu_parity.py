@@ -250,3 +250,34 @@ def test_bad_handle_after_delete():
     assert st == 5  # GM_EBADHANDLE
     with pytest.raises(GpuMatfunError):
         eng.build_subspace()
+
+
+def test_mitchell_schaeffer_vs_oracle():
+    """Mitchell-Schaeffer reaction on an anisotropic 9-point operator (fibre angle
+    rotating 0..pi/3, ratio 9:1), stimulus in a corner patch, alpha = 1.7."""
+    from gpumatfun_amd import problems, quadrature as q
+    n = 32
+    A = problems.anisotropic_9pt(n, ratio=9.0)
+    w = np.linalg.eigvalsh(A.toarray())
+    N = n * n
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    u0 = ((X < 6) & (Y < 6)).astype(float).reshape(-1, order="F")
+    h0 = np.ones(N)
+    p = oracle.stepping.MS_DEFAULT
+    g0 = (h0 * u0 ** 2 * (1 - u0)) / p["tau_in"] - u0 / p["tau_out"]
+    dt = 0.05
+    inp = q.make_inputs(A, u0, g0, m=30, n=24, eta=dt * 1e-3, alpha=1.7, dt=dt,
+                        l1=float(w.min()), lN=float(w.max()), degree=0, ode=h0)
+    eng = LanczosEngine.from_inputs(inp, clamp=False)
+    eng.set_ms_params(**p)
+    for _ in range(5):
+        eng.step(1, monodomain=2)
+    U = eng.get_U()
+    _, _, h = eng.get_state()
+    S = oracle_setup(inp)
+    u, gu, ho = oracle.cpu_steps_ms(S, u0, h0, 5)
+    print("rel err", rel(U[:, 0], u), rel(U[:, 1], dt * gu), rel(h, ho))
+    assert rel(U[:, 0], u) < TOL
+    assert rel(U[:, 1], dt * gu) < TOL
+    assert rel(h, ho) < TOL
+    eng.delete()

@@ -323,3 +323,27 @@ def test_gpu_semantics_clamp_zeroes_negatives():
                             inp.mass, inp.wts1, inp.wts2, inp.shifts, clamp=True)
     G.step()
     assert np.all(G.get_U()[:, 0] >= 0) and np.any(G.get_U()[:, 0] == 0)
+
+
+# ----------------------------------------------------------- Mitchell-Schaeffer
+def test_ms_reaction_gate_closed_form_and_fixed_point():
+    p = oracle.stepping.MS_DEFAULT
+    dt = 0.05
+    u = np.full(4, 0.05)                    # below v_gate: gate opens
+    h = np.array([0.0, 0.2, 0.7, 1.0])
+    h0 = h.copy()
+    for _ in range(40):
+        h, _ = oracle.ms_reaction(u, h, dt)
+    assert np.allclose(h, 1 - (1 - h0) * (1 - dt / p["tau_open"]) ** 40, rtol=1e-13)
+    u = np.full(2, 0.5)                     # above v_gate: gate closes
+    h = np.array([1.0, 0.4])
+    hc = h.copy()
+    for _ in range(30):
+        h, J = oracle.ms_reaction(u, h, dt)
+    assert np.allclose(h, hc * (1 - dt / p["tau_close"]) ** 30, rtol=1e-13)
+    # resting state (u = 0, h = 1) is a fixed point with zero current
+    hn, J = oracle.ms_reaction(np.zeros(3), np.ones(3), dt)
+    assert np.array_equal(hn, np.ones(3)) and np.array_equal(J, np.zeros(3))
+    # u = 1: inward current vanishes, only -1/tau_out remains
+    _, J = oracle.ms_reaction(np.ones(1), np.ones(1), dt)
+    assert J[0] == pytest.approx(-1.0 / p["tau_out"], rel=1e-15)
