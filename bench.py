@@ -269,12 +269,12 @@ def main():
         "data": "synthetic (seeded mesh and initial condition)",
         "config": cfg,
         "graph": use_graph,
-        "spmm_lanes": info["spmm_lanes"],
+        "spmm_schedule": info["spmm_schedule"],
         "setup_s": round(setup_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": "k_spmm<%d,%s>" % (info["spmm_lanes"], "kMid" if mode == 0 else "kLast"),
+                     "kernel": "%s, %s" % (info["spmm_schedule"], "kMid" if mode == 0 else "kLast"),
                      "avg_launch_us": round(spmm_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(bytes_per)},
         "copy_kernel_gbs": round(copy_bytes / (copy_ms * 1e-3) / 1e9, 1),
@@ -336,7 +336,7 @@ def run_batch(args, steps, warm, use_graph):
         "chunk_columns": info["chunk_columns"],
         "setup_s": round(setup_s, 1),
         "plain_same_matrix": {"value": round(plain, 3), "unit": "solves/s",
-                              "spmm_lanes": einfo["spmm_lanes"]},
+                              "spmm_schedule": einfo["spmm_schedule"]},
         "batched_vs_plain": round(value / plain, 3),
         "roofline": {"bound": "hbm", "kernel": "plain k_spmm kLast on the same matrix",
                      "achieved": round(bytes_per / (spmm_ms * 1e-3) / 1e9, 1),

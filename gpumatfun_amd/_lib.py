@@ -15,6 +15,8 @@ GM_OK, GM_EINVAL, GM_ENOMEM, GM_EDEVICE, GM_EBREAKDOWN, GM_EBADHANDLE, GM_ESTATE
 GM_SPARSE_CSC, GM_SPARSE_CSR = 0, 1
 GM_FLAG_NO_CLAMP = 1
 GM_FLAG_AUTOTUNE = 2
+GM_FLAG_ROW_GROUPS = 4
+GM_FLAG_ROW_BLOCKS = 8
 
 STATUS_NAMES = {0: "GM_OK", 1: "GM_EINVAL", 2: "GM_ENOMEM", 3: "GM_EDEVICE",
                 4: "GM_EBREAKDOWN", 5: "GM_EBADHANDLE", 6: "GM_ESTATE"}

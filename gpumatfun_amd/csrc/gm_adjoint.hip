@@ -42,7 +42,9 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // m x ncol (ncol a multiple of 64), X row-major N x ncol.  One wavefront per
 // 16-row tile and 64 columns (4 MFMA 16x16 output tiles); K stepped by 4.
 // MFMA f64 16x16x4 operand layout: A lane l = (row l%16, k l/16), B lane l =
-// (k l/16, col l%16), D lane l holds rows 4(l/16)..4(l/16)+3 of column l%16.
+// (k l/16, col l%16); D register r of lane l is (row 4r + l/16, col l%16)
+// (the f64 form interleaves its output rows, unlike the f32 16x16x4 form,
+// whose register r of lane l is row 4(l/16) + r).
 __global__ __launch_bounds__(256) void k_vy_mfma(const double* __restrict__ V, int N, int m,
                                                  const double* __restrict__ Y, int ncol,
                                                  double* __restrict__ X) {
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(256) void k_vy_mfma(const double* __restrict__ V, i
     const int j = ct * 64 + t * 16 + (lane & 15);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = i0 + 4 * (lane >> 4) + r;
+      const int i = i0 + 4 * r + (lane >> 4);
       if (i < N) X[(size_t)i * ncol + j] = acc[t][r];
     }
   }
@@ -263,7 +265,7 @@ gm_status areduce(gm_fab* f, int G, int nv, double* out, int do_sqrt, int flag_z
   r.do_sqrt = do_sqrt;
   r.flag_zero = flag_zero;
   r.status = f->status;
-  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kBlock), 0, f->stream, r);
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kReduceBlock), 0, f->stream, r);
   A_HIP(hipGetLastError());
   return GM_OK;
 }

@@ -59,8 +59,13 @@ __device__ __forceinline__ void block_partial2(double2 v, double* partials) {
 }
 
 // Sums nv values over G blocks of partials ([G][nv]) in a fixed order and
-// writes out[v * stride] (square-rooted when do_sqrt).  One block.  Status
+// writes out[v * stride] (square-rooted when do_sqrt).  One block of
+// kReduceBlock threads: ns = kReduceBlock / W threads per value each sum a
+// strided subset with four independent accumulators (so four loads are in
+// flight), then a fixed-order LDS tree combines the ns sub-sums.  Status
 // bits: 4 = non-finite result, 1 = exact zero (when flag_zero).
+constexpr int kReduceBlock = 1024;
+
 struct ReduceArgs {
   const double* partials;
   int G, nv;
@@ -72,31 +77,47 @@ struct ReduceArgs {
 
 // (a template so that both translation units may instantiate it)
 template <int kUnused = 0>
-__global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs p) {
-  __shared__ double sub[kBlock];
-  for (int v0 = 0; v0 < p.nv; v0 += kBlock) {
-    const int W = min(p.nv - v0, kBlock);
-    const int ns = kBlock / W;
-    const int v = v0 + (int)threadIdx.x % W, sb = (int)threadIdx.x / W;
+__global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs p) {
+  __shared__ double sub[kReduceBlock];
+  const int t = threadIdx.x;
+  for (int v0 = 0; v0 < p.nv; v0 += kReduceBlock) {
+    const int W = min(p.nv - v0, kReduceBlock);
+    const int ns = kReduceBlock / W;
+    const int vv = t % W, sb = t / W, v = v0 + vv;
     double s = 0.0;
-    if (sb < ns)
-      for (int b = sb; b < p.G; b += ns) s += p.partials[(size_t)b * p.nv + v];
+    if (sb < ns) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int b = sb;
+      for (; b + 3 * ns < p.G; b += 4 * ns) {
+        s0 += p.partials[(size_t)b * p.nv + v];
+        s1 += p.partials[(size_t)(b + ns) * p.nv + v];
+        s2 += p.partials[(size_t)(b + 2 * ns) * p.nv + v];
+        s3 += p.partials[(size_t)(b + 3 * ns) * p.nv + v];
+      }
+      for (; b < p.G; b += ns) s0 += p.partials[(size_t)b * p.nv + v];
+      s = (s0 + s1) + (s2 + s3);
+    }
     __syncthreads();
-    sub[threadIdx.x] = s;
+    sub[t] = s;
     __syncthreads();
-    if ((int)threadIdx.x < W) {
-      double t = 0.0;
-      for (int i = 0; i < ns; ++i) t += sub[i * W + threadIdx.x];
+    int h = 1;
+    while (h < ns) h <<= 1;
+    for (h >>= 1; h > 0; h >>= 1) {
+      if (sb < h && sb + h < ns) sub[t] += sub[t + h * W];
+      __syncthreads();
+    }
+    if (t < W) {
+      double r = sub[t];
       if (p.do_sqrt) {
-        t = sqrt(t);
+        r = sqrt(r);
         if (p.status) {
-          if (!isfinite(t))
+          if (!isfinite(r))
             atomicOr(p.status, 4u);
-          else if (p.flag_zero && t == 0.0)
+          else if (p.flag_zero && r == 0.0)
             atomicOr(p.status, 1u);
         }
       }
-      p.out[(size_t)(v0 + threadIdx.x) * p.stride] = t;
+      p.out[(size_t)v * p.stride] = r;
     }
   }
 }

@@ -14,8 +14,8 @@
 //     (lanczos.h:51-55) so the getters return them unchanged.
 //
 // Reductions (dot products, norms, Q^T U) are deterministic: each block
-// reduces its rows in a fixed tree, writes one partial, and the last block to
-// finish (atomic ticket) sums the partials in block order.  Results do not
+// reduces its rows in a fixed tree and writes one partial; k_reduce, launched
+// after the kernel, sums the partials in a fixed order.  Results do not
 // depend on scheduling, so repeated runs are bit-identical.
 #pragma once
 #include "gm_device.h"
@@ -27,9 +27,8 @@ constexpr int kMaxDegree = 16;        // gamma_size <= 17
 constexpr int kDeflTile = 8;          // deflation vectors per register tile
 
 // ---------------------------------------------------------------------------
-// CSR SpMM with R = 2 interleaved columns and fused Lanczos / Horner epilogues.
-// LANES consecutive lanes share one row (coalesced val/col loads, butterfly
-// reduction inside the lane group); each block owns a contiguous row chunk.
+// CSR SpMM with R = 2 interleaved columns and fused Lanczos / Horner
+// epilogues, in two schedules (row groups, row blocks) picked per matrix.
 // ---------------------------------------------------------------------------
 enum SpmmMode : int {
   kFirst = 0,   // v_k = w/beta_k (stored); y = A(g0 v_k) + g1 v_k
@@ -43,8 +42,9 @@ struct SpmmArgs {
   const int* __restrict__ rowptr;
   const int* __restrict__ col;
   const double* __restrict__ val;
+  const int* __restrict__ rowblk;  // row-block bounds (k_spmm_stream only)
   int nrows;
-  int chunk;                   // rows per block
+  int chunk;                   // rows per block (k_spmm only)
   const double2* x;            // gather source
   double2* y;                  // output (row aligned)
   double2* vk;                 // V[k]: written by kFirst/kOnly, read by kMid/kLast
@@ -60,6 +60,76 @@ struct SpmmArgs {
   unsigned* status;            // bit 0: breakdown seen
 };
 
+// Per-launch constants of the row epilogue: 1/beta_k, beta_k (kFirst, kOnly),
+// beta_k for the -beta_k v_{k-1} term (kLast, kOnly), and the scale of A x.
+struct SpmmScalars {
+  double2 inv, bet, bprev, sx;
+};
+
+template <int MODE>
+__device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p) {
+  SpmmScalars c;
+  c.inv = d2(1.0, 1.0);
+  c.bet = d2(0.0, 0.0);
+  c.bprev = d2(0.0, 0.0);
+  if (MODE == kFirst || MODE == kOnly) {
+    const double b0 = p.aabb[p.k + 2 * p.ld], b1 = p.aabb[p.k + 3 * p.ld];
+    c.inv = d2(safe_inv(b0), safe_inv(b1));
+    c.bet = d2(b0, b1);
+  }
+  if ((MODE == kLast || MODE == kOnly) && p.k > 0)
+    c.bprev = d2(p.aabb[p.k + 2 * p.ld], p.aabb[p.k + 3 * p.ld]);
+  c.sx = (MODE == kFirst || MODE == kOnly) ? d2(p.g0 * c.inv.x, p.g0 * c.inv.y) : d2(1.0, 1.0);
+  return c;
+}
+
+// Row r's output from acc = (A x)_r; kLast/kOnly add y_r v_r to dot.
+template <int MODE>
+__device__ __forceinline__ void spmm_row_out(const SpmmArgs& p, const SpmmScalars& c, int r,
+                                             double2 acc, double2& dot) {
+  if (MODE == kFirst) {
+    const double2 w = p.x[r];
+    const double2 v = d2(safe_div(w.x, c.bet.x), safe_div(w.y, c.bet.y));
+    p.vk[r] = v;
+    p.y[r] = d2(fma(p.gi, v.x, acc.x * c.sx.x), fma(p.gi, v.y, acc.y * c.sx.y));
+  } else if (MODE == kMid) {
+    const double2 v = p.vk[r];
+    p.y[r] = d2(fma(p.gi, v.x, acc.x), fma(p.gi, v.y, acc.y));
+  } else if (MODE == kLast || MODE == kOnly) {
+    double2 v;
+    if (MODE == kOnly) {
+      const double2 w = p.x[r];
+      v = d2(safe_div(w.x, c.bet.x), safe_div(w.y, c.bet.y));
+      p.vk[r] = v;
+      acc.x *= c.sx.x;
+      acc.y *= c.sx.y;
+    } else {
+      v = p.vk[r];
+    }
+    double2 yv = acc;
+    if (p.k > 0) {
+      const double2 vp = p.vkm1[r];
+      yv.x = fma(-c.bprev.x, vp.x, yv.x);
+      yv.y = fma(-c.bprev.y, vp.y, yv.y);
+    }
+    p.y[r] = yv;
+    dot.x = fma(yv.x, v.x, dot.x);
+    dot.y = fma(yv.y, v.y, dot.y);
+  } else {  // kHorner
+    const double2 zz = p.z[r];
+    double2 yv = d2(acc.x + zz.x, acc.y + zz.y);
+    if (p.accumulate) {
+      const double2 old = p.y[r];
+      yv.x += old.x;
+      yv.y += old.y;
+    }
+    p.y[r] = yv;
+  }
+}
+
+// Row-group schedule: LANES consecutive lanes share one row (coalesced
+// val/col loads, butterfly reduction inside the lane group); each block owns
+// a contiguous chunk of rows.  Suits rows of tens of nonzeros and more.
 template <int LANES, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
   constexpr int GROUPS = kBlock / LANES;
@@ -68,18 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
   const int row1 = min(row0 + p.chunk, p.nrows);
   const int sub = threadIdx.x % LANES;
   const int grp = threadIdx.x / LANES;
-
-  double2 inv = d2(1.0, 1.0), bet = d2(0.0, 0.0);
-  if (MODE == kFirst || MODE == kOnly) {
-    const double b0 = p.aabb[p.k + 2 * p.ld], b1 = p.aabb[p.k + 3 * p.ld];
-    inv = d2(safe_inv(b0), safe_inv(b1));
-    bet = d2(b0, b1);
-  }
-  double2 bprev = d2(0.0, 0.0);
-  if ((MODE == kLast || MODE == kOnly) && p.k > 0)
-    bprev = d2(p.aabb[p.k + 2 * p.ld], p.aabb[p.k + 3 * p.ld]);
-  const double2 sx = (MODE == kFirst || MODE == kOnly) ? d2(p.g0 * inv.x, p.g0 * inv.y)
-                                                        : d2(1.0, 1.0);
+  const SpmmScalars c = spmm_scalars<MODE>(p);
   double2 dot = d2(0.0, 0.0);
 
   for (int r = row0 + grp; r < row1; r += GROUPS) {
@@ -96,49 +155,87 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
       acc.x += __shfl_xor(acc.x, off, LANES);
       acc.y += __shfl_xor(acc.y, off, LANES);
     }
-    if (sub == 0) {
-      if (MODE == kFirst) {
-        const double2 w = p.x[r];
-        const double2 v = d2(safe_div(w.x, bet.x), safe_div(w.y, bet.y));
-        p.vk[r] = v;
-        p.y[r] = d2(fma(p.gi, v.x, acc.x * sx.x), fma(p.gi, v.y, acc.y * sx.y));
-      } else if (MODE == kMid) {
-        const double2 v = p.vk[r];
-        p.y[r] = d2(fma(p.gi, v.x, acc.x), fma(p.gi, v.y, acc.y));
-      } else if (MODE == kLast || MODE == kOnly) {
-        double2 v;
-        if (MODE == kOnly) {
-          const double2 w = p.x[r];
-          v = d2(safe_div(w.x, bet.x), safe_div(w.y, bet.y));
-          p.vk[r] = v;
-          acc.x *= sx.x;
-          acc.y *= sx.y;
-        } else {
-          v = p.vk[r];
-        }
-        double2 yv = acc;
-        if (p.k > 0) {
-          const double2 vp = p.vkm1[r];
-          yv.x = fma(-bprev.x, vp.x, yv.x);
-          yv.y = fma(-bprev.y, vp.y, yv.y);
-        }
-        p.y[r] = yv;
-        dot.x = fma(yv.x, v.x, dot.x);
-        dot.y = fma(yv.y, v.y, dot.y);
-      } else {  // kHorner
-        const double2 zz = p.z[r];
-        double2 yv = d2(acc.x + zz.x, acc.y + zz.y);
-        if (p.accumulate) {
-          const double2 old = p.y[r];
-          yv.x += old.x;
-          yv.y += old.y;
-        }
-        p.y[r] = yv;
-      }
-    }
+    if (sub == 0) spmm_row_out<MODE>(p, c, r, acc, dot);
   }
 
   // alpha_k = sum of these partials, formed by k_reduce into aabb(k, 0:1)
+  if (MODE == kLast || MODE == kOnly) block_partial2(dot, p.partials);
+}
+
+// Row-block ("CSR-stream") schedule for short rows.  The host cuts the rows
+// into blocks of at most kBlock rows and at most TILE nonzeros (a row longer
+// than TILE forms a block of its own).  Phase 1 streams the block's
+// nonzeros with all threads -- consecutive lanes take consecutive nonzeros,
+// so the val/col loads are fully coalesced and every thread has TILE/kBlock
+// independent gathers in flight -- and stages the products a_j x[col_j] in
+// LDS.  Phase 2 gives each row to one thread, which sums its products in
+// column order and runs the epilogue; the row-aligned loads and stores of
+// the epilogue are coalesced too.  One long row is summed by the whole block
+// instead (strided FMAs, then a fixed-order block tree).
+template <int TILE, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
+  static_assert(TILE % kBlock == 0, "TILE must be a multiple of the block size");
+  constexpr int PER = TILE / kBlock;
+  __shared__ double2 prod[TILE];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int r0 = p.rowblk[b], r1 = p.rowblk[b + 1];
+  const int t = threadIdx.x;
+  const int j0 = p.rowptr[r0], j1 = p.rowptr[r1];
+  const SpmmScalars c = spmm_scalars<MODE>(p);
+  double2 dot = d2(0.0, 0.0);
+
+  if (j1 - j0 <= TILE) {
+    const int r = r0 + t;
+    int a0 = 0, a1 = 0;
+    if (r < r1) {
+      a0 = p.rowptr[r] - j0;
+      a1 = p.rowptr[r + 1] - j0;
+    }
+    double a[PER];
+    int ci[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = j0 + t + u * kBlock;
+      a[u] = 0.0;
+      ci[u] = 0;
+      if (j < j1) {
+        a[u] = p.val[j];
+        ci[u] = p.col[j];
+      }
+    }
+    double2 xv[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = j0 + t + u * kBlock;
+      xv[u] = j < j1 ? p.x[ci[u]] : d2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = j0 + t + u * kBlock;
+      if (j < j1) prod[t + u * kBlock] = d2(a[u] * xv[u].x, a[u] * xv[u].y);
+    }
+    __syncthreads();
+    if (r < r1) {
+      double2 acc = d2(0.0, 0.0);
+      for (int i = a0; i < a1; ++i) {
+        const double2 q = prod[i];
+        acc.x += q.x;
+        acc.y += q.y;
+      }
+      spmm_row_out<MODE>(p, c, r, acc, dot);
+    }
+  } else {
+    double2 acc = d2(0.0, 0.0);
+    for (int j = j0 + t; j < j1; j += kBlock) {
+      const double av = p.val[j];
+      const double2 xv = p.x[p.col[j]];
+      acc.x = fma(av, xv.x, acc.x);
+      acc.y = fma(av, xv.y, acc.y);
+    }
+    acc = block_sum2(acc);
+    if (t == 0) spmm_row_out<MODE>(p, c, r0, acc, dot);
+  }
+
   if (MODE == kLast || MODE == kOnly) block_partial2(dot, p.partials);
 }
 

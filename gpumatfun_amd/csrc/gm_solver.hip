@@ -53,6 +53,10 @@ gm_status fail(gm_status st, const char* fmt, ...) {
 
 inline int cdiv_i(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+// LDS tile sizes (nonzeros) of the row-block SpMM schedule.
+constexpr int kNumTiles = 3;
+constexpr int kTiles[kNumTiles] = {512, 1024, 2048};
+
 }  // namespace
 
 struct gm_solver {
@@ -69,7 +73,13 @@ struct gm_solver {
   double ms[5] = {0.3, 6.0, 120.0, 150.0, 0.13};
   std::vector<double> gamma;
   // schedules
-  int lanes = 8, G_sp = 1, chunk_sp = 1, G_el = 1, chunk_el = 1, G_df = 1, chunk_df = 1;
+  // SpMM schedule: tile > 0 selects the row-block kernel with that many
+  // nonzeros per LDS tile (row blocks rowblk[tile_idx]); tile == 0 the
+  // row-group kernel with `lanes` lanes per row.
+  int lanes = 8, tile = 0, tile_idx = -1;
+  int G_sp = 1, chunk_sp = 1, G_el = 1, chunk_el = 1, G_df = 1, chunk_df = 1;
+  int* rowblk[kNumTiles] = {};
+  int nrowblk[kNumTiles] = {};
   // device buffers
   int* rowptr = nullptr;
   int* col = nullptr;
@@ -123,6 +133,8 @@ gm_status upload(T** p, const T* src, size_t count, hipStream_t st) {
 void free_all(gm_solver* s) {
   for (auto& g : s->graph)
     if (g) hipGraphExecDestroy(g);
+  for (int* rb : s->rowblk)
+    if (rb) hipFree(rb);
   void* bufs[] = {s->rowptr, s->col, s->val, s->U, s->Usol, s->X, s->upd, s->W[0], s->W[1],
                   s->T[0], s->T[1], s->V, s->Z, s->C, s->Q, s->el, s->pl, s->mass, s->ode,
                   s->aabb, s->Yr, s->Yi, s->wts1, s->wts2, s->shifts, s->gsr, s->gsi,
@@ -139,6 +151,15 @@ using gm::SpmmArgs;
 template <int MODE>
 gm_status launch_spmm_mode(gm_solver* s, const SpmmArgs& a) {
   dim3 grid(s->G_sp), block(kBlock);
+  if (s->tile > 0) {
+    switch (s->tile) {
+      case 512: hipLaunchKernelGGL((gm::k_spmm_stream<512, MODE>), grid, block, 0, s->stream, a); break;
+      case 1024: hipLaunchKernelGGL((gm::k_spmm_stream<1024, MODE>), grid, block, 0, s->stream, a); break;
+      default: hipLaunchKernelGGL((gm::k_spmm_stream<2048, MODE>), grid, block, 0, s->stream, a); break;
+    }
+    GM_HIP(hipGetLastError());
+    return GM_OK;
+  }
   switch (s->lanes) {
     case 1: hipLaunchKernelGGL((gm::k_spmm<1, MODE>), grid, block, 0, s->stream, a); break;
     case 2: hipLaunchKernelGGL((gm::k_spmm<2, MODE>), grid, block, 0, s->stream, a); break;
@@ -164,7 +185,7 @@ gm_status launch_reduce(gm_solver* s, int G, int nv, double* out, int stride, in
   r.do_sqrt = do_sqrt;
   r.flag_zero = flag_zero;
   r.status = s->status;
-  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(kBlock), 0, s->stream, r);
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kReduceBlock), 0, s->stream, r);
   GM_HIP(hipGetLastError());
   return GM_OK;
 }
@@ -174,6 +195,7 @@ SpmmArgs spmm_base(gm_solver* s) {
   a.rowptr = s->rowptr;
   a.col = s->col;
   a.val = s->val;
+  a.rowblk = s->tile > 0 ? s->rowblk[s->tile_idx] : nullptr;
   a.nrows = s->n;
   a.chunk = s->chunk_sp;
   a.aabb = s->aabb;
@@ -184,11 +206,41 @@ SpmmArgs spmm_base(gm_solver* s) {
   return a;
 }
 
+// Row-group schedule with `lanes` lanes per row.
 void set_schedule(gm_solver* s, int lanes) {
   s->lanes = lanes;
+  s->tile = 0;
+  s->tile_idx = -1;
   const int groups = kBlock / lanes;
   s->G_sp = std::max(1, std::min(4096, cdiv_i(s->n, groups)));
   s->chunk_sp = cdiv_i(s->n, s->G_sp);
+}
+
+// Row-block schedule with kTiles[idx] nonzeros per LDS tile.
+void set_tile_schedule(gm_solver* s, int idx) {
+  s->tile = kTiles[idx];
+  s->tile_idx = idx;
+  s->G_sp = s->nrowblk[idx];
+  s->chunk_sp = 0;
+}
+
+// Cut rows 0..n-1 into consecutive blocks of at most kBlock rows and at most
+// `tile` nonzeros; a row with more than `tile` nonzeros is a block alone.
+std::vector<int> row_blocks(const std::vector<int>& rp, int n, int tile) {
+  std::vector<int> blk{0};
+  if (n == 0) blk.push_back(0);  // one empty block
+  int r = 0;
+  while (r < n) {
+    const int start = r;
+    const long long base = rp[r];
+    if (rp[r + 1] - base > tile) {
+      ++r;
+    } else {
+      while (r < n && r - start < kBlock && rp[r + 1] - base <= tile) ++r;
+    }
+    blk.push_back(r);
+  }
+  return blk;
 }
 
 // build_subspace.cu:5-91.  Iteration k reads w_k (U for k = 0) and writes
@@ -549,6 +601,15 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
   GM_TRY(upload(&s->rowptr, rp.data(), N + 1, st));
   GM_TRY(upload(&s->col, ci.data(), (size_t)s->nnz, st));
   GM_TRY(upload(&s->val, vv.data(), (size_t)s->nnz, st));
+  int max_blocks = 1;
+  for (int i = 0; i < kNumTiles; ++i) {
+    const std::vector<int> blk = row_blocks(rp, s->n, kTiles[i]);
+    s->nrowblk[i] = (int)blk.size() - 1;
+    max_blocks = std::max(max_blocks, s->nrowblk[i]);
+    GM_TRY(upload(&s->rowblk[i], blk.data(), blk.size(), st));
+    // upload() returns before the copy ends; blk must outlive it
+    GM_HIP(hipStreamSynchronize(st));
+  }
 
   // U (n x 2 column-major) -> interleaved, column 1 scaled by dt
   // (lanczos_impl.cu:79); Usol = U (lanczos_impl.cu:81-82).
@@ -606,12 +667,18 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
   const double avg = s->n ? (double)s->nnz / s->n : 1.0;
   int lanes = 1;
   while (lanes < 64 && lanes < avg) lanes <<= 1;
-  set_schedule(s, lanes);
+  const bool blocks = (d->flags & GM_FLAG_ROW_BLOCKS) ||
+                      (!(d->flags & GM_FLAG_ROW_GROUPS) && avg <= 16.0);
+  if (blocks)
+    set_tile_schedule(s, 1);  // row blocks of 1024 nonzeros
+  else
+    set_schedule(s, lanes);
   s->G_el = std::max(1, std::min(4096, cdiv_i(s->n, kBlock)));
   s->chunk_el = cdiv_i(s->n, s->G_el);
   s->G_df = std::max(1, std::min(512, cdiv_i(s->n, kBlock)));
   s->chunk_df = cdiv_i(s->n, s->G_df);
-  const size_t npart = std::max<size_t>(2 * 4096, (size_t)s->G_df * 2 * std::max(1, s->K));
+  const size_t npart = std::max<size_t>(2 * (size_t)std::max(4096, max_blocks),
+                                        (size_t)s->G_df * 2 * std::max(1, s->K));
   GM_TRY(dalloc(&s->partials, npart));
   GM_TRY(dalloc(&s->counter, 1));
   GM_TRY(dalloc(&s->status, 1));
@@ -627,15 +694,20 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
     // Time the build-subspace SpMM for each lane width on this matrix and
     // keep the fastest.  Each candidate runs a short Lanczos run; the solver
     // state is rebuilt by the next build_subspace anyway.
-    const int cands[] = {1, 2, 4, 8, 16, 32};
+    // Candidates: row groups of 1..32 lanes (c = lanes), row blocks of each
+    // tile size (c = -1 - tile index).
+    const int cands[] = {1, 2, 4, 8, 16, 32, -1, -2, -3};
     float best = 1e30f;
-    int best_l = s->lanes;
+    int best_c = s->tile > 0 ? -1 - s->tile_idx : s->lanes;
     hipEvent_t e0, e1;
     GM_HIP(hipEventCreate(&e0));
     GM_HIP(hipEventCreate(&e1));
     for (int l : cands) {
-      if (l > 4 * avg + 4) continue;
-      set_schedule(s, l);
+      if (l > 0 && l > 4 * avg + 4) continue;
+      if (l > 0)
+        set_schedule(s, l);
+      else
+        set_tile_schedule(s, -1 - l);
       GM_TRY(enqueue_build(s));  // warm-up
       GM_HIP(hipEventRecord(e0, st));
       GM_TRY(enqueue_build(s));
@@ -643,11 +715,14 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
       GM_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
       GM_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) { best = ms; best_l = l; }
+      if (ms < best) { best = ms; best_c = l; }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    set_schedule(s, best_l);
+    if (best_c > 0)
+      set_schedule(s, best_c);
+    else
+      set_tile_schedule(s, -1 - best_c);
     GM_HIP(hipMemsetAsync(s->status, 0, sizeof(unsigned), st));
     GM_HIP(hipStreamSynchronize(st));
   }
@@ -889,7 +964,7 @@ gm_status gm_get_info(gm_solver* s, int64_t* n, int64_t* nnz, int32_t* m, int32_
   if (gamma_size) *gamma_size = s->nd;
   if (contour_size) *contour_size = s->ns;
   if (num_eigs) *num_eigs = s->K;
-  if (spmm_lanes) *spmm_lanes = s->lanes;
+  if (spmm_lanes) *spmm_lanes = s->tile > 0 ? -s->tile : s->lanes;
   return GM_OK;
 }
 

@@ -35,12 +35,19 @@ def _ptr(a):
 
 
 class LanczosEngine:
-    """One device-resident f(A)b problem (``lanczosEngine.m``)."""
+    """One device-resident f(A)b problem (``lanczosEngine.m``).
+
+    Keyword options beyond the reference's 26 inputs: ``clamp`` (the u >= 0
+    clamp of ``update_U.cu:9``), ``autotune`` (time every SpMM schedule on A
+    at create and keep the fastest), ``schedule`` (without autotune: None
+    picks row blocks when rows average <= 16 nonzeros and row groups
+    otherwise; ``"rows"`` or ``"blocks"`` forces one), ``device``.
+    """
 
     def __init__(self, U, A, Q, lambda_, N, gamma, gamma_s_real, gamma_s_imag,
                  exp_lambda, phi_lambda, dt, ode, aode, bode, code, vrest, vamp, vth,
                  vpeak, c1ion, c2ion, mass, wts1, wts2, shifts, valvec=(-1.0, 0.0, 1.0),
-                 *, clamp=True, autotune=False, device=0):
+                 *, clamp=True, autotune=False, schedule=None, device=0):
         lib = _lib.load()
         self._lib = lib
         self._h = C.c_void_p()
@@ -140,7 +147,11 @@ class LanczosEngine:
         d.wts2 = _ptr(w2f)
         d.shifts = _ptr(sh)
         d.valvec = _ptr(vv)
-        d.flags = (0 if clamp else _lib.GM_FLAG_NO_CLAMP) | (_lib.GM_FLAG_AUTOTUNE if autotune else 0)
+        sched = {None: 0, "rows": _lib.GM_FLAG_ROW_GROUPS, "blocks": _lib.GM_FLAG_ROW_BLOCKS}
+        if schedule not in sched:
+            raise GpuMatfunError(_lib.GM_EINVAL, "schedule must be None, 'rows' or 'blocks'")
+        d.flags = ((0 if clamp else _lib.GM_FLAG_NO_CLAMP)
+                   | (_lib.GM_FLAG_AUTOTUNE if autotune else 0) | sched[schedule])
         d.device = int(device)
         _lib.check(lib.gm_lanczos_create(C.byref(d), C.byref(self._h)))
         self._keep = None
@@ -252,8 +263,11 @@ class LanczosEngine:
         m = C.c_int32(); nd = C.c_int32(); ns = C.c_int32(); K = C.c_int32(); lanes = C.c_int32()
         self._call(self._lib.gm_get_info, C.byref(n), C.byref(nnz), C.byref(m), C.byref(nd),
                    C.byref(ns), C.byref(K), C.byref(lanes))
+        lv = lanes.value
+        sched = f"k_spmm<{lv}>" if lv > 0 else f"k_spmm_stream<{-lv}>"
         return {"n": n.value, "nnz": nnz.value, "m": m.value, "gamma_size": nd.value,
-                "contour_size": ns.value, "num_eigs": K.value, "spmm_lanes": lanes.value}
+                "contour_size": ns.value, "num_eigs": K.value, "spmm_lanes": lv,
+                "spmm_schedule": sched}
 
     def stream(self):
         p = C.c_void_p()

@@ -47,8 +47,13 @@ enum {
 enum {
     GM_FLAG_NO_CLAMP = 1u << 0,  /* update_U without the u >= 0 clamp of
                                     update_U.cu:9 (the CPU path's semantics) */
-    GM_FLAG_AUTOTUNE = 1u << 1   /* time the SpMM row-to-lane schedules on A at
+    GM_FLAG_AUTOTUNE = 1u << 1,  /* time the SpMM row-to-lane schedules on A at
                                     create and keep the fastest              */
+    GM_FLAG_ROW_GROUPS = 1u << 2, /* without AUTOTUNE: the row-group SpMM
+                                    (lanes per row from the mean row length) */
+    GM_FLAG_ROW_BLOCKS = 1u << 3  /* without AUTOTUNE: the row-block SpMM with
+                                    1024-nonzero LDS tiles.  Neither flag: row
+                                    blocks when rows average <= 16 nonzeros */
 };
 
 /*
@@ -169,7 +174,9 @@ gm_status gm_step(gm_solver* s, int32_t nsteps, int32_t monodomain, int32_t use_
  * vectors are set to zero instead of propagating NaN).                     */
 gm_status gm_synchronize(gm_solver* s);
 
-/* Introspection: sizes and the chosen SpMM schedule (lanes per row).      */
+/* Introspection: sizes and the chosen SpMM schedule: spmm_lanes > 0 is the
+ * row-group kernel with that many lanes per row, spmm_lanes < 0 the
+ * row-block kernel with -spmm_lanes nonzeros per LDS tile.                 */
 gm_status gm_get_info(gm_solver* s, int64_t* n, int64_t* nnz, int32_t* m,
                       int32_t* gamma_size, int32_t* contour_size,
                       int32_t* num_eigs, int32_t* spmm_lanes);

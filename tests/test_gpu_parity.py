@@ -281,3 +281,65 @@ def test_mitchell_schaeffer_vs_oracle():
     assert rel(U[:, 1], dt * gu) < TOL
     assert rel(h, ho) < TOL
     eng.delete()
+
+
+@pytest.mark.parametrize("schedule", ["rows", "blocks"])
+def test_spmm_schedules_vs_cpu_path(schedule):
+    """Both SpMM schedules (row groups, LDS row blocks) on the FEM case with
+    deflation and a degree-2 polynomial, against the CPU step loop."""
+    inp = fem_case(m=30)
+    print("case fem m=30 schedule", schedule, flush=True)
+    eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=schedule)
+    assert (eng.info()["spmm_lanes"] > 0) == (schedule == "rows")
+    eng.step(3)
+    U = eng.get_U()
+    S = oracle_setup(inp)
+    u, gu = oracle.cpu_steps_nonuniform(S, inp.U[:, 0], inp.U[:, 1], 3)
+    assert rel(U[:, 0], u) < TOL
+    assert rel(U[:, 1], inp.dt * gu) < TOL
+    eng.delete()
+
+
+def _arrow_matrix(N=3000, wide=(100, 200), width=600, eps=0.01):
+    """SPD: 1D Laplacian plus a dense first row/column (N nonzeros, longer than
+    any LDS tile) and rows `wide` coupled to `width` columns, diagonally
+    dominant."""
+    import scipy.sparse as sp
+    L = sp.diags([-np.ones(N - 1), 2 * np.ones(N), -np.ones(N - 1)], [-1, 0, 1]).tolil()
+    L[0, 1:] = eps
+    L[1:, 0] = eps
+    for r in wide:
+        cols = np.arange(r + 2, r + 2 + width)
+        L[r, cols] = eps
+        L[cols, r] = eps
+    A = L.tocsr()
+    A = A + sp.diags(np.asarray(abs(A).sum(axis=1)).ravel() - A.diagonal() + 0.5)
+    return sp.csr_matrix(A)
+
+
+@pytest.mark.parametrize("schedule", ["rows", "blocks", "autotune"])
+def test_long_rows_vs_cpu_path(schedule):
+    """A row longer than every LDS tile (the row-block kernel's whole-block
+    branch) and rows of 600 nonzeros among rows of 3."""
+    from gpumatfun_amd import quadrature as q
+    A = _arrow_matrix()
+    N = A.shape[0]
+    assert np.diff(A.indptr).max() > 2048
+    w = np.linalg.eigvalsh(A.toarray())
+    assert w.min() > 0
+    rng = np.random.default_rng(11)
+    u0 = rng.random(N)
+    g0 = u0 * (1 - u0)
+    inp = q.make_inputs(A, u0, g0, m=30, n=16, eta=1e-2, alpha=1.5, dt=0.1,
+                        l1=float(w.min()), lN=float(w.max()), degree=1)
+    print("case arrow N=3000 schedule", schedule, flush=True)
+    kw = {"autotune": True} if schedule == "autotune" else {"schedule": schedule}
+    eng = LanczosEngine.from_inputs(inp, clamp=False, **kw)
+    eng.step(2)
+    U = eng.get_U()
+    S = oracle_setup(inp)
+    u, gu = oracle.cpu_steps(S, u0, g0, 2)
+    print("rel err", rel(U[:, 0], u), rel(U[:, 1], 0.1 * gu), eng.info()["spmm_schedule"])
+    assert rel(U[:, 0], u) < TOL
+    assert rel(U[:, 1], 0.1 * gu) < TOL
+    eng.delete()
