@@ -35,6 +35,28 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def kernel_symbol(info, mode):
+    """Demangled name of the SpMM kernel the solver runs in `mode` (1 = kMid, 2 = kLast)."""
+    lanes = info["spmm_lanes"]
+    if lanes > 0:
+        return f"void gm::k_spmm<{lanes}, {mode}>(gm::SpmmArgs)"
+    return f"void gm::k_spmm_stream<{-lanes}, {mode}>(gm::SpmmArgs)"
+
+
+def measured_traffic(workload, symbol):
+    """HBM bytes per launch of `symbol` on `workload` from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, each scaled by what
+    the counter reads for a copy kernel of known size), or None when no
+    summary covers this kernel."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh)
+    k = rec.get("kernels", {}).get(symbol)
+    return None if k is None else int(k["traffic_bytes"])
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -273,8 +295,8 @@ def main():
         "setup_s": round(setup_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "kernel": "%s, %s" % (info["spmm_schedule"], "kMid" if mode == 0 else "kLast"),
+                     "traffic": measured_traffic(cfg["workload"], kernel_symbol(info, 1 if mode == 0 else 2)),
+                     "kernel": kernel_symbol(info, 1 if mode == 0 else 2),
                      "avg_launch_us": round(spmm_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(bytes_per)},
         "copy_kernel_gbs": round(copy_bytes / (copy_ms * 1e-3) / 1e9, 1),
@@ -338,11 +360,12 @@ def run_batch(args, steps, warm, use_graph):
         "plain_same_matrix": {"value": round(plain, 3), "unit": "solves/s",
                               "spmm_schedule": einfo["spmm_schedule"]},
         "batched_vs_plain": round(value / plain, 3),
-        "roofline": {"bound": "hbm", "kernel": "plain k_spmm kLast on the same matrix",
+        "roofline": {"bound": "hbm", "kernel": kernel_symbol(einfo, 2) + " (plain path, same matrix)",
                      "achieved": round(bytes_per / (spmm_ms * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_per / (spmm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_us": round(spmm_ms * 1e3, 3)},
+                     "traffic": measured_traffic("laplace3d_128_plain", kernel_symbol(einfo, 2)),
+                     "avg_launch_us": round(spmm_ms * 1e3, 3)},
         "cpu_baseline": None,
     }
     if not args.no_cpu and not args.profile_only:

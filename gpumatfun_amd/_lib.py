@@ -32,6 +32,8 @@ EXPORTS = (
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
     "gm_fab_last_error", "gm_fab_create", "gm_fab_delete", "gm_fab_set_matrix",
     "gm_fab_set_weights", "gm_fab_forward", "gm_fab_backward",
+    "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
+    "gm_subspace_size",
 )
 GM_FAB_RECOMPUTE = 1
 
@@ -191,6 +193,14 @@ def load():
         "gm_fab_set_weights": (C.c_int, [vp, _dp, _dp]),
         "gm_fab_forward": (C.c_int, [vp, _dp, _dp]),
         "gm_fab_backward": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "gm_polynomial_coeffs": (C.c_int, [C.c_int32, C.c_double, _dp]),
+        "gm_contour_setup": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_double,
+                                       C.c_double, _dp, _dp, _dp, _dp, _dp]),
+        "gm_shift_setup": (C.c_int, [_dp, C.c_int32, _dp, C.c_int32, C.c_int32, _dp, _dp,
+                                     _dp, _dp]),
+        "gm_eig_weights": (C.c_int, [_dp, C.c_int32, C.c_double, C.c_double, _dp, _dp]),
+        "gm_subspace_size": (C.c_int, [_dp, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                       C.POINTER(C.c_int32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

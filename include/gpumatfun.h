@@ -291,6 +291,37 @@ gm_status gm_fab_forward(gm_fab* f, const double* b, double* u);
 gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, double* grad_b,
                           double* grad_kappa, double* grad_alpha);
 
+/* ---------------------------------------------------------------------------
+ * Host set-up of the create inputs (host code only; usable without a GPU).
+ * What problem_startup.m:71-144 computes in MATLAB before lanczos_create.
+ * Complex arrays are [re im] interleaved unless stated; wts1/wts2 are n x 2
+ * column-major [Re Im] as lanczos_create takes them.
+ * ------------------------------------------------------------------------- */
+/* gamma (degree+1), highest power first: polynomial_coeffs.m:11-28.        */
+gm_status gm_polynomial_coeffs(int32_t degree, double prob_dim, double* gamma);
+/* HHT nodes xi (n complex), dxi/dt (n complex, may be NULL), weights
+ * s f1(xi) dxi/dt and s f2(xi) dxi/dt (n x 2 each), s (may be NULL), for
+ * f1 = exp(-eta x^(alpha/2)), f2 = phi(-eta x^(alpha/2)):
+ * problem_startup.m:72-74, 80-103 (n is an input here; the reference picks
+ * it with fminbnd at problem_startup.m:92).                                */
+gm_status gm_contour_setup(double l1, double lN, int32_t n, double eta, double alpha,
+                           double* xi, double* dxidt, double* wts1, double* wts2,
+                           double* s);
+/* gamma_s_real/imag ((d+1) x n column-major), etaxi (n complex, may be
+ * NULL) and the packed shifts vector (4 m n doubles, may be NULL):
+ * problem_startup.m:106-123, 140-144.                                      */
+gm_status gm_shift_setup(const double* gamma, int32_t gamma_size, const double* xi,
+                         int32_t n, int32_t m, double* gamma_s_real, double* gamma_s_imag,
+                         double* etaxi, double* shifts);
+/* exp_lambda, phi_lambda (K) with phi_lambda[0] = 1: problem_startup.m:72-77. */
+gm_status gm_eig_weights(const double* lambda, int32_t K, double eta, double alpha,
+                         double* exp_lambda, double* phi_lambda);
+/* Subspace size m = ceil(log(tol/2) / log(k)), k from the extreme values of
+ * x p(x) on [l1, lN]: problem_startup_nonuniform.m:101-113 (real critical
+ * points only).                                                            */
+gm_status gm_subspace_size(const double* gamma, int32_t gamma_size, double l1, double lN,
+                           double tol, int32_t* m);
+
 #ifdef __cplusplus
 }
 #endif
