@@ -218,6 +218,9 @@ def main():
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "rows", "blocks"],
+                    help="SpMM schedule of the plain path: autotuned at create (default), "
+                         "or forced (fixed kernels for counter runs)")
     ap.add_argument("--profile-only", action="store_true",
                     help="a few steps and the kernel timings only (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -252,9 +255,12 @@ def main():
     inp, cfg, setup_s = (build_fem1m if args.config == "fem1m" else build_dirichlet128)(args)
     log(f"setup {setup_s:.1f}s")
     t0 = time.time()
-    eng = LanczosEngine.from_inputs(inp, clamp=False, autotune=True)
+    if args.schedule == "auto":
+        eng = LanczosEngine.from_inputs(inp, clamp=False, autotune=True)
+    else:
+        eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=args.schedule)
     info = eng.info()
-    log(f"create {time.time() - t0:.2f}s, lanes={info['spmm_lanes']}")
+    log(f"create {time.time() - t0:.2f}s, schedule {info['spmm_schedule']}")
 
     # parity leg: one step from the initial state
     eng.step(1, use_graph=False)

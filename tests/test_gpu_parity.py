@@ -60,7 +60,7 @@ def test_dirichlet128_one_step_stages():
 
 
 def test_dirichlet128_ten_steps_vs_cpu_path():
-    """10 steps against problem_startup_CPU.m's loop (clamp off)."""
+    """128^2 Dirichlet, m = 50, 32 shifts: 10 steps against problem_startup_CPU.m's loop (clamp off)."""
     inp = dirichlet_case()
     print("case dirichlet128 10 steps", flush=True)
     eng = LanczosEngine.from_inputs(inp, clamp=False)
