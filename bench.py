@@ -131,6 +131,93 @@ def build_batch3d(args):
     return inp, U, cfg, time.time() - t0
 
 
+def build_uneven64(args):
+    """64 independent 2D 5-point Dirichlet Laplacians, sides log-uniform in
+    [32, 448] (seed 0), m = 40, 32 contour nodes each on its own spectrum."""
+    from gpumatfun_amd import problems, quadrature as q
+    t0 = time.time()
+    rng = np.random.default_rng(0)
+    sides = np.rint(np.exp(rng.uniform(np.log(32), np.log(448), args.problems))).astype(int)
+    inps = []
+    for p, side in enumerate(sides):
+        A = problems.laplacian_2d_dirichlet(int(side))
+        lam = problems.dirichlet_eigs_2d(int(side))
+        u0 = np.random.default_rng(1000 + p).standard_normal(A.shape[0])
+        inps.append(q.make_inputs(A, u0, u0 * (1 - u0), m=40, n=32, eta=1e-3, alpha=1.5,
+                                  dt=1e-3, l1=float(lam.min()), lN=float(lam.max()), degree=0))
+    rows = int(sum(i.A.shape[0] for i in inps))
+    cfg = {"workload": f"uneven_laplace2d_{len(inps)}", "problems": len(inps),
+           "sides": "log-uniform [32, 448], seed 0", "side_min": int(sides.min()),
+           "side_max": int(sides.max()), "total_rows": rows,
+           "total_nnz": int(sum(i.A.nnz for i in inps)), "m": 40, "poly_degree": 0,
+           "shifts": 32, "deflation": 0, "alpha": 1.5, "tau": 1e-3,
+           "columns_per_step": 2 * len(inps), "rhs_seeds": "1000.."}
+    return inps, cfg, time.time() - t0
+
+
+def run_uneven(args, steps, warm, use_graph):
+    from gpumatfun_amd.engine import LanczosEngine
+    from gpumatfun_amd.ubatch import UnevenBatch
+    inps, cfg, setup_s = build_uneven64(args)
+    P = len(inps)
+    log(f"setup {setup_s:.1f}s, {P} problems, {cfg['total_rows']} rows")
+    ub = UnevenBatch.from_inputs(inps, clamp=False)
+    ub.step(1, use_graph=False)
+    ub.synchronize()
+    U1 = [ub.get_U(p) for p in range(P)]
+    elapsed = timed(lambda k: ub.step(k, use_graph=use_graph), ub.synchronize, steps, warm)
+    value = 2.0 * P * steps / elapsed
+    log(f"uneven batch: {steps} steps in {elapsed:.4f}s -> {value:.1f} solves/s")
+    info = ub.info()
+    ub.delete()
+    # the plain path: one engine per problem, each stepped in turn (graph replays)
+    engs = [LanczosEngine.from_inputs(inp, clamp=False, schedule="blocks") for inp in inps]
+
+    def plain_step(k):
+        for _ in range(k):
+            for e in engs:
+                e.step(1, use_graph=use_graph)
+
+    def plain_sync():
+        for e in engs:
+            e.synchronize()
+
+    pel = timed(plain_step, plain_sync, steps, warm)
+    plain = 2.0 * P * steps / pel
+    for e in engs:
+        e.delete()
+    log(f"plain, problem by problem: {plain:.1f} solves/s; ratio {value / plain:.2f}")
+    result = {
+        "metric": "f(A)b solves/sec (FP64, uneven batch)",
+        "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "ms_per_step": round(1000.0 * elapsed / steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded right-hand sides)", "config": cfg, "graph": use_graph,
+        "row_blocks": info["row_blocks"],
+        "plain_same_problems": {"value": round(plain, 3), "unit": "solves/s",
+                                "how": "one plain engine per problem, stepped in turn"},
+        "batched_vs_plain": round(value / plain, 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    if not args.no_cpu and not args.profile_only:
+        threads = min(16, os.cpu_count() or 1)
+        worst, cpu_s = 0.0, 0.0
+        for p, inp in enumerate(inps):
+            u, gu, t, used = oracle_one_step(inp, inp.U[:, 0], inp.U[:, 1], threads)
+            cpu_s += t
+            worst = max(worst, rel(U1[p][:, 0], u), rel(U1[p][:, 1], inp.dt * gu))
+        result["rel_err_vs_oracle"] = {"max_over_problems": worst, "steps": 1,
+                                       "tolerance": 1e-10}
+        result["cpu_baseline"] = {"value": round(2.0 * P / cpu_s, 6), "unit": "solves/s",
+                                  "cores": used, "kind": "port",
+                                  "sample": f"one time step of each of the {P} problems by the "
+                                            "oracle's C/OpenMP restatement (oracle/c/fab_oracle.c)",
+                                  "seconds_per_step": round(cpu_s, 4),
+                                  "host_cpus": os.cpu_count()}
+        log(f"oracle: {cpu_s:.2f}s per step of all problems, worst rel err {worst:.3e}")
+    print(json.dumps(result), flush=True)
+
+
 def oracle_setup_dict(inp, A=None, perm=None):
     S = {"A": inp.A if A is None else A, "Q": inp.Q if inp.Q.shape[1] else None,
          "exp_lambda": inp.exp_lambda, "phi_lambda": inp.phi_lambda, "dt": inp.dt,
@@ -212,7 +299,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--config", default="fem1m", choices=["fem1m", "dirichlet128", "batch3d"])
+    ap.add_argument("--config", default="fem1m",
+                    choices=["fem1m", "dirichlet128", "batch3d", "uneven64"])
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--side3d", type=int, default=128)
     ap.add_argument("--problems", type=int, default=256)
@@ -250,6 +338,11 @@ def main():
 
     if args.config == "batch3d":
         run_batch(args, steps, warm, use_graph)
+        return
+    if args.config == "uneven64":
+        if args.problems == 256:
+            args.problems = 64
+        run_uneven(args, steps, warm, use_graph)
         return
 
     inp, cfg, setup_s = (build_fem1m if args.config == "fem1m" else build_dirichlet128)(args)

@@ -32,6 +32,8 @@ EXPORTS = (
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
     "gm_fab_last_error", "gm_fab_create", "gm_fab_delete", "gm_fab_set_matrix",
     "gm_fab_set_weights", "gm_fab_forward", "gm_fab_backward",
+    "gm_ubatch_last_error", "gm_ubatch_create", "gm_ubatch_delete", "gm_ubatch_step",
+    "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info",
     "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
     "gm_subspace_size",
 )
@@ -130,6 +132,33 @@ class GmFabDesc(C.Structure):
     ]
 
 
+class GmUbatchDesc(C.Structure):
+    """Mirror of ``gm_ubatch_desc``."""
+    _fields_ = [
+        ("num_problems", C.c_int32),
+        ("n", _ip),
+        ("nnz", C.POINTER(C.c_int64)),
+        ("sparse_format", C.c_int32),
+        ("A_values", C.POINTER(_dp)),
+        ("A_ptr", C.POINTER(_ip)),
+        ("A_ind", C.POINTER(_ip)),
+        ("U", C.POINTER(_dp)),
+        ("mass", C.POINTER(_dp)),
+        ("subspace_size", C.c_int32),
+        ("gamma_size", C.c_int32),
+        ("gamma", _dp),
+        ("contour_size", C.c_int32),
+        ("gamma_s_real", _dp),
+        ("gamma_s_imag", _dp),
+        ("wts1", _dp),
+        ("wts2", _dp),
+        ("shifts", _dp),
+        ("dt", C.c_double),
+        ("flags", C.c_uint32),
+        ("device", C.c_int32),
+    ]
+
+
 class GpuMatfunError(RuntimeError):
     """An entry point returned a non-OK gm_status."""
 
@@ -193,6 +222,14 @@ def load():
         "gm_fab_set_weights": (C.c_int, [vp, _dp, _dp]),
         "gm_fab_forward": (C.c_int, [vp, _dp, _dp]),
         "gm_fab_backward": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "gm_ubatch_last_error": (C.c_char_p, []),
+        "gm_ubatch_create": (C.c_int, [C.POINTER(GmUbatchDesc), C.POINTER(vp)]),
+        "gm_ubatch_delete": (C.c_int, [vp]),
+        "gm_ubatch_step": (C.c_int, [vp, C.c_int32, C.c_int32]),
+        "gm_ubatch_get_U": (C.c_int, [vp, C.c_int32, _dp]),
+        "gm_ubatch_synchronize": (C.c_int, [vp]),
+        "gm_ubatch_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int32)]),
         "gm_polynomial_coeffs": (C.c_int, [C.c_int32, C.c_double, _dp]),
         "gm_contour_setup": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_double,
                                        C.c_double, _dp, _dp, _dp, _dp, _dp]),
@@ -210,10 +247,12 @@ def load():
     return lib
 
 
-def check(status, batch=False, fab=False):
+def check(status, batch=False, fab=False, ubatch=False):
     if status != GM_OK:
         lib = load()
-        if fab:
+        if ubatch:
+            raw = lib.gm_ubatch_last_error()
+        elif fab:
             raw = lib.gm_fab_last_error()
         elif batch:
             raw = lib.gm_batch_last_error()

@@ -50,12 +50,16 @@ __device__ __forceinline__ double2 block_sum2(double2 v) {
 // boundary is the cheapest cross-XCD synchronisation (an in-kernel ticket
 // needs a device-scope release fence per block, i.e. an L2 write-back on
 // every XCD).
-__device__ __forceinline__ void block_partial2(double2 v, double* partials) {
+__device__ __forceinline__ void block_partial2_at(double2 v, double* partials, int idx) {
   const double2 bs = block_sum2(v);
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = bs.x;
-    partials[2 * blockIdx.x + 1] = bs.y;
+    partials[2 * idx] = bs.x;
+    partials[2 * idx + 1] = bs.y;
   }
+}
+
+__device__ __forceinline__ void block_partial2(double2 v, double* partials) {
+  block_partial2_at(v, partials, blockIdx.x);
 }
 
 // Sums nv values over G blocks of partials ([G][nv]) in a fixed order and
@@ -66,6 +70,8 @@ __device__ __forceinline__ void block_partial2(double2 v, double* partials) {
 // bits: 4 = non-finite result, 1 = exact zero (when flag_zero).
 constexpr int kReduceBlock = 1024;
 
+// Segmented form (seg != nullptr): one block per segment q = blockIdx.x sums
+// partials seg[q] .. seg[q+1]-1 into out + q * out_pstride.
 struct ReduceArgs {
   const double* partials;
   int G, nv;
@@ -73,6 +79,8 @@ struct ReduceArgs {
   int stride;
   int do_sqrt, flag_zero;
   unsigned* status;
+  const int* seg;
+  size_t out_pstride;
 };
 
 // (a template so that both translation units may instantiate it)
@@ -80,6 +88,15 @@ template <int kUnused = 0>
 __global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs p) {
   __shared__ double sub[kReduceBlock];
   const int t = threadIdx.x;
+  const double* part = p.partials;
+  double* out = p.out;
+  int G = p.G;
+  if (p.seg) {
+    const int q = blockIdx.x, g0 = p.seg[q];
+    part += (size_t)g0 * p.nv;
+    G = p.seg[q + 1] - g0;
+    out += (size_t)q * p.out_pstride;
+  }
   for (int v0 = 0; v0 < p.nv; v0 += kReduceBlock) {
     const int W = min(p.nv - v0, kReduceBlock);
     const int ns = kReduceBlock / W;
@@ -88,13 +105,13 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs p) {
     if (sb < ns) {
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
       int b = sb;
-      for (; b + 3 * ns < p.G; b += 4 * ns) {
-        s0 += p.partials[(size_t)b * p.nv + v];
-        s1 += p.partials[(size_t)(b + ns) * p.nv + v];
-        s2 += p.partials[(size_t)(b + 2 * ns) * p.nv + v];
-        s3 += p.partials[(size_t)(b + 3 * ns) * p.nv + v];
+      for (; b + 3 * ns < G; b += 4 * ns) {
+        s0 += part[(size_t)b * p.nv + v];
+        s1 += part[(size_t)(b + ns) * p.nv + v];
+        s2 += part[(size_t)(b + 2 * ns) * p.nv + v];
+        s3 += part[(size_t)(b + 3 * ns) * p.nv + v];
       }
-      for (; b < p.G; b += ns) s0 += p.partials[(size_t)b * p.nv + v];
+      for (; b < G; b += ns) s0 += part[(size_t)b * p.nv + v];
       s = (s0 + s1) + (s2 + s3);
     }
     __syncthreads();
@@ -117,7 +134,7 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs p) {
             atomicOr(p.status, 1u);
         }
       }
-      p.out[(size_t)v * p.stride] = r;
+      out[(size_t)v * p.stride] = r;
     }
   }
 }

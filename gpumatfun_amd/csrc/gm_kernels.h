@@ -20,6 +20,9 @@
 #pragma once
 #include "gm_device.h"
 
+// Every kernel here is a template (kUnused where nothing else varies) so that
+// several translation units may include this header without duplicate symbols.
+
 
 namespace gm {
 
@@ -43,6 +46,8 @@ struct SpmmArgs {
   const int* __restrict__ col;
   const double* __restrict__ val;
   const int* __restrict__ rowblk;  // row-block bounds (k_spmm_stream only)
+  const int* __restrict__ blk_prob;  // SEG: problem of each row block
+  int aabb_stride;             // SEG: doubles between problems' aabb
   int nrows;
   int chunk;                   // rows per block (k_spmm only)
   const double2* x;            // gather source
@@ -67,18 +72,18 @@ struct SpmmScalars {
 };
 
 template <int MODE>
-__device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p) {
+__device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p, const double* aabb) {
   SpmmScalars c;
   c.inv = d2(1.0, 1.0);
   c.bet = d2(0.0, 0.0);
   c.bprev = d2(0.0, 0.0);
   if (MODE == kFirst || MODE == kOnly) {
-    const double b0 = p.aabb[p.k + 2 * p.ld], b1 = p.aabb[p.k + 3 * p.ld];
+    const double b0 = aabb[p.k + 2 * p.ld], b1 = aabb[p.k + 3 * p.ld];
     c.inv = d2(safe_inv(b0), safe_inv(b1));
     c.bet = d2(b0, b1);
   }
   if ((MODE == kLast || MODE == kOnly) && p.k > 0)
-    c.bprev = d2(p.aabb[p.k + 2 * p.ld], p.aabb[p.k + 3 * p.ld]);
+    c.bprev = d2(aabb[p.k + 2 * p.ld], aabb[p.k + 3 * p.ld]);
   c.sx = (MODE == kFirst || MODE == kOnly) ? d2(p.g0 * c.inv.x, p.g0 * c.inv.y) : d2(1.0, 1.0);
   return c;
 }
@@ -138,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
   const int row1 = min(row0 + p.chunk, p.nrows);
   const int sub = threadIdx.x % LANES;
   const int grp = threadIdx.x / LANES;
-  const SpmmScalars c = spmm_scalars<MODE>(p);
+  const SpmmScalars c = spmm_scalars<MODE>(p, p.aabb);
   double2 dot = d2(0.0, 0.0);
 
   for (int r = row0 + grp; r < row1; r += GROUPS) {
@@ -172,7 +177,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
 // column order and runs the epilogue; the row-aligned loads and stores of
 // the epilogue are coalesced too.  One long row is summed by the whole block
 // instead (strided FMAs, then a fixed-order block tree).
-template <int TILE, int MODE>
+// SEG: the rows belong to independent problems, no block straddles two, and
+// each block takes its Lanczos scalars from its problem's aabb and writes its
+// partial at its row-block index (so a problem's partials are contiguous).
+template <int TILE, int MODE, bool SEG = false>
 __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
   static_assert(TILE % kBlock == 0, "TILE must be a multiple of the block size");
   constexpr int PER = TILE / kBlock;
@@ -181,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
   const int r0 = p.rowblk[b], r1 = p.rowblk[b + 1];
   const int t = threadIdx.x;
   const int j0 = p.rowptr[r0], j1 = p.rowptr[r1];
-  const SpmmScalars c = spmm_scalars<MODE>(p);
+  const double* aabb = SEG ? p.aabb + (size_t)p.blk_prob[b] * p.aabb_stride : p.aabb;
+  const SpmmScalars c = spmm_scalars<MODE>(p, aabb);
   double2 dot = d2(0.0, 0.0);
 
   if (j1 - j0 <= TILE) {
@@ -236,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
     if (t == 0) spmm_row_out<MODE>(p, c, r0, acc, dot);
   }
 
-  if (MODE == kLast || MODE == kOnly) block_partial2(dot, p.partials);
+  if (MODE == kLast || MODE == kOnly) block_partial2_at(dot, p.partials, SEG ? b : (int)blockIdx.x);
 }
 
 // w <- w - alpha_k v_k (in place), beta_{k+1} = ||w||.   build_subspace.cu:63-76
@@ -250,6 +259,7 @@ struct FinishArgs {
   unsigned* status;
 };
 
+template <int kUnused = 0>
 __global__ __launch_bounds__(kBlock) void k_lanczos_finish(FinishArgs p) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int row0 = b * p.chunk;
@@ -291,12 +301,27 @@ struct SolveArgs {
   double* yr;           // m x 2ns column-major
   double* yi;
   unsigned* status;     // bit 1: singular pivot
+  // independent problems (blockIdx.y = problem): element strides between
+  // consecutive problems' aabb, wts1/wts2, shifts, yr/yi and workspaces
+  size_t aabb_ps, wts_ps, shifts_ps, y_ps, work_ps;
 };
 
+template <int kUnused = 0>
 __global__ __launch_bounds__(64) void k_shifted_solves(SolveArgs p) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int S = 2 * p.ns;
   if (s >= S) return;
+  const size_t q = blockIdx.y;
+  p.aabb += q * p.aabb_ps;
+  p.wts1 += q * p.wts_ps;
+  p.wts2 += q * p.wts_ps;
+  p.shifts += q * p.shifts_ps;
+  p.yr += q * p.y_ps;
+  p.yi += q * p.y_ps;
+  p.wD += q * p.work_ps;
+  p.wDU += q * p.work_ps;
+  p.wDU2 += q * p.work_ps;
+  p.wX += q * p.work_ps;
   const int c = s < p.ns ? 0 : 1;
   const int node = s - c * p.ns;
   const double* w = c == 0 ? p.wts1 : p.wts2;
@@ -386,9 +411,18 @@ struct CollapseArgs {
   const double* gsi;
   int m, ns, nd;       // nd = d + 1
   double2* C;          // [nd][m] double2 (x: exp, y: phi)
+  // independent problems (blockIdx.y = problem): strides of yr/yi, gsr/gsi, C
+  size_t y_ps, g_ps, c_ps;
 };
 
+template <int kUnused = 0>
 __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs p) {
+  const size_t q = blockIdx.y;
+  p.yr += q * p.y_ps;
+  p.yi += q * p.y_ps;
+  p.gsr += q * p.g_ps;
+  p.gsi += q * p.g_ps;
+  p.C += q * p.c_ps;
   const int total = p.nd * p.m;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
     const int j = t / p.m, k = t - j * p.m;
@@ -577,6 +611,7 @@ struct Update2Args {
   unsigned* status;
 };
 
+template <int kUnused = 0>
 __global__ __launch_bounds__(kBlock) void k_update2(Update2Args p) {
   extern __shared__ double sz[];   // 4K: Z (2K) then f*Z (2K)
   for (int t = threadIdx.x; t < p.K; t += kBlock) {
@@ -612,6 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_update2(Update2Args p) {
 }
 
 // de-interleave an N x 2 double2 array into column-major (for the getters)
+template <int kUnused = 0>
 __global__ __launch_bounds__(kBlock) void k_deinterleave(const double2* src, double* dst, int n) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r < n) {
@@ -621,6 +657,7 @@ __global__ __launch_bounds__(kBlock) void k_deinterleave(const double2* src, dou
   }
 }
 
+template <int kUnused = 0>
 __global__ __launch_bounds__(kBlock) void k_zero2(double2* p, int n) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r < n) p[r] = d2(0.0, 0.0);
@@ -628,6 +665,7 @@ __global__ __launch_bounds__(kBlock) void k_zero2(double2* p, int n) {
 
 // Plain streaming copy (16-byte elements), the calibration point for the HBM
 // byte counters.
+template <int kUnused = 0>
 __global__ __launch_bounds__(kBlock) void k_copy16(const double2* __restrict__ src,
                                                    double2* __restrict__ dst, long long n) {
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;

@@ -290,7 +290,7 @@ gm_status enqueue_build(gm_solver* s) {
     f.partials = s->partials;
     f.counter = s->counter;
     f.status = s->status;
-    hipLaunchKernelGGL(gm::k_lanczos_finish, dim3(s->G_el), dim3(kBlock), 0, s->stream, f);
+    hipLaunchKernelGGL(gm::k_lanczos_finish<0>, dim3(s->G_el), dim3(kBlock), 0, s->stream, f);
     GM_HIP(hipGetLastError());
     // beta_{k+1} = ||w||, flagged when zero or non-finite
     GM_TRY(launch_reduce(s, s->G_el, 2, s->aabb + (k + 1) + 2 * (s->m + 1), s->m + 1, 1, 1));
@@ -314,7 +314,7 @@ gm_status enqueue_solves(gm_solver* s) {
   a.yr = s->Yr;
   a.yi = s->Yi;
   a.status = s->status;
-  hipLaunchKernelGGL(gm::k_shifted_solves, dim3(cdiv_i(2 * s->ns, 64)), dim3(64), 0, s->stream, a);
+  hipLaunchKernelGGL(gm::k_shifted_solves<0>, dim3(cdiv_i(2 * s->ns, 64)), dim3(64), 0, s->stream, a);
   GM_HIP(hipGetLastError());
   return GM_OK;
 }
@@ -337,7 +337,7 @@ gm_status enqueue_apply(gm_solver* s) {
   c.ns = s->ns;
   c.nd = s->nd;
   c.C = s->C;
-  hipLaunchKernelGGL(gm::k_collapse, dim3(std::min(64, cdiv_i(s->nd * s->m, kBlock))), dim3(kBlock),
+  hipLaunchKernelGGL(gm::k_collapse<0>, dim3(std::min(64, cdiv_i(s->nd * s->m, kBlock))), dim3(kBlock),
                      0, s->stream, c);
   GM_HIP(hipGetLastError());
   gm::VCArgs v{};
@@ -448,7 +448,7 @@ gm_status enqueue_update(gm_solver* s, int mode) {
     b.counter = s->counter;
     b.beta0 = s->aabb + 2 * (s->m + 1);
     b.status = s->status;
-    hipLaunchKernelGGL(gm::k_update2, dim3(s->G_el), dim3(kBlock), 4 * s->K * sizeof(double),
+    hipLaunchKernelGGL(gm::k_update2<0>, dim3(s->G_el), dim3(kBlock), 4 * s->K * sizeof(double),
                        s->stream, b);
     GM_HIP(hipGetLastError());
     GM_TRY(launch_reduce(s, s->G_el, 2, beta0, s->m + 1, 1, 0));        // beta_0
@@ -480,7 +480,7 @@ gm_status check_status(gm_solver* s) {
 
 gm_status fetch_pair(gm_solver* s, const double2* src, double* dst) {
   if (!dst) return fail(GM_EINVAL, "null destination pointer");
-  hipLaunchKernelGGL(gm::k_deinterleave, dim3(cdiv_i(s->n, kBlock)), dim3(kBlock), 0, s->stream,
+  hipLaunchKernelGGL(gm::k_deinterleave<0>, dim3(cdiv_i(s->n, kBlock)), dim3(kBlock), 0, s->stream,
                      src, s->stage, s->n);
   GM_HIP(hipGetLastError());
   GM_HIP(hipMemcpyAsync(dst, s->stage, 2 * (size_t)s->n * sizeof(double), hipMemcpyDeviceToHost,
@@ -1034,10 +1034,10 @@ gm_status gm_profile_copy(gm_solver* s, int64_t bytes, int32_t reps, double* avg
   GM_HIP(hipEventCreate(&e0));
   GM_HIP(hipEventCreate(&e1));
   const int grid = (int)std::min<long long>(8192, (n + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(gm::k_copy16, dim3(grid), dim3(kBlock), 0, s->stream, src, dst, n);
+  hipLaunchKernelGGL(gm::k_copy16<0>, dim3(grid), dim3(kBlock), 0, s->stream, src, dst, n);
   GM_HIP(hipEventRecord(e0, s->stream));
   for (int i = 0; i < reps; ++i)
-    hipLaunchKernelGGL(gm::k_copy16, dim3(grid), dim3(kBlock), 0, s->stream, src, dst, n);
+    hipLaunchKernelGGL(gm::k_copy16<0>, dim3(grid), dim3(kBlock), 0, s->stream, src, dst, n);
   GM_HIP(hipEventRecord(e1, s->stream));
   GM_HIP(hipEventSynchronize(e1));
   float ms = 0.f;

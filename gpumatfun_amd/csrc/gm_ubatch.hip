@@ -1,0 +1,662 @@
+// gm_ubatch.hip -- independent problems of different sizes, one launch per
+// stage ("uneven batch"; no reference counterpart).
+//
+// P problems, each with its own matrix A_p (n_p rows), its own two columns
+// [u_p, dt g_p] and its own contour (weights, shifts, shifted gammas), share
+// the subspace size m, the polynomial gamma, the number of nodes n_s and dt.
+// The matrices are concatenated into one block-diagonal CSR and the vectors
+// into one N = sum n_p row range, laid out exactly as the plain path lays out
+// one problem (double2 rows [exp column, phi column], V[k][i]).  Rows are cut
+// into blocks of at most kBlock rows and 1024 nonzeros that never straddle two
+// problems, so every kernel of a step covers all problems in one launch:
+//   * SpMM: k_spmm_stream<1024, MODE, SEG> (each block reads its problem's
+//     alpha/beta, writes its partial at its row-block index);
+//   * reductions: k_reduce in segmented form, one block per problem;
+//   * finish, V*C, update: row-block kernels below (one thread per row);
+//   * shifted solves and collapse: the plain kernels with blockIdx.y = problem.
+// Small problems that are launch-bound on their own thus share each launch
+// with the large ones.  No deflation in this form.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/gpumatfun.h"
+#include "gm_kernels.h"
+
+namespace gmu {
+using namespace gm;
+
+constexpr int kTile = 1024;   // nonzeros per row block
+
+// w <- w - alpha_k v_k with the problem's alpha; partial ||w||^2 per block.
+// build_subspace.cu:63-76 (per problem).
+struct UFinish {
+  double2* w;
+  const double2* vk;
+  const double* aabb;
+  size_t aabb_stride;
+  int ld, k;
+  const int* rowblk;
+  const int* blk_prob;
+  double* partials;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_ufinish(UFinish p) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int r = p.rowblk[b] + (int)threadIdx.x;
+  const double* aabb = p.aabb + (size_t)p.blk_prob[b] * p.aabb_stride;
+  const double2 al = d2(aabb[p.k], aabb[p.k + p.ld]);
+  double2 nrm = d2(0.0, 0.0);
+  if (r < p.rowblk[b + 1]) {
+    const double2 v = p.vk[r];
+    double2 w = p.w[r];
+    w.x = fma(-al.x, v.x, w.x);
+    w.y = fma(-al.y, v.y, w.y);
+    p.w[r] = w;
+    nrm.x = w.x * w.x;
+    nrm.y = w.y * w.y;
+  }
+  block_partial2_at(nrm, p.partials, b);
+}
+
+// Z_j = V C_j (j < ND) with the block's problem's C staged in LDS.
+struct UVC {
+  const double2* V;    // [m][N]
+  const double2* C;    // [P][ND][m]
+  int m;
+  size_t ldv;          // N
+  const int* rowblk;
+  const int* blk_prob;
+  double2* Z;          // [ND][N]; ND == 1: the output X
+};
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_uvc(UVC p) {
+  extern __shared__ double2 sC[];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const double2* C = p.C + (size_t)p.blk_prob[b] * ND * p.m;
+  for (int t = threadIdx.x; t < ND * p.m; t += kBlock) sC[t] = C[t];
+  __syncthreads();
+  const int r = p.rowblk[b] + (int)threadIdx.x;
+  if (r >= p.rowblk[b + 1]) return;
+  double2 acc[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) acc[j] = d2(0.0, 0.0);
+  for (int k = 0; k < p.m; ++k) {
+    const double2 v = p.V[(size_t)k * p.ldv + r];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      const double2 c = sC[j * p.m + k];
+      acc[j].x = fma(v.x, c.x, acc[j].x);
+      acc[j].y = fma(v.y, c.y, acc[j].y);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < ND; ++j) p.Z[(size_t)j * p.ldv + r] = acc[j];
+}
+
+// Fisher-KPP update (update_U.cu:8-16, no deflation) or, when prepare, the
+// create-time copy; U = Usol and the partial norms for beta_0.
+struct UUpdate {
+  const double2* X;
+  const double* mass;
+  double2* Usol;
+  double2* U;
+  double dt;
+  int clamp, prepare;
+  const int* rowblk;
+  double* partials;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_uupdate(UUpdate p) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int r = p.rowblk[b] + (int)threadIdx.x;
+  double2 nrm = d2(0.0, 0.0);
+  if (r < p.rowblk[b + 1]) {
+    double2 us;
+    if (p.prepare) {
+      us = p.Usol[r];
+    } else {
+      const double2 x = p.X[r];
+      double u = x.x + x.y;
+      if (p.clamp) u = u * (u >= 0.0 ? 1.0 : 0.0);
+      const double ms = p.mass[r];
+      const double ub = u / ms;
+      const double g = (ms * ub) * (1.0 - ub);
+      us = d2(u, g * p.dt);
+      p.Usol[r] = us;
+    }
+    p.U[r] = us;
+    nrm.x = us.x * us.x;
+    nrm.y = us.y * us.y;
+  }
+  block_partial2_at(nrm, p.partials, b);
+}
+
+}  // namespace gmu
+
+// ====================================================================== host
+namespace {
+
+thread_local std::string g_uerr;
+std::mutex g_umu;
+std::unordered_set<const void*> g_ureg;
+
+gm_status ufail(gm_status st, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_uerr = buf;
+  return st;
+}
+
+#define U_HIP(call)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ufail(e_ == hipErrorOutOfMemory ? GM_ENOMEM : GM_EDEVICE, "%s: %s (%s:%d)", \
+                   #call, hipGetErrorString(e_), __FILE__, __LINE__);                    \
+  } while (0)
+#define U_TRY(x)              \
+  do {                        \
+    gm_status s_ = (x);       \
+    if (s_ != GM_OK) return s_; \
+  } while (0)
+
+}  // namespace
+
+struct gm_ubatch {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int P = 0, N = 0, m = 0, nd = 0, ns = 0, G = 0;
+  long long nnz = 0;
+  double dt = 0.0;
+  bool clamp = true;
+  std::vector<double> gamma;
+  std::vector<int> off;          // P + 1 row offsets
+  int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr, *blk_prob = nullptr,
+      *prob_blk = nullptr;
+  double *val = nullptr, *mass = nullptr, *aabb = nullptr, *wts1 = nullptr, *wts2 = nullptr,
+         *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *yr = nullptr, *yi = nullptr,
+         *partials = nullptr;
+  double2 *U = nullptr, *Usol = nullptr, *X = nullptr, *W[2] = {nullptr, nullptr},
+          *T[2] = {nullptr, nullptr}, *V = nullptr, *Z = nullptr, *C = nullptr, *wD = nullptr,
+          *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
+  unsigned* status = nullptr;
+  hipGraphExec_t graph = nullptr;
+};
+
+namespace {
+
+bool ulive(const gm_ubatch* u) {
+  std::lock_guard<std::mutex> lk(g_umu);
+  return u && g_ureg.count(u);
+}
+
+template <typename T>
+gm_status ualloc(T** p, size_t count) {
+  *p = nullptr;
+  U_HIP(hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)));
+  return GM_OK;
+}
+
+template <typename T>
+gm_status uupload(T** p, const T* src, size_t count, hipStream_t st) {
+  U_TRY(ualloc(p, count));
+  if (count) U_HIP(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+  return GM_OK;
+}
+
+void ufree_all(gm_ubatch* u) {
+  if (u->graph) (void)hipGraphExecDestroy(u->graph);
+  void* bufs[] = {u->rowptr, u->col, u->rowblk, u->blk_prob, u->prob_blk, u->val, u->mass,
+                  u->aabb, u->wts1, u->wts2, u->shifts, u->gsr, u->gsi, u->yr, u->yi,
+                  u->partials, u->U, u->Usol, u->X, u->W[0], u->W[1], u->T[0], u->T[1], u->V,
+                  u->Z, u->C, u->wD, u->wDU, u->wDU2, u->wX, u->status};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (u->stream) (void)hipStreamDestroy(u->stream);
+}
+
+size_t aabb_stride(const gm_ubatch* u) { return 4 * (size_t)(u->m + 1); }
+
+// Per-problem sums of the [G][2] partials into out_q[0], out_q[stride].
+gm_status ureduce(gm_ubatch* u, double* out, int stride, int do_sqrt, int flag_zero) {
+  gm::ReduceArgs r{};
+  r.partials = u->partials;
+  r.G = u->G;
+  r.nv = 2;
+  r.out = out;
+  r.stride = stride;
+  r.do_sqrt = do_sqrt;
+  r.flag_zero = flag_zero;
+  r.status = u->status;
+  r.seg = u->prob_blk;
+  r.out_pstride = aabb_stride(u);
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(u->P), dim3(gm::kReduceBlock), 0, u->stream, r);
+  U_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+template <int MODE>
+gm_status uspmm(gm_ubatch* u, const gm::SpmmArgs& a) {
+  hipLaunchKernelGGL((gm::k_spmm_stream<gmu::kTile, MODE, true>), dim3(u->G), dim3(gm::kBlock), 0,
+                     u->stream, a);
+  U_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+gm::SpmmArgs uspmm_base(gm_ubatch* u) {
+  gm::SpmmArgs a{};
+  a.rowptr = u->rowptr;
+  a.col = u->col;
+  a.val = u->val;
+  a.rowblk = u->rowblk;
+  a.blk_prob = u->blk_prob;
+  a.aabb_stride = (int)aabb_stride(u);
+  a.nrows = u->N;
+  a.aabb = u->aabb;
+  a.ld = u->m + 1;
+  a.partials = u->partials;
+  a.status = u->status;
+  a.g0 = u->gamma[0];
+  return a;
+}
+
+gm_status enqueue_update(gm_ubatch* u, int prepare) {
+  gmu::UUpdate p{};
+  p.X = u->X;
+  p.mass = u->mass;
+  p.Usol = u->Usol;
+  p.U = u->U;
+  p.dt = u->dt;
+  p.clamp = u->clamp ? 1 : 0;
+  p.prepare = prepare;
+  p.rowblk = u->rowblk;
+  p.partials = u->partials;
+  hipLaunchKernelGGL(gmu::k_uupdate<0>, dim3(u->G), dim3(gm::kBlock), 0, u->stream, p);
+  U_HIP(hipGetLastError());
+  return ureduce(u, u->aabb + 2 * (u->m + 1), u->m + 1, 1, 0);   // beta_0 per problem
+}
+
+template <int ND>
+void launch_uvc(gm_ubatch* u, const gmu::UVC& a) {
+  hipLaunchKernelGGL((gmu::k_uvc<ND>), dim3(u->G), dim3(gm::kBlock), ND * u->m * sizeof(double2),
+                     u->stream, a);
+}
+
+// One time step of every problem: build_subspace, lin_sys_solves, apply_V,
+// update_U of the plain path, each stage one launch over all problems.
+gm_status enqueue_ustep(gm_ubatch* u) {
+  const int d = u->nd - 1;
+  const size_t N = (size_t)u->N;
+  const int ld = u->m + 1;
+  for (int k = 0; k < u->m; ++k) {
+    const double2* wsrc = k == 0 ? u->U : u->W[(k - 1) & 1];
+    double2* wnext = u->W[k & 1];
+    gm::SpmmArgs a = uspmm_base(u);
+    a.k = k;
+    a.vk = u->V + (size_t)k * N;
+    a.vkm1 = k > 0 ? u->V + (size_t)(k - 1) * N : nullptr;
+    a.x = wsrc;
+    if (d == 0) {
+      a.y = wnext;
+      U_TRY(uspmm<gm::kOnly>(u, a));
+    } else {
+      a.y = u->T[0];
+      a.gi = u->gamma[1];
+      U_TRY(uspmm<gm::kFirst>(u, a));
+      for (int i = 2; i <= d; ++i) {
+        a.x = u->T[(i - 2) & 1];
+        a.y = u->T[(i - 1) & 1];
+        a.gi = u->gamma[i];
+        U_TRY(uspmm<gm::kMid>(u, a));
+      }
+      a.x = u->T[(d - 1) & 1];
+      a.y = wnext;
+      U_TRY(uspmm<gm::kLast>(u, a));
+    }
+    U_TRY(ureduce(u, u->aabb + k, ld, 0, 0));                          // alpha_k
+    gmu::UFinish f{};
+    f.w = wnext;
+    f.vk = a.vk;
+    f.aabb = u->aabb;
+    f.aabb_stride = aabb_stride(u);
+    f.ld = ld;
+    f.k = k;
+    f.rowblk = u->rowblk;
+    f.blk_prob = u->blk_prob;
+    f.partials = u->partials;
+    hipLaunchKernelGGL(gmu::k_ufinish<0>, dim3(u->G), dim3(gm::kBlock), 0, u->stream, f);
+    U_HIP(hipGetLastError());
+    U_TRY(ureduce(u, u->aabb + (k + 1) + 2 * ld, ld, 1, 1));          // beta_{k+1}
+  }
+  // shifted solves, every problem's 2 n_s systems (blockIdx.y = problem)
+  gm::SolveArgs s{};
+  s.aabb = u->aabb;
+  s.ld = ld;
+  s.m = u->m;
+  s.ns = u->ns;
+  s.wts1 = u->wts1;
+  s.wts2 = u->wts2;
+  s.shifts = u->shifts;
+  s.wD = u->wD;
+  s.wDU = u->wDU;
+  s.wDU2 = u->wDU2;
+  s.wX = u->wX;
+  s.yr = u->yr;
+  s.yi = u->yi;
+  s.status = u->status;
+  s.aabb_ps = aabb_stride(u);
+  s.wts_ps = 2 * (size_t)u->ns;
+  s.shifts_ps = 4 * (size_t)u->m * u->ns;
+  s.y_ps = 2 * (size_t)u->m * u->ns;
+  s.work_ps = 2 * (size_t)u->m * u->ns;
+  hipLaunchKernelGGL(gm::k_shifted_solves<0>, dim3((2 * u->ns + 63) / 64, u->P), dim3(64), 0,
+                     u->stream, s);
+  U_HIP(hipGetLastError());
+  // collapse over the nodes, then V*C and the Horner SpMMs (apply_V)
+  gm::CollapseArgs c{};
+  c.yr = u->yr;
+  c.yi = u->yi;
+  c.gsr = u->gsr;
+  c.gsi = u->gsi;
+  c.m = u->m;
+  c.ns = u->ns;
+  c.nd = u->nd;
+  c.C = u->C;
+  c.y_ps = 2 * (size_t)u->m * u->ns;
+  c.g_ps = (size_t)u->nd * u->ns;
+  c.c_ps = (size_t)u->nd * u->m;
+  hipLaunchKernelGGL(gm::k_collapse<0>, dim3(std::max(1, std::min(64, (u->nd * u->m + 255) / 256)), u->P),
+                     dim3(gm::kBlock), 0, u->stream, c);
+  U_HIP(hipGetLastError());
+  gmu::UVC v{};
+  v.V = u->V;
+  v.C = u->C;
+  v.m = u->m;
+  v.ldv = N;
+  v.rowblk = u->rowblk;
+  v.blk_prob = u->blk_prob;
+  v.Z = d == 0 ? u->X : u->Z;
+  switch (u->nd) {
+#define U_VC(K) \
+  case K: launch_uvc<K>(u, v); break;
+    U_VC(1) U_VC(2) U_VC(3) U_VC(4) U_VC(5) U_VC(6) U_VC(7) U_VC(8) U_VC(9)
+    U_VC(10) U_VC(11) U_VC(12) U_VC(13) U_VC(14) U_VC(15) U_VC(16) U_VC(17)
+#undef U_VC
+    default: return ufail(GM_EINVAL, "gamma_size %d exceeds %d", u->nd, gm::kMaxDegree + 1);
+  }
+  U_HIP(hipGetLastError());
+  const double2* src = u->Z;
+  for (int j = 1; j <= d; ++j) {
+    gm::SpmmArgs a = uspmm_base(u);
+    double2* dst = (j == d) ? u->X : u->T[(j - 1) & 1];
+    a.x = src;
+    a.y = dst;
+    a.z = u->Z + (size_t)j * N;
+    a.accumulate = 0;
+    U_TRY(uspmm<gm::kHorner>(u, a));
+    src = dst;
+  }
+  return enqueue_update(u, 0);
+}
+
+// Rows of problem q cut into blocks of <= kBlock rows and <= kTile nonzeros
+// (a longer row alone), appended to blk with global row numbers.
+void problem_blocks(const std::vector<int>& rp, int r0, int r1, std::vector<int>& blk) {
+  int r = r0;
+  while (r < r1) {
+    const int start = r;
+    const long long base = rp[r];
+    if (rp[r + 1] - base > gmu::kTile) {
+      ++r;
+    } else {
+      while (r < r1 && r - start < gm::kBlock && rp[r + 1] - base <= gmu::kTile) ++r;
+    }
+    blk.push_back(r);
+  }
+}
+
+gm_status create_impl(const gm_ubatch_desc* d, gm_ubatch* u) {
+  u->dev = d->device;
+  U_HIP(hipSetDevice(u->dev));
+  U_HIP(hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking));
+  hipStream_t st = u->stream;
+  u->P = d->num_problems;
+  u->m = d->subspace_size;
+  u->nd = d->gamma_size;
+  u->ns = d->contour_size;
+  u->dt = d->dt;
+  u->clamp = !(d->flags & GM_FLAG_NO_CLAMP);
+  u->gamma.assign(d->gamma, d->gamma + u->nd);
+  u->off.assign(u->P + 1, 0);
+  long long nnz = 0;
+  for (int q = 0; q < u->P; ++q) {
+    u->off[q + 1] = u->off[q] + d->n[q];
+    nnz += d->nnz[q];
+  }
+  u->N = u->off[u->P];
+  u->nnz = nnz;
+  const size_t N = (size_t)u->N;
+  // block-diagonal CSR, row blocks that never straddle two problems
+  std::vector<int> rp(N + 1), ci((size_t)nnz), blk{0}, bprob, pblk{0};
+  std::vector<double> vv((size_t)nnz), mass(N, 1.0);
+  std::vector<double2> u2(N);
+  long long e = 0;
+  for (int q = 0; q < u->P; ++q) {
+    const int o = u->off[q], n = d->n[q];
+    for (int i = 0; i < n; ++i) {
+      rp[o + i] = (int)(e + d->A_ptr[q][i]);
+      if (d->mass && d->mass[q]) mass[o + i] = d->mass[q][i];
+      u2[o + i] = make_double2(d->U[q][i], d->U[q][n + i] * u->dt);   // lanczos_impl.cu:79
+    }
+    for (long long j = 0; j < d->nnz[q]; ++j) {
+      ci[e + j] = d->A_ind[q][j] + o;
+      vv[e + j] = d->A_values[q][j];
+    }
+    e += d->nnz[q];
+    rp[o + n] = (int)e;
+    const size_t before = blk.size();
+    problem_blocks(rp, o, o + n, blk);
+    for (size_t t = before; t < blk.size(); ++t) bprob.push_back(q);
+    pblk.push_back((int)blk.size() - 1);
+  }
+  u->G = (int)bprob.size();
+  U_TRY(uupload(&u->rowptr, rp.data(), N + 1, st));
+  U_TRY(uupload(&u->col, ci.data(), (size_t)nnz, st));
+  U_TRY(uupload(&u->val, vv.data(), (size_t)nnz, st));
+  U_TRY(uupload(&u->rowblk, blk.data(), blk.size(), st));
+  U_TRY(uupload(&u->blk_prob, bprob.data(), bprob.size(), st));
+  U_TRY(uupload(&u->prob_blk, pblk.data(), pblk.size(), st));
+  U_TRY(uupload(&u->mass, mass.data(), N, st));
+  U_TRY(uupload(&u->Usol, u2.data(), N, st));
+  const size_t P = (size_t)u->P, m = (size_t)u->m, ns = (size_t)u->ns, nd = (size_t)u->nd;
+  U_TRY(uupload(&u->wts1, d->wts1, P * 2 * ns, st));
+  U_TRY(uupload(&u->wts2, d->wts2, P * 2 * ns, st));
+  U_TRY(uupload(&u->shifts, d->shifts, P * 4 * m * ns, st));
+  U_TRY(uupload(&u->gsr, d->gamma_s_real, P * nd * ns, st));
+  U_TRY(uupload(&u->gsi, d->gamma_s_imag, P * nd * ns, st));
+  // host vectors must outlive the asynchronous uploads
+  U_HIP(hipStreamSynchronize(st));
+  U_TRY(ualloc(&u->U, N));
+  U_TRY(ualloc(&u->X, N));
+  U_TRY(ualloc(&u->W[0], N));
+  U_TRY(ualloc(&u->W[1], N));
+  if (u->nd > 1) {
+    U_TRY(ualloc(&u->T[0], N));
+    U_TRY(ualloc(&u->T[1], N));
+    U_TRY(ualloc(&u->Z, N * nd));
+  }
+  U_TRY(ualloc(&u->V, N * m));
+  U_TRY(ualloc(&u->C, P * nd * m));
+  U_TRY(ualloc(&u->aabb, P * aabb_stride(u)));
+  U_HIP(hipMemsetAsync(u->aabb, 0, P * aabb_stride(u) * sizeof(double), st));
+  U_TRY(ualloc(&u->yr, P * 2 * m * ns));
+  U_TRY(ualloc(&u->yi, P * 2 * m * ns));
+  U_TRY(ualloc(&u->wD, P * 2 * m * ns));
+  U_TRY(ualloc(&u->wDU, P * 2 * m * ns));
+  U_TRY(ualloc(&u->wDU2, P * 2 * m * ns));
+  U_TRY(ualloc(&u->wX, P * 2 * m * ns));
+  U_TRY(ualloc(&u->partials, 2 * (size_t)u->G));
+  U_TRY(ualloc(&u->status, 1));
+  U_HIP(hipMemsetAsync(u->status, 0, sizeof(unsigned), st));
+  U_TRY(enqueue_update(u, 1));   // U = Usol and beta_0 (lanczos_impl.cu:79-91, K = 0)
+  U_HIP(hipStreamSynchronize(st));
+  return GM_OK;
+}
+
+gm_status validate(const gm_ubatch_desc* d) {
+  if (!d) return ufail(GM_EINVAL, "null descriptor");
+  if (d->num_problems < 1 || !d->n || !d->nnz || !d->A_values || !d->A_ptr || !d->A_ind || !d->U)
+    return ufail(GM_EINVAL, "bad problem count or null per-problem arrays");
+  if (d->sparse_format != GM_SPARSE_CSR)
+    return ufail(GM_EINVAL, "the uneven batch takes each A in CSR");
+  if (d->subspace_size < 1 || d->gamma_size < 1 || d->gamma_size > gm::kMaxDegree + 1 || !d->gamma)
+    return ufail(GM_EINVAL, "bad subspace size or gamma");
+  if (d->contour_size < 1 || !d->gamma_s_real || !d->gamma_s_imag || !d->wts1 || !d->wts2 ||
+      !d->shifts)
+    return ufail(GM_EINVAL, "bad contour data");
+  long long rows = 0, nnz = 0;
+  for (int q = 0; q < d->num_problems; ++q) {
+    const int n = d->n[q];
+    if (n < 1 || !d->A_ptr[q] || !d->A_ind[q] || !d->A_values[q] || !d->U[q])
+      return ufail(GM_EINVAL, "problem %d: bad size or null array", q);
+    if (d->A_ptr[q][0] != 0 || d->A_ptr[q][n] != d->nnz[q])
+      return ufail(GM_EINVAL, "problem %d: bad pointer array", q);
+    for (int i = 0; i < n; ++i)
+      if (d->A_ptr[q][i + 1] < d->A_ptr[q][i])
+        return ufail(GM_EINVAL, "problem %d: pointer array not monotone", q);
+    for (long long j = 0; j < d->nnz[q]; ++j)
+      if (d->A_ind[q][j] < 0 || d->A_ind[q][j] >= n)
+        return ufail(GM_EINVAL, "problem %d: column index out of range", q);
+    rows += n;
+    nnz += d->nnz[q];
+  }
+  if (rows > 0x7fffffffLL || nnz > 0x7fffffffLL)
+    return ufail(GM_EINVAL, "total rows and nonzeros must fit int32");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= d->device || d->device < 0)
+    return ufail(GM_EDEVICE, "no HIP device %d", d->device);
+  return GM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gm_ubatch_last_error(void) { return g_uerr.c_str(); }
+
+gm_status gm_ubatch_create(const gm_ubatch_desc* desc, gm_ubatch** out) {
+  g_uerr.clear();
+  if (!out) return ufail(GM_EINVAL, "null output handle");
+  *out = nullptr;
+  U_TRY(validate(desc));
+  gm_ubatch* u = new (std::nothrow) gm_ubatch();
+  if (!u) return ufail(GM_ENOMEM, "host allocation failed");
+  const gm_status s = create_impl(desc, u);
+  if (s != GM_OK) {
+    const std::string keep = g_uerr;
+    ufree_all(u);
+    delete u;
+    g_uerr = keep;
+    return s;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_umu);
+    g_ureg.insert(u);
+  }
+  *out = u;
+  return GM_OK;
+}
+
+gm_status gm_ubatch_delete(gm_ubatch* u) {
+  {
+    std::lock_guard<std::mutex> lk(g_umu);
+    if (!u || !g_ureg.count(u)) return ufail(GM_EBADHANDLE, "invalid uneven-batch handle");
+    g_ureg.erase(u);
+  }
+  (void)hipSetDevice(u->dev);
+  if (u->stream) (void)hipStreamSynchronize(u->stream);
+  ufree_all(u);
+  delete u;
+  return GM_OK;
+}
+
+gm_status gm_ubatch_step(gm_ubatch* u, int32_t nsteps, int32_t use_graph) {
+  if (!ulive(u)) return ufail(GM_EBADHANDLE, "invalid uneven-batch handle");
+  U_HIP(hipSetDevice(u->dev));
+  if (nsteps < 0) return ufail(GM_EINVAL, "nsteps < 0");
+  if (!use_graph) {
+    for (int i = 0; i < nsteps; ++i) U_TRY(enqueue_ustep(u));
+    return GM_OK;
+  }
+  if (!u->graph) {
+    hipGraph_t g = nullptr;
+    U_HIP(hipStreamBeginCapture(u->stream, hipStreamCaptureModeThreadLocal));
+    const gm_status s = enqueue_ustep(u);
+    hipError_t e = hipStreamEndCapture(u->stream, &g);
+    if (s != GM_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    if (e != hipSuccess) return ufail(GM_EDEVICE, "capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&u->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      u->graph = nullptr;
+      return ufail(GM_EDEVICE, "instantiate: %s", hipGetErrorString(e));
+    }
+  }
+  for (int i = 0; i < nsteps; ++i) U_HIP(hipGraphLaunch(u->graph, u->stream));
+  return GM_OK;
+}
+
+gm_status gm_ubatch_get_U(gm_ubatch* u, int32_t problem, double* dst) {
+  if (!ulive(u)) return ufail(GM_EBADHANDLE, "invalid uneven-batch handle");
+  if (!dst || problem < 0 || problem >= u->P) return ufail(GM_EINVAL, "bad problem index or null");
+  U_HIP(hipSetDevice(u->dev));
+  const int o = u->off[problem], n = u->off[problem + 1] - o;
+  std::vector<double2> tmp(n);
+  U_HIP(hipMemcpyAsync(tmp.data(), u->Usol + o, n * sizeof(double2), hipMemcpyDeviceToHost,
+                       u->stream));
+  U_HIP(hipStreamSynchronize(u->stream));
+  for (int i = 0; i < n; ++i) {
+    dst[i] = tmp[i].x;
+    dst[n + i] = tmp[i].y;
+  }
+  return GM_OK;
+}
+
+gm_status gm_ubatch_synchronize(gm_ubatch* u) {
+  if (!ulive(u)) return ufail(GM_EBADHANDLE, "invalid uneven-batch handle");
+  U_HIP(hipSetDevice(u->dev));
+  unsigned stv = 0;
+  U_HIP(hipMemcpyAsync(&stv, u->status, sizeof(unsigned), hipMemcpyDeviceToHost, u->stream));
+  U_HIP(hipStreamSynchronize(u->stream));
+  if (stv & 6u) return ufail(GM_EBREAKDOWN, "Lanczos breakdown in the uneven batch (status %u)", stv);
+  return GM_OK;
+}
+
+gm_status gm_ubatch_info(gm_ubatch* u, int32_t* num_problems, int64_t* total_rows,
+                         int32_t* row_blocks) {
+  if (!ulive(u)) return ufail(GM_EBADHANDLE, "invalid uneven-batch handle");
+  if (num_problems) *num_problems = u->P;
+  if (total_rows) *total_rows = u->N;
+  if (row_blocks) *row_blocks = u->G;
+  return GM_OK;
+}
+
+}  // extern "C"

@@ -292,6 +292,51 @@ gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, doubl
                           double* grad_kappa, double* grad_alpha);
 
 /* ---------------------------------------------------------------------------
+ * Uneven batch (no reference counterpart): P independent problems of
+ * different sizes -- each its own A_p, [u_p, g_p], contour weights, shifted
+ * gammas and shifts, as lanczos_create takes them for one problem -- stepped
+ * together, every stage of a step one launch over all problems.  m, gamma,
+ * n_s and dt are shared; no deflation; A_p in CSR.
+ * ------------------------------------------------------------------------- */
+typedef struct gm_ubatch gm_ubatch;
+
+typedef struct gm_ubatch_desc {
+    int32_t num_problems;               /* P                                  */
+    const int32_t* n;                   /* P row counts                       */
+    const int64_t* nnz;                 /* P nonzero counts                   */
+    int32_t sparse_format;              /* GM_SPARSE_CSR                      */
+    const double* const* A_values;      /* P arrays (CSR of each A_p)         */
+    const int32_t* const* A_ptr;
+    const int32_t* const* A_ind;
+    const double* const* U;             /* P arrays, n_p x 2 column-major
+                                           [u, g(u)] (column 1 scaled by dt) */
+    const double* const* mass;          /* P arrays of n_p, or NULL (ones)    */
+    int32_t subspace_size;              /* m                                  */
+    int32_t gamma_size;                 /* d + 1                              */
+    const double* gamma;                /* shared polynomial, highest first   */
+    int32_t contour_size;               /* n_s                                */
+    const double* gamma_s_real;         /* P blocks of (d+1) x n_s            */
+    const double* gamma_s_imag;
+    const double* wts1;                 /* P blocks of n_s x 2 [Re Im]        */
+    const double* wts2;
+    const double* shifts;               /* P blocks of 4 m n_s                */
+    double dt;
+    uint32_t flags;                     /* GM_FLAG_NO_CLAMP                   */
+    int32_t device;
+} gm_ubatch_desc;
+
+const char* gm_ubatch_last_error(void);
+gm_status gm_ubatch_create(const gm_ubatch_desc* desc, gm_ubatch** out);
+gm_status gm_ubatch_delete(gm_ubatch* u);
+/* nsteps time steps of every problem (asynchronous; HIP graph when use_graph). */
+gm_status gm_ubatch_step(gm_ubatch* u, int32_t nsteps, int32_t use_graph);
+/* dst: n_p x 2 column-major [u_p, dt g_p] of problem p.                    */
+gm_status gm_ubatch_get_U(gm_ubatch* u, int32_t problem, double* dst);
+gm_status gm_ubatch_synchronize(gm_ubatch* u);
+gm_status gm_ubatch_info(gm_ubatch* u, int32_t* num_problems, int64_t* total_rows,
+                         int32_t* row_blocks);
+
+/* ---------------------------------------------------------------------------
  * Host set-up of the create inputs (host code only; usable without a GPU).
  * What problem_startup.m:71-144 computes in MATLAB before lanczos_create.
  * Complex arrays are [re im] interleaved unless stated; wts1/wts2 are n x 2

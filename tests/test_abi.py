@@ -31,12 +31,18 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTS) == decl
 
 
-def test_struct_layout_matches_header_field_order():
+@pytest.mark.parametrize("cname,pyname", [("gm_create_desc", "GmCreateDesc"),
+                                           ("gm_ms_params", "GmMsParams"),
+                                           ("gm_batch_desc", "GmBatchDesc"),
+                                           ("gm_fab_desc", "GmFabDesc"),
+                                           ("gm_ubatch_desc", "GmUbatchDesc")])
+def test_struct_layout_matches_header_field_order(cname, pyname):
     text = open(HEADER).read()
-    body = text[text.index("typedef struct gm_create_desc"):text.index("} gm_create_desc;")]
+    body = text[text.index("typedef struct " + cname):text.index("} " + cname + ";")]
+    body = body[body.index("{") + 1:]
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     names = re.findall(r"\b([a-zA-Z_][a-zA-Z0-9_]*)\s*[,;]", body)
-    py = [f[0].rstrip("_") for f in _lib.GmCreateDesc._fields_]
+    py = [f[0].rstrip("_") for f in getattr(_lib, pyname)._fields_]
     assert names == py
 
 
@@ -139,3 +145,41 @@ def test_engine_python_side_validation():
         LanczosEngine(inp.U, inp.A, inp.Q, inp.lam, inp.m, inp.gamma, inp.gamma_s_real,
                       inp.gamma_s_imag, inp.exp_lambda, inp.phi_lambda, inp.dt, inp.ode, 0, 0,
                       0, 0, 0, 0, 0, 0, 0, inp.mass, inp.wts1[:2], inp.wts2, inp.shifts)
+
+
+def test_ubatch_create_rejects_bad_descriptors():
+    """Validation runs before any device call, so this needs no GPU."""
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.gm_ubatch_create(None, C.byref(h)) == _lib.GM_EINVAL
+    d = _lib.GmUbatchDesc()
+    d.num_problems = 0
+    assert lib.gm_ubatch_create(C.byref(d), C.byref(h)) == _lib.GM_EINVAL
+    assert b"problem count" in lib.gm_ubatch_last_error()
+    # one 2-row problem whose column index is out of range
+    n = np.array([2], dtype=np.int32)
+    nnz = np.array([2], dtype=np.int64)
+    ptr = np.array([0, 1, 2], dtype=np.int32)
+    ind = np.array([0, 5], dtype=np.int32)
+    val = np.array([1.0, 1.0])
+    U = np.zeros(4)
+    g = np.array([1.0])
+    sm = np.zeros(64)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    d.num_problems = 1
+    d.n = n.ctypes.data_as(ip)
+    d.nnz = nnz.ctypes.data_as(C.POINTER(C.c_int64))
+    d.sparse_format = _lib.GM_SPARSE_CSR
+    pv = (dp * 1)(val.ctypes.data_as(dp))
+    pp = (ip * 1)(ptr.ctypes.data_as(ip))
+    pi = (ip * 1)(ind.ctypes.data_as(ip))
+    pu = (dp * 1)(U.ctypes.data_as(dp))
+    d.A_values, d.A_ptr, d.A_ind, d.U = pv, pp, pi, pu
+    d.subspace_size, d.gamma_size, d.contour_size = 2, 1, 1
+    d.gamma = g.ctypes.data_as(dp)
+    for f in ("gamma_s_real", "gamma_s_imag", "wts1", "wts2", "shifts"):
+        setattr(d, f, sm.ctypes.data_as(dp))
+    assert lib.gm_ubatch_create(C.byref(d), C.byref(h)) == _lib.GM_EINVAL
+    assert b"out of range" in lib.gm_ubatch_last_error()
+    assert lib.gm_ubatch_step(None, 1, 0) == _lib.GM_EBADHANDLE

@@ -1,0 +1,88 @@
+"""The uneven batch (gm_ubatch_*): independent problems of different sizes in
+one launch per stage, each problem against the CPU oracle (relative 2-norm
+<= 1e-10) and against the plain engine run on it alone (<= 1e-12: the same
+arithmetic per row, reductions over differently cut row ranges)."""
+
+import numpy as np
+import pytest
+
+import oracle
+from cases import fem_case, oracle_setup
+from conftest import rel
+from gpumatfun_amd import problems, quadrature as q
+from gpumatfun_amd.engine import LanczosEngine
+from gpumatfun_amd.ubatch import UnevenBatch
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _dirichlet(side, seed, m, n, degree, prob_dim):
+    A = problems.laplacian_2d_dirichlet(side)
+    lam = problems.dirichlet_eigs_2d(side)
+    u0 = np.random.default_rng(seed).standard_normal(A.shape[0])
+    return q.make_inputs(A, u0, u0 * (1 - u0), m=m, n=n, eta=1e-3, alpha=1.5, dt=1e-3,
+                         l1=float(lam.min()), lN=float(lam.max()), degree=degree,
+                         prob_dim=prob_dim)
+
+
+def _cpu(inp, steps):
+    S = oracle_setup(inp)
+    if np.all(inp.mass == 1.0):
+        return oracle.cpu_steps(S, inp.U[:, 0], inp.U[:, 1], steps)
+    return oracle.cpu_steps_nonuniform(S, inp.U[:, 0], inp.U[:, 1], steps)
+
+
+@pytest.mark.parametrize("degree", [0, 2])
+def test_uneven_batch_vs_oracle_and_plain(degree):
+    """Five 2D Dirichlet Laplacians (7^2 .. 64^2 rows, one polynomial shared at
+    the largest problem's scale) and a 3000-node FEM disc with lumped mass."""
+    m, n, steps = 30, 16, 3
+    prob_dim = float(problems.dirichlet_eigs_2d(64).max()) / 4.0
+    inps = [_dirichlet(s, i, m, n, degree, prob_dim) for i, s in enumerate([16, 40, 7, 64, 23])]
+    fem = fem_case(K=0, m=m, n=n, degree=degree)
+    if degree:
+        # the shared polynomial: rebuild the FEM inputs with the same prob_dim
+        import scipy.sparse.linalg  # noqa: F401
+        fem = q.make_inputs(fem.A, fem.U[:, 0], fem.U[:, 1], m=m, n=n, eta=1e-3, alpha=1.7,
+                            dt=1e-3, l1=fem.extra["l1"], lN=fem.extra["lN"], degree=degree,
+                            prob_dim=prob_dim, mass=fem.mass)
+    inps.insert(2, fem)
+    print(f"case uneven batch degree={degree} sizes={[i.A.shape[0] for i in inps]}", flush=True)
+    ub = UnevenBatch.from_inputs(inps, clamp=False)
+    info = ub.info()
+    assert info["num_problems"] == len(inps)
+    assert info["total_rows"] == sum(i.A.shape[0] for i in inps)
+    ub.step(steps)
+    ub.synchronize()
+    for i, inp in enumerate(inps):
+        U = ub.get_U(i)
+        u, gu = _cpu(inp, steps)
+        e0, e1 = rel(U[:, 0], u), rel(U[:, 1], inp.dt * gu)
+        eng = LanczosEngine.from_inputs(inp, clamp=False)
+        eng.step(steps)
+        Up = eng.get_U()
+        eng.delete()
+        ep = rel(U, Up)
+        print(f"problem {i} n={inp.A.shape[0]}: vs oracle {e0:.2e} {e1:.2e}, vs plain {ep:.2e}")
+        assert e0 < TOL and e1 < TOL
+        assert ep < 1e-12
+    ub.delete()
+
+
+def test_uneven_batch_graph_matches_eager_bitwise():
+    m, n = 24, 12
+    inps = [_dirichlet(s, 10 + i, m, n, 0, 1.0) for i, s in enumerate([9, 30, 17])]
+    a = UnevenBatch.from_inputs(inps, clamp=False)
+    b = UnevenBatch.from_inputs(inps, clamp=False)
+    a.step(2)
+    b.step(2, use_graph=True)
+    for _ in range(3):
+        a.synchronize()
+        b.synchronize()
+        for i in range(len(inps)):
+            assert np.array_equal(a.get_U(i), b.get_U(i))
+        a.step(1)
+        b.step(1, use_graph=True)
+    a.delete()
+    b.delete()
