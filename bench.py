@@ -40,11 +40,16 @@ def kernel_symbol(info, mode):
     lanes = info["spmm_lanes"]
     if lanes > 0:
         return f"void gm::k_spmm<{lanes}, {mode}>(gm::SpmmArgs)"
-    return f"void gm::k_spmm_stream<{-lanes}, {mode}>(gm::SpmmArgs)"
+    return f"void gm::k_spmm_stream<{-lanes}, {mode}, false>(gm::SpmmArgs)"
 
 
-def measured_traffic(workload, symbol):
-    """HBM bytes per launch of `symbol` on `workload` from the committed
+def kernel_key(info, mode):
+    """Schedule/mode key of the committed counter summaries."""
+    return f"{info['spmm_schedule']}/mode{mode}"
+
+
+def measured_traffic(workload, key):
+    """HBM bytes per launch of the SpMM `key` on `workload` from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, each scaled by what
     the counter reads for a copy kernel of known size), or None when no
     summary covers this kernel."""
@@ -53,7 +58,7 @@ def measured_traffic(workload, symbol):
         return None
     with open(path) as fh:
         rec = json.load(fh)
-    k = rec.get("kernels", {}).get(symbol)
+    k = rec.get("kernels", {}).get(key)
     return None if k is None else int(k["traffic_bytes"])
 
 
@@ -218,6 +223,81 @@ def run_uneven(args, steps, warm, use_graph):
     print(json.dumps(result), flush=True)
 
 
+def run_adjoint(args, steps, warm):
+    """Forward u = f(A)b and the full backward (dL/db, dL/dkappa, dL/dalpha) of
+    L = ||u - y||^2 on the 256^2 variable-coefficient Laplacian,
+    kappa = exp(0.5 GRF) (correlation length 0.1, seed 1), tau = 1e-3,
+    alpha = 1.5, m = 600, 32 contour nodes."""
+    import scipy.sparse.linalg as sla
+    from gpumatfun_amd import problems, quadrature as q
+    from gpumatfun_amd.adjoint import FabEngine, contour_weights
+    t0 = time.time()
+    n, m, ns, tau, alpha = 256, 600, 32, 1e-3, 1.5
+    kappa = np.exp(0.5 * problems.gaussian_random_field(n, 0.1, seed=1))
+    A = problems.varcoef_laplacian_2d(kappa, n)
+    lo = sla.eigsh(A, k=1, sigma=0, which="LM", return_eigenvectors=False)[0]
+    hi = sla.eigsh(A, k=1, which="LA", return_eigenvectors=False)[0]
+    c = q.contour_nodes(0.95 * lo, 1.05 * hi, ns, tau, 1.0)
+    cw, dc = contour_weights(c["xi"], c["dxidt"], c["s"], tau, alpha)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(n * n)
+    y = rng.standard_normal(n * n)
+    setup_s = time.time() - t0
+    cfg = {"workload": "adjoint_grf256", "n": n * n, "nnz": int(A.nnz), "m": m, "shifts": ns,
+           "alpha": alpha, "tau": tau, "kappa": "exp(0.5 GRF), l = 0.1, seed 1",
+           "loss": "||f(A)b - y||^2", "gradients": "b, kappa (65536), alpha"}
+    out = {}
+    for recompute in (False, True):
+        eng = FabEngine(A, m=m, xi=c["xi"], wts=cw, dwts=dc, recompute=recompute)
+        for _ in range(warm):
+            u = eng.forward(b)
+            eng.backward(2 * (u - y), kappa)
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            u = eng.forward(b)
+        tf = (time.perf_counter() - t1) / steps
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            u = eng.forward(b)
+            grads = eng.backward(2 * (u - y), kappa)
+        tfb = (time.perf_counter() - t1) / steps
+        eng.delete()
+        out[recompute] = (tf, tfb, u, grads)
+        log(f"recompute={recompute}: forward {tf * 1e3:.2f} ms, forward+backward {tfb * 1e3:.2f} ms")
+    tf, tfb, u, (gb, gk, ga) = out[False]
+    result = {
+        "metric": "f(A)b value-and-gradient evaluations/sec (FP64, adjoint)",
+        "value": round(1.0 / tfb, 4), "unit": "evaluations/s", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "ms_per_step": round(1e3 * tfb, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded GRF conductivity, b and y)", "config": cfg,
+        "setup_s": round(setup_s, 1),
+        "forward_ms": round(1e3 * tf, 4),
+        "adjoint_cost_vs_forward": round(tfb / tf, 3),
+        "recompute_basis": {"forward_ms": round(1e3 * out[True][0], 4),
+                            "forward_backward_ms": round(1e3 * out[True][1], 4),
+                            "adjoint_cost_vs_forward": round(out[True][1] / out[True][0], 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    if not args.no_cpu and not args.profile_only:
+        from oracle import adjoint as oadj
+        t1 = time.perf_counter()
+        rb, rk, ra = oadj.krylov_gradients(A, kappa, b, 2 * (u - y), c["xi"], cw, dc, m)
+        cpu_s = time.perf_counter() - t1
+        result["rel_err_vs_oracle"] = {
+            "what": "same Krylov approximation (m = 600) on the CPU, NumPy",
+            "grad_b": rel(gb, rb), "grad_kappa": rel(gk, rk),
+            "grad_alpha": float(abs(ga - ra) / abs(ra))}
+        result["cpu_baseline"] = {"value": round(1.0 / cpu_s, 6), "unit": "evaluations/s",
+                                  "cores": 1, "kind": "port",
+                                  "sample": "one backward pass (both Krylov runs, resolvent "
+                                            "blocks, kappa and alpha reductions) of the oracle's "
+                                            "NumPy restatement (oracle/adjoint.py)",
+                                  "seconds": round(cpu_s, 3), "host_cpus": os.cpu_count()}
+        log(f"oracle backward {cpu_s:.1f}s; rel err {result['rel_err_vs_oracle']}")
+    print(json.dumps(result), flush=True)
+
+
 def oracle_setup_dict(inp, A=None, perm=None):
     S = {"A": inp.A if A is None else A, "Q": inp.Q if inp.Q.shape[1] else None,
          "exp_lambda": inp.exp_lambda, "phi_lambda": inp.phi_lambda, "dt": inp.dt,
@@ -300,7 +380,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--config", default="fem1m",
-                    choices=["fem1m", "dirichlet128", "batch3d", "uneven64"])
+                    choices=["fem1m", "dirichlet128", "batch3d", "uneven64", "adjoint256"])
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--side3d", type=int, default=128)
     ap.add_argument("--problems", type=int, default=256)
@@ -338,6 +418,9 @@ def main():
 
     if args.config == "batch3d":
         run_batch(args, steps, warm, use_graph)
+        return
+    if args.config == "adjoint256":
+        run_adjoint(args, steps, warm)
         return
     if args.config == "uneven64":
         if args.problems == 256:
@@ -394,7 +477,7 @@ def main():
         "setup_s": round(setup_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(cfg["workload"], kernel_symbol(info, 1 if mode == 0 else 2)),
+                     "traffic": measured_traffic(cfg["workload"], kernel_key(info, 1 if mode == 0 else 2)),
                      "kernel": kernel_symbol(info, 1 if mode == 0 else 2),
                      "avg_launch_us": round(spmm_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(bytes_per)},
@@ -463,7 +546,7 @@ def run_batch(args, steps, warm, use_graph):
                      "achieved": round(bytes_per / (spmm_ms * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_per / (spmm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("laplace3d_128_plain", kernel_symbol(einfo, 2)),
+                     "traffic": measured_traffic("laplace3d_128_plain", kernel_key(einfo, 2)),
                      "avg_launch_us": round(spmm_ms * 1e3, 3)},
         "cpu_baseline": None,
     }
