@@ -13,6 +13,7 @@ namespace gmb {
 using namespace gm;
 
 constexpr int kMaxND = 17;
+constexpr int kUnroll = 8;   // nonzeros per gather batch in k_bspmm
 
 enum Mode : int { kFirst = 0, kMid = 1, kLast = 2, kOnly = 3, kHorner = 4 };
 
@@ -79,11 +80,33 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
     double acc[CL];
 #pragma unroll
     for (int q = 0; q < CL; ++q) acc[q] = 0.0;
-    for (int j = j0; j < j1; ++j) {
-      const double a = p.val[j];
-      const double* xr = p.x + (size_t)p.col[j] * p.ldx + c0;
+    // kUnroll nonzeros at a time: their val/col loads, then all their
+    // gathers, are issued before the first FMA waits, so a wavefront has
+    // kUnroll * CL gathers in flight instead of one dependent chain per
+    // nonzero.  The FMAs still run in column order.
+    for (int jb = j0; jb < j1; jb += kUnroll) {
+      double a[kUnroll];
+      int ci[kUnroll];
 #pragma unroll
-      for (int q = 0; q < CL; ++q) acc[q] = fma(a, xr[q * CPW], acc[q]);
+      for (int t = 0; t < kUnroll; ++t) {
+        const bool ok = jb + t < j1;
+        a[t] = ok ? p.val[jb + t] : 0.0;
+        ci[t] = ok ? p.col[jb + t] : -1;
+      }
+      double xv[kUnroll][CL];
+#pragma unroll
+      for (int t = 0; t < kUnroll; ++t) {
+        const double* xr = p.x + (size_t)(ci[t] < 0 ? r : ci[t]) * p.ldx + c0;
+#pragma unroll
+        for (int q = 0; q < CL; ++q) xv[t][q] = ci[t] < 0 ? 0.0 : xr[q * CPW];
+      }
+#pragma unroll
+      for (int t = 0; t < kUnroll; ++t) {
+        if (ci[t] >= 0) {
+#pragma unroll
+          for (int q = 0; q < CL; ++q) acc[q] = fma(a[t], xv[t][q], acc[q]);
+        }
+      }
     }
 #pragma unroll
     for (int q = 0; q < CL; ++q) {
@@ -147,13 +170,35 @@ __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
     al[q] = p.al[(size_t)p.k * R + c0 + q * CPW];
     nrm[q] = 0.0;
   }
-  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
+  // two rows per pass, all loads before the stores (w and vk are distinct
+  // buffers, which the compiler cannot assume)
+  constexpr int STEP = kWaves * RPW;
+  for (int r = row0 + wave * RPW + slot; r < row1; r += 2 * STEP) {
+    const int r2 = r + STEP;
+    const bool two = r2 < row1;
+    double wv[2][CL], vv[2][CL];
 #pragma unroll
     for (int q = 0; q < CL; ++q) {
       const size_t i = (size_t)r * R + c0 + q * CPW;
-      const double w = fma(-al[q], p.vk[i], p.w[i]);
-      p.w[i] = w;
+      wv[0][q] = p.w[i];
+      vv[0][q] = p.vk[i];
+      const size_t i2 = (size_t)(two ? r2 : r) * R + c0 + q * CPW;
+      wv[1][q] = p.w[i2];
+      vv[1][q] = p.vk[i2];
+    }
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const double w = fma(-al[q], vv[0][q], wv[0][q]);
+      p.w[(size_t)r * R + c0 + q * CPW] = w;
       nrm[q] = fma(w, w, nrm[q]);
+    }
+    if (two) {
+#pragma unroll
+      for (int q = 0; q < CL; ++q) {
+        const double w = fma(-al[q], vv[1][q], wv[1][q]);
+        p.w[(size_t)r2 * R + c0 + q * CPW] = w;
+        nrm[q] = fma(w, w, nrm[q]);
+      }
     }
   }
   block_partial_cols<R>(nrm, p.partials);
