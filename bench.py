@@ -298,6 +298,116 @@ def run_adjoint(args, steps, warm):
     print(json.dumps(result), flush=True)
 
 
+def build_ms512(args):
+    """Monodomain Mitchell-Schaeffer on a 512^2 anisotropic 9-point operator
+    (fibre angle 0 -> pi/3 across x, conductivity ratio 9:1), alpha = 1.7,
+    dt = 0.05, 20 deflation vectors reused every step, 16 stimulus patches of
+    20 x 20 nodes (one problem each)."""
+    import scipy.sparse.linalg as sla
+    from gpumatfun_amd import problems, quadrature as q
+    t0 = time.time()
+    n, K, P, dt = 512, 20, 16, 0.05
+    A = problems.anisotropic_9pt(n, ratio=9.0)
+    lam, Q = sla.eigsh(A, k=K + 1, sigma=0, which="LM")
+    order = np.argsort(lam)
+    lam, Q = lam[order], Q[:, order]
+    lN = float(sla.eigsh(A, k=1, which="LA", return_eigenvectors=False)[0])
+    N = n * n
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    stims = []
+    for k in range(P):
+        cx, cy = 40 + 120 * (k % 4), 40 + 120 * (k // 4)
+        stims.append(((X >= cx) & (X < cx + 20) & (Y >= cy) & (Y < cy + 20))
+                     .astype(float).reshape(-1, order="F"))
+    from gpumatfun_amd._lib import MS_DEFAULTS as prm
+    U = np.empty((N, 2 * P))
+    for k, u0 in enumerate(stims):
+        U[:, 2 * k] = u0
+        U[:, 2 * k + 1] = (u0 * u0 * (1 - u0)) / prm["tau_in"] - u0 / prm["tau_out"]
+    inp = q.make_inputs(A, U[:, 0], U[:, 1], m=40, n=32, eta=dt * 1e-3, alpha=1.7, dt=dt,
+                        l1=float(lam[K]), lN=lN, degree=0, ode=np.ones(N),
+                        Q=Q[:, :K], lam=lam[:K])
+    cfg = {"workload": "ms_aniso512_16stim", "n": N, "nnz": int(A.nnz), "problems": P,
+           "m": 40, "poly_degree": 0, "shifts": 32, "deflation": K, "alpha": 1.7, "dt": dt,
+           "eta": dt * 1e-3, "reaction": "Mitchell-Schaeffer", "fibre": "0 -> pi/3, ratio 9:1",
+           "stimuli": "16 patches of 20 x 20", "columns_per_step": 2 * P}
+    return inp, U, stims, dict(prm), cfg, time.time() - t0
+
+
+def run_ms512(args, steps, warm, use_graph):
+    from gpumatfun_amd.batch import BatchEngine
+    from gpumatfun_amd.engine import LanczosEngine
+    inp, U, stims, prm, cfg, setup_s = build_ms512(args)
+    N, P = inp.A.shape[0], len(stims)
+    log(f"setup {setup_s:.1f}s")
+    b = BatchEngine.from_inputs(inp, U, clamp=False, reaction="ms", ode=np.ones((N, P)), ms=prm)
+    b.step(1, use_graph=False)
+    b.synchronize()
+    U1 = b.get_U()
+    elapsed = timed(lambda k: b.step(k, use_graph=use_graph), b.synchronize, steps, warm)
+    value = 2.0 * P * steps / elapsed
+    log(f"batched: {steps} steps in {elapsed:.3f}s -> {value:.1f} solves/s")
+    info = b.info()
+    b.delete()
+    # the plain path: one engine per stimulus, stepped in turn (MS, same deflation)
+    engs = []
+    for k in range(P):
+        inp.U = U[:, 2 * k:2 * k + 2].copy()
+        e = LanczosEngine.from_inputs(inp, clamp=False, autotune=(k == 0))
+        e.set_ms_params(**prm)
+        engs.append(e)
+    e0 = engs[0]
+    e0.step(1, monodomain=2, use_graph=False)
+    e0.synchronize()
+    Up1 = e0.get_U()
+    psteps = max(5, min(steps, 100))
+
+    def plain_step(k):
+        for _ in range(k):
+            for e in engs:
+                e.step(1, monodomain=2, use_graph=use_graph)
+
+    def plain_sync():
+        for e in engs:
+            e.synchronize()
+
+    pel = timed(plain_step, plain_sync, psteps, 2)
+    plain = 2.0 * P * psteps / pel
+    for e in engs:
+        e.delete()
+    log(f"plain, stimulus by stimulus: {plain:.1f} solves/s; ratio {value / plain:.2f}")
+    result = {
+        "metric": "f(A)b solves/sec (FP64, batched monodomain)",
+        "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "ms_per_step": round(1000.0 * elapsed / steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded-free stimulus patches)", "config": cfg, "graph": use_graph,
+        "chunk_columns": info["chunk_columns"], "setup_s": round(setup_s, 1),
+        "plain_same_problems": {"value": round(plain, 3), "unit": "solves/s",
+                                "steps": psteps,
+                                "how": "one plain engine per stimulus, stepped in turn"},
+        "batched_vs_plain": round(value / plain, 3),
+        "batched_vs_plain_rel_err_step1": rel(U1[:, 0:2], Up1),
+        "roofline": None, "cpu_baseline": None,
+    }
+    if not args.no_cpu and not args.profile_only:
+        import oracle
+        S = oracle_setup_dict(inp)
+        t1 = time.perf_counter()
+        u, gu, h = oracle.cpu_steps_ms(S, stims[0], np.ones(N), 1, prm)
+        cpu_s = time.perf_counter() - t1
+        result["rel_err_vs_oracle"] = {"u": rel(U1[:, 0], u), "dt_g": rel(U1[:, 1], inp.dt * gu),
+                                       "steps": 1, "problem": 0, "tolerance": 1e-10}
+        result["cpu_baseline"] = {"value": round(2.0 / cpu_s, 6), "unit": "solves/s",
+                                  "cores": 1, "kind": "port",
+                                  "sample": "one time step of stimulus 0 by the oracle's NumPy "
+                                            "restatement (oracle/stepping.py:cpu_steps_ms)",
+                                  "seconds_per_step": round(cpu_s, 3),
+                                  "host_cpus": os.cpu_count()}
+        log(f"oracle step {cpu_s:.2f}s, rel err {result['rel_err_vs_oracle']}")
+    print(json.dumps(result), flush=True)
+
+
 def oracle_setup_dict(inp, A=None, perm=None):
     S = {"A": inp.A if A is None else A, "Q": inp.Q if inp.Q.shape[1] else None,
          "exp_lambda": inp.exp_lambda, "phi_lambda": inp.phi_lambda, "dt": inp.dt,
@@ -380,7 +490,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--config", default="fem1m",
-                    choices=["fem1m", "dirichlet128", "batch3d", "uneven64", "adjoint256"])
+                    choices=["fem1m", "dirichlet128", "batch3d", "uneven64", "adjoint256",
+                             "ms512"])
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--side3d", type=int, default=128)
     ap.add_argument("--problems", type=int, default=256)
@@ -418,6 +529,11 @@ def main():
 
     if args.config == "batch3d":
         run_batch(args, steps, warm, use_graph)
+        return
+    if args.config == "ms512":
+        if args.steps is None:
+            steps = 1000
+        run_ms512(args, steps, warm, use_graph)
         return
     if args.config == "adjoint256":
         run_adjoint(args, steps, warm)

@@ -17,6 +17,9 @@ GM_FLAG_NO_CLAMP = 1
 GM_FLAG_AUTOTUNE = 2
 GM_FLAG_ROW_GROUPS = 4
 GM_FLAG_ROW_BLOCKS = 8
+GM_REACTION_FISHER, GM_REACTION_MS = 0, 2
+# Mitchell-Schaeffer defaults (the library's own, gm_solver.hip)
+MS_DEFAULTS = dict(tau_in=0.3, tau_out=6.0, tau_open=120.0, tau_close=150.0, v_gate=0.13)
 
 STATUS_NAMES = {0: "GM_OK", 1: "GM_EINVAL", 2: "GM_ENOMEM", 3: "GM_EDEVICE",
                 4: "GM_EBREAKDOWN", 5: "GM_EBADHANDLE", 6: "GM_ESTATE"}
@@ -30,6 +33,7 @@ EXPORTS = (
     "gm_profile_spmm", "gm_profile_copy", "gm_set_ms_params", "gm_update_U_ms",
     "gm_batch_last_error", "gm_batch_create", "gm_batch_delete", "gm_batch_step",
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
+    "gm_batch_get_ode",
     "gm_fab_last_error", "gm_fab_create", "gm_fab_delete", "gm_fab_set_matrix",
     "gm_fab_set_weights", "gm_fab_forward", "gm_fab_backward",
     "gm_ubatch_last_error", "gm_ubatch_create", "gm_ubatch_delete", "gm_ubatch_step",
@@ -110,6 +114,13 @@ class GmBatchDesc(C.Structure):
         ("chunk_columns", C.c_int32),
         ("flags", C.c_uint32),
         ("device", C.c_int32),
+        ("num_eigs", C.c_int32),
+        ("Q", _dp),
+        ("exp_lambda", _dp),
+        ("phi_lambda", _dp),
+        ("reaction", C.c_int32),
+        ("ode", _dp),
+        ("ms", GmMsParams),
     ]
 
 
@@ -215,6 +226,7 @@ def load():
         "gm_batch_synchronize": (C.c_int, [vp]),
         "gm_batch_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "gm_batch_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
+        "gm_batch_get_ode": (C.c_int, [vp, _dp]),
         "gm_fab_last_error": (C.c_char_p, []),
         "gm_fab_create": (C.c_int, [C.POINTER(GmFabDesc), C.POINTER(vp)]),
         "gm_fab_delete": (C.c_int, [vp]),

@@ -18,8 +18,18 @@ def _p(a):
 
 
 class BatchEngine:
+    """P problems sharing A, stepped together (``gm_batch_*``).
+
+    Optional: deflation by shared eigenpairs (``Q`` n x K, ``exp_lambda``,
+    ``phi_lambda``), and ``reaction="ms"`` (Mitchell-Schaeffer, gate ``ode``
+    n x P, parameters ``ms`` as for ``LanczosEngine.set_ms_params``) instead
+    of Fisher-KPP.
+    """
+
     def __init__(self, A, U, *, m, gamma, gamma_s_real, gamma_s_imag, wts1, wts2, etaxi, dt,
-                 mass=None, chunk_columns=0, clamp=True, autotune=False, device=0):
+                 mass=None, chunk_columns=0, clamp=True, autotune=False, device=0,
+                 Q=None, exp_lambda=None, phi_lambda=None, reaction="fisher", ode=None,
+                 ms=None):
         lib = _lib.load()
         self._lib = lib
         self._h = C.c_void_p()
@@ -75,15 +85,48 @@ class BatchEngine:
         d.chunk_columns = int(chunk_columns)
         d.flags = (0 if clamp else _lib.GM_FLAG_NO_CLAMP) | (_lib.GM_FLAG_AUTOTUNE if autotune else 0)
         d.device = int(device)
+        K = 0 if Q is None else np.asarray(Q).reshape(n, -1).shape[1]
+        if K:
+            keep["Q"] = np.ascontiguousarray(np.asarray(Q, dtype=np.float64).reshape(n, K)
+                                             .reshape(-1, order="F"))
+            keep["el"] = np.ascontiguousarray(exp_lambda, dtype=np.float64).ravel()
+            keep["pl"] = np.ascontiguousarray(phi_lambda, dtype=np.float64).ravel()
+            if keep["el"].size != K or keep["pl"].size != K:
+                raise ValueError("exp_lambda and phi_lambda need K values")
+            d.Q, d.exp_lambda, d.phi_lambda = _p(keep["Q"]), _p(keep["el"]), _p(keep["pl"])
+        d.num_eigs = K
+        if reaction not in ("fisher", "ms"):
+            raise ValueError("reaction must be 'fisher' or 'ms'")
+        d.reaction = _lib.GM_REACTION_MS if reaction == "ms" else _lib.GM_REACTION_FISHER
+        if reaction == "ms":
+            prm = dict(_lib.MS_DEFAULTS)
+            prm.update(ms or {})
+            d.ms = _lib.GmMsParams(**prm)
+            if ode is not None:
+                keep["ode"] = np.ascontiguousarray(np.asarray(ode, dtype=np.float64)
+                                                   .reshape(n, twoP // 2).reshape(-1, order="F"))
+                d.ode = _p(keep["ode"])
         _lib.check(lib.gm_batch_create(C.byref(d), C.byref(self._h)), batch=True)
+        self.reaction = reaction
         self.n, self.P = n, twoP // 2
 
     @classmethod
     def from_inputs(cls, inp, U, **kw):
-        """Share A, contour and polynomial of a plain-path ``CreateInputs``; U is n x 2P."""
+        """Share A, contour, polynomial and deflation of a plain-path ``CreateInputs``;
+        U is n x 2P."""
+        if inp.Q.shape[1]:
+            kw.setdefault("Q", inp.Q)
+            kw.setdefault("exp_lambda", inp.exp_lambda)
+            kw.setdefault("phi_lambda", inp.phi_lambda)
         return cls(inp.A, U, m=inp.m, gamma=inp.gamma, gamma_s_real=inp.gamma_s_real,
                    gamma_s_imag=inp.gamma_s_imag, wts1=inp.wts1, wts2=inp.wts2,
                    etaxi=inp.extra["etaxi"], dt=inp.dt, mass=inp.mass, **kw)
+
+    def get_ode(self):
+        """n x P Mitchell-Schaeffer gates."""
+        out = np.empty(self.n * self.P)
+        self._call(self._lib.gm_batch_get_ode, _p(out))
+        return out.reshape(self.P, self.n).T.copy()
 
     def _call(self, fn, *args):
         if not self._h:

@@ -15,8 +15,11 @@
 //
 // Per step and chunk: build (m iterations of SpMM kernels + finish), shifted
 // solves (Rc x n_s systems, one lane each), collapse, V*C, Horner SpMMs, and
-// the Fisher-KPP update (update_U.cu:8-16, optional clamp) with the column
-// norms for the next step.  No deflation in the batched form.
+// the update: Fisher-KPP (update_U.cu:8-16, optional clamp) or
+// Mitchell-Schaeffer (gate h per problem).  With K shared deflation vectors
+// the step ends with Z = Q^T Usol on FP64 MFMA over all R columns, then the
+// deflated starting vectors Ud = Usol - Q Z and upd = Q f(Lambda) Z
+// (lanczos_impl.cu:84-89, update_U.cu:18-27).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -84,6 +87,15 @@ struct gm_batch {
   double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
   unsigned *counter = nullptr, *status = nullptr;
   int G_sp = 1, chunk_sp = 1;
+  // deflation by the shared (Q, Lambda): Q row-major [N][K], Zd [K][R],
+  // Ud = deflated starting vectors, upd = the deflated part's update
+  int K = 0, G_df = 1, chunk_df = 1;
+  double *Q = nullptr, *el = nullptr, *pl = nullptr, *Zd = nullptr, *Ud = nullptr,
+         *upd = nullptr;
+  // reaction: GM_REACTION_FISHER or GM_REACTION_MS (gate h in ode, [N][P])
+  int reaction = 0;
+  double* ode = nullptr;
+  double ms[5] = {0.3, 6.0, 120.0, 150.0, 0.13};
   hipGraphExec_t graph = nullptr;
 };
 
@@ -106,16 +118,16 @@ void bfree_all(gm_batch* b) {
   void* bufs[] = {b->rowptr, b->col, b->val, b->Usol, b->X, b->W0, b->W1, b->T0, b->T1, b->V,
                   b->Z, b->C, b->al, b->be, b->yr, b->yi, b->gsr, b->gsi, b->wts1, b->wts2,
                   b->etaxi, b->mass, b->partials, b->stage, b->wD, b->wDU, b->wDU2, b->wX,
-                  b->counter, b->status};
+                  b->counter, b->status, b->Q, b->el, b->pl, b->Zd, b->Ud, b->upd, b->ode};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
-gm_status breduce(gm_batch* b, int nv, double* out, int do_sqrt, int flag_zero) {
+gm_status breduce(gm_batch* b, int nv, double* out, int do_sqrt, int flag_zero, int G = 0) {
   gm::ReduceArgs r{};
   r.partials = b->partials;
-  r.G = b->G_sp;
+  r.G = G > 0 ? G : b->G_sp;
   r.nv = nv;
   r.out = out;
   r.stride = 1;
@@ -182,7 +194,7 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
   // the chunk's starting vectors: Usol columns c0..c0+R-1 (row stride R_total)
   {
     gmb::BNorm nn{};
-    nn.u = b->Usol + c0;
+    nn.u = (b->K ? b->Ud : b->Usol) + c0;
     nn.ldu = ldu;
     nn.be = b->be;
     nn.nrows = b->n;
@@ -196,7 +208,7 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
   }
   for (int k = 0; k < b->m; ++k) {
     gmb::BSpmm a = base;
-    const double* wsrc = k == 0 ? b->Usol + c0 : Wb[(k - 1) & 1];
+    const double* wsrc = k == 0 ? (b->K ? b->Ud : b->Usol) + c0 : Wb[(k - 1) & 1];
     const size_t ldw = k == 0 ? ldu : (size_t)R;
     double* wnext = Wb[k & 1];
     a.k = k;
@@ -316,6 +328,15 @@ gm_status enqueue_update_chunk(gm_batch* b, int c0, int prepare) {
   gmb::BUpdate u{};
   u.X = b->X + c0;
   u.Usol = b->Usol + c0;
+  u.upd = b->K ? b->upd + c0 : nullptr;
+  u.reaction = b->reaction;
+  u.ode = b->ode ? b->ode + c0 / 2 : nullptr;
+  u.ldh = (size_t)b->P;
+  u.ms_tau_in = b->ms[0];
+  u.ms_tau_out = b->ms[1];
+  u.ms_tau_open = b->ms[2];
+  u.ms_tau_close = b->ms[3];
+  u.ms_v_gate = b->ms[4];
   u.mass = b->mass;
   u.dt = b->dt;
   u.clamp = b->clamp ? 1 : 0;
@@ -360,10 +381,46 @@ gm_status dispatch_update(gm_batch* b, int c0, int prepare) {
   return bfail(GM_EINVAL, "unsupported chunk width %d", b->Rc);
 }
 
-// A whole step: every chunk's f(A) products, then every chunk's update.
+// Z = Q^T Usol (MFMA partials + fixed-order reduce), then Ud = Usol - Q Z
+// and upd = Q f(Lambda) Z for all R columns.
+gm_status enqueue_deflate(gm_batch* b) {
+  gmb::BDeflQtU q{};
+  q.Q = b->Q;
+  q.U = b->Usol;
+  q.K = b->K;
+  q.R = b->R;
+  q.nrows = b->n;
+  q.chunk = b->chunk_df;
+  q.KT = cdivi(b->K, 16);
+  q.CT = cdivi(b->R, 16);
+  q.partials = b->partials;
+  hipLaunchKernelGGL(gmb::k_bdefl_qtu<0>, dim3(b->G_df, cdivi((long long)q.KT * q.CT, gm::kWaves)),
+                     dim3(gm::kBlock), 0, b->stream, q);
+  B_HIP(hipGetLastError());
+  B_TRY(breduce(b, b->K * b->R, b->Zd, 0, 0, b->G_df));
+  gmb::BDeflApply a{};
+  a.Q = b->Q;
+  a.Z = b->Zd;
+  a.el = b->el;
+  a.pl = b->pl;
+  a.Usol = b->Usol;
+  a.Ud = b->Ud;
+  a.upd = b->upd;
+  a.K = b->K;
+  a.R = b->R;
+  a.nrows = b->n;
+  hipLaunchKernelGGL(gmb::k_bdefl_apply<0>, dim3(std::min(4096, cdivi(b->n, gm::kWaves)), cdivi(b->R, 64)),
+                     dim3(gm::kBlock), 2 * (size_t)b->K * 64 * sizeof(double), b->stream, a);
+  B_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+// A whole step: every chunk's f(A) products, then every chunk's update, then
+// the deflation of the new starting vectors.
 gm_status enqueue_bstep(gm_batch* b) {
   for (int c0 = 0; c0 < b->R; c0 += b->Rc) B_TRY(dispatch_chunk(b, c0));
   for (int c0 = 0; c0 < b->R; c0 += b->Rc) B_TRY(dispatch_update(b, c0, 0));
+  if (b->K) B_TRY(enqueue_deflate(b));
   b->x_zero = true;
   return GM_OK;
 }
@@ -392,7 +449,8 @@ gm_status alloc_chunk_buffers(gm_batch* b) {
   B_TRY(balloc(&b->wDU, ny));
   B_TRY(balloc(&b->wDU2, ny));
   B_TRY(balloc(&b->wX, ny));
-  B_TRY(balloc(&b->partials, 4096 * std::max<size_t>(Rc, 2)));
+  B_TRY(balloc(&b->partials, std::max<size_t>(4096 * std::max<size_t>(Rc, 2),
+                                              (size_t)b->G_df * b->K * b->R)));
   return GM_OK;
 }
 
@@ -429,6 +487,14 @@ gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
     return bfail(GM_EINVAL, "bad contour data");
   if (d->sparse_format != GM_SPARSE_CSR && d->sparse_format != GM_SPARSE_CSC)
     return bfail(GM_EINVAL, "bad sparse format");
+  if (d->num_eigs < 0 || d->num_eigs > 64 ||
+      (d->num_eigs > 0 && (!d->Q || !d->exp_lambda || !d->phi_lambda)))
+    return bfail(GM_EINVAL, "num_eigs must be 0..64, with Q, exp_lambda and phi_lambda");
+  if (d->reaction != GM_REACTION_FISHER && d->reaction != GM_REACTION_MS)
+    return bfail(GM_EINVAL, "reaction must be GM_REACTION_FISHER or GM_REACTION_MS");
+  if (d->reaction == GM_REACTION_MS &&
+      !(d->ms.tau_in > 0 && d->ms.tau_out > 0 && d->ms.tau_open > 0 && d->ms.tau_close > 0))
+    return bfail(GM_EINVAL, "Mitchell-Schaeffer time constants must be positive");
   if (d->A_ptr[0] != 0 || d->A_ptr[d->n] != d->nnz) return bfail(GM_EINVAL, "bad pointer array");
   for (int i = 0; i < d->n; ++i)
     if (d->A_ptr[i + 1] < d->A_ptr[i]) return bfail(GM_EINVAL, "pointer array not monotone");
@@ -518,6 +584,37 @@ gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
   std::vector<double> ones;
   if (!d->mass) ones.assign(N, 1.0);
   B_CREATE(up(&b->mass, d->mass ? d->mass : ones.data(), N));
+  b->reaction = d->reaction;
+  if (b->reaction == GM_REACTION_MS) {
+    b->ms[0] = d->ms.tau_in;
+    b->ms[1] = d->ms.tau_out;
+    b->ms[2] = d->ms.tau_open;
+    b->ms[3] = d->ms.tau_close;
+    b->ms[4] = d->ms.v_gate;
+    // gate h, n x P column-major -> [N][P]
+    std::vector<double> h((size_t)N * b->P, 1.0);
+    if (d->ode)
+      for (int q = 0; q < b->P; ++q)
+        for (size_t i = 0; i < N; ++i) h[i * b->P + q] = d->ode[(size_t)q * N + i];
+    B_CREATE(up(&b->ode, h.data(), h.size()));
+    if (hipStreamSynchronize(st) != hipSuccess) return fail_clean(bfail(GM_EDEVICE, "sync"));
+  }
+  b->K = d->num_eigs;
+  b->G_df = std::max(1, std::min(256, cdivi(b->n, 1024)));
+  b->chunk_df = cdivi(b->n, b->G_df);
+  if (b->K) {
+    // Q n x K column-major -> [N][K]
+    std::vector<double> qr(N * b->K);
+    for (int k = 0; k < b->K; ++k)
+      for (size_t i = 0; i < N; ++i) qr[i * b->K + k] = d->Q[(size_t)k * N + i];
+    B_CREATE(up(&b->Q, qr.data(), qr.size()));
+    B_CREATE(up(&b->el, d->exp_lambda, (size_t)b->K));
+    B_CREATE(up(&b->pl, d->phi_lambda, (size_t)b->K));
+    B_CREATE(balloc(&b->Zd, (size_t)b->K * b->R));
+    B_CREATE(balloc(&b->Ud, N * b->R));
+    B_CREATE(balloc(&b->upd, N * b->R));
+    if (hipStreamSynchronize(st) != hipSuccess) return fail_clean(bfail(GM_EDEVICE, "sync"));
+  }
   B_CREATE(balloc(&b->stage, N * b->R));
   B_CREATE(balloc(&b->counter, 1));
   B_CREATE(balloc(&b->status, 1));
@@ -534,6 +631,8 @@ gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
     if (s != GM_ENOMEM || b->Rc <= 2) return fail_clean(s);
     b->Rc /= 2;
   }
+  // deflate the initial vectors (lanczos_impl.cu:84-89)
+  if (b->K) B_CREATE(enqueue_deflate(b));
   if (hipStreamSynchronize(st) != hipSuccess) return fail_clean(bfail(GM_EDEVICE, "sync"));
   if (d->flags & GM_FLAG_AUTOTUNE) {
     // Time one chunk of the step for each chunk width that the allocated
@@ -650,6 +749,21 @@ gm_status gm_batch_info(gm_batch* b, int32_t* num_problems, int32_t* chunk_colum
   if (!blive(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
   if (num_problems) *num_problems = b->P;
   if (chunk_columns) *chunk_columns = b->Rc;
+  return GM_OK;
+}
+
+gm_status gm_batch_get_ode(gm_batch* b, double* dst) {
+  if (!blive(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
+  if (!dst) return bfail(GM_EINVAL, "null destination");
+  if (!b->ode) return bfail(GM_ESTATE, "the batch has no gate state (reaction is not MS)");
+  B_HIP(hipSetDevice(b->dev));
+  const size_t N = (size_t)b->n;
+  std::vector<double> tmp(N * b->P);
+  B_HIP(hipMemcpyAsync(tmp.data(), b->ode, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       b->stream));
+  B_HIP(hipStreamSynchronize(b->stream));
+  for (int q = 0; q < b->P; ++q)
+    for (size_t i = 0; i < N; ++i) dst[(size_t)q * N + i] = tmp[i * b->P + q];
   return GM_OK;
 }
 

@@ -391,15 +391,23 @@ __global__ __launch_bounds__(kBlock) void k_bvc(BVC p) {
   }
 }
 
-// Fisher-KPP update per column pair (update_U.cu:8-16), column norms after.
+// Update per column pair: u = X0 + X1 (+ upd0 + upd1 with deflation), then
+// the reaction -- Fisher-KPP (update_U.cu:8-16) or Mitchell-Schaeffer (gate
+// h per problem, forward Euler with the new u, g = h u^2 (1-u)/tau_in -
+// u/tau_out) -- and Usol = [u, dt g].
 struct BUpdate {
   const double* X;    // ld ldu (chunk columns at offset)
   double* Usol;       // ld ldu
+  const double* upd;  // ld ldu, or null (no deflation)
   const double* mass;
   double dt;
   int clamp, x_zero, prepare;
   size_t ldu;
   int R, nrows, chunk;
+  int reaction;       // GM_REACTION_FISHER or GM_REACTION_MS
+  double* ode;        // MS: gate h, [N][ldh] with the chunk's first problem at offset
+  size_t ldh;
+  double ms_tau_in, ms_tau_out, ms_tau_open, ms_tau_close, ms_v_gate;
   double* be;         // be[0][c]
   double* partials;
   unsigned* counter;
@@ -430,12 +438,26 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(BUpdate p) {
         // both columns of the pair are read by both lanes of the pair
         const size_t i0 = i - (c & 1);
         double u = p.x_zero ? 0.0 : (p.X[i0] + p.X[i0 + 1]);
-        if (p.clamp) u = u * (u >= 0.0 ? 1.0 : 0.0);
-        if (c & 1) {
-          const double ub = u / ms;
-          val = ((ms * ub) * (1.0 - ub)) * p.dt;
+        if (p.upd) u = (u + p.upd[i0]) + p.upd[i0 + 1];
+        if (p.reaction == 2) {
+          if (c & 1) {
+            double* hp = p.ode + (size_t)r * p.ldh + (c >> 1);
+            const double h = *hp;
+            const double dh = u < p.ms_v_gate ? (1.0 - h) / p.ms_tau_open : -h / p.ms_tau_close;
+            const double hn = h + p.dt * dh;
+            *hp = hn;
+            val = ((hn * (u * u) * (1.0 - u)) / p.ms_tau_in - u / p.ms_tau_out) * p.dt;
+          } else {
+            val = u;
+          }
         } else {
-          val = u;
+          if (p.clamp) u = u * (u >= 0.0 ? 1.0 : 0.0);
+          if (c & 1) {
+            const double ub = u / ms;
+            val = ((ms * ub) * (1.0 - ub)) * p.dt;
+          } else {
+            val = u;
+          }
         }
       }
       if (!p.prepare) p.Usol[i] = val;   // X and Usol are distinct buffers
@@ -443,6 +465,91 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(BUpdate p) {
   }
   // the column norms of the next step's starting vectors are taken per chunk
   // by k_bnorm
+}
+
+// ---------------------------------------------------------------------------
+// Deflation of all R columns by the shared (Q, Lambda), lanczos_impl.cu:84-89
+// and update_U.cu:18-27 per column:  Z = Q^T Usol,  Ud = Usol - Q Z,
+// upd = Q diag(f) Z with f = exp_lambda (even columns) or phi_lambda (odd).
+// ---------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Block partials of Z = Q^T Usol on FP64 MFMA (v_mfma_f64_16x16x4f64): block
+// x covers rows [x chunk, (x+1) chunk), each wavefront one 16 x 16 tile of Z
+// (16 deflation vectors x 16 columns), the row dimension stepped by 4.
+// Operand layout: A lane l = (row l%16, k l/16), B lane l = (k l/16, col
+// l%16); D register i of lane l = (row 4i + l/16, col l%16).
+struct BDeflQtU {
+  const double* Q;    // [N][K]
+  const double* U;    // [N][R]
+  int K, R, nrows, chunk, KT, CT;
+  double* partials;   // [G][K][R]
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_bdefl_qtu(BDeflQtU p) {
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.y * kWaves + (threadIdx.x >> 6);
+  if (tile >= p.KT * p.CT) return;   // whole wavefronts; no barrier follows
+  const int kt = tile % p.KT, ct = tile / p.KT;
+  const int row0 = blockIdx.x * p.chunk, row1 = min(row0 + p.chunk, p.nrows);
+  const int kk = kt * 16 + (lane & 15);
+  const int c = ct * 16 + (lane & 15);
+  const int sub = lane >> 4;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int r = row0; r < row1; r += 4) {
+    const int row = r + sub;
+    const bool ok = row < row1;
+    const double a = (ok && kk < p.K) ? p.Q[(size_t)row * p.K + kk] : 0.0;
+    const double b = (ok && c < p.R) ? p.U[(size_t)row * p.R + c] : 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k2 = kt * 16 + 4 * i + sub;
+    if (k2 < p.K && c < p.R) p.partials[((size_t)blockIdx.x * p.K + k2) * p.R + c] = acc[i];
+  }
+}
+
+// Ud = Usol - Q Z and upd = Q (f Z); block y covers 64 columns, whose Z and
+// f Z are staged in LDS (2 K 64 doubles); one wavefront per row.
+struct BDeflApply {
+  const double* Q;    // [N][K]
+  const double* Z;    // [K][R]
+  const double* el;   // K
+  const double* pl;   // K
+  const double* Usol; // [N][R]
+  double* Ud;
+  double* upd;
+  int K, R, nrows;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_bdefl_apply(BDeflApply p) {
+  extern __shared__ double sz[];     // [K][64] Z, then [K][64] f Z
+  const int cg = blockIdx.y * 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = threadIdx.x; t < p.K * 64; t += kBlock) {
+    const int k = t / 64, cc = cg + t % 64;
+    const double z = cc < p.R ? p.Z[(size_t)k * p.R + cc] : 0.0;
+    sz[t] = z;
+    sz[p.K * 64 + t] = z * ((cc & 1) ? p.pl[k] : p.el[k]);
+  }
+  __syncthreads();
+  const int c = cg + lane;
+  if (c >= p.R) return;
+  for (int r = blockIdx.x * kWaves + wave; r < p.nrows; r += gridDim.x * kWaves) {
+    const double* q = p.Q + (size_t)r * p.K;
+    double a = 0.0, u = 0.0;
+    for (int k = 0; k < p.K; ++k) {
+      const double qv = q[k];
+      a = fma(qv, sz[k * 64 + lane], a);
+      u = fma(qv, sz[p.K * 64 + k * 64 + lane], u);
+    }
+    const size_t i = (size_t)r * p.R + c;
+    p.Ud[i] = p.Usol[i] - a;
+    p.upd[i] = u;
+  }
 }
 
 }  // namespace gmb

@@ -204,9 +204,13 @@ gm_status gm_profile_copy(gm_solver* s, int64_t bytes, int32_t reps, double* avg
  * together.  Problem p owns the column pair [u_p, g_p] (the plain path's exp
  * and phi columns), so one step is 2P f(A)b solves; the engine processes
  * columns in chunks whose Lanczos bases fit in device memory.  The step is
- * the plain path's step without deflation (update_U.cu:8-16 per pair).
+ * the plain path's step per pair: deflation by the shared (Q, Lambda) when
+ * num_eigs > 0 (lanczos_impl.cu:84-89, update_U.cu:18-27), and the Fisher-KPP
+ * (update_U.cu:8-16) or Mitchell-Schaeffer reaction.
  * ------------------------------------------------------------------------- */
 typedef struct gm_batch gm_batch;
+
+enum { GM_REACTION_FISHER = 0, GM_REACTION_MS = 2 };
 
 typedef struct gm_batch_desc {
     int32_t n;
@@ -234,6 +238,13 @@ typedef struct gm_batch_desc {
                                       each chunk width <= the allocated one and
                                       keeps the cheapest per column          */
     int32_t device;
+    int32_t num_eigs;              /* K <= 64 shared deflation vectors (0: none) */
+    const double* Q;               /* n x K column-major                     */
+    const double* exp_lambda;      /* K                                      */
+    const double* phi_lambda;      /* K                                      */
+    int32_t reaction;              /* GM_REACTION_FISHER or GM_REACTION_MS   */
+    const double* ode;             /* MS: n x P column-major gate h (NULL: ones) */
+    gm_ms_params ms;               /* MS parameters                          */
 } gm_batch_desc;
 
 const char* gm_batch_last_error(void);
@@ -246,6 +257,8 @@ gm_status gm_batch_get_U(gm_batch* b, double* dst);
 gm_status gm_batch_synchronize(gm_batch* b);
 gm_status gm_batch_info(gm_batch* b, int32_t* num_problems, int32_t* chunk_columns);
 gm_status gm_batch_get_stream(gm_batch* b, void** stream);
+/* dst: n x P column-major MS gate h of every problem.                      */
+gm_status gm_batch_get_ode(gm_batch* b, double* dst);
 
 /* ---------------------------------------------------------------------------
  * Differentiation (no reference counterpart): u = f(A) b for one f given by

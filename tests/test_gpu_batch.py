@@ -81,3 +81,86 @@ def test_batch_agrees_with_plain_engine():
         assert rel(out[:, 2 * p + 1], ref[:, 1]) < TOL
         e.delete()
     b.delete()
+
+
+def test_batch_deflation_fisher_vs_oracle():
+    """Shared deflation (K = 6) and lumped mass on the FEM disc (m = 30, where the
+    oracle agrees with itself to 5e-12): each problem against
+    problem_startup_CPU_nonuniform.m's loop, and against the plain engine."""
+    from cases import fem_case
+    inp = fem_case(m=30, n=16)
+    N = inp.A.shape[0]
+    U = np.empty((N, 8))
+    for p in range(4):
+        w = np.random.default_rng(40 + p).random(N)
+        U[:, 2 * p] = inp.mass * w
+        U[:, 2 * p + 1] = inp.mass * w * (1 - w)
+    print("case batch deflation K=6 P=4", flush=True)
+    b = BatchEngine.from_inputs(inp, U, clamp=False)
+    b.step(3)
+    out = b.get_U()
+    b.synchronize()
+    b.delete()
+    for p in range(4):
+        u, gu = oracle_pair(inp, U[:, 2 * p], U[:, 2 * p + 1], 3)
+        print(p, rel(out[:, 2 * p], u), rel(out[:, 2 * p + 1], inp.dt * gu))
+        assert rel(out[:, 2 * p], u) < TOL
+        assert rel(out[:, 2 * p + 1], inp.dt * gu) < TOL
+    inp.U = U[:, 0:2].copy()
+    e = LanczosEngine.from_inputs(inp, clamp=False)
+    e.step(3)
+    assert rel(out[:, 0:2], e.get_U()) < 1e-12
+    e.delete()
+
+
+def _ms_case(n=24, K=0, m=30, ns=16, dt=0.05):
+    from gpumatfun_amd import problems, quadrature as q
+    A = problems.anisotropic_9pt(n, ratio=9.0)
+    w, V = np.linalg.eigh(A.toarray())
+    N = n * n
+    p = oracle.stepping.MS_DEFAULT
+    u0 = np.zeros(N)
+    h0 = np.ones(N)
+    l1 = float(w[K])
+    inp = q.make_inputs(A, u0, u0, m=m, n=ns, eta=dt * 1e-3, alpha=1.7, dt=dt, l1=l1,
+                        lN=float(w.max()), degree=0, ode=h0, Q=V[:, :K] if K else None,
+                        lam=w[:K] if K else None)
+    return inp, p
+
+
+def _stimuli(n, P, size=5):
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    out = []
+    for k in range(P):
+        cx, cy = (3 * k) % (n - size), (7 * k) % (n - size)
+        out.append(((X >= cx) & (X < cx + size) & (Y >= cy) & (Y < cy + size))
+                   .astype(float).reshape(-1, order="F"))
+    return out
+
+
+@pytest.mark.parametrize("K", [0, 8])
+def test_batch_mitchell_schaeffer_vs_oracle(K):
+    """Mitchell-Schaeffer with P = 4 stimulus patches on the anisotropic 9-point
+    operator, without and with shared deflation; u, dt g and the gate h of every
+    problem against the oracle's step loop."""
+    n, P, steps = 24, 4, 5
+    inp, prm = _ms_case(n, K=K)
+    N = n * n
+    stims = _stimuli(n, P)
+    U = np.empty((N, 2 * P))
+    for k, u0 in enumerate(stims):
+        U[:, 2 * k] = u0
+        U[:, 2 * k + 1] = (u0 ** 2 * (1 - u0)) / prm["tau_in"] - u0 / prm["tau_out"]
+    print(f"case batch MS K={K} P={P}", flush=True)
+    b = BatchEngine.from_inputs(inp, U, clamp=False, reaction="ms", ode=np.ones((N, P)), ms=prm)
+    b.step(steps)
+    out = b.get_U()
+    h = b.get_ode()
+    b.synchronize()
+    b.delete()
+    S = oracle_setup(inp)
+    for k, u0 in enumerate(stims):
+        u, gu, ho = oracle.cpu_steps_ms(S, u0, np.ones(N), steps, prm)
+        e = (rel(out[:, 2 * k], u), rel(out[:, 2 * k + 1], inp.dt * gu), rel(h[:, k], ho))
+        print(k, e)
+        assert max(e) < TOL
