@@ -37,10 +37,18 @@ HBM_PEAK_GBS = 8000.0
 
 def kernel_symbol(info, mode):
     """Demangled name of the SpMM kernel the solver runs in `mode` (1 = kMid, 2 = kLast)."""
-    lanes = info["spmm_lanes"]
-    if lanes > 0:
-        return f"void gm::k_spmm<{lanes}, {mode}>(gm::SpmmArgs)"
-    return f"void gm::k_spmm_stream<{-lanes}, {mode}, false>(gm::SpmmArgs)"
+    if info["spmm_kind"] == 0:
+        return f"void gm::k_spmm<{info['spmm_param']}, {mode}>(gm::SpmmArgs)"
+    c16 = "true" if info["index_bytes"] == 2 else "false"
+    return f"void gm::k_spmm_stream<{info['spmm_param']}, {mode}, false, {c16}>(gm::SpmmArgs)"
+
+
+def spmm_bytes(info, mode):
+    """Algorithmic bytes of one SpMM launch: values and stored column indices,
+    row pointers, and 16 B per row for each of x (gathered once), the row's
+    own operand(s) and y (kLast adds v_{k-1})."""
+    n, nnz = info["n"], info["nnz"]
+    return (8 + info["index_bytes"]) * nnz + 4 * (n + 1) + (64 if mode == 2 else 48) * n
 
 
 def kernel_key(info, mode):
@@ -568,7 +576,7 @@ def main():
     n, nnz = info["n"], info["nnz"]
     mode = 0 if inp.gamma.size > 1 else 1
     spmm_ms = eng.profile_spmm(mode=mode, reps=10 if args.profile_only else 200)
-    bytes_per = 12 * nnz + 4 * (n + 1) + (48 if mode == 0 else 64) * n
+    bytes_per = spmm_bytes(info, 1 if mode == 0 else 2)
     achieved = bytes_per / (spmm_ms * 1e-3) / 1e9
     log(f"spmm mode {mode}: {spmm_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
     copy_bytes = 2 * 16 * n * 8   # read + write of 8 column pairs
@@ -635,7 +643,7 @@ def run_batch(args, steps, warm, use_graph):
     mode = 1
     spmm_ms = e.profile_spmm(mode=mode, reps=10 if args.profile_only else 100)
     n, nnz = einfo["n"], einfo["nnz"]
-    bytes_per = 12 * nnz + 4 * (n + 1) + 64 * n
+    bytes_per = spmm_bytes(einfo, 2)
     e.delete()
     log(f"plain same matrix: {plain:.1f} solves/s; batched/plain = {value / plain:.2f}")
     result = {

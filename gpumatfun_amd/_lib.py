@@ -17,6 +17,7 @@ GM_FLAG_NO_CLAMP = 1
 GM_FLAG_AUTOTUNE = 2
 GM_FLAG_ROW_GROUPS = 4
 GM_FLAG_ROW_BLOCKS = 8
+GM_FLAG_INDEX32 = 16
 GM_REACTION_FISHER, GM_REACTION_MS = 0, 2
 # Mitchell-Schaeffer defaults (the library's own, gm_solver.hip)
 MS_DEFAULTS = dict(tau_in=0.3, tau_out=6.0, tau_open=120.0, tau_close=150.0, v_gate=0.13)
@@ -29,7 +30,8 @@ EXPORTS = (
     "gm_last_error", "gm_lanczos_create", "gm_lanczos_delete", "gm_build_subspace",
     "gm_lin_sys_solves", "gm_apply_V", "gm_update_U", "gm_update_U_monodomain",
     "gm_get_U", "gm_get_xsol", "gm_get_aabb", "gm_set_y", "gm_get_y", "gm_get_state",
-    "gm_set_state", "gm_step", "gm_synchronize", "gm_get_info", "gm_get_stream",
+    "gm_set_state", "gm_step", "gm_synchronize", "gm_get_info", "gm_get_schedule",
+    "gm_get_stream",
     "gm_profile_spmm", "gm_profile_copy", "gm_set_ms_params", "gm_update_U_ms",
     "gm_batch_last_error", "gm_batch_create", "gm_batch_delete", "gm_batch_step",
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
@@ -214,6 +216,8 @@ def load():
                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32)]),
         "gm_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
+        "gm_get_schedule": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32)]),
         "gm_profile_spmm": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "gm_profile_copy": (C.c_int, [vp, C.c_int64, C.c_int32, _dp]),
         "gm_set_ms_params": (C.c_int, [vp, C.POINTER(GmMsParams)]),

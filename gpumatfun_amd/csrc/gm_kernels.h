@@ -46,6 +46,7 @@ struct SpmmArgs {
   const int* __restrict__ col;
   const double* __restrict__ val;
   const int* __restrict__ rowblk;  // row-block bounds (k_spmm_stream only)
+  const short* __restrict__ col16;  // C16: column - first row of the nonzero's row block
   const int* __restrict__ blk_prob;  // SEG: problem of each row block
   int aabb_stride;             // SEG: doubles between problems' aabb
   int nrows;
@@ -180,7 +181,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
 // SEG: the rows belong to independent problems, no block straddles two, and
 // each block takes its Lanczos scalars from its problem's aabb and writes its
 // partial at its row-block index (so a problem's partials are contiguous).
-template <int TILE, int MODE, bool SEG = false>
+// C16: column indices are stored as 16-bit offsets from the block's first row
+// (banded orderings such as symrcm keep them small), 2 bytes per nonzero
+// instead of 4.
+template <int TILE, int MODE, bool SEG = false, bool C16 = false>
 __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
   static_assert(TILE % kBlock == 0, "TILE must be a multiple of the block size");
   constexpr int PER = TILE / kBlock;
@@ -209,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
       ci[u] = 0;
       if (j < j1) {
         a[u] = p.val[j];
-        ci[u] = p.col[j];
+        ci[u] = C16 ? r0 + (int)p.col16[j] : p.col[j];
       }
     }
     double2 xv[PER];
@@ -237,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
     double2 acc = d2(0.0, 0.0);
     for (int j = j0 + t; j < j1; j += kBlock) {
       const double av = p.val[j];
-      const double2 xv = p.x[p.col[j]];
+      const double2 xv = p.x[C16 ? r0 + (int)p.col16[j] : p.col[j]];
       acc.x = fma(av, xv.x, acc.x);
       acc.y = fma(av, xv.y, acc.y);
     }

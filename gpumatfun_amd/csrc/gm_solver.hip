@@ -80,6 +80,9 @@ struct gm_solver {
   int G_sp = 1, chunk_sp = 1, G_el = 1, chunk_el = 1, G_df = 1, chunk_df = 1;
   int* rowblk[kNumTiles] = {};
   int nrowblk[kNumTiles] = {};
+  // 16-bit column offsets from each nonzero's row-block start, per tile size
+  // (null when some offset does not fit, or GM_FLAG_INDEX32)
+  short* col16[kNumTiles] = {};
   // device buffers
   int* rowptr = nullptr;
   int* col = nullptr;
@@ -135,6 +138,8 @@ void free_all(gm_solver* s) {
     if (g) hipGraphExecDestroy(g);
   for (int* rb : s->rowblk)
     if (rb) hipFree(rb);
+  for (short* c : s->col16)
+    if (c) hipFree(c);
   void* bufs[] = {s->rowptr, s->col, s->val, s->U, s->Usol, s->X, s->upd, s->W[0], s->W[1],
                   s->T[0], s->T[1], s->V, s->Z, s->C, s->Q, s->el, s->pl, s->mass, s->ode,
                   s->aabb, s->Yr, s->Yi, s->wts1, s->wts2, s->shifts, s->gsr, s->gsi,
@@ -152,11 +157,21 @@ template <int MODE>
 gm_status launch_spmm_mode(gm_solver* s, const SpmmArgs& a) {
   dim3 grid(s->G_sp), block(kBlock);
   if (s->tile > 0) {
+    const bool c16 = a.col16 != nullptr;
+#define GM_STREAM(T)                                                                          \
+  case T:                                                                                     \
+    if (c16)                                                                                  \
+      hipLaunchKernelGGL((gm::k_spmm_stream<T, MODE, false, true>), grid, block, 0, s->stream, a); \
+    else                                                                                      \
+      hipLaunchKernelGGL((gm::k_spmm_stream<T, MODE, false, false>), grid, block, 0, s->stream, a); \
+    break;
     switch (s->tile) {
-      case 512: hipLaunchKernelGGL((gm::k_spmm_stream<512, MODE>), grid, block, 0, s->stream, a); break;
-      case 1024: hipLaunchKernelGGL((gm::k_spmm_stream<1024, MODE>), grid, block, 0, s->stream, a); break;
-      default: hipLaunchKernelGGL((gm::k_spmm_stream<2048, MODE>), grid, block, 0, s->stream, a); break;
+      GM_STREAM(512)
+      GM_STREAM(1024)
+      default:
+      GM_STREAM(2048)
     }
+#undef GM_STREAM
     GM_HIP(hipGetLastError());
     return GM_OK;
   }
@@ -196,6 +211,7 @@ SpmmArgs spmm_base(gm_solver* s) {
   a.col = s->col;
   a.val = s->val;
   a.rowblk = s->tile > 0 ? s->rowblk[s->tile_idx] : nullptr;
+  a.col16 = s->tile > 0 ? s->col16[s->tile_idx] : nullptr;
   a.nrows = s->n;
   a.chunk = s->chunk_sp;
   a.aabb = s->aabb;
@@ -607,7 +623,23 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
     s->nrowblk[i] = (int)blk.size() - 1;
     max_blocks = std::max(max_blocks, s->nrowblk[i]);
     GM_TRY(upload(&s->rowblk[i], blk.data(), blk.size(), st));
-    // upload() returns before the copy ends; blk must outlive it
+    // column offsets from each block's first row, if every one fits 16 bits
+    std::vector<short> c16;
+    if (!(d->flags & GM_FLAG_INDEX32)) {
+      c16.resize((size_t)s->nnz);
+      bool fits = true;
+      for (int b = 0; b < s->nrowblk[i] && fits; ++b)
+        for (long long j = rp[blk[b]]; j < rp[blk[b + 1]]; ++j) {
+          const int off = ci[j] - blk[b];
+          if (off < -32768 || off > 32767) {
+            fits = false;
+            break;
+          }
+          c16[j] = (short)off;
+        }
+      if (fits) GM_TRY(upload(&s->col16[i], c16.data(), c16.size(), st));
+    }
+    // upload() returns before the copy ends; blk and c16 must outlive it
     GM_HIP(hipStreamSynchronize(st));
   }
 
@@ -972,6 +1004,14 @@ gm_status gm_get_stream(gm_solver* s, void** stream) {
   GM_CHECK_HANDLE(s);
   if (!stream) return fail(GM_EINVAL, "null stream pointer");
   *stream = reinterpret_cast<void*>(s->stream);
+  return GM_OK;
+}
+
+gm_status gm_get_schedule(gm_solver* s, int32_t* kind, int32_t* param, int32_t* index_bytes) {
+  GM_CHECK_HANDLE(s);
+  if (kind) *kind = s->tile > 0 ? 1 : 0;
+  if (param) *param = s->tile > 0 ? s->tile : s->lanes;
+  if (index_bytes) *index_bytes = (s->tile > 0 && s->col16[s->tile_idx]) ? 2 : 4;
   return GM_OK;
 }
 

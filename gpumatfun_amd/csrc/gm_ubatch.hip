@@ -188,6 +188,7 @@ struct gm_ubatch {
   std::vector<int> off;          // P + 1 row offsets
   int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr, *blk_prob = nullptr,
       *prob_blk = nullptr;
+  short* col16 = nullptr;        // 16-bit column offsets from each block's first row
   double *val = nullptr, *mass = nullptr, *aabb = nullptr, *wts1 = nullptr, *wts2 = nullptr,
          *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *yr = nullptr, *yi = nullptr,
          *partials = nullptr;
@@ -221,7 +222,7 @@ gm_status uupload(T** p, const T* src, size_t count, hipStream_t st) {
 
 void ufree_all(gm_ubatch* u) {
   if (u->graph) (void)hipGraphExecDestroy(u->graph);
-  void* bufs[] = {u->rowptr, u->col, u->rowblk, u->blk_prob, u->prob_blk, u->val, u->mass,
+  void* bufs[] = {u->col16, u->rowptr, u->col, u->rowblk, u->blk_prob, u->prob_blk, u->val, u->mass,
                   u->aabb, u->wts1, u->wts2, u->shifts, u->gsr, u->gsi, u->yr, u->yi,
                   u->partials, u->U, u->Usol, u->X, u->W[0], u->W[1], u->T[0], u->T[1], u->V,
                   u->Z, u->C, u->wD, u->wDU, u->wDU2, u->wX, u->status};
@@ -252,8 +253,12 @@ gm_status ureduce(gm_ubatch* u, double* out, int stride, int do_sqrt, int flag_z
 
 template <int MODE>
 gm_status uspmm(gm_ubatch* u, const gm::SpmmArgs& a) {
-  hipLaunchKernelGGL((gm::k_spmm_stream<gmu::kTile, MODE, true>), dim3(u->G), dim3(gm::kBlock), 0,
-                     u->stream, a);
+  if (a.col16)
+    hipLaunchKernelGGL((gm::k_spmm_stream<gmu::kTile, MODE, true, true>), dim3(u->G),
+                       dim3(gm::kBlock), 0, u->stream, a);
+  else
+    hipLaunchKernelGGL((gm::k_spmm_stream<gmu::kTile, MODE, true, false>), dim3(u->G),
+                       dim3(gm::kBlock), 0, u->stream, a);
   U_HIP(hipGetLastError());
   return GM_OK;
 }
@@ -264,6 +269,7 @@ gm::SpmmArgs uspmm_base(gm_ubatch* u) {
   a.col = u->col;
   a.val = u->val;
   a.rowblk = u->rowblk;
+  a.col16 = u->col16;
   a.blk_prob = u->blk_prob;
   a.aabb_stride = (int)aabb_stride(u);
   a.nrows = u->N;
@@ -479,6 +485,18 @@ gm_status create_impl(const gm_ubatch_desc* d, gm_ubatch* u) {
   U_TRY(uupload(&u->col, ci.data(), (size_t)nnz, st));
   U_TRY(uupload(&u->val, vv.data(), (size_t)nnz, st));
   U_TRY(uupload(&u->rowblk, blk.data(), blk.size(), st));
+  std::vector<short> c16((size_t)nnz);
+  bool fits = true;
+  for (int b = 0; b < u->G && fits; ++b)
+    for (int j = rp[blk[b]]; j < rp[blk[b + 1]]; ++j) {
+      const int off = ci[j] - blk[b];
+      if (off < -32768 || off > 32767) {
+        fits = false;
+        break;
+      }
+      c16[j] = (short)off;
+    }
+  if (fits) U_TRY(uupload(&u->col16, c16.data(), c16.size(), st));
   U_TRY(uupload(&u->blk_prob, bprob.data(), bprob.size(), st));
   U_TRY(uupload(&u->prob_blk, pblk.data(), pblk.size(), st));
   U_TRY(uupload(&u->mass, mass.data(), N, st));

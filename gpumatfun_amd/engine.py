@@ -41,13 +41,15 @@ class LanczosEngine:
     clamp of ``update_U.cu:9``), ``autotune`` (time every SpMM schedule on A
     at create and keep the fastest), ``schedule`` (without autotune: None
     picks row blocks when rows average <= 16 nonzeros and row groups
-    otherwise; ``"rows"`` or ``"blocks"`` forces one), ``device``.
+    otherwise; ``"rows"`` or ``"blocks"`` forces one), ``index32`` (keep
+    32-bit column indices in the row-block SpMM instead of 16-bit block
+    offsets), ``device``.
     """
 
     def __init__(self, U, A, Q, lambda_, N, gamma, gamma_s_real, gamma_s_imag,
                  exp_lambda, phi_lambda, dt, ode, aode, bode, code, vrest, vamp, vth,
                  vpeak, c1ion, c2ion, mass, wts1, wts2, shifts, valvec=(-1.0, 0.0, 1.0),
-                 *, clamp=True, autotune=False, schedule=None, device=0):
+                 *, clamp=True, autotune=False, schedule=None, index32=False, device=0):
         lib = _lib.load()
         self._lib = lib
         self._h = C.c_void_p()
@@ -151,7 +153,8 @@ class LanczosEngine:
         if schedule not in sched:
             raise GpuMatfunError(_lib.GM_EINVAL, "schedule must be None, 'rows' or 'blocks'")
         d.flags = ((0 if clamp else _lib.GM_FLAG_NO_CLAMP)
-                   | (_lib.GM_FLAG_AUTOTUNE if autotune else 0) | sched[schedule])
+                   | (_lib.GM_FLAG_AUTOTUNE if autotune else 0) | sched[schedule]
+                   | (_lib.GM_FLAG_INDEX32 if index32 else 0))
         d.device = int(device)
         _lib.check(lib.gm_lanczos_create(C.byref(d), C.byref(self._h)))
         self._keep = None
@@ -263,11 +266,16 @@ class LanczosEngine:
         m = C.c_int32(); nd = C.c_int32(); ns = C.c_int32(); K = C.c_int32(); lanes = C.c_int32()
         self._call(self._lib.gm_get_info, C.byref(n), C.byref(nnz), C.byref(m), C.byref(nd),
                    C.byref(ns), C.byref(K), C.byref(lanes))
-        lv = lanes.value
-        sched = f"k_spmm<{lv}>" if lv > 0 else f"k_spmm_stream<{-lv}>"
+        kind = C.c_int32(); param = C.c_int32(); ib = C.c_int32()
+        self._call(self._lib.gm_get_schedule, C.byref(kind), C.byref(param), C.byref(ib))
+        if kind.value == 0:
+            sched = f"k_spmm<{param.value}>"
+        else:
+            sched = f"k_spmm_stream<{param.value}{',c16' if ib.value == 2 else ''}>"
         return {"n": n.value, "nnz": nnz.value, "m": m.value, "gamma_size": nd.value,
-                "contour_size": ns.value, "num_eigs": K.value, "spmm_lanes": lv,
-                "spmm_schedule": sched}
+                "contour_size": ns.value, "num_eigs": K.value, "spmm_lanes": lanes.value,
+                "spmm_schedule": sched, "spmm_kind": kind.value, "spmm_param": param.value,
+                "index_bytes": ib.value}
 
     def stream(self):
         p = C.c_void_p()

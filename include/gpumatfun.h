@@ -51,9 +51,13 @@ enum {
                                     create and keep the fastest              */
     GM_FLAG_ROW_GROUPS = 1u << 2, /* without AUTOTUNE: the row-group SpMM
                                     (lanes per row from the mean row length) */
-    GM_FLAG_ROW_BLOCKS = 1u << 3  /* without AUTOTUNE: the row-block SpMM with
+    GM_FLAG_ROW_BLOCKS = 1u << 3, /* without AUTOTUNE: the row-block SpMM with
                                     1024-nonzero LDS tiles.  Neither flag: row
                                     blocks when rows average <= 16 nonzeros */
+    GM_FLAG_INDEX32 = 1u << 4     /* keep 32-bit column indices in the
+                                    row-block SpMM (by default they are
+                                    stored as 16-bit offsets from the block's
+                                    first row whenever every offset fits)   */
 };
 
 /*
@@ -180,6 +184,10 @@ gm_status gm_synchronize(gm_solver* s);
 gm_status gm_get_info(gm_solver* s, int64_t* n, int64_t* nnz, int32_t* m,
                       int32_t* gamma_size, int32_t* contour_size,
                       int32_t* num_eigs, int32_t* spmm_lanes);
+/* The SpMM schedule in use: kind 0 = row groups (param = lanes per row),
+ * 1 = row blocks (param = nonzeros per LDS tile); index_bytes = bytes per
+ * stored column index (4, or 2 for 16-bit block offsets).                  */
+gm_status gm_get_schedule(gm_solver* s, int32_t* kind, int32_t* param, int32_t* index_bytes);
 
 /* The solver's HIP stream (as void*), for callers that time it with
  * events of their own.                                                      */

@@ -283,14 +283,19 @@ def test_mitchell_schaeffer_vs_oracle():
     eng.delete()
 
 
-@pytest.mark.parametrize("schedule", ["rows", "blocks"])
-def test_spmm_schedules_vs_cpu_path(schedule):
-    """Both SpMM schedules (row groups, LDS row blocks) on the FEM case with
-    deflation and a degree-2 polynomial, against the CPU step loop."""
+@pytest.mark.parametrize("schedule,index32", [("rows", False), ("blocks", False),
+                                              ("blocks", True)])
+def test_spmm_schedules_vs_cpu_path(schedule, index32):
+    """Both SpMM schedules (row groups, LDS row blocks with 16- or 32-bit column
+    indices) on the FEM case with deflation and a degree-2 polynomial, against
+    the CPU step loop."""
     inp = fem_case(m=30)
-    print("case fem m=30 schedule", schedule, flush=True)
-    eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=schedule)
-    assert (eng.info()["spmm_lanes"] > 0) == (schedule == "rows")
+    print("case fem m=30 schedule", schedule, "index32", index32, flush=True)
+    eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=schedule, index32=index32)
+    info = eng.info()
+    assert info["spmm_kind"] == (0 if schedule == "rows" else 1)
+    if schedule == "blocks":
+        assert info["index_bytes"] == (4 if index32 else 2)
     eng.step(3)
     U = eng.get_U()
     S = oracle_setup(inp)
@@ -317,7 +322,7 @@ def _arrow_matrix(N=3000, wide=(100, 200), width=600, eps=0.01):
     return sp.csr_matrix(A)
 
 
-@pytest.mark.parametrize("schedule", ["rows", "blocks", "autotune"])
+@pytest.mark.parametrize("schedule", ["rows", "blocks", "blocks32", "autotune"])
 def test_long_rows_vs_cpu_path(schedule):
     """A row longer than every LDS tile (the row-block kernel's whole-block
     branch) and rows of 600 nonzeros among rows of 3."""
@@ -333,7 +338,9 @@ def test_long_rows_vs_cpu_path(schedule):
     inp = q.make_inputs(A, u0, g0, m=30, n=16, eta=1e-2, alpha=1.5, dt=0.1,
                         l1=float(w.min()), lN=float(w.max()), degree=1)
     print("case arrow N=3000 schedule", schedule, flush=True)
-    kw = {"autotune": True} if schedule == "autotune" else {"schedule": schedule}
+    kw = ({"autotune": True} if schedule == "autotune" else
+          {"schedule": "blocks", "index32": True} if schedule == "blocks32" else
+          {"schedule": schedule})
     eng = LanczosEngine.from_inputs(inp, clamp=False, **kw)
     eng.step(2)
     U = eng.get_U()
