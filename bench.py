@@ -539,7 +539,7 @@ def main():
         run_batch(args, steps, warm, use_graph)
         return
     if args.config == "ms512":
-        if args.steps is None:
+        if args.steps is None and not args.profile_only:
             steps = 1000
         run_ms512(args, steps, warm, use_graph)
         return
