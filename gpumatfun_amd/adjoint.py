@@ -102,6 +102,24 @@ class FabEngine:
         self._call(self._lib.gm_fab_backward, _p(g), _p(kap), _p(gb), _p(gk), _p(ga))
         return gb, gk, (None if ga is None else float(ga[0]))
 
+    def backward_param(self, g, dA_list, want_b=False):
+        """dL/dtheta_q = Im sum_j c_j z_j^T (dA/dtheta_q) x_j for each sparse dA/dtheta_q
+        (A's sparsity pattern) after a forward pass; returns (grad_b or None, grads)."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        vals = []
+        for E in dA_list:
+            E = sp.csr_matrix(E)
+            E.sort_indices()
+            if E.nnz != self.nnz or not np.array_equal(E.indptr, self._ptr) or \
+                    not np.array_equal(E.indices, self._ind):
+                raise ValueError("dA/dtheta must have A's sparsity pattern")
+            vals.append(np.ascontiguousarray(E.data, dtype=np.float64))
+        arr = (C.POINTER(C.c_double) * max(1, len(vals)))(*[_p(v) for v in vals])
+        out = np.zeros(max(1, len(vals)))
+        gb = np.empty(self.n) if want_b else None
+        self._call(self._lib.gm_fab_backward_param, _p(g), len(vals), arr, _p(gb), _p(out))
+        return gb, out[:len(vals)].copy()
+
     def delete(self):
         if self._h:
             _lib.check(self._lib.gm_fab_delete(self._h), fab=True)

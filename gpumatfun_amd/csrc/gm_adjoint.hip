@@ -9,6 +9,8 @@
 //   dL/db       = f(A) g                      (the same machinery on g)
 //   dL/dkappa_i = Im sum_j c_j z_j^T (dA/dkappa_i) x_j,  z_j = (xi_j - A)^{-1} g
 //   dL/dalpha   = Im sum_j (dc_j/dalpha) g^T x_j
+//   dL/dtheta   = Im sum_j c_j z_j^T (dA/dtheta) x_j   for a scalar parameter
+//                 theta of A given dA/dtheta on A's pattern (gm_fab_backward_param)
 // with dA/dkappa of the variable-coefficient model (edge conductance
 // (kappa_i + kappa_k)/2, boundary terms linear in kappa_i):
 //   z^T dA/dkappa_i x = sum_{k != i} w_ik (z_i - z_k)(x_i - x_k) + s_i z_i x_i,
@@ -167,6 +169,41 @@ __global__ __launch_bounds__(256) void k_gx(GX p) {
   }
 }
 
+// dL/dtheta = Im sum_j c_j z_j^T E x_j for E = dA/dtheta (A's pattern): row i
+// contributes Im sum_j c_j z_{j,i} (E x_j)_i; block partials, fixed order.
+struct PGrad {
+  const int* rowptr;
+  const int* col;
+  const double* E;
+  const double* X;     // [N][ncol]
+  const double* Z;
+  const double* c;     // ns complex
+  int N, ns, ncol;
+  double* partials;    // [G][2]
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_param_grad(PGrad p) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double acc = 0.0;
+  if (i < p.N) {
+    const double* zi = p.Z + (size_t)i * p.ncol;
+    const int e0 = p.rowptr[i], e1 = p.rowptr[i + 1];
+    for (int s = 0; s < p.ns; ++s) {
+      double er = 0.0, ei = 0.0;
+      for (int e = e0; e < e1; ++e) {
+        const double* xk = p.X + (size_t)p.col[e] * p.ncol;
+        er = fma(p.E[e], xk[2 * s], er);
+        ei = fma(p.E[e], xk[2 * s + 1], ei);
+      }
+      const double zr = zi[2 * s], zim = zi[2 * s + 1];
+      const double tr = zr * er - zim * ei, ti = zr * ei + zim * er;
+      acc += p.c[2 * s] * ti + p.c[2 * s + 1] * tr;   // Im(c_j t)
+    }
+  }
+  block_partial2_at(d2(acc, 0.0), p.partials, blockIdx.x);
+}
+
 }  // namespace gma
 
 namespace {
@@ -221,6 +258,7 @@ struct gm_fab {
          *gxv = nullptr;
   double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
   unsigned* status = nullptr;
+  double* dA = nullptr;     // nnz values of dA/dtheta (gm_fab_backward_param)
   std::vector<double> dc;   // ns complex dc/dalpha (host)
   Side sb, sg;
 };
@@ -246,7 +284,7 @@ void afree(void* p) {
 void free_fab(gm_fab* f) {
   void* bufs[] = {f->rowptr, f->col, f->val, f->xi, f->ones, f->c, f->gsr, f->gsi, f->C,
                   f->partials, f->kappa, f->grad, f->gxv, f->wD, f->wDU, f->wDU2, f->wX,
-                  f->status};
+                  f->status, f->dA};
   for (void* p : bufs) afree(p);
   for (Side* s : {&f->sb, &f->sg}) {
     void* sbufs[] = {s->U, s->V, s->W0, s->W1, s->al, s->be, s->yr, s->yi, s->Yg, s->Xc, s->out};
@@ -608,6 +646,58 @@ gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, doubl
       ga += dr * xi + di * xr;       // Im(dc * g^T x)
     }
     *grad_alpha = ga;
+  }
+  A_HIP(hipStreamSynchronize(st));
+  unsigned stv = 0;
+  A_HIP(hipMemcpy(&stv, f->status, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (stv & 6u) return afail(GM_EBREAKDOWN, "Lanczos breakdown in the backward pass (%u)", stv);
+  return GM_OK;
+}
+
+gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
+                                const double* const* dA_values, double* grad_b,
+                                double* grad_theta) {
+  if (!alive(f)) return afail(GM_EBADHANDLE, "invalid fab handle");
+  if (!f->have_forward) return afail(GM_ESTATE, "backward needs a forward pass first");
+  if (!g || nparams < 0 || (nparams > 0 && (!dA_values || !grad_theta)))
+    return afail(GM_EINVAL, "null upstream gradient or parameter arrays");
+  for (int q = 0; q < nparams; ++q)
+    if (!dA_values[q]) return afail(GM_EINVAL, "null dA/dtheta values for parameter %d", q);
+  A_HIP(hipSetDevice(f->dev));
+  const size_t N = (size_t)f->n;
+  hipStream_t st = f->stream;
+  A_HIP(hipMemcpyAsync(f->sg.U, g, N * sizeof(double), hipMemcpyHostToDevice, st));
+  A_TRY(krylov(f, f->sg, grad_b != nullptr));
+  if (grad_b)
+    A_HIP(hipMemcpyAsync(grad_b, f->sg.out, N * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (nparams > 0) {
+    if (f->recompute) A_TRY(krylov(f, f->sb, false));
+    A_TRY(resolvent_block(f, f->sb));
+    A_TRY(resolvent_block(f, f->sg));
+    if (!f->dA) A_TRY(aalloc(&f->dA, (size_t)f->nnz));
+    const int G = adiv(f->n, 256);
+    for (int q = 0; q < nparams; ++q) {
+      A_HIP(hipMemcpyAsync(f->dA, dA_values[q], (size_t)f->nnz * sizeof(double),
+                           hipMemcpyHostToDevice, st));
+      gma::PGrad pg{};
+      pg.rowptr = f->rowptr;
+      pg.col = f->col;
+      pg.E = f->dA;
+      pg.X = f->sb.Xc;
+      pg.Z = f->sg.Xc;
+      pg.c = f->c;
+      pg.N = f->n;
+      pg.ns = f->ns;
+      pg.ncol = f->ncol;
+      pg.partials = f->partials;
+      hipLaunchKernelGGL(gma::k_param_grad<0>, dim3(G), dim3(256), 0, st, pg);
+      A_HIP(hipGetLastError());
+      A_TRY(areduce(f, G, 2, f->gxv, 0, 0));
+      double out[2];
+      A_HIP(hipMemcpyAsync(out, f->gxv, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+      A_HIP(hipStreamSynchronize(st));   // dA_values[q] and out are host memory
+      grad_theta[q] = out[0];
+    }
   }
   A_HIP(hipStreamSynchronize(st));
   unsigned stv = 0;

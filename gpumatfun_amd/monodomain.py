@@ -1,0 +1,106 @@
+"""Gradient of a monodomain Mitchell-Schaeffer run with respect to a parameter
+of the diffusion operator (e.g. the conductivity ratio), by the discrete
+adjoint of the exponential-integrator time stepping (no reference
+counterpart).
+
+Forward scheme, per stimulus -- the plain path's step with the
+Mitchell-Schaeffer reaction (``gm_step`` mode 2), f(A) applied without
+deflation:
+
+    u_{n+1} = f1(A) u_n + dt f2(A) g_n,           g_n = G(u_n, h_n)
+    h_{n+1} = h_n + dt D(u_{n+1}, h_n)
+    G(u, h) = h u^2 (1 - u)/tau_in - u/tau_out
+    D(u, h) = (1 - h)/tau_open if u < v_gate else -h/tau_close
+
+with f1 = exp(-eta x^(alpha/2)), f2 = phi(-eta x^(alpha/2)) on the contour of
+``problem_startup.m:85-103``.  For L = ||u_T - y||^2 the adjoint runs the
+steps backwards (A symmetric, so f(A)^T = f(A)):
+
+    w       = ubar_{n+1}                       (D is piecewise constant in u)
+    ubar_n  = f1(A) w + dG/du(u_n, h_n) * (dt f2(A) w)
+    hbar_n  = hbar_{n+1} (1 + dt dD/dh) + dG/dh(u_n, h_n) * (dt f2(A) w)
+    dL/dtheta += w^T (d f1(A)/dtheta) u_n + dt w^T (d f2(A)/dtheta) g_n
+
+where each w^T (d f(A)/dtheta) v is the contour sum Im sum_j c_j z_j^T
+(dA/dtheta) x_j of ``gm_fab_backward_param``.  Each backward step
+re-runs the two forward Krylov solves it needs, from the stored states.
+"""
+
+import numpy as np
+
+from . import _lib
+from .adjoint import FabEngine
+
+
+def ms_g(u, h, p):
+    return (h * (u * u) * (1.0 - u)) / p["tau_in"] - u / p["tau_out"]
+
+
+def ms_gate(u, h, dt, p):
+    dh = np.where(u < p["v_gate"], (1.0 - h) / p["tau_open"], -h / p["tau_close"])
+    return h + dt * dh
+
+
+class MonodomainAdjoint:
+    """Forward steps and the adjoint for one stimulus on a fixed contour.
+
+    ``xi`` (complex nodes), ``wts1``/``wts2`` (complex s f1(xi) dxi/dt and
+    s f2(xi) dxi/dt), subspace size ``m`` for the Krylov resolvents.
+    """
+
+    def __init__(self, A, *, m, xi, wts1, wts2, dt, ms=None, device=0):
+        self.p = dict(_lib.MS_DEFAULTS)
+        self.p.update(ms or {})
+        self.dt = float(dt)
+        self.f1 = FabEngine(A, m=m, xi=xi, wts=wts1, recompute=False, device=device)
+        self.f2 = FabEngine(A, m=m, xi=xi, wts=wts2, recompute=False, device=device)
+        self.traj = None
+
+    def set_matrix(self, A):
+        self.f1.set_matrix(A)
+        self.f2.set_matrix(A)
+
+    def forward(self, u0, h0, steps):
+        """Runs the steps; returns u_T and keeps (u_n, h_n, g_n) for the adjoint."""
+        u = np.array(u0, dtype=np.float64)
+        h = np.array(h0, dtype=np.float64)
+        traj = []
+        for _ in range(steps):
+            g = ms_g(u, h, self.p)
+            traj.append((u, h, g))
+            u = self.f1.forward(u) + self.dt * self.f2.forward(g)
+            h = ms_gate(u, h, self.dt, self.p)
+        self.traj = traj
+        self.uT, self.hT = u, h
+        return u
+
+    def backward(self, ubar_T, dA_list=()):
+        """Adjoint of the stored run for dL/du_T = ubar_T: returns (dL/du_0,
+        dL/dh_0, [dL/dtheta_q]) for the given dA/dtheta_q."""
+        if self.traj is None:
+            raise RuntimeError("backward needs a forward run first")
+        p, dt = self.p, self.dt
+        w = np.array(ubar_T, dtype=np.float64)
+        hbar = np.zeros_like(w)
+        gtheta = np.zeros(len(dA_list))
+        for n in range(len(self.traj) - 1, -1, -1):
+            u, h, g = self.traj[n]
+            u_next = self.traj[n + 1][0] if n + 1 < len(self.traj) else self.uT
+            # h_{n+1} = h_n + dt D(u_{n+1}, h_n): hbar flows to h_n, nothing to u_{n+1}
+            dDdh = np.where(u_next < p["v_gate"], -1.0 / p["tau_open"], -1.0 / p["tau_close"])
+            hbar_prev = hbar * (1.0 + dt * dDdh)
+            # u_{n+1} = f1(A) u_n + dt f2(A) g_n
+            self.f1.forward(u)
+            f1w, gt1 = self.f1.backward_param(w, dA_list, want_b=True)
+            self.f2.forward(g)
+            f2w, gt2 = self.f2.backward_param(w, dA_list, want_b=True)
+            gtheta += gt1 + dt * gt2
+            dGdu = h * (2.0 * u - 3.0 * u * u) / p["tau_in"] - 1.0 / p["tau_out"]
+            dGdh = (u * u) * (1.0 - u) / p["tau_in"]
+            w = f1w + dGdu * (dt * f2w)
+            hbar = hbar_prev + dGdh * (dt * f2w)
+        return w, hbar, gtheta
+
+    def delete(self):
+        self.f1.delete()
+        self.f2.delete()

@@ -182,6 +182,19 @@ def anisotropic_9pt(n, ratio=9.0, theta_max=np.pi / 3, length=1.0):
     return _csr(A)
 
 
+def anisotropic_9pt_ratio_derivative(n, theta_max=np.pi / 3, length=1.0):
+    """dA/d(ratio) of ``anisotropic_9pt``: the operator is linear in the
+    conductivity ratio (D = ratio f f^T + s s^T), so this is
+    anisotropic_9pt(ratio=1) - anisotropic_9pt(ratio=0), returned on A's
+    sparsity pattern (same indptr and indices for every ratio)."""
+    A1 = anisotropic_9pt(n, ratio=1.0, theta_max=theta_max, length=length)
+    A0 = anisotropic_9pt(n, ratio=0.0, theta_max=theta_max, length=length)
+    assert np.array_equal(A1.indptr, A0.indptr) and np.array_equal(A1.indices, A0.indices)
+    E = A1.copy()
+    E.data = A1.data - A0.data
+    return E
+
+
 def uneven_batch_sides(count=64, lo=32, hi=448, seed=0):
     """Side lengths drawn log-uniformly in [lo, hi] (integers), seeded."""
     rng = np.random.default_rng(seed)

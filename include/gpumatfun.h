@@ -311,6 +311,14 @@ gm_status gm_fab_forward(gm_fab* f, const double* b, double* u);
  * kappa (n) is needed for grad_kappa                                       */
 gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, double* grad_b,
                           double* grad_kappa, double* grad_alpha);
+/* after a forward: gradients with respect to nparams scalar parameters theta_q
+ * of A, given each dA/dtheta_q on A's sparsity pattern (nnz values):
+ *   grad_theta[q] = Im sum_j c_j z_j^T (dA/dtheta_q) x_j
+ * (e.g. theta = the conductivity ratio of an anisotropic operator, which A
+ * depends on linearly); grad_b (may be NULL) = f(A) g as in gm_fab_backward. */
+gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
+                                const double* const* dA_values, double* grad_b,
+                                double* grad_theta);
 
 /* ---------------------------------------------------------------------------
  * Uneven batch (no reference counterpart): P independent problems of

@@ -15,6 +15,12 @@ For an upstream gradient g = dL/du (A symmetric, so (xi - A)^T = xi - A):
     dL/dalpha   = Im sum_j (dc_j/dalpha) g^T x_j,
     dc_j/dalpha = c_j * (-tau xi_j^(alpha/2) log(xi_j) / 2).
 
+For a scalar parameter theta of A with derivative E = dA/dtheta (any
+symmetric sparse matrix, e.g. the fibre part of an anisotropic conductivity
+when theta is the conductivity ratio):
+
+    dL/dtheta   = Im sum_j c_j z_j^T E x_j.
+
 dA/dkappa_i follows the variable-coefficient model of
 ``gpumatfun_amd.problems.varcoef_laplacian_2d``: off-diagonals
 -(kappa_i + kappa_k)/(2 h^2) and boundary terms linear in kappa_i, so
@@ -90,3 +96,28 @@ def gradients(A, kappa, b, g, xi, c, dc_dalpha, solver=None):
         grad_k += np.imag(c[j] * per)
     grad_a = float(np.imag(np.sum(dc_dalpha * (g @ X))))
     return grad_b, grad_k, grad_a
+
+
+def param_gradient(A, E, b, g, xi, c, solver=None):
+    """dL/dtheta = Im sum_j c_j z_j^T E x_j for E = dA/dtheta (see the module text)."""
+    solver = solver or resolvents
+    X = solver(A, xi, b)
+    Z = solver(A, xi, g)
+    E = sp.csr_matrix(E)
+    return float(np.imag(sum(c[j] * (Z[:, j] @ (E @ X[:, j])) for j in range(len(xi)))))
+
+
+def ms_forward(A, u0, h0, steps, xi, c1, c2, dt, p):
+    """The monodomain Mitchell-Schaeffer steps with exact resolvents:
+    u_{n+1} = Im sum_j c1_j x_j(u_n) + dt Im sum_j c2_j x_j(g_n) (the contour
+    rule of problem_startup.m:95-103 for f1 and f2 without deflation), then the
+    gate and g from ``stepping.ms_reaction``.  Returns (u_T, h_T)."""
+    from .stepping import ms_reaction
+    u = np.array(u0, dtype=np.float64)
+    h = np.array(h0, dtype=np.float64)
+    g = (h * (u * u) * (1.0 - u)) / p["tau_in"] - u / p["tau_out"]
+    for _ in range(steps):
+        u = forward(A, u, xi, c1)[0] + dt * forward(A, g, xi, c2)[0]
+        h, g = ms_reaction(u, h, dt, p)
+    return u, h
+

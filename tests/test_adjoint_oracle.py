@@ -77,3 +77,23 @@ def test_b_and_alpha_gradients_central_differences():
     xm, cm, _ = weights(l1, lN, ALPHA - ha)
     fda = (loss(A, b, y, xp, cp)[0] - loss(A, b, y, xm, cm)[0]) / (2 * ha)
     assert abs(fda - ga) <= 1e-5 * abs(fda)
+
+
+def test_param_gradient_conductivity_ratio_central_differences():
+    """dL/dr for the conductivity ratio r of the anisotropic operator (A linear in r)."""
+    n, r, h = 12, 4.0, 1e-5
+    A = problems.anisotropic_9pt(n, ratio=r)
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    # linear in r: A(r) = A(0) + r E, to rounding
+    assert rel((problems.anisotropic_9pt(n, ratio=0.0) + r * E).toarray(), A.toarray()) < 1e-14
+    w = np.linalg.eigvalsh(A.toarray())
+    xi, cw, _ = weights(0.9 * w.min(), 1.1 * w.max(), ALPHA)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(n * n)
+    y = rng.standard_normal(n * n)
+    L0, u = loss(A, b, y, xi, cw)
+    g = adjoint.param_gradient(A, E, b, 2 * (u - y), xi, cw)
+    Lp, _ = loss(problems.anisotropic_9pt(n, ratio=r + h), b, y, xi, cw)
+    Lm, _ = loss(problems.anisotropic_9pt(n, ratio=r - h), b, y, xi, cw)
+    fd = (Lp - Lm) / (2 * h)
+    assert abs(fd - g) <= 1e-6 * abs(g), (fd, g)

@@ -192,3 +192,27 @@ def test_torch_autograd_function_gradcheck():
     a = torch.tensor(ALPHA, dtype=torch.float64, requires_grad=True)
     assert torch.autograd.gradcheck(lambda b_, k_, a_: op.apply(b_, k_, a_), (b, k, a),
                                     eps=1e-6, atol=1e-6, rtol=1e-4)
+
+
+def test_param_gradient_conductivity_ratio_vs_exact_reference():
+    """dL/dr for the conductivity ratio of the anisotropic 9-point operator
+    (gm_fab_backward_param) against the exact-resolvent oracle, which central
+    differences pin (tests/test_adjoint_oracle.py)."""
+    n, m, r = 40, 300, 4.0
+    A = problems.anisotropic_9pt(n, ratio=r)
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    c, cw, dc = contour(A, ns=24)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(n * n)
+    y = rng.standard_normal(n * n)
+    print(f"case param gradient n={n} m={m}", flush=True)
+    eng = FabEngine(A, m=m, xi=c["xi"], wts=cw, dwts=dc)
+    u = eng.forward(b)
+    g = 2 * (u - y)
+    gb, (gr,) = eng.backward_param(g, [E], want_b=True)
+    eng.delete()
+    ref = oadj.param_gradient(A, E, b, g, c["xi"], cw)
+    rb = np.imag(oadj.resolvents(A, c["xi"], g) @ cw)
+    print("dL/dr", gr, ref, "grad_b rel", rel(gb, rb))
+    assert abs(gr - ref) <= 1e-7 * abs(ref)
+    assert rel(gb, rb) < 1e-7

@@ -1,0 +1,55 @@
+"""Gradient of a monodomain Mitchell-Schaeffer run with respect to the
+conductivity ratio (and the initial state), through the time steps, against
+central finite differences of the oracle's exact-resolvent forward.
+
+Tolerances: the HIP forward uses m-step Lanczos resolvents (m = 160 on the
+24^2 grid, converged to ~1e-11 here), so its end state is held to 1e-9 of the
+oracle's exact one; the gradients to 1e-6 relative (finite differences with
+step 1e-5 agree with steps 1e-4 and 1e-6 to ~1e-9 on this problem)."""
+
+import numpy as np
+import pytest
+
+from oracle import adjoint as oadj
+from conftest import rel
+from gpumatfun_amd import problems, quadrature as q
+from gpumatfun_amd._lib import MS_DEFAULTS
+from gpumatfun_amd.monodomain import MonodomainAdjoint
+
+pytestmark = pytest.mark.gpu
+
+
+def test_ratio_and_initial_state_gradients_vs_finite_differences():
+    n, steps, dt, r, m, h = 24, 5, 0.05, 4.0, 160, 1e-5
+    A = problems.anisotropic_9pt(n, ratio=r)
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    w = np.linalg.eigvalsh(A.toarray())
+    c = q.contour_nodes(0.95 * w.min(), 1.05 * w.max(), 24, dt * 1e-3, 1.7)
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    u0 = ((X >= 3) & (X < 9) & (Y >= 4) & (Y < 10)).astype(float).reshape(-1, order="F")
+    h0 = np.ones(n * n)
+    y = 0.3 * np.random.default_rng(0).random(n * n)
+    p = dict(MS_DEFAULTS)
+    print(f"case monodomain adjoint n={n} steps={steps} m={m}", flush=True)
+    md = MonodomainAdjoint(A, m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"], dt=dt, ms=p)
+    uT = md.forward(u0, h0, steps)
+    ubar0, hbar0, (gr,) = md.backward(2 * (uT - y), [E])
+    md.delete()
+
+    def loss(rr, uu=u0):
+        Ar = problems.anisotropic_9pt(n, ratio=rr)
+        u, _ = oadj.ms_forward(Ar, uu, h0, steps, c["xi"], c["wts1"], c["wts2"], dt, p)
+        return float(np.sum((u - y) ** 2)), u
+
+    L0, u_ref = loss(r)
+    print("forward rel err", rel(uT, u_ref))
+    assert rel(uT, u_ref) < 1e-9
+    fd = (loss(r + h)[0] - loss(r - h)[0]) / (2 * h)
+    print("dL/dratio", gr, "fd", fd)
+    assert abs(gr - fd) <= 1e-6 * abs(fd)
+    for i in np.random.default_rng(1).choice(n * n, 4, replace=False):
+        up = u0.copy(); up[i] += h
+        um = u0.copy(); um[i] -= h
+        fdu = (loss(r, up)[0] - loss(r, um)[0]) / (2 * h)
+        print("dL/du0", i, ubar0[i], fdu)
+        assert abs(ubar0[i] - fdu) <= 1e-6 * max(abs(fdu), 1e-3 * np.abs(ubar0).max())
