@@ -80,6 +80,23 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
     double acc[CL];
 #pragma unroll
     for (int q = 0; q < CL; ++q) acc[q] = 0.0;
+    // the epilogue's own-row operands, loaded first so that their latency
+    // overlaps the gathers
+    double o0[CL], o1[CL];
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const int c = c0 + q * CPW;
+      const size_t iv = (size_t)r * R + c;
+      o0[q] = 0.0;
+      o1[q] = 0.0;
+      if (MODE == kFirst || MODE == kOnly) o0[q] = p.x[(size_t)r * p.ldx + c];
+      if (MODE == kMid || MODE == kLast) o0[q] = p.vk[iv];
+      if ((MODE == kLast || MODE == kOnly) && p.k > 0) o1[q] = p.vkm1[iv];
+      if (MODE == kHorner) {
+        o0[q] = p.z[(size_t)r * p.ldz + c];
+        if (p.accumulate) o1[q] = p.y[(size_t)r * p.ldy + c];
+      }
+    }
     // kUnroll nonzeros at a time: their val/col loads, then all their
     // gathers, are issued before the first FMA waits, so a wavefront has
     // kUnroll * CL gathers in flight instead of one dependent chain per
@@ -114,27 +131,27 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
       const size_t iy = (size_t)r * p.ldy + c;
       const size_t iv = (size_t)r * R + c;
       if (MODE == kFirst) {
-        const double v = safe_div(p.x[(size_t)r * p.ldx + c0 + q * CPW], bet[q]);
+        const double v = safe_div(o0[q], bet[q]);
         p.vk[iv] = v;
         p.y[iy] = fma(p.gi, v, acc[q] * inv[q]);
       } else if (MODE == kMid) {
-        p.y[iy] = fma(p.gi, p.vk[iv], acc[q]);
+        p.y[iy] = fma(p.gi, o0[q], acc[q]);
       } else if (MODE == kLast || MODE == kOnly) {
         double v, yv;
         if (MODE == kOnly) {
-          v = safe_div(p.x[(size_t)r * p.ldx + c0 + q * CPW], bet[q]);
+          v = safe_div(o0[q], bet[q]);
           p.vk[iv] = v;
           yv = acc[q] * inv[q];
         } else {
-          v = p.vk[iv];
+          v = o0[q];
           yv = acc[q];
         }
-        if (p.k > 0) yv = fma(-bprev[q], p.vkm1[iv], yv);
+        if (p.k > 0) yv = fma(-bprev[q], o1[q], yv);
         p.y[iy] = yv;
         dot[q] = fma(yv, v, dot[q]);
       } else {
-        double yv = acc[q] + p.z[(size_t)r * p.ldz + c];
-        if (p.accumulate) yv += p.y[iy];
+        double yv = acc[q] + o0[q];
+        if (p.accumulate) yv += o1[q];
         p.y[iy] = yv;
       }
     }

@@ -89,48 +89,66 @@ __device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p, const dou
   return c;
 }
 
-// Row r's output from acc = (A x)_r; kLast/kOnly add y_r v_r to dot.
+// Row r's operands besides A x: w_r (kFirst, kOnly), v_k,r (kMid, kLast),
+// v_{k-1},r (kLast, kOnly with k > 0), z_r and the old y_r (kHorner).
 template <int MODE>
-__device__ __forceinline__ void spmm_row_out(const SpmmArgs& p, const SpmmScalars& c, int r,
-                                             double2 acc, double2& dot) {
+__device__ __forceinline__ void spmm_row_load(const SpmmArgs& p, int r, double2& o0, double2& o1) {
+  o0 = d2(0.0, 0.0);
+  o1 = d2(0.0, 0.0);
+  if (MODE == kFirst || MODE == kOnly) o0 = p.x[r];
+  if (MODE == kMid || MODE == kLast) o0 = p.vk[r];
+  if ((MODE == kLast || MODE == kOnly) && p.k > 0) o1 = p.vkm1[r];
+  if (MODE == kHorner) {
+    o0 = p.z[r];
+    if (p.accumulate) o1 = p.y[r];
+  }
+}
+
+// Row r's output from acc = (A x)_r and the operands of spmm_row_load;
+// kLast/kOnly add y_r v_r to dot.
+template <int MODE>
+__device__ __forceinline__ void spmm_row_store(const SpmmArgs& p, const SpmmScalars& c, int r,
+                                               double2 acc, double2 o0, double2 o1, double2& dot) {
   if (MODE == kFirst) {
-    const double2 w = p.x[r];
-    const double2 v = d2(safe_div(w.x, c.bet.x), safe_div(w.y, c.bet.y));
+    const double2 v = d2(safe_div(o0.x, c.bet.x), safe_div(o0.y, c.bet.y));
     p.vk[r] = v;
     p.y[r] = d2(fma(p.gi, v.x, acc.x * c.sx.x), fma(p.gi, v.y, acc.y * c.sx.y));
   } else if (MODE == kMid) {
-    const double2 v = p.vk[r];
-    p.y[r] = d2(fma(p.gi, v.x, acc.x), fma(p.gi, v.y, acc.y));
+    p.y[r] = d2(fma(p.gi, o0.x, acc.x), fma(p.gi, o0.y, acc.y));
   } else if (MODE == kLast || MODE == kOnly) {
     double2 v;
     if (MODE == kOnly) {
-      const double2 w = p.x[r];
-      v = d2(safe_div(w.x, c.bet.x), safe_div(w.y, c.bet.y));
+      v = d2(safe_div(o0.x, c.bet.x), safe_div(o0.y, c.bet.y));
       p.vk[r] = v;
       acc.x *= c.sx.x;
       acc.y *= c.sx.y;
     } else {
-      v = p.vk[r];
+      v = o0;
     }
     double2 yv = acc;
     if (p.k > 0) {
-      const double2 vp = p.vkm1[r];
-      yv.x = fma(-c.bprev.x, vp.x, yv.x);
-      yv.y = fma(-c.bprev.y, vp.y, yv.y);
+      yv.x = fma(-c.bprev.x, o1.x, yv.x);
+      yv.y = fma(-c.bprev.y, o1.y, yv.y);
     }
     p.y[r] = yv;
     dot.x = fma(yv.x, v.x, dot.x);
     dot.y = fma(yv.y, v.y, dot.y);
   } else {  // kHorner
-    const double2 zz = p.z[r];
-    double2 yv = d2(acc.x + zz.x, acc.y + zz.y);
+    double2 yv = d2(acc.x + o0.x, acc.y + o0.y);
     if (p.accumulate) {
-      const double2 old = p.y[r];
-      yv.x += old.x;
-      yv.y += old.y;
+      yv.x += o1.x;
+      yv.y += o1.y;
     }
     p.y[r] = yv;
   }
+}
+
+template <int MODE>
+__device__ __forceinline__ void spmm_row_out(const SpmmArgs& p, const SpmmScalars& c, int r,
+                                             double2 acc, double2& dot) {
+  double2 o0, o1;
+  spmm_row_load<MODE>(p, r, o0, o1);
+  spmm_row_store<MODE>(p, c, r, acc, o0, o1, dot);
 }
 
 // Row-group schedule: LANES consecutive lanes share one row (coalesced
@@ -200,9 +218,13 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
   if (j1 - j0 <= TILE) {
     const int r = r0 + t;
     int a0 = 0, a1 = 0;
+    double2 o0 = d2(0.0, 0.0), o1 = d2(0.0, 0.0);
     if (r < r1) {
       a0 = p.rowptr[r] - j0;
       a1 = p.rowptr[r + 1] - j0;
+      // the epilogue's own-row operands, loaded now so that their latency
+      // overlaps the gathers of phase 1
+      spmm_row_load<MODE>(p, r, o0, o1);
     }
     double a[PER];
     int ci[PER];
@@ -235,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
         acc.x += q.x;
         acc.y += q.y;
       }
-      spmm_row_out<MODE>(p, c, r, acc, dot);
+      spmm_row_store<MODE>(p, c, r, acc, o0, o1, dot);
     }
   } else {
     double2 acc = d2(0.0, 0.0);

@@ -77,6 +77,7 @@ struct gm_solver {
   // nonzeros per LDS tile (row blocks rowblk[tile_idx]); tile == 0 the
   // row-group kernel with `lanes` lanes per row.
   int lanes = 8, tile = 0, tile_idx = -1;
+  bool use16 = false;   // row blocks: 16-bit column offsets (col16[tile_idx])
   int G_sp = 1, chunk_sp = 1, G_el = 1, chunk_el = 1, G_df = 1, chunk_df = 1;
   int* rowblk[kNumTiles] = {};
   int nrowblk[kNumTiles] = {};
@@ -211,7 +212,7 @@ SpmmArgs spmm_base(gm_solver* s) {
   a.col = s->col;
   a.val = s->val;
   a.rowblk = s->tile > 0 ? s->rowblk[s->tile_idx] : nullptr;
-  a.col16 = s->tile > 0 ? s->col16[s->tile_idx] : nullptr;
+  a.col16 = (s->tile > 0 && s->use16) ? s->col16[s->tile_idx] : nullptr;
   a.nrows = s->n;
   a.chunk = s->chunk_sp;
   a.aabb = s->aabb;
@@ -227,15 +228,18 @@ void set_schedule(gm_solver* s, int lanes) {
   s->lanes = lanes;
   s->tile = 0;
   s->tile_idx = -1;
+  s->use16 = false;
   const int groups = kBlock / lanes;
   s->G_sp = std::max(1, std::min(4096, cdiv_i(s->n, groups)));
   s->chunk_sp = cdiv_i(s->n, s->G_sp);
 }
 
-// Row-block schedule with kTiles[idx] nonzeros per LDS tile.
-void set_tile_schedule(gm_solver* s, int idx) {
+// Row-block schedule with kTiles[idx] nonzeros per LDS tile, with 16-bit
+// column offsets when use16 and they fit.
+void set_tile_schedule(gm_solver* s, int idx, bool use16 = false) {
   s->tile = kTiles[idx];
   s->tile_idx = idx;
+  s->use16 = use16 && s->col16[idx] != nullptr;
   s->G_sp = s->nrowblk[idx];
   s->chunk_sp = 0;
 }
@@ -702,7 +706,7 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
   const bool blocks = (d->flags & GM_FLAG_ROW_BLOCKS) ||
                       (!(d->flags & GM_FLAG_ROW_GROUPS) && avg <= 16.0);
   if (blocks)
-    set_tile_schedule(s, 1);  // row blocks of 1024 nonzeros
+    set_tile_schedule(s, 1, (d->flags & GM_FLAG_INDEX16) != 0);  // 1024-nonzero tiles
   else
     set_schedule(s, lanes);
   s->G_el = std::max(1, std::min(4096, cdiv_i(s->n, kBlock)));
@@ -727,17 +731,21 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
     // keep the fastest.  Each candidate runs a short Lanczos run; the solver
     // state is rebuilt by the next build_subspace anyway.
     // Candidates: row groups of 1..32 lanes (c = lanes), row blocks of each
-    // tile size (c = -1 - tile index).
-    const int cands[] = {1, 2, 4, 8, 16, 32, -1, -2, -3};
+    // tile size with 32-bit indices (c = -1 - tile index) and with 16-bit
+    // column offsets where they fit (c = -101 - tile index).
+    const int cands[] = {1, 2, 4, 8, 16, 32, -1, -2, -3, -101, -102, -103};
     float best = 1e30f;
-    int best_c = s->tile > 0 ? -1 - s->tile_idx : s->lanes;
+    int best_c = s->tile > 0 ? (s->use16 ? -101 : -1) - s->tile_idx : s->lanes;
     hipEvent_t e0, e1;
     GM_HIP(hipEventCreate(&e0));
     GM_HIP(hipEventCreate(&e1));
     for (int l : cands) {
       if (l > 0 && l > 4 * avg + 4) continue;
+      if (l <= -101 && !s->col16[-101 - l]) continue;
       if (l > 0)
         set_schedule(s, l);
+      else if (l <= -101)
+        set_tile_schedule(s, -101 - l, true);
       else
         set_tile_schedule(s, -1 - l);
       GM_TRY(enqueue_build(s));  // warm-up
@@ -753,6 +761,8 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
     hipEventDestroy(e1);
     if (best_c > 0)
       set_schedule(s, best_c);
+    else if (best_c <= -101)
+      set_tile_schedule(s, -101 - best_c, true);
     else
       set_tile_schedule(s, -1 - best_c);
     GM_HIP(hipMemsetAsync(s->status, 0, sizeof(unsigned), st));
@@ -1011,7 +1021,7 @@ gm_status gm_get_schedule(gm_solver* s, int32_t* kind, int32_t* param, int32_t* 
   GM_CHECK_HANDLE(s);
   if (kind) *kind = s->tile > 0 ? 1 : 0;
   if (param) *param = s->tile > 0 ? s->tile : s->lanes;
-  if (index_bytes) *index_bytes = (s->tile > 0 && s->col16[s->tile_idx]) ? 2 : 4;
+  if (index_bytes) *index_bytes = (s->tile > 0 && s->use16) ? 2 : 4;
   return GM_OK;
 }
 

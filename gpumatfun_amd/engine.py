@@ -41,15 +41,17 @@ class LanczosEngine:
     clamp of ``update_U.cu:9``), ``autotune`` (time every SpMM schedule on A
     at create and keep the fastest), ``schedule`` (without autotune: None
     picks row blocks when rows average <= 16 nonzeros and row groups
-    otherwise; ``"rows"`` or ``"blocks"`` forces one), ``index32`` (keep
-    32-bit column indices in the row-block SpMM instead of 16-bit block
-    offsets), ``device``.
+    otherwise; ``"rows"`` or ``"blocks"`` forces one), ``index16`` (without
+    autotune: store the row-block SpMM's column indices as 16-bit offsets from
+    the block's first row where they fit), ``index32`` (never use those
+    offsets, not even when autotuning), ``device``.
     """
 
     def __init__(self, U, A, Q, lambda_, N, gamma, gamma_s_real, gamma_s_imag,
                  exp_lambda, phi_lambda, dt, ode, aode, bode, code, vrest, vamp, vth,
                  vpeak, c1ion, c2ion, mass, wts1, wts2, shifts, valvec=(-1.0, 0.0, 1.0),
-                 *, clamp=True, autotune=False, schedule=None, index32=False, device=0):
+                 *, clamp=True, autotune=False, schedule=None, index32=False, index16=False,
+                 device=0):
         lib = _lib.load()
         self._lib = lib
         self._h = C.c_void_p()
@@ -154,7 +156,8 @@ class LanczosEngine:
             raise GpuMatfunError(_lib.GM_EINVAL, "schedule must be None, 'rows' or 'blocks'")
         d.flags = ((0 if clamp else _lib.GM_FLAG_NO_CLAMP)
                    | (_lib.GM_FLAG_AUTOTUNE if autotune else 0) | sched[schedule]
-                   | (_lib.GM_FLAG_INDEX32 if index32 else 0))
+                   | (_lib.GM_FLAG_INDEX32 if index32 else 0)
+                   | (_lib.GM_FLAG_INDEX16 if index16 else 0))
         d.device = int(device)
         _lib.check(lib.gm_lanczos_create(C.byref(d), C.byref(self._h)))
         self._keep = None

@@ -54,10 +54,12 @@ enum {
     GM_FLAG_ROW_BLOCKS = 1u << 3, /* without AUTOTUNE: the row-block SpMM with
                                     1024-nonzero LDS tiles.  Neither flag: row
                                     blocks when rows average <= 16 nonzeros */
-    GM_FLAG_INDEX32 = 1u << 4     /* keep 32-bit column indices in the
-                                    row-block SpMM (by default they are
-                                    stored as 16-bit offsets from the block's
-                                    first row whenever every offset fits)   */
+    GM_FLAG_INDEX32 = 1u << 4,    /* never store the row-block SpMM's column
+                                    indices as 16-bit offsets from the
+                                    block's first row (the autotuner times
+                                    both widths where the offsets fit)      */
+    GM_FLAG_INDEX16 = 1u << 5     /* without AUTOTUNE: use the 16-bit offsets
+                                    where they fit (default: 32-bit)        */
 };
 
 /*

@@ -283,19 +283,19 @@ def test_mitchell_schaeffer_vs_oracle():
     eng.delete()
 
 
-@pytest.mark.parametrize("schedule,index32", [("rows", False), ("blocks", False),
+@pytest.mark.parametrize("schedule,index16", [("rows", False), ("blocks", False),
                                               ("blocks", True)])
-def test_spmm_schedules_vs_cpu_path(schedule, index32):
-    """Both SpMM schedules (row groups, LDS row blocks with 16- or 32-bit column
+def test_spmm_schedules_vs_cpu_path(schedule, index16):
+    """Both SpMM schedules (row groups, LDS row blocks with 32- or 16-bit column
     indices) on the FEM case with deflation and a degree-2 polynomial, against
     the CPU step loop."""
     inp = fem_case(m=30)
-    print("case fem m=30 schedule", schedule, "index32", index32, flush=True)
-    eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=schedule, index32=index32)
+    print("case fem m=30 schedule", schedule, "index16", index16, flush=True)
+    eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=schedule, index16=index16)
     info = eng.info()
     assert info["spmm_kind"] == (0 if schedule == "rows" else 1)
     if schedule == "blocks":
-        assert info["index_bytes"] == (4 if index32 else 2)
+        assert info["index_bytes"] == (2 if index16 else 4)
     eng.step(3)
     U = eng.get_U()
     S = oracle_setup(inp)
@@ -322,7 +322,7 @@ def _arrow_matrix(N=3000, wide=(100, 200), width=600, eps=0.01):
     return sp.csr_matrix(A)
 
 
-@pytest.mark.parametrize("schedule", ["rows", "blocks", "blocks32", "autotune"])
+@pytest.mark.parametrize("schedule", ["rows", "blocks", "blocks16", "autotune"])
 def test_long_rows_vs_cpu_path(schedule):
     """A row longer than every LDS tile (the row-block kernel's whole-block
     branch) and rows of 600 nonzeros among rows of 3."""
@@ -339,7 +339,7 @@ def test_long_rows_vs_cpu_path(schedule):
                         l1=float(w.min()), lN=float(w.max()), degree=1)
     print("case arrow N=3000 schedule", schedule, flush=True)
     kw = ({"autotune": True} if schedule == "autotune" else
-          {"schedule": "blocks", "index32": True} if schedule == "blocks32" else
+          {"schedule": "blocks", "index16": True} if schedule == "blocks16" else
           {"schedule": schedule})
     eng = LanczosEngine.from_inputs(inp, clamp=False, **kw)
     eng.step(2)
