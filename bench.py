@@ -77,13 +77,41 @@ def log(*a):
 def build_fem1m(args):
     from gpumatfun_amd import problems, quadrature as q
     t0 = time.time()
+    Kdefl = 20
+    cache = (os.path.join(args.setup_cache, f"fem1m_{args.nodes}.npz")
+             if args.setup_cache else None)
+    if cache and os.path.exists(cache):
+        import scipy.sparse as sp
+        z = np.load(cache)
+        A = sp.csr_matrix((z["data"], z["indices"], z["indptr"]), shape=tuple(z["shape"]))
+        mass, lam, Q, lN, u0, g0 = z["mass"], z["lam"], z["Q"], float(z["lN"]), z["u0"], z["g0"]
+        log(f"set-up read from {cache} ({time.time() - t0:.1f}s)")
+    else:
+        A, mass, lam, Q, lN, u0, g0 = _fem1m_setup(args, t0, Kdefl)
+        if cache:
+            os.makedirs(args.setup_cache, exist_ok=True)
+            np.savez(cache, data=A.data, indices=A.indices, indptr=A.indptr,
+                     shape=np.array(A.shape), mass=mass, lam=lam, Q=Q, lN=lN, u0=u0, g0=g0)
+    inp = q.make_inputs(A, u0, g0, m=80, n=64, eta=1e-3, alpha=1.7, dt=1e-3,
+                        l1=float(lam[Kdefl]), lN=lN, degree=8, prob_dim=lN / 4.0,
+                        Q=Q[:, :Kdefl], lam=lam[:Kdefl], mass=mass)
+    cfg = {"workload": "fem_disc_1m_plain", "n": int(A.shape[0]), "nnz": int(A.nnz),
+           "m": 80, "poly_degree": 8, "shifts": 64, "deflation": Kdefl, "alpha": 1.7,
+           "tau": 1e-3, "kappa": 1.0, "columns_per_step": 2, "mesh_seed": 0,
+           "lambda_max": lN, "l1": float(lam[Kdefl])}
+    return inp, cfg, time.time() - t0
+
+
+def _fem1m_setup(args, t0, Kdefl):
+    """Seeded Delaunay disc, P1 stiffness and lumped mass, symrcm order, the
+    Kdefl + 1 smallest eigenpairs and the largest eigenvalue (host)."""
+    from gpumatfun_amd import problems
     nodes, tri = problems.disc_mesh(args.nodes, 4096, seed=0)
     log(f"mesh {nodes.shape[0]} nodes, {tri.shape[0]} triangles ({time.time() - t0:.1f}s)")
     K, mdiag = problems.p1_stiffness_lumped_mass(nodes, tri)
     A, mass, perm = problems.fem_operator(K, mdiag)
     nodes = nodes[perm]
     log(f"A: n={A.shape[0]} nnz={A.nnz} ({time.time() - t0:.1f}s)")
-    Kdefl = 20
     lam, Q = problems.smallest_eigenpairs(A, Kdefl + 1)
     lN = problems.largest_eigenvalue(A)
     log(f"eigenpairs: l1={lam[Kdefl]:.6g} lN={lN:.6g} ({time.time() - t0:.1f}s)")
@@ -96,14 +124,7 @@ def build_fem1m(args):
         xh[np.argmin(r)] = 0.01
     u0 = mass * xh
     g0 = mass * xh * (1 - xh)
-    inp = q.make_inputs(A, u0, g0, m=80, n=64, eta=1e-3, alpha=1.7, dt=1e-3,
-                        l1=float(lam[Kdefl]), lN=lN, degree=8, prob_dim=lN / 4.0,
-                        Q=Q[:, :Kdefl], lam=lam[:Kdefl], mass=mass)
-    cfg = {"workload": "fem_disc_1m_plain", "n": int(A.shape[0]), "nnz": int(A.nnz),
-           "m": 80, "poly_degree": 8, "shifts": 64, "deflation": Kdefl, "alpha": 1.7,
-           "tau": 1e-3, "kappa": 1.0, "columns_per_step": 2, "mesh_seed": 0,
-           "lambda_max": lN, "l1": float(lam[Kdefl])}
-    return inp, cfg, time.time() - t0
+    return A, mass, lam, Q, lN, u0, g0
 
 
 def build_dirichlet128(args):
@@ -505,7 +526,9 @@ def main():
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--schedule", default="auto", choices=["auto", "rows", "blocks"],
+    ap.add_argument("--setup-cache", default=None,
+                    help="directory to keep the FEM set-up (mesh, operator, eigenpairs) between runs")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "rows", "blocks", "blocks16"],
                     help="SpMM schedule of the plain path: autotuned at create (default), "
                          "or forced (fixed kernels for counter runs)")
     ap.add_argument("--profile-only", action="store_true",
@@ -558,7 +581,8 @@ def main():
     if args.schedule == "auto":
         eng = LanczosEngine.from_inputs(inp, clamp=False, autotune=True)
     else:
-        eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=args.schedule)
+        eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=args.schedule[:6],
+                                        index16=args.schedule == "blocks16")
     info = eng.info()
     log(f"create {time.time() - t0:.2f}s, schedule {info['spmm_schedule']}")
 

@@ -259,6 +259,7 @@ struct gm_fab {
   double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
   unsigned* status = nullptr;
   double* dA = nullptr;     // nnz values of dA/dtheta (gm_fab_backward_param)
+  double* partials2 = nullptr;  // finish partials when the reductions are fused
   std::vector<double> dc;   // ns complex dc/dalpha (host)
   Side sb, sg;
 };
@@ -284,7 +285,7 @@ void afree(void* p) {
 void free_fab(gm_fab* f) {
   void* bufs[] = {f->rowptr, f->col, f->val, f->xi, f->ones, f->c, f->gsr, f->gsi, f->C,
                   f->partials, f->kappa, f->grad, f->gxv, f->wD, f->wDU, f->wDU2, f->wX,
-                  f->status, f->dA};
+                  f->status, f->dA, f->partials2};
   for (void* p : bufs) afree(p);
   for (Side* s : {&f->sb, &f->sg}) {
     void* sbufs[] = {s->U, s->V, s->W0, s->W1, s->al, s->be, s->yr, s->yi, s->Yg, s->Xc, s->out};
@@ -293,9 +294,10 @@ void free_fab(gm_fab* f) {
   if (f->stream) (void)hipStreamDestroy(f->stream);
 }
 
-gm_status areduce(gm_fab* f, int G, int nv, double* out, int do_sqrt, int flag_zero) {
+gm_status areduce(gm_fab* f, int G, int nv, double* out, int do_sqrt, int flag_zero,
+                  const double* part = nullptr) {
   gm::ReduceArgs r{};
-  r.partials = f->partials;
+  r.partials = part ? part : f->partials;
   r.G = G;
   r.nv = nv;
   r.out = out;
@@ -334,6 +336,9 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
   base.partials = f->partials;
   base.g0 = 1.0;
   double* Wb[2] = {s.W0, s.W1};
+  // small grids: alpha_k and beta_{k+1} are summed inside the consuming
+  // kernels instead of by k_reduce launches
+  const bool fuse = f->G <= 1024;
   for (int k = 0; k < f->m; ++k) {
     gmb::BSpmm a = base;
     a.k = k;
@@ -343,9 +348,14 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
     a.ldy = 1;
     a.vk = s.V + (size_t)k * N;
     a.vkm1 = k > 0 ? s.V + (size_t)(k - 1) * N : nullptr;
+    a.status = f->status;
+    if (fuse && k > 0) {
+      a.bpart = f->partials2;
+      a.bG = f->G;
+    }
     hipLaunchKernelGGL((gmb::k_bspmm<1, gmb::kOnly>), dim3(f->G), dim3(gm::kBlock), 0, f->stream, a);
     A_HIP(hipGetLastError());
-    A_TRY(areduce(f, f->G, 1, s.al + k, 0, 0));
+    if (!fuse) A_TRY(areduce(f, f->G, 1, s.al + k, 0, 0));
     gmb::BFinish fn{};
     fn.w = Wb[k & 1];
     fn.vk = a.vk;
@@ -354,11 +364,17 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
     fn.k = k;
     fn.nrows = f->n;
     fn.chunk = f->chunk;
-    fn.partials = f->partials;
+    fn.partials = fuse ? f->partials2 : f->partials;
     fn.status = f->status;
+    if (fuse) {
+      fn.apart = f->partials;
+      fn.aG = f->G;
+      fn.al_out = s.al;
+    }
     hipLaunchKernelGGL((gmb::k_bfinish<1>), dim3(f->G), dim3(gm::kBlock), 0, f->stream, fn);
     A_HIP(hipGetLastError());
-    A_TRY(areduce(f, f->G, 1, s.be + k + 1, 1, 1));
+    if (!fuse || k == f->m - 1)
+      A_TRY(areduce(f, f->G, 1, s.be + k + 1, 1, 1, fuse ? f->partials2 : nullptr));
   }
   gmb::BSolve sv{};
   sv.al = s.al;
@@ -506,6 +522,7 @@ gm_status gm_fab_create(const gm_fab_desc* d, gm_fab** out) {
   if (d->dwts) std::copy(d->dwts, d->dwts + 2 * f->ns, f->dc.begin());
   A_CREATE(aalloc(&f->C, (size_t)f->m));
   A_CREATE(aalloc(&f->partials, (size_t)4096 * std::max(2, f->ncol)));
+  A_CREATE(aalloc(&f->partials2, (size_t)4096));
   A_CREATE(aalloc(&f->kappa, (size_t)f->n));
   A_CREATE(aalloc(&f->grad, (size_t)f->n));
   A_CREATE(aalloc(&f->gxv, (size_t)f->ncol));

@@ -35,7 +35,39 @@ struct BSpmm {
   double* alpha_out;                // kLast/kOnly: alpha[k][R]
   double* partials;                 // [G][R]
   unsigned* counter;
+  // kFirst/kOnly with bpart: beta_k[c] = sqrt(sum of the previous finish's bG
+  // partials [bG][R]) formed in every block; block 0 stores it in be[k][c]
+  const double* bpart;
+  int bG;
+  unsigned* status;
 };
+
+// Column sums of G partials [G][R] (written by the previous kernel) into
+// out[0..R) in LDS, the same fixed order in every block: thread t sums
+// column t % W over g = t / W, t / W + ns, ... (ns = kBlock / W, W = min(R,
+// kBlock)), then a fixed tree over the ns sub-sums.
+__device__ __forceinline__ void block_reduce_cols(const double* part, int G, int R, double* out) {
+  __shared__ double sub[kBlock];
+  for (int v0 = 0; v0 < R; v0 += kBlock) {
+    const int W = min(R - v0, kBlock);
+    const int ns = kBlock / W;
+    const int vv = (int)threadIdx.x % W, sb = (int)threadIdx.x / W;
+    double acc = 0.0;
+    if (sb < ns)
+      for (int g = sb; g < G; g += ns) acc += part[(size_t)g * R + v0 + vv];
+    __syncthreads();
+    sub[threadIdx.x] = acc;
+    __syncthreads();
+    int h = 1;
+    while (h < ns) h <<= 1;
+    for (h >>= 1; h > 0; h >>= 1) {
+      if (sb < h && sb + h < ns) sub[threadIdx.x] += sub[threadIdx.x + h * W];
+      __syncthreads();
+    }
+    if ((int)threadIdx.x < W) out[v0 + threadIdx.x] = sub[threadIdx.x];
+    __syncthreads();
+  }
+}
 
 // Per-column block sums (fixed order) written to partials[block][R]; the grid
 // total is formed by k_reduce after the kernel.
@@ -67,12 +99,28 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int slot = lane / CPW, c0 = lane % CPW;
   double bet[CL], inv[CL], bprev[CL], dot[CL];
+  __shared__ double fused[(MODE == kFirst || MODE == kOnly) ? R : 1];
+  if ((MODE == kFirst || MODE == kOnly) && p.bpart) {
+    block_reduce_cols(p.bpart, p.bG, R, fused);
+    for (int c = threadIdx.x; c < R; c += kBlock) {
+      const double bv = sqrt(fused[c]);
+      fused[c] = bv;
+      if (blockIdx.x == 0) {
+        p.be[(size_t)p.k * R + c] = bv;
+        if (!isfinite(bv)) atomicOr(p.status, 4u);
+        else if (bv == 0.0) atomicOr(p.status, 1u);
+      }
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int q = 0; q < CL; ++q) {
     const int c = c0 + q * CPW;
-    bet[q] = (MODE == kFirst || MODE == kOnly) ? p.be[(size_t)p.k * R + c] : 0.0;
+    bet[q] = (MODE == kFirst || MODE == kOnly)
+                 ? (p.bpart ? fused[c] : p.be[(size_t)p.k * R + c]) : 0.0;
     inv[q] = safe_inv(bet[q]) * p.g0;
-    bprev[q] = ((MODE == kLast || MODE == kOnly) && p.k > 0) ? p.be[(size_t)p.k * R + c] : 0.0;
+    bprev[q] = ((MODE == kLast || MODE == kOnly) && p.k > 0)
+                   ? ((MODE == kOnly && p.bpart) ? bet[q] : p.be[(size_t)p.k * R + c]) : 0.0;
     dot[q] = 0.0;
   }
   for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
@@ -169,6 +217,11 @@ struct BFinish {
   double* partials;
   unsigned* counter;
   unsigned* status;
+  // with apart: alpha_k[c] = sum of the SpMM's aG partials [aG][R], formed in
+  // every block; block 0 stores it in al_out[k][c]
+  const double* apart;
+  int aG;
+  double* al_out;
 };
 
 template <int R>
@@ -182,9 +235,15 @@ __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int slot = lane / CPW, c0 = lane % CPW;
   double al[CL], nrm[CL];
+  __shared__ double fused[R];
+  if (p.apart) {
+    block_reduce_cols(p.apart, p.aG, R, fused);
+    if (blockIdx.x == 0)
+      for (int c = threadIdx.x; c < R; c += kBlock) p.al_out[(size_t)p.k * R + c] = fused[c];
+  }
 #pragma unroll
   for (int q = 0; q < CL; ++q) {
-    al[q] = p.al[(size_t)p.k * R + c0 + q * CPW];
+    al[q] = p.apart ? fused[c0 + q * CPW] : p.al[(size_t)p.k * R + c0 + q * CPW];
     nrm[q] = 0.0;
   }
   // two rows per pass, all loads before the stores (w and vk are distinct

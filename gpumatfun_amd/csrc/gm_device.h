@@ -62,6 +62,25 @@ __device__ __forceinline__ void block_partial2(double2 v, double* partials) {
   block_partial2_at(v, partials, blockIdx.x);
 }
 
+// Sum of G [G][2] partials written by the previous kernel, in a fixed order
+// (thread t sums g = t, t + kBlock, ..., then the block tree), returned to
+// every thread.  Every block that calls it gets the same bits; used instead
+// of a separate k_reduce launch when G is small.
+__device__ __forceinline__ double2 block_reduce_partials2(const double* part, int G) {
+  __shared__ double2 bc;
+  double2 s = d2(0.0, 0.0);
+  for (int g = threadIdx.x; g < G; g += kBlock) {
+    s.x += part[2 * g];
+    s.y += part[2 * g + 1];
+  }
+  s = block_sum2(s);
+  if (threadIdx.x == 0) bc = s;
+  __syncthreads();
+  const double2 r = bc;
+  __syncthreads();
+  return r;
+}
+
 // Sums nv values over G blocks of partials ([G][nv]) in a fixed order and
 // writes out[v * stride] (square-rooted when do_sqrt).  One block of
 // kReduceBlock threads: ns = kReduceBlock / W threads per value each sum a

@@ -63,7 +63,13 @@ struct SpmmArgs {
   int accumulate;              // kHorner: y += result instead of y = result
   double* partials;
   unsigned* counter;
-  unsigned* status;            // bit 0: breakdown seen
+  unsigned* status;            // bit 0: beta = 0 seen, bit 2: non-finite beta
+  // kFirst/kOnly with bpart: beta_k = sqrt(sum of the previous finish's bG
+  // partials), formed in every block (no separate reduction launch); block 0
+  // stores it at beta_out[0] / beta_out[ld] and flags it in status
+  const double* bpart;
+  int bG;
+  double* beta_out;
 };
 
 // Per-launch constants of the row epilogue: 1/beta_k, beta_k (kFirst, kOnly),
@@ -79,12 +85,27 @@ __device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p, const dou
   c.bet = d2(0.0, 0.0);
   c.bprev = d2(0.0, 0.0);
   if (MODE == kFirst || MODE == kOnly) {
-    const double b0 = aabb[p.k + 2 * p.ld], b1 = aabb[p.k + 3 * p.ld];
+    double b0, b1;
+    if (p.bpart) {
+      const double2 t = block_reduce_partials2(p.bpart, p.bG);
+      b0 = sqrt(t.x);
+      b1 = sqrt(t.y);
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        p.beta_out[0] = b0;
+        p.beta_out[p.ld] = b1;
+        if (!isfinite(b0) || !isfinite(b1)) atomicOr(p.status, 4u);
+        else if (b0 == 0.0 || b1 == 0.0) atomicOr(p.status, 1u);
+      }
+    } else {
+      b0 = aabb[p.k + 2 * p.ld];
+      b1 = aabb[p.k + 3 * p.ld];
+    }
     c.inv = d2(safe_inv(b0), safe_inv(b1));
     c.bet = d2(b0, b1);
   }
   if ((MODE == kLast || MODE == kOnly) && p.k > 0)
-    c.bprev = d2(aabb[p.k + 2 * p.ld], aabb[p.k + 3 * p.ld]);
+    c.bprev = (MODE == kOnly && p.bpart) ? c.bet
+                                         : d2(aabb[p.k + 2 * p.ld], aabb[p.k + 3 * p.ld]);
   c.sx = (MODE == kFirst || MODE == kOnly) ? d2(p.g0 * c.inv.x, p.g0 * c.inv.y) : d2(1.0, 1.0);
   return c;
 }
@@ -275,6 +296,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
 }
 
 // w <- w - alpha_k v_k (in place), beta_{k+1} = ||w||.   build_subspace.cu:63-76
+// With apart, alpha_k = sum of the SpMM's aG partials is formed in every block
+// and block 0 stores it in aabb (no separate reduction launch).
 struct FinishArgs {
   double2* w;
   const double2* vk;
@@ -283,6 +306,8 @@ struct FinishArgs {
   double* partials;
   unsigned* counter;
   unsigned* status;
+  const double* apart;
+  int aG;
 };
 
 template <int kUnused = 0>
@@ -290,7 +315,16 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_finish(FinishArgs p) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int row0 = b * p.chunk;
   const int row1 = min(row0 + p.chunk, p.nrows);
-  const double2 al = d2(p.aabb[p.k], p.aabb[p.k + p.ld]);
+  double2 al;
+  if (p.apart) {
+    al = block_reduce_partials2(p.apart, p.aG);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      p.aabb[p.k] = al.x;
+      p.aabb[p.k + p.ld] = al.y;
+    }
+  } else {
+    al = d2(p.aabb[p.k], p.aabb[p.k + p.ld]);
+  }
   double2 nrm = d2(0.0, 0.0);
   for (int r = row0 + threadIdx.x; r < row1; r += kBlock) {
     const double2 v = p.vk[r];

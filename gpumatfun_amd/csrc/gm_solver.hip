@@ -98,6 +98,7 @@ struct gm_solver {
          *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *Zdefl = nullptr;
   double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
   double* partials = nullptr;
+  double* partials2 = nullptr;   // finish norm partials when reductions are fused
   unsigned* counter = nullptr;
   unsigned* status = nullptr;
   double* stage = nullptr;  // n x 2 column-major staging for the getters
@@ -144,7 +145,7 @@ void free_all(gm_solver* s) {
   void* bufs[] = {s->rowptr, s->col, s->val, s->U, s->Usol, s->X, s->upd, s->W[0], s->W[1],
                   s->T[0], s->T[1], s->V, s->Z, s->C, s->Q, s->el, s->pl, s->mass, s->ode,
                   s->aabb, s->Yr, s->Yi, s->wts1, s->wts2, s->shifts, s->gsr, s->gsi,
-                  s->Zdefl, s->wD, s->wDU, s->wDU2, s->wX, s->partials, s->counter,
+                  s->Zdefl, s->wD, s->wDU, s->wDU2, s->wX, s->partials, s->partials2, s->counter,
                   s->status, s->stage};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -191,9 +192,9 @@ gm_status launch_spmm_mode(gm_solver* s, const SpmmArgs& a) {
 
 // Sum [G][nv] block partials into out[v * stride] (k_reduce, one block).
 gm_status launch_reduce(gm_solver* s, int G, int nv, double* out, int stride, int do_sqrt,
-                        int flag_zero) {
+                        int flag_zero, const double* part = nullptr) {
   gm::ReduceArgs r{};
-  r.partials = s->partials;
+  r.partials = part ? part : s->partials;
   r.G = G;
   r.nv = nv;
   r.out = out;
@@ -265,9 +266,17 @@ std::vector<int> row_blocks(const std::vector<int>& rp, int n, int tile) {
 
 // build_subspace.cu:5-91.  Iteration k reads w_k (U for k = 0) and writes
 // w_{k+1} into W[k & 1]; beta_k lives in aabb and is applied on the fly.
+// When both grids are small (at most kFuseG blocks), alpha_k and beta_{k+1}
+// are summed inside the kernels that consume them (every block forms the
+// same fixed-order sum of the producer's partials) instead of by k_reduce
+// launches: two launches fewer per Lanczos step on launch-bound sizes.
+constexpr int kFuseG = 1024;
+
 gm_status enqueue_build(gm_solver* s) {
   const int d = s->nd - 1;
   const size_t N = (size_t)s->n;
+  const int ld = s->m + 1;
+  const bool fuse = s->G_sp <= kFuseG && s->G_el <= kFuseG;
   for (int k = 0; k < s->m; ++k) {
     const double2* wsrc = k == 0 ? s->U : s->W[(k - 1) & 1];
     double2* wnext = s->W[k & 1];
@@ -278,16 +287,21 @@ gm_status enqueue_build(gm_solver* s) {
     a.vk = vk;
     a.vkm1 = vkm1;
     a.g0 = s->gamma[0];
+    if (fuse && k > 0) {   // beta_k from the previous finish's partials
+      a.bpart = s->partials2;
+      a.bG = s->G_el;
+      a.beta_out = s->aabb + k + 2 * ld;
+    }
     if (d == 0) {
       a.x = wsrc;
       a.y = wnext;
       GM_TRY(launch_spmm_mode<gm::kOnly>(s, a));
-      GM_TRY(launch_reduce(s, s->G_sp, 2, s->aabb + k, s->m + 1, 0, 0));   // alpha_k
     } else {
       a.x = wsrc;
       a.y = s->T[0];
       a.gi = s->gamma[1];
       GM_TRY(launch_spmm_mode<gm::kFirst>(s, a));
+      a.bpart = nullptr;
       for (int i = 2; i <= d; ++i) {
         a.x = s->T[(i - 2) & 1];
         a.y = s->T[(i - 1) & 1];
@@ -297,8 +311,8 @@ gm_status enqueue_build(gm_solver* s) {
       a.x = s->T[(d - 1) & 1];
       a.y = wnext;
       GM_TRY(launch_spmm_mode<gm::kLast>(s, a));
-      GM_TRY(launch_reduce(s, s->G_sp, 2, s->aabb + k, s->m + 1, 0, 0));   // alpha_k
     }
+    if (!fuse) GM_TRY(launch_reduce(s, s->G_sp, 2, s->aabb + k, ld, 0, 0));   // alpha_k
     gm::FinishArgs f{};
     f.w = wnext;
     f.vk = vk;
@@ -307,13 +321,20 @@ gm_status enqueue_build(gm_solver* s) {
     f.k = k;
     f.nrows = s->n;
     f.chunk = s->chunk_el;
-    f.partials = s->partials;
+    f.partials = fuse ? s->partials2 : s->partials;
     f.counter = s->counter;
     f.status = s->status;
+    if (fuse) {
+      f.apart = s->partials;
+      f.aG = s->G_sp;
+    }
     hipLaunchKernelGGL(gm::k_lanczos_finish<0>, dim3(s->G_el), dim3(kBlock), 0, s->stream, f);
     GM_HIP(hipGetLastError());
-    // beta_{k+1} = ||w||, flagged when zero or non-finite
-    GM_TRY(launch_reduce(s, s->G_el, 2, s->aabb + (k + 1) + 2 * (s->m + 1), s->m + 1, 1, 1));
+    // beta_{k+1} = ||w||, flagged when zero or non-finite (fused: formed by
+    // the next iteration's first SpMM, except after the last iteration)
+    if (!fuse || k == s->m - 1)
+      GM_TRY(launch_reduce(s, s->G_el, 2, s->aabb + (k + 1) + 2 * ld, ld, 1, 1,
+                           fuse ? s->partials2 : nullptr));
   }
   return GM_OK;
 }
@@ -716,6 +737,7 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
   const size_t npart = std::max<size_t>(2 * (size_t)std::max(4096, max_blocks),
                                         (size_t)s->G_df * 2 * std::max(1, s->K));
   GM_TRY(dalloc(&s->partials, npart));
+  GM_TRY(dalloc(&s->partials2, 2 * (size_t)kFuseG));
   GM_TRY(dalloc(&s->counter, 1));
   GM_TRY(dalloc(&s->status, 1));
   GM_HIP(hipMemsetAsync(s->counter, 0, sizeof(unsigned), st));
