@@ -40,7 +40,9 @@ def kernel_symbol(info, mode):
     if info["spmm_kind"] == 0:
         return f"void gm::k_spmm<{info['spmm_param']}, {mode}>(gm::SpmmArgs)"
     c16 = "true" if info["index_bytes"] == 2 else "false"
-    return f"void gm::k_spmm_stream<{info['spmm_param']}, {mode}, false, {c16}>(gm::SpmmArgs)"
+    pre = "true" if info["spmm_kind"] == 2 else "false"
+    return (f"void gm::k_spmm_stream<{info['spmm_param']}, {mode}, false, {c16}, {pre}>"
+            "(gm::SpmmArgs)")
 
 
 def spmm_bytes(info, mode):

@@ -70,6 +70,17 @@ struct SpmmArgs {
   const double* bpart;
   int bG;
   double* beta_out;
+  const int* prob_blk;         // SEG with bpart: problem q's partials are
+                               // prob_blk[q] .. prob_blk[q+1]-1, its beta_k goes to
+                               // its aabb (k + 2 ld), written by its first block
+};
+
+// Where kFirst/kOnly take beta_k from when it is formed in the kernel.
+struct FusedBeta {
+  const double* part;   // null: read beta_k from aabb
+  int G;
+  double* out;
+  bool writer;
 };
 
 // Per-launch constants of the row epilogue: 1/beta_k, beta_k (kFirst, kOnly),
@@ -79,20 +90,21 @@ struct SpmmScalars {
 };
 
 template <int MODE>
-__device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p, const double* aabb) {
+__device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p, const double* aabb,
+                                                    const FusedBeta& fb) {
   SpmmScalars c;
   c.inv = d2(1.0, 1.0);
   c.bet = d2(0.0, 0.0);
   c.bprev = d2(0.0, 0.0);
   if (MODE == kFirst || MODE == kOnly) {
     double b0, b1;
-    if (p.bpart) {
-      const double2 t = block_reduce_partials2(p.bpart, p.bG);
+    if (fb.part) {
+      const double2 t = block_reduce_partials2(fb.part, fb.G);
       b0 = sqrt(t.x);
       b1 = sqrt(t.y);
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        p.beta_out[0] = b0;
-        p.beta_out[p.ld] = b1;
+      if (fb.writer && threadIdx.x == 0) {
+        fb.out[0] = b0;
+        fb.out[p.ld] = b1;
         if (!isfinite(b0) || !isfinite(b1)) atomicOr(p.status, 4u);
         else if (b0 == 0.0 || b1 == 0.0) atomicOr(p.status, 1u);
       }
@@ -104,7 +116,7 @@ __device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p, const dou
     c.bet = d2(b0, b1);
   }
   if ((MODE == kLast || MODE == kOnly) && p.k > 0)
-    c.bprev = (MODE == kOnly && p.bpart) ? c.bet
+    c.bprev = (MODE == kOnly && fb.part) ? c.bet
                                          : d2(aabb[p.k + 2 * p.ld], aabb[p.k + 3 * p.ld]);
   c.sx = (MODE == kFirst || MODE == kOnly) ? d2(p.g0 * c.inv.x, p.g0 * c.inv.y) : d2(1.0, 1.0);
   return c;
@@ -183,7 +195,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
   const int row1 = min(row0 + p.chunk, p.nrows);
   const int sub = threadIdx.x % LANES;
   const int grp = threadIdx.x / LANES;
-  const SpmmScalars c = spmm_scalars<MODE>(p, p.aabb);
+  const FusedBeta fb{p.bpart, p.bG, p.beta_out, blockIdx.x == 0};
+  const SpmmScalars c = spmm_scalars<MODE>(p, p.aabb, fb);
   double2 dot = d2(0.0, 0.0);
 
   for (int r = row0 + grp; r < row1; r += GROUPS) {
@@ -222,8 +235,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
 // partial at its row-block index (so a problem's partials are contiguous).
 // C16: column indices are stored as 16-bit offsets from the block's first row
 // (banded orderings such as symrcm keep them small), 2 bytes per nonzero
-// instead of 4.
-template <int TILE, int MODE, bool SEG = false, bool C16 = false>
+// instead of 4.  PRE: the epilogue's own-row operands are loaded before the
+// gathers (their latency overlaps phase 1) instead of after the LDS sums;
+// which is faster depends on the matrix, so the autotuner times both.
+template <int TILE, int MODE, bool SEG = false, bool C16 = false, bool PRE = true>
 __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
   static_assert(TILE % kBlock == 0, "TILE must be a multiple of the block size");
   constexpr int PER = TILE / kBlock;
@@ -232,8 +247,17 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
   const int r0 = p.rowblk[b], r1 = p.rowblk[b + 1];
   const int t = threadIdx.x;
   const int j0 = p.rowptr[r0], j1 = p.rowptr[r1];
-  const double* aabb = SEG ? p.aabb + (size_t)p.blk_prob[b] * p.aabb_stride : p.aabb;
-  const SpmmScalars c = spmm_scalars<MODE>(p, aabb);
+  const int q = SEG ? p.blk_prob[b] : 0;
+  const double* aabb = SEG ? p.aabb + (size_t)q * p.aabb_stride : p.aabb;
+  FusedBeta fb{p.bpart, p.bG, p.beta_out, blockIdx.x == 0};
+  if (SEG && p.bpart) {
+    const int g0 = p.prob_blk[q];
+    fb.part = p.bpart + 2 * (size_t)g0;
+    fb.G = p.prob_blk[q + 1] - g0;
+    fb.out = const_cast<double*>(aabb) + p.k + 2 * p.ld;
+    fb.writer = b == g0;
+  }
+  const SpmmScalars c = spmm_scalars<MODE>(p, aabb, fb);
   double2 dot = d2(0.0, 0.0);
 
   if (j1 - j0 <= TILE) {
@@ -243,9 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
     if (r < r1) {
       a0 = p.rowptr[r] - j0;
       a1 = p.rowptr[r + 1] - j0;
-      // the epilogue's own-row operands, loaded now so that their latency
-      // overlaps the gathers of phase 1
-      spmm_row_load<MODE>(p, r, o0, o1);
+      if (PRE) spmm_row_load<MODE>(p, r, o0, o1);
     }
     double a[PER];
     int ci[PER];
@@ -278,6 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
         acc.x += q.x;
         acc.y += q.y;
       }
+      if (!PRE) spmm_row_load<MODE>(p, r, o0, o1);
       spmm_row_store<MODE>(p, c, r, acc, o0, o1, dot);
     }
   } else {

@@ -78,6 +78,7 @@ struct gm_solver {
   // row-group kernel with `lanes` lanes per row.
   int lanes = 8, tile = 0, tile_idx = -1;
   bool use16 = false;   // row blocks: 16-bit column offsets (col16[tile_idx])
+  bool pre = false;     // row blocks: epilogue operands loaded before the gathers
   int G_sp = 1, chunk_sp = 1, G_el = 1, chunk_el = 1, G_df = 1, chunk_df = 1;
   int* rowblk[kNumTiles] = {};
   int nrowblk[kNumTiles] = {};
@@ -159,13 +160,15 @@ template <int MODE>
 gm_status launch_spmm_mode(gm_solver* s, const SpmmArgs& a) {
   dim3 grid(s->G_sp), block(kBlock);
   if (s->tile > 0) {
-    const bool c16 = a.col16 != nullptr;
-#define GM_STREAM(T)                                                                          \
-  case T:                                                                                     \
-    if (c16)                                                                                  \
-      hipLaunchKernelGGL((gm::k_spmm_stream<T, MODE, false, true>), grid, block, 0, s->stream, a); \
-    else                                                                                      \
-      hipLaunchKernelGGL((gm::k_spmm_stream<T, MODE, false, false>), grid, block, 0, s->stream, a); \
+    const int v = (a.col16 != nullptr ? 1 : 0) + (s->pre ? 2 : 0);
+#define GM_STREAM(T)                                                                                  \
+  case T:                                                                                             \
+    switch (v) {                                                                                      \
+      case 0: hipLaunchKernelGGL((gm::k_spmm_stream<T, MODE, false, false, false>), grid, block, 0, s->stream, a); break; \
+      case 1: hipLaunchKernelGGL((gm::k_spmm_stream<T, MODE, false, true, false>), grid, block, 0, s->stream, a); break; \
+      case 2: hipLaunchKernelGGL((gm::k_spmm_stream<T, MODE, false, false, true>), grid, block, 0, s->stream, a); break; \
+      default: hipLaunchKernelGGL((gm::k_spmm_stream<T, MODE, false, true, true>), grid, block, 0, s->stream, a); break; \
+    }                                                                                                 \
     break;
     switch (s->tile) {
       GM_STREAM(512)
@@ -230,17 +233,20 @@ void set_schedule(gm_solver* s, int lanes) {
   s->tile = 0;
   s->tile_idx = -1;
   s->use16 = false;
+  s->pre = false;
   const int groups = kBlock / lanes;
   s->G_sp = std::max(1, std::min(4096, cdiv_i(s->n, groups)));
   s->chunk_sp = cdiv_i(s->n, s->G_sp);
 }
 
 // Row-block schedule with kTiles[idx] nonzeros per LDS tile, with 16-bit
-// column offsets when use16 and they fit.
-void set_tile_schedule(gm_solver* s, int idx, bool use16 = false) {
+// column offsets when use16 and they fit, epilogue operands loaded early
+// when pre.
+void set_tile_schedule(gm_solver* s, int idx, bool use16 = false, bool pre = false) {
   s->tile = kTiles[idx];
   s->tile_idx = idx;
   s->use16 = use16 && s->col16[idx] != nullptr;
+  s->pre = pre;
   s->G_sp = s->nrowblk[idx];
   s->chunk_sp = 0;
 }
@@ -727,7 +733,8 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
   const bool blocks = (d->flags & GM_FLAG_ROW_BLOCKS) ||
                       (!(d->flags & GM_FLAG_ROW_GROUPS) && avg <= 16.0);
   if (blocks)
-    set_tile_schedule(s, 1, (d->flags & GM_FLAG_INDEX16) != 0);  // 1024-nonzero tiles
+    set_tile_schedule(s, 1, (d->flags & GM_FLAG_INDEX16) != 0,   // 1024-nonzero tiles
+                      (d->flags & GM_FLAG_EARLY_LOADS) != 0);
   else
     set_schedule(s, lanes);
   s->G_el = std::max(1, std::min(4096, cdiv_i(s->n, kBlock)));
@@ -752,24 +759,32 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
     // Time the build-subspace SpMM for each lane width on this matrix and
     // keep the fastest.  Each candidate runs a short Lanczos run; the solver
     // state is rebuilt by the next build_subspace anyway.
-    // Candidates: row groups of 1..32 lanes (c = lanes), row blocks of each
-    // tile size with 32-bit indices (c = -1 - tile index) and with 16-bit
-    // column offsets where they fit (c = -101 - tile index).
-    const int cands[] = {1, 2, 4, 8, 16, 32, -1, -2, -3, -101, -102, -103};
+    // Candidates: row groups of 1..32 lanes (c = lanes), and row blocks
+    // encoded as c = -(1 + tile index + 100 c16 + 1000 pre): every tile size
+    // with 32-bit indices or 16-bit column offsets (where they fit), with the
+    // epilogue operands loaded early or late.
+    std::vector<int> cands = {1, 2, 4, 8, 16, 32};
+    for (int pre = 0; pre < 2; ++pre)
+      for (int c16 = 0; c16 < 2; ++c16)
+        for (int i = 0; i < kNumTiles; ++i) cands.push_back(-(1 + i + 100 * c16 + 1000 * pre));
+    auto apply = [&](int c) {
+      if (c > 0) {
+        set_schedule(s, c);
+      } else {
+        const int e = -c - 1;
+        set_tile_schedule(s, e % 100, (e / 100) % 10 == 1, e / 1000 == 1);
+      }
+    };
     float best = 1e30f;
-    int best_c = s->tile > 0 ? (s->use16 ? -101 : -1) - s->tile_idx : s->lanes;
+    int best_c = s->tile > 0 ? -(1 + s->tile_idx + (s->use16 ? 100 : 0) + (s->pre ? 1000 : 0))
+                             : s->lanes;
     hipEvent_t e0, e1;
     GM_HIP(hipEventCreate(&e0));
     GM_HIP(hipEventCreate(&e1));
     for (int l : cands) {
       if (l > 0 && l > 4 * avg + 4) continue;
-      if (l <= -101 && !s->col16[-101 - l]) continue;
-      if (l > 0)
-        set_schedule(s, l);
-      else if (l <= -101)
-        set_tile_schedule(s, -101 - l, true);
-      else
-        set_tile_schedule(s, -1 - l);
+      if (l < 0 && ((-l - 1) / 100) % 10 == 1 && !s->col16[(-l - 1) % 100]) continue;
+      apply(l);
       GM_TRY(enqueue_build(s));  // warm-up
       GM_HIP(hipEventRecord(e0, st));
       GM_TRY(enqueue_build(s));
@@ -781,12 +796,7 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    if (best_c > 0)
-      set_schedule(s, best_c);
-    else if (best_c <= -101)
-      set_tile_schedule(s, -101 - best_c, true);
-    else
-      set_tile_schedule(s, -1 - best_c);
+    apply(best_c);
     GM_HIP(hipMemsetAsync(s->status, 0, sizeof(unsigned), st));
     GM_HIP(hipStreamSynchronize(st));
   }
@@ -1041,7 +1051,7 @@ gm_status gm_get_stream(gm_solver* s, void** stream) {
 
 gm_status gm_get_schedule(gm_solver* s, int32_t* kind, int32_t* param, int32_t* index_bytes) {
   GM_CHECK_HANDLE(s);
-  if (kind) *kind = s->tile > 0 ? 1 : 0;
+  if (kind) *kind = s->tile > 0 ? (s->pre ? 2 : 1) : 0;
   if (param) *param = s->tile > 0 ? s->tile : s->lanes;
   if (index_bytes) *index_bytes = (s->tile > 0 && s->use16) ? 2 : 4;
   return GM_OK;

@@ -44,14 +44,16 @@ class LanczosEngine:
     otherwise; ``"rows"`` or ``"blocks"`` forces one), ``index16`` (without
     autotune: store the row-block SpMM's column indices as 16-bit offsets from
     the block's first row where they fit), ``index32`` (never use those
-    offsets, not even when autotuning), ``device``.
+    offsets, not even when autotuning), ``early_loads`` (without autotune: the
+    row-block SpMM loads each row's own operands before its gathers),
+    ``device``.
     """
 
     def __init__(self, U, A, Q, lambda_, N, gamma, gamma_s_real, gamma_s_imag,
                  exp_lambda, phi_lambda, dt, ode, aode, bode, code, vrest, vamp, vth,
                  vpeak, c1ion, c2ion, mass, wts1, wts2, shifts, valvec=(-1.0, 0.0, 1.0),
                  *, clamp=True, autotune=False, schedule=None, index32=False, index16=False,
-                 device=0):
+                 early_loads=False, device=0):
         lib = _lib.load()
         self._lib = lib
         self._h = C.c_void_p()
@@ -157,7 +159,8 @@ class LanczosEngine:
         d.flags = ((0 if clamp else _lib.GM_FLAG_NO_CLAMP)
                    | (_lib.GM_FLAG_AUTOTUNE if autotune else 0) | sched[schedule]
                    | (_lib.GM_FLAG_INDEX32 if index32 else 0)
-                   | (_lib.GM_FLAG_INDEX16 if index16 else 0))
+                   | (_lib.GM_FLAG_INDEX16 if index16 else 0)
+                   | (_lib.GM_FLAG_EARLY_LOADS if early_loads else 0))
         d.device = int(device)
         _lib.check(lib.gm_lanczos_create(C.byref(d), C.byref(self._h)))
         self._keep = None
@@ -274,7 +277,8 @@ class LanczosEngine:
         if kind.value == 0:
             sched = f"k_spmm<{param.value}>"
         else:
-            sched = f"k_spmm_stream<{param.value}{',c16' if ib.value == 2 else ''}>"
+            sched = (f"k_spmm_stream<{param.value}{',c16' if ib.value == 2 else ''}"
+                     f"{',pre' if kind.value == 2 else ''}>")
         return {"n": n.value, "nnz": nnz.value, "m": m.value, "gamma_size": nd.value,
                 "contour_size": ns.value, "num_eigs": K.value, "spmm_lanes": lanes.value,
                 "spmm_schedule": sched, "spmm_kind": kind.value, "spmm_param": param.value,

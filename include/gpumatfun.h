@@ -58,8 +58,11 @@ enum {
                                     indices as 16-bit offsets from the
                                     block's first row (the autotuner times
                                     both widths where the offsets fit)      */
-    GM_FLAG_INDEX16 = 1u << 5     /* without AUTOTUNE: use the 16-bit offsets
+    GM_FLAG_INDEX16 = 1u << 5,    /* without AUTOTUNE: use the 16-bit offsets
                                     where they fit (default: 32-bit)        */
+    GM_FLAG_EARLY_LOADS = 1u << 6 /* without AUTOTUNE: the row-block SpMM loads
+                                    each row's own operands before its
+                                    gathers (default: after)                */
 };
 
 /*
@@ -187,7 +190,8 @@ gm_status gm_get_info(gm_solver* s, int64_t* n, int64_t* nnz, int32_t* m,
                       int32_t* gamma_size, int32_t* contour_size,
                       int32_t* num_eigs, int32_t* spmm_lanes);
 /* The SpMM schedule in use: kind 0 = row groups (param = lanes per row),
- * 1 = row blocks (param = nonzeros per LDS tile); index_bytes = bytes per
+ * 1 = row blocks (param = nonzeros per LDS tile), 2 = row blocks with the
+ * epilogue operands loaded before the gathers; index_bytes = bytes per
  * stored column index (4, or 2 for 16-bit block offsets).                  */
 gm_status gm_get_schedule(gm_solver* s, int32_t* kind, int32_t* param, int32_t* index_bytes);
 

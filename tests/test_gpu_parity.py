@@ -283,17 +283,21 @@ def test_mitchell_schaeffer_vs_oracle():
     eng.delete()
 
 
-@pytest.mark.parametrize("schedule,index16", [("rows", False), ("blocks", False),
-                                              ("blocks", True)])
-def test_spmm_schedules_vs_cpu_path(schedule, index16):
+@pytest.mark.parametrize("schedule,index16,early", [("rows", False, False),
+                                                    ("blocks", False, False),
+                                                    ("blocks", True, False),
+                                                    ("blocks", False, True),
+                                                    ("blocks", True, True)])
+def test_spmm_schedules_vs_cpu_path(schedule, index16, early):
     """Both SpMM schedules (row groups, LDS row blocks with 32- or 16-bit column
     indices) on the FEM case with deflation and a degree-2 polynomial, against
     the CPU step loop."""
     inp = fem_case(m=30)
     print("case fem m=30 schedule", schedule, "index16", index16, flush=True)
-    eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=schedule, index16=index16)
+    eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=schedule, index16=index16,
+                                    early_loads=early)
     info = eng.info()
-    assert info["spmm_kind"] == (0 if schedule == "rows" else 1)
+    assert info["spmm_kind"] == (0 if schedule == "rows" else 2 if early else 1)
     if schedule == "blocks":
         assert info["index_bytes"] == (2 if index16 else 4)
     eng.step(3)
