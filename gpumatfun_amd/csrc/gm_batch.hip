@@ -85,7 +85,7 @@ struct gm_batch {
          *yr = nullptr, *yi = nullptr, *gsr = nullptr, *gsi = nullptr, *wts1 = nullptr,
          *wts2 = nullptr, *etaxi = nullptr, *mass = nullptr, *partials = nullptr, *stage = nullptr;
   double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
-  unsigned *counter = nullptr, *status = nullptr;
+  unsigned* status = nullptr;
   int G_sp = 1, chunk_sp = 1;
   // deflation by the shared (Q, Lambda): Q row-major [N][K], Zd [K][R],
   // Ud = deflated starting vectors, upd = the deflated part's update
@@ -118,7 +118,7 @@ void bfree_all(gm_batch* b) {
   void* bufs[] = {b->rowptr, b->col, b->val, b->Usol, b->X, b->W0, b->W1, b->T0, b->T1, b->V,
                   b->Z, b->C, b->al, b->be, b->yr, b->yi, b->gsr, b->gsi, b->wts1, b->wts2,
                   b->etaxi, b->mass, b->partials, b->stage, b->wD, b->wDU, b->wDU2, b->wX,
-                  b->counter, b->status, b->Q, b->el, b->pl, b->Zd, b->Ud, b->upd, b->ode};
+                  b->status, b->Q, b->el, b->pl, b->Zd, b->Ud, b->upd, b->ode};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -189,7 +189,6 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
   base.be = b->be;
   base.alpha_out = b->al;
   base.partials = b->partials;
-  base.counter = b->counter;
   base.g0 = b->gamma[0];
   // the chunk's starting vectors: Usol columns c0..c0+R-1 (row stride R_total)
   {
@@ -200,7 +199,6 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
     nn.nrows = b->n;
     nn.chunk = b->chunk_sp;
     nn.partials = b->partials;
-    nn.counter = b->counter;
     nn.status = b->status;
     hipLaunchKernelGGL((gmb::k_bnorm<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, nn);
     B_HIP(hipGetLastError());
@@ -248,7 +246,6 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
     f.nrows = b->n;
     f.chunk = b->chunk_sp;
     f.partials = b->partials;
-    f.counter = b->counter;
     f.status = b->status;
     B_TRY(Launch<R>::finish(b, f));
     B_TRY(breduce(b, R, b->be + (size_t)(k + 1) * R, 1, 1));          // beta_{k+1}
@@ -348,7 +345,6 @@ gm_status enqueue_update_chunk(gm_batch* b, int c0, int prepare) {
   u.chunk = b->chunk_sp;
   u.be = b->be;   // overwritten by the next chunk's norm kernel anyway
   u.partials = b->partials;
-  u.counter = b->counter;
   u.status = b->status;
   return Launch<R>::update(b, u);
 }
@@ -616,10 +612,8 @@ gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
     if (hipStreamSynchronize(st) != hipSuccess) return fail_clean(bfail(GM_EDEVICE, "sync"));
   }
   B_CREATE(balloc(&b->stage, N * b->R));
-  B_CREATE(balloc(&b->counter, 1));
   B_CREATE(balloc(&b->status, 1));
-  if (hipMemsetAsync(b->counter, 0, sizeof(unsigned), st) != hipSuccess ||
-      hipMemsetAsync(b->status, 0, sizeof(unsigned), st) != hipSuccess)
+  if (hipMemsetAsync(b->status, 0, sizeof(unsigned), st) != hipSuccess)
     return fail_clean(bfail(GM_EDEVICE, "memset"));
   // chunk buffers: halve the chunk until they fit
   for (;;) {

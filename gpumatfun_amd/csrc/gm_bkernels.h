@@ -34,7 +34,6 @@ struct BSpmm {
   int accumulate;
   double* alpha_out;                // kLast/kOnly: alpha[k][R]
   double* partials;                 // [G][R]
-  unsigned* counter;
   // kFirst/kOnly with bpart: beta_k[c] = sqrt(sum of the previous finish's bG
   // partials [bG][R]) formed in every block; block 0 stores it in be[k][c]
   const double* bpart;
@@ -215,7 +214,6 @@ struct BFinish {
   double* be;
   int k, nrows, chunk;
   double* partials;
-  unsigned* counter;
   unsigned* status;
   // with apart: alpha_k[c] = sum of the SpMM's aG partials [aG][R], formed in
   // every block; block 0 stores it in al_out[k][c]
@@ -287,7 +285,6 @@ struct BNorm {
   double* be;
   int nrows, chunk;
   double* partials;
-  unsigned* counter;
   unsigned* status;
 };
 
@@ -486,7 +483,6 @@ struct BUpdate {
   double ms_tau_in, ms_tau_out, ms_tau_open, ms_tau_close, ms_v_gate;
   double* be;         // be[0][c]
   double* partials;
-  unsigned* counter;
   unsigned* status;
 };
 

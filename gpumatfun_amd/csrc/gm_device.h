@@ -14,14 +14,6 @@ constexpr int kWaves = kBlock / 64;
 
 __device__ __forceinline__ double2 d2(double a, double b) { return make_double2(a, b); }
 
-// Load that bypasses the non-coherent per-CU cache: used only by the last
-// block to read partials that other CUs wrote during the same kernel.
-__device__ __forceinline__ double ld_coherent(const double* p) {
-  unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double(static_cast<long long>(v));
-}
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

@@ -62,7 +62,6 @@ struct SpmmArgs {
   double g0, gi;               // gamma_0, gamma_i
   int accumulate;              // kHorner: y += result instead of y = result
   double* partials;
-  unsigned* counter;
   unsigned* status;            // bit 0: beta = 0 seen, bit 2: non-finite beta
   // kFirst/kOnly with bpart: beta_k = sqrt(sum of the previous finish's bG
   // partials), formed in every block (no separate reduction launch); block 0
@@ -327,7 +326,6 @@ struct FinishArgs {
   double* aabb;
   int ld, k, nrows, chunk;
   double* partials;
-  unsigned* counter;
   unsigned* status;
   const double* apart;
   int aG;
@@ -586,7 +584,6 @@ struct UpdateArgs {
   const double* Q;     // row-major N x K
   int K, nrows, chunk;
   double* partials;    // K == 0: 2 per block; K > 0: 2K per block
-  unsigned* counter;
   double* beta0;       // &aabb[2 ld] and &aabb[3 ld] -> beta0[0], beta0[ld]
   int ld;
   double* Zout;        // 2K (K > 0): Z = Q^T Usol, [k][c]
@@ -689,7 +686,6 @@ struct Update2Args {
   const double* pl;      // K
   int K, nrows, chunk, ld;
   double* partials;
-  unsigned* counter;
   double* beta0;
   unsigned* status;
 };

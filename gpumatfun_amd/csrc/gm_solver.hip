@@ -100,7 +100,6 @@ struct gm_solver {
   double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
   double* partials = nullptr;
   double* partials2 = nullptr;   // finish norm partials when reductions are fused
-  unsigned* counter = nullptr;
   unsigned* status = nullptr;
   double* stage = nullptr;  // n x 2 column-major staging for the getters
   // host-side state machine
@@ -146,7 +145,7 @@ void free_all(gm_solver* s) {
   void* bufs[] = {s->rowptr, s->col, s->val, s->U, s->Usol, s->X, s->upd, s->W[0], s->W[1],
                   s->T[0], s->T[1], s->V, s->Z, s->C, s->Q, s->el, s->pl, s->mass, s->ode,
                   s->aabb, s->Yr, s->Yi, s->wts1, s->wts2, s->shifts, s->gsr, s->gsi,
-                  s->Zdefl, s->wD, s->wDU, s->wDU2, s->wX, s->partials, s->partials2, s->counter,
+                  s->Zdefl, s->wD, s->wDU, s->wDU2, s->wX, s->partials, s->partials2,
                   s->status, s->stage};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -222,7 +221,6 @@ SpmmArgs spmm_base(gm_solver* s) {
   a.aabb = s->aabb;
   a.ld = s->m + 1;
   a.partials = s->partials;
-  a.counter = s->counter;
   a.status = s->status;
   return a;
 }
@@ -328,7 +326,6 @@ gm_status enqueue_build(gm_solver* s) {
     f.nrows = s->n;
     f.chunk = s->chunk_el;
     f.partials = fuse ? s->partials2 : s->partials;
-    f.counter = s->counter;
     f.status = s->status;
     if (fuse) {
       f.apart = s->partials;
@@ -450,7 +447,6 @@ gm_status enqueue_update(gm_solver* s, int mode) {
   a.K = s->K;
   a.nrows = s->n;
   a.partials = s->partials;
-  a.counter = s->counter;
   a.beta0 = s->aabb + 2 * (s->m + 1);
   a.ld = s->m + 1;
   a.Zout = s->Zdefl;
@@ -492,7 +488,6 @@ gm_status enqueue_update(gm_solver* s, int mode) {
     b.chunk = s->chunk_el;
     b.ld = s->m + 1;
     b.partials = s->partials;
-    b.counter = s->counter;
     b.beta0 = s->aabb + 2 * (s->m + 1);
     b.status = s->status;
     hipLaunchKernelGGL(gm::k_update2<0>, dim3(s->G_el), dim3(kBlock), 4 * s->K * sizeof(double),
@@ -745,9 +740,7 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
                                         (size_t)s->G_df * 2 * std::max(1, s->K));
   GM_TRY(dalloc(&s->partials, npart));
   GM_TRY(dalloc(&s->partials2, 2 * (size_t)kFuseG));
-  GM_TRY(dalloc(&s->counter, 1));
   GM_TRY(dalloc(&s->status, 1));
-  GM_HIP(hipMemsetAsync(s->counter, 0, sizeof(unsigned), st));
   GM_HIP(hipMemsetAsync(s->status, 0, sizeof(unsigned), st));
 
   // deflation of the initial vectors (lanczos_impl.cu:84-89) and beta_0
