@@ -69,9 +69,6 @@ struct SpmmArgs {
   const double* bpart;
   int bG;
   double* beta_out;
-  const int* prob_blk;         // SEG with bpart: problem q's partials are
-                               // prob_blk[q] .. prob_blk[q+1]-1, its beta_k goes to
-                               // its aabb (k + 2 ld), written by its first block
 };
 
 // Where kFirst/kOnly take beta_k from when it is formed in the kernel.
@@ -246,16 +243,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
   const int r0 = p.rowblk[b], r1 = p.rowblk[b + 1];
   const int t = threadIdx.x;
   const int j0 = p.rowptr[r0], j1 = p.rowptr[r1];
-  const int q = SEG ? p.blk_prob[b] : 0;
-  const double* aabb = SEG ? p.aabb + (size_t)q * p.aabb_stride : p.aabb;
-  FusedBeta fb{p.bpart, p.bG, p.beta_out, blockIdx.x == 0};
-  if (SEG && p.bpart) {
-    const int g0 = p.prob_blk[q];
-    fb.part = p.bpart + 2 * (size_t)g0;
-    fb.G = p.prob_blk[q + 1] - g0;
-    fb.out = const_cast<double*>(aabb) + p.k + 2 * p.ld;
-    fb.writer = b == g0;
-  }
+  const double* aabb = SEG ? p.aabb + (size_t)p.blk_prob[b] * p.aabb_stride : p.aabb;
+  const FusedBeta fb{SEG ? nullptr : p.bpart, p.bG, p.beta_out, blockIdx.x == 0};
   const SpmmScalars c = spmm_scalars<MODE>(p, aabb, fb);
   double2 dot = d2(0.0, 0.0);
 

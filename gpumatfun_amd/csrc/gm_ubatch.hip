@@ -38,39 +38,23 @@ constexpr int kTile = 1024;   // nonzeros per row block
 
 // w <- w - alpha_k v_k with the problem's alpha; partial ||w||^2 per block.
 // build_subspace.cu:63-76 (per problem).
-// With apart, alpha_k of the block's problem is the sum of that problem's
-// SpMM partials (apart[prob_blk[q] .. prob_blk[q+1]-1]), formed in every
-// block of the problem; its first block stores it in the problem's aabb.
 struct UFinish {
   double2* w;
   const double2* vk;
-  double* aabb;
+  const double* aabb;
   size_t aabb_stride;
   int ld, k;
   const int* rowblk;
   const int* blk_prob;
   double* partials;
-  const double* apart;
-  const int* prob_blk;
 };
 
 template <int kUnused = 0>
 __global__ __launch_bounds__(kBlock) void k_ufinish(UFinish p) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int r = p.rowblk[b] + (int)threadIdx.x;
-  const int q = p.blk_prob[b];
-  double* aabb = p.aabb + (size_t)q * p.aabb_stride;
-  double2 al;
-  if (p.apart) {
-    const int g0 = p.prob_blk[q];
-    al = block_reduce_partials2(p.apart + 2 * (size_t)g0, p.prob_blk[q + 1] - g0);
-    if (b == g0 && threadIdx.x == 0) {
-      aabb[p.k] = al.x;
-      aabb[p.k + p.ld] = al.y;
-    }
-  } else {
-    al = d2(aabb[p.k], aabb[p.k + p.ld]);
-  }
+  const double* aabb = p.aabb + (size_t)p.blk_prob[b] * p.aabb_stride;
+  const double2 al = d2(aabb[p.k], aabb[p.k + p.ld]);
   double2 nrm = d2(0.0, 0.0);
   if (r < p.rowblk[b + 1]) {
     const double2 v = p.vk[r];
@@ -207,7 +191,7 @@ struct gm_ubatch {
   short* col16 = nullptr;        // 16-bit column offsets from each block's first row
   double *val = nullptr, *mass = nullptr, *aabb = nullptr, *wts1 = nullptr, *wts2 = nullptr,
          *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *yr = nullptr, *yi = nullptr,
-         *partials = nullptr, *partials2 = nullptr;
+         *partials = nullptr;
   double2 *U = nullptr, *Usol = nullptr, *X = nullptr, *W[2] = {nullptr, nullptr},
           *T[2] = {nullptr, nullptr}, *V = nullptr, *Z = nullptr, *C = nullptr, *wD = nullptr,
           *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
@@ -240,7 +224,7 @@ void ufree_all(gm_ubatch* u) {
   if (u->graph) (void)hipGraphExecDestroy(u->graph);
   void* bufs[] = {u->col16, u->rowptr, u->col, u->rowblk, u->blk_prob, u->prob_blk, u->val, u->mass,
                   u->aabb, u->wts1, u->wts2, u->shifts, u->gsr, u->gsi, u->yr, u->yi,
-                  u->partials, u->partials2, u->U, u->Usol, u->X, u->W[0], u->W[1], u->T[0], u->T[1], u->V,
+                  u->partials, u->U, u->Usol, u->X, u->W[0], u->W[1], u->T[0], u->T[1], u->V,
                   u->Z, u->C, u->wD, u->wDU, u->wDU2, u->wX, u->status};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -250,10 +234,9 @@ void ufree_all(gm_ubatch* u) {
 size_t aabb_stride(const gm_ubatch* u) { return 4 * (size_t)(u->m + 1); }
 
 // Per-problem sums of the [G][2] partials into out_q[0], out_q[stride].
-gm_status ureduce(gm_ubatch* u, double* out, int stride, int do_sqrt, int flag_zero,
-                  const double* part = nullptr) {
+gm_status ureduce(gm_ubatch* u, double* out, int stride, int do_sqrt, int flag_zero) {
   gm::ReduceArgs r{};
-  r.partials = part ? part : u->partials;
+  r.partials = u->partials;
   r.G = u->G;
   r.nv = 2;
   r.out = out;
@@ -322,9 +305,6 @@ void launch_uvc(gm_ubatch* u, const gmu::UVC& a) {
 
 // One time step of every problem: build_subspace, lin_sys_solves, apply_V,
 // update_U of the plain path, each stage one launch over all problems.
-// alpha_k and beta_{k+1} of each problem are summed inside the consuming
-// kernels (finish, next SpMM) from that problem's partials, so a Lanczos step
-// is two launches; beta_m, after the last step, by a segmented k_reduce.
 gm_status enqueue_ustep(gm_ubatch* u) {
   const int d = u->nd - 1;
   const size_t N = (size_t)u->N;
@@ -337,10 +317,6 @@ gm_status enqueue_ustep(gm_ubatch* u) {
     a.vk = u->V + (size_t)k * N;
     a.vkm1 = k > 0 ? u->V + (size_t)(k - 1) * N : nullptr;
     a.x = wsrc;
-    if (k > 0) {   // beta_k from the previous finish's partials, per problem
-      a.bpart = u->partials2;
-      a.prob_blk = u->prob_blk;
-    }
     if (d == 0) {
       a.y = wnext;
       U_TRY(uspmm<gm::kOnly>(u, a));
@@ -348,7 +324,6 @@ gm_status enqueue_ustep(gm_ubatch* u) {
       a.y = u->T[0];
       a.gi = u->gamma[1];
       U_TRY(uspmm<gm::kFirst>(u, a));
-      a.bpart = nullptr;
       for (int i = 2; i <= d; ++i) {
         a.x = u->T[(i - 2) & 1];
         a.y = u->T[(i - 1) & 1];
@@ -359,6 +334,7 @@ gm_status enqueue_ustep(gm_ubatch* u) {
       a.y = wnext;
       U_TRY(uspmm<gm::kLast>(u, a));
     }
+    U_TRY(ureduce(u, u->aabb + k, ld, 0, 0));                          // alpha_k
     gmu::UFinish f{};
     f.w = wnext;
     f.vk = a.vk;
@@ -368,13 +344,10 @@ gm_status enqueue_ustep(gm_ubatch* u) {
     f.k = k;
     f.rowblk = u->rowblk;
     f.blk_prob = u->blk_prob;
-    f.partials = u->partials2;
-    f.apart = u->partials;
-    f.prob_blk = u->prob_blk;
+    f.partials = u->partials;
     hipLaunchKernelGGL(gmu::k_ufinish<0>, dim3(u->G), dim3(gm::kBlock), 0, u->stream, f);
     U_HIP(hipGetLastError());
-    if (k == u->m - 1)
-      U_TRY(ureduce(u, u->aabb + (k + 1) + 2 * ld, ld, 1, 1, u->partials2));   // beta_m
+    U_TRY(ureduce(u, u->aabb + (k + 1) + 2 * ld, ld, 1, 1));          // beta_{k+1}
   }
   // shifted solves, every problem's 2 n_s systems (blockIdx.y = problem)
   gm::SolveArgs s{};
@@ -556,7 +529,6 @@ gm_status create_impl(const gm_ubatch_desc* d, gm_ubatch* u) {
   U_TRY(ualloc(&u->wDU2, P * 2 * m * ns));
   U_TRY(ualloc(&u->wX, P * 2 * m * ns));
   U_TRY(ualloc(&u->partials, 2 * (size_t)u->G));
-  U_TRY(ualloc(&u->partials2, 2 * (size_t)u->G));
   U_TRY(ualloc(&u->status, 1));
   U_HIP(hipMemsetAsync(u->status, 0, sizeof(unsigned), st));
   U_TRY(enqueue_update(u, 1));   // U = Usol and beta_0 (lanczos_impl.cu:79-91, K = 0)
