@@ -44,6 +44,7 @@ EXPORTS = (
     "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info",
     "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
     "gm_subspace_size",
+    "gm_eigs_last_error", "gm_eigs_smallest", "gm_sym_eig",
 )
 GM_FAB_RECOMPUTE = 1
 
@@ -174,6 +175,26 @@ class GmUbatchDesc(C.Structure):
     ]
 
 
+class GmEigsDesc(C.Structure):
+    """Mirror of ``gm_eigs_desc``."""
+    _fields_ = [
+        ("n", C.c_int32),
+        ("nnz", C.c_int64),
+        ("sparse_format", C.c_int32),
+        ("A_values", _dp),
+        ("A_ptr", _ip),
+        ("A_ind", _ip),
+        ("num_eigs", C.c_int32),
+        ("block_size", C.c_int32),
+        ("degree", C.c_int32),
+        ("max_passes", C.c_int32),
+        ("tol", C.c_double),
+        ("upper_bound", C.c_double),
+        ("seed", C.c_uint64),
+        ("device", C.c_int32),
+    ]
+
+
 class GpuMatfunError(RuntimeError):
     """An entry point returned a non-OK gm_status."""
 
@@ -257,6 +278,10 @@ def load():
         "gm_eig_weights": (C.c_int, [_dp, C.c_int32, C.c_double, C.c_double, _dp, _dp]),
         "gm_subspace_size": (C.c_int, [_dp, C.c_int32, C.c_double, C.c_double, C.c_double,
                                        C.POINTER(C.c_int32)]),
+        "gm_eigs_last_error": (C.c_char_p, []),
+        "gm_eigs_smallest": (C.c_int, [C.POINTER(GmEigsDesc), _dp, _dp, C.POINTER(C.c_int32),
+                                       _dp]),
+        "gm_sym_eig": (C.c_int, [C.c_int32, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -266,10 +291,12 @@ def load():
     return lib
 
 
-def check(status, batch=False, fab=False, ubatch=False):
+def check(status, batch=False, fab=False, ubatch=False, eigs=False):
     if status != GM_OK:
         lib = load()
-        if ubatch:
+        if eigs:
+            raw = lib.gm_eigs_last_error()
+        elif ubatch:
             raw = lib.gm_ubatch_last_error()
         elif fab:
             raw = lib.gm_fab_last_error()

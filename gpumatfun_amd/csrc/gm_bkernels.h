@@ -15,7 +15,9 @@ using namespace gm;
 constexpr int kMaxND = 17;
 constexpr int kUnroll = 8;   // nonzeros per gather batch in k_bspmm
 
-enum Mode : int { kFirst = 0, kMid = 1, kLast = 2, kOnly = 3, kHorner = 4 };
+// kCheb: y = ca (A x - shift_c x) + cb z (a Chebyshev filter step, or with
+// ca = 1, cb = 0 a residual A x - theta_c x); shift_c = shiftc[c], or cc.
+enum Mode : int { kFirst = 0, kMid = 1, kLast = 2, kOnly = 3, kHorner = 4, kCheb = 5 };
 
 struct BSpmm {
   const int* __restrict__ rowptr;
@@ -32,6 +34,8 @@ struct BSpmm {
   int k;
   double g0, gi;
   int accumulate;
+  double ca, cb, cc;                // kCheb coefficients
+  const double* shiftc;             // kCheb per-column shift (or null: cc)
   double* alpha_out;                // kLast/kOnly: alpha[k][R]
   double* partials;                 // [G][R]
   // kFirst/kOnly with bpart: beta_k[c] = sqrt(sum of the previous finish's bG
@@ -143,6 +147,10 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
         o0[q] = p.z[(size_t)r * p.ldz + c];
         if (p.accumulate) o1[q] = p.y[(size_t)r * p.ldy + c];
       }
+      if (MODE == kCheb) {
+        o0[q] = p.x[(size_t)r * p.ldx + c];
+        if (p.z) o1[q] = p.z[(size_t)r * p.ldz + c];
+      }
     }
     // kUnroll nonzeros at a time: their val/col loads, then all their
     // gathers, are issued before the first FMA waits, so a wavefront has
@@ -196,6 +204,9 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
         if (p.k > 0) yv = fma(-bprev[q], o1[q], yv);
         p.y[iy] = yv;
         dot[q] = fma(yv, v, dot[q]);
+      } else if (MODE == kCheb) {
+        const double sh = p.shiftc ? p.shiftc[c] : p.cc;
+        p.y[iy] = fma(p.cb, o1[q], p.ca * fma(-sh, o0[q], acc[q]));
       } else {
         double yv = acc[q] + o0[q];
         if (p.accumulate) yv += o1[q];

@@ -402,6 +402,45 @@ gm_status gm_eig_weights(const double* lambda, int32_t K, double eta, double alp
 gm_status gm_subspace_size(const double* gamma, int32_t gamma_size, double l1, double lN,
                            double tol, int32_t* m);
 
+/* ---------------------------------------------------------------------------
+ * Deflation eigenpairs on the GPU.  Replaces the host eigensolve that
+ * produces Q and Lambda before lanczos_create -- eigs(K, M, 'sm') in
+ * process_nonuniform.m:106-110 (MATLAB, sparse factorisation), ARPACK
+ * (scipy eigsh) in the Python drivers.  The K smallest eigenpairs of the
+ * symmetric A by Chebyshev-filtered subspace iteration on a block of
+ * block_size columns; converges quickly when lambda_max / lambda_K is
+ * moderate (grid operators), slowly when it is huge (sliver elements).
+ * ------------------------------------------------------------------------- */
+typedef struct gm_eigs_desc {
+    int32_t n;
+    int64_t nnz;
+    int32_t sparse_format;         /* CSR or CSC: equal arrays, A symmetric */
+    const double* A_values;
+    const int32_t* A_ptr;
+    const int32_t* A_ind;
+    int32_t num_eigs;              /* K >= 1                                */
+    int32_t block_size;            /* 16, 32 or 64; > K, <= n               */
+    int32_t degree;                /* Chebyshev filter degree per pass      */
+    int32_t max_passes;            /* filter passes before giving up        */
+    double tol;                    /* max_i ||A q_i - l_i q_i|| / |l_i|     */
+    double upper_bound;            /* >= lambda_max, or <= 0: Gershgorin    */
+    uint64_t seed;                 /* start block (normal deviates)         */
+    int32_t device;
+} gm_eigs_desc;
+
+const char* gm_eigs_last_error(void);
+/* lambda (K, ascending) and, when Q is not NULL, Q (n x K column-major,
+ * orthonormal columns).  passes / max_residual (either may be NULL) report
+ * the filter passes taken and the final max relative residual.  Returns
+ * GM_EBREAKDOWN, with the outputs filled, when tol is not met in
+ * max_passes.                                                             */
+gm_status gm_eigs_smallest(const gm_eigs_desc* desc, double* lambda, double* Q,
+                           int32_t* passes, double* max_residual);
+/* Host helper (no GPU): A = V diag(w) V^T for a symmetric n x n row-major A
+ * by cyclic Jacobi; w ascending, V (may be NULL) row-major with the
+ * eigenvectors as columns.                                                */
+gm_status gm_sym_eig(int32_t n, const double* A, double* w, double* V);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,7 +35,8 @@ def test_library_exports_every_declared_symbol():
                                            ("gm_ms_params", "GmMsParams"),
                                            ("gm_batch_desc", "GmBatchDesc"),
                                            ("gm_fab_desc", "GmFabDesc"),
-                                           ("gm_ubatch_desc", "GmUbatchDesc")])
+                                           ("gm_ubatch_desc", "GmUbatchDesc"),
+                                           ("gm_eigs_desc", "GmEigsDesc")])
 def test_struct_layout_matches_header_field_order(cname, pyname):
     text = open(HEADER).read()
     body = text[text.index("typedef struct " + cname):text.index("} " + cname + ";")]
