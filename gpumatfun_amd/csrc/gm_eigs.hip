@@ -29,7 +29,6 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
-#include <random>
 #include <string>
 #include <vector>
 
@@ -64,6 +63,42 @@ __global__ __launch_bounds__(kBlock) void k_combine(Combine p) {
     for (int k = 0; k < S; ++k) acc = fma(row[k], sW[k * S + c], acc);
     p.out[(size_t)r * S + c] = acc;
   }
+}
+
+// Start block: x[i] uniform in [-1, 1) from a counter-based generator
+// (splitmix64 of seed + i; Steele, Lea & Flood, OOPSLA 2014), the same for
+// a given seed on any device or grid.
+struct Fill {
+  double* x;
+  size_t count;
+  unsigned long long seed;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_fill_uniform(Fill p) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < p.count;
+       i += (size_t)gridDim.x * kBlock) {
+    unsigned long long z = p.seed + 0x9e3779b97f4a7c15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    p.x[i] = (double)(z >> 11) * 0x1.0p-52 - 1.0;
+  }
+}
+
+// q[k][i] = x[i][k] for k < K (the first K columns of the N x S block as a
+// column-major n x K array).
+struct Extract {
+  const double* x;
+  double* q;
+  int S, K, nrows;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_extract(Extract p) {
+  const int k = blockIdx.y;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < p.nrows; i += gridDim.x * kBlock)
+    p.q[(size_t)k * p.nrows + i] = p.x[(size_t)i * p.S + k];
 }
 
 }  // namespace gme
@@ -292,12 +327,10 @@ gm_status run(const gm_eigs_desc* d, Eigs& e, double* lambda, double* Qout, int3
     }
   }
   {
-    std::vector<double> h(N * S);
-    std::mt19937_64 rng(d->seed);
-    std::normal_distribution<double> nd(0.0, 1.0);
-    for (double& v : h) v = nd(rng);
-    E_HIP(hipMemcpyAsync(e.X, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, e.st));
-    E_HIP(hipStreamSynchronize(e.st));
+    gme::Fill f{e.X, N * S, (unsigned long long)d->seed};
+    hipLaunchKernelGGL(gme::k_fill_uniform<0>, dim3(std::min<size_t>(8192, (N * S + 255) / 256)),
+                       dim3(gm::kBlock), 0, e.st, f);
+    E_HIP(hipGetLastError());
   }
   std::vector<double> G, W, w, V, theta(S, 0.0), nrm;
   // src -> dst with orthonormal columns spanning the same space, by scaled
@@ -385,11 +418,12 @@ gm_status run(const gm_eigs_desc* d, Eigs& e, double* lambda, double* Qout, int3
   }
   // the K smallest Ritz pairs; Q column-major n x K
   if (Qout) {
-    std::vector<double> xr(N * S);
-    E_HIP(hipMemcpyAsync(xr.data(), e.X, xr.size() * sizeof(double), hipMemcpyDeviceToHost, e.st));
+    gme::Extract x{e.X, e.Y, S, K, e.n};
+    hipLaunchKernelGGL(gme::k_extract<0>, dim3(std::min(4096, ediv(e.n, gm::kBlock)), K),
+                       dim3(gm::kBlock), 0, e.st, x);
+    E_HIP(hipGetLastError());
+    E_HIP(hipMemcpyAsync(Qout, e.Y, N * K * sizeof(double), hipMemcpyDeviceToHost, e.st));
     E_HIP(hipStreamSynchronize(e.st));
-    for (int k = 0; k < K; ++k)
-      for (size_t i = 0; i < N; ++i) Qout[(size_t)k * N + i] = xr[i * S + k];
   }
   for (int k = 0; k < K; ++k) lambda[k] = theta[k];
   if (passes_out) *passes_out = pass;

@@ -424,7 +424,7 @@ typedef struct gm_eigs_desc {
     int32_t max_passes;            /* filter passes before giving up        */
     double tol;                    /* max_i ||A q_i - l_i q_i|| / |l_i|     */
     double upper_bound;            /* >= lambda_max, or <= 0: Gershgorin    */
-    uint64_t seed;                 /* start block (normal deviates)         */
+    uint64_t seed;                 /* start block: uniform [-1, 1) deviates */
     int32_t device;
 } gm_eigs_desc;
 

@@ -31,7 +31,7 @@ def _check_pairs(A, lam, Q, tol):
 def test_dirichlet_closed_form(block, degree):
     side, k = 64, 7 if block == 16 else 12
     A = problems.laplacian_2d_dirichlet(side)
-    exact = np.sort(problems.dirichlet_eigs_2d(side))[:k]
+    exact = np.sort(problems.dirichlet_eigs_2d(side).ravel())[:k]
     print(f"case dirichlet side={side} k={k} block={block} degree={degree}", flush=True)
     lam, Q, info = eigs_smallest(A, k, block_size=block, degree=degree, tol=1e-10, info=True)
     print(f"passes {info['passes']} residual {info['max_residual']:.2e} "
