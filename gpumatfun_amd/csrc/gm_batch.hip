@@ -79,7 +79,8 @@ struct gm_batch {
   double dt = 0.0;
   bool clamp = true, x_zero = true;
   std::vector<double> gamma;
-  int *rowptr = nullptr, *col = nullptr;
+  int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr;
+  int nrowblk = 0;                  // LDS-staged SpMM row blocks
   double *val = nullptr, *Usol = nullptr, *X = nullptr, *W0 = nullptr, *W1 = nullptr, *T0 = nullptr,
          *T1 = nullptr, *V = nullptr, *Z = nullptr, *C = nullptr, *al = nullptr, *be = nullptr,
          *yr = nullptr, *yi = nullptr, *gsr = nullptr, *gsi = nullptr, *wts1 = nullptr,
@@ -115,7 +116,7 @@ gm_status balloc(T** p, size_t count) {
 
 void bfree_all(gm_batch* b) {
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
-  void* bufs[] = {b->rowptr, b->col, b->val, b->Usol, b->X, b->W0, b->W1, b->T0, b->T1, b->V,
+  void* bufs[] = {b->rowptr, b->col, b->rowblk, b->val, b->Usol, b->X, b->W0, b->W1, b->T0, b->T1, b->V,
                   b->Z, b->C, b->al, b->be, b->yr, b->yi, b->gsr, b->gsi, b->wts1, b->wts2,
                   b->etaxi, b->mass, b->partials, b->stage, b->wD, b->wDU, b->wDU2, b->wX,
                   b->status, b->Q, b->el, b->pl, b->Zd, b->Ud, b->upd, b->ode};
@@ -144,11 +145,11 @@ struct Launch {
   static gm_status spmm(gm_batch* b, int mode, const gmb::BSpmm& a) {
     dim3 g(b->G_sp), t(gm::kBlock);
     switch (mode) {
-      case gmb::kFirst: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kFirst>), g, t, 0, b->stream, a); break;
-      case gmb::kMid: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kMid>), g, t, 0, b->stream, a); break;
-      case gmb::kLast: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kLast>), g, t, 0, b->stream, a); break;
-      case gmb::kOnly: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kOnly>), g, t, 0, b->stream, a); break;
-      default: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kHorner>), g, t, 0, b->stream, a); break;
+      case gmb::kFirst: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kFirst, true>), g, t, 0, b->stream, a); break;
+      case gmb::kMid: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kMid, true>), g, t, 0, b->stream, a); break;
+      case gmb::kLast: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kLast, true>), g, t, 0, b->stream, a); break;
+      case gmb::kOnly: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kOnly, true>), g, t, 0, b->stream, a); break;
+      default: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kHorner, true>), g, t, 0, b->stream, a); break;
     }
     B_HIP(hipGetLastError());
     return GM_OK;
@@ -185,6 +186,8 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
   base.val = b->val;
   base.nrows = b->n;
   base.chunk = b->chunk_sp;
+  base.rowblk = b->rowblk;
+  base.nrowblk = b->nrowblk;
   base.al = b->al;
   base.be = b->be;
   base.alpha_out = b->al;
@@ -565,6 +568,15 @@ gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
       hipMemcpyAsync(b->col, ci.data(), ci.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
     return fail_clean(bfail(GM_EDEVICE, "upload A"));
   B_CREATE(up(&b->val, vv.data(), vv.size()));
+  {
+    std::vector<int> rb;
+    gmb::bspmm_row_blocks(rp.data(), b->n, rb);
+    b->nrowblk = (int)rb.size() - 1;
+    B_CREATE(balloc(&b->rowblk, rb.size()));
+    if (hipMemcpyAsync(b->rowblk, rb.data(), rb.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail_clean(bfail(GM_EDEVICE, "upload row blocks"));
+  }
   // U (n x 2P column-major) -> row-major Usol with dt on the g columns
   std::vector<double> u((size_t)N * b->R);
   for (int c = 0; c < b->R; ++c)

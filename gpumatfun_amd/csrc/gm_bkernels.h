@@ -7,6 +7,9 @@
 // on one row at a time, with R < 64 on 64/R rows.  All kernels are templates
 // so that several translation units may instantiate them.
 #pragma once
+#include <type_traits>
+#include <vector>
+
 #include "gm_device.h"
 
 namespace gmb {
@@ -14,6 +17,11 @@ using namespace gm;
 
 constexpr int kMaxND = 17;
 constexpr int kUnroll = 8;   // nonzeros per gather batch in k_bspmm
+#ifndef GMB_STAGED_UNROLL
+#define GMB_STAGED_UNROLL 12
+#endif
+// nonzeros per gather batch when the matrix is staged in LDS
+constexpr int kUnrollStaged = GMB_STAGED_UNROLL;
 
 // kCheb: y = ca (A x - shift_c x) + cb z (a Chebyshev filter step, or with
 // ca = 1, cb = 0 a residual A x - theta_c x); shift_c = shiftc[c], or cc.
@@ -24,6 +32,8 @@ struct BSpmm {
   const int* __restrict__ col;
   const double* __restrict__ val;
   int nrows, chunk;
+  const int* rowblk;                // staged form: row-block starts (+ n)
+  int nrowblk;                      // staged form: number of row blocks
   const double* x;  size_t ldx;     // gather source (+ column offset applied by caller)
   double* y;        size_t ldy;     // output
   double* vk;                       // V[k] (ld = R)
@@ -91,18 +101,50 @@ __device__ void block_partial_cols(const double (&v)[(R < 64 ? 1 : R / 64)], dou
   }
 }
 
-template <int R, int MODE>
+// Row blocks for the staged form: consecutive rows with at most kBTile
+// nonzeros and kBRows rows per block; a longer row is a block of its own
+// (read from global memory).  rb gets the block starts and n at the end.
+constexpr int kBTile = 1024;
+constexpr int kBRows = 256;
+
+inline void bspmm_row_blocks(const int* rowptr, int n, std::vector<int>& rb) {
+  rb.clear();
+  int r = 0;
+  while (r < n) {
+    rb.push_back(r);
+    const int j0 = rowptr[r];
+    int e = r + 1;
+    while (e < n && e - r < kBRows && rowptr[e + 1] - j0 <= kBTile) ++e;
+    r = e;
+  }
+  rb.push_back(n);
+}
+
+// kStaged: the grid's blocks take contiguous runs of the nrowblk row blocks
+// (rows rowblk[i] .. rowblk[i+1]-1); each row block's values, column indices
+// and row pointers are first staged in LDS by coalesced loads (one HBM
+// latency per row block instead of one per row).  Otherwise block b covers
+// p.chunk rows read straight from global memory.  Partials are per grid
+// block either way.
+template <int R, int MODE, bool kStaged = false>
 __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
   constexpr int CPW = R < 64 ? R : 64;
   constexpr int RPW = 64 / CPW;
   constexpr int CL = R / CPW;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int row0 = b * p.chunk;
-  const int row1 = min(row0 + p.chunk, p.nrows);
+  int it0 = 0, it1 = 1;
+  if (kStaged) {
+    const int per = (p.nrowblk + gridDim.x - 1) / gridDim.x;
+    it0 = min(b * per, p.nrowblk);
+    it1 = min(it0 + per, p.nrowblk);
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int slot = lane / CPW, c0 = lane % CPW;
   double bet[CL], inv[CL], bprev[CL], dot[CL];
   __shared__ double fused[(MODE == kFirst || MODE == kOnly) ? R : 1];
+  __shared__ double sval[kStaged ? kBTile : 1];
+  __shared__ int scol[kStaged ? kBTile : 1];
+  __shared__ int srp[kStaged ? kBRows + 1 : 1];
   if ((MODE == kFirst || MODE == kOnly) && p.bpart) {
     block_reduce_cols(p.bpart, p.bG, R, fused);
     for (int c = threadIdx.x; c < R; c += kBlock) {
@@ -126,92 +168,129 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
                    ? ((MODE == kOnly && p.bpart) ? bet[q] : p.be[(size_t)p.k * R + c]) : 0.0;
     dot[q] = 0.0;
   }
-  for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
-    const int j0 = p.rowptr[r], j1 = p.rowptr[r + 1];
-    double acc[CL];
-#pragma unroll
-    for (int q = 0; q < CL; ++q) acc[q] = 0.0;
-    // the epilogue's own-row operands, loaded first so that their latency
-    // overlaps the gathers
-    double o0[CL], o1[CL];
-#pragma unroll
-    for (int q = 0; q < CL; ++q) {
-      const int c = c0 + q * CPW;
-      const size_t iv = (size_t)r * R + c;
-      o0[q] = 0.0;
-      o1[q] = 0.0;
-      if (MODE == kFirst || MODE == kOnly) o0[q] = p.x[(size_t)r * p.ldx + c];
-      if (MODE == kMid || MODE == kLast) o0[q] = p.vk[iv];
-      if ((MODE == kLast || MODE == kOnly) && p.k > 0) o1[q] = p.vkm1[iv];
-      if (MODE == kHorner) {
-        o0[q] = p.z[(size_t)r * p.ldz + c];
-        if (p.accumulate) o1[q] = p.y[(size_t)r * p.ldy + c];
-      }
-      if (MODE == kCheb) {
-        o0[q] = p.x[(size_t)r * p.ldx + c];
-        if (p.z) o1[q] = p.z[(size_t)r * p.ldz + c];
-      }
+  for (int it = it0; it < it1; ++it) {
+    int row0, row1;
+    if (kStaged) {
+      row0 = p.rowblk[it];
+      row1 = p.rowblk[it + 1];
+    } else {
+      row0 = b * p.chunk;
+      row1 = min(row0 + p.chunk, p.nrows);
     }
-    // kUnroll nonzeros at a time: their val/col loads, then all their
-    // gathers, are issued before the first FMA waits, so a wavefront has
-    // kUnroll * CL gathers in flight instead of one dependent chain per
-    // nonzero.  The FMAs still run in column order.
-    for (int jb = j0; jb < j1; jb += kUnroll) {
-      double a[kUnroll];
-      int ci[kUnroll];
-#pragma unroll
-      for (int t = 0; t < kUnroll; ++t) {
-        const bool ok = jb + t < j1;
-        a[t] = ok ? p.val[jb + t] : 0.0;
-        ci[t] = ok ? p.col[jb + t] : -1;
+    bool staged = false;
+    if (kStaged) {
+      const int jbase = p.rowptr[row0];
+      const int nz = p.rowptr[row1] - jbase;
+      staged = nz <= kBTile;   // uniform over the block
+      __syncthreads();         // the previous row block's LDS reads are done
+      if (staged) {
+        for (int t = threadIdx.x; t < nz; t += kBlock) {
+          sval[t] = p.val[jbase + t];
+          scol[t] = p.col[jbase + t];
+        }
+        for (int t = threadIdx.x; t <= row1 - row0; t += kBlock)
+          srp[t] = p.rowptr[row0 + t] - jbase;
       }
-      double xv[kUnroll][CL];
-#pragma unroll
-      for (int t = 0; t < kUnroll; ++t) {
-        const double* xr = p.x + (size_t)(ci[t] < 0 ? r : ci[t]) * p.ldx + c0;
-#pragma unroll
-        for (int q = 0; q < CL; ++q) xv[t][q] = ci[t] < 0 ? 0.0 : xr[q * CPW];
-      }
-#pragma unroll
-      for (int t = 0; t < kUnroll; ++t) {
-        if (ci[t] >= 0) {
-#pragma unroll
-          for (int q = 0; q < CL; ++q) acc[q] = fma(a[t], xv[t][q], acc[q]);
+      __syncthreads();
+    }
+    // the row loop, with the matrix read from LDS (from_lds) or global memory
+    auto rows = [&](auto from_lds) {
+      constexpr bool L = decltype(from_lds)::value;
+      for (int r = row0 + wave * RPW + slot; r < row1; r += kWaves * RPW) {
+        const int j0 = L ? srp[r - row0] : p.rowptr[r];
+        const int j1 = L ? srp[r - row0 + 1] : p.rowptr[r + 1];
+        double acc[CL];
+  #pragma unroll
+        for (int q = 0; q < CL; ++q) acc[q] = 0.0;
+        // the epilogue's own-row operands, loaded first so that their latency
+        // overlaps the gathers
+        double o0[CL], o1[CL];
+  #pragma unroll
+        for (int q = 0; q < CL; ++q) {
+          const int c = c0 + q * CPW;
+          const size_t iv = (size_t)r * R + c;
+          o0[q] = 0.0;
+          o1[q] = 0.0;
+          if (MODE == kFirst || MODE == kOnly) o0[q] = p.x[(size_t)r * p.ldx + c];
+          if (MODE == kMid || MODE == kLast) o0[q] = p.vk[iv];
+          if ((MODE == kLast || MODE == kOnly) && p.k > 0) o1[q] = p.vkm1[iv];
+          if (MODE == kHorner) {
+            o0[q] = p.z[(size_t)r * p.ldz + c];
+            if (p.accumulate) o1[q] = p.y[(size_t)r * p.ldy + c];
+          }
+          if (MODE == kCheb) {
+            o0[q] = p.x[(size_t)r * p.ldx + c];
+            if (p.z) o1[q] = p.z[(size_t)r * p.ldz + c];
+          }
+        }
+        // kUnroll nonzeros at a time: their val/col loads, then all their
+        // gathers, are issued before the first FMA waits, so a wavefront has
+        // kUnroll * CL gathers in flight instead of one dependent chain per
+        // nonzero.  The FMAs still run in column order.
+        constexpr int U = L ? kUnrollStaged : kUnroll;
+        for (int jb = j0; jb < j1; jb += U) {
+          double a[U];
+          int ci[U];
+  #pragma unroll
+          for (int t = 0; t < U; ++t) {
+            const bool ok = jb + t < j1;
+            a[t] = ok ? (L ? sval[jb + t] : p.val[jb + t]) : 0.0;
+            ci[t] = ok ? (L ? scol[jb + t] : p.col[jb + t]) : -1;
+          }
+          double xv[U][CL];
+  #pragma unroll
+          for (int t = 0; t < U; ++t) {
+            const double* xr = p.x + (size_t)(ci[t] < 0 ? r : ci[t]) * p.ldx + c0;
+  #pragma unroll
+            for (int q = 0; q < CL; ++q) xv[t][q] = ci[t] < 0 ? 0.0 : xr[q * CPW];
+          }
+  #pragma unroll
+          for (int t = 0; t < U; ++t) {
+            if (ci[t] >= 0) {
+  #pragma unroll
+              for (int q = 0; q < CL; ++q) acc[q] = fma(a[t], xv[t][q], acc[q]);
+            }
+          }
+        }
+  #pragma unroll
+        for (int q = 0; q < CL; ++q) {
+          const int c = c0 + q * CPW;
+          const size_t iy = (size_t)r * p.ldy + c;
+          const size_t iv = (size_t)r * R + c;
+          if (MODE == kFirst) {
+            const double v = safe_div(o0[q], bet[q]);
+            p.vk[iv] = v;
+            p.y[iy] = fma(p.gi, v, acc[q] * inv[q]);
+          } else if (MODE == kMid) {
+            p.y[iy] = fma(p.gi, o0[q], acc[q]);
+          } else if (MODE == kLast || MODE == kOnly) {
+            double v, yv;
+            if (MODE == kOnly) {
+              v = safe_div(o0[q], bet[q]);
+              p.vk[iv] = v;
+              yv = acc[q] * inv[q];
+            } else {
+              v = o0[q];
+              yv = acc[q];
+            }
+            if (p.k > 0) yv = fma(-bprev[q], o1[q], yv);
+            p.y[iy] = yv;
+            dot[q] = fma(yv, v, dot[q]);
+          } else if (MODE == kCheb) {
+            const double sh = p.shiftc ? p.shiftc[c] : p.cc;
+            p.y[iy] = fma(p.cb, o1[q], p.ca * fma(-sh, o0[q], acc[q]));
+          } else {
+            double yv = acc[q] + o0[q];
+            if (p.accumulate) yv += o1[q];
+            p.y[iy] = yv;
+          }
         }
       }
-    }
-#pragma unroll
-    for (int q = 0; q < CL; ++q) {
-      const int c = c0 + q * CPW;
-      const size_t iy = (size_t)r * p.ldy + c;
-      const size_t iv = (size_t)r * R + c;
-      if (MODE == kFirst) {
-        const double v = safe_div(o0[q], bet[q]);
-        p.vk[iv] = v;
-        p.y[iy] = fma(p.gi, v, acc[q] * inv[q]);
-      } else if (MODE == kMid) {
-        p.y[iy] = fma(p.gi, o0[q], acc[q]);
-      } else if (MODE == kLast || MODE == kOnly) {
-        double v, yv;
-        if (MODE == kOnly) {
-          v = safe_div(o0[q], bet[q]);
-          p.vk[iv] = v;
-          yv = acc[q] * inv[q];
-        } else {
-          v = o0[q];
-          yv = acc[q];
-        }
-        if (p.k > 0) yv = fma(-bprev[q], o1[q], yv);
-        p.y[iy] = yv;
-        dot[q] = fma(yv, v, dot[q]);
-      } else if (MODE == kCheb) {
-        const double sh = p.shiftc ? p.shiftc[c] : p.cc;
-        p.y[iy] = fma(p.cb, o1[q], p.ca * fma(-sh, o0[q], acc[q]));
-      } else {
-        double yv = acc[q] + o0[q];
-        if (p.accumulate) yv += o1[q];
-        p.y[iy] = yv;
-      }
+    };
+    if (kStaged && staged) {
+      rows(std::true_type{});
+    } else {
+      rows(std::false_type{});
     }
   }
   if (MODE == kLast || MODE == kOnly)

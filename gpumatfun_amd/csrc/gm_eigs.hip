@@ -10,7 +10,8 @@
 //   1. Y = p(A) X with p the degree-d Chebyshev polynomial of the first kind
 //      on [a, b] (damping the spectrum above a, the largest Ritz value of the
 //      block, up to the upper bound b; amplifying below a), by the scaled
-//      three-term recurrence, each term one batched SpMM (gmb::k_bspmm kCheb);
+//      three-term recurrence, each term one batched SpMM (gmb::k_bspmm kCheb,
+//      row blocks staged in LDS);
 //   2. orthonormalise: G = Y^T Y on FP64 MFMA, G = V diag(g) V^T on the
 //      host, X = Y V diag(g)^(-1/2);
 //   3. Rayleigh-Ritz: H = X^T (A X) on FP64 MFMA, H = W diag(theta) W^T,
@@ -188,12 +189,12 @@ void sym_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<d
 // roles rotate between stages (the pointers are permuted, never the data).
 struct Eigs {
   hipStream_t st = nullptr;
-  int n = 0, s = 0, G_sp = 1, chunk_sp = 1, G_df = 1, chunk_df = 1;
-  int *rowptr = nullptr, *col = nullptr;
+  int n = 0, s = 0, G_sp = 1, chunk_sp = 1, G_df = 1, chunk_df = 1, G_rb = 1;
+  int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr;
   double *val = nullptr, *X = nullptr, *Y = nullptr, *Z = nullptr, *AX = nullptr,
          *partials = nullptr, *small = nullptr, *wmat = nullptr, *shift = nullptr;
   ~Eigs() {
-    void* bufs[] = {rowptr, col, val, X, Y, Z, AX, partials, small, wmat, shift};
+    void* bufs[] = {rowptr, col, rowblk, val, X, Y, Z, AX, partials, small, wmat, shift};
     for (void* p : bufs)
       if (p) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
@@ -269,6 +270,8 @@ gm_status cheb_step(Eigs& e, const double* x, const double* z, double* y, double
   a.val = e.val;
   a.nrows = e.n;
   a.chunk = e.chunk_sp;
+  a.rowblk = e.rowblk;
+  a.nrowblk = e.G_rb;
   a.x = x;
   a.ldx = S;
   a.y = y;
@@ -280,7 +283,8 @@ gm_status cheb_step(Eigs& e, const double* x, const double* z, double* y, double
   a.cb = cb;
   a.cc = cc;
   a.shiftc = shiftc;
-  hipLaunchKernelGGL((gmb::k_bspmm<S, gmb::kCheb>), dim3(e.G_sp), dim3(gm::kBlock), 0, e.st, a);
+  hipLaunchKernelGGL((gmb::k_bspmm<S, gmb::kCheb, true>), dim3(std::min(e.G_rb, 8192)),
+                     dim3(gm::kBlock), 0, e.st, a);
   E_HIP(hipGetLastError());
   return GM_OK;
 }
@@ -494,6 +498,14 @@ gm_status gm_eigs_smallest(const gm_eigs_desc* d, double* lambda, double* Q, int
   E_HIP(hipMemcpyAsync(e.col, d->A_ind, (size_t)d->nnz * sizeof(int), hipMemcpyHostToDevice, e.st));
   E_HIP(hipMemcpyAsync(e.val, d->A_values, (size_t)d->nnz * sizeof(double), hipMemcpyHostToDevice,
                        e.st));
+  {
+    std::vector<int> rb;
+    gmb::bspmm_row_blocks(d->A_ptr, e.n, rb);
+    e.G_rb = (int)rb.size() - 1;
+    E_TRY(ealloc(&e.rowblk, rb.size()));
+    E_HIP(hipMemcpyAsync(e.rowblk, rb.data(), rb.size() * sizeof(int), hipMemcpyHostToDevice, e.st));
+    E_HIP(hipStreamSynchronize(e.st));   // rb is freed at the end of the scope
+  }
   E_TRY(ealloc(&e.X, N * s));
   E_TRY(ealloc(&e.Y, N * s));
   E_TRY(ealloc(&e.Z, N * s));
