@@ -15,6 +15,9 @@
 //   9 get_xsol_mex            handle -> N x 2             (get_xsol.cpp)
 //  10 get_aabb_mex            handle -> (m+1) x 4         (called at lanczosEngine.m:26)
 //  11 set_y_mex               handle, yreal, yimag        (called at lanczosEngine.m:30)
+//  12 eigs_smallest_mex       A, k [, opts] -> Q, lambda  (no reference MEX: replaces
+//                             [Q, lambda] = eigs(K, M, k, 'sm') and Q = M^(1/2) Q of
+//                             process_nonuniform.m:106-110, on A = M^(-1/2) K M^(-1/2))
 //
 // Argument errors use the reference's message identifiers; a non-OK status
 // of the library becomes mexErrMsgIdAndTxt("gpumatfun:<status>", message).
@@ -28,7 +31,7 @@
 #include "gpumatfun.h"
 
 #ifndef GM_GATEWAY
-#error "compile with -DGM_GATEWAY=<1..11>"
+#error "compile with -DGM_GATEWAY=<1..12>"
 #endif
 
 namespace {
@@ -46,10 +49,10 @@ const char* status_name(gm_status s) {
   return "GM_UNKNOWN";
 }
 
-void check(gm_status s) {
+void check(gm_status s, const char* (*last_error)() = gm_last_error) {
   if (s != GM_OK) {
     std::string id = std::string("gpumatfun:") + status_name(s);
-    mexErrMsgIdAndTxt(id.c_str(), "%s", gm_last_error());
+    mexErrMsgIdAndTxt(id.c_str(), "%s", last_error());
   }
 }
 
@@ -255,6 +258,60 @@ extern "C" void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
         mxGetN(prhs[i]) != 2 * static_cast<size_t>(ns))
       mexErrMsgIdAndTxt("MATLAB:set_y:badY", "y must be m x 2*contour_size real.");
   check(gm_set_y(s, mxGetPr(prhs[1]), mxGetPr(prhs[2])));
+#elif GM_GATEWAY == 12
+  // [Q, lambda] = eigs_smallest_mex(A, k [, [block degree max_passes tol]])
+  if (nrhs < 2 || nrhs > 3)
+    mexErrMsgIdAndTxt("MATLAB:eigs_smallest:invalidNumInputs",
+                      "Two or three input arguments are required.");
+  if (nlhs < 1 || nlhs > 2)
+    mexErrMsgIdAndTxt("MATLAB:eigs_smallest:maxlhs", "One or two output arguments are required.");
+  const mxArray* A = prhs[0];
+  if (!mxIsSparse(A) || !mxIsDouble(A))
+    mexErrMsgIdAndTxt("MATLAB:eigs_smallest:inputNotSparse", "A must be a sparse double matrix.");
+  const size_t n = mxGetM(A);
+  if (mxGetN(A) != n || n > 0x7fffffff)
+    mexErrMsgIdAndTxt("MATLAB:eigs_smallest:notSquare", "A must be square.");
+  if (!mxIsDouble(prhs[1]) || mxGetM(prhs[1]) * mxGetN(prhs[1]) != 1)
+    mexErrMsgIdAndTxt("MATLAB:eigs_smallest:bad_k", "k must be a scalar.");
+  const int k = static_cast<int>(mxGetPr(prhs[1])[0]);
+  if (k < 1 || static_cast<double>(k) != mxGetPr(prhs[1])[0])
+    mexErrMsgIdAndTxt("MATLAB:eigs_smallest:bad_k", "k must be a positive integer.");
+  double opt[4] = {0.0, 40.0, 200.0, 1e-10};   // block (0: smallest of 16/32/64 >= 2k)
+  if (nrhs == 3) {
+    if (!mxIsDouble(prhs[2]) || mxGetM(prhs[2]) * mxGetN(prhs[2]) != 4)
+      mexErrMsgIdAndTxt("MATLAB:eigs_smallest:bad_opts",
+                        "opts must be [block degree max_passes tol].");
+    std::memcpy(opt, mxGetPr(prhs[2]), sizeof(opt));
+  }
+  int block = static_cast<int>(opt[0]);
+  if (block == 0) block = 2 * k <= 16 ? 16 : 2 * k <= 32 ? 32 : 64;
+  // CSC of a symmetric matrix: the same arrays as its CSR
+  const mwIndex* jc = mxGetJc(A);
+  const mwIndex* ir = mxGetIr(A);
+  const size_t nnz = jc[n];
+  std::vector<int32_t> ptr(n + 1), ind(nnz);
+  for (size_t i = 0; i <= n; ++i) ptr[i] = static_cast<int32_t>(jc[i]);
+  for (size_t j = 0; j < nnz; ++j) ind[j] = static_cast<int32_t>(ir[j]);
+  gm_eigs_desc d{};
+  d.n = static_cast<int32_t>(n);
+  d.nnz = static_cast<int64_t>(nnz);
+  d.sparse_format = GM_SPARSE_CSC;
+  d.A_values = mxGetPr(A);
+  d.A_ptr = ptr.data();
+  d.A_ind = ind.data();
+  d.num_eigs = k;
+  d.block_size = block;
+  d.degree = static_cast<int32_t>(opt[1]);
+  d.max_passes = static_cast<int32_t>(opt[2]);
+  d.tol = opt[3];
+  d.upper_bound = 0.0;
+  d.seed = 0;
+  d.device = 0;
+  mxArray* Q = mxCreateDoubleMatrix(static_cast<mwSize>(n), static_cast<mwSize>(k), mxREAL);
+  mxArray* lam = mxCreateDoubleMatrix(static_cast<mwSize>(k), 1, mxREAL);
+  check(gm_eigs_smallest(&d, mxGetPr(lam), mxGetPr(Q), nullptr, nullptr), gm_eigs_last_error);
+  plhs[0] = Q;
+  if (nlhs == 2) plhs[1] = lam;
 #else
 #error "unknown GM_GATEWAY"
 #endif

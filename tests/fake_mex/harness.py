@@ -23,7 +23,7 @@ GATEWAYS = {
     "lanczos_create": 1, "lanczos_delete": 2, "build_subspace_mex": 3,
     "lin_sys_solves_mex": 4, "apply_V_mex": 5, "update_U_mex": 6,
     "update_U_monodomain_mex": 7, "get_U_mex": 8, "get_xsol_mex": 9,
-    "get_aabb_mex": 10, "set_y_mex": 11,
+    "get_aabb_mex": 10, "set_y_mex": 11, "eigs_smallest_mex": 12,
 }
 
 

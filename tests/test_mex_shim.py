@@ -123,3 +123,33 @@ def test_mex_step_sequence_matches_engine_and_oracle(mex):
         mex("get_U_mex", h)
     assert e.value.ident == "gpumatfun:GM_EBADHANDLE"
     eng.delete()
+
+
+def test_eigs_gateway_argument_errors(mex):
+    from gpumatfun_amd import problems
+    A = problems.laplacian_2d_dirichlet(8)
+    cases = [((A.toarray(), 4.0), "MATLAB:eigs_smallest:inputNotSparse"),
+             ((A, np.array([1.0, 2.0])), "MATLAB:eigs_smallest:bad_k"),
+             ((A, 2.5), "MATLAB:eigs_smallest:bad_k"),
+             ((A, 4.0, np.array([16.0, 40.0])), "MATLAB:eigs_smallest:bad_opts"),
+             ((A,), "MATLAB:eigs_smallest:invalidNumInputs"),
+             ((A, 4.0, np.array([24.0, 40.0, 10.0, 1e-8])), "gpumatfun:GM_EINVAL")]
+    for args, ident in cases:
+        with pytest.raises(harness.MexError) as e:
+            mex("eigs_smallest_mex", *args, nlhs=2)
+        assert e.value.ident == ident, (args, e.value)
+
+
+@pytest.mark.gpu
+def test_eigs_gateway_matches_library(mex):
+    """[Q, lambda] = eigs_smallest_mex(A, k): the 2D Dirichlet closed form, and
+    the same pairs as the Python wrapper (same seed, same arithmetic)."""
+    from gpumatfun_amd import problems
+    from gpumatfun_amd.eigs import eigs_smallest
+    A = problems.laplacian_2d_dirichlet(40)
+    Q, lam = mex("eigs_smallest_mex", A.tocsc(), 6.0, nlhs=2)
+    exact = np.sort(problems.dirichlet_eigs_2d(40).ravel())[:6]
+    assert Q.shape == (1600, 6) and lam.shape == (6, 1)
+    assert np.abs(lam[:, 0] - exact).max() <= 1e-10 * exact[0]
+    lam2, Q2 = eigs_smallest(A, 6, block_size=16, degree=40, max_passes=200, tol=1e-10)
+    assert np.array_equal(lam[:, 0], lam2) and np.array_equal(Q, Q2)
