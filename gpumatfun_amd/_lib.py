@@ -145,6 +145,11 @@ class GmFabDesc(C.Structure):
         ("dwts", _dp),
         ("flags", C.c_uint32),
         ("device", C.c_int32),
+        ("num_eigs", C.c_int32),
+        ("Q", _dp),
+        ("lambda_", _dp),
+        ("f_lambda", _dp),
+        ("f_divdiff", _dp),
     ]
 
 

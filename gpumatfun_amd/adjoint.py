@@ -40,7 +40,11 @@ def contour_weights(xi, dxidt, s, tau, alpha):
 class FabEngine:
     """Device-resident u = f(A) b with its adjoint (one matrix, one right-hand side)."""
 
-    def __init__(self, A, *, m, xi, wts, dwts=None, recompute=False, device=0):
+    def __init__(self, A, *, m, xi, wts, dwts=None, recompute=False, device=0, Q=None, lam=None,
+                 f_lambda=None, f_divdiff=None):
+        """``Q`` (n x K), ``lam``, ``f_lambda`` = f(lam) deflate the forward and
+        backward solves; ``f_divdiff`` (K x K divided differences of f at lam)
+        is needed for ``backward_param`` with deflation."""
         lib = _lib.load()
         self._lib = lib
         self._h = C.c_void_p()
@@ -68,6 +72,17 @@ class FabEngine:
         d.dwts = _p(dw_i)
         d.flags = _lib.GM_FAB_RECOMPUTE if recompute else 0
         d.device = int(device)
+        if Q is not None and np.shape(Q)[1] > 0:
+            self._Q = np.asfortranarray(Q, dtype=np.float64)
+            self._lam = np.ascontiguousarray(lam, dtype=np.float64)
+            self._flam = np.ascontiguousarray(f_lambda, dtype=np.float64)
+            self._fdd = (None if f_divdiff is None
+                         else np.asfortranarray(f_divdiff, dtype=np.float64))
+            d.num_eigs = self._Q.shape[1]
+            d.Q = _p(self._Q)
+            d.lambda_ = _p(self._lam)
+            d.f_lambda = _p(self._flam)
+            d.f_divdiff = _p(self._fdd)
         _lib.check(lib.gm_fab_create(C.byref(d), C.byref(self._h)), fab=True)
         self.ns = int(np.size(xi))
 

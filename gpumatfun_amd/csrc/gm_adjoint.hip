@@ -204,6 +204,85 @@ __global__ __launch_bounds__(256) void k_param_grad(PGrad p) {
   block_partial2_at(d2(acc, 0.0), p.partials, blockIdx.x);
 }
 
+// Block partials of the dot products C[a][b] = sum_i P[a][i] R[b][i] over
+// the block's rows, for na P-vectors (this block's 64 of them: blockIdx.y)
+// and nb <= 64 R-vectors, each contiguous over N (vector stride ldp / ldr).
+// Tiles of 32 rows of both sets are staged in LDS; thread t accumulates
+// P-vector t % 64 against R-vectors t / 64, t / 64 + 4, ...
+// partials[G][na * nb], index a * nb + b.
+struct Dots {
+  const double* P;
+  size_t ldp;
+  int na;
+  const double* R;
+  size_t ldr;
+  int nb;
+  int N, chunk;
+  double* partials;
+};
+
+constexpr int kDotTile = 32;
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_dots(Dots p) {
+  __shared__ double sp[64 * (kDotTile + 1)];
+  __shared__ double sr[64 * (kDotTile + 1)];
+  const int r0 = blockIdx.x * p.chunk, r1 = min(r0 + p.chunk, p.N);
+  const int a0 = blockIdx.y * 64;
+  const int na_here = min(64, p.na - a0);
+  const int ta = threadIdx.x & 63, tb = threadIdx.x >> 6;
+  double acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+  for (int t0 = r0; t0 < r1; t0 += kDotTile) {
+    const int nt = min(kDotTile, r1 - t0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * kDotTile; e += 256) {
+      const int v = e / kDotTile, i = e % kDotTile;
+      sp[v * (kDotTile + 1) + i] =
+          (v < na_here && i < nt) ? p.P[(size_t)(a0 + v) * p.ldp + t0 + i] : 0.0;
+      sr[v * (kDotTile + 1) + i] = (v < p.nb && i < nt) ? p.R[(size_t)v * p.ldr + t0 + i] : 0.0;
+    }
+    __syncthreads();
+    for (int i = 0; i < nt; ++i) {
+      const double pv = sp[ta * (kDotTile + 1) + i];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] = fma(pv, sr[(tb + 4 * q) * (kDotTile + 1) + i], acc[q]);
+    }
+  }
+  if (ta < na_here)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int b = tb + 4 * q;
+      if (b < p.nb)
+        p.partials[(size_t)blockIdx.x * p.na * p.nb + (size_t)(a0 + ta) * p.nb + b] = acc[q];
+    }
+}
+
+// y[i] = x[i] + sgn sum_k Q[k][i] a[k] (scale[k] when given): the deflation
+// projection (sgn = -1) and the eigen part of f(A) b (scale = f(lambda)).
+struct DeflCM {
+  const double* Q;
+  int K, N;
+  const double* a;
+  const double* scale;
+  double sgn;
+  const double* x;
+  double* y;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_defl_cm(DeflCM p) {
+  __shared__ double co[64];
+  if ((int)threadIdx.x < p.K) co[threadIdx.x] = p.sgn * p.a[threadIdx.x] * (p.scale ? p.scale[threadIdx.x] : 1.0);
+  __syncthreads();
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < p.N; i += gridDim.x * 256) {
+    double acc = p.x[i];
+    for (int k = 0; k < p.K; ++k) acc = fma(p.Q[(size_t)k * p.N + i], co[k], acc);
+    p.y[i] = acc;
+  }
+}
+
 }  // namespace gma
 
 namespace {
@@ -240,7 +319,8 @@ inline int adiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 // One Krylov side (b or g): basis, tridiagonal data, ytilde.
 struct Side {
   double *U = nullptr, *V = nullptr, *W0 = nullptr, *W1 = nullptr, *al = nullptr, *be = nullptr,
-         *yr = nullptr, *yi = nullptr, *Yg = nullptr, *Xc = nullptr, *out = nullptr;
+         *yr = nullptr, *yi = nullptr, *Yg = nullptr, *Xc = nullptr, *out = nullptr,
+         *C = nullptr, *a = nullptr;   // V-coefficients of f(A) rhs; Q^T rhs (deflation)
 };
 
 }  // namespace
@@ -262,6 +342,10 @@ struct gm_fab {
   double* partials2 = nullptr;  // finish partials when the reductions are fused
   std::vector<double> dc;   // ns complex dc/dalpha (host)
   Side sb, sg;
+  // deflation: Q (N x K column-major), f(lambda); E Q, dot partials and sums
+  int K = 0, G_dot = 1, chunk_dot = 1;
+  double *Q = nullptr, *flam = nullptr, *EQ = nullptr, *dpart = nullptr, *dsum = nullptr;
+  std::vector<double> lam, flam_h, fdd;   // host copies (fdd K x K column-major)
 };
 
 namespace {
@@ -285,10 +369,11 @@ void afree(void* p) {
 void free_fab(gm_fab* f) {
   void* bufs[] = {f->rowptr, f->col, f->val, f->xi, f->ones, f->c, f->gsr, f->gsi, f->C,
                   f->partials, f->kappa, f->grad, f->gxv, f->wD, f->wDU, f->wDU2, f->wX,
-                  f->status, f->dA, f->partials2};
+                  f->status, f->dA, f->partials2, f->Q, f->flam, f->EQ, f->dpart, f->dsum};
   for (void* p : bufs) afree(p);
   for (Side* s : {&f->sb, &f->sg}) {
-    void* sbufs[] = {s->U, s->V, s->W0, s->W1, s->al, s->be, s->yr, s->yi, s->Yg, s->Xc, s->out};
+    void* sbufs[] = {s->U, s->V, s->W0, s->W1, s->al, s->be, s->yr, s->yi, s->Yg, s->Xc, s->out,
+                     s->C, s->a};
     for (void* p : sbufs) afree(p);
   }
   if (f->stream) (void)hipStreamDestroy(f->stream);
@@ -404,12 +489,12 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
     cc.m = f->m;
     cc.ns = f->ns;
     cc.nd = 1;
-    cc.C = f->C;
+    cc.C = s.C;
     hipLaunchKernelGGL(gmb::k_bcollapse<0>, dim3(adiv(f->m, 256)), dim3(256), 0, f->stream, cc);
     A_HIP(hipGetLastError());
     gmb::BVC v{};
     v.V = s.V;
-    v.C = f->C;
+    v.C = s.C;
     v.R = 1;
     v.m = f->m;
     v.nd = 1;
@@ -448,7 +533,93 @@ gm_status alloc_side(gm_fab* f, Side& s) {
   A_TRY(aalloc(&s.Yg, (size_t)f->m * f->ncol));
   A_TRY(aalloc(&s.Xc, N * f->ncol));
   A_TRY(aalloc(&s.out, N));
+  A_TRY(aalloc(&s.C, (size_t)f->m));
+  A_TRY(aalloc(&s.a, (size_t)std::max(f->K, 1)));
   return GM_OK;
+}
+
+// out (host, na x nb row-major) = P^T R for na P-vectors and nb R-vectors
+// (contiguous over N, strides ldp / ldr), in a fixed summation order.
+gm_status dots(gm_fab* f, const double* P, size_t ldp, int na, const double* R, size_t ldr, int nb,
+               double* out_dev, std::vector<double>* out_host) {
+  gma::Dots d{};
+  d.P = P;
+  d.ldp = ldp;
+  d.na = na;
+  d.R = R;
+  d.ldr = ldr;
+  d.nb = nb;
+  d.N = f->n;
+  d.chunk = f->chunk_dot;
+  d.partials = f->dpart;
+  hipLaunchKernelGGL(gma::k_dots<0>, dim3(f->G_dot, adiv(na, 64)), dim3(256), 0, f->stream, d);
+  A_HIP(hipGetLastError());
+  A_TRY(areduce(f, f->G_dot, na * nb, out_dev, 0, 0, f->dpart));
+  if (out_host) {
+    out_host->resize((size_t)na * nb);
+    A_HIP(hipMemcpyAsync(out_host->data(), out_dev, out_host->size() * sizeof(double),
+                         hipMemcpyDeviceToHost, f->stream));
+    A_HIP(hipStreamSynchronize(f->stream));
+  }
+  return GM_OK;
+}
+
+// Deflation of a side's start vector: a = Q^T U, U <- U - Q a.
+gm_status deflate(gm_fab* f, Side& s) {
+  A_TRY(dots(f, f->Q, (size_t)f->n, f->K, s.U, (size_t)f->n, 1, s.a, nullptr));
+  gma::DeflCM d{f->Q, f->K, f->n, s.a, nullptr, -1.0, s.U, s.U};
+  hipLaunchKernelGGL(gma::k_defl_cm<0>, dim3(std::min(4096, adiv(f->n, 256))), dim3(256), 0,
+                     f->stream, d);
+  A_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+// s.out += Q (f(lambda) * a): the eigen part of f(A) rhs.
+gm_status add_eigen_part(gm_fab* f, Side& s) {
+  gma::DeflCM d{f->Q, f->K, f->n, s.a, f->flam, 1.0, s.out, s.out};
+  hipLaunchKernelGGL(gma::k_defl_cm<0>, dim3(std::min(4096, adiv(f->n, 256))), dim3(256), 0,
+                     f->stream, d);
+  A_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+// Solves (shift I - T) h = r for the m x m Lanczos tridiagonal T (diagonal
+// al, off-diagonal be[1..m-1]) by Gaussian elimination with partial pivoting
+// (row interchanges between neighbours, as in LAPACK's dgtsv).
+void tridiag_shift_solve(int m, const double* al, const double* be, double shift,
+                         std::vector<double>& r) {
+  std::vector<double> dl(m), d(m), du(m), du2(m, 0.0);
+  for (int i = 0; i < m; ++i) {
+    d[i] = shift - al[i];
+    dl[i] = i + 1 < m ? -be[i + 1] : 0.0;   // below the diagonal, row i+1
+    du[i] = i + 1 < m ? -be[i + 1] : 0.0;   // above the diagonal, row i
+  }
+  for (int i = 0; i + 1 < m; ++i) {
+    if (std::fabs(d[i]) >= std::fabs(dl[i])) {
+      const double fct = d[i] != 0.0 ? dl[i] / d[i] : 0.0;
+      d[i + 1] -= fct * du[i];
+      r[i + 1] -= fct * r[i];
+      dl[i] = 0.0;
+    } else {   // swap rows i and i+1
+      const double fct = d[i] / dl[i];
+      d[i] = dl[i];
+      const double t = d[i + 1];
+      d[i + 1] = du[i] - fct * t;
+      if (i + 2 < m) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -fct * du2[i];
+      }
+      du[i] = t;
+      std::swap(r[i], r[i + 1]);
+      r[i + 1] -= fct * r[i];
+    }
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double v = r[i];
+    if (i + 1 < m) v -= du[i] * r[i + 1];
+    if (i + 2 < m) v -= du2[i] * r[i + 2];
+    r[i] = d[i] != 0.0 ? v / d[i] : 0.0;
+  }
 }
 
 }  // namespace
@@ -471,6 +642,10 @@ gm_status gm_fab_create(const gm_fab_desc* d, gm_fab** out) {
     if (d->A_ptr[i + 1] < d->A_ptr[i]) return afail(GM_EINVAL, "pointer array not monotone");
   for (long long j = 0; j < d->nnz; ++j)
     if (d->A_ind[j] < 0 || d->A_ind[j] >= d->n) return afail(GM_EINVAL, "index out of range");
+  if (d->num_eigs < 0 || d->num_eigs > 64 || d->num_eigs > d->n)
+    return afail(GM_EINVAL, "num_eigs must be in [0, min(64, n)]");
+  if (d->num_eigs > 0 && (!d->Q || !d->lambda || !d->f_lambda))
+    return afail(GM_EINVAL, "deflation needs Q, lambda and f_lambda");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= d->device || d->device < 0)
     return afail(GM_EDEVICE, "no HIP device %d", d->device);
@@ -534,6 +709,24 @@ gm_status gm_fab_create(const gm_fab_desc* d, gm_fab** out) {
   A_CREATE(aalloc(&f->status, 1));
   if (hipMemsetAsync(f->status, 0, sizeof(unsigned), st) != hipSuccess)
     return bail(afail(GM_EDEVICE, "memset"));
+  f->K = d->num_eigs;
+  if (f->K > 0) {
+    const size_t NK = (size_t)f->n * f->K;
+    A_CREATE(up(&f->Q, d->Q, NK));
+    A_CREATE(up(&f->flam, d->f_lambda, (size_t)f->K));
+    A_CREATE(aalloc(&f->EQ, NK));
+    f->lam.assign(d->lambda, d->lambda + f->K);
+    f->flam_h.assign(d->f_lambda, d->f_lambda + f->K);
+    if (d->f_divdiff) f->fdd.assign(d->f_divdiff, d->f_divdiff + (size_t)f->K * f->K);
+  }
+  f->G_dot = std::max(1, std::min(256, adiv(f->n, 1024)));
+  f->chunk_dot = adiv(f->n, f->G_dot);
+  f->G_dot = adiv(f->n, f->chunk_dot);
+  if (f->K > 0) {
+    const int na = std::max(f->m, f->K);
+    A_CREATE(aalloc(&f->dpart, (size_t)f->G_dot * na * f->K));
+    A_CREATE(aalloc(&f->dsum, (size_t)na * f->K));
+  }
   A_CREATE(alloc_side(f, f->sb));
   A_CREATE(alloc_side(f, f->sg));
   if (hipStreamSynchronize(st) != hipSuccess) return bail(afail(GM_EDEVICE, "sync"));
@@ -594,7 +787,9 @@ gm_status gm_fab_forward(gm_fab* f, const double* b, double* u) {
   A_HIP(hipSetDevice(f->dev));
   const size_t N = (size_t)f->n;
   A_HIP(hipMemcpyAsync(f->sb.U, b, N * sizeof(double), hipMemcpyHostToDevice, f->stream));
+  if (f->K) A_TRY(deflate(f, f->sb));
   A_TRY(krylov(f, f->sb, true));
+  if (f->K) A_TRY(add_eigen_part(f, f->sb));
   A_HIP(hipMemcpyAsync(u, f->sb.out, N * sizeof(double), hipMemcpyDeviceToHost, f->stream));
   A_HIP(hipStreamSynchronize(f->stream));
   f->have_forward = true;
@@ -610,12 +805,16 @@ gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, doubl
   if (!f->have_forward) return afail(GM_ESTATE, "backward needs a forward pass first");
   if (!g) return afail(GM_EINVAL, "null upstream gradient");
   if (grad_kappa && !kappa) return afail(GM_EINVAL, "kappa is needed for the kappa gradient");
+  if (f->K && (grad_kappa || grad_alpha))
+    return afail(GM_EINVAL, "the kappa and alpha gradients are built without deflation");
   A_HIP(hipSetDevice(f->dev));
   const size_t N = (size_t)f->n;
   hipStream_t st = f->stream;
   A_HIP(hipMemcpyAsync(f->sg.U, g, N * sizeof(double), hipMemcpyHostToDevice, st));
   // z side: f(A) g is dL/db
+  if (f->K) A_TRY(deflate(f, f->sg));
   A_TRY(krylov(f, f->sg, grad_b != nullptr));
+  if (f->K && grad_b) A_TRY(add_eigen_part(f, f->sg));
   if (grad_b)
     A_HIP(hipMemcpyAsync(grad_b, f->sg.out, N * sizeof(double), hipMemcpyDeviceToHost, st));
   if (grad_kappa || grad_alpha) {
@@ -680,13 +879,42 @@ gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
     return afail(GM_EINVAL, "null upstream gradient or parameter arrays");
   for (int q = 0; q < nparams; ++q)
     if (!dA_values[q]) return afail(GM_EINVAL, "null dA/dtheta values for parameter %d", q);
+  if (f->K && nparams > 0 && f->fdd.empty())
+    return afail(GM_EINVAL, "the parameter gradient with deflation needs f_divdiff");
   A_HIP(hipSetDevice(f->dev));
   const size_t N = (size_t)f->n;
   hipStream_t st = f->stream;
   A_HIP(hipMemcpyAsync(f->sg.U, g, N * sizeof(double), hipMemcpyHostToDevice, st));
-  A_TRY(krylov(f, f->sg, grad_b != nullptr));
+  if (f->K) A_TRY(deflate(f, f->sg));
+  A_TRY(krylov(f, f->sg, grad_b != nullptr || (f->K && nparams > 0)));
+  if (f->K && grad_b) A_TRY(add_eigen_part(f, f->sg));
   if (grad_b)
     A_HIP(hipMemcpyAsync(grad_b, f->sg.out, N * sizeof(double), hipMemcpyDeviceToHost, st));
+  // deflation: the Krylov data of both sides, Q^T b, Q^T g and, per side,
+  // H[:, k] = (lambda_k - T)^{-1} (f(lambda_k) beta0 e1 - C), the V-coefficients
+  // of g_k(A) P rhs with g_k(x) = (f(lambda_k) - f(x)) / (lambda_k - x)
+  const int m = f->m, K = f->K;
+  std::vector<double> Hb, Hg, ab, ag;
+  if (K && nparams > 0) {
+    for (int side = 0; side < 2; ++side) {
+      Side& sd = side ? f->sg : f->sb;
+      std::vector<double> al(m), be(m + 1), Cv(m), a(K);
+      A_HIP(hipMemcpyAsync(al.data(), sd.al, m * sizeof(double), hipMemcpyDeviceToHost, st));
+      A_HIP(hipMemcpyAsync(be.data(), sd.be, (m + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+      A_HIP(hipMemcpyAsync(Cv.data(), sd.C, m * sizeof(double), hipMemcpyDeviceToHost, st));
+      A_HIP(hipMemcpyAsync(a.data(), sd.a, K * sizeof(double), hipMemcpyDeviceToHost, st));
+      A_HIP(hipStreamSynchronize(st));
+      std::vector<double> H((size_t)m * K), r(m);
+      for (int k = 0; k < K; ++k) {
+        for (int j = 0; j < m; ++j) r[j] = -Cv[j];
+        r[0] += f->flam_h[k] * be[0];
+        tridiag_shift_solve(m, al.data(), be.data(), f->lam[k], r);
+        for (int j = 0; j < m; ++j) H[(size_t)j * K + k] = r[j];
+      }
+      (side ? Hg : Hb).swap(H);
+      (side ? ag : ab).swap(a);
+    }
+  }
   if (nparams > 0) {
     if (f->recompute) A_TRY(krylov(f, f->sb, false));
     A_TRY(resolvent_block(f, f->sb));
@@ -713,7 +941,44 @@ gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
       double out[2];
       A_HIP(hipMemcpyAsync(out, f->gxv, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
       A_HIP(hipStreamSynchronize(st));   // dA_values[q] and out are host memory
-      grad_theta[q] = out[0];
+      double gt = out[0];
+      if (K) {
+        // E Q (K SpMVs with E's values on A's pattern)
+        for (int k = 0; k < K; ++k) {
+          gmb::BSpmm a{};
+          a.rowptr = f->rowptr;
+          a.col = f->col;
+          a.val = f->dA;
+          a.nrows = f->n;
+          a.chunk = f->chunk;
+          a.x = f->Q + (size_t)k * N;
+          a.ldx = 1;
+          a.y = f->EQ + (size_t)k * N;
+          a.ldy = 1;
+          a.g0 = 1.0;
+          a.ca = 1.0;
+          hipLaunchKernelGGL((gmb::k_bspmm<1, gmb::kCheb>), dim3(f->G), dim3(gm::kBlock), 0, st, a);
+          A_HIP(hipGetLastError());
+        }
+        std::vector<double> QEQ, Mb, Mg;
+        A_TRY(dots(f, f->Q, N, K, f->EQ, N, K, f->dsum, &QEQ));
+        A_TRY(dots(f, f->sb.V, N, m, f->EQ, N, K, f->dsum, &Mb));
+        A_TRY(dots(f, f->sg.V, N, m, f->EQ, N, K, f->dsum, &Mg));
+        // Q-Q block: sum_kl (Q^T g)_k f[l_k, l_l] (Q^T E Q)_kl (Q^T b)_l
+        for (int k = 0; k < K; ++k)
+          for (int l = 0; l < K; ++l)
+            gt += ag[k] * f->fdd[(size_t)l * K + k] * QEQ[(size_t)k * K + l] * ab[l];
+        // cross blocks: (Q^T g)_k q_k^T E V_b H_b[:, k] + (Q^T b)_k q_k^T E V_g H_g[:, k]
+        for (int k = 0; k < K; ++k) {
+          double tb = 0.0, tg = 0.0;
+          for (int j = 0; j < m; ++j) {
+            tb += Mb[(size_t)j * K + k] * Hb[(size_t)j * K + k];
+            tg += Mg[(size_t)j * K + k] * Hg[(size_t)j * K + k];
+          }
+          gt += ag[k] * tb + ab[k] * tg;
+        }
+      }
+      grad_theta[q] = gt;
     }
   }
   A_HIP(hipStreamSynchronize(st));

@@ -24,6 +24,12 @@ steps backwards (A symmetric, so f(A)^T = f(A)):
 where each w^T (d f(A)/dtheta) v is the contour sum Im sum_j c_j z_j^T
 (dA/dtheta) x_j of ``gm_fab_backward_param``.  Each backward step
 re-runs the two forward Krylov solves it needs, from the stored states.
+
+With deflation (K eigenpairs Q, Lambda of A) f(A) v = Q f(Lambda) Q^T v +
+f(A)(I - QQ^T) v as in the forward engines, and the derivative of f(A) is
+taken exactly over that split (divided differences of f on the Q-Q block,
+real-shift resolvents (lambda_k - A)^{-1} on the cross blocks, the contour on
+the rest), so dQ/dtheta and dLambda/dtheta never enter.
 """
 
 import numpy as np
@@ -48,12 +54,29 @@ class MonodomainAdjoint:
     s f2(xi) dxi/dt), subspace size ``m`` for the Krylov resolvents.
     """
 
-    def __init__(self, A, *, m, xi, wts1, wts2, dt, ms=None, device=0):
+    def __init__(self, A, *, m, xi, wts1, wts2, dt, ms=None, device=0, Q=None, lam=None,
+                 eta=None, alpha=None):
+        """With ``Q``, ``lam`` (K eigenpairs of A) the solves are deflated as in
+        the forward engines; ``eta`` and ``alpha`` then give the exact f1, f2 and
+        their divided differences at ``lam`` (phi is taken exactly at every
+        eigenvalue, without the phi_lambda(1) = 1 guard of the set-up)."""
         self.p = dict(_lib.MS_DEFAULTS)
         self.p.update(ms or {})
         self.dt = float(dt)
-        self.f1 = FabEngine(A, m=m, xi=xi, wts=wts1, recompute=False, device=device)
-        self.f2 = FabEngine(A, m=m, xi=xi, wts=wts2, recompute=False, device=device)
+        kw1, kw2 = {}, {}
+        if Q is not None and np.shape(Q)[1] > 0:
+            if eta is None or alpha is None:
+                raise ValueError("deflation needs eta and alpha")
+            from . import quadrature as qd
+            lam = np.asarray(lam, dtype=np.float64)
+            fa = (lambda x: qd.f1(x, eta, alpha), lambda x: qd.df1(x, eta, alpha))
+            fb = (lambda x: qd.f2(x, eta, alpha), lambda x: qd.df2(x, eta, alpha))
+            kw1 = dict(Q=Q, lam=lam, f_lambda=fa[0](lam),
+                       f_divdiff=qd.divided_differences(fa[0], fa[1], lam))
+            kw2 = dict(Q=Q, lam=lam, f_lambda=fb[0](lam),
+                       f_divdiff=qd.divided_differences(fb[0], fb[1], lam))
+        self.f1 = FabEngine(A, m=m, xi=xi, wts=wts1, recompute=False, device=device, **kw1)
+        self.f2 = FabEngine(A, m=m, xi=xi, wts=wts2, recompute=False, device=device, **kw2)
         self.traj = None
 
     def set_matrix(self, A):

@@ -96,6 +96,53 @@ def f2(x, eta, alpha):
     return np.expm1(z) / z
 
 
+def _dphi(z):
+    """phi'(z) for phi(z) = (e^z - 1)/z: sum_{n>=1} n z^(n-1)/(n+1)! near 0, else
+    (e^z (z - 1) + 1)/z^2."""
+    z = np.asarray(z, dtype=np.float64)
+    small = np.abs(z) < 0.5
+    out = np.empty_like(z)
+    zs = z[small]
+    acc = np.zeros_like(zs)
+    term_fact = 1.0
+    for n in range(1, 24):
+        term_fact *= (n + 1)
+        acc += n * zs ** (n - 1) / term_fact
+    out[small] = acc
+    zb = z[~small]
+    out[~small] = (np.exp(zb) * (zb - 1.0) + 1.0) / (zb * zb)
+    return out
+
+
+def df1(x, eta, alpha):
+    """d/dx exp(-eta x^(alpha/2))."""
+    x = np.asarray(x, dtype=np.float64)
+    a = alpha / 2.0
+    return -eta * a * np.power(x, a - 1.0) * f1(x, eta, alpha)
+
+
+def df2(x, eta, alpha):
+    """d/dx phi(-eta x^(alpha/2))."""
+    x = np.asarray(x, dtype=np.float64)
+    a = alpha / 2.0
+    return _dphi(-eta * np.power(x, a)) * (-eta * a * np.power(x, a - 1.0))
+
+
+def divided_differences(fun, dfun, lam, rel=1e-6):
+    """K x K matrix f[lam_k, lam_l] = (f(lam_k) - f(lam_l)) / (lam_k - lam_l), with
+    f' at the midpoint where |lam_k - lam_l| <= rel |lam_k| (the diagonal and
+    near-degenerate pairs)."""
+    lam = np.asarray(lam, dtype=np.float64)
+    fl = fun(lam)
+    dl = lam[:, None] - lam[None, :]
+    close = np.abs(dl) <= rel * np.maximum(np.abs(lam[:, None]), np.abs(lam[None, :]))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        F = (fl[:, None] - fl[None, :]) / dl
+    mid = 0.5 * (lam[:, None] + lam[None, :])
+    F[close] = dfun(mid[close])
+    return F
+
+
 def contour_nodes(l1, lN, n, eta, alpha):
     """HHT quadrature on [l1, lN]: dict of xi, dxidt, wts1, wts2 (GPU form, with s), s."""
     r = np.sqrt(lN / l1)

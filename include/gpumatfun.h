@@ -302,6 +302,17 @@ typedef struct gm_fab_desc {
     const double* dwts;            /* n_s complex dc_j/dalpha (may be NULL)  */
     uint32_t flags;                /* GM_FAB_*                               */
     int32_t device;
+    /* Deflation (as lanczos_create takes it): f(A) b = Q f(Lambda) Q^T b +
+     * the Krylov/contour f of (I - Q Q^T) b.  The contour then only needs to
+     * cover the rest of the spectrum.                                       */
+    int32_t num_eigs;              /* K (0: none), K <= 64                   */
+    const double* Q;               /* n x K column-major, orthonormal        */
+    const double* lambda;          /* K eigenvalues of A                     */
+    const double* f_lambda;        /* K values f(lambda_k)                   */
+    const double* f_divdiff;       /* K x K column-major divided differences
+                                      f[lambda_k, lambda_l], diagonal
+                                      f'(lambda_k); used by
+                                      gm_fab_backward_param (may be NULL)   */
 } gm_fab_desc;
 
 const char* gm_fab_last_error(void);
@@ -318,10 +329,14 @@ gm_status gm_fab_forward(gm_fab* f, const double* b, double* u);
 gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, double* grad_b,
                           double* grad_kappa, double* grad_alpha);
 /* after a forward: gradients with respect to nparams scalar parameters theta_q
- * of A, given each dA/dtheta_q on A's sparsity pattern (nnz values):
+ * of A, given each dA/dtheta_q on A's sparsity pattern (nnz values,
+ * symmetric):
  *   grad_theta[q] = Im sum_j c_j z_j^T (dA/dtheta_q) x_j
  * (e.g. theta = the conductivity ratio of an anisotropic operator, which A
- * depends on linearly); grad_b (may be NULL) = f(A) g as in gm_fab_backward. */
+ * depends on linearly); grad_b (may be NULL) = f(A) g as in gm_fab_backward.
+ * With deflation the Frechet derivative of f splits over the eigenpairs Q
+ * and the rest: the Q-Q block by f_divdiff, the cross blocks by real-shift
+ * solves (lambda_k - T)^{-1} in the Krylov bases, the rest by the contour. */
 gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
                                 const double* const* dA_values, double* grad_b,
                                 double* grad_theta);

@@ -107,17 +107,63 @@ def param_gradient(A, E, b, g, xi, c, solver=None):
     return float(np.imag(sum(c[j] * (Z[:, j] @ (E @ X[:, j])) for j in range(len(xi)))))
 
 
-def ms_forward(A, u0, h0, steps, xi, c1, c2, dt, p):
+def deflated_forward(A, b, xi, c, Q, f_lambda):
+    """f(A) b = Q (f(Lambda) Q^T b) + Im sum_j c_j (xi_j - A)^{-1} (b - Q Q^T b): the
+    deflated split the GPU path uses (``lanczos_impl.cu:79-89``, ``update_U.cu:18-27``),
+    here with exact resolvents."""
+    a = Q.T @ b
+    u, _ = forward(A, b - Q @ a, xi, c)
+    return u + Q @ (f_lambda * a)
+
+
+def deflated_param_gradient(A, E, b, g, xi, c, Q, lam, f_lambda, f_divdiff, parts=False):
+    """dL/dtheta = g^T L_f(A, E) b for the deflated forward, E = dA/dtheta symmetric.
+
+    The Frechet derivative of f in A's eigenbasis is sum_kl f[l_k, l_l] u_k u_k^T E
+    u_l u_l^T; split over Q (the K deflated pairs) and P = I - QQ^T:
+      Q-Q:  (Q^T g)^T (F o Q^T E Q) (Q^T b),  F = f_divdiff;
+      Q-P:  sum_k (Q^T g)_k q_k^T E g_k(A) P b + sum_k (Q^T b)_k q_k^T E g_k(A) P g,
+            g_k(x) = (f(l_k) - f_c(x)) / (l_k - x);
+      P-P:  Im sum_j c_j z_j^T E x_j, x_j = (xi_j - A)^{-1} P b, z_j likewise for g,
+    with f_c(x) = Im sum_j c_j / (xi_j - x) the contour rule on the P part.  Dense
+    (small grids): g_k(A) P is formed from the full eigendecomposition.  With parts,
+    returns the three blocks (pp, qq, cross) instead of their sum."""
+    E = sp.csr_matrix(E)
+    K = Q.shape[1]
+    ab, ag = Q.T @ b, Q.T @ g
+    Pb, Pg = b - Q @ ab, g - Q @ ag
+    X = resolvents(A, xi, Pb)
+    Z = resolvents(A, xi, Pg)
+    pp = float(np.imag(sum(c[j] * (Z[:, j] @ (E @ X[:, j])) for j in range(len(xi)))))
+    EQ = E @ Q
+    qq = float(ag @ ((f_divdiff * (Q.T @ EQ)) @ ab))
+    w, U = np.linalg.eigh(A.toarray() if sp.issparse(A) else A)
+    Up, wp = U[:, K:], w[K:]
+    fc = np.imag((c[None, :] / (np.asarray(xi)[None, :] - wp[:, None])).sum(axis=1))
+    cross = 0.0
+    for k in range(K):
+        gk = (f_lambda[k] - fc) / (lam[k] - wp)
+        cross += ag[k] * (EQ[:, k] @ (Up @ (gk * (Up.T @ Pb))))
+        cross += ab[k] * (EQ[:, k] @ (Up @ (gk * (Up.T @ Pg))))
+    return (pp, qq, cross) if parts else pp + qq + cross
+
+
+def ms_forward(A, u0, h0, steps, xi, c1, c2, dt, p, defl=None):
     """The monodomain Mitchell-Schaeffer steps with exact resolvents:
     u_{n+1} = Im sum_j c1_j x_j(u_n) + dt Im sum_j c2_j x_j(g_n) (the contour
-    rule of problem_startup.m:95-103 for f1 and f2 without deflation), then the
-    gate and g from ``stepping.ms_reaction``.  Returns (u_T, h_T)."""
+    rule of problem_startup.m:95-103 for f1 and f2), then the gate and g from
+    ``stepping.ms_reaction``; with defl = (Q, f1(Lambda), f2(Lambda)) each
+    product is the deflated split of ``deflated_forward``.  Returns (u_T, h_T)."""
     from .stepping import ms_reaction
     u = np.array(u0, dtype=np.float64)
     h = np.array(h0, dtype=np.float64)
     g = (h * (u * u) * (1.0 - u)) / p["tau_in"] - u / p["tau_out"]
     for _ in range(steps):
-        u = forward(A, u, xi, c1)[0] + dt * forward(A, g, xi, c2)[0]
+        if defl is None:
+            u = forward(A, u, xi, c1)[0] + dt * forward(A, g, xi, c2)[0]
+        else:   # defl = (Q, f1(lambda), f2(lambda))
+            Q, fl1, fl2 = defl
+            u = deflated_forward(A, u, xi, c1, Q, fl1) + dt * deflated_forward(A, g, xi, c2, Q, fl2)
         h, g = ms_reaction(u, h, dt, p)
     return u, h
 

@@ -97,3 +97,67 @@ def test_param_gradient_conductivity_ratio_central_differences():
     Lm, _ = loss(problems.anisotropic_9pt(n, ratio=r - h), b, y, xi, cw)
     fd = (Lp - Lm) / (2 * h)
     assert abs(fd - g) <= 1e-6 * abs(g), (fd, g)
+
+
+def test_deflated_param_gradient_central_differences():
+    """The deflated forward Q f(Lambda) Q^T b + f_c(A)(b - QQ^T b), with its K = 6
+    eigenpairs recomputed at every ratio, differentiated through the split of the
+    Frechet derivative (Q-Q divided differences, cross-block real-shift
+    resolvents, contour on the rest), against central differences.  Dropping any
+    one of the three blocks moves the result by more than the tolerance (checked
+    below), so each is pinned."""
+    from gpumatfun_amd import quadrature as q
+    n, r, h, K = 12, 4.0, 1e-5, 6
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    A = problems.anisotropic_9pt(n, ratio=r)
+    w = np.linalg.eigvalsh(A.toarray())
+    xi, cw, _ = weights(0.9 * w[K], 1.1 * w.max(), ALPHA)
+    f = (lambda x: q.f1(x, TAU, ALPHA))
+    df = (lambda x: q.df1(x, TAU, ALPHA))
+    rng = np.random.default_rng(4)
+    Q0 = np.linalg.eigh(A.toarray())[1][:, :K]
+    b = rng.standard_normal(n * n) + 5.0 * Q0 @ rng.standard_normal(K)   # strong low modes
+    y = rng.standard_normal(n * n)
+
+    def defl(rr):
+        Ar = problems.anisotropic_9pt(n, ratio=rr)
+        wr, Ur = np.linalg.eigh(Ar.toarray())
+        return Ar, Ur[:, :K], wr[:K]
+
+    def L(rr):
+        Ar, Qr, lr = defl(rr)
+        u = adjoint.deflated_forward(Ar, b, xi, cw, Qr, f(lr))
+        return float(np.sum((u - y) ** 2)), u
+
+    L0, u = L(r)
+    _, Q, lam = defl(r)
+    F = q.divided_differences(f, df, lam)
+    g = adjoint.deflated_param_gradient(A, E, b, 2 * (u - y), xi, cw, Q, lam, f(lam), F)
+    fd = (L(r + h)[0] - L(r - h)[0]) / (2 * h)
+    assert abs(fd - g) <= 1e-8 * abs(g), (fd, g)
+    # the deflated forward is f(A) b to the contour's accuracy
+    u_full, _ = adjoint.forward(A, b, *weights(0.9 * w.min(), 1.1 * w.max(), ALPHA)[:2])
+    assert rel(u, u_full) < 1e-9
+    # each block is far above the agreement, so dropping any one would fail it
+    parts = adjoint.deflated_param_gradient(A, E, b, 2 * (u - y), xi, cw, Q, lam, f(lam), F,
+                                            parts=True)
+    print("blocks", parts, "fd", fd, "rel", abs(fd - g) / abs(g))
+    assert min(abs(v) for v in parts) > 1e-4 * abs(g)
+
+
+def test_divided_differences_and_derivatives():
+    """f1', f2' against central differences in extended steps; the divided
+    difference matrix is symmetric, its diagonal the derivative, and an
+    off-diagonal entry the secant slope."""
+    from gpumatfun_amd import quadrature as q
+    eta, al = 5e-5, 1.7
+    x = np.array([3.0, 80.0, 1014.0, 2.5e4])
+    for fn, dfn in ((q.f1, q.df1), (q.f2, q.df2)):
+        for xv in x:
+            hh = 1e-4 * xv
+            fd = (fn(xv + hh, eta, al) - fn(xv - hh, eta, al)) / (2 * hh)
+            assert abs(fd - dfn(xv, eta, al)) <= 1e-7 * abs(dfn(xv, eta, al))
+        F = q.divided_differences(lambda t: fn(t, eta, al), lambda t: dfn(t, eta, al), x)
+        assert np.allclose(F, F.T, rtol=1e-14, atol=0)
+        assert np.allclose(np.diag(F), dfn(x, eta, al), rtol=1e-14, atol=0)
+        assert abs(F[0, 2] - (fn(x[0], eta, al) - fn(x[2], eta, al)) / (x[0] - x[2])) < 1e-18

@@ -216,3 +216,38 @@ def test_param_gradient_conductivity_ratio_vs_exact_reference():
     print("dL/dr", gr, ref, "grad_b rel", rel(gb, rb))
     assert abs(gr - ref) <= 1e-7 * abs(ref)
     assert rel(gb, rb) < 1e-7
+
+
+def test_deflated_param_gradient_vs_exact_reference():
+    """gm_fab_* with K = 8 deflation pairs: the forward against the oracle's
+    deflated split with exact resolvents, dL/db (the same split applied to g),
+    and dL/dr through the block split of the Frechet derivative
+    (oracle/adjoint.deflated_param_gradient, pinned by central differences in
+    tests/test_adjoint_oracle.py).  m = 120 resolves the deflated Krylov
+    problem to ~1e-15 on the CPU; tolerances 1e-10 (vectors) and 1e-8 (dL/dr,
+    whose cross blocks go through real-shift tridiagonal solves)."""
+    n, K, m, r = 40, 8, 120, 4.0
+    A = problems.anisotropic_9pt(n, ratio=r)
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    w, U = np.linalg.eigh(A.toarray())
+    Q, lam = U[:, :K], w[:K]
+    c = q.contour_nodes(0.95 * w[K], 1.05 * w.max(), 24, TAU, 1.0)
+    cw, dc = contour_weights(c["xi"], c["dxidt"], c["s"], TAU, ALPHA)
+    f = (lambda x: q.f1(x, TAU, ALPHA))
+    F = q.divided_differences(f, lambda x: q.df1(x, TAU, ALPHA), lam)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(n * n) + 5.0 * Q @ rng.standard_normal(K)
+    y = rng.standard_normal(n * n)
+    print(f"case deflated param gradient n={n} K={K} m={m}", flush=True)
+    eng = FabEngine(A, m=m, xi=c["xi"], wts=cw, Q=Q, lam=lam, f_lambda=f(lam), f_divdiff=F)
+    u = eng.forward(b)
+    ref_u = oadj.deflated_forward(A, b, c["xi"], cw, Q, f(lam))
+    g = 2 * (u - y)
+    gb, (gr,) = eng.backward_param(g, [E], want_b=True)
+    eng.delete()
+    ref_gb = oadj.deflated_forward(A, g, c["xi"], cw, Q, f(lam))
+    ref = oadj.deflated_param_gradient(A, E, b, g, c["xi"], cw, Q, lam, f(lam), F)
+    print("forward", rel(u, ref_u), "grad_b", rel(gb, ref_gb), "dL/dr", gr, ref)
+    assert rel(u, ref_u) < 1e-10
+    assert rel(gb, ref_gb) < 1e-10
+    assert abs(gr - ref) <= 1e-8 * abs(ref)

@@ -53,3 +53,42 @@ def test_ratio_and_initial_state_gradients_vs_finite_differences():
         fdu = (loss(r, up)[0] - loss(r, um)[0]) / (2 * h)
         print("dL/du0", i, ubar0[i], fdu)
         assert abs(ubar0[i] - fdu) <= 1e-6 * max(abs(fdu), 1e-3 * np.abs(ubar0).max())
+
+
+def test_deflated_ratio_gradient_vs_finite_differences():
+    """The same run with K = 6 deflation pairs and the contour on the rest of
+    the spectrum (what the 512^2 workload does): dL/dratio against central
+    differences of the oracle's deflated exact-resolvent forward, with the
+    eigenpairs recomputed at each perturbed ratio (the derivative of f(A), not
+    of a frozen eigenbasis).  m = 60 (the deflated problem converges faster)."""
+    n, steps, dt, r, m, h, K = 24, 5, 0.05, 4.0, 60, 1e-5, 6
+    eta, alpha = dt * 1e-3, 1.7
+    A = problems.anisotropic_9pt(n, ratio=r)
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    w, U = np.linalg.eigh(A.toarray())
+    c = q.contour_nodes(0.95 * w[K], 1.05 * w.max(), 24, eta, alpha)
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    u0 = ((X >= 3) & (X < 9) & (Y >= 4) & (Y < 10)).astype(float).reshape(-1, order="F")
+    h0 = np.ones(n * n)
+    y = 0.3 * np.random.default_rng(0).random(n * n)
+    p = dict(MS_DEFAULTS)
+    print(f"case deflated monodomain adjoint n={n} K={K} steps={steps} m={m}", flush=True)
+    md = MonodomainAdjoint(A, m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"], dt=dt, ms=p,
+                           Q=U[:, :K], lam=w[:K], eta=eta, alpha=alpha)
+    uT = md.forward(u0, h0, steps)
+    _, _, (gr,) = md.backward(2 * (uT - y), [E])
+    md.delete()
+
+    def loss(rr):
+        Ar = problems.anisotropic_9pt(n, ratio=rr)
+        wr, Ur = np.linalg.eigh(Ar.toarray())
+        defl = (Ur[:, :K], q.f1(wr[:K], eta, alpha), q.f2(wr[:K], eta, alpha))
+        u, _ = oadj.ms_forward(Ar, u0, h0, steps, c["xi"], c["wts1"], c["wts2"], dt, p, defl=defl)
+        return float(np.sum((u - y) ** 2)), u
+
+    L0, u_ref = loss(r)
+    print("forward rel err", rel(uT, u_ref))
+    assert rel(uT, u_ref) < 1e-9
+    fd = (loss(r + h)[0] - loss(r - h)[0]) / (2 * h)
+    print("dL/dratio", gr, "fd", fd)
+    assert abs(gr - fd) <= 1e-6 * abs(fd)
