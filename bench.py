@@ -21,7 +21,12 @@ Workloads (``--config``):
   (Chebyshev-filtered subspace iteration, block 32, degree 200, relative
   residual 1e-10), timed beside scipy's ARPACK shift-invert on the host.
 
-(``uneven64``, ``adjoint256`` and ``ms512`` are described at their builders.)
+* ``msadj512`` -- the gradient of the ``ms512`` run's end-state mismatch with
+  respect to the conductivity ratio, through 1000 deflated time steps of one
+  stimulus (the adjoint of the monodomain time stepping, deflation reused).
+
+(``uneven64``, ``adjoint256``, ``ms512`` and ``msadj512`` are described at their
+builders.)
 One step is one time step of the exponential integrator = 2 f(A)b solves per
 problem; inputs are resident in HBM when the timed region starts.  Prints one
 JSON line.
@@ -507,6 +512,87 @@ def run_eigs512(args):
     print(json.dumps(result), flush=True)
 
 
+def run_msadj512(args, steps):
+    """dL/dratio for L = ||u_T - y||^2 on the 512^2 Mitchell-Schaeffer monodomain
+    (alpha = 1.7, dt = 0.05, m = 40, 32 nodes, K = 20 deflation pairs reused every
+    step, eigenpairs from the GPU eigensolver), through `steps` steps of one
+    stimulus: a 20 x 20 corner patch.  y is the end state at ratio 8 (a synthetic
+    observation); the gradient is taken at ratio 9.  A 20-step run of the same
+    code is checked against central differences (step 1e-2) of the HIP forward
+    with the eigenpairs recomputed at each perturbed ratio."""
+    import scipy.sparse.linalg as sla
+    from gpumatfun_amd import problems, quadrature as q
+    from gpumatfun_amd.eigs import eigs_smallest
+    from gpumatfun_amd.monodomain import MonodomainAdjoint
+    from gpumatfun_amd._lib import MS_DEFAULTS
+    t0 = time.time()
+    n, K, dt, m, ns, r0, r_obs = 512, 20, 0.05, args.adj_m, 32, 9.0, 8.0
+    eta, alpha = dt * 1e-3, 1.7
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    N = n * n
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    u0 = ((X < 20) & (Y < 20)).astype(float).reshape(-1, order="F")
+    h0 = np.ones(N)
+    prm = dict(MS_DEFAULTS)
+
+    def engine(r, c=None):
+        A = problems.anisotropic_9pt(n, ratio=r)
+        lam, Q = eigs_smallest(A, K + 1, block_size=32, degree=200, tol=1e-10)
+        if c is None:
+            lN = float(sla.eigsh(A, k=1, which="LA", return_eigenvectors=False)[0])
+            c = q.contour_nodes(0.98 * lam[K], 1.02 * lN, ns, eta, alpha)
+        md = MonodomainAdjoint(A, m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"], dt=dt, ms=prm,
+                               Q=Q[:, :K], lam=lam[:K], eta=eta, alpha=alpha)
+        return md, c
+
+    md_obs, _ = engine(r_obs)
+    y = md_obs.forward(u0, h0, steps)
+    y20 = md_obs.forward(u0, h0, 20)
+    md_obs.delete()
+    md, c = engine(r0)
+    setup_s = time.time() - t0
+    log(f"set-up {setup_s:.1f}s (eigenpairs on the GPU, observation at ratio {r_obs})")
+    t1 = time.perf_counter()
+    uT = md.forward(u0, h0, steps)
+    fwd_s = time.perf_counter() - t1
+    L = float(np.sum((uT - y) ** 2))
+    t1 = time.perf_counter()
+    _, _, (gr,) = md.backward(2 * (uT - y), [E])
+    bwd_s = time.perf_counter() - t1
+    log(f"{steps} steps: forward {fwd_s:.2f}s, backward {bwd_s:.2f}s; L = {L:.6e}, "
+        f"dL/dratio = {gr:.6e}")
+    # the same code over 20 steps against central differences of the HIP forward
+    u20 = md.forward(u0, h0, 20)
+    _, _, (g20,) = md.backward(2 * (u20 - y20), [E])
+    md.delete()
+    hh = 1e-2
+    Ls = []
+    for r in (r0 + hh, r0 - hh):
+        mdr, _ = engine(r, c)
+        Ls.append(float(np.sum((mdr.forward(u0, h0, 20) - y20) ** 2)))
+        mdr.delete()
+    fd = (Ls[0] - Ls[1]) / (2 * hh)
+    log(f"20 steps: dL/dratio {g20:.10e}, central difference {fd:.10e}, "
+        f"rel {abs(g20 - fd) / abs(fd):.2e}")
+    cfg = {"workload": "msadj_aniso512_1stim", "n": N, "nnz": int(E.nnz), "steps": steps,
+           "m": m, "shifts": ns, "deflation": K, "alpha": alpha, "dt": dt, "eta": eta,
+           "reaction": "Mitchell-Schaeffer", "stimulus": "20 x 20 corner patch",
+           "ratio": r0, "observation_ratio": r_obs, "loss": "||u_T - y||^2"}
+    result = {
+        "metric": "adjoint cost vs forward (FP64, monodomain conductivity-ratio gradient)",
+        "value": round(bwd_s / fwd_s, 3), "unit": "x forward", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "ms_per_step": round(1e3 * (fwd_s + bwd_s) / steps, 3),
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (stimulus patch; observation simulated at ratio 8)", "config": cfg,
+        "setup_s": round(setup_s, 1), "forward_s": round(fwd_s, 3), "backward_s": round(bwd_s, 3),
+        "loss": L, "dL_dratio": gr,
+        "fd_check_20_steps": {"adjoint": g20, "central_difference": fd, "step": hh,
+                              "rel_diff": abs(g20 - fd) / abs(fd)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def oracle_setup_dict(inp, A=None, perm=None):
     S = {"A": inp.A if A is None else A, "Q": inp.Q if inp.Q.shape[1] else None,
          "exp_lambda": inp.exp_lambda, "phi_lambda": inp.phi_lambda, "dt": inp.dt,
@@ -590,7 +676,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--config", default="fem1m",
                     choices=["fem1m", "dirichlet128", "batch3d", "uneven64", "adjoint256",
-                             "ms512", "eigs512"])
+                             "ms512", "eigs512", "msadj512"])
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--side3d", type=int, default=128)
     ap.add_argument("--problems", type=int, default=256)
@@ -601,6 +687,8 @@ def main():
     ap.add_argument("--schedule", default="auto", choices=["auto", "rows", "blocks", "blocks16"],
                     help="SpMM schedule of the plain path: autotuned at create (default), "
                          "or forced (fixed kernels for counter runs)")
+    ap.add_argument("--adj-m", type=int, default=40,
+                    help="msadj512: Lanczos steps of the adjoint's solves (40 as in ms512)")
     ap.add_argument("--profile-only", action="store_true",
                     help="a few steps and the kernel timings only (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -641,6 +729,9 @@ def main():
         return
     if args.config == "eigs512":
         run_eigs512(args)
+        return
+    if args.config == "msadj512":
+        run_msadj512(args, args.steps or 1000)
         return
     if args.config == "uneven64":
         if args.problems == 256:

@@ -40,6 +40,7 @@ EXPORTS = (
     "gm_batch_get_ode",
     "gm_fab_last_error", "gm_fab_create", "gm_fab_delete", "gm_fab_set_matrix",
     "gm_fab_set_weights", "gm_fab_forward", "gm_fab_backward", "gm_fab_backward_param",
+    "gm_fab_set_param_derivs",
     "gm_ubatch_last_error", "gm_ubatch_create", "gm_ubatch_delete", "gm_ubatch_step",
     "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info",
     "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
@@ -267,6 +268,7 @@ def load():
         "gm_fab_forward": (C.c_int, [vp, _dp, _dp]),
         "gm_fab_backward": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "gm_fab_backward_param": (C.c_int, [vp, _dp, C.c_int32, C.POINTER(_dp), _dp, _dp]),
+        "gm_fab_set_param_derivs": (C.c_int, [vp, C.c_int32, C.POINTER(_dp)]),
         "gm_ubatch_last_error": (C.c_char_p, []),
         "gm_ubatch_create": (C.c_int, [C.POINTER(GmUbatchDesc), C.POINTER(vp)]),
         "gm_ubatch_delete": (C.c_int, [vp]),

@@ -102,9 +102,10 @@ class FabEngine:
         self._call(self._lib.gm_fab_set_weights, _p(_cplx(wts)),
                    None if dwts is None else _p(_cplx(dwts)))
 
-    def forward(self, b):
+    def forward(self, b, want_u=True):
+        """u = f(A) b; with want_u=False only the state a backward pass needs."""
         b = np.ascontiguousarray(b, dtype=np.float64)
-        u = np.empty(self.n)
+        u = np.empty(self.n) if want_u else None
         self._call(self._lib.gm_fab_forward, _p(b), _p(u))
         return u
 
@@ -117,10 +118,7 @@ class FabEngine:
         self._call(self._lib.gm_fab_backward, _p(g), _p(kap), _p(gb), _p(gk), _p(ga))
         return gb, gk, (None if ga is None else float(ga[0]))
 
-    def backward_param(self, g, dA_list, want_b=False):
-        """dL/dtheta_q = Im sum_j c_j z_j^T (dA/dtheta_q) x_j for each sparse dA/dtheta_q
-        (A's sparsity pattern) after a forward pass; returns (grad_b or None, grads)."""
-        g = np.ascontiguousarray(g, dtype=np.float64)
+    def _derivs(self, dA_list):
         vals = []
         for E in dA_list:
             E = sp.csr_matrix(E)
@@ -129,11 +127,28 @@ class FabEngine:
                     not np.array_equal(E.indices, self._ind):
                 raise ValueError("dA/dtheta must have A's sparsity pattern")
             vals.append(np.ascontiguousarray(E.data, dtype=np.float64))
-        arr = (C.POINTER(C.c_double) * max(1, len(vals)))(*[_p(v) for v in vals])
-        out = np.zeros(max(1, len(vals)))
+        return vals, (C.POINTER(C.c_double) * max(1, len(vals)))(*[_p(v) for v in vals])
+
+    def set_param_derivs(self, dA_list):
+        """Keep the dA/dtheta_q on the device for ``backward_param(g, None)``."""
+        vals, arr = self._derivs(dA_list)
+        self._call(self._lib.gm_fab_set_param_derivs, len(vals), arr)
+        self._nstored = len(vals)
+
+    def backward_param(self, g, dA_list, want_b=False):
+        """dL/dtheta_q = Im sum_j c_j z_j^T (dA/dtheta_q) x_j for each sparse dA/dtheta_q
+        (A's sparsity pattern) after a forward pass; returns (grad_b or None, grads).
+        ``dA_list=None`` uses the derivatives kept by ``set_param_derivs``."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        if dA_list is None:
+            n_p, arr = getattr(self, "_nstored", 0), None
+        else:
+            vals, arr = self._derivs(dA_list)
+            n_p = len(vals)
+        out = np.zeros(max(1, n_p))
         gb = np.empty(self.n) if want_b else None
-        self._call(self._lib.gm_fab_backward_param, _p(g), len(vals), arr, _p(gb), _p(out))
-        return gb, out[:len(vals)].copy()
+        self._call(self._lib.gm_fab_backward_param, _p(g), n_p, arr, _p(gb), _p(out))
+        return gb, out[:n_p].copy()
 
     def delete(self):
         if self._h:

@@ -346,6 +346,10 @@ struct gm_fab {
   int K = 0, G_dot = 1, chunk_dot = 1;
   double *Q = nullptr, *flam = nullptr, *EQ = nullptr, *dpart = nullptr, *dsum = nullptr;
   std::vector<double> lam, flam_h, fdd;   // host copies (fdd K x K column-major)
+  // parameter derivatives kept on the device (gm_fab_set_param_derivs): dA/dtheta_q
+  // values, and with deflation E_q Q and Q^T E_q Q
+  std::vector<double*> pdA, pEQ;
+  std::vector<std::vector<double>> pQEQ;
 };
 
 namespace {
@@ -371,6 +375,8 @@ void free_fab(gm_fab* f) {
                   f->partials, f->kappa, f->grad, f->gxv, f->wD, f->wDU, f->wDU2, f->wX,
                   f->status, f->dA, f->partials2, f->Q, f->flam, f->EQ, f->dpart, f->dsum};
   for (void* p : bufs) afree(p);
+  for (double* p : f->pdA) afree(p);
+  for (double* p : f->pEQ) afree(p);
   for (Side* s : {&f->sb, &f->sg}) {
     void* sbufs[] = {s->U, s->V, s->W0, s->W1, s->al, s->be, s->yr, s->yi, s->Yg, s->Xc, s->out,
                      s->C, s->a};
@@ -622,6 +628,29 @@ void tridiag_shift_solve(int m, const double* al, const double* be, double shift
   }
 }
 
+// EQ = E Q (K SpMVs with E's values on A's pattern) and, on the host,
+// QEQ = Q^T E Q (K x K row-major).
+gm_status eq_products(gm_fab* f, const double* E, double* EQ, std::vector<double>& QEQ) {
+  const size_t N = (size_t)f->n;
+  for (int k = 0; k < f->K; ++k) {
+    gmb::BSpmm a{};
+    a.rowptr = f->rowptr;
+    a.col = f->col;
+    a.val = E;
+    a.nrows = f->n;
+    a.chunk = f->chunk;
+    a.x = f->Q + (size_t)k * N;
+    a.ldx = 1;
+    a.y = EQ + (size_t)k * N;
+    a.ldy = 1;
+    a.g0 = 1.0;
+    a.ca = 1.0;
+    hipLaunchKernelGGL((gmb::k_bspmm<1, gmb::kCheb>), dim3(f->G), dim3(gm::kBlock), 0, f->stream, a);
+    A_HIP(hipGetLastError());
+  }
+  return dots(f, f->Q, N, f->K, EQ, N, f->K, f->dsum, &QEQ);
+}
+
 }  // namespace
 
 extern "C" {
@@ -783,14 +812,16 @@ gm_status gm_fab_set_weights(gm_fab* f, const double* wts, const double* dwts) {
 
 gm_status gm_fab_forward(gm_fab* f, const double* b, double* u) {
   if (!alive(f)) return afail(GM_EBADHANDLE, "invalid fab handle");
-  if (!b || !u) return afail(GM_EINVAL, "null vector");
+  if (!b) return afail(GM_EINVAL, "null vector");
   A_HIP(hipSetDevice(f->dev));
   const size_t N = (size_t)f->n;
   A_HIP(hipMemcpyAsync(f->sb.U, b, N * sizeof(double), hipMemcpyHostToDevice, f->stream));
   if (f->K) A_TRY(deflate(f, f->sb));
   A_TRY(krylov(f, f->sb, true));
-  if (f->K) A_TRY(add_eigen_part(f, f->sb));
-  A_HIP(hipMemcpyAsync(u, f->sb.out, N * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+  if (u) {
+    if (f->K) A_TRY(add_eigen_part(f, f->sb));
+    A_HIP(hipMemcpyAsync(u, f->sb.out, N * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+  }
   A_HIP(hipStreamSynchronize(f->stream));
   f->have_forward = true;
   unsigned stv = 0;
@@ -875,9 +906,13 @@ gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
                                 double* grad_theta) {
   if (!alive(f)) return afail(GM_EBADHANDLE, "invalid fab handle");
   if (!f->have_forward) return afail(GM_ESTATE, "backward needs a forward pass first");
-  if (!g || nparams < 0 || (nparams > 0 && (!dA_values || !grad_theta)))
+  if (!g || nparams < 0 || (nparams > 0 && !grad_theta))
     return afail(GM_EINVAL, "null upstream gradient or parameter arrays");
-  for (int q = 0; q < nparams; ++q)
+  const bool stored = nparams > 0 && !dA_values;
+  if (stored && nparams != (int)f->pdA.size())
+    return afail(GM_EINVAL, "%d parameters asked, %d stored by gm_fab_set_param_derivs", nparams,
+                 (int)f->pdA.size());
+  for (int q = 0; q < nparams && !stored; ++q)
     if (!dA_values[q]) return afail(GM_EINVAL, "null dA/dtheta values for parameter %d", q);
   if (f->K && nparams > 0 && f->fdd.empty())
     return afail(GM_EINVAL, "the parameter gradient with deflation needs f_divdiff");
@@ -919,15 +954,17 @@ gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
     if (f->recompute) A_TRY(krylov(f, f->sb, false));
     A_TRY(resolvent_block(f, f->sb));
     A_TRY(resolvent_block(f, f->sg));
-    if (!f->dA) A_TRY(aalloc(&f->dA, (size_t)f->nnz));
+    if (!stored && !f->dA) A_TRY(aalloc(&f->dA, (size_t)f->nnz));
     const int G = adiv(f->n, 256);
     for (int q = 0; q < nparams; ++q) {
-      A_HIP(hipMemcpyAsync(f->dA, dA_values[q], (size_t)f->nnz * sizeof(double),
-                           hipMemcpyHostToDevice, st));
+      const double* Edev = stored ? f->pdA[q] : f->dA;
+      if (!stored)
+        A_HIP(hipMemcpyAsync(f->dA, dA_values[q], (size_t)f->nnz * sizeof(double),
+                             hipMemcpyHostToDevice, st));
       gma::PGrad pg{};
       pg.rowptr = f->rowptr;
       pg.col = f->col;
-      pg.E = f->dA;
+      pg.E = Edev;
       pg.X = f->sb.Xc;
       pg.Z = f->sg.Xc;
       pg.c = f->c;
@@ -943,27 +980,12 @@ gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
       A_HIP(hipStreamSynchronize(st));   // dA_values[q] and out are host memory
       double gt = out[0];
       if (K) {
-        // E Q (K SpMVs with E's values on A's pattern)
-        for (int k = 0; k < K; ++k) {
-          gmb::BSpmm a{};
-          a.rowptr = f->rowptr;
-          a.col = f->col;
-          a.val = f->dA;
-          a.nrows = f->n;
-          a.chunk = f->chunk;
-          a.x = f->Q + (size_t)k * N;
-          a.ldx = 1;
-          a.y = f->EQ + (size_t)k * N;
-          a.ldy = 1;
-          a.g0 = 1.0;
-          a.ca = 1.0;
-          hipLaunchKernelGGL((gmb::k_bspmm<1, gmb::kCheb>), dim3(f->G), dim3(gm::kBlock), 0, st, a);
-          A_HIP(hipGetLastError());
-        }
-        std::vector<double> QEQ, Mb, Mg;
-        A_TRY(dots(f, f->Q, N, K, f->EQ, N, K, f->dsum, &QEQ));
-        A_TRY(dots(f, f->sb.V, N, m, f->EQ, N, K, f->dsum, &Mb));
-        A_TRY(dots(f, f->sg.V, N, m, f->EQ, N, K, f->dsum, &Mg));
+        std::vector<double> QEQt, Mb, Mg;
+        const double* EQ = stored ? f->pEQ[q] : f->EQ;
+        if (!stored) A_TRY(eq_products(f, f->dA, f->EQ, QEQt));
+        const std::vector<double>& QEQ = stored ? f->pQEQ[q] : QEQt;
+        A_TRY(dots(f, f->sb.V, N, m, EQ, N, K, f->dsum, &Mb));
+        A_TRY(dots(f, f->sg.V, N, m, EQ, N, K, f->dsum, &Mg));
         // Q-Q block: sum_kl (Q^T g)_k f[l_k, l_l] (Q^T E Q)_kl (Q^T b)_l
         for (int k = 0; k < K; ++k)
           for (int l = 0; l < K; ++l)
@@ -985,6 +1007,36 @@ gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
   unsigned stv = 0;
   A_HIP(hipMemcpy(&stv, f->status, sizeof(unsigned), hipMemcpyDeviceToHost));
   if (stv & 6u) return afail(GM_EBREAKDOWN, "Lanczos breakdown in the backward pass (%u)", stv);
+  return GM_OK;
+}
+
+gm_status gm_fab_set_param_derivs(gm_fab* f, int32_t nparams, const double* const* dA_values) {
+  if (!alive(f)) return afail(GM_EBADHANDLE, "invalid fab handle");
+  if (nparams < 0 || (nparams > 0 && !dA_values)) return afail(GM_EINVAL, "bad parameter arrays");
+  for (int q = 0; q < nparams; ++q)
+    if (!dA_values[q]) return afail(GM_EINVAL, "null dA/dtheta values for parameter %d", q);
+  A_HIP(hipSetDevice(f->dev));
+  A_HIP(hipStreamSynchronize(f->stream));
+  for (double* p : f->pdA) afree(p);
+  for (double* p : f->pEQ) afree(p);
+  f->pdA.clear();
+  f->pEQ.clear();
+  f->pQEQ.clear();
+  for (int q = 0; q < nparams; ++q) {
+    double* E = nullptr;
+    A_TRY(aalloc(&E, (size_t)f->nnz));
+    f->pdA.push_back(E);
+    A_HIP(hipMemcpyAsync(E, dA_values[q], (size_t)f->nnz * sizeof(double), hipMemcpyHostToDevice,
+                         f->stream));
+    if (f->K) {
+      double* EQ = nullptr;
+      A_TRY(aalloc(&EQ, (size_t)f->n * f->K));
+      f->pEQ.push_back(EQ);
+      f->pQEQ.emplace_back();
+      A_TRY(eq_products(f, E, EQ, f->pQEQ.back()));
+    }
+  }
+  A_HIP(hipStreamSynchronize(f->stream));
   return GM_OK;
 }
 

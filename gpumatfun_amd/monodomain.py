@@ -106,6 +106,8 @@ class MonodomainAdjoint:
         w = np.array(ubar_T, dtype=np.float64)
         hbar = np.zeros_like(w)
         gtheta = np.zeros(len(dA_list))
+        for eng in (self.f1, self.f2):      # dA/dtheta kept on the device for every step
+            eng.set_param_derivs(list(dA_list))
         for n in range(len(self.traj) - 1, -1, -1):
             u, h, g = self.traj[n]
             u_next = self.traj[n + 1][0] if n + 1 < len(self.traj) else self.uT
@@ -113,10 +115,10 @@ class MonodomainAdjoint:
             dDdh = np.where(u_next < p["v_gate"], -1.0 / p["tau_open"], -1.0 / p["tau_close"])
             hbar_prev = hbar * (1.0 + dt * dDdh)
             # u_{n+1} = f1(A) u_n + dt f2(A) g_n
-            self.f1.forward(u)
-            f1w, gt1 = self.f1.backward_param(w, dA_list, want_b=True)
-            self.f2.forward(g)
-            f2w, gt2 = self.f2.backward_param(w, dA_list, want_b=True)
+            self.f1.forward(u, want_u=False)
+            f1w, gt1 = self.f1.backward_param(w, None, want_b=True)
+            self.f2.forward(g, want_u=False)
+            f2w, gt2 = self.f2.backward_param(w, None, want_b=True)
             gtheta += gt1 + dt * gt2
             dGdu = h * (2.0 * u - 3.0 * u * u) / p["tau_in"] - 1.0 / p["tau_out"]
             dGdh = (u * u) * (1.0 - u) / p["tau_in"]

@@ -322,7 +322,8 @@ gm_status gm_fab_delete(gm_fab* f);
 gm_status gm_fab_set_matrix(gm_fab* f, const double* A_values);
 /* new weights (and their alpha derivatives), e.g. after alpha changes      */
 gm_status gm_fab_set_weights(gm_fab* f, const double* wts, const double* dwts);
-/* u = f(A) b, host vectors of length n                                     */
+/* u = f(A) b, host vectors of length n; u may be NULL to (re)build only the
+ * forward state a backward pass needs                                      */
 gm_status gm_fab_forward(gm_fab* f, const double* b, double* u);
 /* after a forward: any output pointer may be NULL to skip that gradient;
  * kappa (n) is needed for grad_kappa                                       */
@@ -340,6 +341,11 @@ gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, doubl
 gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
                                 const double* const* dA_values, double* grad_b,
                                 double* grad_theta);
+/* Keeps nparams dA/dtheta_q (A's pattern) on the device, with their products
+ * with Q under deflation; gm_fab_backward_param with dA_values = NULL then
+ * uses them instead of uploading the derivatives on every call (a time
+ * adjoint calls it twice per step with the same ones).                     */
+gm_status gm_fab_set_param_derivs(gm_fab* f, int32_t nparams, const double* const* dA_values);
 
 /* ---------------------------------------------------------------------------
  * Uneven batch (no reference counterpart): P independent problems of

@@ -244,6 +244,11 @@ def test_deflated_param_gradient_vs_exact_reference():
     ref_u = oadj.deflated_forward(A, b, c["xi"], cw, Q, f(lam))
     g = 2 * (u - y)
     gb, (gr,) = eng.backward_param(g, [E], want_b=True)
+    # the same with the derivative kept on the device: bitwise equal
+    eng.set_param_derivs([E])
+    eng.forward(b, want_u=False)
+    gb2, (gr2,) = eng.backward_param(g, None, want_b=True)
+    assert gr2 == gr and np.array_equal(gb2, gb)
     eng.delete()
     ref_gb = oadj.deflated_forward(A, g, c["xi"], cw, Q, f(lam))
     ref = oadj.deflated_param_gradient(A, E, b, g, c["xi"], cw, Q, lam, f(lam), F)
