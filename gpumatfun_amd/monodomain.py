@@ -13,8 +13,9 @@ deflation:
     D(u, h) = (1 - h)/tau_open if u < v_gate else -h/tau_close
 
 with f1 = exp(-eta x^(alpha/2)), f2 = phi(-eta x^(alpha/2)) on the contour of
-``problem_startup.m:85-103``.  For L = ||u_T - y||^2 the adjoint runs the
-steps backwards (A symmetric, so f(A)^T = f(A)):
+``problem_startup.m:85-103``.  For L = ||u_T - y||^2 (or any loss of the
+states, through per-step sources, e.g. a smooth activation-time map) the
+adjoint runs the steps backwards (A symmetric, so f(A)^T = f(A)):
 
     w       = ubar_{n+1}                       (D is piecewise constant in u)
     ubar_n  = f1(A) w + dG/du(u_n, h_n) * (dt f2(A) w)
@@ -45,6 +46,28 @@ def ms_g(u, h, p):
 def ms_gate(u, h, dt, p):
     dh = np.where(u < p["v_gate"], (1.0 - h) / p["tau_open"], -h / p["tau_close"])
     return h + dt * dh
+
+
+def activation_times(us, dt, v_thr=0.5, eps=0.05):
+    """Smooth activation-time map of a run u_1 .. u_T: t(x) = dt sum_n s(u_n(x)),
+    s(u) = 1 - sigma((u - v_thr) / eps), the time a node spends below v_thr (its
+    activation time when it activates once within the window).  Returns t and, for
+    dL/dt = r, the direct sources dL/du_n = r * dt * s'(u_n) via ``sources(r)``."""
+    def sig(z):
+        return 0.5 * (1.0 + np.tanh(0.5 * z))
+
+    t = np.zeros_like(us[0])
+    for u in us:
+        t += dt * (1.0 - sig((u - v_thr) / eps))
+
+    def sources(r):
+        out = []
+        for u in us:
+            sg = sig((u - v_thr) / eps)
+            out.append(-r * dt * sg * (1.0 - sg) / eps)
+        return out
+
+    return t, sources
 
 
 class MonodomainAdjoint:
@@ -97,9 +120,17 @@ class MonodomainAdjoint:
         self.uT, self.hT = u, h
         return u
 
-    def backward(self, ubar_T, dA_list=()):
+    def states(self):
+        """u_1 .. u_T of the stored run (views, not copies)."""
+        if self.traj is None:
+            raise RuntimeError("no stored run")
+        return [t[0] for t in self.traj[1:]] + [self.uT]
+
+    def backward(self, ubar_T, dA_list=(), sources=None):
         """Adjoint of the stored run for dL/du_T = ubar_T: returns (dL/du_0,
-        dL/dh_0, [dL/dtheta_q]) for the given dA/dtheta_q."""
+        dL/dh_0, [dL/dtheta_q]) for the given dA/dtheta_q.  ``sources`` (a list
+        of T - 1 arrays, or None) adds a loss's direct dL/du_n for n = 1 .. T-1,
+        e.g. of a loss over the whole trajectory such as activation times."""
         if self.traj is None:
             raise RuntimeError("backward needs a forward run first")
         p, dt = self.p, self.dt
@@ -124,6 +155,8 @@ class MonodomainAdjoint:
             dGdh = (u * u) * (1.0 - u) / p["tau_in"]
             w = f1w + dGdu * (dt * f2w)
             hbar = hbar_prev + dGdh * (dt * f2w)
+            if sources is not None and n >= 1:
+                w = w + sources[n - 1]
         return w, hbar, gtheta
 
     def delete(self):

@@ -148,16 +148,18 @@ def deflated_param_gradient(A, E, b, g, xi, c, Q, lam, f_lambda, f_divdiff, part
     return (pp, qq, cross) if parts else pp + qq + cross
 
 
-def ms_forward(A, u0, h0, steps, xi, c1, c2, dt, p, defl=None):
+def ms_forward(A, u0, h0, steps, xi, c1, c2, dt, p, defl=None, states=False):
     """The monodomain Mitchell-Schaeffer steps with exact resolvents:
     u_{n+1} = Im sum_j c1_j x_j(u_n) + dt Im sum_j c2_j x_j(g_n) (the contour
     rule of problem_startup.m:95-103 for f1 and f2), then the gate and g from
     ``stepping.ms_reaction``; with defl = (Q, f1(Lambda), f2(Lambda)) each
-    product is the deflated split of ``deflated_forward``.  Returns (u_T, h_T)."""
+    product is the deflated split of ``deflated_forward``.  Returns (u_T, h_T), and
+    with states also the list u_1 .. u_T."""
     from .stepping import ms_reaction
     u = np.array(u0, dtype=np.float64)
     h = np.array(h0, dtype=np.float64)
     g = (h * (u * u) * (1.0 - u)) / p["tau_in"] - u / p["tau_out"]
+    us = []
     for _ in range(steps):
         if defl is None:
             u = forward(A, u, xi, c1)[0] + dt * forward(A, g, xi, c2)[0]
@@ -165,5 +167,6 @@ def ms_forward(A, u0, h0, steps, xi, c1, c2, dt, p, defl=None):
             Q, fl1, fl2 = defl
             u = deflated_forward(A, u, xi, c1, Q, fl1) + dt * deflated_forward(A, g, xi, c2, Q, fl2)
         h, g = ms_reaction(u, h, dt, p)
-    return u, h
+        us.append(u)
+    return (u, h, us) if states else (u, h)
 

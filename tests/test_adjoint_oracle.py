@@ -161,3 +161,26 @@ def test_divided_differences_and_derivatives():
         assert np.allclose(F, F.T, rtol=1e-14, atol=0)
         assert np.allclose(np.diag(F), dfn(x, eta, al), rtol=1e-14, atol=0)
         assert abs(F[0, 2] - (fn(x[0], eta, al) - fn(x[2], eta, al)) / (x[0] - x[2])) < 1e-18
+
+
+def test_activation_time_sources_match_finite_differences():
+    """The smooth activation-time map t = dt sum_n s(u_n) and its direct sources
+    dL/du_n for L = ||t - t_obs||^2, against central differences per entry."""
+    from gpumatfun_amd.monodomain import activation_times
+    rng = np.random.default_rng(2)
+    us = [rng.random(30) for _ in range(4)]
+    t_obs = rng.random(30) * 0.2
+    dt = 0.05
+    t, sources = activation_times(us, dt, v_thr=0.5, eps=0.05)
+    srcs = sources(2 * (t - t_obs))
+    for n in range(4):
+        for i in (0, 7, 19):
+            h = 1e-6
+            up = [u.copy() for u in us]
+            um = [u.copy() for u in us]
+            up[n][i] += h
+            um[n][i] -= h
+            Lp = np.sum((activation_times(up, dt)[0] - t_obs) ** 2)
+            Lm = np.sum((activation_times(um, dt)[0] - t_obs) ** 2)
+            fd = (Lp - Lm) / (2 * h)
+            assert abs(fd - srcs[n][i]) <= 1e-6 * abs(fd) + 1e-10, (n, i, fd, srcs[n][i])

@@ -92,3 +92,46 @@ def test_deflated_ratio_gradient_vs_finite_differences():
     fd = (loss(r + h)[0] - loss(r - h)[0]) / (2 * h)
     print("dL/dratio", gr, "fd", fd)
     assert abs(gr - fd) <= 1e-6 * abs(fd)
+
+
+def test_activation_time_loss_gradient_vs_finite_differences():
+    """The activation-map mismatch L = ||t - t_obs||^2 of the smooth activation
+    times (t = dt sum_n s(u_n)), differentiated through the deflated run by
+    per-step sources, against central differences of the oracle's deflated
+    exact-resolvent forward (eigenpairs recomputed per ratio).  t_obs is the
+    map at ratio 3."""
+    from gpumatfun_amd.monodomain import activation_times
+    n, steps, dt, r, m, h, K = 24, 10, 0.05, 4.0, 60, 1e-5, 6
+    eta, alpha = dt * 1e-3, 1.7
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    A = problems.anisotropic_9pt(n, ratio=r)
+    w, U = np.linalg.eigh(A.toarray())
+    c = q.contour_nodes(0.9 * w[K], 1.1 * w.max(), 24, eta, alpha)
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    u0 = ((X >= 3) & (X < 9) & (Y >= 4) & (Y < 10)).astype(float).reshape(-1, order="F")
+    h0 = np.ones(n * n)
+    p = dict(MS_DEFAULTS)
+
+    def oracle_states(rr):
+        Ar = problems.anisotropic_9pt(n, ratio=rr)
+        wr, Ur = np.linalg.eigh(Ar.toarray())
+        defl = (Ur[:, :K], q.f1(wr[:K], eta, alpha), q.f2(wr[:K], eta, alpha))
+        return oadj.ms_forward(Ar, u0, h0, steps, c["xi"], c["wts1"], c["wts2"], dt, p,
+                               defl=defl, states=True)[2]
+
+    t_obs, _ = activation_times(oracle_states(3.0), dt)
+    print(f"case activation-time adjoint n={n} K={K} steps={steps}", flush=True)
+    md = MonodomainAdjoint(A, m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"], dt=dt, ms=p,
+                           Q=U[:, :K], lam=w[:K], eta=eta, alpha=alpha)
+    md.forward(u0, h0, steps)
+    t, sources = activation_times(md.states(), dt)
+    srcs = sources(2 * (t - t_obs))
+    _, _, (gr,) = md.backward(srcs[-1], [E], sources=srcs[:-1])
+    md.delete()
+
+    def loss(rr):
+        return float(np.sum((activation_times(oracle_states(rr), dt)[0] - t_obs) ** 2))
+
+    fd = (loss(r + h) - loss(r - h)) / (2 * h)
+    print("dL/dratio", gr, "fd", fd, "L", float(np.sum((t - t_obs) ** 2)))
+    assert abs(gr - fd) <= 1e-6 * abs(fd)
