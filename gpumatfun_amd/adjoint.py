@@ -17,7 +17,32 @@ from . import problems, quadrature as q
 
 
 def _p(a):
-    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+    """A double* for a NumPy array or a float64 torch tensor on the GPU (the
+    gm_fab_* vector arguments take host or device memory)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        if a.dtype.itemsize != 8 or not a.is_contiguous():
+            raise ValueError("device vectors must be contiguous float64")
+        return C.cast(C.c_void_p(a.data_ptr()), C.POINTER(C.c_double))
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _vec(x):
+    """x as contiguous float64 (a GPU tensor is synchronised with torch's stream,
+    since the engine runs on its own)."""
+    if hasattr(x, "data_ptr"):
+        import torch
+        torch.cuda.current_stream().synchronize()
+        return x.contiguous()
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _empty_like(x, n):
+    if hasattr(x, "data_ptr"):
+        import torch
+        return torch.empty(n, dtype=torch.float64, device=x.device)
+    return np.empty(n)
 
 
 def _cplx(z):
@@ -103,9 +128,10 @@ class FabEngine:
                    None if dwts is None else _p(_cplx(dwts)))
 
     def forward(self, b, want_u=True):
-        """u = f(A) b; with want_u=False only the state a backward pass needs."""
-        b = np.ascontiguousarray(b, dtype=np.float64)
-        u = np.empty(self.n) if want_u else None
+        """u = f(A) b; with want_u=False only the state a backward pass needs.  A
+        float64 torch tensor on the engine's GPU in, one out (no host copies)."""
+        b = _vec(b)
+        u = _empty_like(b, self.n) if want_u else None
         self._call(self._lib.gm_fab_forward, _p(b), _p(u))
         return u
 
@@ -139,14 +165,14 @@ class FabEngine:
         """dL/dtheta_q = Im sum_j c_j z_j^T (dA/dtheta_q) x_j for each sparse dA/dtheta_q
         (A's sparsity pattern) after a forward pass; returns (grad_b or None, grads).
         ``dA_list=None`` uses the derivatives kept by ``set_param_derivs``."""
-        g = np.ascontiguousarray(g, dtype=np.float64)
+        g = _vec(g)
         if dA_list is None:
             n_p, arr = getattr(self, "_nstored", 0), None
         else:
             vals, arr = self._derivs(dA_list)
             n_p = len(vals)
         out = np.zeros(max(1, n_p))
-        gb = np.empty(self.n) if want_b else None
+        gb = _empty_like(g, self.n) if want_b else None
         self._call(self._lib.gm_fab_backward_param, _p(g), n_p, arr, _p(gb), _p(out))
         return gb, out[:n_p].copy()
 

@@ -39,12 +39,33 @@ from . import _lib
 from .adjoint import FabEngine
 
 
+def _xp(a):
+    """The array module of a: numpy, or torch for tensors."""
+    if hasattr(a, "data_ptr"):
+        import torch
+        return torch
+    return np
+
+
+def _where(cond, a, b, like):
+    """where(cond, a, b) with a, b arrays or Python floats, in like's dtype (torch
+    would otherwise turn float scalars into its default float32)."""
+    if _xp(like) is np:
+        return np.where(cond, a, b)
+    import torch
+
+    def t(v):
+        return v if hasattr(v, "data_ptr") else torch.tensor(v, dtype=like.dtype,
+                                                            device=like.device)
+    return torch.where(cond, t(a), t(b))
+
+
 def ms_g(u, h, p):
     return (h * (u * u) * (1.0 - u)) / p["tau_in"] - u / p["tau_out"]
 
 
 def ms_gate(u, h, dt, p):
-    dh = np.where(u < p["v_gate"], (1.0 - h) / p["tau_open"], -h / p["tau_close"])
+    dh = _where(u < p["v_gate"], (1.0 - h) / p["tau_open"], -h / p["tau_close"], u)
     return h + dt * dh
 
 
@@ -53,10 +74,12 @@ def activation_times(us, dt, v_thr=0.5, eps=0.05):
     s(u) = 1 - sigma((u - v_thr) / eps), the time a node spends below v_thr (its
     activation time when it activates once within the window).  Returns t and, for
     dL/dt = r, the direct sources dL/du_n = r * dt * s'(u_n) via ``sources(r)``."""
-    def sig(z):
-        return 0.5 * (1.0 + np.tanh(0.5 * z))
+    xp = _xp(us[0])
 
-    t = np.zeros_like(us[0])
+    def sig(z):
+        return 0.5 * (1.0 + xp.tanh(0.5 * z))
+
+    t = xp.zeros_like(us[0])
     for u in us:
         t += dt * (1.0 - sig((u - v_thr) / eps))
 
@@ -78,11 +101,16 @@ class MonodomainAdjoint:
     """
 
     def __init__(self, A, *, m, xi, wts1, wts2, dt, ms=None, device=0, Q=None, lam=None,
-                 eta=None, alpha=None):
+                 eta=None, alpha=None, resident=False):
         """With ``Q``, ``lam`` (K eigenpairs of A) the solves are deflated as in
         the forward engines; ``eta`` and ``alpha`` then give the exact f1, f2 and
         their divided differences at ``lam`` (phi is taken exactly at every
-        eigenvalue, without the phi_lambda(1) = 1 guard of the set-up)."""
+        eigenvalue, without the phi_lambda(1) = 1 guard of the set-up).  With
+        ``resident`` the states, the trajectory and the reaction arithmetic stay
+        in GPU memory (float64 torch tensors), and the engines read and write
+        them there."""
+        self.resident = bool(resident)
+        self.device = int(device)
         self.p = dict(_lib.MS_DEFAULTS)
         self.p.update(ms or {})
         self.dt = float(dt)
@@ -106,10 +134,18 @@ class MonodomainAdjoint:
         self.f1.set_matrix(A)
         self.f2.set_matrix(A)
 
+    def _arr(self, x):
+        if self.resident:
+            import torch
+            return torch.as_tensor(np.asarray(x, dtype=np.float64) if not hasattr(x, "data_ptr")
+                                   else x, dtype=torch.float64,
+                                   device=f"cuda:{self.device}").clone()
+        return np.array(x, dtype=np.float64)
+
     def forward(self, u0, h0, steps):
         """Runs the steps; returns u_T and keeps (u_n, h_n, g_n) for the adjoint."""
-        u = np.array(u0, dtype=np.float64)
-        h = np.array(h0, dtype=np.float64)
+        u = self._arr(u0)
+        h = self._arr(h0)
         traj = []
         for _ in range(steps):
             g = ms_g(u, h, self.p)
@@ -134,8 +170,9 @@ class MonodomainAdjoint:
         if self.traj is None:
             raise RuntimeError("backward needs a forward run first")
         p, dt = self.p, self.dt
-        w = np.array(ubar_T, dtype=np.float64)
-        hbar = np.zeros_like(w)
+        w = self._arr(ubar_T)
+        xp = _xp(w)
+        hbar = xp.zeros_like(w)
         gtheta = np.zeros(len(dA_list))
         for eng in (self.f1, self.f2):      # dA/dtheta kept on the device for every step
             eng.set_param_derivs(list(dA_list))
@@ -143,7 +180,7 @@ class MonodomainAdjoint:
             u, h, g = self.traj[n]
             u_next = self.traj[n + 1][0] if n + 1 < len(self.traj) else self.uT
             # h_{n+1} = h_n + dt D(u_{n+1}, h_n): hbar flows to h_n, nothing to u_{n+1}
-            dDdh = np.where(u_next < p["v_gate"], -1.0 / p["tau_open"], -1.0 / p["tau_close"])
+            dDdh = _where(u_next < p["v_gate"], -1.0 / p["tau_open"], -1.0 / p["tau_close"], w)
             hbar_prev = hbar * (1.0 + dt * dDdh)
             # u_{n+1} = f1(A) u_n + dt f2(A) g_n
             self.f1.forward(u, want_u=False)

@@ -322,8 +322,11 @@ gm_status gm_fab_delete(gm_fab* f);
 gm_status gm_fab_set_matrix(gm_fab* f, const double* A_values);
 /* new weights (and their alpha derivatives), e.g. after alpha changes      */
 gm_status gm_fab_set_weights(gm_fab* f, const double* wts, const double* dwts);
-/* u = f(A) b, host vectors of length n; u may be NULL to (re)build only the
- * forward state a backward pass needs                                      */
+/* u = f(A) b, vectors of length n; u may be NULL to (re)build only the
+ * forward state a backward pass needs.  In the gm_fab_* calls the vectors
+ * b, u, g, grad_b, kappa and grad_kappa may be host memory or device memory
+ * of the engine's GPU (copies by unified addressing); the calls return with
+ * the engine's stream synchronised.                                        */
 gm_status gm_fab_forward(gm_fab* f, const double* b, double* u);
 /* after a forward: any output pointer may be NULL to skip that gradient;
  * kappa (n) is needed for grad_kappa                                       */

@@ -135,3 +135,39 @@ def test_activation_time_loss_gradient_vs_finite_differences():
     fd = (loss(r + h) - loss(r - h)) / (2 * h)
     print("dL/dratio", gr, "fd", fd, "L", float(np.sum((t - t_obs) ** 2)))
     assert abs(gr - fd) <= 1e-6 * abs(fd)
+
+
+def test_resident_run_matches_host_run():
+    """The device-resident mode (states, trajectory and reaction arithmetic as
+    float64 torch tensors on the GPU, engines reading and writing GPU memory)
+    against the host mode: the same engines and arithmetic, so the end state
+    and the gradients agree to rounding of the elementwise reaction terms."""
+    import torch
+    n, steps, dt, r, m, K = 24, 6, 0.05, 4.0, 60, 6
+    eta, alpha = dt * 1e-3, 1.7
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    A = problems.anisotropic_9pt(n, ratio=r)
+    w, U = np.linalg.eigh(A.toarray())
+    c = q.contour_nodes(0.9 * w[K], 1.1 * w.max(), 24, eta, alpha)
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    u0 = ((X >= 3) & (X < 9) & (Y >= 4) & (Y < 10)).astype(float).reshape(-1, order="F")
+    h0 = np.ones(n * n)
+    y = 0.3 * np.random.default_rng(0).random(n * n)
+    out = {}
+    for resident in (False, True):
+        print(f"case resident={resident}", flush=True)
+        md = MonodomainAdjoint(A, m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"], dt=dt,
+                               Q=U[:, :K], lam=w[:K], eta=eta, alpha=alpha, resident=resident)
+        uT = md.forward(u0, h0, steps)
+        if resident:
+            assert isinstance(uT, torch.Tensor) and uT.is_cuda and uT.dtype == torch.float64
+            uT = uT.cpu().numpy()
+        ub0, hb0, (gr,) = md.backward(2 * (uT - y), [E])
+        if resident:
+            ub0, hb0 = ub0.cpu().numpy(), hb0.cpu().numpy()
+        md.delete()
+        out[resident] = (uT, ub0, hb0, gr)
+    a, b = out[False], out[True]
+    print("uT", rel(b[0], a[0]), "ubar0", rel(b[1], a[1]), "dL/dr", a[3], b[3])
+    assert rel(b[0], a[0]) < 1e-13 and rel(b[1], a[1]) < 1e-12 and rel(b[2], a[2]) < 1e-12
+    assert abs(b[3] - a[3]) <= 1e-12 * abs(a[3])
