@@ -329,7 +329,8 @@ gm_status gm_fab_set_weights(gm_fab* f, const double* wts, const double* dwts);
  * the engine's stream synchronised.                                        */
 gm_status gm_fab_forward(gm_fab* f, const double* b, double* u);
 /* after a forward: any output pointer may be NULL to skip that gradient;
- * kappa (n) is needed for grad_kappa                                       */
+ * kappa (n) is needed for grad_kappa.  With deflation (num_eigs > 0) only
+ * grad_b is available here (GM_EINVAL for grad_kappa / grad_alpha).        */
 gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, double* grad_b,
                           double* grad_kappa, double* grad_alpha);
 /* after a forward: gradients with respect to nparams scalar parameters theta_q
