@@ -23,7 +23,7 @@ Workloads (``--config``):
 
 * ``msadj512`` -- the gradient of the ``ms512`` run's activation-map mismatch
   with respect to the conductivity ratio, through 1000 deflated time steps of
-  one stimulus (the adjoint of the monodomain time stepping, deflation reused).
+  its 16 stimuli (the adjoint of the monodomain time stepping, deflation reused).
 
 (``uneven64``, ``adjoint256``, ``ms512`` and ``msadj512`` are described at their
 builders.)
@@ -513,27 +513,37 @@ def run_eigs512(args):
 
 
 def run_msadj512(args, steps):
-    """dL/dratio for the activation-map mismatch L = ||t - t_obs||^2 on the 512^2
-    Mitchell-Schaeffer monodomain (alpha = 1.7, dt = 0.05, m = 40, 32 nodes, K = 20
-    deflation pairs reused every step, eigenpairs from the GPU eigensolver), through
-    `steps` steps of one stimulus: a 20 x 20 corner patch.  t is the smooth
-    activation-time map dt sum_n s(u_n) (threshold 0.5, width 0.05); t_obs is the
-    map of a run at ratio 8 (a synthetic observation), and the gradient is taken at
-    ratio 9.  A 100-step run of the same code is checked against central
-    differences (step 1e-2) of the HIP forward with the eigenpairs recomputed at
-    each perturbed ratio."""
+    """dL/dratio for the activation-map mismatch L = sum_p ||t_p - t_obs,p||^2 on the
+    512^2 Mitchell-Schaeffer monodomain (alpha = 1.7, dt = 0.05, m = 40, 32 nodes,
+    K = 20 deflation pairs reused every step, eigenpairs from the GPU eigensolver),
+    through `steps` steps of P stimuli (the 16 patches of ms512, 20 x 20 each; P = 1
+    is a corner patch).  t_p is the smooth activation-time map dt sum_n s(u_n)
+    (threshold 0.5, width 0.05); t_obs,p the map of a run at ratio 8 (a synthetic
+    observation); the gradient is taken at ratio 9.  States and trajectories stay
+    in HBM; the stimuli run concurrently on 4 host threads.  A run of the same code
+    over min(steps, 100) steps is checked against central differences (step 1e-2)
+    of the HIP forward with the eigenpairs recomputed at each perturbed ratio."""
     import scipy.sparse.linalg as sla
     from gpumatfun_amd import problems, quadrature as q
     from gpumatfun_amd.eigs import eigs_smallest
-    from gpumatfun_amd.monodomain import MonodomainAdjoint, activation_times
+    from gpumatfun_amd.monodomain import StimulusSetAdjoint, activation_times
     from gpumatfun_amd._lib import MS_DEFAULTS
+    import torch
     t0 = time.time()
     n, K, dt, m, ns, r0, r_obs = 512, 20, 0.05, args.adj_m, 32, 9.0, 8.0
+    P, workers = args.stimuli, 4
     eta, alpha = dt * 1e-3, 1.7
     E = problems.anisotropic_9pt_ratio_derivative(n)
     N = n * n
     X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
-    u0 = ((X < 20) & (Y < 20)).astype(float).reshape(-1, order="F")
+    if P == 1:
+        u0s = [((X < 20) & (Y < 20)).astype(float).reshape(-1, order="F")]
+    else:
+        u0s = []
+        for k in range(P):
+            cx, cy = 40 + 120 * (k % 4), 40 + 120 * (k // 4)
+            u0s.append(((X >= cx) & (X < cx + 20) & (Y >= cy) & (Y < cy + 20))
+                       .astype(float).reshape(-1, order="F"))
     h0 = np.ones(N)
     prm = dict(MS_DEFAULTS)
     nfd = min(steps, 100)
@@ -544,60 +554,71 @@ def run_msadj512(args, steps):
         if c is None:
             lN = float(sla.eigsh(A, k=1, which="LA", return_eigenvectors=False)[0])
             c = q.contour_nodes(0.98 * lam[K], 1.02 * lN, ns, eta, alpha)
-        md = MonodomainAdjoint(A, m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"], dt=dt, ms=prm,
-                               Q=Q[:, :K], lam=lam[:K], eta=eta, alpha=alpha,
-                               resident=not args.host_adjoint)
-        return md, c
+        st = StimulusSetAdjoint(A, P, workers=workers, m=m, xi=c["xi"], wts1=c["wts1"],
+                                wts2=c["wts2"], dt=dt, ms=prm, Q=Q[:, :K], lam=lam[:K],
+                                eta=eta, alpha=alpha)
+        return st, c
 
-    def tmap(md, nsteps):
-        md.forward(u0, h0, nsteps)
-        return activation_times(md.states(), dt)
+    def tmaps(st, nsteps):
+        st.forward(u0s, h0, nsteps)
+        return [activation_times(st.states(p), dt) for p in range(P)]
 
-    md_obs, _ = engine(r_obs)
-    t_obs = tmap(md_obs, steps)[0]
-    t_obs_fd = tmap(md_obs, nfd)[0]
-    md_obs.delete()
-    md, c = engine(r0)
+    def loss(ts, obs):
+        return float(sum(((t - o) ** 2).sum() for (t, _), o in zip(ts, obs)))
+
+    def grad(st, ts, obs):
+        srcs = [so(2 * (t - o)) for (t, so), o in zip(ts, obs)]
+        return st.backward([s_[-1] for s_ in srcs], [E], sources=[s_[:-1] for s_ in srcs])[2][0]
+
+    st_obs, _ = engine(r_obs)
+    t_obs = [t for t, _ in tmaps(st_obs, steps)]
+    t_obs_fd = [t for t, _ in tmaps(st_obs, nfd)]
+    st_obs.delete()
+    del st_obs
+    st, c = engine(r0)
     setup_s = time.time() - t0
-    log(f"set-up {setup_s:.1f}s (eigenpairs on the GPU, observation at ratio {r_obs})")
+    log(f"set-up {setup_s:.1f}s (eigenpairs on the GPU, observation at ratio {r_obs}, "
+        f"{P} stimuli)")
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
-    md.forward(u0, h0, steps)
+    st.forward(u0s, h0, steps)
+    torch.cuda.synchronize()
     fwd_s = time.perf_counter() - t1
-    t, sources = activation_times(md.states(), dt)
-    L = float(((t - t_obs) ** 2).sum())
+    log(f"forward {fwd_s:.2f}s")
+    ts = [activation_times(st.states(p), dt) for p in range(P)]
+    L = loss(ts, t_obs)
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
-    srcs = sources(2 * (t - t_obs))
-    _, _, (gr,) = md.backward(srcs[-1], [E], sources=srcs[:-1])
+    gr = grad(st, ts, t_obs)
+    torch.cuda.synchronize()
     bwd_s = time.perf_counter() - t1
-    del srcs
-    activated = float((t < (steps * dt - 1e-3)).sum()) / N
-    log(f"{steps} steps: forward {fwd_s:.2f}s, backward {bwd_s:.2f}s; L = {L:.6e}, "
-        f"dL/dratio = {gr:.6e}; {100 * activated:.1f}% of nodes activated")
+    activated = float(sum(float((t < (steps * dt - 1e-3)).sum()) for t, _ in ts)) / (N * P)
+    log(f"{steps} steps x {P} stimuli: forward {fwd_s:.2f}s, backward {bwd_s:.2f}s; "
+        f"L = {L:.6e}, dL/dratio = {gr:.6e}; {100 * activated:.1f}% of node-stimuli activated")
     # the same code over nfd steps against central differences of the HIP forward
-    t, sources = tmap(md, nfd)
-    srcs = sources(2 * (t - t_obs_fd))
-    _, _, (gfd,) = md.backward(srcs[-1], [E], sources=srcs[:-1])
-    md.delete()
+    gfd = grad(st, tmaps(st, nfd), t_obs_fd)
+    st.delete()
     hh = 1e-2
     Ls = []
     for r in (r0 + hh, r0 - hh):
-        mdr, _ = engine(r, c)
-        Ls.append(float(((tmap(mdr, nfd)[0] - t_obs_fd) ** 2).sum()))
-        mdr.delete()
+        str_, _ = engine(r, c)
+        Ls.append(loss(tmaps(str_, nfd), t_obs_fd))
+        str_.delete()
     fd = (Ls[0] - Ls[1]) / (2 * hh)
     log(f"{nfd} steps: dL/dratio {gfd:.10e}, central difference {fd:.10e}, "
         f"rel {abs(gfd - fd) / abs(fd):.2e}")
-    cfg = {"workload": "msadj_aniso512_1stim", "n": N, "nnz": int(E.nnz), "steps": steps,
-           "m": m, "shifts": ns, "deflation": K, "alpha": alpha, "dt": dt, "eta": eta,
-           "reaction": "Mitchell-Schaeffer", "stimulus": "20 x 20 corner patch",
-           "ratio": r0, "observation_ratio": r_obs, "resident": not args.host_adjoint,
-           "loss": "||t - t_obs||^2, t = dt sum_n (1 - sigmoid((u_n - 0.5) / 0.05))"}
+    cfg = {"workload": f"msadj_aniso512_{P}stim", "n": N, "nnz": int(E.nnz), "steps": steps,
+           "stimuli": P, "m": m, "shifts": ns, "deflation": K, "alpha": alpha, "dt": dt,
+           "eta": eta, "reaction": "Mitchell-Schaeffer",
+           "stimulus_patches": "20 x 20" + (" corner" if P == 1 else ", the ms512 layout"),
+           "ratio": r0, "observation_ratio": r_obs, "host_threads": workers,
+           "loss": "sum_p ||t_p - t_obs,p||^2, t = dt sum_n (1 - sigmoid((u_n - 0.5) / 0.05))"}
     result = {
         "metric": "adjoint cost vs forward (FP64, monodomain conductivity-ratio gradient)",
         "value": round(bwd_s / fwd_s, 3), "unit": "x forward", "n_gpus": 1, "steps": steps,
         "warmup": 0, "ms_per_step": round(1e3 * (fwd_s + bwd_s) / steps, 3),
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (stimulus patch; observation simulated at ratio 8)", "config": cfg,
+        "data": "synthetic (stimulus patches; observation simulated at ratio 8)", "config": cfg,
         "setup_s": round(setup_s, 1), "forward_s": round(fwd_s, 3), "backward_s": round(bwd_s, 3),
         "loss": L, "dL_dratio": gr, "activated_fraction": activated,
         "fd_check": {"steps": nfd, "adjoint": gfd, "central_difference": fd, "step": hh,
@@ -715,8 +736,8 @@ def main():
     ap.add_argument("--schedule", default="auto", choices=["auto", "rows", "blocks", "blocks16"],
                     help="SpMM schedule of the plain path: autotuned at create (default), "
                          "or forced (fixed kernels for counter runs)")
-    ap.add_argument("--host-adjoint", action="store_true",
-                    help="msadj512: keep the states on the host instead of in GPU memory")
+    ap.add_argument("--stimuli", type=int, default=16,
+                    help="msadj512: number of stimulus patches (16 as in ms512)")
     ap.add_argument("--adj-m", type=int, default=40,
                     help="msadj512: Lanczos steps of the adjoint's solves (40 as in ms512)")
     ap.add_argument("--profile-only", action="store_true",

@@ -83,14 +83,30 @@ def activation_times(us, dt, v_thr=0.5, eps=0.05):
     for u in us:
         t += dt * (1.0 - sig((u - v_thr) / eps))
 
+    def one(u, r):
+        sg = sig((u - v_thr) / eps)
+        return -r * dt * sg * (1.0 - sg) / eps
+
     def sources(r):
-        out = []
-        for u in us:
-            sg = sig((u - v_thr) / eps)
-            out.append(-r * dt * sg * (1.0 - sg) / eps)
-        return out
+        return _Lazy(us, lambda u: one(u, r))
 
     return t, sources
+
+
+class _Lazy:
+    """A read-only sequence f(items[i]) evaluated on access (slices stay lazy), so
+    a loss's per-step sources take no memory until the backward pass reads them."""
+
+    def __init__(self, items, f):
+        self.items, self.f = items, f
+
+    def __len__(self):
+        return len(self.items)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return _Lazy(self.items[i], self.f)
+        return self.f(self.items[i])
 
 
 class MonodomainAdjoint:
@@ -199,3 +215,60 @@ class MonodomainAdjoint:
     def delete(self):
         self.f1.delete()
         self.f2.delete()
+        self.traj = None
+
+
+class StimulusSetAdjoint:
+    """Several stimuli of one operator: one device-resident MonodomainAdjoint per
+    stimulus, run concurrently on ``workers`` host threads, each issuing on its
+    own torch stream and its engines' streams, so that the small single-column
+    kernels of different stimuli overlap on the GPU.  The loss is the sum over
+    stimuli, so the parameter gradient is the sum of the per-stimulus ones."""
+
+    def __init__(self, A, n_stimuli, *, workers=4, **kw):
+        import torch
+        kw = dict(kw)
+        kw["resident"] = True
+        self.runs = [MonodomainAdjoint(A, **kw) for _ in range(n_stimuli)]
+        self.workers = max(1, min(int(workers), n_stimuli))
+        dev = f"cuda:{int(kw.get('device', 0))}"
+        self.streams = [torch.cuda.Stream(device=dev) for _ in range(self.workers)]
+
+    def _map(self, fn):
+        """fn(p, run) for every stimulus, stimulus p on worker p % workers (in
+        order within a worker); returns the results in stimulus order."""
+        import torch
+        from concurrent.futures import ThreadPoolExecutor
+        torch.cuda.current_stream().synchronize()   # the caller's inputs are ready
+        out = [None] * len(self.runs)
+
+        def work(w):
+            with torch.cuda.stream(self.streams[w]):
+                for p in range(w, len(self.runs), self.workers):
+                    out[p] = fn(p, self.runs[p])
+                torch.cuda.current_stream().synchronize()
+
+        if self.workers == 1:
+            work(0)
+        else:
+            with ThreadPoolExecutor(self.workers) as ex:
+                for f in [ex.submit(work, w) for w in range(self.workers)]:
+                    f.result()
+        return out
+
+    def forward(self, u0s, h0, steps):
+        """u_T of every stimulus (u0s: one initial state per stimulus)."""
+        return self._map(lambda p, r: r.forward(u0s[p], h0, steps))
+
+    def states(self, p):
+        return self.runs[p].states()
+
+    def backward(self, ubar_Ts, dA_list=(), sources=None):
+        """Sum over stimuli of dL/dtheta (and the per-stimulus dL/du_0, dL/dh_0)."""
+        res = self._map(lambda p, r: r.backward(ubar_Ts[p], dA_list,
+                                                None if sources is None else sources[p]))
+        return [r[0] for r in res], [r[1] for r in res], sum(r[2] for r in res)
+
+    def delete(self):
+        for r in self.runs:
+            r.delete()

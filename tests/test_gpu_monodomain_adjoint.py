@@ -171,3 +171,41 @@ def test_resident_run_matches_host_run():
     print("uT", rel(b[0], a[0]), "ubar0", rel(b[1], a[1]), "dL/dr", a[3], b[3])
     assert rel(b[0], a[0]) < 1e-13 and rel(b[1], a[1]) < 1e-12 and rel(b[2], a[2]) < 1e-12
     assert abs(b[3] - a[3]) <= 1e-12 * abs(a[3])
+
+
+def test_stimulus_set_concurrent_matches_sequential():
+    """Three stimuli run concurrently on two host threads (own torch and engine
+    streams) give bitwise the per-stimulus results of sequential runs, and the
+    set's gradient is their sum."""
+    import torch
+    from gpumatfun_amd.monodomain import StimulusSetAdjoint
+    n, steps, dt, r, m, K = 24, 4, 0.05, 4.0, 60, 6
+    eta, alpha = dt * 1e-3, 1.7
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    A = problems.anisotropic_9pt(n, ratio=r)
+    w, U = np.linalg.eigh(A.toarray())
+    c = q.contour_nodes(0.9 * w[K], 1.1 * w.max(), 24, eta, alpha)
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    u0s = [((X >= a) & (X < a + 5) & (Y >= b) & (Y < b + 5)).astype(float).reshape(-1, order="F")
+           for a, b in ((2, 3), (12, 5), (7, 15))]
+    h0 = np.ones(n * n)
+    y = 0.3 * np.random.default_rng(0).random(n * n)
+    kw = dict(m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"], dt=dt, Q=U[:, :K], lam=w[:K],
+              eta=eta, alpha=alpha)
+    print("case stimulus set P=3 workers=2", flush=True)
+    seq = []
+    for u0 in u0s:
+        md = MonodomainAdjoint(A, resident=True, **kw)
+        uT = md.forward(u0, h0, steps)
+        ub, hb, (gr,) = md.backward(2 * (uT - torch.as_tensor(y, device=uT.device)), [E])
+        seq.append((uT.cpu().numpy(), ub.cpu().numpy(), gr))
+        md.delete()
+    st = StimulusSetAdjoint(A, len(u0s), workers=2, **kw)
+    uTs = st.forward(u0s, h0, steps)
+    yt = torch.as_tensor(y, device=uTs[0].device)
+    ubs, hbs, (gsum,) = st.backward([2 * (u - yt) for u in uTs], [E])
+    st.delete()
+    for p in range(len(u0s)):
+        assert np.array_equal(uTs[p].cpu().numpy(), seq[p][0])
+        assert np.array_equal(ubs[p].cpu().numpy(), seq[p][1])
+    assert gsum == sum(s_[2] for s_ in seq) or abs(gsum - sum(s_[2] for s_ in seq)) <= 1e-15 * abs(gsum)
