@@ -520,13 +520,15 @@ def run_msadj512(args, steps):
     is a corner patch).  t_p is the smooth activation-time map dt sum_n s(u_n)
     (threshold 0.5, width 0.05); t_obs,p the map of a run at ratio 8 (a synthetic
     observation); the gradient is taken at ratio 9.  States and trajectories stay
-    in HBM; the stimuli run concurrently on 4 host threads.  A run of the same code
+    in HBM; all stimuli share one multi-column engine (or, with --adjoint-engine
+    streams, one engine each on 4 concurrent host threads).  A run of the same code
     over min(steps, 100) steps is checked against central differences (step 1e-2)
     of the HIP forward with the eigenpairs recomputed at each perturbed ratio."""
     import scipy.sparse.linalg as sla
     from gpumatfun_amd import problems, quadrature as q
     from gpumatfun_amd.eigs import eigs_smallest
-    from gpumatfun_amd.monodomain import StimulusSetAdjoint, activation_times
+    from gpumatfun_amd.monodomain import (MonodomainBatchAdjoint, StimulusSetAdjoint,
+                                          activation_times)
     from gpumatfun_amd._lib import MS_DEFAULTS
     import torch
     t0 = time.time()
@@ -554,9 +556,12 @@ def run_msadj512(args, steps):
         if c is None:
             lN = float(sla.eigsh(A, k=1, which="LA", return_eigenvectors=False)[0])
             c = q.contour_nodes(0.98 * lam[K], 1.02 * lN, ns, eta, alpha)
-        st = StimulusSetAdjoint(A, P, workers=workers, m=m, xi=c["xi"], wts1=c["wts1"],
-                                wts2=c["wts2"], dt=dt, ms=prm, Q=Q[:, :K], lam=lam[:K],
-                                eta=eta, alpha=alpha)
+        kw = dict(m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"], dt=dt, ms=prm,
+                  Q=Q[:, :K], lam=lam[:K], eta=eta, alpha=alpha)
+        if args.adjoint_engine == "batch":
+            st = MonodomainBatchAdjoint(A, P, **kw)
+        else:
+            st = StimulusSetAdjoint(A, P, workers=workers, **kw)
         return st, c
 
     def tmaps(st, nsteps):
@@ -568,7 +573,11 @@ def run_msadj512(args, steps):
 
     def grad(st, ts, obs):
         srcs = [so(2 * (t - o)) for (t, so), o in zip(ts, obs)]
-        return st.backward([s_[-1] for s_ in srcs], [E], sources=[s_[:-1] for s_ in srcs])[2][0]
+        if args.adjoint_engine == "batch":
+            ubar = torch.stack([s_[len(s_) - 1] for s_ in srcs], dim=1)
+        else:
+            ubar = [s_[len(s_) - 1] for s_ in srcs]
+        return st.backward(ubar, [E], sources=[s_[:-1] for s_ in srcs])[2][0]
 
     st_obs, _ = engine(r_obs)
     t_obs = [t for t, _ in tmaps(st_obs, steps)]
@@ -611,7 +620,7 @@ def run_msadj512(args, steps):
            "stimuli": P, "m": m, "shifts": ns, "deflation": K, "alpha": alpha, "dt": dt,
            "eta": eta, "reaction": "Mitchell-Schaeffer",
            "stimulus_patches": "20 x 20" + (" corner" if P == 1 else ", the ms512 layout"),
-           "ratio": r0, "observation_ratio": r_obs, "host_threads": workers,
+           "ratio": r0, "observation_ratio": r_obs, "engine": args.adjoint_engine,
            "loss": "sum_p ||t_p - t_obs,p||^2, t = dt sum_n (1 - sigmoid((u_n - 0.5) / 0.05))"}
     result = {
         "metric": "adjoint cost vs forward (FP64, monodomain conductivity-ratio gradient)",
@@ -736,6 +745,9 @@ def main():
     ap.add_argument("--schedule", default="auto", choices=["auto", "rows", "blocks", "blocks16"],
                     help="SpMM schedule of the plain path: autotuned at create (default), "
                          "or forced (fixed kernels for counter runs)")
+    ap.add_argument("--adjoint-engine", default="batch", choices=["batch", "streams"],
+                    help="msadj512: all stimuli in one multi-column engine, or one engine "
+                         "per stimulus on concurrent host threads")
     ap.add_argument("--stimuli", type=int, default=16,
                     help="msadj512: number of stimulus patches (16 as in ms512)")
     ap.add_argument("--adj-m", type=int, default=40,

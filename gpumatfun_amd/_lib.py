@@ -46,6 +46,8 @@ EXPORTS = (
     "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
     "gm_subspace_size",
     "gm_eigs_last_error", "gm_eigs_smallest", "gm_sym_eig",
+    "gm_bfab_last_error", "gm_bfab_create", "gm_bfab_delete", "gm_bfab_forward",
+    "gm_bfab_set_param_derivs", "gm_bfab_backward_param", "gm_bfab_info",
 )
 GM_FAB_RECOMPUTE = 1
 
@@ -201,6 +203,32 @@ class GmEigsDesc(C.Structure):
     ]
 
 
+class GmBfabDesc(C.Structure):
+    """Mirror of ``gm_bfab_desc``."""
+    _fields_ = [
+        ("n", C.c_int32),
+        ("sparse_format", C.c_int32),
+        ("nnz", C.c_int64),
+        ("A_values", _dp),
+        ("A_ptr", _ip),
+        ("A_ind", _ip),
+        ("columns", C.c_int32),
+        ("subspace_size", C.c_int32),
+        ("contour_size", C.c_int32),
+        ("xi", _dp),
+        ("wts1", _dp),
+        ("wts2", _dp),
+        ("num_eigs", C.c_int32),
+        ("Q", _dp),
+        ("lambda_", _dp),
+        ("f1_lambda", _dp),
+        ("f2_lambda", _dp),
+        ("f1_divdiff", _dp),
+        ("f2_divdiff", _dp),
+        ("device", C.c_int32),
+    ]
+
+
 class GpuMatfunError(RuntimeError):
     """An entry point returned a non-OK gm_status."""
 
@@ -289,6 +317,13 @@ def load():
         "gm_eigs_smallest": (C.c_int, [C.POINTER(GmEigsDesc), _dp, _dp, C.POINTER(C.c_int32),
                                        _dp]),
         "gm_sym_eig": (C.c_int, [C.c_int32, _dp, _dp, _dp]),
+        "gm_bfab_last_error": (C.c_char_p, []),
+        "gm_bfab_create": (C.c_int, [C.POINTER(GmBfabDesc), C.POINTER(vp)]),
+        "gm_bfab_delete": (C.c_int, [vp]),
+        "gm_bfab_forward": (C.c_int, [vp, _dp, _dp]),
+        "gm_bfab_set_param_derivs": (C.c_int, [vp, C.c_int32, C.POINTER(_dp)]),
+        "gm_bfab_backward_param": (C.c_int, [vp, _dp, _dp, _dp]),
+        "gm_bfab_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -298,10 +333,12 @@ def load():
     return lib
 
 
-def check(status, batch=False, fab=False, ubatch=False, eigs=False):
+def check(status, batch=False, fab=False, ubatch=False, eigs=False, bfab=False):
     if status != GM_OK:
         lib = load()
-        if eigs:
+        if bfab:
+            raw = lib.gm_bfab_last_error()
+        elif eigs:
             raw = lib.gm_eigs_last_error()
         elif ubatch:
             raw = lib.gm_ubatch_last_error()

@@ -255,3 +255,106 @@ try:
                     torch.tensor(ga, dtype=torch.float64, device=dev), None)
 except ImportError:   # pragma: no cover - torch is part of the image
     FabFunction = None
+
+
+class BatchFabEngine:
+    """f(A) on R columns at once with the parameter adjoint (C ABI ``gm_bfab_*``).
+
+    Columns pair like the batched engine's: even columns use ``wts1`` and
+    ``f1_lambda``, odd columns ``wts2`` and ``f2_lambda``.  Vectors are N x R
+    (row-major: NumPy C order, or float64 torch tensors on the GPU)."""
+
+    def __init__(self, A, *, R, m, xi, wts1, wts2, Q=None, lam=None, f1_lambda=None,
+                 f2_lambda=None, f1_divdiff=None, f2_divdiff=None, device=0):
+        lib = _lib.load()
+        self._lib = lib
+        self._h = C.c_void_p()
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        self.n, self.nnz, self.R = A.shape[0], A.nnz, int(R)
+        self._ptr = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        self._ind = np.ascontiguousarray(A.indices, dtype=np.int32)
+        keep = [np.ascontiguousarray(A.data, dtype=np.float64), _cplx(xi), _cplx(wts1),
+                _cplx(wts2)]
+        d = _lib.GmBfabDesc()
+        d.n = self.n
+        d.sparse_format = _lib.GM_SPARSE_CSR
+        d.nnz = self.nnz
+        d.A_values = _p(keep[0])
+        d.A_ptr = self._ptr.ctypes.data_as(C.POINTER(C.c_int32))
+        d.A_ind = self._ind.ctypes.data_as(C.POINTER(C.c_int32))
+        d.columns = self.R
+        d.subspace_size = int(m)
+        d.contour_size = int(np.size(xi))
+        d.xi, d.wts1, d.wts2 = _p(keep[1]), _p(keep[2]), _p(keep[3])
+        d.device = int(device)
+        if Q is not None and np.shape(Q)[1] > 0:
+            arrs = [np.asfortranarray(Q, dtype=np.float64),
+                    np.ascontiguousarray(lam, dtype=np.float64),
+                    np.ascontiguousarray(f1_lambda, dtype=np.float64),
+                    np.ascontiguousarray(f2_lambda, dtype=np.float64)]
+            dds = [None if x is None else np.asfortranarray(x, dtype=np.float64)
+                   for x in (f1_divdiff, f2_divdiff)]
+            keep += arrs + [x for x in dds if x is not None]
+            d.num_eigs = arrs[0].shape[1]
+            d.Q, d.lambda_, d.f1_lambda, d.f2_lambda = [_p(x) for x in arrs]
+            d.f1_divdiff, d.f2_divdiff = _p(dds[0]), _p(dds[1])
+        _lib.check(lib.gm_bfab_create(C.byref(d), C.byref(self._h)), bfab=True)
+        self._np = 0
+
+    def _call(self, fn, *args):
+        if not self._h:
+            raise _lib.GpuMatfunError(_lib.GM_EBADHANDLE, "engine has been deleted")
+        _lib.check(fn(self._h, *args), bfab=True)
+
+    def _mat(self, B):
+        if hasattr(B, "data_ptr"):
+            import torch
+            torch.cuda.current_stream().synchronize()
+            return B.contiguous()
+        return np.ascontiguousarray(B, dtype=np.float64)
+
+    def _out(self, B):
+        if hasattr(B, "data_ptr"):
+            import torch
+            return torch.empty((self.n, self.R), dtype=torch.float64, device=B.device)
+        return np.empty((self.n, self.R))
+
+    def forward(self, B, want_u=True):
+        B = self._mat(B)
+        U = self._out(B) if want_u else None
+        self._call(self._lib.gm_bfab_forward, _p(B), _p(U))
+        return U
+
+    def set_param_derivs(self, dA_list):
+        vals = []
+        for E in dA_list:
+            E = sp.csr_matrix(E)
+            E.sort_indices()
+            if E.nnz != self.nnz or not np.array_equal(E.indptr, self._ptr) or \
+                    not np.array_equal(E.indices, self._ind):
+                raise ValueError("dA/dtheta must have A's sparsity pattern")
+            vals.append(np.ascontiguousarray(E.data, dtype=np.float64))
+        arr = (C.POINTER(C.c_double) * max(1, len(vals)))(*[_p(v) for v in vals])
+        self._call(self._lib.gm_bfab_set_param_derivs, len(vals), arr)
+        self._np = len(vals)
+
+    def backward_param(self, G, want_b=True):
+        """(f(A) G or None, grad_theta [nparams][R]) after a forward pass."""
+        G = self._mat(G)
+        gb = self._out(G) if want_b else None
+        gt = np.zeros((max(1, self._np), self.R))
+        self._call(self._lib.gm_bfab_backward_param, _p(G), _p(gb), _p(gt))
+        return gb, gt[:self._np]
+
+    def delete(self):
+        if self._h:
+            _lib.check(self._lib.gm_bfab_delete(self._h), bfab=True)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self._lib.gm_bfab_delete(self._h)
+        except Exception:
+            pass

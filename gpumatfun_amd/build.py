@@ -13,7 +13,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("gm_solver.hip", "gm_batch.hip", "gm_adjoint.hip",
-                                                      "gm_ubatch.hip", "gm_eigs.hip", "gm_setup.cpp")]
+                                                      "gm_ubatch.hip", "gm_eigs.hip", "gm_bfab.hip",
+                                                      "gm_setup.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "gm_kernels.h"), os.path.join(ROOT, "include", "gpumatfun.h")]
 OUT = os.path.join(HERE, "libgpumatfun_amd.so")
 ARCH = os.environ.get("GPUMATFUN_ARCH", "gfx950")

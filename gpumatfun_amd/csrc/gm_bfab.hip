@@ -1,0 +1,890 @@
+// gm_bfab.hip -- f(A) applied to R columns at once, with the adjoint with
+// respect to parameters of A (no reference counterpart): the multi-column form
+// of gm_fab_* for problems that share A, the contour and the deflation pairs,
+// e.g. the stimuli of a monodomain run.  Columns are paired like the batched
+// engine's: even columns take the f1 weights and f1(Lambda), odd columns f2.
+//
+// Forward, per column c (deflated when K > 0):
+//   a = Q^T b, b <- b - Q a, Lanczos (m steps, R columns per launch), shifted
+//   solves y_s = beta0 w_s (xi_s - T)^{-1} e1 with the column's weights,
+//   u = V Im(sum_s y_s) + Q (f(Lambda) a).
+// Backward for an upstream G (R columns, same pairing): f(A) G by the same
+// steps, and for each parameter theta with E = dA/dtheta (A's pattern,
+// symmetric) and each column c the derivative of g^T f(A) b:
+//   P-P block   Im sum_s z_s^T E (w_s x_s),  x_s = V_b ytilde_b,s (weighted
+//               solves of the b side), z_s = V_g ytilde_g,s (unit weights);
+//   Q-Q block   (Q^T g)^T (F o Q^T E Q) (Q^T b), F the divided differences of f;
+//   cross       sum_k (Q^T g)_k q_k^T E V_b H_b[:, k] + (b <-> g), with
+//               H[:, k] = (lambda_k - T)^{-1} (f(lambda_k) beta0 e1 - C).
+// The resolvent blocks V Y (N x R x 2 n_s) run on FP64 MFMA, the E-weighted
+// contraction one wavefront per (row, column) with lanes over 2 n_s.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/gpumatfun.h"
+#include "gm_bkernels.h"
+
+namespace gmf {
+using namespace gm;
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Yg[c][k][2s + 0/1] = Re/Im y[c][s][k]; padding columns 0.
+struct PackY {
+  const double* yr;
+  const double* yi;
+  int R, m, ns, ncol;
+  double* Yg;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_bpack_y(PackY p) {
+  const long long total = (long long)p.R * p.m * p.ncol;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < total;
+       t += (long long)gridDim.x * 256) {
+    const int j = (int)(t % p.ncol);
+    const long long ck = t / p.ncol;
+    const int k = (int)(ck % p.m), c = (int)(ck / p.m);
+    const int s = j >> 1;
+    double v = 0.0;
+    if (s < p.ns) {
+      const size_t e = ((size_t)c * p.ns + s) * p.m + k;
+      v = (j & 1) ? p.yi[e] : p.yr[e];
+    }
+    p.Yg[t] = v;
+  }
+}
+
+// X[i][c][j] = sum_k V[k][i][c] Yg[c][k][j] on FP64 MFMA (v_mfma_f64_16x16x4f64):
+// block x = column c (so that the blocks of one row tile, which share V's cache
+// lines, run together), each wavefront one 16-row tile and 64 output columns.
+// Operand layout: A lane l = (row l%16, k l/16), B lane l = (k l/16, col l%16);
+// D register r of lane l = (row 4r + l/16, col l%16).
+struct BVY {
+  const double* V;    // [m][N][R]
+  const double* Y;    // [R][m][ncol]
+  int N, R, m, ncol;
+  double* X;          // [N][R][ncol]
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_bvy_mfma(BVY p) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x;
+  const int i0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * 16;
+  const int ct = blockIdx.z;
+  if (i0 >= p.N) return;   // whole wavefronts; no barrier follows
+  d4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+  const int ia = i0 + (lane & 15);
+  const size_t slice = (size_t)p.N * p.R;
+  for (int k0 = 0; k0 < p.m; k0 += 4) {
+    const int k = k0 + (lane >> 4);
+    const double a = (k < p.m && ia < p.N) ? p.V[(size_t)k * slice + (size_t)ia * p.R + c] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int j = ct * 64 + t * 16 + (lane & 15);
+      const double bv = k < p.m ? p.Y[((size_t)c * p.m + k) * p.ncol + j] : 0.0;
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = ct * 64 + t * 16 + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + 4 * r + (lane >> 4);
+      if (i < p.N) p.X[((size_t)i * p.R + c) * p.ncol + j] = acc[t][r];
+    }
+  }
+}
+
+// Per column c (block y): sum over rows i of Im sum_s Z[i][c][s] (E X)[i][c][s],
+// one wavefront per row, lanes over the ncol real columns (s = lane / 2, Re on
+// even lanes), the partner lane's value by a lane swap.  partials[c][Gx][2].
+struct BPGrad {
+  const int* rowptr;
+  const int* col;
+  const double* E;
+  const double* X;
+  const double* Z;
+  int N, R, ns, ncol;
+  double* partials;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_bparam_grad(BPGrad p) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.y;
+  double acc = 0.0;
+  for (int i = blockIdx.x * 4 + wave; i < p.N; i += gridDim.x * 4) {
+    const int e0 = p.rowptr[i], e1 = p.rowptr[i + 1];
+    for (int c0 = 0; c0 < p.ncol; c0 += 64) {
+      const int cc = c0 + lane;
+      double ex = 0.0;
+      for (int eb = e0; eb < e1; eb += 8) {
+        double ev[8], xv[8];
+        int ci[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const bool ok = eb + t < e1;
+          ev[t] = ok ? p.E[eb + t] : 0.0;
+          ci[t] = ok ? p.col[eb + t] : i;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) xv[t] = p.X[((size_t)ci[t] * p.R + c) * p.ncol + cc];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          if (eb + t < e1) ex = fma(ev[t], xv[t], ex);
+      }
+      const double z = p.Z[((size_t)i * p.R + c) * p.ncol + cc];
+      const double ex_o = __shfl_xor(ex, 1);
+      const double z_o = __shfl_xor(z, 1);
+      if (!(lane & 1) && (cc >> 1) < p.ns) acc += z * ex_o + z_o * ex;   // Im(z (E x))
+    }
+  }
+  block_partial2_at(d2(acc, 0.0), p.partials, c * gridDim.x + blockIdx.x);
+}
+
+// Block partials of M[j][k][c] = sum_i V[j][i][c] EQ[i][k] for every Krylov
+// vector j (block z) on FP64 MFMA: each wavefront one 16 x 16 tile of (k, c),
+// rows stepped by 4.  partials[G][m][K][R].
+struct BVtQ {
+  const double* V;    // [m][N][R]
+  const double* EQ;   // [N][K]
+  int K, R, m, nrows, chunk, KT, CT;
+  double* partials;
+};
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_bvtq(BVtQ p) {
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.y * kWaves + (threadIdx.x >> 6);
+  if (tile >= p.KT * p.CT) return;   // whole wavefronts; no barrier follows
+  const int j = blockIdx.z;
+  const int kt = tile % p.KT, ct = tile / p.KT;
+  const int row0 = blockIdx.x * p.chunk, row1 = min(row0 + p.chunk, p.nrows);
+  const int kk = kt * 16 + (lane & 15);
+  const int c = ct * 16 + (lane & 15);
+  const int sub = lane >> 4;
+  const double* Vj = p.V + (size_t)j * p.nrows * p.R;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int r = row0; r < row1; r += 4) {
+    const int row = r + sub;
+    const bool ok = row < row1;
+    const double a = (ok && kk < p.K) ? p.EQ[(size_t)row * p.K + kk] : 0.0;
+    const double b = (ok && c < p.R) ? Vj[(size_t)row * p.R + c] : 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k2 = kt * 16 + 4 * i + sub;
+    if (k2 < p.K && c < p.R)
+      p.partials[(((size_t)blockIdx.x * p.m + j) * p.K + k2) * p.R + c] = acc[i];
+  }
+}
+
+// y += x (n values).
+template <int kUnused = 0>
+__global__ __launch_bounds__(256) void k_add(const double* x, double* y, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)gridDim.x * 256)
+    y[t] += x[t];
+}
+
+}  // namespace gmf
+
+namespace {
+
+thread_local std::string g_ferr;
+std::mutex g_fmu;
+std::unordered_set<const void*> g_freg;
+
+gm_status ffail(gm_status st, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_ferr = buf;
+  return st;
+}
+
+#define F_HIP(call)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ffail(e_ == hipErrorOutOfMemory ? GM_ENOMEM : GM_EDEVICE, "%s: %s (%s:%d)", \
+                   #call, hipGetErrorString(e_), __FILE__, __LINE__);                    \
+  } while (0)
+#define F_TRY(x)              \
+  do {                        \
+    gm_status s_ = (x);       \
+    if (s_ != GM_OK) return s_; \
+  } while (0)
+
+inline int bdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+struct FSide {
+  double *U = nullptr, *V = nullptr, *W0 = nullptr, *W1 = nullptr, *al = nullptr, *be = nullptr,
+         *yr = nullptr, *yi = nullptr, *ur = nullptr, *ui = nullptr, *C = nullptr,
+         *out = nullptr, *Zd = nullptr, *upd = nullptr, *Yg = nullptr, *X = nullptr;
+};
+
+}  // namespace
+
+struct gm_bfab {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int n = 0, R = 0, m = 0, ns = 0, ncol = 0, K = 0;
+  long long nnz = 0;
+  bool have_forward = false;
+  int G_sp = 1, chunk_sp = 1, nrowblk = 0, G_df = 1, chunk_df = 1, G_pg = 1;
+  int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr;
+  double *val = nullptr, *xi = nullptr, *wts1 = nullptr, *wts2 = nullptr, *ones = nullptr,
+         *gsr = nullptr, *gsi = nullptr, *Q = nullptr, *el = nullptr, *pl = nullptr,
+         *partials = nullptr, *ppart = nullptr;
+  double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
+  unsigned* status = nullptr;
+  std::vector<double> lam, el_h, pl_h, dd1, dd2;   // host (dd K x K column-major)
+  std::vector<double*> pdA, pEQ;                    // stored parameter derivatives
+  std::vector<std::vector<double>> pQEQ;
+  FSide sb, sg;
+};
+
+namespace {
+
+bool falive(const gm_bfab* f) {
+  std::lock_guard<std::mutex> lk(g_fmu);
+  return f && g_freg.count(f);
+}
+
+template <typename T>
+gm_status falloc(T** p, size_t count) {
+  *p = nullptr;
+  F_HIP(hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)));
+  return GM_OK;
+}
+
+void ffree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+void free_bfab(gm_bfab* f) {
+  void* bufs[] = {f->rowptr, f->col, f->rowblk, f->val, f->xi, f->wts1, f->wts2, f->ones, f->gsr,
+                  f->gsi, f->Q, f->el, f->pl, f->partials, f->ppart, f->wD, f->wDU, f->wDU2,
+                  f->wX, f->status};
+  for (void* p : bufs) ffree(p);
+  for (double* p : f->pdA) ffree(p);
+  for (double* p : f->pEQ) ffree(p);
+  for (FSide* s : {&f->sb, &f->sg}) {
+    void* sbufs[] = {s->U, s->V, s->W0, s->W1, s->al, s->be, s->yr, s->yi, s->ur, s->ui, s->C,
+                     s->out, s->Zd, s->upd, s->Yg, s->X};
+    for (void* p : sbufs) ffree(p);
+  }
+  if (f->stream) (void)hipStreamDestroy(f->stream);
+}
+
+gm_status freduce(gm_bfab* f, int G, int nv, double* out, int do_sqrt, int flag_zero) {
+  gm::ReduceArgs r{};
+  r.partials = f->partials;
+  r.G = G;
+  r.nv = nv;
+  r.out = out;
+  r.stride = 1;
+  r.do_sqrt = do_sqrt;
+  r.flag_zero = flag_zero;
+  r.status = f->status;
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kReduceBlock), 0, f->stream, r);
+  F_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+// Q^T U -> s.Zd (K x R); U <- U - Q Zd; s.upd = Q (f(Lambda) Zd) by column parity.
+gm_status fdeflate(gm_bfab* f, FSide& s) {
+  gmb::BDeflQtU q{};
+  q.Q = f->Q;
+  q.U = s.U;
+  q.K = f->K;
+  q.R = f->R;
+  q.nrows = f->n;
+  q.chunk = f->chunk_df;
+  q.KT = bdiv(f->K, 16);
+  q.CT = bdiv(f->R, 16);
+  q.partials = f->partials;
+  hipLaunchKernelGGL(gmb::k_bdefl_qtu<0>, dim3(f->G_df, bdiv((long long)q.KT * q.CT, gm::kWaves)),
+                     dim3(gm::kBlock), 0, f->stream, q);
+  F_HIP(hipGetLastError());
+  F_TRY(freduce(f, f->G_df, f->K * f->R, s.Zd, 0, 0));
+  gmb::BDeflApply a{};
+  a.Q = f->Q;
+  a.Z = s.Zd;
+  a.el = f->el;
+  a.pl = f->pl;
+  a.Usol = s.U;
+  a.Ud = s.U;    // in place: each element is read and written by the same thread
+  a.upd = s.upd;
+  a.K = f->K;
+  a.R = f->R;
+  a.nrows = f->n;
+  hipLaunchKernelGGL(gmb::k_bdefl_apply<0>, dim3(std::min(4096, bdiv(f->n, gm::kWaves)), bdiv(f->R, 64)),
+                     dim3(gm::kBlock), 2 * (size_t)f->K * 64 * sizeof(double), f->stream, a);
+  F_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+// Lanczos on s.U (R columns), the weighted shifted solves (s.yr, s.yi), and
+// with want_out s.out = V C (+ s.upd); with unit also the unit-weight solves
+// (s.ur, s.ui).
+template <int R>
+gm_status fkrylov(gm_bfab* f, FSide& s, bool want_out, bool unit) {
+  const size_t N = (size_t)f->n;
+  gmb::BNorm nn{};
+  nn.u = s.U;
+  nn.ldu = R;
+  nn.nrows = f->n;
+  nn.chunk = f->chunk_sp;
+  nn.partials = f->partials;
+  nn.status = f->status;
+  hipLaunchKernelGGL((gmb::k_bnorm<R>), dim3(f->G_sp), dim3(gm::kBlock), 0, f->stream, nn);
+  F_HIP(hipGetLastError());
+  F_TRY(freduce(f, f->G_sp, R, s.be, 1, 0));
+  gmb::BSpmm base{};
+  base.rowptr = f->rowptr;
+  base.col = f->col;
+  base.val = f->val;
+  base.nrows = f->n;
+  base.chunk = f->chunk_sp;
+  base.rowblk = f->rowblk;
+  base.nrowblk = f->nrowblk;
+  base.al = s.al;
+  base.be = s.be;
+  base.partials = f->partials;
+  base.g0 = 1.0;
+  base.status = f->status;
+  double* Wb[2] = {s.W0, s.W1};
+  for (int k = 0; k < f->m; ++k) {
+    gmb::BSpmm a = base;
+    a.k = k;
+    a.vk = s.V + (size_t)k * N * R;
+    a.vkm1 = k > 0 ? s.V + (size_t)(k - 1) * N * R : nullptr;
+    a.x = k == 0 ? s.U : Wb[(k - 1) & 1];
+    a.ldx = R;
+    a.y = Wb[k & 1];
+    a.ldy = R;
+    hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kOnly, true>), dim3(f->G_sp), dim3(gm::kBlock), 0,
+                       f->stream, a);
+    F_HIP(hipGetLastError());
+    F_TRY(freduce(f, f->G_sp, R, s.al + (size_t)k * R, 0, 0));
+    gmb::BFinish fn{};
+    fn.w = Wb[k & 1];
+    fn.vk = a.vk;
+    fn.al = s.al;
+    fn.be = s.be;
+    fn.k = k;
+    fn.nrows = f->n;
+    fn.chunk = f->chunk_sp;
+    fn.partials = f->partials;
+    fn.status = f->status;
+    hipLaunchKernelGGL((gmb::k_bfinish<R>), dim3(f->G_sp), dim3(gm::kBlock), 0, f->stream, fn);
+    F_HIP(hipGetLastError());
+    F_TRY(freduce(f, f->G_sp, R, s.be + (size_t)(k + 1) * R, 1, 1));
+  }
+  for (int pass = 0; pass < (unit ? 2 : 1); ++pass) {
+    gmb::BSolve sv{};
+    sv.al = s.al;
+    sv.be = s.be;
+    sv.R = R;
+    sv.m = f->m;
+    sv.ns = f->ns;
+    sv.wts1 = pass ? f->ones : f->wts1;
+    sv.wts2 = pass ? f->ones : f->wts2;
+    sv.etaxi = f->xi;
+    sv.wD = f->wD;
+    sv.wDU = f->wDU;
+    sv.wDU2 = f->wDU2;
+    sv.wX = f->wX;
+    sv.yr = pass ? s.ur : s.yr;
+    sv.yi = pass ? s.ui : s.yi;
+    sv.status = f->status;
+    hipLaunchKernelGGL(gmb::k_bsolves<0>, dim3(bdiv((long long)R * f->ns, 64)), dim3(64), 0,
+                       f->stream, sv);
+    F_HIP(hipGetLastError());
+  }
+  gmb::BCollapse cc{};
+  cc.yr = s.yr;
+  cc.yi = s.yi;
+  cc.gsr = f->gsr;
+  cc.gsi = f->gsi;
+  cc.R = R;
+  cc.m = f->m;
+  cc.ns = f->ns;
+  cc.nd = 1;
+  cc.C = s.C;
+  hipLaunchKernelGGL(gmb::k_bcollapse<0>, dim3(std::max(1, std::min(1024, bdiv((long long)f->m * R, gm::kBlock)))),
+                     dim3(gm::kBlock), 0, f->stream, cc);
+  F_HIP(hipGetLastError());
+  if (want_out) {
+    gmb::BVC v{};
+    v.V = s.V;
+    v.C = s.C;
+    v.R = R;
+    v.m = f->m;
+    v.nd = 1;
+    v.nrows = f->n;
+    v.Z = s.out;
+    v.ldz = R;
+    v.accumulate = 0;
+    hipLaunchKernelGGL((gmb::k_bvc<1>), dim3(bdiv((long long)f->n * R, gm::kBlock)), dim3(gm::kBlock), 0,
+                       f->stream, v);
+    F_HIP(hipGetLastError());
+    if (f->K) {
+      hipLaunchKernelGGL(gmf::k_add<0>, dim3(std::min(8192, bdiv((long long)N * R, 256))), dim3(256), 0,
+                         f->stream, s.upd, s.out, N * R);
+      F_HIP(hipGetLastError());
+    }
+  }
+  return GM_OK;
+}
+
+gm_status fkrylov_dispatch(gm_bfab* f, FSide& s, bool want_out, bool unit) {
+  switch (f->R) {
+    case 2: return fkrylov<2>(f, s, want_out, unit);
+    case 4: return fkrylov<4>(f, s, want_out, unit);
+    case 8: return fkrylov<8>(f, s, want_out, unit);
+    case 16: return fkrylov<16>(f, s, want_out, unit);
+    case 32: return fkrylov<32>(f, s, want_out, unit);
+    default: return fkrylov<64>(f, s, want_out, unit);
+  }
+}
+
+// s.X = V Y (N x R x ncol) from the solves in (yr, yi).
+gm_status fresolvents(gm_bfab* f, FSide& s, const double* yr, const double* yi) {
+  const size_t N = (size_t)f->n;
+  if (!s.Yg) F_TRY(falloc(&s.Yg, (size_t)f->R * f->m * f->ncol));
+  if (!s.X) F_TRY(falloc(&s.X, N * f->R * f->ncol));
+  gmf::PackY pk{yr, yi, f->R, f->m, f->ns, f->ncol, s.Yg};
+  hipLaunchKernelGGL(gmf::k_bpack_y<0>, dim3(std::min(4096, bdiv((long long)f->R * f->m * f->ncol, 256))),
+                     dim3(256), 0, f->stream, pk);
+  F_HIP(hipGetLastError());
+  gmf::BVY v{s.V, s.Yg, f->n, f->R, f->m, f->ncol, s.X};
+  hipLaunchKernelGGL(gmf::k_bvy_mfma<0>, dim3(f->R, bdiv(bdiv(f->n, 16), 4), f->ncol / 64), dim3(256), 0,
+                     f->stream, v);
+  F_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+// EQ = E Q ([N][K]) for E on A's pattern, and QEQ = Q^T E Q (host, K x K row-major).
+gm_status feq(gm_bfab* f, const double* E, double* EQ, std::vector<double>& QEQ) {
+  const int G1 = std::max(1, std::min(4096, bdiv(f->n, 256)));   // one column: 256 rows a pass
+  const int chunk1 = bdiv(f->n, G1);
+  for (int k = 0; k < f->K; ++k) {
+    gmb::BSpmm a{};
+    a.rowptr = f->rowptr;
+    a.col = f->col;
+    a.val = E;
+    a.nrows = f->n;
+    a.chunk = chunk1;
+    a.x = f->Q + k;
+    a.ldx = (size_t)f->K;
+    a.y = EQ + k;
+    a.ldy = (size_t)f->K;
+    a.g0 = 1.0;
+    a.ca = 1.0;
+    hipLaunchKernelGGL((gmb::k_bspmm<1, gmb::kCheb>), dim3(bdiv(f->n, chunk1)), dim3(gm::kBlock), 0,
+                       f->stream, a);
+    F_HIP(hipGetLastError());
+  }
+  // Q^T (E Q): the Gram kernel with U = EQ (K columns)
+  gmb::BDeflQtU q{};
+  q.Q = f->Q;
+  q.U = EQ;
+  q.K = f->K;
+  q.R = f->K;
+  q.nrows = f->n;
+  q.chunk = f->chunk_df;
+  q.KT = bdiv(f->K, 16);
+  q.CT = bdiv(f->K, 16);
+  q.partials = f->partials;
+  hipLaunchKernelGGL(gmb::k_bdefl_qtu<0>, dim3(f->G_df, bdiv((long long)q.KT * q.CT, gm::kWaves)),
+                     dim3(gm::kBlock), 0, f->stream, q);
+  F_HIP(hipGetLastError());
+  F_TRY(freduce(f, f->G_df, f->K * f->K, f->ppart, 0, 0));
+  QEQ.resize((size_t)f->K * f->K);
+  F_HIP(hipMemcpyAsync(QEQ.data(), f->ppart, QEQ.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       f->stream));
+  F_HIP(hipStreamSynchronize(f->stream));
+  return GM_OK;
+}
+
+// Same pivoted elimination as the single-column engine: (shift I - T) h = r for
+// the m x m tridiagonal of column c (al[k][R], be[k][R]).
+void fshift_solve(int m, int R, int c, const double* al, const double* be, double shift,
+                  std::vector<double>& r) {
+  std::vector<double> d(m), du(m), dl(m), du2(m, 0.0);
+  for (int i = 0; i < m; ++i) {
+    d[i] = shift - al[(size_t)i * R + c];
+    const double o = i + 1 < m ? -be[(size_t)(i + 1) * R + c] : 0.0;
+    dl[i] = o;
+    du[i] = o;
+  }
+  for (int i = 0; i + 1 < m; ++i) {
+    if (std::fabs(d[i]) >= std::fabs(dl[i])) {
+      const double fct = d[i] != 0.0 ? dl[i] / d[i] : 0.0;
+      d[i + 1] -= fct * du[i];
+      r[i + 1] -= fct * r[i];
+    } else {
+      const double fct = d[i] / dl[i];
+      d[i] = dl[i];
+      const double t = d[i + 1];
+      d[i + 1] = du[i] - fct * t;
+      if (i + 2 < m) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -fct * du2[i];
+      }
+      du[i] = t;
+      std::swap(r[i], r[i + 1]);
+      r[i + 1] -= fct * r[i];
+    }
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double v = r[i];
+    if (i + 1 < m) v -= du[i] * r[i + 1];
+    if (i + 2 < m) v -= du2[i] * r[i + 2];
+    r[i] = d[i] != 0.0 ? v / d[i] : 0.0;
+  }
+}
+
+gm_status fcheck_status(gm_bfab* f, const char* what) {
+  unsigned stv = 0;
+  F_HIP(hipMemcpyAsync(&stv, f->status, sizeof(unsigned), hipMemcpyDeviceToHost, f->stream));
+  F_HIP(hipStreamSynchronize(f->stream));
+  if (stv & 6u) return ffail(GM_EBREAKDOWN, "Lanczos breakdown in the %s (%u)", what, stv);
+  return GM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gm_bfab_last_error(void) { return g_ferr.c_str(); }
+
+gm_status gm_bfab_create(const gm_bfab_desc* d, gm_bfab** out) {
+  g_ferr.clear();
+  if (!d || !out) return ffail(GM_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->n <= 0 || !d->A_ptr || !d->A_ind || !d->A_values || d->subspace_size < 1 ||
+      d->contour_size < 1 || !d->xi || !d->wts1 || !d->wts2)
+    return ffail(GM_EINVAL, "bad sizes or null arrays");
+  const int R = d->columns;
+  if (R != 2 && R != 4 && R != 8 && R != 16 && R != 32 && R != 64)
+    return ffail(GM_EINVAL, "columns must be 2, 4, 8, 16, 32 or 64");
+  if (d->sparse_format != GM_SPARSE_CSR) return ffail(GM_EINVAL, "A must be CSR");
+  if (d->nnz > INT32_MAX) return ffail(GM_EINVAL, "nnz beyond 32-bit row pointers");
+  if (d->A_ptr[0] != 0 || d->A_ptr[d->n] != d->nnz) return ffail(GM_EINVAL, "bad pointer array");
+  for (int i = 0; i < d->n; ++i)
+    if (d->A_ptr[i + 1] < d->A_ptr[i]) return ffail(GM_EINVAL, "pointer array not monotone");
+  for (long long j = 0; j < d->nnz; ++j)
+    if (d->A_ind[j] < 0 || d->A_ind[j] >= d->n) return ffail(GM_EINVAL, "index out of range");
+  if (d->num_eigs < 0 || d->num_eigs > 64 || d->num_eigs > d->n)
+    return ffail(GM_EINVAL, "num_eigs must be in [0, min(64, n)]");
+  if (d->num_eigs > 0 && (!d->Q || !d->lambda || !d->f1_lambda || !d->f2_lambda))
+    return ffail(GM_EINVAL, "deflation needs Q, lambda, f1_lambda and f2_lambda");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= d->device || d->device < 0)
+    return ffail(GM_EDEVICE, "no HIP device %d", d->device);
+  gm_bfab* f = new gm_bfab();
+  auto bail = [&](gm_status st) {
+    std::string keep = g_ferr;
+    free_bfab(f);
+    delete f;
+    g_ferr = keep;
+    return st;
+  };
+#define F_CREATE(x)                  \
+  do {                               \
+    gm_status s_ = (x);              \
+    if (s_ != GM_OK) return bail(s_); \
+  } while (0)
+  f->dev = d->device;
+  if (hipSetDevice(f->dev) != hipSuccess ||
+      hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(ffail(GM_EDEVICE, "device/stream"));
+  f->n = d->n;
+  f->nnz = d->nnz;
+  f->R = R;
+  f->m = d->subspace_size;
+  f->ns = d->contour_size;
+  f->ncol = bdiv(2 * f->ns, 64) * 64;
+  f->K = d->num_eigs;
+  const size_t N = (size_t)f->n;
+  const int rows_per_pass = gm::kWaves * (R < 64 ? 64 / R : 1);
+  f->G_sp = std::max(1, std::min(4096, bdiv(f->n, rows_per_pass)));
+  f->chunk_sp = bdiv(f->n, f->G_sp);
+  f->G_sp = bdiv(f->n, f->chunk_sp);
+  f->G_df = std::max(1, std::min(256, bdiv(f->n, 1024)));
+  f->chunk_df = bdiv(f->n, f->G_df);
+  f->G_df = bdiv(f->n, f->chunk_df);
+  f->G_pg = std::max(1, std::min(1024, bdiv(f->n, 64)));
+  hipStream_t st = f->stream;
+  auto up = [&](auto** p, const auto* src, size_t cnt) -> gm_status {
+    F_TRY(falloc(p, cnt));
+    F_HIP(hipMemcpyAsync(*p, src, cnt * sizeof(**p), hipMemcpyHostToDevice, st));
+    return GM_OK;
+  };
+  F_CREATE(up(&f->rowptr, d->A_ptr, N + 1));
+  F_CREATE(up(&f->col, d->A_ind, (size_t)f->nnz));
+  F_CREATE(up(&f->val, d->A_values, (size_t)f->nnz));
+  std::vector<int> rb;
+  gmb::bspmm_row_blocks(d->A_ptr, f->n, rb);
+  f->nrowblk = (int)rb.size() - 1;
+  F_CREATE(up(&f->rowblk, rb.data(), rb.size()));
+  // nodes and weights: xi interleaved; weights ns x 2 [Re Im] for the solves
+  std::vector<double> w1(2 * (size_t)f->ns), w2(2 * (size_t)f->ns), ones(2 * (size_t)f->ns, 0.0),
+      gr(f->ns, 1.0), gi(f->ns, 0.0);
+  for (int s = 0; s < f->ns; ++s) {
+    w1[s] = d->wts1[2 * s];
+    w1[s + f->ns] = d->wts1[2 * s + 1];
+    w2[s] = d->wts2[2 * s];
+    w2[s + f->ns] = d->wts2[2 * s + 1];
+    ones[s] = 1.0;
+  }
+  F_CREATE(up(&f->xi, d->xi, 2 * (size_t)f->ns));
+  F_CREATE(up(&f->wts1, w1.data(), w1.size()));
+  F_CREATE(up(&f->wts2, w2.data(), w2.size()));
+  F_CREATE(up(&f->ones, ones.data(), ones.size()));
+  F_CREATE(up(&f->gsr, gr.data(), gr.size()));
+  F_CREATE(up(&f->gsi, gi.data(), gi.size()));
+  if (f->K) {
+    std::vector<double> qr(N * f->K);   // column-major n x K -> row-major [N][K]
+    for (int k = 0; k < f->K; ++k)
+      for (size_t i = 0; i < N; ++i) qr[i * f->K + k] = d->Q[(size_t)k * N + i];
+    F_CREATE(up(&f->Q, qr.data(), qr.size()));
+    F_CREATE(up(&f->el, d->f1_lambda, (size_t)f->K));
+    F_CREATE(up(&f->pl, d->f2_lambda, (size_t)f->K));
+    f->lam.assign(d->lambda, d->lambda + f->K);
+    f->el_h.assign(d->f1_lambda, d->f1_lambda + f->K);
+    f->pl_h.assign(d->f2_lambda, d->f2_lambda + f->K);
+    if (d->f1_divdiff) f->dd1.assign(d->f1_divdiff, d->f1_divdiff + (size_t)f->K * f->K);
+    if (d->f2_divdiff) f->dd2.assign(d->f2_divdiff, d->f2_divdiff + (size_t)f->K * f->K);
+    if (hipStreamSynchronize(st) != hipSuccess) return bail(ffail(GM_EDEVICE, "sync"));
+  }
+  const size_t ny = (size_t)f->m * R * f->ns;
+  const size_t np = std::max<size_t>({(size_t)f->G_sp * R, (size_t)f->G_df * std::max(f->K, 1) * R,
+                                      (size_t)f->G_df * std::max(f->K, 1) * std::max(f->K, 1),
+                                      (size_t)f->G_df * f->m * std::max(f->K, 1) * R});
+  F_CREATE(falloc(&f->partials, np));
+  F_CREATE(falloc(&f->ppart, std::max<size_t>((size_t)2 * f->G_pg * R,
+                                               (size_t)f->m * std::max(f->K, 1) * R)));
+  F_CREATE(falloc(&f->wD, ny));
+  F_CREATE(falloc(&f->wDU, ny));
+  F_CREATE(falloc(&f->wDU2, ny));
+  F_CREATE(falloc(&f->wX, ny));
+  F_CREATE(falloc(&f->status, 1));
+  if (hipMemsetAsync(f->status, 0, sizeof(unsigned), st) != hipSuccess)
+    return bail(ffail(GM_EDEVICE, "memset"));
+  for (FSide* s : {&f->sb, &f->sg}) {
+    F_CREATE(falloc(&s->U, N * R));
+    F_CREATE(falloc(&s->V, N * R * f->m));
+    F_CREATE(falloc(&s->W0, N * R));
+    F_CREATE(falloc(&s->W1, N * R));
+    F_CREATE(falloc(&s->al, (size_t)f->m * R));
+    F_CREATE(falloc(&s->be, (size_t)(f->m + 1) * R));
+    F_CREATE(falloc(&s->yr, ny));
+    F_CREATE(falloc(&s->yi, ny));
+    F_CREATE(falloc(&s->ur, ny));
+    F_CREATE(falloc(&s->ui, ny));
+    F_CREATE(falloc(&s->C, (size_t)f->m * R));
+    F_CREATE(falloc(&s->out, N * R));
+    F_CREATE(falloc(&s->Zd, (size_t)std::max(f->K, 1) * R));
+    F_CREATE(falloc(&s->upd, f->K ? N * R : 1));
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return bail(ffail(GM_EDEVICE, "sync"));
+  {
+    std::lock_guard<std::mutex> lk(g_fmu);
+    g_freg.insert(f);
+  }
+  *out = f;
+  return GM_OK;
+#undef F_CREATE
+}
+
+gm_status gm_bfab_delete(gm_bfab* f) {
+  {
+    std::lock_guard<std::mutex> lk(g_fmu);
+    if (!f || !g_freg.count(f)) return ffail(GM_EBADHANDLE, "invalid bfab handle");
+    g_freg.erase(f);
+  }
+  (void)hipSetDevice(f->dev);
+  if (f->stream) (void)hipStreamSynchronize(f->stream);
+  free_bfab(f);
+  delete f;
+  return GM_OK;
+}
+
+gm_status gm_bfab_forward(gm_bfab* f, const double* B, double* U) {
+  if (!falive(f)) return ffail(GM_EBADHANDLE, "invalid bfab handle");
+  if (!B) return ffail(GM_EINVAL, "null input");
+  F_HIP(hipSetDevice(f->dev));
+  const size_t NR = (size_t)f->n * f->R;
+  F_HIP(hipMemcpyAsync(f->sb.U, B, NR * sizeof(double), hipMemcpyDefault, f->stream));
+  if (f->K) F_TRY(fdeflate(f, f->sb));
+  F_TRY(fkrylov_dispatch(f, f->sb, U != nullptr, false));
+  if (U) F_HIP(hipMemcpyAsync(U, f->sb.out, NR * sizeof(double), hipMemcpyDefault, f->stream));
+  F_TRY(fcheck_status(f, "forward pass"));
+  f->have_forward = true;
+  return GM_OK;
+}
+
+gm_status gm_bfab_set_param_derivs(gm_bfab* f, int32_t nparams, const double* const* dA_values) {
+  if (!falive(f)) return ffail(GM_EBADHANDLE, "invalid bfab handle");
+  if (nparams < 0 || (nparams > 0 && !dA_values)) return ffail(GM_EINVAL, "bad parameter arrays");
+  for (int q = 0; q < nparams; ++q)
+    if (!dA_values[q]) return ffail(GM_EINVAL, "null dA/dtheta values for parameter %d", q);
+  if (f->K && nparams > 0 && (f->dd1.empty() || f->dd2.empty()))
+    return ffail(GM_EINVAL, "the parameter gradient with deflation needs f1_divdiff and f2_divdiff");
+  F_HIP(hipSetDevice(f->dev));
+  F_HIP(hipStreamSynchronize(f->stream));
+  for (double* p : f->pdA) ffree(p);
+  for (double* p : f->pEQ) ffree(p);
+  f->pdA.clear();
+  f->pEQ.clear();
+  f->pQEQ.clear();
+  for (int q = 0; q < nparams; ++q) {
+    double* E = nullptr;
+    F_TRY(falloc(&E, (size_t)f->nnz));
+    f->pdA.push_back(E);
+    F_HIP(hipMemcpyAsync(E, dA_values[q], (size_t)f->nnz * sizeof(double), hipMemcpyHostToDevice,
+                         f->stream));
+    if (f->K) {
+      double* EQ = nullptr;
+      F_TRY(falloc(&EQ, (size_t)f->n * f->K));
+      f->pEQ.push_back(EQ);
+      f->pQEQ.emplace_back();
+      F_TRY(feq(f, E, EQ, f->pQEQ.back()));
+    }
+  }
+  F_HIP(hipStreamSynchronize(f->stream));
+  return GM_OK;
+}
+
+gm_status gm_bfab_backward_param(gm_bfab* f, const double* G, double* grad_B, double* grad_theta) {
+  if (!falive(f)) return ffail(GM_EBADHANDLE, "invalid bfab handle");
+  if (!f->have_forward) return ffail(GM_ESTATE, "backward needs a forward pass first");
+  if (!G) return ffail(GM_EINVAL, "null upstream gradient");
+  const int P = (int)f->pdA.size();
+  if (P > 0 && !grad_theta) return ffail(GM_EINVAL, "null grad_theta");
+  F_HIP(hipSetDevice(f->dev));
+  const int R = f->R, m = f->m, K = f->K;
+  const size_t N = (size_t)f->n, NR = N * R;
+  hipStream_t st = f->stream;
+  F_HIP(hipMemcpyAsync(f->sg.U, G, NR * sizeof(double), hipMemcpyDefault, st));
+  if (K) F_TRY(fdeflate(f, f->sg));
+  F_TRY(fkrylov_dispatch(f, f->sg, grad_B != nullptr, P > 0));
+  if (grad_B) F_HIP(hipMemcpyAsync(grad_B, f->sg.out, NR * sizeof(double), hipMemcpyDefault, st));
+  if (P > 0) {
+    // P-P blocks: X_b from the weighted solves, Z_g from the unit-weight ones
+    F_TRY(fresolvents(f, f->sb, f->sb.yr, f->sb.yi));
+    F_TRY(fresolvents(f, f->sg, f->sg.ur, f->sg.ui));
+    // deflation data of both sides, and H per column (host)
+    std::vector<double> Hb, Hg, ab, ag;
+    if (K) {
+      for (int side = 0; side < 2; ++side) {
+        FSide& sd = side ? f->sg : f->sb;
+        std::vector<double> al((size_t)m * R), be((size_t)(m + 1) * R), Cv((size_t)m * R),
+            a((size_t)K * R);
+        F_HIP(hipMemcpyAsync(al.data(), sd.al, al.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        F_HIP(hipMemcpyAsync(be.data(), sd.be, be.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        F_HIP(hipMemcpyAsync(Cv.data(), sd.C, Cv.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        F_HIP(hipMemcpyAsync(a.data(), sd.Zd, a.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        F_HIP(hipStreamSynchronize(st));
+        std::vector<double> H((size_t)R * m * K), r(m);   // [c][j][k]
+        for (int c = 0; c < R; ++c) {
+          const std::vector<double>& fl = (c & 1) ? f->pl_h : f->el_h;
+          for (int k = 0; k < K; ++k) {
+            for (int j = 0; j < m; ++j) r[j] = -Cv[(size_t)j * R + c];
+            r[0] += fl[k] * be[c];
+            fshift_solve(m, R, c, al.data(), be.data(), f->lam[k], r);
+            for (int j = 0; j < m; ++j) H[((size_t)c * m + j) * K + k] = r[j];
+          }
+        }
+        (side ? Hg : Hb).swap(H);
+        (side ? ag : ab).swap(a);
+      }
+    }
+    for (int q = 0; q < P; ++q) {
+      gmf::BPGrad pg{f->rowptr, f->col, f->pdA[q], f->sb.X, f->sg.X, f->n, R, f->ns, f->ncol,
+                     f->ppart};
+      hipLaunchKernelGGL(gmf::k_bparam_grad<0>, dim3(f->G_pg, R), dim3(256), 0, st, pg);
+      F_HIP(hipGetLastError());
+      std::vector<double> pp((size_t)2 * f->G_pg * R);
+      F_HIP(hipMemcpyAsync(pp.data(), f->ppart, pp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      F_HIP(hipStreamSynchronize(st));
+      for (int c = 0; c < R; ++c) {
+        double sum = 0.0;
+        for (int g = 0; g < f->G_pg; ++g) sum += pp[2 * ((size_t)c * f->G_pg + g)];
+        grad_theta[(size_t)q * R + c] = sum;
+      }
+      if (K) {
+        std::vector<double> Mb, Mg;
+        for (int side = 0; side < 2; ++side) {
+          FSide& sd = side ? f->sg : f->sb;
+          gmf::BVtQ v{};
+          v.V = sd.V;
+          v.EQ = f->pEQ[q];
+          v.K = K;
+          v.R = R;
+          v.m = m;
+          v.nrows = f->n;
+          v.chunk = f->chunk_df;
+          v.KT = bdiv(K, 16);
+          v.CT = bdiv(R, 16);
+          v.partials = f->partials;
+          hipLaunchKernelGGL(gmf::k_bvtq<0>, dim3(f->G_df, bdiv((long long)v.KT * v.CT, gm::kWaves), m),
+                             dim3(256), 0, st, v);
+          F_HIP(hipGetLastError());
+          F_TRY(freduce(f, f->G_df, m * K * R, f->ppart, 0, 0));
+          std::vector<double>& M = side ? Mg : Mb;   // [j][k][c]
+          M.resize((size_t)m * K * R);
+          F_HIP(hipMemcpyAsync(M.data(), f->ppart, M.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+          F_HIP(hipStreamSynchronize(st));
+        }
+        const std::vector<double>& QEQ = f->pQEQ[q];
+        for (int c = 0; c < R; ++c) {
+          const std::vector<double>& F = (c & 1) ? f->dd2 : f->dd1;
+          double g = 0.0;
+          for (int k = 0; k < K; ++k)
+            for (int l = 0; l < K; ++l)
+              g += ag[(size_t)k * R + c] * F[(size_t)l * K + k] * QEQ[(size_t)k * K + l] *
+                   ab[(size_t)l * R + c];
+          for (int k = 0; k < K; ++k) {
+            double tb = 0.0, tg = 0.0;
+            for (int j = 0; j < m; ++j) {
+              tb += Mb[((size_t)j * K + k) * R + c] * Hb[((size_t)c * m + j) * K + k];
+              tg += Mg[((size_t)j * K + k) * R + c] * Hg[((size_t)c * m + j) * K + k];
+            }
+            g += ag[(size_t)k * R + c] * tb + ab[(size_t)k * R + c] * tg;
+          }
+          grad_theta[(size_t)q * R + c] += g;
+        }
+      }
+    }
+  }
+  return fcheck_status(f, "backward pass");
+}
+
+gm_status gm_bfab_info(gm_bfab* f, int32_t* columns, int32_t* num_params) {
+  if (!falive(f)) return ffail(GM_EBADHANDLE, "invalid bfab handle");
+  if (columns) *columns = f->R;
+  if (num_params) *num_params = (int32_t)f->pdA.size();
+  return GM_OK;
+}
+
+}  // extern "C"

@@ -272,3 +272,102 @@ class StimulusSetAdjoint:
     def delete(self):
         for r in self.runs:
             r.delete()
+
+
+class MonodomainBatchAdjoint:
+    """All stimuli in one multi-column engine (``gm_bfab_*``): the columns are
+    [u_1, g_1, u_2, g_2, ...] (even columns f1, odd f2), so each stage of a
+    forward step -- and of a backward step, on [w_p, w_p] -- is one launch over
+    every stimulus.  States, trajectory and reaction arithmetic are float64
+    torch tensors in GPU memory, N x P.  Same scheme and adjoint as
+    :class:`MonodomainAdjoint`; P in {1, 2, 4, 8, 16, 32}."""
+
+    def __init__(self, A, n_stimuli, *, m, xi, wts1, wts2, dt, ms=None, device=0, Q=None,
+                 lam=None, eta=None, alpha=None):
+        from .adjoint import BatchFabEngine
+        self.P = int(n_stimuli)
+        if 2 * self.P not in (2, 4, 8, 16, 32, 64):
+            raise ValueError("n_stimuli must be 1, 2, 4, 8, 16 or 32")
+        self.p = dict(_lib.MS_DEFAULTS)
+        self.p.update(ms or {})
+        self.dt = float(dt)
+        self.device = int(device)
+        kw = {}
+        if Q is not None and np.shape(Q)[1] > 0:
+            if eta is None or alpha is None:
+                raise ValueError("deflation needs eta and alpha")
+            from . import quadrature as qd
+            lam = np.asarray(lam, dtype=np.float64)
+            f1 = (lambda x: qd.f1(x, eta, alpha))
+            f2 = (lambda x: qd.f2(x, eta, alpha))
+            kw = dict(Q=Q, lam=lam, f1_lambda=f1(lam), f2_lambda=f2(lam),
+                      f1_divdiff=qd.divided_differences(f1, lambda x: qd.df1(x, eta, alpha), lam),
+                      f2_divdiff=qd.divided_differences(f2, lambda x: qd.df2(x, eta, alpha), lam))
+        self.eng = BatchFabEngine(A, R=2 * self.P, m=m, xi=xi, wts1=wts1, wts2=wts2,
+                                  device=device, **kw)
+        self.traj = None
+
+    def _pairs(self, a, b):
+        import torch
+        return torch.stack((a, b), dim=2).reshape(a.shape[0], 2 * self.P)
+
+    def _cols(self, xs):
+        import torch
+        return torch.stack([torch.as_tensor(np.asarray(x, dtype=np.float64) if not
+                                            hasattr(x, "data_ptr") else x, dtype=torch.float64,
+                                            device=f"cuda:{self.device}") for x in xs], dim=1)
+
+    def forward(self, u0s, h0, steps):
+        """u_T (N x P) from the P initial states u0s and the common gate h0."""
+        U = self._cols(u0s).contiguous()
+        H = self._cols([h0] * self.P).contiguous()
+        traj = []
+        for _ in range(steps):
+            G = ms_g(U, H, self.p)
+            traj.append((U, H, G))
+            out = self.eng.forward(self._pairs(U, G))
+            U = (out[:, 0::2] + self.dt * out[:, 1::2]).contiguous()
+            H = ms_gate(U, H, self.dt, self.p)
+        self.traj = traj
+        self.uT, self.hT = U, H
+        return U
+
+    def states(self, p):
+        """u_1 .. u_T of stimulus p."""
+        if self.traj is None:
+            raise RuntimeError("no stored run")
+        return [t[0][:, p] for t in self.traj[1:]] + [self.uT[:, p]]
+
+    def backward(self, ubar_T, dA_list=(), sources=None):
+        """(dL/du_0, dL/dh_0 (N x P), sum over stimuli of dL/dtheta_q); ``sources``
+        a list over stimuli of per-step sources (as in MonodomainAdjoint)."""
+        import torch
+        if self.traj is None:
+            raise RuntimeError("backward needs a forward run first")
+        p, dt = self.p, self.dt
+        w = torch.as_tensor(ubar_T, dtype=torch.float64,
+                            device=f"cuda:{self.device}").clone().contiguous()
+        hbar = torch.zeros_like(w)
+        gtheta = np.zeros(len(dA_list))
+        self.eng.set_param_derivs(list(dA_list))
+        for n in range(len(self.traj) - 1, -1, -1):
+            U, H, G = self.traj[n]
+            u_next = self.traj[n + 1][0] if n + 1 < len(self.traj) else self.uT
+            dDdh = _where(u_next < p["v_gate"], -1.0 / p["tau_open"], -1.0 / p["tau_close"], w)
+            hbar_prev = hbar * (1.0 + dt * dDdh)
+            self.eng.forward(self._pairs(U, G), want_u=False)
+            fw, gt = self.eng.backward_param(self._pairs(w, w), want_b=True)
+            if len(dA_list):
+                gtheta += gt[:, 0::2].sum(axis=1) + dt * gt[:, 1::2].sum(axis=1)
+            f1w, f2w = fw[:, 0::2], fw[:, 1::2]
+            dGdu = H * (2.0 * U - 3.0 * U * U) / p["tau_in"] - 1.0 / p["tau_out"]
+            dGdh = (U * U) * (1.0 - U) / p["tau_in"]
+            w = (f1w + dGdu * (dt * f2w)).contiguous()
+            hbar = hbar_prev + dGdh * (dt * f2w)
+            if sources is not None and n >= 1:
+                w = w + torch.stack([src[n - 1] for src in sources], dim=1)
+        return w, hbar, gtheta
+
+    def delete(self):
+        self.eng.delete()
+        self.traj = None

@@ -428,6 +428,51 @@ gm_status gm_subspace_size(const double* gamma, int32_t gamma_size, double l1, d
                            double tol, int32_t* m);
 
 /* ---------------------------------------------------------------------------
+ * f(A) on R columns at once with the parameter adjoint (no reference
+ * counterpart): the multi-column form of gm_fab_* for problems that share A,
+ * the contour and the deflation pairs (e.g. the stimuli of a monodomain run).
+ * Columns pair like the batched engine's: even columns use wts1 and
+ * f1_lambda, odd columns wts2 and f2_lambda.  Vectors are N x R row-major
+ * (column c of row i at i*R + c), in host or device memory.
+ * ------------------------------------------------------------------------- */
+typedef struct gm_bfab gm_bfab;
+
+typedef struct gm_bfab_desc {
+    int32_t n;
+    int32_t sparse_format;         /* GM_SPARSE_CSR, A symmetric             */
+    int64_t nnz;
+    const double* A_values;
+    const int32_t* A_ptr;
+    const int32_t* A_ind;
+    int32_t columns;               /* R: 2, 4, 8, 16, 32 or 64               */
+    int32_t subspace_size;         /* m                                      */
+    int32_t contour_size;          /* n_s                                    */
+    const double* xi;              /* n_s complex nodes, interleaved         */
+    const double* wts1;            /* n_s complex weights, even columns      */
+    const double* wts2;            /* n_s complex weights, odd columns       */
+    int32_t num_eigs;              /* K (0: none), K <= 64                   */
+    const double* Q;               /* n x K column-major, orthonormal        */
+    const double* lambda;          /* K                                      */
+    const double* f1_lambda;       /* K: f1(lambda_k) (even columns)         */
+    const double* f2_lambda;       /* K: f2(lambda_k) (odd columns)          */
+    const double* f1_divdiff;      /* K x K column-major divided differences */
+    const double* f2_divdiff;      /*   of f1, f2 (parameter gradient only)  */
+    int32_t device;
+} gm_bfab_desc;
+
+const char* gm_bfab_last_error(void);
+gm_status gm_bfab_create(const gm_bfab_desc* desc, gm_bfab** out);
+gm_status gm_bfab_delete(gm_bfab* f);
+/* U = f(A) B column by column; U may be NULL to build only the forward state. */
+gm_status gm_bfab_forward(gm_bfab* f, const double* B, double* U);
+/* dA/dtheta_q (A's pattern, symmetric) kept on the device for the backward.  */
+gm_status gm_bfab_set_param_derivs(gm_bfab* f, int32_t nparams, const double* const* dA_values);
+/* After a forward: grad_B = f(A) G (may be NULL) and, per stored parameter q
+ * and column c, grad_theta[q*R + c] = d(g_c^T f(A) b_c)/dtheta_q.          */
+gm_status gm_bfab_backward_param(gm_bfab* f, const double* G, double* grad_B, double* grad_theta);
+gm_status gm_bfab_info(gm_bfab* f, int32_t* columns, int32_t* num_params);
+
+/* ---------------------------------------------------------------------------
  * Deflation eigenpairs on the GPU.  Replaces the host eigensolve that
  * produces Q and Lambda before lanczos_create -- eigs(K, M, 'sm') in
  * process_nonuniform.m:106-110 (MATLAB, sparse factorisation), ARPACK

@@ -36,7 +36,8 @@ def test_library_exports_every_declared_symbol():
                                            ("gm_batch_desc", "GmBatchDesc"),
                                            ("gm_fab_desc", "GmFabDesc"),
                                            ("gm_ubatch_desc", "GmUbatchDesc"),
-                                           ("gm_eigs_desc", "GmEigsDesc")])
+                                           ("gm_eigs_desc", "GmEigsDesc"),
+                                           ("gm_bfab_desc", "GmBfabDesc")])
 def test_struct_layout_matches_header_field_order(cname, pyname):
     text = open(HEADER).read()
     body = text[text.index("typedef struct " + cname):text.index("} " + cname + ";")]
@@ -184,3 +185,29 @@ def test_ubatch_create_rejects_bad_descriptors():
     assert lib.gm_ubatch_create(C.byref(d), C.byref(h)) == _lib.GM_EINVAL
     assert b"out of range" in lib.gm_ubatch_last_error()
     assert lib.gm_ubatch_step(None, 1, 0) == _lib.GM_EBADHANDLE
+
+
+def test_bfab_create_rejects_bad_descriptors():
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.gm_bfab_create(None, C.byref(h)) == _lib.GM_EINVAL
+    d = _lib.GmBfabDesc()
+    ptr = np.array([0, 1, 2], dtype=np.int32)
+    ind = np.array([0, 1], dtype=np.int32)
+    val = np.ones(2)
+    z = np.zeros(8)
+    dp = C.POINTER(C.c_double)
+    d.n, d.nnz, d.sparse_format = 2, 2, _lib.GM_SPARSE_CSR
+    d.A_values = val.ctypes.data_as(dp)
+    d.A_ptr = ptr.ctypes.data_as(C.POINTER(C.c_int32))
+    d.A_ind = ind.ctypes.data_as(C.POINTER(C.c_int32))
+    d.subspace_size, d.contour_size = 2, 2
+    d.xi = d.wts1 = d.wts2 = z.ctypes.data_as(dp)
+    d.columns = 3
+    assert lib.gm_bfab_create(C.byref(d), C.byref(h)) == _lib.GM_EINVAL
+    assert b"columns" in lib.gm_bfab_last_error()
+    d.columns = 2
+    d.num_eigs = 1
+    assert lib.gm_bfab_create(C.byref(d), C.byref(h)) == _lib.GM_EINVAL
+    assert b"deflation" in lib.gm_bfab_last_error()
+    assert lib.gm_bfab_forward(None, None, None) == _lib.GM_EBADHANDLE

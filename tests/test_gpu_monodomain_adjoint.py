@@ -209,3 +209,50 @@ def test_stimulus_set_concurrent_matches_sequential():
         assert np.array_equal(uTs[p].cpu().numpy(), seq[p][0])
         assert np.array_equal(ubs[p].cpu().numpy(), seq[p][1])
     assert gsum == sum(s_[2] for s_ in seq) or abs(gsum - sum(s_[2] for s_ in seq)) <= 1e-15 * abs(gsum)
+
+
+@pytest.mark.parametrize("P", [1, 2, 4])
+def test_batch_adjoint_matches_per_stimulus(P):
+    """The one-launch multi-stimulus engine (gm_bfab_*, columns [u_p, g_p]) against
+    the single-column engine run stimulus by stimulus (itself pinned by central
+    differences above): end states, dL/du_0 and the summed dL/dratio, with K = 6
+    deflation pairs.  The two differ only in the reduction partitions of the
+    Lanczos sums, so they agree to rounding amplified by m = 60 steps."""
+    import torch
+    from gpumatfun_amd.monodomain import MonodomainBatchAdjoint
+    n, steps, dt, r, m, K = 24, 5, 0.05, 4.0, 60, 6
+    eta, alpha = dt * 1e-3, 1.7
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    A = problems.anisotropic_9pt(n, ratio=r)
+    w, U = np.linalg.eigh(A.toarray())
+    c = q.contour_nodes(0.9 * w[K], 1.1 * w.max(), 24, eta, alpha)
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    spots = [(2, 3), (12, 5), (7, 15), (15, 14)][:P]
+    u0s = [((X >= a) & (X < a + 5) & (Y >= b) & (Y < b + 5)).astype(float).reshape(-1, order="F")
+           for a, b in spots]
+    h0 = np.ones(n * n)
+    y = 0.3 * np.random.default_rng(0).random(n * n)
+    kw = dict(m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"], dt=dt, Q=U[:, :K], lam=w[:K],
+              eta=eta, alpha=alpha)
+    print(f"case batch adjoint P={P}", flush=True)
+    ref_u, ref_ub, ref_g = [], [], 0.0
+    for u0 in u0s:
+        md = MonodomainAdjoint(A, resident=True, **kw)
+        uT = md.forward(u0, h0, steps)
+        ub, _, (gr,) = md.backward(2 * (uT - torch.as_tensor(y, device=uT.device)), [E])
+        ref_u.append(uT.cpu().numpy())
+        ref_ub.append(ub.cpu().numpy())
+        ref_g += gr
+        md.delete()
+    bm = MonodomainBatchAdjoint(A, P, **kw)
+    UT = bm.forward(u0s, h0, steps)
+    yt = torch.as_tensor(y, device=UT.device)[:, None]
+    UB, _, (gb,) = bm.backward(2 * (UT - yt), [E])
+    bm.delete()
+    for p_ in range(P):
+        eu = rel(UT[:, p_].cpu().numpy(), ref_u[p_])
+        eb = rel(UB[:, p_].cpu().numpy(), ref_ub[p_])
+        print(f"stimulus {p_}: u_T {eu:.2e}, dL/du0 {eb:.2e}")
+        assert eu < 1e-10 and eb < 1e-9
+    print("dL/dratio", gb, ref_g)
+    assert abs(gb - ref_g) <= 1e-9 * abs(ref_g)
