@@ -107,51 +107,86 @@ __global__ __launch_bounds__(256) void k_bvy_mfma(BVY p) {
   }
 }
 
-// Per column c (block y): sum over rows i of Im sum_s Z[i][c][s] (E X)[i][c][s],
-// one wavefront per row, lanes over the ncol real columns (s = lane / 2, Re on
-// even lanes), the partner lane's value by a lane swap.  partials[c][Gx][2].
+// Per column c: sum over rows i of Im sum_s Z[i][c][s] (E X)[i][c][s].  One
+// wavefront per row, lanes over the ncol real columns (s = lane / 2, Re on even
+// lanes, the partner lane's value by a lane swap), looping over the R columns
+// inside, so that the gathered rows X[col][0..R) (R ncol contiguous doubles)
+// are read together; the row's nonzeros are loaded once, 12 at a time.
+// partials[G][R] (fixed order: waves, then blocks on the host).
 struct BPGrad {
   const int* rowptr;
   const int* col;
   const double* E;
   const double* X;
   const double* Z;
-  int N, R, ns, ncol;
+  int N, ns, ncol;
   double* partials;
 };
 
-template <int kUnused = 0>
+template <int R>
 __global__ __launch_bounds__(256) void k_bparam_grad(BPGrad p) {
+  constexpr int U = 12;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int c = blockIdx.y;
-  double acc = 0.0;
+  __shared__ double sw[4][R];   // long rows: per-wavefront column sums
+  for (int c = lane; c < R; c += 64) sw[wave][c] = 0.0;
+  double acc[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) acc[c] = 0.0;
   for (int i = blockIdx.x * 4 + wave; i < p.N; i += gridDim.x * 4) {
     const int e0 = p.rowptr[i], e1 = p.rowptr[i + 1];
     for (int c0 = 0; c0 < p.ncol; c0 += 64) {
       const int cc = c0 + lane;
-      double ex = 0.0;
-      for (int eb = e0; eb < e1; eb += 8) {
-        double ev[8], xv[8];
-        int ci[8];
+      const bool live = !(lane & 1) && (cc >> 1) < p.ns;
+      if (e1 - e0 <= U) {
+        // the row's nonzeros once, then every column: U gathers in flight each
+        double ev[U];
+        int ci[U];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const bool ok = eb + t < e1;
-          ev[t] = ok ? p.E[eb + t] : 0.0;
-          ci[t] = ok ? p.col[eb + t] : i;
+        for (int t = 0; t < U; ++t) {
+          const bool ok = e0 + t < e1;
+          ev[t] = ok ? p.E[e0 + t] : 0.0;
+          ci[t] = ok ? p.col[e0 + t] : i;
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t) xv[t] = p.X[((size_t)ci[t] * p.R + c) * p.ncol + cc];
+        for (int c = 0; c < R; ++c) {
+          double xv[U];
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if (eb + t < e1) ex = fma(ev[t], xv[t], ex);
+          for (int t = 0; t < U; ++t) xv[t] = p.X[((size_t)ci[t] * R + c) * p.ncol + cc];
+          double ex = 0.0;
+#pragma unroll
+          for (int t = 0; t < U; ++t)
+            if (e0 + t < e1) ex = fma(ev[t], xv[t], ex);
+          const double z = p.Z[((size_t)i * R + c) * p.ncol + cc];
+          const double ex_o = __shfl_xor(ex, 1);
+          const double z_o = __shfl_xor(z, 1);
+          if (live) acc[c] += z * ex_o + z_o * ex;   // Im(z (E x))
+        }
+      } else {
+        // a long row: column by column, summed straight into the wavefront's slot
+        for (int c = 0; c < R; ++c) {
+          double ex = 0.0;
+          for (int e = e0; e < e1; ++e) ex = fma(p.E[e], p.X[((size_t)p.col[e] * R + c) * p.ncol + cc], ex);
+          const double z = p.Z[((size_t)i * R + c) * p.ncol + cc];
+          const double ex_o = __shfl_xor(ex, 1);
+          const double z_o = __shfl_xor(z, 1);
+          double v = live ? z * ex_o + z_o * ex : 0.0;
+          for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+          if (lane == 0) sw[wave][c] += v;
+        }
       }
-      const double z = p.Z[((size_t)i * p.R + c) * p.ncol + cc];
-      const double ex_o = __shfl_xor(ex, 1);
-      const double z_o = __shfl_xor(z, 1);
-      if (!(lane & 1) && (cc >> 1) < p.ns) acc += z * ex_o + z_o * ex;   // Im(z (E x))
     }
   }
-  block_partial2_at(d2(acc, 0.0), p.partials, c * gridDim.x + blockIdx.x);
+  // per column: wavefront sum (fixed tree) added to the slot, then the 4
+  // wavefronts in order
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+    double v = acc[c];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) sw[wave][c] += v;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < R; c += 256)
+    p.partials[(size_t)blockIdx.x * R + c] = ((sw[0][c] + sw[1][c]) + sw[2][c]) + sw[3][c];
 }
 
 // Block partials of M[j][k][c] = sum_i V[j][i][c] EQ[i][k] for every Krylov
@@ -820,16 +855,24 @@ gm_status gm_bfab_backward_param(gm_bfab* f, const double* G, double* grad_B, do
       }
     }
     for (int q = 0; q < P; ++q) {
-      gmf::BPGrad pg{f->rowptr, f->col, f->pdA[q], f->sb.X, f->sg.X, f->n, R, f->ns, f->ncol,
+      gmf::BPGrad pg{f->rowptr, f->col, f->pdA[q], f->sb.X, f->sg.X, f->n, f->ns, f->ncol,
                      f->ppart};
-      hipLaunchKernelGGL(gmf::k_bparam_grad<0>, dim3(f->G_pg, R), dim3(256), 0, st, pg);
+      const dim3 grid(f->G_pg), blk(256);
+      switch (R) {
+        case 2: hipLaunchKernelGGL(gmf::k_bparam_grad<2>, grid, blk, 0, st, pg); break;
+        case 4: hipLaunchKernelGGL(gmf::k_bparam_grad<4>, grid, blk, 0, st, pg); break;
+        case 8: hipLaunchKernelGGL(gmf::k_bparam_grad<8>, grid, blk, 0, st, pg); break;
+        case 16: hipLaunchKernelGGL(gmf::k_bparam_grad<16>, grid, blk, 0, st, pg); break;
+        case 32: hipLaunchKernelGGL(gmf::k_bparam_grad<32>, grid, blk, 0, st, pg); break;
+        default: hipLaunchKernelGGL(gmf::k_bparam_grad<64>, grid, blk, 0, st, pg); break;
+      }
       F_HIP(hipGetLastError());
-      std::vector<double> pp((size_t)2 * f->G_pg * R);
+      std::vector<double> pp((size_t)f->G_pg * R);
       F_HIP(hipMemcpyAsync(pp.data(), f->ppart, pp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
       F_HIP(hipStreamSynchronize(st));
       for (int c = 0; c < R; ++c) {
         double sum = 0.0;
-        for (int g = 0; g < f->G_pg; ++g) sum += pp[2 * ((size_t)c * f->G_pg + g)];
+        for (int g = 0; g < f->G_pg; ++g) sum += pp[(size_t)g * R + c];
         grad_theta[(size_t)q * R + c] = sum;
       }
       if (K) {
