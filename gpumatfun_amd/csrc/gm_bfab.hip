@@ -663,7 +663,8 @@ gm_status gm_bfab_create(const gm_bfab_desc* d, gm_bfab** out) {
   f->K = d->num_eigs;
   const size_t N = (size_t)f->n;
   const int rows_per_pass = gm::kWaves * (R < 64 ? 64 / R : 1);
-  f->G_sp = std::max(1, std::min(4096, bdiv(f->n, rows_per_pass)));
+  // at most 1024 blocks: few block partials per Lanczos sum (as in the batched form)
+  f->G_sp = std::max(1, std::min(1024, bdiv(f->n, rows_per_pass)));
   f->chunk_sp = bdiv(f->n, f->G_sp);
   f->G_sp = bdiv(f->n, f->chunk_sp);
   f->G_df = std::max(1, std::min(256, bdiv(f->n, 1024)));
