@@ -104,7 +104,10 @@ __device__ void block_partial_cols(const double (&v)[(R < 64 ? 1 : R / 64)], dou
 // Row blocks for the staged form: consecutive rows with at most kBTile
 // nonzeros and kBRows rows per block; a longer row is a block of its own
 // (read from global memory).  rb gets the block starts and n at the end.
-constexpr int kBTile = 1024;
+// 512 nonzeros measured best on the 512^2 9-point operator at 32 columns (5%
+// over 1024 and 256 for the batched step and the eigensolver; 2048 slower),
+// and equal on the 128^3 7-point one at 256 columns.
+constexpr int kBTile = 512;
 constexpr int kBRows = 256;
 
 inline void bspmm_row_blocks(const int* rowptr, int n, std::vector<int>& rb) {
