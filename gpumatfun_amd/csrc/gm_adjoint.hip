@@ -415,7 +415,7 @@ gm_status areduce(gm_fab* f, int G, int nv, double* out, int do_sqrt, int flag_z
   r.do_sqrt = do_sqrt;
   r.flag_zero = flag_zero;
   r.status = f->status;
-  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kReduceBlock), 0, f->stream, r);
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, f->stream, r);
   A_HIP(hipGetLastError());
   return GM_OK;
 }

@@ -136,7 +136,7 @@ gm_status breduce(gm_batch* b, int nv, double* out, int do_sqrt, int flag_zero, 
   r.do_sqrt = do_sqrt;
   r.flag_zero = flag_zero;
   r.status = b->status;
-  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kReduceBlock), 0, b->stream, r);
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, b->stream, r);
   B_HIP(hipGetLastError());
   return GM_OK;
 }

@@ -16,8 +16,9 @@
 //   Q-Q block   (Q^T g)^T (F o Q^T E Q) (Q^T b), F the divided differences of f;
 //   cross       sum_k (Q^T g)_k q_k^T E V_b H_b[:, k] + (b <-> g), with
 //               H[:, k] = (lambda_k - T)^{-1} (f(lambda_k) beta0 e1 - C).
-// The resolvent blocks V Y (N x R x 2 n_s) run on FP64 MFMA, the E-weighted
-// contraction one wavefront per (row, column) with lanes over 2 n_s.
+// The P-P block is contracted in the Krylov bases, (Vg^T E Vb) against an
+// m x m matrix M per column: T = Vg M on FP64 MFMA, then (E Vb) . T by the
+// batched SpMM, one launch per Krylov vector.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,194 +37,167 @@ namespace gmf {
 using namespace gm;
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// Yg[c][k][2s + 0/1] = Re/Im y[c][s][k]; padding columns 0.
-struct PackY {
+// The P-P block of column c in the Krylov bases:
+//   Im sum_s z_s^T E x_s = sum_{k,l} (Vg^T E Vb)[k][l] M_c[k][l],
+//   M_c[k][l] = Im sum_s u_s[k] y_s[l] = sum_s (Re u Im y + Im u Re y),
+// with z_s = Vg u_s (unit-weight solves of the g side) and x_s = Vb y_s
+// (weighted solves of the b side).  M is [R][mk][mp], zero beyond m (mk = m
+// rounded up to 4, mp to the l-tile width) so that MFMA operands need no guard.
+struct BPPM {
+  const double* ur;   // [R][ns][m]
+  const double* ui;
   const double* yr;
   const double* yi;
-  int R, m, ns, ncol;
-  double* Yg;
+  int R, m, ns, mk, mp;
+  double* M;
 };
 
 template <int kUnused = 0>
-__global__ __launch_bounds__(256) void k_bpack_y(PackY p) {
-  const long long total = (long long)p.R * p.m * p.ncol;
+__global__ __launch_bounds__(256) void k_bpp_m(BPPM p) {
+  const long long total = (long long)p.R * p.mk * p.mp;
   for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < total;
        t += (long long)gridDim.x * 256) {
-    const int j = (int)(t % p.ncol);
-    const long long ck = t / p.ncol;
-    const int k = (int)(ck % p.m), c = (int)(ck / p.m);
-    const int s = j >> 1;
+    const int l = (int)(t % p.mp);
+    const long long ck = t / p.mp;
+    const int k = (int)(ck % p.mk), c = (int)(ck / p.mk);
     double v = 0.0;
-    if (s < p.ns) {
-      const size_t e = ((size_t)c * p.ns + s) * p.m + k;
-      v = (j & 1) ? p.yi[e] : p.yr[e];
-    }
-    p.Yg[t] = v;
-  }
-}
-
-// X[i][c][j] = sum_k V[k][i][c] Yg[c][k][j] on FP64 MFMA (v_mfma_f64_16x16x4f64):
-// block x = column c (so that the blocks of one row tile, which share V's cache
-// lines, run together), each wavefront one 16-row tile and 64 output columns.
-// Operand layout: A lane l = (row l%16, k l/16), B lane l = (k l/16, col l%16);
-// D register r of lane l = (row 4r + l/16, col l%16).
-struct BVY {
-  const double* V;    // [m][N][R]
-  const double* Y;    // [R][m][ncol]
-  int N, R, m, ncol;
-  double* X;          // [N][R][ncol]
-};
-
-template <int kUnused = 0>
-__global__ __launch_bounds__(256) void k_bvy_mfma(BVY p) {
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x;
-  const int i0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * 16;
-  const int ct = blockIdx.z;
-  if (i0 >= p.N) return;   // whole wavefronts; no barrier follows
-  d4 acc[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-  const int ia = i0 + (lane & 15);
-  const size_t slice = (size_t)p.N * p.R;
-  for (int k0 = 0; k0 < p.m; k0 += 4) {
-    const int k = k0 + (lane >> 4);
-    const double a = (k < p.m && ia < p.N) ? p.V[(size_t)k * slice + (size_t)ia * p.R + c] : 0.0;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int j = ct * 64 + t * 16 + (lane & 15);
-      const double bv = k < p.m ? p.Y[((size_t)c * p.m + k) * p.ncol + j] : 0.0;
-      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int j = ct * 64 + t * 16 + (lane & 15);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = i0 + 4 * r + (lane >> 4);
-      if (i < p.N) p.X[((size_t)i * p.R + c) * p.ncol + j] = acc[t][r];
-    }
-  }
-}
-
-// Per column c: sum over rows i of Im sum_s Z[i][c][s] (E X)[i][c][s].  One
-// wavefront per row, lanes over the ncol real columns (s = lane / 2, Re on even
-// lanes, the partner lane's value by a lane swap), looping over the R columns
-// inside, so that the gathered rows X[col][0..R) (R ncol contiguous doubles)
-// are read together; the row's nonzeros are loaded once, 12 at a time.
-// partials[G][R] (fixed order: waves, then blocks on the host).
-struct BPGrad {
-  const int* rowptr;
-  const int* col;
-  const double* E;
-  const double* X;
-  const double* Z;
-  int N, ns, ncol;
-  double* partials;
-};
-
-template <int R>
-__global__ __launch_bounds__(256) void k_bparam_grad(BPGrad p) {
-  constexpr int U = 12;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __shared__ double sw[4][R];   // long rows: per-wavefront column sums
-  for (int c = lane; c < R; c += 64) sw[wave][c] = 0.0;
-  double acc[R];
-#pragma unroll
-  for (int c = 0; c < R; ++c) acc[c] = 0.0;
-  for (int i = blockIdx.x * 4 + wave; i < p.N; i += gridDim.x * 4) {
-    const int e0 = p.rowptr[i], e1 = p.rowptr[i + 1];
-    for (int c0 = 0; c0 < p.ncol; c0 += 64) {
-      const int cc = c0 + lane;
-      const bool live = !(lane & 1) && (cc >> 1) < p.ns;
-      if (e1 - e0 <= U) {
-        // the row's nonzeros once, then every column: U gathers in flight each
-        double ev[U];
-        int ci[U];
-#pragma unroll
-        for (int t = 0; t < U; ++t) {
-          const bool ok = e0 + t < e1;
-          ev[t] = ok ? p.E[e0 + t] : 0.0;
-          ci[t] = ok ? p.col[e0 + t] : i;
-        }
-#pragma unroll
-        for (int c = 0; c < R; ++c) {
-          double xv[U];
-#pragma unroll
-          for (int t = 0; t < U; ++t) xv[t] = p.X[((size_t)ci[t] * R + c) * p.ncol + cc];
-          double ex = 0.0;
-#pragma unroll
-          for (int t = 0; t < U; ++t)
-            if (e0 + t < e1) ex = fma(ev[t], xv[t], ex);
-          const double z = p.Z[((size_t)i * R + c) * p.ncol + cc];
-          const double ex_o = __shfl_xor(ex, 1);
-          const double z_o = __shfl_xor(z, 1);
-          if (live) acc[c] += z * ex_o + z_o * ex;   // Im(z (E x))
-        }
-      } else {
-        // a long row: column by column, summed straight into the wavefront's slot
-        for (int c = 0; c < R; ++c) {
-          double ex = 0.0;
-          for (int e = e0; e < e1; ++e) ex = fma(p.E[e], p.X[((size_t)p.col[e] * R + c) * p.ncol + cc], ex);
-          const double z = p.Z[((size_t)i * R + c) * p.ncol + cc];
-          const double ex_o = __shfl_xor(ex, 1);
-          const double z_o = __shfl_xor(z, 1);
-          double v = live ? z * ex_o + z_o * ex : 0.0;
-          for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-          if (lane == 0) sw[wave][c] += v;
-        }
+    if (k < p.m && l < p.m) {
+      for (int s = 0; s < p.ns; ++s) {
+        const size_t b = ((size_t)c * p.ns + s) * p.m;
+        v = fma(p.ur[b + k], p.yi[b + l], v);
+        v = fma(p.ui[b + k], p.yr[b + l], v);
       }
     }
+    p.M[t] = v;
   }
-  // per column: wavefront sum (fixed tree) added to the slot, then the 4
-  // wavefronts in order
+}
+
+// T[l][i][c] = sum_k M_c[k][l] Vg[k][i][c] on FP64 MFMA (v_mfma_f64_16x16x4f64:
+// A = M_c with rows l, B = Vg with columns i; A lane (row lane%16, k lane/16),
+// B lane (k lane/16, col lane%16), D register r = (row 4r + lane/16, col
+// lane%16)).  A wavefront takes one item (16-row tile, CW columns; the items of
+// one tile are consecutive, so a block's wavefronts share Vg's lines) and l in
+// blocks of 16 LT; each lane stores its row's CW columns as one vector per l.
+// The P-P sum of a parameter is then sum_l sum_i (E Vb[l])[i][c] T[l][i][c]
+// (the batched SpMM in its kDot form, one launch per l).
+struct BPPT {
+  const double* Vg;   // [m][N][R]
+  const double* M;    // [R][mk][mp]
+  int N, R, m, mk, mp, items;
+  double* T;          // [m][N][R]
+};
+
+template <int CW, int LT>
+__global__ __launch_bounds__(256) void k_bpp_t(BPPT p) {
+  typedef double vcw __attribute__((ext_vector_type(CW)));
+  const int lane = threadIdx.x & 63;
+  const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (it >= p.items) return;   // whole wavefronts; no barrier follows
+  const int CQ = p.R / CW;
+  const int c0 = (it % CQ) * CW;
+  const int i = (it / CQ) * 16 + (lane & 15);
+  const bool rowok = i < p.N;
+  const size_t slice = (size_t)p.N * p.R;
+  for (int lb = 0; lb < p.m; lb += 16 * LT) {
+    d4 acc[LT][CW];
 #pragma unroll
-  for (int c = 0; c < R; ++c) {
-    double v = acc[c];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (lane == 0) sw[wave][c] += v;
+    for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+      for (int q = 0; q < CW; ++q) acc[lt][q] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < p.mk; k0 += 4) {
+      const int k = k0 + (lane >> 4);
+      vcw b;
+#pragma unroll
+      for (int q = 0; q < CW; ++q) b[q] = 0.0;
+      if (k < p.m && rowok) b = *reinterpret_cast<const vcw*>(p.Vg + (size_t)k * slice + (size_t)i * p.R + c0);
+#pragma unroll
+      for (int q = 0; q < CW; ++q) {
+        const double* Mq = p.M + ((size_t)(c0 + q) * p.mk + k) * p.mp + lb + (lane & 15);
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt)
+          acc[lt][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Mq[lt * 16], b[q], acc[lt][q], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int l = lb + lt * 16 + 4 * r + (lane >> 4);
+        if (l < p.m && rowok) {
+          vcw v;
+#pragma unroll
+          for (int q = 0; q < CW; ++q) v[q] = acc[lt][q][r];
+          *reinterpret_cast<vcw*>(p.T + (size_t)l * slice + (size_t)i * p.R + c0) = v;
+        }
+      }
   }
-  __syncthreads();
-  for (int c = threadIdx.x; c < R; c += 256)
-    p.partials[(size_t)blockIdx.x * R + c] = ((sw[0][c] + sw[1][c]) + sw[2][c]) + sw[3][c];
 }
 
 // Block partials of M[j][k][c] = sum_i V[j][i][c] EQ[i][k] for every Krylov
-// vector j (block z) on FP64 MFMA: each wavefront one 16 x 16 tile of (k, c),
-// rows stepped by 4.  partials[G][m][K][R].
+// vector j (block y) on FP64 MFMA: each wavefront holds all KT x CT 16 x 16
+// tiles of (k, c), so one EQ load feeds CT chains and one V load KT chains;
+// the 4 wavefronts take interleaved 4-row steps of the block's rows and are
+// summed in LDS in wavefront order.  partials[G][m][K][R].
 struct BVtQ {
   const double* V;    // [m][N][R]
   const double* EQ;   // [N][K]
-  int K, R, m, nrows, chunk, KT, CT;
+  int K, R, m, nrows, chunk;
   double* partials;
 };
 
-template <int kUnused = 0>
+template <int KT, int CT>
 __global__ __launch_bounds__(256) void k_bvtq(BVtQ p) {
-  const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.y * kWaves + (threadIdx.x >> 6);
-  if (tile >= p.KT * p.CT) return;   // whole wavefronts; no barrier follows
-  const int j = blockIdx.z;
-  const int kt = tile % p.KT, ct = tile / p.KT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = blockIdx.y;
   const int row0 = blockIdx.x * p.chunk, row1 = min(row0 + p.chunk, p.nrows);
-  const int kk = kt * 16 + (lane & 15);
-  const int c = ct * 16 + (lane & 15);
   const int sub = lane >> 4;
   const double* Vj = p.V + (size_t)j * p.nrows * p.R;
-  d4 acc = {0.0, 0.0, 0.0, 0.0};
-  for (int r = row0; r < row1; r += 4) {
+  __shared__ double sh[KT * CT * 4 * 64];
+  d4 acc[KT][CT];
+#pragma unroll
+  for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) acc[kt][ct] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int r = row0 + 4 * wave; r < row1; r += 16) {
     const int row = r + sub;
     const bool ok = row < row1;
-    const double a = (ok && kk < p.K) ? p.EQ[(size_t)row * p.K + kk] : 0.0;
-    const double b = (ok && c < p.R) ? Vj[(size_t)row * p.R + c] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-  }
+    double a[KT], b[CT];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int k2 = kt * 16 + 4 * i + sub;
+    for (int kt = 0; kt < KT; ++kt) {
+      const int kk = kt * 16 + (lane & 15);
+      a[kt] = (ok && kk < p.K) ? p.EQ[(size_t)row * p.K + kk] : 0.0;
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      const int c = ct * 16 + (lane & 15);
+      b[ct] = (ok && c < p.R) ? Vj[(size_t)row * p.R + c] : 0.0;
+    }
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct)
+        acc[kt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kt], b[ct], acc[kt][ct], 0, 0, 0);
+  }
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            double& d = sh[((kt * CT + ct) * 4 + i) * 64 + lane];
+            d = w == 0 ? acc[kt][ct][i] : d + acc[kt][ct][i];
+          }
+    }
+    __syncthreads();
+  }
+  // D register i of lane l = (k 16 kt + 4 i + l/16, c 16 ct + l%16)
+  for (int t = threadIdx.x; t < KT * CT * 4 * 64; t += 256) {
+    const int ln = t & 63, i = (t >> 6) & 3, tile = t >> 8;
+    const int k2 = (tile / CT) * 16 + 4 * i + (ln >> 4), c = (tile % CT) * 16 + (ln & 15);
     if (k2 < p.K && c < p.R)
-      p.partials[(((size_t)blockIdx.x * p.m + j) * p.K + k2) * p.R + c] = acc[i];
+      p.partials[(((size_t)blockIdx.x * p.m + j) * p.K + k2) * p.R + c] = sh[t];
   }
 }
 
@@ -270,7 +244,7 @@ inline int bdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 struct FSide {
   double *U = nullptr, *V = nullptr, *W0 = nullptr, *W1 = nullptr, *al = nullptr, *be = nullptr,
          *yr = nullptr, *yi = nullptr, *ur = nullptr, *ui = nullptr, *C = nullptr,
-         *out = nullptr, *Zd = nullptr, *upd = nullptr, *Yg = nullptr, *X = nullptr;
+         *out = nullptr, *Zd = nullptr, *upd = nullptr;
 };
 
 }  // namespace
@@ -278,14 +252,15 @@ struct FSide {
 struct gm_bfab {
   int dev = 0;
   hipStream_t stream = nullptr;
-  int n = 0, R = 0, m = 0, ns = 0, ncol = 0, K = 0;
+  int n = 0, R = 0, m = 0, ns = 0, K = 0;
   long long nnz = 0;
   bool have_forward = false;
-  int G_sp = 1, chunk_sp = 1, nrowblk = 0, G_df = 1, chunk_df = 1, G_pg = 1;
+  int G_sp = 1, chunk_sp = 1, nrowblk = 0, G_df = 1, chunk_df = 1;
+  int CW = 4, LT = 1, mk = 4, mp = 16, pp_items = 0;   // T kernel shape
   int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr;
   double *val = nullptr, *xi = nullptr, *wts1 = nullptr, *wts2 = nullptr, *ones = nullptr,
          *gsr = nullptr, *gsi = nullptr, *Q = nullptr, *el = nullptr, *pl = nullptr,
-         *partials = nullptr, *ppart = nullptr;
+         *partials = nullptr, *ppart = nullptr, *Mpp = nullptr, *Tpp = nullptr, *pdot = nullptr;
   double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
   unsigned* status = nullptr;
   std::vector<double> lam, el_h, pl_h, dd1, dd2;   // host (dd K x K column-major)
@@ -315,13 +290,13 @@ void ffree(void* p) {
 void free_bfab(gm_bfab* f) {
   void* bufs[] = {f->rowptr, f->col, f->rowblk, f->val, f->xi, f->wts1, f->wts2, f->ones, f->gsr,
                   f->gsi, f->Q, f->el, f->pl, f->partials, f->ppart, f->wD, f->wDU, f->wDU2,
-                  f->wX, f->status};
+                  f->wX, f->status, f->Mpp, f->Tpp, f->pdot};
   for (void* p : bufs) ffree(p);
   for (double* p : f->pdA) ffree(p);
   for (double* p : f->pEQ) ffree(p);
   for (FSide* s : {&f->sb, &f->sg}) {
     void* sbufs[] = {s->U, s->V, s->W0, s->W1, s->al, s->be, s->yr, s->yi, s->ur, s->ui, s->C,
-                     s->out, s->Zd, s->upd, s->Yg, s->X};
+                     s->out, s->Zd, s->upd};
     for (void* p : sbufs) ffree(p);
   }
   if (f->stream) (void)hipStreamDestroy(f->stream);
@@ -337,7 +312,7 @@ gm_status freduce(gm_bfab* f, int G, int nv, double* out, int do_sqrt, int flag_
   r.do_sqrt = do_sqrt;
   r.flag_zero = flag_zero;
   r.status = f->status;
-  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kReduceBlock), 0, f->stream, r);
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, f->stream, r);
   F_HIP(hipGetLastError());
   return GM_OK;
 }
@@ -500,22 +475,85 @@ gm_status fkrylov_dispatch(gm_bfab* f, FSide& s, bool want_out, bool unit) {
   }
 }
 
-// s.X = V Y (N x R x ncol) from the solves in (yr, yi).
-gm_status fresolvents(gm_bfab* f, FSide& s, const double* yr, const double* yi) {
+// M for the P-P block from the unit-weight solves of the g side and the
+// weighted solves of the b side, then T = Vg M (both per backward, for all
+// parameters).
+template <int CW>
+void launch_bpp_t(const gmf::BPPT& p, int LT, hipStream_t st) {
+  const dim3 grid(bdiv(p.items, 4));
+  switch (LT) {
+    case 1: hipLaunchKernelGGL((gmf::k_bpp_t<CW, 1>), grid, dim3(256), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((gmf::k_bpp_t<CW, 2>), grid, dim3(256), 0, st, p); break;
+    case 3: hipLaunchKernelGGL((gmf::k_bpp_t<CW, 3>), grid, dim3(256), 0, st, p); break;
+    default: hipLaunchKernelGGL((gmf::k_bpp_t<CW, 4>), grid, dim3(256), 0, st, p); break;
+  }
+}
+
+gm_status fppm(gm_bfab* f) {
   const size_t N = (size_t)f->n;
-  if (!s.Yg) F_TRY(falloc(&s.Yg, (size_t)f->R * f->m * f->ncol));
-  if (!s.X) F_TRY(falloc(&s.X, N * f->R * f->ncol));
-  gmf::PackY pk{yr, yi, f->R, f->m, f->ns, f->ncol, s.Yg};
-  hipLaunchKernelGGL(gmf::k_bpack_y<0>, dim3(std::min(4096, bdiv((long long)f->R * f->m * f->ncol, 256))),
-                     dim3(256), 0, f->stream, pk);
+  if (!f->Mpp) F_TRY(falloc(&f->Mpp, (size_t)f->R * f->mk * f->mp));
+  if (!f->Tpp) F_TRY(falloc(&f->Tpp, N * f->R * f->m));
+  if (!f->pdot) F_TRY(falloc(&f->pdot, ((size_t)f->G_sp + 1) * f->R));
+  gmf::BPPM a{f->sg.ur, f->sg.ui, f->sb.yr, f->sb.yi, f->R, f->m, f->ns, f->mk, f->mp, f->Mpp};
+  hipLaunchKernelGGL(gmf::k_bpp_m<0>, dim3(std::min(4096, bdiv((long long)f->R * f->mk * f->mp, 256))),
+                     dim3(256), 0, f->stream, a);
   F_HIP(hipGetLastError());
-  gmf::BVY v{s.V, s.Yg, f->n, f->R, f->m, f->ncol, s.X};
-  // (a vector-unit form with 8 rows a wavefront and scalar V loads measured
-  // 1.7x slower over the whole backward step)
-  hipLaunchKernelGGL(gmf::k_bvy_mfma<0>, dim3(f->R, bdiv(bdiv(f->n, 16), 4), f->ncol / 64), dim3(256), 0,
-                     f->stream, v);
+  gmf::BPPT t{f->sg.V, f->Mpp, f->n, f->R, f->m, f->mk, f->mp, f->pp_items, f->Tpp};
+  if (f->CW == 2)
+    launch_bpp_t<2>(t, f->LT, f->stream);
+  else
+    launch_bpp_t<4>(t, f->LT, f->stream);
   F_HIP(hipGetLastError());
   return GM_OK;
+}
+
+// The P-P sums of one parameter (E on A's pattern) for all R columns:
+// sum_l (E Vb[l]) . T[l] per column.  Each grid block adds its column sums for
+// l = 0, 1, ... to its own partials [G_sp][R] (launches in stream order),
+// which are then summed into pdot[G_sp R .. + R).  (One block per row block
+// measured 102 against 77 us per launch on 512^2 at 32 columns.)
+template <int R>
+gm_status fpp(gm_bfab* f, const double* E) {
+  const size_t NR = (size_t)f->n * R;
+  gmb::BSpmm a{};
+  a.rowptr = f->rowptr;
+  a.col = f->col;
+  a.val = E;
+  a.nrows = f->n;
+  a.chunk = f->chunk_sp;
+  a.rowblk = f->rowblk;
+  a.nrowblk = f->nrowblk;
+  a.ldx = R;
+  a.g0 = 1.0;
+  a.status = f->status;
+  a.partials = f->pdot;
+  for (int l = 0; l < f->m; ++l) {
+    a.x = f->sb.V + (size_t)l * NR;
+    a.vk = f->Tpp + (size_t)l * NR;
+    a.accumulate = l > 0;
+    hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kDot, true>), dim3(f->G_sp), dim3(gm::kBlock), 0, f->stream, a);
+    F_HIP(hipGetLastError());
+  }
+  gm::ReduceArgs r{};
+  r.partials = f->pdot;
+  r.G = f->G_sp;
+  r.nv = R;
+  r.out = f->pdot + (size_t)f->G_sp * R;
+  r.stride = 1;
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, f->stream, r);
+  F_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+gm_status fpp_dispatch(gm_bfab* f, const double* E) {
+  switch (f->R) {
+    case 2: return fpp<2>(f, E);
+    case 4: return fpp<4>(f, E);
+    case 8: return fpp<8>(f, E);
+    case 16: return fpp<16>(f, E);
+    case 32: return fpp<32>(f, E);
+    default: return fpp<64>(f, E);
+  }
 }
 
 // EQ = E Q ([N][K]) for E on A's pattern, and QEQ = Q^T E Q (host, K x K row-major).
@@ -599,6 +637,25 @@ void fshift_solve(int m, int R, int c, const double* al, const double* be, doubl
   }
 }
 
+template <int KT>
+void launch_bvtq_kt(const gmf::BVtQ& v, int G, hipStream_t st) {
+  const dim3 grid(G, v.m);
+  switch ((v.R + 15) / 16) {
+    case 1: hipLaunchKernelGGL((gmf::k_bvtq<KT, 1>), grid, dim3(256), 0, st, v); break;
+    case 2: hipLaunchKernelGGL((gmf::k_bvtq<KT, 2>), grid, dim3(256), 0, st, v); break;
+    default: hipLaunchKernelGGL((gmf::k_bvtq<KT, 4>), grid, dim3(256), 0, st, v); break;
+  }
+}
+
+void launch_bvtq(const gmf::BVtQ& v, int G, hipStream_t st) {
+  switch ((v.K + 15) / 16) {
+    case 1: launch_bvtq_kt<1>(v, G, st); break;
+    case 2: launch_bvtq_kt<2>(v, G, st); break;
+    case 3: launch_bvtq_kt<3>(v, G, st); break;
+    default: launch_bvtq_kt<4>(v, G, st); break;
+  }
+}
+
 gm_status fcheck_status(gm_bfab* f, const char* what) {
   unsigned stv = 0;
   F_HIP(hipMemcpyAsync(&stv, f->status, sizeof(unsigned), hipMemcpyDeviceToHost, f->stream));
@@ -659,7 +716,6 @@ gm_status gm_bfab_create(const gm_bfab_desc* d, gm_bfab** out) {
   f->R = R;
   f->m = d->subspace_size;
   f->ns = d->contour_size;
-  f->ncol = bdiv(2 * f->ns, 64) * 64;
   f->K = d->num_eigs;
   const size_t N = (size_t)f->n;
   const int rows_per_pass = gm::kWaves * (R < 64 ? 64 / R : 1);
@@ -670,7 +726,12 @@ gm_status gm_bfab_create(const gm_bfab_desc* d, gm_bfab** out) {
   f->G_df = std::max(1, std::min(256, bdiv(f->n, 1024)));
   f->chunk_df = bdiv(f->n, f->G_df);
   f->G_df = bdiv(f->n, f->chunk_df);
-  f->G_pg = std::max(1, std::min(1024, bdiv(f->n, 64)));
+  // T kernel: items (16-row tile, CW columns), l in tiles of 16 (at most 4 a pass)
+  f->CW = R == 2 ? 2 : 4;
+  f->LT = std::min(4, bdiv(f->m, 16));
+  f->mk = bdiv(f->m, 4) * 4;
+  f->mp = bdiv(f->m, 16 * f->LT) * 16 * f->LT;
+  f->pp_items = bdiv(f->n, 16) * (R / f->CW);
   hipStream_t st = f->stream;
   auto up = [&](auto** p, const auto* src, size_t cnt) -> gm_status {
     F_TRY(falloc(p, cnt));
@@ -719,8 +780,7 @@ gm_status gm_bfab_create(const gm_bfab_desc* d, gm_bfab** out) {
                                       (size_t)f->G_df * std::max(f->K, 1) * std::max(f->K, 1),
                                       (size_t)f->G_df * f->m * std::max(f->K, 1) * R});
   F_CREATE(falloc(&f->partials, np));
-  F_CREATE(falloc(&f->ppart, std::max<size_t>((size_t)2 * f->G_pg * R,
-                                               (size_t)f->m * std::max(f->K, 1) * R)));
+  F_CREATE(falloc(&f->ppart, (size_t)f->m * std::max(f->K, 1) * R));
   F_CREATE(falloc(&f->wD, ny));
   F_CREATE(falloc(&f->wDU, ny));
   F_CREATE(falloc(&f->wDU2, ny));
@@ -828,9 +888,7 @@ gm_status gm_bfab_backward_param(gm_bfab* f, const double* G, double* grad_B, do
   F_TRY(fkrylov_dispatch(f, f->sg, grad_B != nullptr, P > 0));
   if (grad_B) F_HIP(hipMemcpyAsync(grad_B, f->sg.out, NR * sizeof(double), hipMemcpyDefault, st));
   if (P > 0) {
-    // P-P blocks: X_b from the weighted solves, Z_g from the unit-weight ones
-    F_TRY(fresolvents(f, f->sb, f->sb.yr, f->sb.yi));
-    F_TRY(fresolvents(f, f->sg, f->sg.ur, f->sg.ui));
+    F_TRY(fppm(f));
     // deflation data of both sides, and H per column (host)
     std::vector<double> Hb, Hg, ab, ag;
     if (K) {
@@ -858,26 +916,10 @@ gm_status gm_bfab_backward_param(gm_bfab* f, const double* G, double* grad_B, do
       }
     }
     for (int q = 0; q < P; ++q) {
-      gmf::BPGrad pg{f->rowptr, f->col, f->pdA[q], f->sb.X, f->sg.X, f->n, f->ns, f->ncol,
-                     f->ppart};
-      const dim3 grid(f->G_pg), blk(256);
-      switch (R) {
-        case 2: hipLaunchKernelGGL(gmf::k_bparam_grad<2>, grid, blk, 0, st, pg); break;
-        case 4: hipLaunchKernelGGL(gmf::k_bparam_grad<4>, grid, blk, 0, st, pg); break;
-        case 8: hipLaunchKernelGGL(gmf::k_bparam_grad<8>, grid, blk, 0, st, pg); break;
-        case 16: hipLaunchKernelGGL(gmf::k_bparam_grad<16>, grid, blk, 0, st, pg); break;
-        case 32: hipLaunchKernelGGL(gmf::k_bparam_grad<32>, grid, blk, 0, st, pg); break;
-        default: hipLaunchKernelGGL(gmf::k_bparam_grad<64>, grid, blk, 0, st, pg); break;
-      }
-      F_HIP(hipGetLastError());
-      std::vector<double> pp((size_t)f->G_pg * R);
-      F_HIP(hipMemcpyAsync(pp.data(), f->ppart, pp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      F_TRY(fpp_dispatch(f, f->pdA[q]));
+      F_HIP(hipMemcpyAsync(grad_theta + (size_t)q * R, f->pdot + (size_t)f->G_sp * R, R * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
       F_HIP(hipStreamSynchronize(st));
-      for (int c = 0; c < R; ++c) {
-        double sum = 0.0;
-        for (int g = 0; g < f->G_pg; ++g) sum += pp[(size_t)g * R + c];
-        grad_theta[(size_t)q * R + c] = sum;
-      }
       if (K) {
         std::vector<double> Mb, Mg;
         for (int side = 0; side < 2; ++side) {
@@ -890,11 +932,8 @@ gm_status gm_bfab_backward_param(gm_bfab* f, const double* G, double* grad_B, do
           v.m = m;
           v.nrows = f->n;
           v.chunk = f->chunk_df;
-          v.KT = bdiv(K, 16);
-          v.CT = bdiv(R, 16);
           v.partials = f->partials;
-          hipLaunchKernelGGL(gmf::k_bvtq<0>, dim3(f->G_df, bdiv((long long)v.KT * v.CT, gm::kWaves), m),
-                             dim3(256), 0, st, v);
+          launch_bvtq(v, f->G_df, st);
           F_HIP(hipGetLastError());
           F_TRY(freduce(f, f->G_df, m * K * R, f->ppart, 0, 0));
           std::vector<double>& M = side ? Mg : Mb;   // [j][k][c]

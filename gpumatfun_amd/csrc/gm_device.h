@@ -94,6 +94,12 @@ struct ReduceArgs {
   size_t out_pstride;
 };
 
+// Unsegmented grid for nv values: one block per kReduceBlock of them.
+inline int reduce_grid(int nv) {
+  const int g = (nv + kReduceBlock - 1) / kReduceBlock;
+  return g < 1 ? 1 : (g > 256 ? 256 : g);
+}
+
 // (a template so that both translation units may instantiate it)
 template <int kUnused = 0>
 __global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs p) {
@@ -102,13 +108,18 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs p) {
   const double* part = p.partials;
   double* out = p.out;
   int G = p.G;
+  // unsegmented, the blocks split the values in runs of kReduceBlock (each
+  // value's sum is the same whatever the grid)
+  int vstart = blockIdx.x * kReduceBlock, vstep = gridDim.x * kReduceBlock;
   if (p.seg) {
     const int q = blockIdx.x, g0 = p.seg[q];
     part += (size_t)g0 * p.nv;
     G = p.seg[q + 1] - g0;
     out += (size_t)q * p.out_pstride;
+    vstart = 0;
+    vstep = kReduceBlock;
   }
-  for (int v0 = 0; v0 < p.nv; v0 += kReduceBlock) {
+  for (int v0 = vstart; v0 < p.nv; v0 += vstep) {
     const int W = min(p.nv - v0, kReduceBlock);
     const int ns = kReduceBlock / W;
     const int vv = t % W, sb = t / W, v = v0 + vv;
@@ -116,6 +127,20 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs p) {
     if (sb < ns) {
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
       int b = sb;
+      // 16 loads in flight, then the same additions in the same order as the
+      // loop below (the partials may sit in another XCD's L2 or in HBM)
+      for (; b + 15 * ns < G; b += 16 * ns) {
+        double w[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) w[u] = part[(size_t)(b + u * ns) * p.nv + v];
+#pragma unroll
+        for (int u = 0; u < 16; u += 4) {
+          s0 += w[u];
+          s1 += w[u + 1];
+          s2 += w[u + 2];
+          s3 += w[u + 3];
+        }
+      }
       for (; b + 3 * ns < G; b += 4 * ns) {
         s0 += part[(size_t)b * p.nv + v];
         s1 += part[(size_t)(b + ns) * p.nv + v];

@@ -229,7 +229,7 @@ gm_status gram(Eigs& e, const double* P, const double* U, std::vector<double>& o
   r.nv = e.s * e.s;
   r.out = e.small;
   r.stride = 1;
-  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kReduceBlock), 0, e.st, r);
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, e.st, r);
   E_HIP(hipGetLastError());
   out.resize((size_t)e.s * e.s);
   E_HIP(hipMemcpyAsync(out.data(), e.small, out.size() * sizeof(double), hipMemcpyDeviceToHost, e.st));
@@ -307,7 +307,7 @@ gm_status colnorms(Eigs& e, const double* y, std::vector<double>& nrm) {
   r.out = e.small;
   r.stride = 1;
   r.do_sqrt = 1;
-  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kReduceBlock), 0, e.st, r);
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, e.st, r);
   E_HIP(hipGetLastError());
   nrm.resize(S);
   E_HIP(hipMemcpyAsync(nrm.data(), e.small, S * sizeof(double), hipMemcpyDeviceToHost, e.st));

@@ -192,7 +192,7 @@ gm_status launch_spmm_mode(gm_solver* s, const SpmmArgs& a) {
   return GM_OK;
 }
 
-// Sum [G][nv] block partials into out[v * stride] (k_reduce, one block).
+// Sum [G][nv] block partials into out[v * stride] (k_reduce, a block per 1024 values).
 gm_status launch_reduce(gm_solver* s, int G, int nv, double* out, int stride, int do_sqrt,
                         int flag_zero, const double* part = nullptr) {
   gm::ReduceArgs r{};
@@ -204,7 +204,7 @@ gm_status launch_reduce(gm_solver* s, int G, int nv, double* out, int stride, in
   r.do_sqrt = do_sqrt;
   r.flag_zero = flag_zero;
   r.status = s->status;
-  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(1), dim3(gm::kReduceBlock), 0, s->stream, r);
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, s->stream, r);
   GM_HIP(hipGetLastError());
   return GM_OK;
 }
