@@ -438,39 +438,53 @@ __global__ __launch_bounds__(64) void k_bsolves(BSolve p) {
   const int m = p.m;
   auto diag = [&](int k) { return d2(p.al[(size_t)k * p.R + c] - ex.x, -ex.y); };
   auto off = [&](int k) { return d2(p.be[(size_t)(k + 1) * p.R + c], 0.0); };
+  // Both sweeps load the operands of kSolveSteps steps together (the steps
+  // themselves are unchanged), so a dependent chain waits on memory once per
+  // kSolveSteps steps rather than on every step.
+  constexpr int U = kSolveSteps;
   double2 cur_d = diag(0);
   double2 cur_x = d2(-(b0 * wt.x), -(b0 * wt.y));
   double2 cur_du = m > 1 ? off(0) : d2(0.0, 0.0);
   bool singular = false;
-  for (int k = 0; k < m - 1; ++k) {
-    const double2 dl = off(k);
-    double2 next_d = diag(k + 1), next_x = d2(0.0, 0.0);
-    double2 next_du = (k + 1 < m - 1) ? off(k + 1) : d2(0.0, 0.0);
-    double2 sD, sDU, sDU2 = d2(0.0, 0.0), sX;
-    if (dl.x == 0.0 && dl.y == 0.0) {
-      if (cur_d.x == 0.0 && cur_d.y == 0.0) singular = true;
-      sD = cur_d; sDU = cur_du; sX = cur_x;
-    } else if (cabs1(cur_d) >= cabs1(dl)) {
-      const double2 mult = cdiv(dl, cur_d);
-      next_d = csub(next_d, cmul(mult, cur_du));
-      next_x = csub(next_x, cmul(mult, cur_x));
-      sD = cur_d; sDU = cur_du; sX = cur_x;
-    } else {
-      const double2 mult = cdiv(cur_d, dl);
-      sD = dl;
-      const double2 temp = next_d;
-      next_d = csub(cur_du, cmul(mult, temp));
-      if (k < m - 2) {
-        sDU2 = next_du;
-        next_du = d2(-(mult.x * sDU2.x - mult.y * sDU2.y), -(mult.x * sDU2.y + mult.y * sDU2.x));
+  for (int k0 = 0; k0 < m - 1; k0 += U) {
+    double av[U], bv[U + 1];   // al[k0 + 1 + u], be[k0 + 1 + u]
+#pragma unroll
+    for (int u = 0; u < U; ++u) av[u] = k0 + 1 + u < m ? p.al[(size_t)(k0 + 1 + u) * p.R + c] : 0.0;
+#pragma unroll
+    for (int u = 0; u <= U; ++u) bv[u] = k0 + 1 + u < m ? p.be[(size_t)(k0 + 1 + u) * p.R + c] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      if (k >= m - 1) break;
+      const double2 dl = d2(bv[u], 0.0);                                   // off(k)
+      double2 next_d = d2(av[u] - ex.x, -ex.y), next_x = d2(0.0, 0.0);     // diag(k + 1)
+      double2 next_du = (k + 1 < m - 1) ? d2(bv[u + 1], 0.0) : d2(0.0, 0.0);   // off(k + 1)
+      double2 sD, sDU, sDU2 = d2(0.0, 0.0), sX;
+      if (dl.x == 0.0 && dl.y == 0.0) {
+        if (cur_d.x == 0.0 && cur_d.y == 0.0) singular = true;
+        sD = cur_d; sDU = cur_du; sX = cur_x;
+      } else if (cabs1(cur_d) >= cabs1(dl)) {
+        const double2 mult = cdiv(dl, cur_d);
+        next_d = csub(next_d, cmul(mult, cur_du));
+        next_x = csub(next_x, cmul(mult, cur_x));
+        sD = cur_d; sDU = cur_du; sX = cur_x;
+      } else {
+        const double2 mult = cdiv(cur_d, dl);
+        sD = dl;
+        const double2 temp = next_d;
+        next_d = csub(cur_du, cmul(mult, temp));
+        if (k < m - 2) {
+          sDU2 = next_du;
+          next_du = d2(-(mult.x * sDU2.x - mult.y * sDU2.y), -(mult.x * sDU2.y + mult.y * sDU2.x));
+        }
+        sDU = temp;
+        sX = next_x;
+        next_x = csub(cur_x, cmul(mult, next_x));
       }
-      sDU = temp;
-      sX = next_x;
-      next_x = csub(cur_x, cmul(mult, next_x));
+      const size_t o = (size_t)k * S + sys;
+      p.wD[o] = sD; p.wDU[o] = sDU; p.wDU2[o] = sDU2; p.wX[o] = sX;
+      cur_d = next_d; cur_x = next_x; cur_du = next_du;
     }
-    const size_t o = (size_t)k * S + sys;
-    p.wD[o] = sD; p.wDU[o] = sDU; p.wDU2[o] = sDU2; p.wX[o] = sX;
-    cur_d = next_d; cur_x = next_x; cur_du = next_du;
   }
   if (cur_d.x == 0.0 && cur_d.y == 0.0) singular = true;
   double* yr = p.yr + (size_t)sys * m;
@@ -488,15 +502,27 @@ __global__ __launch_bounds__(64) void k_bsolves(BSolve p) {
     double2 x1 = cdiv(csub(p.wX[o], cmul(p.wDU[o], x2)), p.wD[o]);
     yr[m - 2] = x1.x;
     yi[m - 2] = x1.y;
-    for (int k = m - 3; k >= 0; --k) {
-      const size_t q = (size_t)k * S + sys;
-      double2 t = csub(p.wX[q], cmul(p.wDU[q], x1));
-      t = csub(t, cmul(p.wDU2[q], x2));
-      const double2 xk = cdiv(t, p.wD[q]);
-      yr[k] = xk.x;
-      yi[k] = xk.y;
-      x2 = x1;
-      x1 = xk;
+    for (int k1 = m - 3; k1 >= 0; k1 -= U) {
+      double2 fX[U], fDU[U], fDU2[U], fD[U];   // step k = k1 - u
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (k1 - u >= 0) {
+          const size_t q = (size_t)(k1 - u) * S + sys;
+          fX[u] = p.wX[q]; fDU[u] = p.wDU[q]; fDU2[u] = p.wDU2[q]; fD[u] = p.wD[q];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k1 - u;
+        if (k < 0) break;
+        double2 t = csub(fX[u], cmul(fDU[u], x1));
+        t = csub(t, cmul(fDU2[u], x2));
+        const double2 xk = cdiv(t, fD[u]);
+        yr[k] = xk.x;
+        yi[k] = xk.y;
+        x2 = x1;
+        x1 = xk;
+      }
     }
   }
 }

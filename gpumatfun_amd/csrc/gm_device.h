@@ -81,6 +81,10 @@ __device__ __forceinline__ double2 block_reduce_partials2(const double* part, in
 // bits: 4 = non-finite result, 1 = exact zero (when flag_zero).
 constexpr int kReduceBlock = 1024;
 
+// Steps of a tridiagonal sweep whose operands are loaded together in the
+// shifted solves.
+constexpr int kSolveSteps = 8;
+
 // Segmented form (seg != nullptr): one block per segment q = blockIdx.x sums
 // partials seg[q] .. seg[q+1]-1 into out + q * out_pstride.
 struct ReduceArgs {
