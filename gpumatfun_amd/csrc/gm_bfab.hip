@@ -17,8 +17,8 @@
 //   cross       sum_k (Q^T g)_k q_k^T E V_b H_b[:, k] + (b <-> g), with
 //               H[:, k] = (lambda_k - T)^{-1} (f(lambda_k) beta0 e1 - C).
 // The P-P block is contracted in the Krylov bases, (Vg^T E Vb) against an
-// m x m matrix M per column: T = Vg M on FP64 MFMA, then (E Vb) . T by the
-// batched SpMM, one launch per Krylov vector.
+// m x m matrix M per column: T = Vg M on FP64 MFMA, then (E Vb) . T in one
+// pass over the rows.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,6 +32,7 @@
 
 #include "../../include/gpumatfun.h"
 #include "gm_bkernels.h"
+
 
 namespace gmf {
 using namespace gm;
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(256) void k_bpp_m(BPPM p) {
 // one tile are consecutive, so a block's wavefronts share Vg's lines) and l in
 // blocks of 16 LT; each lane stores its row's CW columns as one vector per l.
 // The P-P sum of a parameter is then sum_l sum_i (E Vb[l])[i][c] T[l][i][c]
-// (the batched SpMM in its kDot form, one launch per l).
+// (k_bpp_dot).
 struct BPPT {
   const double* Vg;   // [m][N][R]
   const double* M;    // [R][mk][mp]
@@ -131,6 +132,88 @@ __global__ __launch_bounds__(256) void k_bpp_t(BPPT p) {
         }
       }
   }
+}
+
+// sum_l sum_i (E Vb[l])[i][c] T[l][i][c] per column in one pass: a lane takes
+// one column of a row (rows interleaved over the whole grid, so the blocks
+// work on one front of rows and share the gathered rows of Vb in L2), loads
+// the row's nonzeros once, and runs the Krylov vectors l in pairs, the
+// gathers of both issued before the first FMA.  Each (row, l) term is summed
+// over the nonzeros in order, and a lane adds its terms row by row, l by l.
+// partials[G][R] (lanes, then the wavefronts of a block, in a fixed order).
+struct BPD {
+  const int* rowptr;
+  const int* col;
+  const double* E;
+  const double* Vb;   // [m][N][R]
+  const double* T;    // [m][N][R]
+  int N, m;
+  double* partials;
+};
+
+template <int R>
+__global__ __launch_bounds__(256) void k_bpp_dot(BPD p) {
+  constexpr int CPW = R < 64 ? R : 64;
+  constexpr int RPW = 64 / CPW;
+  constexpr int CL = R / CPW;
+  constexpr int U = 12, LB = 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int slot = lane / CPW, c0 = lane % CPW;
+  const size_t slice = (size_t)p.N * R;
+  double dot[CL];
+#pragma unroll
+  for (int q = 0; q < CL; ++q) dot[q] = 0.0;
+  for (int i = (blockIdx.x * kWaves + wave) * RPW + slot; i < p.N; i += gridDim.x * kWaves * RPW) {
+    const int e0 = p.rowptr[i], e1 = p.rowptr[i + 1];
+    if (e1 - e0 <= U) {
+      double ev[U];
+      int ci[U];
+#pragma unroll
+      for (int t = 0; t < U; ++t) {
+        const bool ok = e0 + t < e1;
+        ev[t] = ok ? p.E[e0 + t] : 0.0;
+        ci[t] = ok ? p.col[e0 + t] : i;
+      }
+      for (int l0 = 0; l0 < p.m; l0 += LB) {
+        double xv[LB][U][CL], tv[LB][CL];
+#pragma unroll
+        for (int b = 0; b < LB; ++b) {
+          const int l = l0 + b < p.m ? l0 + b : l0;   // (a repeated l is not added)
+          const double* Vl = p.Vb + (size_t)l * slice + c0;
+#pragma unroll
+          for (int t = 0; t < U; ++t)
+#pragma unroll
+            for (int q = 0; q < CL; ++q) xv[b][t][q] = e0 + t < e1 ? Vl[(size_t)ci[t] * R + q * CPW] : 0.0;
+#pragma unroll
+          for (int q = 0; q < CL; ++q) tv[b][q] = p.T[(size_t)l * slice + (size_t)i * R + c0 + q * CPW];
+        }
+#pragma unroll
+        for (int b = 0; b < LB; ++b) {
+          if (l0 + b >= p.m) break;
+#pragma unroll
+          for (int q = 0; q < CL; ++q) {
+            double sv = 0.0;
+#pragma unroll
+            for (int t = 0; t < U; ++t)
+              if (e0 + t < e1) sv = fma(ev[t], xv[b][t][q], sv);
+            dot[q] = fma(sv, tv[b][q], dot[q]);
+          }
+        }
+      }
+    } else {
+      // a long row: nonzero by nonzero for each l
+      for (int l = 0; l < p.m; ++l) {
+        const double* Vl = p.Vb + (size_t)l * slice + c0;
+#pragma unroll
+        for (int q = 0; q < CL; ++q) {
+          double sv = 0.0;
+          for (int e = e0; e < e1; ++e) sv = fma(p.E[e], Vl[(size_t)p.col[e] * R + q * CPW], sv);
+          dot[q] = fma(sv, p.T[(size_t)l * slice + (size_t)i * R + c0 + q * CPW], dot[q]);
+        }
+      }
+    }
+  }
+  gmb::block_partial_cols<R>(dot, p.partials);
 }
 
 // Block partials of M[j][k][c] = sum_i V[j][i][c] EQ[i][k] for every Krylov
@@ -257,6 +340,7 @@ struct gm_bfab {
   bool have_forward = false;
   int G_sp = 1, chunk_sp = 1, nrowblk = 0, G_df = 1, chunk_df = 1;
   int CW = 4, LT = 1, mk = 4, mp = 16, pp_items = 0;   // T kernel shape
+  int G_pd = 1;                                         // one-pass P-P dot grid
   int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr;
   double *val = nullptr, *xi = nullptr, *wts1 = nullptr, *wts2 = nullptr, *ones = nullptr,
          *gsr = nullptr, *gsi = nullptr, *Q = nullptr, *el = nullptr, *pl = nullptr,
@@ -493,7 +577,7 @@ gm_status fppm(gm_bfab* f) {
   const size_t N = (size_t)f->n;
   if (!f->Mpp) F_TRY(falloc(&f->Mpp, (size_t)f->R * f->mk * f->mp));
   if (!f->Tpp) F_TRY(falloc(&f->Tpp, N * f->R * f->m));
-  if (!f->pdot) F_TRY(falloc(&f->pdot, ((size_t)f->G_sp + 1) * f->R));
+  if (!f->pdot) F_TRY(falloc(&f->pdot, ((size_t)f->G_pd + 1) * f->R));
   gmf::BPPM a{f->sg.ur, f->sg.ui, f->sb.yr, f->sb.yi, f->R, f->m, f->ns, f->mk, f->mp, f->Mpp};
   hipLaunchKernelGGL(gmf::k_bpp_m<0>, dim3(std::min(4096, bdiv((long long)f->R * f->mk * f->mp, 256))),
                      dim3(256), 0, f->stream, a);
@@ -508,37 +592,19 @@ gm_status fppm(gm_bfab* f) {
 }
 
 // The P-P sums of one parameter (E on A's pattern) for all R columns:
-// sum_l (E Vb[l]) . T[l] per column.  Each grid block adds its column sums for
-// l = 0, 1, ... to its own partials [G_sp][R] (launches in stream order),
-// which are then summed into pdot[G_sp R .. + R).  (One block per row block
-// measured 102 against 77 us per launch on 512^2 at 32 columns.)
+// sum_l (E Vb[l]) . T[l] per column by k_bpp_dot, block partials [G_pd][R]
+// summed into pdot[G_pd R .. + R).  (m launches of the batched SpMM, one per
+// l, measured 3.1 against 2.25 ms on 512^2 at 32 columns, m = 40.)
 template <int R>
 gm_status fpp(gm_bfab* f, const double* E) {
-  const size_t NR = (size_t)f->n * R;
-  gmb::BSpmm a{};
-  a.rowptr = f->rowptr;
-  a.col = f->col;
-  a.val = E;
-  a.nrows = f->n;
-  a.chunk = f->chunk_sp;
-  a.rowblk = f->rowblk;
-  a.nrowblk = f->nrowblk;
-  a.ldx = R;
-  a.g0 = 1.0;
-  a.status = f->status;
-  a.partials = f->pdot;
-  for (int l = 0; l < f->m; ++l) {
-    a.x = f->sb.V + (size_t)l * NR;
-    a.vk = f->Tpp + (size_t)l * NR;
-    a.accumulate = l > 0;
-    hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kDot, true>), dim3(f->G_sp), dim3(gm::kBlock), 0, f->stream, a);
-    F_HIP(hipGetLastError());
-  }
+  gmf::BPD d{f->rowptr, f->col, E, f->sb.V, f->Tpp, f->n, f->m, f->pdot};
+  hipLaunchKernelGGL((gmf::k_bpp_dot<R>), dim3(f->G_pd), dim3(256), 0, f->stream, d);
+  F_HIP(hipGetLastError());
   gm::ReduceArgs r{};
   r.partials = f->pdot;
-  r.G = f->G_sp;
+  r.G = f->G_pd;
   r.nv = R;
-  r.out = f->pdot + (size_t)f->G_sp * R;
+  r.out = f->pdot + (size_t)f->G_pd * R;
   r.stride = 1;
   hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, f->stream, r);
   F_HIP(hipGetLastError());
@@ -732,6 +798,7 @@ gm_status gm_bfab_create(const gm_bfab_desc* d, gm_bfab** out) {
   f->mk = bdiv(f->m, 4) * 4;
   f->mp = bdiv(f->m, 16 * f->LT) * 16 * f->LT;
   f->pp_items = bdiv(f->n, 16) * (R / f->CW);
+  f->G_pd = std::max(1, std::min(2048, bdiv(f->n, gm::kWaves * (R < 64 ? 64 / R : 1))));
   hipStream_t st = f->stream;
   auto up = [&](auto** p, const auto* src, size_t cnt) -> gm_status {
     F_TRY(falloc(p, cnt));
@@ -917,7 +984,7 @@ gm_status gm_bfab_backward_param(gm_bfab* f, const double* G, double* grad_B, do
     }
     for (int q = 0; q < P; ++q) {
       F_TRY(fpp_dispatch(f, f->pdA[q]));
-      F_HIP(hipMemcpyAsync(grad_theta + (size_t)q * R, f->pdot + (size_t)f->G_sp * R, R * sizeof(double),
+      F_HIP(hipMemcpyAsync(grad_theta + (size_t)q * R, f->pdot + (size_t)f->G_pd * R, R * sizeof(double),
                            hipMemcpyDeviceToHost, st));
       F_HIP(hipStreamSynchronize(st));
       if (K) {

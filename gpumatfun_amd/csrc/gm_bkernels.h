@@ -25,9 +25,7 @@ constexpr int kUnrollStaged = GMB_STAGED_UNROLL;
 
 // kCheb: y = ca (A x - shift_c x) + cb z (a Chebyshev filter step, or with
 // ca = 1, cb = 0 a residual A x - theta_c x); shift_c = shiftc[c], or cc.
-// kDot: no y; the column sums of (A x) * vk go to the block partials (added to
-// them with accumulate).
-enum Mode : int { kFirst = 0, kMid = 1, kLast = 2, kOnly = 3, kHorner = 4, kCheb = 5, kDot = 6 };
+enum Mode : int { kFirst = 0, kMid = 1, kLast = 2, kOnly = 3, kHorner = 4, kCheb = 5 };
 
 struct BSpmm {
   const int* __restrict__ rowptr;
@@ -87,8 +85,7 @@ __device__ __forceinline__ void block_reduce_cols(const double* part, int G, int
 // Per-column block sums (fixed order) written to partials[block][R]; the grid
 // total is formed by k_reduce after the kernel.
 template <int R>
-__device__ void block_partial_cols(const double (&v)[(R < 64 ? 1 : R / 64)], double* partials,
-                                   bool add = false) {
+__device__ void block_partial_cols(const double (&v)[(R < 64 ? 1 : R / 64)], double* partials) {
   constexpr int CPW = R < 64 ? R : 64;
   constexpr int RPW = 64 / CPW;
   constexpr int CL = R / CPW;
@@ -100,8 +97,7 @@ __device__ void block_partial_cols(const double (&v)[(R < 64 ? 1 : R / 64)], dou
     double s = 0.0;
     for (int w = 0; w < kWaves; ++w)
       for (int sl = 0; sl < RPW; ++sl) s += sh[(w * 64 + sl * CPW + lanec) * CL + q];
-    double* dst = partials + (size_t)blockIdx.x * R + c;
-    *dst = add ? *dst + s : s;
+    partials[(size_t)blockIdx.x * R + c] = s;
   }
 }
 
@@ -219,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
           o0[q] = 0.0;
           o1[q] = 0.0;
           if (MODE == kFirst || MODE == kOnly) o0[q] = p.x[(size_t)r * p.ldx + c];
-          if (MODE == kMid || MODE == kLast || MODE == kDot) o0[q] = p.vk[iv];
+          if (MODE == kMid || MODE == kLast) o0[q] = p.vk[iv];
           if ((MODE == kLast || MODE == kOnly) && p.k > 0) o1[q] = p.vkm1[iv];
           if (MODE == kHorner) {
             o0[q] = p.z[(size_t)r * p.ldz + c];
@@ -283,8 +279,6 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
             if (p.k > 0) yv = fma(-bprev[q], o1[q], yv);
             p.y[iy] = yv;
             dot[q] = fma(yv, v, dot[q]);
-          } else if (MODE == kDot) {
-            dot[q] = fma(acc[q], o0[q], dot[q]);
           } else if (MODE == kCheb) {
             const double sh = p.shiftc ? p.shiftc[c] : p.cc;
             p.y[iy] = fma(p.cb, o1[q], p.ca * fma(-sh, o0[q], acc[q]));
@@ -302,8 +296,8 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
       rows(std::false_type{});
     }
   }
-  if (MODE == kLast || MODE == kOnly || MODE == kDot)
-    block_partial_cols<R>(dot, p.partials, MODE == kDot && p.accumulate);
+  if (MODE == kLast || MODE == kOnly)
+    block_partial_cols<R>(dot, p.partials);
 }
 
 struct BFinish {

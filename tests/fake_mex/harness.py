@@ -28,21 +28,31 @@ GATEWAYS = {
 
 
 def build():
+    """Compile the gateways and the fake MEX runtime into OUT.  Several pytest
+    workers may call this at once: the build runs under an exclusive lock,
+    from per-process object files, and OUT is replaced atomically."""
+    import fcntl
     bdir = os.path.dirname(OUT)
     os.makedirs(bdir, exist_ok=True)
-    objs = []
-    inc = ["-I" + HERE, "-I" + os.path.join(ROOT, "include")]
-    for name, gid in GATEWAYS.items():
-        o = os.path.join(bdir, f"{name}.o")
-        subprocess.run(["g++", "-O2", "-fPIC", "-std=c++17", "-c", SRC, "-o", o,
-                        f"-DGM_GATEWAY={gid}", f"-DmexFunction=gmx_{name}"] + inc, check=True)
-        objs.append(o)
-    fo = os.path.join(bdir, "fake_mex.o")
-    subprocess.run(["g++", "-O2", "-fPIC", "-std=c++17", "-c",
-                    os.path.join(HERE, "fake_mex.cpp"), "-o", fo] + inc, check=True)
-    subprocess.run(["g++", "-shared", "-o", OUT + ".tmp"] + objs + [fo] +
-                   ["-L" + LIBDIR, "-lgpumatfun_amd", "-Wl,-rpath," + LIBDIR], check=True)
-    os.replace(OUT + ".tmp", OUT)
+    with open(os.path.join(bdir, ".lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        tag = str(os.getpid())
+        objs = []
+        inc = ["-I" + HERE, "-I" + os.path.join(ROOT, "include")]
+        for name, gid in GATEWAYS.items():
+            o = os.path.join(bdir, f"{name}.{tag}.o")
+            subprocess.run(["g++", "-O2", "-fPIC", "-std=c++17", "-c", SRC, "-o", o,
+                            f"-DGM_GATEWAY={gid}", f"-DmexFunction=gmx_{name}"] + inc, check=True)
+            objs.append(o)
+        fo = os.path.join(bdir, f"fake_mex.{tag}.o")
+        subprocess.run(["g++", "-O2", "-fPIC", "-std=c++17", "-c",
+                        os.path.join(HERE, "fake_mex.cpp"), "-o", fo] + inc, check=True)
+        tmp = f"{OUT}.{tag}.tmp"
+        subprocess.run(["g++", "-shared", "-o", tmp] + objs + [fo] +
+                       ["-L" + LIBDIR, "-lgpumatfun_amd", "-Wl,-rpath," + LIBDIR], check=True)
+        os.replace(tmp, OUT)
+        for o in objs + [fo]:
+            os.remove(o)
     return OUT
 
 

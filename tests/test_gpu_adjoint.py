@@ -256,3 +256,45 @@ def test_deflated_param_gradient_vs_exact_reference():
     assert rel(u, ref_u) < 1e-10
     assert rel(gb, ref_gb) < 1e-10
     assert abs(gr - ref) <= 1e-8 * abs(ref)
+
+
+@pytest.mark.parametrize("m,K", [(30, 0), (70, 0), (70, 4)])
+def test_batch_param_gradient_matches_single_column(m, K):
+    """gm_bfab_backward_param (the P-P block contracted in the Krylov bases:
+    T = V_g M on MFMA, l in tiles of 16 up to 64 a pass, so m = 70 takes two
+    passes) against gm_fab_backward_param column by column (resolvent blocks,
+    pinned against exact resolvents above), with and without deflation.  Both
+    run the same Krylov approximation; they differ in reduction partitions, so
+    they agree to rounding amplified by m Lanczos steps (1e-9)."""
+    from gpumatfun_amd.adjoint import BatchFabEngine
+    n, r, R = 24, 4.0, 4
+    A = problems.anisotropic_9pt(n, ratio=r)
+    E = problems.anisotropic_9pt_ratio_derivative(n)
+    w, U = np.linalg.eigh(A.toarray())
+    c = q.contour_nodes(0.95 * w[K], 1.05 * w.max(), 24, TAU, 1.0)
+    cw, _ = contour_weights(c["xi"], c["dxidt"], c["s"], TAU, ALPHA)
+    kw1, kwb = {}, {}
+    if K:
+        f = (lambda x: q.f1(x, TAU, ALPHA))
+        F = q.divided_differences(f, lambda x: q.df1(x, TAU, ALPHA), w[:K])
+        kw1 = dict(Q=U[:, :K], lam=w[:K], f_lambda=f(w[:K]), f_divdiff=F)
+        kwb = dict(Q=U[:, :K], lam=w[:K], f1_lambda=f(w[:K]), f2_lambda=f(w[:K]),
+                   f1_divdiff=F, f2_divdiff=F)
+    rng = np.random.default_rng(11)
+    B = rng.standard_normal((n * n, R))
+    G = rng.standard_normal((n * n, R))
+    print(f"case batch param gradient m={m} K={K}", flush=True)
+    be = BatchFabEngine(A, R=R, m=m, xi=c["xi"], wts1=cw, wts2=cw, **kwb)
+    be.set_param_derivs([E])
+    UB = be.forward(B)
+    GB, gt = be.backward_param(G)
+    be.delete()
+    for col in range(R):
+        eng = FabEngine(A, m=m, xi=c["xi"], wts=cw, **kw1)
+        u = eng.forward(B[:, col])
+        gb, (gr,) = eng.backward_param(G[:, col], [E], want_b=True)
+        eng.delete()
+        print(f"column {col}: u {rel(UB[:, col], u):.2e} grad_b {rel(GB[:, col], gb):.2e} "
+              f"dL/dr {gt[0, col]:.15e} {gr:.15e}")
+        assert rel(UB[:, col], u) < 1e-10 and rel(GB[:, col], gb) < 1e-10
+        assert abs(gt[0, col] - gr) <= 1e-9 * abs(gr)
