@@ -394,8 +394,7 @@ gm_status enqueue_deflate(gm_batch* b) {
   q.KT = cdivi(b->K, 16);
   q.CT = cdivi(b->R, 16);
   q.partials = b->partials;
-  hipLaunchKernelGGL(gmb::k_bdefl_qtu<0>, dim3(b->G_df, cdivi((long long)q.KT * q.CT, gm::kWaves)),
-                     dim3(gm::kBlock), 0, b->stream, q);
+  gmb::launch_bdefl_qtu(q, b->G_df, b->stream);
   B_HIP(hipGetLastError());
   B_TRY(breduce(b, b->K * b->R, b->Zd, 0, 0, b->G_df));
   gmb::BDeflApply a{};

@@ -413,8 +413,7 @@ gm_status fdeflate(gm_bfab* f, FSide& s) {
   q.KT = bdiv(f->K, 16);
   q.CT = bdiv(f->R, 16);
   q.partials = f->partials;
-  hipLaunchKernelGGL(gmb::k_bdefl_qtu<0>, dim3(f->G_df, bdiv((long long)q.KT * q.CT, gm::kWaves)),
-                     dim3(gm::kBlock), 0, f->stream, q);
+  gmb::launch_bdefl_qtu(q, f->G_df, f->stream);
   F_HIP(hipGetLastError());
   F_TRY(freduce(f, f->G_df, f->K * f->R, s.Zd, 0, 0));
   gmb::BDeflApply a{};
@@ -654,8 +653,7 @@ gm_status feq(gm_bfab* f, const double* E, double* EQ, std::vector<double>& QEQ)
   q.KT = bdiv(f->K, 16);
   q.CT = bdiv(f->K, 16);
   q.partials = f->partials;
-  hipLaunchKernelGGL(gmb::k_bdefl_qtu<0>, dim3(f->G_df, bdiv((long long)q.KT * q.CT, gm::kWaves)),
-                     dim3(gm::kBlock), 0, f->stream, q);
+  gmb::launch_bdefl_qtu(q, f->G_df, f->stream);
   F_HIP(hipGetLastError());
   F_TRY(freduce(f, f->G_df, f->K * f->K, f->ppart, 0, 0));
   QEQ.resize((size_t)f->K * f->K);

@@ -677,28 +677,86 @@ struct BDeflQtU {
   double* partials;   // [G][K][R]
 };
 
-template <int kUnused = 0>
+// Block partials of Q^T U on FP64 MFMA (v_mfma_f64_16x16x4f64; A = Q rows k,
+// B = U columns c, D register i of lane l = (k 4i + l/16, c l%16)).  Each
+// wavefront holds all KT k-tiles and CTW c-tiles of the block's column group
+// (block y), so one load of Q feeds CTW chains and one of U feeds KT; the 4
+// wavefronts take interleaved 4-row steps of the block's rows and are summed
+// in LDS in wavefront order.  K <= 64.
+template <int KT, int CTW>
 __global__ __launch_bounds__(kBlock) void k_bdefl_qtu(BDeflQtU p) {
-  const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.y * kWaves + (threadIdx.x >> 6);
-  if (tile >= p.KT * p.CT) return;   // whole wavefronts; no barrier follows
-  const int kt = tile % p.KT, ct = tile / p.KT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ct0 = blockIdx.y * CTW;
   const int row0 = blockIdx.x * p.chunk, row1 = min(row0 + p.chunk, p.nrows);
-  const int kk = kt * 16 + (lane & 15);
-  const int c = ct * 16 + (lane & 15);
   const int sub = lane >> 4;
-  d4 acc = {0.0, 0.0, 0.0, 0.0};
-  for (int r = row0; r < row1; r += 4) {
+  __shared__ double sh[KT * CTW * 4 * 64];
+  d4 acc[KT][CTW];
+#pragma unroll
+  for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+    for (int ct = 0; ct < CTW; ++ct) acc[kt][ct] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int r = row0 + 4 * wave; r < row1; r += 4 * kWaves) {
     const int row = r + sub;
     const bool ok = row < row1;
-    const double a = (ok && kk < p.K) ? p.Q[(size_t)row * p.K + kk] : 0.0;
-    const double b = (ok && c < p.R) ? p.U[(size_t)row * p.R + c] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-  }
+    double a[KT], b[CTW];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int k2 = kt * 16 + 4 * i + sub;
-    if (k2 < p.K && c < p.R) p.partials[((size_t)blockIdx.x * p.K + k2) * p.R + c] = acc[i];
+    for (int kt = 0; kt < KT; ++kt) {
+      const int kk = kt * 16 + (lane & 15);
+      a[kt] = (ok && kk < p.K) ? p.Q[(size_t)row * p.K + kk] : 0.0;
+    }
+#pragma unroll
+    for (int ct = 0; ct < CTW; ++ct) {
+      const int c = (ct0 + ct) * 16 + (lane & 15);
+      b[ct] = (ok && c < p.R) ? p.U[(size_t)row * p.R + c] : 0.0;
+    }
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+      for (int ct = 0; ct < CTW; ++ct)
+        acc[kt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kt], b[ct], acc[kt][ct], 0, 0, 0);
+  }
+  for (int w = 0; w < kWaves; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+        for (int ct = 0; ct < CTW; ++ct)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            double& d = sh[((kt * CTW + ct) * 4 + i) * 64 + lane];
+            d = w == 0 ? acc[kt][ct][i] : d + acc[kt][ct][i];
+          }
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < KT * CTW * 4 * 64; t += kBlock) {
+    const int ln = t & 63, i = (t >> 6) & 3, tile = t >> 8;
+    const int k2 = (tile / CTW) * 16 + 4 * i + (ln >> 4);
+    const int c = (ct0 + tile % CTW) * 16 + (ln & 15);
+    if (k2 < p.K && c < p.R) p.partials[((size_t)blockIdx.x * p.K + k2) * p.R + c] = sh[t];
+  }
+}
+
+// Launch over G row chunks (q.chunk rows each) and the column groups of up to
+// four 16-column tiles.
+template <int KT>
+inline void launch_bdefl_qtu_kt(const BDeflQtU& q, int G, hipStream_t st) {
+  const int CT = (q.R + 15) / 16;
+  if (CT == 1) {
+    hipLaunchKernelGGL((k_bdefl_qtu<KT, 1>), dim3(G, 1), dim3(kBlock), 0, st, q);
+  } else if (CT == 2) {
+    hipLaunchKernelGGL((k_bdefl_qtu<KT, 2>), dim3(G, 1), dim3(kBlock), 0, st, q);
+  } else {
+    hipLaunchKernelGGL((k_bdefl_qtu<KT, 4>), dim3(G, (CT + 3) / 4), dim3(kBlock), 0, st, q);
+  }
+}
+
+inline void launch_bdefl_qtu(const BDeflQtU& q, int G, hipStream_t st) {
+  switch ((q.K + 15) / 16) {
+    case 1: launch_bdefl_qtu_kt<1>(q, G, st); break;
+    case 2: launch_bdefl_qtu_kt<2>(q, G, st); break;
+    case 3: launch_bdefl_qtu_kt<3>(q, G, st); break;
+    default: launch_bdefl_qtu_kt<4>(q, G, st); break;
   }
 }
 

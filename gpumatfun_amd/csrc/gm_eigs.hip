@@ -220,8 +220,7 @@ gm_status gram(Eigs& e, const double* P, const double* U, std::vector<double>& o
   q.KT = ediv(e.s, 16);
   q.CT = ediv(e.s, 16);
   q.partials = e.partials;
-  hipLaunchKernelGGL(gmb::k_bdefl_qtu<0>, dim3(e.G_df, ediv((long long)q.KT * q.CT, gm::kWaves)),
-                     dim3(gm::kBlock), 0, e.st, q);
+  gmb::launch_bdefl_qtu(q, e.G_df, e.st);
   E_HIP(hipGetLastError());
   gm::ReduceArgs r{};
   r.partials = e.partials;
