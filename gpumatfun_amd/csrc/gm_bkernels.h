@@ -22,6 +22,8 @@ constexpr int kUnroll = 8;   // nonzeros per gather batch in k_bspmm
 #endif
 // nonzeros per gather batch when the matrix is staged in LDS
 constexpr int kUnrollStaged = GMB_STAGED_UNROLL;
+// rows per pass of a lane in the finish kernel (4 and 8 measured the same)
+constexpr int kFinishRows = 2;
 
 // kCheb: y = ca (A x - shift_c x) + cb z (a Chebyshev filter step, or with
 // ca = 1, cb = 0 a residual A x - theta_c x); shift_c = shiftc[c], or cc.
@@ -337,33 +339,30 @@ __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
     al[q] = p.apart ? fused[c0 + q * CPW] : p.al[(size_t)p.k * R + c0 + q * CPW];
     nrm[q] = 0.0;
   }
-  // two rows per pass, all loads before the stores (w and vk are distinct
-  // buffers, which the compiler cannot assume)
+  // NR rows per pass, all loads before the stores (w and vk are distinct
+  // buffers, which the compiler cannot assume); a lane's rows are added into
+  // nrm in the same order whatever NR is
   constexpr int STEP = kWaves * RPW;
-  for (int r = row0 + wave * RPW + slot; r < row1; r += 2 * STEP) {
-    const int r2 = r + STEP;
-    const bool two = r2 < row1;
-    double wv[2][CL], vv[2][CL];
+  constexpr int NR = kFinishRows;
+  for (int r = row0 + wave * RPW + slot; r < row1; r += NR * STEP) {
+    double wv[NR][CL], vv[NR][CL];
 #pragma unroll
-    for (int q = 0; q < CL; ++q) {
-      const size_t i = (size_t)r * R + c0 + q * CPW;
-      wv[0][q] = p.w[i];
-      vv[0][q] = p.vk[i];
-      const size_t i2 = (size_t)(two ? r2 : r) * R + c0 + q * CPW;
-      wv[1][q] = p.w[i2];
-      vv[1][q] = p.vk[i2];
-    }
-#pragma unroll
-    for (int q = 0; q < CL; ++q) {
-      const double w = fma(-al[q], vv[0][q], wv[0][q]);
-      p.w[(size_t)r * R + c0 + q * CPW] = w;
-      nrm[q] = fma(w, w, nrm[q]);
-    }
-    if (two) {
+    for (int h = 0; h < NR; ++h) {
+      const int rh = r + h * STEP < row1 ? r + h * STEP : r;
 #pragma unroll
       for (int q = 0; q < CL; ++q) {
-        const double w = fma(-al[q], vv[1][q], wv[1][q]);
-        p.w[(size_t)r2 * R + c0 + q * CPW] = w;
+        const size_t i = (size_t)rh * R + c0 + q * CPW;
+        wv[h][q] = p.w[i];
+        vv[h][q] = p.vk[i];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+      if (r + h * STEP >= row1) break;
+#pragma unroll
+      for (int q = 0; q < CL; ++q) {
+        const double w = fma(-al[q], vv[h][q], wv[h][q]);
+        p.w[(size_t)(r + h * STEP) * R + c0 + q * CPW] = w;
         nrm[q] = fma(w, w, nrm[q]);
       }
     }
