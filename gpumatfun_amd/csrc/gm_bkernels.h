@@ -760,7 +760,8 @@ inline void launch_bdefl_qtu(const BDeflQtU& q, int G, hipStream_t st) {
 }
 
 // Ud = Usol - Q Z and upd = Q (f Z); block y covers 64 columns, whose Z and
-// f Z are staged in LDS (2 K 64 doubles); one wavefront per row.
+// f Z are staged in LDS (2 K 64 doubles); a wavefront takes 64 / cpw rows of
+// cpw lanes (cpw = the group's width rounded up to a power of two).
 struct BDeflApply {
   const double* Q;    // [N][K]
   const double* Z;    // [K][R]
@@ -784,15 +785,21 @@ __global__ __launch_bounds__(kBlock) void k_bdefl_apply(BDeflApply p) {
     sz[p.K * 64 + t] = z * ((cc & 1) ? p.pl[k] : p.el[k]);
   }
   __syncthreads();
-  const int c = cg + lane;
-  if (c >= p.R) return;
-  for (int r = blockIdx.x * kWaves + wave; r < p.nrows; r += gridDim.x * kWaves) {
+  // a group narrower than 64 columns packs 64 / cpw rows into a wavefront
+  const int width = min(64, p.R - cg);
+  int cpw = 1;
+  while (cpw < width) cpw <<= 1;
+  const int rpw = 64 / cpw;
+  const int slot = lane / cpw, cl = lane % cpw;
+  if (cl >= width) return;   // no barrier follows
+  const int c = cg + cl;
+  for (int r = (blockIdx.x * kWaves + wave) * rpw + slot; r < p.nrows; r += gridDim.x * kWaves * rpw) {
     const double* q = p.Q + (size_t)r * p.K;
     double a = 0.0, u = 0.0;
     for (int k = 0; k < p.K; ++k) {
       const double qv = q[k];
-      a = fma(qv, sz[k * 64 + lane], a);
-      u = fma(qv, sz[p.K * 64 + k * 64 + lane], u);
+      a = fma(qv, sz[k * 64 + cl], a);
+      u = fma(qv, sz[p.K * 64 + k * 64 + cl], u);
     }
     const size_t i = (size_t)r * p.R + c;
     p.Ud[i] = p.Usol[i] - a;
