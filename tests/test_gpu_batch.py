@@ -138,6 +138,40 @@ def _stimuli(n, P, size=5):
     return out
 
 
+def test_batch_deflation_80_columns():
+    """40 problems (80 columns) sharing K = 6 deflation pairs: QtU spans two
+    groups of up to 64 columns, and the deflation apply packs four rows into a
+    wavefront in the 16-column group.  Every problem against the plain engine,
+    and two against the oracle."""
+    from cases import fem_case
+    inp = fem_case(m=30, n=16)
+    N = inp.A.shape[0]
+    P = 40
+    U = np.empty((N, 2 * P))
+    for p in range(P):
+        w = np.random.default_rng(200 + p).random(N)
+        U[:, 2 * p] = inp.mass * w
+        U[:, 2 * p + 1] = inp.mass * w * (1 - w)
+    print(f"case batch deflation K=6 P={P}", flush=True)
+    b = BatchEngine.from_inputs(inp, U, clamp=False)
+    b.step(3)
+    out = b.get_U()
+    b.synchronize()
+    b.delete()
+    for p in range(P):
+        inp.U = U[:, 2 * p:2 * p + 2].copy()
+        e = LanczosEngine.from_inputs(inp, clamp=False)
+        e.step(3)
+        err = rel(out[:, 2 * p:2 * p + 2], e.get_U())
+        e.delete()
+        assert err < 1e-12, (p, err)
+    for p in (0, P - 1):
+        u, gu = oracle_pair(inp, U[:, 2 * p], U[:, 2 * p + 1], 3)
+        print(p, rel(out[:, 2 * p], u), rel(out[:, 2 * p + 1], inp.dt * gu))
+        assert rel(out[:, 2 * p], u) < TOL
+        assert rel(out[:, 2 * p + 1], inp.dt * gu) < TOL
+
+
 @pytest.mark.parametrize("K", [0, 8])
 def test_batch_mitchell_schaeffer_vs_oracle(K):
     """Mitchell-Schaeffer with P = 4 stimulus patches on the anisotropic 9-point
