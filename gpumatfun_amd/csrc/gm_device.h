@@ -83,6 +83,7 @@ constexpr int kReduceBlock = 1024;
 
 // Steps of a tridiagonal sweep whose operands are loaded together in the
 // shifted solves.
+// (16 spills the back-substitution operands to scratch and measured slower)
 constexpr int kSolveSteps = 8;
 
 // Segmented form (seg != nullptr): one block per segment q = blockIdx.x sums
