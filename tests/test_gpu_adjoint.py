@@ -298,3 +298,45 @@ def test_batch_param_gradient_matches_single_column(m, K):
               f"dL/dr {gt[0, col]:.15e} {gr:.15e}")
         assert rel(UB[:, col], u) < 1e-10 and rel(GB[:, col], gb) < 1e-10
         assert abs(gt[0, col] - gr) <= 1e-9 * abs(gr)
+
+
+def test_batch_param_gradient_long_rows():
+    """A 24^2 Laplacian with a hub row (node 0 coupled to 30 others, 31
+    nonzeros): the one-pass P-P dot takes its long-row path (rows over 12
+    nonzeros) there.  dL/dtheta for a random symmetric E on A's pattern, the
+    multi-column engine against the single-column one, column by column."""
+    import scipy.sparse as sp
+    from gpumatfun_amd.adjoint import BatchFabEngine
+    n, R, m = 24, 4, 40
+    A = problems.laplacian_2d_dirichlet(n).tolil()
+    for j in range(1, 31):
+        A[0, j] = A[j, 0] = -50.0
+    A[0, 0] += 50.0 * 30 + 1.0
+    for j in range(1, 31):
+        A[j, j] += 50.0 + 1.0
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    assert np.diff(A.indptr).max() > 12
+    rng = np.random.default_rng(21)
+    Eu = sp.triu(sp.csr_matrix((rng.standard_normal(A.nnz), A.indices, A.indptr), shape=A.shape))
+    E = sp.csr_matrix(Eu + sp.triu(Eu, 1).T)
+    E.sort_indices()
+    assert np.array_equal(E.indptr, A.indptr) and np.array_equal(E.indices, A.indices)
+    w = np.linalg.eigvalsh(A.toarray())
+    c = q.contour_nodes(0.95 * w[0], 1.05 * w.max(), 24, TAU, 1.0)
+    cw, _ = contour_weights(c["xi"], c["dxidt"], c["s"], TAU, ALPHA)
+    B = rng.standard_normal((n * n, R))
+    G = rng.standard_normal((n * n, R))
+    print("case batch param gradient long rows", flush=True)
+    be = BatchFabEngine(A, R=R, m=m, xi=c["xi"], wts1=cw, wts2=cw)
+    be.set_param_derivs([E])
+    be.forward(B)
+    _, gt = be.backward_param(G, want_b=False)
+    be.delete()
+    for col in range(R):
+        eng = FabEngine(A, m=m, xi=c["xi"], wts=cw)
+        eng.forward(B[:, col])
+        _, (gr,) = eng.backward_param(G[:, col], [E])
+        eng.delete()
+        print(f"column {col}: dL/dtheta {gt[0, col]:.15e} {gr:.15e}")
+        assert abs(gt[0, col] - gr) <= 1e-9 * abs(gr)
