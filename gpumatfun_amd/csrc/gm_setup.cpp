@@ -188,9 +188,12 @@ gm_status gm_eig_weights(const double* lambda, int32_t K, double eta, double alp
                          double* exp_lambda, double* phi_lambda) {
   if (K < 0 || (K > 0 && (!lambda || !exp_lambda || !phi_lambda))) return GM_EINVAL;
   for (int i = 0; i < K; ++i) {
-    const double z = -eta * std::pow(lambda[i], 0.5 * alpha);
+    // A Neumann operator's zero mode comes out of an eigensolver as a tiny
+    // eigenvalue of either sign; a negative one is that zero (its real power
+    // would be NaN), and phi(0) = 1.
+    const double z = -eta * std::pow(std::fmax(lambda[i], 0.0), 0.5 * alpha);
     exp_lambda[i] = std::exp(z);
-    phi_lambda[i] = std::expm1(z) / z;
+    phi_lambda[i] = z == 0.0 ? 1.0 : std::expm1(z) / z;
   }
   if (K > 0) phi_lambda[0] = 1.0;  // problem_startup.m:77
   return GM_OK;

@@ -186,11 +186,17 @@ def pack_shifts(etaxi, m):
 
 
 def f_of_eigs(lam, eta, alpha):
-    """exp_lambda and phi_lambda of deflated eigenvalues, ``problem_startup.m:75-77``."""
-    lam = np.asarray(lam, dtype=np.float64)
+    """exp_lambda and phi_lambda of deflated eigenvalues, ``problem_startup.m:75-77``.
+
+    A negative eigenvalue is the rounding of a Neumann operator's zero mode
+    (the FEM disc's constant vector) and is evaluated as 0 (its real power
+    would be NaN); phi(0) = 1.
+    """
+    lam = np.maximum(np.asarray(lam, dtype=np.float64), 0.0)
     el = f1(lam, eta, alpha)
     with np.errstate(divide="ignore", invalid="ignore"):
         pl = f2(lam.astype(np.float64), eta, alpha)
+    pl[lam == 0.0] = 1.0
     if pl.size:
         pl[0] = 1.0                               # phi_lambda(1) = 1 guard
     return el, pl

@@ -15,6 +15,25 @@
  * order, so they are bit-identical for any thread count; dot products and
  * norms are summed in fixed 4096-element chunks, then over chunks in order,
  * so they are also independent of the thread count.
+ *
+ * Arithmetic variants (fab_set_variant, all off by default).  Each switches
+ * one rounding choice of the restatement to the one a fused GPU evaluation
+ * makes, or adds full reorthogonalisation, so that the effect of each on the
+ * result can be measured on its own:
+ *   FAB_V_SCALE   the first polynomial term (and, at degree 0, the Lanczos
+ *                 product) as (A w) * (gamma_0 * (1/beta)) from the
+ *                 unnormalised w, in place of A (gamma_0 * (w / beta));
+ *   FAB_V_FMA     fused multiply-adds in the Horner terms, the three-term
+ *                 recurrence, the dot products and the norms;
+ *   FAB_V_TREE    dot products and norms summed per 256-element block by a
+ *                 pairwise tree, then the block sums pairwise, in place of
+ *                 sequential 4096-element chunks;
+ *   FAB_V_DEFL    the deflation split U - Q Z and Q f(L) Z accumulated with
+ *                 fused multiply-adds;
+ *   FAB_V_REORTH  full reorthogonalisation of every new Lanczos vector
+ *                 against all earlier ones (classical Gram-Schmidt, twice):
+ *                 in exact arithmetic the same Krylov approximation, so it
+ *                 gives the answer that rounding-free Lanczos would.
  */
 #include <complex.h>
 #include <math.h>
@@ -29,14 +48,55 @@ typedef double complex zd;
 
 #define CHUNK 4096
 
+enum { FAB_V_SCALE = 1, FAB_V_FMA = 2, FAB_V_TREE = 4, FAB_V_DEFL = 8, FAB_V_REORTH = 16 };
+static unsigned g_variant = 0;
+
+unsigned fab_set_variant(unsigned flags) {
+  unsigned old = g_variant;
+  g_variant = flags;
+  return old;
+}
+
+#define TREE_BLOCK 256
+
+/* pairwise sum of v[0..n) (n >= 1) */
+static double pairwise(const double* v, long n) {
+  if (n == 1) return v[0];
+  long h = n / 2;
+  return pairwise(v, h) + pairwise(v + h, n - h);
+}
+
+/* sum of a[i] * b[i] per 256-element block by a pairwise tree, then the block
+ * sums pairwise */
+static double dot_tree(long n, const double* a, const double* b, int fused) {
+  if (n <= 0) return 0.0;
+  long nb = (n + TREE_BLOCK - 1) / TREE_BLOCK;
+  double* part = (double*)malloc(sizeof(double) * nb);
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nb; ++c) {
+    long lo = c * TREE_BLOCK, hi = lo + TREE_BLOCK < n ? lo + TREE_BLOCK : n;
+    double t[TREE_BLOCK];
+    for (long i = lo; i < hi; ++i) t[i - lo] = fused ? fma(a[i], b[i], 0.0) : a[i] * b[i];
+    part[c] = pairwise(t, hi - lo);
+  }
+  double s = pairwise(part, nb);
+  free(part);
+  return s;
+}
+
 static double dot_fixed(long n, const double* a, const double* b) {
+  if (g_variant & FAB_V_TREE) return dot_tree(n, a, b, (g_variant & FAB_V_FMA) != 0);
+  const int fused = (g_variant & FAB_V_FMA) != 0;
   long nc = (n + CHUNK - 1) / CHUNK;
   double* part = (double*)malloc(sizeof(double) * (nc > 0 ? nc : 1));
 #pragma omp parallel for schedule(static)
   for (long c = 0; c < nc; ++c) {
     long lo = c * CHUNK, hi = lo + CHUNK < n ? lo + CHUNK : n;
     double s = 0.0;
-    for (long i = lo; i < hi; ++i) s += a[i] * b[i];
+    if (fused)
+      for (long i = lo; i < hi; ++i) s = fma(a[i], b[i], s);
+    else
+      for (long i = lo; i < hi; ++i) s += a[i] * b[i];
     part[c] = s;
   }
   double s = 0.0;
@@ -81,11 +141,37 @@ static void poly_eval(long n, const int* rp, const int* ci, const double* va, in
   }
 }
 
+/* w[:, 0..1] -= V1/V2[:, 0..cols) (V^T w), twice: full reorthogonalisation */
+static void reorth2(long n, int cols, const double* V1, const double* V2, double* w) {
+  double* h = (double*)malloc(sizeof(double) * (cols > 0 ? cols : 1));
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int c = 0; c < 2; ++c) {
+      const double* V = c == 0 ? V1 : V2;
+      double* wc = w + (size_t)c * n;
+      for (int i = 0; i < cols; ++i) h[i] = dot_fixed(n, V + (size_t)i * n, wc);
+#pragma omp parallel for schedule(static)
+      for (long k = 0; k < n; ++k) {
+        double s = 0.0;
+        for (int i = 0; i < cols; ++i) s += V[(size_t)i * n + k] * h[i];
+        wc[k] -= s;
+      }
+    }
+  }
+  free(h);
+}
+
+/* a * x + y, fused when FAB_V_FMA is set */
+static inline double axpy1(double a, double x, double y, int fused) {
+  return fused ? fma(a, x, y) : y + x * a;
+}
+
 /* lanczos.m for two columns; V1, V2 n x m column-major; al (m), be (m+1). */
 static void lanczos2(long n, const int* rp, const int* ci, const double* va, int m, int nd,
                      const double* gamma, const double* a, const double* b, double* V1,
                      double* V2, double* al1, double* be1, double* al2, double* be2,
                      double* work /* 8n */) {
+  const int fused = (g_variant & FAB_V_FMA) != 0;
+  const int scale = (g_variant & FAB_V_SCALE) != 0;
   double* w = work;               /* n x 2 */
   double* t = work + 2 * n;       /* n x 2 */
   double* u = work + 4 * n;       /* n x 2 */
@@ -100,36 +186,67 @@ static void lanczos2(long n, const int* rp, const int* ci, const double* va, int
   for (int j = 0; j < m; ++j) {
     const double* v1 = V1 + (size_t)j * n;
     const double* v2 = V2 + (size_t)j * n;
-    /* w = p(A) [v1 v2]  (jacobi_poly_eval with zero imaginary parts) */
-    for (long k = 0; k < n; ++k) {
-      u[k] = v1[k] * gamma[0];
-      u[n + k] = v2[k] * gamma[0];
+    int first = 1;       /* first Horner term still to form */
+    int product = 0;     /* w already holds A p(A) v */
+    if (scale) {
+      /* (A w)(gamma_0 (1/beta_j)) from the unnormalised w (a, b at j = 0) */
+      if (j == 0) {
+        memcpy(u, a, sizeof(double) * n);
+        memcpy(u + n, b, sizeof(double) * n);
+      } else {
+        memcpy(u, w, sizeof(double) * 2 * n);
+      }
+      spmm(n, rp, ci, va, 2, u, t);
+      const double s1 = gamma[0] * (1.0 / be1[j]), s2 = gamma[0] * (1.0 / be2[j]);
+      if (nd >= 2) {
+#pragma omp parallel for schedule(static)
+        for (long k = 0; k < n; ++k) {
+          u[k] = axpy1(gamma[1], v1[k], t[k] * s1, fused);
+          u[n + k] = axpy1(gamma[1], v2[k], t[n + k] * s2, fused);
+        }
+      } else {
+#pragma omp parallel for schedule(static)
+        for (long k = 0; k < n; ++k) {
+          w[k] = t[k] * s1;
+          w[n + k] = t[n + k] * s2;
+        }
+        product = 1;
+      }
+      first = 0;
     }
-    for (int i = 1; i < nd; ++i) {
+    if (first) {
+      /* w = p(A) [v1 v2]  (jacobi_poly_eval with zero imaginary parts) */
+      for (long k = 0; k < n; ++k) {
+        u[k] = v1[k] * gamma[0];
+        u[n + k] = v2[k] * gamma[0];
+      }
+    }
+    for (int i = first ? 1 : 2; i < nd; ++i) {
       spmm(n, rp, ci, va, 2, u, t);
 #pragma omp parallel for schedule(static)
       for (long k = 0; k < n; ++k) {
-        u[k] = t[k] + v1[k] * gamma[i];
-        u[n + k] = t[n + k] + v2[k] * gamma[i];
+        u[k] = axpy1(gamma[i], v1[k], t[k], fused);
+        u[n + k] = axpy1(gamma[i], v2[k], t[n + k], fused);
       }
     }
-    spmm(n, rp, ci, va, 2, u, w);                                  /* lanczos.m:27 */
+    if (!product) spmm(n, rp, ci, va, 2, u, w);                    /* lanczos.m:27 */
     if (j != 0) {                                                  /* :30-32 */
       const double* p1 = V1 + (size_t)(j - 1) * n;
       const double* p2 = V2 + (size_t)(j - 1) * n;
 #pragma omp parallel for schedule(static)
       for (long k = 0; k < n; ++k) {
-        w[k] = w[k] - be1[j] * p1[k];
-        w[n + k] = w[n + k] - be2[j] * p2[k];
+        w[k] = axpy1(-be1[j], p1[k], w[k], fused);
+        w[n + k] = axpy1(-be2[j], p2[k], w[n + k], fused);
       }
     }
     al1[j] = dot_fixed(n, w, v1);                                  /* :35-36 */
     al2[j] = dot_fixed(n, w + n, v2);
 #pragma omp parallel for schedule(static)
     for (long k = 0; k < n; ++k) {                                 /* :38 */
-      w[k] = w[k] - al1[j] * v1[k];
-      w[n + k] = w[n + k] - al2[j] * v2[k];
+      w[k] = axpy1(-al1[j], v1[k], w[k], fused);
+      w[n + k] = axpy1(-al2[j], v2[k], w[n + k], fused);
     }
+    if (g_variant & FAB_V_REORTH) reorth2(n, j + 1, V1, V2, w);
     be1[j + 1] = sqrt(dot_fixed(n, w, w));                         /* :40-41 */
     be2[j + 1] = sqrt(dot_fixed(n, w + n, w + n));
     if (j < m - 1) {                                               /* :44-47 */
@@ -297,12 +414,22 @@ int fab_cpu_steps(long n, const int* rp, const int* ci, const double* va, int K,
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) {
       double cu = 0.0, cg = 0.0, e1 = 0.0, e2 = 0.0;
-      for (int k = 0; k < K; ++k) {
-        const double q = Q[(size_t)k * n + i];
-        cu += q * Qu[k];
-        cg += q * Qg[k];
-        e1 += q * (el[k] * Qu[k]);
-        e2 += q * (pl[k] * Qg[k]);
+      if (g_variant & FAB_V_DEFL) {
+        for (int k = 0; k < K; ++k) {
+          const double q = Q[(size_t)k * n + i];
+          cu = fma(q, Qu[k], cu);
+          cg = fma(q, Qg[k], cg);
+          e1 = fma(q, el[k] * Qu[k], e1);
+          e2 = fma(q, pl[k] * Qg[k], e2);
+        }
+      } else {
+        for (int k = 0; k < K; ++k) {
+          const double q = Q[(size_t)k * n + i];
+          cu += q * Qu[k];
+          cg += q * Qg[k];
+          e1 += q * (el[k] * Qu[k]);
+          e2 += q * (pl[k] * Qg[k]);
+        }
       }
       xb[i] = u[i] - cu;
       yb[i] = gu[i] - cg;

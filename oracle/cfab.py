@@ -41,8 +41,16 @@ def load():
                                       dp, dp, C.c_int, dp, dp]
         lib.fab_set_threads.restype = C.c_int
         lib.fab_set_threads.argtypes = [C.c_int]
+        lib.fab_set_variant.restype = C.c_uint
+        lib.fab_set_variant.argtypes = [C.c_uint]
         _lib = lib
     return _lib
+
+
+# Arithmetic variants of the C restatement (fab_oracle.c): each switches one
+# rounding choice to the one a fused GPU evaluation makes, or adds full
+# reorthogonalisation (the rounding-free Lanczos answer).
+V_SCALE, V_FMA, V_TREE, V_DEFL, V_REORTH = 1, 2, 4, 8, 16
 
 
 def _p(a):
@@ -57,10 +65,13 @@ def _cplx(z):
     return out
 
 
-def cpu_steps_c(setup, u0, gu0, steps, threads=0):
-    """Same contract as ``oracle.cpu_steps_nonuniform`` (mass from setup, default ones)."""
+def cpu_steps_c(setup, u0, gu0, steps, threads=0, variant=0):
+    """Same contract as ``oracle.cpu_steps_nonuniform`` (mass from setup, default ones).
+
+    ``variant`` ORs the V_* switches (0: the literal restatement)."""
     lib = load()
     used = lib.fab_set_threads(int(threads))
+    lib.fab_set_variant(int(variant))
     A = setup["A"].tocsr()
     A.sort_indices()
     n = A.shape[0]
@@ -89,6 +100,7 @@ def cpu_steps_c(setup, u0, gu0, steps, threads=0):
                             _p(pl), float(setup["dt"]), int(setup["m"]), nd, _p(gamma), ns,
                             _p(gsr), _p(gsi), _p(w1), _p(w2), float(setup["s"]), _p(ex),
                             _p(mass), int(steps), _p(u), _p(gu))
+    lib.fab_set_variant(0)
     if bad:
         raise ZeroDivisionError("singular shifted tridiagonal system in the C oracle")
     return u, gu, used

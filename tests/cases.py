@@ -68,3 +68,37 @@ def monodomain_case(side=48, m=30, n=24, degree=0, dt=0.05, eta=2e-3, alpha=1.7)
     inp = q.make_inputs(A, xh, yh, m=m, n=n, eta=eta, alpha=alpha, dt=dt,
                         l1=float(lam.min()), lN=float(lam.max()), degree=degree, ode=ode, fhn=fhn)
     return inp
+
+
+def dirichlet_sine_eigenpairs(side, K):
+    """The K smallest eigenpairs of ``laplacian_2d_dirichlet(side)`` in closed form:
+    lambda = (4/h^2)(sin^2(j pi h/2) + sin^2(k pi h/2)), eigenvector
+    sin(j pi x) sin(k pi y) normalised, in natural order (index i + side*j)."""
+    h = 1.0 / (side + 1)
+    js = np.arange(1, side + 1)
+    lx = 4.0 * np.sin(js * np.pi * h / 2) ** 2 / h ** 2
+    L = lx[:, None] + lx[None, :]
+    order = np.argsort(L.ravel(), kind="stable")[:K]
+    jj, kk = np.unravel_index(order, L.shape)
+    xs = js * h
+    S1 = np.sqrt(2 * h) * np.sin(np.pi * np.outer(xs, js))   # column j: mode j on the 1D grid
+    Q = np.stack([np.kron(S1[:, k], S1[:, j]) for j, k in zip(jj, kk)], 1)
+    return L.ravel()[order], Q
+
+
+def headline_parameters_case(side=128, m=80, seed=0):
+    """The full-scale plain workload's parameters on a grid whose answer FP64 fixes to
+    ~1e-13: degree-8 Jacobi polynomial, m = 80, 64 shifts, K = 20 deflation vectors
+    (closed-form sine eigenpairs), alpha = 1.7, tau = 1e-3, a non-unit mass and
+    g = mass ubar (1 - ubar) (problem_startup_CPU_nonuniform.m's reaction)."""
+    A = problems.laplacian_2d_dirichlet(side)
+    lam_all = problems.dirichlet_eigs_2d(side)
+    lam, Q = dirichlet_sine_eigenpairs(side, 21)
+    rng = np.random.default_rng(seed)
+    N = A.shape[0]
+    u0 = rng.standard_normal(N)
+    mass = 1.0 + 0.5 * rng.uniform(size=N)
+    ub = u0 / mass
+    return q.make_inputs(A, u0, mass * ub * (1 - ub), m=m, n=64, eta=1e-3, alpha=1.7, dt=1e-3,
+                         l1=float(lam[20]), lN=float(lam_all.max()), degree=8,
+                         Q=Q[:, :20], lam=lam[:20], mass=mass)

@@ -7,7 +7,8 @@ import numpy as np
 import pytest
 
 import oracle
-from cases import dirichlet_case, fem_case, monodomain_case, oracle_setup, FHN
+from cases import (dirichlet_case, fem_case, monodomain_case, oracle_setup, FHN,
+                   headline_parameters_case)
 from conftest import rel
 from gpumatfun_amd.engine import LanczosEngine
 
@@ -137,6 +138,45 @@ def test_fem_deflation_mass_vs_cpu_nonuniform():
     print("rel err", rel(U[:, 0], u), rel(U[:, 1], inp.dt * gu))
     assert rel(U[:, 0], u) < TOL
     assert rel(U[:, 1], inp.dt * gu) < TOL
+    eng.delete()
+
+
+def test_headline_parameters_dirichlet128():
+    """The full-scale plain workload's parameters -- d = 8, m = 80, n_s = 64, K = 20,
+    alpha = 1.7, non-unit mass -- on the 128^2 Dirichlet grid, where any two FP64
+    evaluation orders of the oracle agree to ~1e-13.  Runs the V*C kernel with
+    nine coefficient vectors, all three 8-vector deflation tiles and the two-block
+    grid of 128 shifted systems.  Stage by stage after one step against the
+    GPU-semantics model, then three steps against problem_startup_CPU_nonuniform.m's
+    loop, each to 1e-10."""
+    inp = headline_parameters_case()
+    print("case headline parameters on dirichlet128: d=8 m=80 n_s=64 K=20", flush=True)
+    eng = LanczosEngine.from_inputs(inp, clamp=False)
+    ref = _gpu_sem(inp)
+    assert rel(eng.get_U(), ref.get_U()) < 1e-14
+    eng.build_subspace()
+    ref.build_subspace()
+    e_ab = rel(eng.get_aabb(), ref.aabb)
+    eng.lin_sys_solves()
+    ref.lin_sys_solves()
+    yr, yi = eng.get_y()
+    e_y = rel(yr + 1j * yi, ref.Yr + 1j * ref.Yi)
+    eng.apply_V()
+    ref.apply_V()
+    X = eng.get_xsol()
+    e_x = max(rel(X[:, 0], ref.X[:, 0]), rel(X[:, 1], ref.X[:, 1]))
+    eng.update_U()
+    ref.update_U()
+    U = eng.get_U()
+    e_u = max(rel(U[:, 0], ref.Usol[:, 0]), rel(U[:, 1], ref.Usol[:, 1]))
+    print(f"stages: aabb {e_ab:.2e} Y {e_y:.2e} X {e_x:.2e} Usol {e_u:.2e}", flush=True)
+    assert max(e_ab, e_y, e_x, e_u) < TOL
+    eng.step(2)
+    U = eng.get_U()
+    u, gu = oracle.cpu_steps_nonuniform(oracle_setup(inp), inp.U[:, 0], inp.U[:, 1], 3)
+    e0, e1 = rel(U[:, 0], u), rel(U[:, 1], inp.dt * gu)
+    print(f"3 steps: rel err u {e0:.2e} dt*g {e1:.2e}", flush=True)
+    assert e0 < TOL and e1 < TOL
     eng.delete()
 
 
