@@ -650,18 +650,23 @@ def oracle_setup_dict(inp, A=None, perm=None):
     return S
 
 
-def oracle_one_step(inp, u0, g0, threads, perm_seed=None):
+def oracle_one_step(inp, u0, g0, threads, perm_seed=None, variant=0, extended=False, m=None):
     """One time step of the oracle's C restatement (oracle/c/fab_oracle.c) on `threads`
     host threads: the step loop of problem_startup_CPU_nonuniform.m (mass = 1 is
     problem_startup_CPU.m's loop).  With perm_seed, the same problem with rows and
-    columns permuted (only the floating-point summation order changes)."""
+    columns permuted (only the floating-point summation order changes); `variant`
+    switches arithmetic choices (oracle.cfab.V_*), `extended` runs it in long double,
+    `m` overrides the Lanczos steps (with a matching shift vector)."""
     import scipy.sparse as sp
     from oracle import cfab
-    cfab.load()
+    cfab.load(extended)
     if perm_seed is None:
         S = oracle_setup_dict(inp)
+        if m is not None:
+            S["m"] = int(m)
         t0 = time.perf_counter()
-        u, gu, used = cfab.cpu_steps_c(S, u0, g0, 1, threads=threads)
+        u, gu, used = cfab.cpu_steps_c(S, u0, g0, 1, threads=threads, variant=variant,
+                                       extended=extended)
         return u, gu, time.perf_counter() - t0, used
     N = inp.A.shape[0]
     p = np.random.default_rng(perm_seed).permutation(N)
@@ -677,7 +682,48 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
-def cpu_leg(inp, u0, g0, U1, result):
+def parity_study(inp, u0, g0, U1, threads, runs, extd, result):
+    """Where a result depends on the FP64 evaluation order: each arithmetic choice of a
+    fused GPU evaluation switched on alone in the oracle (oracle.cfab.V_*), which
+    stage's FP64 rounding moves the long-double result how far, and the Krylov
+    truncation error of m steps against reorthogonalised Lanczos at twice m."""
+    from oracle import cfab
+    ue, ge = extd
+    d = lambda a, b: max(rel(a[0], b[0]), rel(a[1], b[1]))
+    gpu = (U1[:, 0], U1[:, 1] / inp.dt)
+    base = runs[0]
+    out = {"variants": {}, "extended_with_fp64_stage": {}}
+    for name, v in (("scale_order", cfab.V_SCALE), ("fma", cfab.V_FMA), ("tree_sums", cfab.V_TREE),
+                    ("deflation_fma", cfab.V_DEFL),
+                    ("all_four", cfab.V_SCALE | cfab.V_FMA | cfab.V_TREE | cfab.V_DEFL),
+                    ("reorthogonalised", cfab.V_REORTH)):
+        r = oracle_one_step(inp, u0, g0, threads, variant=v)[:2]
+        out["variants"][name] = {"vs_oracle": d(r, base), "vs_extended": d(r, (ue, ge)),
+                                 "vs_gpu": d(r, gpu)}
+        log(f"variant {name}: vs oracle {d(r, base):.2e}, vs extended {d(r, (ue, ge)):.2e}, "
+            f"vs GPU {d(r, gpu):.2e}")
+    for name, v in (("deflation", cfab.V_RDEFL), ("lanczos_vectors", cfab.V_RLANCZOS),
+                    ("sparse_products", cfab.V_RSPMV)):
+        r = oracle_one_step(inp, u0, g0, threads, variant=v, extended=True)[:2]
+        out["extended_with_fp64_stage"][name] = {"vs_extended": d(r, (ue, ge)),
+                                                 "vs_oracle": d(r, base)}
+        log(f"long double with {name} rounded to double: moves {d(r, (ue, ge)):.2e}")
+    m2 = 2 * inp.m
+    from gpumatfun_amd import quadrature as q
+    inp2 = q.make_inputs(inp.A, u0, g0, m=m2, n=inp.extra["n"], eta=inp.extra["eta"],
+                         alpha=inp.extra["alpha"], dt=inp.dt, l1=inp.extra["l1"],
+                         lN=inp.extra["lN"], degree=inp.gamma.size - 1,
+                         prob_dim=None, Q=inp.Q, lam=inp.lam, mass=inp.mass)
+    inp2.gamma = inp.gamma
+    conv = oracle_one_step(inp2, u0, g0, threads, variant=cfab.V_REORTH)[:2]
+    out["krylov_truncation"] = {"m": inp.m, "against": f"reorthogonalised Lanczos, m = {m2}",
+                                "extended": d((ue, ge), conv), "gpu": d(gpu, conv),
+                                "oracle": d(base, conv)}
+    log(f"Krylov truncation of m={inp.m} against m={m2}: {d((ue, ge), conv):.2e}")
+    result["parity_study"] = out
+
+
+def cpu_leg(inp, u0, g0, U1, result, study=False):
     """Parity of the GPU's first step against the oracle, the oracle's own rounding
     floor (a row-permuted run), and the timed CPU baseline."""
     threads = min(16, os.cpu_count() or 1)
@@ -702,6 +748,20 @@ def cpu_leg(inp, u0, g0, U1, result):
             "gpu_vs_runs_min": min(gpu), "gpu_vs_runs_max": max(gpu)}
         log(f"oracle spread over 4 summation orders: {min(pair):.2e} .. {max(pair):.2e}; "
             f"GPU against them {min(gpu):.2e} .. {max(gpu):.2e}")
+        # the same evaluation in long double: rounding errors 2^11 times smaller, the
+        # reference point for every FP64 evaluation order, the GPU's included
+        ue, ge, ext_s, _ = oracle_one_step(inp, u0, g0, threads, extended=True)
+        ext = lambda a: max(rel(a[0], ue), rel(a[1], ge))
+        result["rel_err_vs_oracle"]["vs_extended_precision"] = {
+            "reference": "the oracle's C restatement in long double (64-bit significand)",
+            "gpu": max(rel(U1[:, 0], ue), rel(U1[:, 1], inp.dt * ge)),
+            "oracle_runs_min": min(ext(a) for a in runs),
+            "oracle_runs_max": max(ext(a) for a in runs),
+            "seconds": round(ext_s, 1)}
+        log(f"vs the long-double evaluation: GPU {result['rel_err_vs_oracle']['vs_extended_precision']['gpu']:.2e}, "
+            f"oracle runs {min(ext(a) for a in runs):.2e} .. {max(ext(a) for a in runs):.2e}")
+        if study:
+            parity_study(inp, u0, g0, U1, threads, runs, (ue, ge), result)
     reps = 1
     if cpu_s < 5.0:   # small workloads: repeat the step to a ~10 s sample
         reps = int(min(1000, max(2, np.ceil(10.0 / max(cpu_s, 1e-3)))))
@@ -752,6 +812,9 @@ def main():
                     help="msadj512: number of stimulus patches (16 as in ms512)")
     ap.add_argument("--adj-m", type=int, default=40,
                     help="msadj512: Lanczos steps of the adjoint's solves (40 as in ms512)")
+    ap.add_argument("--parity-study", action="store_true",
+                    help="add the oracle's arithmetic variants, stage-by-stage rounding of "
+                         "the long-double evaluation and the Krylov truncation error")
     ap.add_argument("--profile-only", action="store_true",
                     help="a few steps and the kernel timings only (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -860,7 +923,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu and not args.profile_only:
-        cpu_leg(inp, inp.U[:, 0], inp.U[:, 1], U1, result)
+        cpu_leg(inp, inp.U[:, 0], inp.U[:, 1], U1, result, study=args.parity_study)
     eng.delete()
     print(json.dumps(result), flush=True)
 

@@ -1,6 +1,9 @@
 """ctypes wrapper of the C restatement ``oracle/c/fab_oracle.c`` (test infrastructure).
 
-``build()`` compiles it with gcc/OpenMP into ``oracle/_build/libfab_oracle.so``;
+``build()`` compiles it with gcc/OpenMP twice: in double precision into
+``oracle/_build/libfab_oracle.so`` (the restatement) and in long double into
+``oracle/_build/libfab_oracle_ld.so`` (the same evaluation with rounding errors
+2^11 times smaller, the reference point for comparing FP64 evaluation orders).
 ``cpu_steps_c`` runs the step loop of ``problem_startup_CPU_nonuniform.m``
 (mass = 1 gives ``problem_startup_CPU.m``) on ``threads`` host threads.
 """
@@ -14,25 +17,30 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "c", "fab_oracle.c")
 OUT = os.path.join(HERE, "_build", "libfab_oracle.so")
+OUT_LD = os.path.join(HERE, "_build", "libfab_oracle_ld.so")
 
-_lib = None
+_libs = {}
 
 
 def build():
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    tmp = OUT + ".tmp"
-    cmd = ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c11", "-o", tmp, SRC, "-lm"]
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, OUT)
+    for out, extra in ((OUT, []), (OUT_LD, ["-DFAB_REAL=long double"])):
+        tmp = out + ".tmp"
+        cmd = (["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c11"] + extra
+               + ["-o", tmp, SRC, "-lm"])
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, out)
     return OUT
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(OUT):
+def load(extended=False):
+    """The double-precision library, or with ``extended`` the long-double one."""
+    key = bool(extended)
+    if key not in _libs:
+        path = OUT_LD if extended else OUT
+        if not os.path.exists(path):
             build()
-        lib = C.CDLL(OUT)
+        lib = C.CDLL(path)
         dp = C.POINTER(C.c_double)
         ip = C.POINTER(C.c_int)
         lib.fab_cpu_steps.restype = C.c_int
@@ -43,14 +51,16 @@ def load():
         lib.fab_set_threads.argtypes = [C.c_int]
         lib.fab_set_variant.restype = C.c_uint
         lib.fab_set_variant.argtypes = [C.c_uint]
-        _lib = lib
-    return _lib
+        _libs[key] = lib
+    return _libs[key]
 
 
 # Arithmetic variants of the C restatement (fab_oracle.c): each switches one
 # rounding choice to the one a fused GPU evaluation makes, or adds full
 # reorthogonalisation (the rounding-free Lanczos answer).
 V_SCALE, V_FMA, V_TREE, V_DEFL, V_REORTH = 1, 2, 4, 8, 16
+# long-double build only: round one stage's results to double
+V_RDEFL, V_RLANCZOS, V_RSPMV = 32, 64, 128
 
 
 def _p(a):
@@ -65,11 +75,12 @@ def _cplx(z):
     return out
 
 
-def cpu_steps_c(setup, u0, gu0, steps, threads=0, variant=0):
+def cpu_steps_c(setup, u0, gu0, steps, threads=0, variant=0, extended=False):
     """Same contract as ``oracle.cpu_steps_nonuniform`` (mass from setup, default ones).
 
-    ``variant`` ORs the V_* switches (0: the literal restatement)."""
-    lib = load()
+    ``variant`` ORs the V_* switches (0: the literal restatement); ``extended``
+    runs the long-double build (inputs and outputs stay float64)."""
+    lib = load(extended)
     used = lib.fab_set_threads(int(threads))
     lib.fab_set_variant(int(variant))
     A = setup["A"].tocsr()

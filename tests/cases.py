@@ -102,3 +102,15 @@ def headline_parameters_case(side=128, m=80, seed=0):
     return q.make_inputs(A, u0, mass * ub * (1 - ub), m=m, n=64, eta=1e-3, alpha=1.7, dt=1e-3,
                          l1=float(lam[20]), lN=float(lam_all.max()), degree=8,
                          Q=Q[:, :20], lam=lam[:20], mass=mass)
+
+
+def laplace3d_case(side=20, m=30, n=64, seed=0, alpha=1.5, tau=1e-3):
+    """3D 7-point Dirichlet Laplacian side^3 scaled by 1/h^2 (the multi-RHS workload's
+    operator at a test size), closed-form spectrum bounds, no preconditioner."""
+    A = problems.laplacian_3d_dirichlet(side)
+    h = 1.0 / (side + 1)
+    lo = 3 * 4.0 * np.sin(np.pi * h / 2) ** 2 / h ** 2
+    hi = 3 * 4.0 * np.sin(side * np.pi * h / 2) ** 2 / h ** 2
+    u0 = np.random.default_rng(seed).standard_normal(A.shape[0])
+    return q.make_inputs(A, u0, u0 * (1 - u0), m=m, n=n, eta=tau, alpha=alpha, dt=tau,
+                         l1=lo, lN=hi, degree=0)

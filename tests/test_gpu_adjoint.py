@@ -258,8 +258,8 @@ def test_deflated_param_gradient_vs_exact_reference():
     assert abs(gr - ref) <= 1e-8 * abs(ref)
 
 
-@pytest.mark.parametrize("m,K", [(30, 0), (70, 0), (70, 4)])
-def test_batch_param_gradient_matches_single_column(m, K):
+@pytest.mark.parametrize("m,K,R", [(30, 0, 4), (70, 0, 4), (70, 4, 4), (40, 6, 32)])
+def test_batch_param_gradient_matches_single_column(m, K, R):
     """gm_bfab_backward_param (the P-P block contracted in the Krylov bases:
     T = V_g M on MFMA, l in tiles of 16 up to 64 a pass, so m = 70 takes two
     passes) against gm_fab_backward_param column by column (resolvent blocks,
@@ -267,7 +267,7 @@ def test_batch_param_gradient_matches_single_column(m, K):
     run the same Krylov approximation; they differ in reduction partitions, so
     they agree to rounding amplified by m Lanczos steps (1e-9)."""
     from gpumatfun_amd.adjoint import BatchFabEngine
-    n, r, R = 24, 4.0, 4
+    n, r = 24, 4.0
     A = problems.anisotropic_9pt(n, ratio=r)
     E = problems.anisotropic_9pt_ratio_derivative(n)
     w, U = np.linalg.eigh(A.toarray())
@@ -283,7 +283,7 @@ def test_batch_param_gradient_matches_single_column(m, K):
     rng = np.random.default_rng(11)
     B = rng.standard_normal((n * n, R))
     G = rng.standard_normal((n * n, R))
-    print(f"case batch param gradient m={m} K={K}", flush=True)
+    print(f"case batch param gradient m={m} K={K} R={R}", flush=True)
     be = BatchFabEngine(A, R=R, m=m, xi=c["xi"], wts1=cw, wts2=cw, **kwb)
     be.set_param_derivs([E])
     UB = be.forward(B)

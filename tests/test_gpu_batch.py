@@ -48,6 +48,39 @@ def test_batch_matches_oracle_per_problem(P, chunk, degree):
     b.delete()
 
 
+_L3D = {}
+
+
+@pytest.mark.parametrize("chunk", [32, 64, 128, 256])
+def test_batch_laplace3d_chunk_widths(chunk):
+    """The multi-RHS workload's shape at a test size: 128 problems (256 columns) on
+    the 3D 7-point Laplacian 20^3, 64 shifts, m = 30, in chunks of 32, 64, 128 or
+    256 columns (the column kernels' widths).  Problems 0, 63 and 127 against the
+    oracle to 1e-10, and every column against the 32-column chunking to 1e-13."""
+    from cases import laplace3d_case
+    inp = laplace3d_case()
+    P = 128
+    U = batch_U(inp, P, seed0=0)
+    print(f"case batch laplace3d 20^3 P={P} chunk={chunk}", flush=True)
+    b = BatchEngine.from_inputs(inp, U, chunk_columns=chunk, clamp=False)
+    assert b.info()["chunk_columns"] == chunk
+    b.step(2)
+    out = b.get_U()
+    b.synchronize()
+    b.delete()
+    for p in (0, 63, 127):
+        u, gu = oracle_pair(inp, U[:, 2 * p], U[:, 2 * p + 1], 2)
+        e = (rel(out[:, 2 * p], u), rel(out[:, 2 * p + 1], inp.dt * gu))
+        print(p, e, flush=True)
+        assert max(e) < TOL
+    if chunk == 32:
+        _L3D["ref"] = out
+    elif "ref" in _L3D:
+        errs = [rel(out[:, c], _L3D["ref"][:, c]) for c in range(2 * P)]
+        print("max column difference to chunk 32:", max(errs))
+        assert max(errs) < 1e-13
+
+
 def test_batch_graph_matches_eager_bitwise():
     inp = dirichlet_case(side=32, m=24, n=16)
     U = batch_U(inp, 8, seed0=10)

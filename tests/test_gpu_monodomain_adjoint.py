@@ -211,7 +211,7 @@ def test_stimulus_set_concurrent_matches_sequential():
     assert gsum == sum(s_[2] for s_ in seq) or abs(gsum - sum(s_[2] for s_ in seq)) <= 1e-15 * abs(gsum)
 
 
-@pytest.mark.parametrize("P", [1, 2, 4])
+@pytest.mark.parametrize("P", [1, 2, 4, 16])
 def test_batch_adjoint_matches_per_stimulus(P):
     """The one-launch multi-stimulus engine (gm_bfab_*, columns [u_p, g_p]) against
     the single-column engine run stimulus by stimulus (itself pinned by central
@@ -227,7 +227,8 @@ def test_batch_adjoint_matches_per_stimulus(P):
     w, U = np.linalg.eigh(A.toarray())
     c = q.contour_nodes(0.9 * w[K], 1.1 * w.max(), 24, eta, alpha)
     X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
-    spots = [(2, 3), (12, 5), (7, 15), (15, 14)][:P]
+    spots = ([(2, 3), (12, 5), (7, 15), (15, 14)] if P <= 4 else
+             [(1 + 5 * (k % 4), 1 + 5 * (k // 4)) for k in range(P)])[:P]
     u0s = [((X >= a) & (X < a + 5) & (Y >= b) & (Y < b + 5)).astype(float).reshape(-1, order="F")
            for a, b in spots]
     h0 = np.ones(n * n)

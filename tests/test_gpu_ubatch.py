@@ -85,3 +85,26 @@ def test_uneven_batch_graph_matches_eager_bitwise():
         b.step(1, use_graph=True)
     a.delete()
     b.delete()
+
+
+def test_uneven_batch_log_uniform_sides_vs_oracle():
+    """The uneven workload's shape at a test size: 16 2D Dirichlet Laplacians with
+    sides drawn log-uniformly in [32, 448] (seed 3: sides 40 to 399, 1.6k to 159k
+    rows, several problems spanning hundreds of row blocks), m = 40, 32 shifts, no
+    preconditioner.  Every problem against the oracle's C restatement to 1e-10."""
+    from oracle import cfab
+    m, n, steps = 40, 32, 2
+    sides = problems.uneven_batch_sides(16, 32, 448, seed=3)
+    inps = [_dirichlet(int(s), 100 + i, m, n, 0, 1.0) for i, s in enumerate(sides)]
+    print(f"case uneven batch 16 problems sides={sorted(int(x) for x in sides)}", flush=True)
+    ub = UnevenBatch.from_inputs(inps, clamp=False)
+    assert ub.info()["row_blocks"] > len(inps)
+    ub.step(steps)
+    ub.synchronize()
+    for i, inp in enumerate(inps):
+        U = ub.get_U(i)
+        u, gu, _ = cfab.cpu_steps_c(oracle_setup(inp), inp.U[:, 0], inp.U[:, 1], steps)
+        e0, e1 = rel(U[:, 0], u), rel(U[:, 1], inp.dt * gu)
+        print(f"problem {i} n={inp.A.shape[0]}: {e0:.2e} {e1:.2e}", flush=True)
+        assert e0 < TOL and e1 < TOL
+    ub.delete()
