@@ -1,8 +1,10 @@
 // gm_device.h -- device helpers shared by the kernels of the f(A)b engine.
 //
-// Deterministic reductions: each block reduces in a fixed tree, writes one
-// partial, and the last block to finish (atomic ticket) sums the partials in
-// block order, so results do not depend on scheduling.
+// Deterministic reductions: each block reduces its rows in a fixed tree and
+// writes one partial; the partials are then summed in a fixed order, either by
+// k_reduce launched after the producing kernel or, on small grids, by every
+// block of the consuming kernel itself (block_reduce_partials2), so results do
+// not depend on scheduling.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

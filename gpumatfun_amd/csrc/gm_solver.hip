@@ -686,9 +686,11 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
   GM_TRY(dalloc(&s->V, N * s->m));
   GM_TRY(dalloc(&s->Z, s->nd > 1 ? N * s->nd : 1));
   GM_TRY(dalloc(&s->C, (size_t)s->nd * s->m));
-  // Q column-major n x K -> row-major
+  // Q column-major n x K -> row-major.  The host copy lives until the stream
+  // is synchronised below: upload() returns before the copy has finished.
+  std::vector<double> qr;
   if (s->K > 0) {
-    std::vector<double> qr(N * s->K);
+    qr.resize(N * s->K);
     for (int k = 0; k < s->K; ++k)
       for (size_t i = 0; i < N; ++i) qr[i * s->K + k] = d->Q[(size_t)k * N + i];
     GM_TRY(upload(&s->Q, qr.data(), qr.size(), st));

@@ -35,8 +35,13 @@ jacobi_poly_eval        dense polynomial in A on random complex inputs;
                         ``test_poly_eval_dense``
 nicksmethodcpu          dense f(A) b (eigh) on N <= 64 and the DST-I closed form
                         for the 2D Dirichlet Laplacian; ``test_fab_*``
-step loops              dense exponential-Euler recurrence on tiny N;
-                        ``test_step_dense``
+step loops              dense exponential-Euler recurrence on tiny N (Fisher,
+                        FitzHugh-Nagumo, Mitchell-Schaeffer; each with planted
+                        slips -- recovery variable or gate advanced with the
+                        old u, current with the old gate -- shown to fail);
+                        ``test_step_dense_recurrence``,
+                        ``test_fhn_step_loop_dense_recurrence``,
+                        ``test_ms_step_loop_dense_recurrence``
 ms_reaction             forward-Euler gate in closed form for constant u
                         (h_n = 1 - (1-h0)(1-dt/tau_open)^n), resting state a
                         fixed point, J sign/zeros at u = 0, 1;
@@ -45,7 +50,13 @@ adjoint.gradients       central differences in kappa, b and alpha;
                         ``tests/test_adjoint_oracle.py``
 gpu_semantics pipeline  equal to the CPU step loop when the clamp is off and
                         u >= 0; its own clamp/deflation algebra checked densely;
+                        update_U_monodomain in its GPU parametrisation against
+                        the dense FitzHugh-Nagumo recurrence;
                         ``test_gpu_semantics_*``
+C restatement           the NumPy oracle (bitwise-independent of thread count);
+                        its long-double build and arithmetic variants are
+                        measurement tools, compared to the same
+                        ``test_c_restatement_*``
 ======================  ======================================================
 """
 

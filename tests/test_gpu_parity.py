@@ -148,7 +148,8 @@ def test_headline_parameters_dirichlet128():
     nine coefficient vectors, all three 8-vector deflation tiles and the two-block
     grid of 128 shifted systems.  Stage by stage after one step against the
     GPU-semantics model, then three steps against problem_startup_CPU_nonuniform.m's
-    loop, each to 1e-10."""
+    loop, each to 1e-10 (the Lanczos coefficients on the rows that the oracle's own
+    evaluation order fixes, see below)."""
     inp = headline_parameters_case()
     print("case headline parameters on dirichlet128: d=8 m=80 n_s=64 K=20", flush=True)
     eng = LanczosEngine.from_inputs(inp, clamp=False)
@@ -156,7 +157,28 @@ def test_headline_parameters_dirichlet128():
     assert rel(eng.get_U(), ref.get_U()) < 1e-14
     eng.build_subspace()
     ref.build_subspace()
-    e_ab = rel(eng.get_aabb(), ref.aabb)
+    ab = eng.get_aabb()
+    # Once Ritz values converge (row ~33 here) the later Lanczos coefficients are
+    # forward-unstable: the oracle run on P A P^T (only the summation order
+    # changes) moves them by up to a few percent, while the f(A)b output stays
+    # fixed to 1e-13.  So aabb is held to 1e-10 on the rows that the oracle's own
+    # evaluation order fixes below 1e-12, and to the oracle's own spread after.
+    import scipy.sparse as sp
+    N = inp.A.shape[0]
+    p = np.random.default_rng(1).permutation(N)
+    Ap = sp.csr_matrix(inp.A[p][:, p])
+    Ap.sort_indices()
+    _, _, (a1, b1, a2, b2), _ = oracle.lanczos(Ap, ref.U[p, 0], ref.U[p, 1], inp.m, inp.gamma)
+    perm = np.column_stack([np.append(a1, 0.0), np.append(a2, 0.0), b1, b2])
+    scale = np.abs(ref.aabb).max(axis=0)
+    spread = np.abs(perm - ref.aabb) / scale
+    fixed = int(np.argmax((spread > 1e-12).any(axis=1)))
+    assert fixed >= 20
+    e_ab = rel(ab[:fixed], ref.aabb[:fixed])
+    late = np.abs(ab[fixed:] - ref.aabb[fixed:]) / scale
+    print(f"aabb rows 0..{fixed - 1}: {e_ab:.2e}; later rows: GPU {late.max():.2e}, "
+          f"permuted oracle {spread[fixed:].max():.2e}", flush=True)
+    assert late.max() <= 10.0 * spread[fixed:].max()
     eng.lin_sys_solves()
     ref.lin_sys_solves()
     yr, yi = eng.get_y()

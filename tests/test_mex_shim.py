@@ -125,6 +125,49 @@ def test_mex_step_sequence_matches_engine_and_oracle(mex):
     eng.delete()
 
 
+@pytest.mark.gpu
+def test_mex_get_aabb_and_set_y(mex):
+    """The two gateways lanczosEngine.m:25-31 calls: get_aabb_mex returns the
+    Lanczos coefficients [alpha1 alpha2 beta1 beta2] ((m+1) x 4) of the last
+    build -- against the GPU-semantics model and bit for bit against the library
+    call -- and set_y_mex replaces the shifted solves' output, so apply_V_mex
+    recombines a caller-supplied Y (here the model's solutions scaled by 2 and
+    with the phi systems negated: apply_V is linear in Y)."""
+    import oracle
+    from cases import dirichlet_case
+    from gpumatfun_amd.engine import LanczosEngine
+    inp = dirichlet_case(side=32, m=24, n=16)
+    h = mex("lanczos_create", *create_args(inp))
+    eng = LanczosEngine.from_inputs(inp, clamp=True)
+    ref = oracle.GpuSemantics(inp.U, inp.A, inp.Q, inp.m, inp.gamma, inp.gamma_s_real,
+                              inp.gamma_s_imag, inp.exp_lambda, inp.phi_lambda, inp.dt, inp.mass,
+                              inp.wts1, inp.wts2, inp.shifts, clamp=True)
+    mex("build_subspace_mex", h)
+    eng.build_subspace()
+    ref.build_subspace()
+    ab = mex("get_aabb_mex", h)
+    assert ab.shape == (inp.m + 1, 4)
+    assert np.array_equal(ab, eng.get_aabb())
+    assert rel(ab, ref.aabb) < 1e-10
+    ref.lin_sys_solves()
+    n = inp.extra["n"]
+    sign = np.concatenate([np.ones(n), -np.ones(n)])
+    yr = np.asfortranarray(2.0 * ref.Yr * sign)
+    yi = np.asfortranarray(2.0 * ref.Yi * sign)
+    with pytest.raises(harness.MexError) as e:
+        mex("set_y_mex", h, yr[:, :-1], yi)
+    assert e.value.ident == "MATLAB:set_y:badY"
+    mex("set_y_mex", h, yr, yi)
+    mex("apply_V_mex", h)
+    X = mex("get_xsol_mex", h)
+    ref.Yr, ref.Yi = yr.copy(), yi.copy()
+    ref.apply_V()
+    Xr = ref.get_xsol()
+    assert rel(X[:, 0], Xr[:, 0]) < 1e-10 and rel(X[:, 1], Xr[:, 1]) < 1e-10
+    mex("lanczos_delete", h)
+    eng.delete()
+
+
 def test_eigs_gateway_argument_errors(mex):
     from gpumatfun_amd import problems
     A = problems.laplacian_2d_dirichlet(8)

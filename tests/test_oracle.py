@@ -315,6 +315,67 @@ def test_c_restatement_matches_numpy_oracle(which):
     assert rel(u1, u) < 1e-10 and rel(g1, g) < 1e-10
 
 
+def test_c_restatement_variants_and_long_double():
+    """The measurement switches of oracle/c/fab_oracle.c: variant 0 is the literal
+    restatement bit for bit; on a case whose answer FP64 fixes to ~1e-14 every
+    arithmetic variant and the long-double build agree with it to 1e-12, and full
+    reorthogonalisation with m = N reproduces dense f(A)b; the long-double build's
+    rounding is visibly smaller (it agrees with a 40-digit dense evaluation more
+    closely than the double build does)."""
+    from cases import dirichlet_case, headline_parameters_case, oracle_setup
+    from oracle import cfab
+    inp = dirichlet_case(side=32, m=24, n=16, degree=3)
+    S = oracle_setup(inp)
+    u0, g0 = inp.U[:, 0], inp.U[:, 1]
+    base = cfab.cpu_steps_c(S, u0, g0, 2, threads=2)[:2]
+    again = cfab.cpu_steps_c(S, u0, g0, 2, threads=2, variant=0)[:2]
+    assert np.array_equal(base[0], again[0]) and np.array_equal(base[1], again[1])
+    for v in (cfab.V_SCALE, cfab.V_FMA, cfab.V_TREE, cfab.V_DEFL, cfab.V_REORTH,
+              cfab.V_SCALE | cfab.V_FMA | cfab.V_TREE | cfab.V_DEFL):
+        r = cfab.cpu_steps_c(S, u0, g0, 2, threads=2, variant=v)[:2]
+        assert max(rel(r[0], base[0]), rel(r[1], base[1])) < 1e-12, v
+    ext = cfab.cpu_steps_c(S, u0, g0, 2, threads=2, extended=True)[:2]
+    assert max(rel(ext[0], base[0]), rel(ext[1], base[1])) < 1e-12
+    for v in (cfab.V_RDEFL, cfab.V_RLANCZOS, cfab.V_RSPMV):
+        r = cfab.cpu_steps_c(S, u0, g0, 2, threads=2, variant=v, extended=True)[:2]
+        assert max(rel(r[0], ext[0]), rel(r[1], ext[1])) < 1e-12, v
+    # deflation on: the headline parameters on a small grid
+    hp = headline_parameters_case(side=32, m=40)
+    Sh = oracle_setup(hp)
+    b = cfab.cpu_steps_c(Sh, hp.U[:, 0], hp.U[:, 1], 1, threads=2)[:2]
+    e = cfab.cpu_steps_c(Sh, hp.U[:, 0], hp.U[:, 1], 1, threads=2, extended=True)[:2]
+    r = cfab.cpu_steps_c(Sh, hp.U[:, 0], hp.U[:, 1], 1, threads=2, variant=cfab.V_REORTH)[:2]
+    assert rel(e[0], b[0]) < 1e-12 and rel(r[0], b[0]) < 1e-12
+    # one f(A)b with the 1D Laplacian on 30 points (distinct eigenvalues, so m = N
+    # Lanczos steps span the whole space): with full reorthogonalisation it is exact
+    # up to the quadrature; the long-double build's rounding is far below the double
+    # build's against the same 40-digit evaluation of the same quadrature
+    import mpmath as mp
+    from gpumatfun_amd import quadrature as q
+    N = 30
+    A = sp.csr_matrix(problems.laplacian_1d_dirichlet(N) * float((N + 1) ** 2))
+    A.sort_indices()
+    lam = 4.0 * np.sin(np.arange(1, N + 1) * np.pi / (2 * (N + 1))) ** 2 * (N + 1) ** 2
+    rng = np.random.default_rng(12)
+    a = rng.standard_normal(N)
+    tiny = q.make_inputs(A, a, a, m=N, n=24, eta=1e-2, alpha=1.5, dt=1e-3,
+                         l1=float(lam.min()), lN=float(lam.max()), degree=0)
+    St = oracle_setup(tiny)
+    # u = f1(A) a + dt f2(A) a (g = a, so neither Krylov run starts from zero)
+    vd = cfab.cpu_steps_c(St, a, a, 1, threads=1, variant=cfab.V_REORTH)[0]
+    vl = cfab.cpu_steps_c(St, a, a, 1, threads=1, variant=cfab.V_REORTH, extended=True)[0]
+    mp.mp.dps = 40
+    w, V = mp.eigsy(mp.matrix(A.toarray()))
+    xi, s_ = tiny.extra["xi"], tiny.extra["s"]
+    w1, w2 = tiny.extra["wts1_cpu"] * s_, tiny.extra["wts2_cpu"] * s_
+    r = lambda wt, x: mp.im(sum(mp.mpc(wt[j]) / (mp.mpc(xi[j]) - x) for j in range(len(xi))))
+    fl = [r(w1, w[k]) + mp.mpf(tiny.dt) * r(w2, w[k]) for k in range(N)]
+    c = V.T * mp.matrix(a.tolist())
+    ref = np.array([float(sum(V[i, k] * fl[k] * c[k] for k in range(N))) for i in range(N)])
+    ed, el = rel(vd, ref), rel(vl, ref)
+    assert ed < 1e-13 and el < 0.05 * ed, (ed, el)
+
+
 def test_gpu_semantics_clamp_zeroes_negatives():
     from cases import dirichlet_case
     inp = dirichlet_case(side=16, m=20, n=16)
@@ -347,3 +408,136 @@ def test_ms_reaction_gate_closed_form_and_fixed_point():
     # u = 1: inward current vanishes, only -1/tau_out remains
     _, J = oracle.ms_reaction(np.ones(1), np.ones(1), dt)
     assert J[0] == pytest.approx(-1.0 / p["tau_out"], rel=1e-15)
+
+
+# --------------------------------------------- monodomain step loops (FHN, MS)
+# Each loop is pinned against the exponential-Euler recurrence of its reaction
+# model evaluated with dense matrix functions (eigh) on a tiny grid.  With m = N
+# Lanczos steps the Krylov approximation is exact, so the oracle's whole chain
+# (deflation split, preconditioned Lanczos, contour solves, recombination,
+# reaction update) must reproduce the dense recurrence to the quadrature's
+# accuracy.  The planted variants show that the plausible slips -- the recovery
+# variable or gate advanced with the previous u, the current evaluated with the
+# previous gate -- move the result far beyond the tolerance, so the check
+# discriminates them.
+
+def _mono_problem(K=3, dt=0.05, eta=1e-2, alpha=1.5, seed=7, ode0=None):
+    from gpumatfun_amd import quadrature as q
+    n = 6
+    A = problems.laplacian_2d_dirichlet(n)
+    N = A.shape[0]
+    lamd = problems.dirichlet_eigs_2d(n)
+    w, Qf = np.linalg.eigh(A.toarray())
+    rng = np.random.default_rng(seed)
+    u0 = rng.uniform(0, 1, N)
+    v0 = rng.uniform(0, 0.2, N) if ode0 is None else ode0
+    inp = q.make_inputs(A, u0, u0, m=N, n=48, eta=eta, alpha=alpha, dt=dt, l1=float(w[K]),
+                        lN=float(lamd.max()), degree=1, prob_dim=lamd.max() / 4,
+                        Q=Qf[:, :K], lam=w[:K], ode=v0)
+    z = -eta * w[:K] ** (alpha / 2)
+    inp.phi_lambda[:] = np.expm1(z) / z      # exact phi (no phi(lambda_1) = 1 guard)
+    return A, inp, u0, v0
+
+
+def _dense_mono(A, u, v, steps, dt, eta, alpha, react):
+    """u_{n+1} = exp(-eta A^(alpha/2)) u_n + dt phi(-eta A^(alpha/2)) J(u_n, v_n), with
+    (v_{n+1}, J_{n+1}) = react(u_{n+1}, v_n)."""
+    F1 = lambda x: _dense_f(A, x, eta, alpha, 1)
+    F2 = lambda x: _dense_f(A, x, eta, alpha, 2)
+    _, J = react(u, v, initial=True)
+    for _ in range(steps):
+        u = F1(u) + dt * F2(J)
+        v, J = react(u, v)
+    return u, J, v
+
+
+def _fhn(eps, beta, gam, delta, a, dt, slip=None):
+    """FitzHugh-Nagumo: du/dt = ... + u(1-u)(u-a) - v, dv/dt = eps(beta u - gam v - delta)."""
+    def react(u, v, initial=False, u_prev=[None]):
+        if initial:
+            u_prev[0] = u
+            return v, u * (1 - u) * (u - a) - v
+        ua = u_prev[0] if slip == "old_u" else u
+        vn = v + dt * eps * (beta * ua - gam * v - delta)
+        u_prev[0] = u
+        return vn, u * (1 - u) * (u - a) - vn
+    return react
+
+
+def test_fhn_step_loop_dense_recurrence():
+    """cpu_steps_monodomain (problem_startup_CPU_monodomain.m:128-141) against the
+    dense FitzHugh-Nagumo recurrence, with deflation and a degree-1 polynomial."""
+    from cases import oracle_setup
+    dt, eta, alpha = 0.05, 1e-2, 1.5
+    eps, beta, gam, delta, a = 0.3, 0.5, 1.0, 0.02, 0.1
+    A, inp, u0, v0 = _mono_problem(dt=dt, eta=eta, alpha=alpha)
+    g0 = u0 * (1 - u0) * (u0 - a) - v0
+    u, gu, v = oracle.cpu_steps_monodomain(oracle_setup(inp), u0, g0, v0, 3, a, eps, beta,
+                                           gam, delta)
+    ud, Jd, vd = _dense_mono(A, u0, v0, 3, dt, eta, alpha, _fhn(eps, beta, gam, delta, a, dt))
+    e = max(rel(u, ud), rel(gu, Jd), rel(v, vd))
+    assert e < 1e-9, e
+    us, Js, vs = _dense_mono(A, u0, v0, 3, dt, eta, alpha,
+                             _fhn(eps, beta, gam, delta, a, dt, slip="old_u"))
+    assert max(rel(us, ud), rel(vs, vd)) > 1e3 * max(e, 1e-12)
+
+
+def test_gpu_semantics_monodomain_dense_recurrence():
+    """GpuSemantics.update_U_monodomain (update_U_monodomain.cu:5-43) in its GPU
+    parametrisation (aode, bode, code, vrest, vamp, vpeak, c1ion) against the same
+    dense FitzHugh-Nagumo recurrence: aode = eps beta, bode = -eps gam,
+    code = -dt eps delta, vrest = a, vpeak = 1, c1ion / vamp^2 = -1."""
+    dt, eta, alpha = 0.05, 1e-2, 1.5
+    eps, beta, gam, delta, a = 0.3, 0.5, 1.0, 0.02, 0.1
+    A, inp, u0, v0 = _mono_problem(dt=dt, eta=eta, alpha=alpha, seed=8)
+    g0 = u0 * (1 - u0) * (u0 - a) - v0
+    vamp = 2.0
+    fhn = dict(aode=eps * beta, bode=-eps * gam, code=-dt * eps * delta, vrest=a, vamp=vamp,
+               vpeak=1.0, c1ion=-vamp * vamp)
+    G = oracle.GpuSemantics(np.column_stack([u0, g0]), inp.A, inp.Q, inp.m, inp.gamma,
+                            inp.gamma_s_real, inp.gamma_s_imag, inp.exp_lambda, inp.phi_lambda,
+                            dt, inp.mass, inp.wts1, inp.wts2, inp.shifts, ode=v0, fhn=fhn,
+                            clamp=False)
+    for _ in range(3):
+        G.step(monodomain=True)
+    ud, Jd, vd = _dense_mono(A, u0, v0, 3, dt, eta, alpha, _fhn(eps, beta, gam, delta, a, dt))
+    U = G.get_U()
+    e = max(rel(U[:, 0], ud), rel(U[:, 1], dt * Jd), rel(G.ode, vd))
+    assert e < 1e-9, e
+
+
+def _ms(p, dt, slip=None):
+    """Mitchell-Schaeffer: dh/dt = (1-h)/tau_open (u < v_gate), -h/tau_close otherwise;
+    J = h u^2 (1-u)/tau_in - u/tau_out."""
+    def react(u, h, initial=False, u_prev=[None]):
+        if initial:
+            u_prev[0] = u
+            return h, h * u * u * (1 - u) / p["tau_in"] - u / p["tau_out"]
+        ug = u_prev[0] if slip == "old_u" else u
+        rate = np.where(ug < p["v_gate"], (1 - h) / p["tau_open"], -h / p["tau_close"])
+        hn = h + dt * rate
+        hj = h if slip == "old_gate" else hn
+        u_prev[0] = u
+        return hn, hj * u * u * (1 - u) / p["tau_in"] - u / p["tau_out"]
+    return react
+
+
+def test_ms_step_loop_dense_recurrence():
+    """cpu_steps_ms against the dense Mitchell-Schaeffer recurrence (gate advanced with
+    the new u, current with the new gate), with u0 straddling v_gate so both gate
+    branches are taken."""
+    from cases import oracle_setup
+    p = dict(oracle.stepping.MS_DEFAULT)
+    p.update(tau_open=2.0, tau_close=3.0)     # gate moves visibly within 3 steps
+    dt, eta, alpha = 0.05, 1e-2, 1.5
+    A, inp, u0, _ = _mono_problem(dt=dt, eta=eta, alpha=alpha, seed=9)
+    u0 = 0.26 * u0                             # values on both sides of v_gate = 0.13
+    h0 = np.random.default_rng(3).uniform(0.3, 1.0, u0.size)
+    u, gu, h = oracle.cpu_steps_ms(oracle_setup(inp), u0, h0, 3, p)
+    ud, Jd, hd = _dense_mono(A, u0, h0, 3, dt, eta, alpha, _ms(p, dt))
+    e = max(rel(u, ud), rel(gu, Jd), rel(h, hd))
+    assert e < 1e-9, e
+    assert np.any(u0 < p["v_gate"]) and np.any(u0 >= p["v_gate"])
+    for slip in ("old_u", "old_gate"):
+        us, Js, hs = _dense_mono(A, u0, h0, 3, dt, eta, alpha, _ms(p, dt, slip=slip))
+        assert max(rel(us, ud), rel(Js, Jd), rel(hs, hd)) > 1e3 * max(e, 1e-12), slip
