@@ -39,7 +39,8 @@ EXPORTS = (
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
     "gm_batch_get_ode",
     "gm_fab_last_error", "gm_fab_create", "gm_fab_delete", "gm_fab_set_matrix",
-    "gm_fab_set_weights", "gm_fab_forward", "gm_fab_backward", "gm_fab_backward_param",
+    "gm_fab_set_weights", "gm_fab_forward", "gm_fab_forward_noalloc", "gm_fab_backward",
+    "gm_fab_backward_param",
     "gm_fab_set_param_derivs",
     "gm_ubatch_last_error", "gm_ubatch_create", "gm_ubatch_delete", "gm_ubatch_step",
     "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info",
@@ -294,6 +295,9 @@ def load():
         "gm_fab_set_matrix": (C.c_int, [vp, _dp]),
         "gm_fab_set_weights": (C.c_int, [vp, _dp, _dp]),
         "gm_fab_forward": (C.c_int, [vp, _dp, _dp]),
+        "gm_fab_forward_noalloc": (C.c_int, [C.c_int32, C.c_int64, vp, vp, vp, C.c_int32,
+                                             C.c_int32, vp, vp, vp, vp, vp, C.c_int32,
+                                             C.POINTER(C.c_uint32)]),
         "gm_fab_backward": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "gm_fab_backward_param": (C.c_int, [vp, _dp, C.c_int32, C.POINTER(_dp), _dp, _dp]),
         "gm_fab_set_param_derivs": (C.c_int, [vp, C.c_int32, C.POINTER(_dp)]),

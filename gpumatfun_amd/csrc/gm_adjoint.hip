@@ -1062,3 +1062,266 @@ gm_status gm_fab_set_param_derivs(gm_fab* f, int32_t nparams, const double* cons
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// gm_fab_forward_noalloc: u = f(A) b exactly as gm_fab_forward computes it
+// (K = 0), for callers that cannot allocate device memory.  Every buffer
+// beyond the caller's inputs and outputs lives in `scratch`, a file-scope
+// device array of kNoallocScratchBytes that each GPU holds from the library's
+// first use on it until the process ends; its device address comes from
+// hipGetSymbolAddress once per device, with that device current.  All calls
+// on one GPU share the array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+constexpr size_t kNoallocScratchDoubles = kNoallocScratchBytes / sizeof(double);
+constexpr int kNoallocMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoallocMaxDevices] = {};
+
+// The device address of `scratch` on the current device (`dev`).
+gm_status scratch_address(int dev, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return afail(GM_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                   hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return GM_OK;
+}
+
+// Clears the status word and derives the unit solve weights (ns x 2 [Re Im])
+// and the split contour weights gsr, gsi from the caller's wts: a fill kernel
+// rather than hipMemsetAsync, so the sequence can be captured into a graph.
+__global__ __launch_bounds__(256) void k_noalloc_prepare(const double* __restrict__ wts, int ns,
+                                                         double* ones, double* gsr, double* gsi,
+                                                         unsigned* status) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) *status = 0u;
+  if (s < ns) {
+    ones[s] = 1.0;
+    ones[ns + s] = 0.0;
+    gsr[s] = wts[2 * s];
+    gsi[s] = wts[2 * s + 1];
+  }
+}
+
+// Carves `count` doubles from the scratch cursor, 256-byte aligned.
+double* carve(double*& cur, size_t count) {
+  double* p = cur;
+  cur += (std::max<size_t>(count, 1) + 31) & ~size_t(31);
+  return p;
+}
+
+size_t carve_size(size_t count) { return (std::max<size_t>(count, 1) + 31) & ~size_t(31); }
+
+}  // namespace
+
+extern "C" gm_status gm_fab_forward_noalloc(int32_t n, int64_t nnz, const int32_t* A_ptr,
+                                            const int32_t* A_ind, const double* A_values,
+                                            int32_t subspace_size, int32_t contour_size,
+                                            const double* xi, const double* wts, const double* b,
+                                            double* u, void* stream, int32_t device,
+                                            uint32_t* status) {
+  g_aerr.clear();
+  if (n <= 0 || nnz < 0 || subspace_size < 1 || contour_size < 1)
+    return afail(GM_EINVAL, "bad sizes (n %d, nnz %lld, m %d, n_s %d)", n, (long long)nnz,
+                 subspace_size, contour_size);
+  if (!A_ptr || !A_ind || !A_values || !xi || !wts || !b || !u)
+    return afail(GM_EINVAL, "null device array");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count ||
+      device >= kNoallocMaxDevices)
+    return afail(GM_EDEVICE, "no HIP device %d", device);
+  const size_t N = (size_t)n, m = (size_t)subspace_size, ns = (size_t)contour_size;
+  const size_t ny = m * ns;
+  const size_t need = carve_size(N * m) + 2 * carve_size(N) + carve_size(m) + carve_size(m + 1) +
+                      2 * carve_size(ny) + carve_size(m) + 2 * carve_size(4096) +
+                      4 * carve_size(2 * ny) + carve_size(2 * ns) + 2 * carve_size(ns) +
+                      carve_size(1);
+  if (need > kNoallocScratchDoubles)
+    return afail(GM_EINVAL,
+                 "n = %d, m = %d, n_s = %d needs %zu bytes of scratch; the limit is %zu bytes "
+                 "(about n (m + 2) <= %zu doubles)",
+                 n, subspace_size, contour_size, need * sizeof(double), kNoallocScratchBytes,
+                 kNoallocScratchDoubles);
+  int prev = -1;
+  A_HIP(hipGetDevice(&prev));
+  if (prev != device) A_HIP(hipSetDevice(device));
+  struct Restore {
+    int prev, dev;
+    ~Restore() {
+      if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+    }
+  } restore{prev, device};
+  double* base = nullptr;
+  A_TRY(scratch_address(device, &base));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+
+  // the buffers of gm_fab's Side and engine, carved from the scratch array
+  double* cur = base;
+  double* V = carve(cur, N * m);
+  double* W0 = carve(cur, N);
+  double* W1 = carve(cur, N);
+  double* al = carve(cur, m);
+  double* be = carve(cur, m + 1);
+  double* yr = carve(cur, ny);
+  double* yi = carve(cur, ny);
+  double* C = carve(cur, m);
+  double* partials = carve(cur, 4096);
+  double* partials2 = carve(cur, 4096);
+  double2* wD = reinterpret_cast<double2*>(carve(cur, 2 * ny));
+  double2* wDU = reinterpret_cast<double2*>(carve(cur, 2 * ny));
+  double2* wDU2 = reinterpret_cast<double2*>(carve(cur, 2 * ny));
+  double2* wX = reinterpret_cast<double2*>(carve(cur, 2 * ny));
+  double* ones = carve(cur, 2 * ns);
+  double* gsr = carve(cur, ns);
+  double* gsi = carve(cur, ns);
+  unsigned* stat = reinterpret_cast<unsigned*>(carve(cur, 1));
+
+  const int G = std::max(1, std::min(4096, adiv(n, 256)));
+  const int chunk = adiv(n, G);
+  hipLaunchKernelGGL(k_noalloc_prepare, dim3(adiv(contour_size, 256)), dim3(256), 0, st, wts,
+                     contour_size, ones, gsr, gsi, stat);
+  A_HIP(hipGetLastError());
+  auto reduce = [&](int nv, double* out, int do_sqrt, int flag_zero,
+                    const double* part) -> gm_status {
+    gm::ReduceArgs r{};
+    r.partials = part;
+    r.G = G;
+    r.nv = nv;
+    r.out = out;
+    r.stride = 1;
+    r.do_sqrt = do_sqrt;
+    r.flag_zero = flag_zero;
+    r.status = stat;
+    hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0,
+                       st, r);
+    A_HIP(hipGetLastError());
+    return GM_OK;
+  };
+
+  // the launch sequence of krylov() in gm_fab_forward, reading b in place
+  gmb::BNorm nn{};
+  nn.u = b;
+  nn.ldu = 1;
+  nn.be = be;
+  nn.nrows = n;
+  nn.chunk = chunk;
+  nn.partials = partials;
+  nn.status = stat;
+  hipLaunchKernelGGL((gmb::k_bnorm<1>), dim3(G), dim3(gm::kBlock), 0, st, nn);
+  A_HIP(hipGetLastError());
+  A_TRY(reduce(1, be, 1, 0, partials));
+  gmb::BSpmm sb{};
+  sb.rowptr = A_ptr;
+  sb.col = A_ind;
+  sb.val = A_values;
+  sb.nrows = n;
+  sb.chunk = chunk;
+  sb.al = al;
+  sb.be = be;
+  sb.partials = partials;
+  sb.g0 = 1.0;
+  double* Wb[2] = {W0, W1};
+  const bool fuse = G <= 1024;
+  for (int k = 0; k < subspace_size; ++k) {
+    gmb::BSpmm a = sb;
+    a.k = k;
+    a.x = k == 0 ? b : Wb[(k - 1) & 1];
+    a.ldx = 1;
+    a.y = Wb[k & 1];
+    a.ldy = 1;
+    a.vk = V + (size_t)k * N;
+    a.vkm1 = k > 0 ? V + (size_t)(k - 1) * N : nullptr;
+    a.status = stat;
+    if (fuse && k > 0) {
+      a.bpart = partials2;
+      a.bG = G;
+    }
+    hipLaunchKernelGGL((gmb::k_bspmm<1, gmb::kOnly>), dim3(G), dim3(gm::kBlock), 0, st, a);
+    A_HIP(hipGetLastError());
+    if (!fuse) A_TRY(reduce(1, al + k, 0, 0, partials));
+    gmb::BFinish fn{};
+    fn.w = Wb[k & 1];
+    fn.vk = a.vk;
+    fn.al = al;
+    fn.be = be;
+    fn.k = k;
+    fn.nrows = n;
+    fn.chunk = chunk;
+    fn.partials = fuse ? partials2 : partials;
+    fn.status = stat;
+    if (fuse) {
+      fn.apart = partials;
+      fn.aG = G;
+      fn.al_out = al;
+    }
+    hipLaunchKernelGGL((gmb::k_bfinish<1>), dim3(G), dim3(gm::kBlock), 0, st, fn);
+    A_HIP(hipGetLastError());
+    if (!fuse || k == subspace_size - 1)
+      A_TRY(reduce(1, be + k + 1, 1, 1, fuse ? partials2 : partials));
+  }
+  gmb::BSolve sv{};
+  sv.al = al;
+  sv.be = be;
+  sv.R = 1;
+  sv.m = subspace_size;
+  sv.ns = contour_size;
+  sv.wts1 = ones;
+  sv.wts2 = ones;
+  sv.etaxi = xi;
+  sv.wD = wD;
+  sv.wDU = wDU;
+  sv.wDU2 = wDU2;
+  sv.wX = wX;
+  sv.yr = yr;
+  sv.yi = yi;
+  sv.status = stat;
+  hipLaunchKernelGGL(gmb::k_bsolves<0>, dim3(adiv(contour_size, 64)), dim3(64), 0, st, sv);
+  A_HIP(hipGetLastError());
+  gmb::BCollapse cc{};
+  cc.yr = yr;
+  cc.yi = yi;
+  cc.gsr = gsr;
+  cc.gsi = gsi;
+  cc.R = 1;
+  cc.m = subspace_size;
+  cc.ns = contour_size;
+  cc.nd = 1;
+  cc.C = C;
+  hipLaunchKernelGGL(gmb::k_bcollapse<0>, dim3(adiv(subspace_size, 256)), dim3(256), 0, st, cc);
+  A_HIP(hipGetLastError());
+  gmb::BVC v{};
+  v.V = V;
+  v.C = C;
+  v.R = 1;
+  v.m = subspace_size;
+  v.nd = 1;
+  v.nrows = n;
+  v.Z = u;
+  v.ldz = 1;
+  v.accumulate = 0;
+  hipLaunchKernelGGL((gmb::k_bvc<1>), dim3(adiv(n, 256)), dim3(256), 0, st, v);
+  A_HIP(hipGetLastError());
+  if (status) {
+    unsigned sv_host = 0;
+    A_HIP(hipMemcpyAsync(&sv_host, stat, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    A_HIP(hipStreamSynchronize(st));
+    *status = sv_host;
+    if (sv_host & 6u) return afail(GM_EBREAKDOWN, "Lanczos breakdown (%u)", sv_host);
+  }
+  return GM_OK;
+}

@@ -351,6 +351,26 @@ gm_status gm_fab_backward_param(gm_fab* f, const double* g, int32_t nparams,
  * adjoint calls it twice per step with the same ones).                     */
 gm_status gm_fab_set_param_derivs(gm_fab* f, int32_t nparams, const double* const* dA_values);
 
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): u = f(A) b as gm_fab_forward computes it without deflation, the
+ * same kernels and the same results, with no handle and no allocation.  All
+ * arrays are device memory of `device`: A in CSR (n + 1, nnz, nnz), the n_s
+ * complex nodes xi and weights wts ([re im] interleaved), b and u (n).  The
+ * Krylov basis and the other work buffers live in one device array of 64 MiB
+ * that the library keeps on each GPU it runs on until the process ends, which
+ * limits n (m + 2) + 10 m n_s to about 8.3 million doubles (GM_EINVAL beyond:
+ * e.g. n = 100,000 at m = 80); calls that overlap on one GPU race for it.
+ * Work is enqueued on `stream` (NULL: the null stream); with status non-NULL
+ * the call waits for it and stores the status bits (non-zero bit 1 or 2:
+ * breakdown, returned as GM_EBREAKDOWN), otherwise it returns once enqueued.
+ * Errors: gm_fab_last_error().                                              */
+gm_status gm_fab_forward_noalloc(int32_t n, int64_t nnz, const int32_t* A_ptr,
+                                 const int32_t* A_ind, const double* A_values,
+                                 int32_t subspace_size, int32_t contour_size, const double* xi,
+                                 const double* wts, const double* b, double* u, void* stream,
+                                 int32_t device, uint32_t* status);
+
 /* ---------------------------------------------------------------------------
  * Uneven batch (no reference counterpart): P independent problems of
  * different sizes -- each its own A_p, [u_p, g_p], contour weights, shifted
