@@ -8,7 +8,10 @@ Workloads (``--config``):
   equispaced boundary points), A = M^{-1/2} K M^{-1/2} in CSR (symrcm order),
   f1 = exp(-tau kappa A^{alpha/2}), f2 = phi(...), alpha = 1.7, tau = 1e-3,
   kappa = 1, degree-8 Jacobi polynomial preconditioner, Lanczos m = 80, 64
-  contour shifts, 20 deflated eigenvectors.
+  contour shifts, 20 deflated eigenvectors.  Its line also carries the other
+  two parts of the metric: ``batched_form`` (the ``uneven64`` batch against the
+  plain path on the same problems) and ``adjoint_form`` (``adjoint256``: the
+  value-and-gradient cost against one forward solve), each with its parity.
 * ``dirichlet128`` -- 2D 5-point Dirichlet Laplacian 128 x 128 scaled by 1/h^2,
   b ~ N(0,1) seed 0, alpha = 1.5, m = 50, 32 shifts, no preconditioner, no deflation.
 * ``batch3d`` -- batched form: 3D 7-point Dirichlet Laplacian 128^3 (scaled by
@@ -262,7 +265,7 @@ def run_uneven(args, steps, warm, use_graph):
                                   "seconds_per_step": round(cpu_s, 4),
                                   "host_cpus": os.cpu_count()}
         log(f"oracle: {cpu_s:.2f}s per step of all problems, worst rel err {worst:.3e}")
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def run_adjoint(args, steps, warm):
@@ -337,7 +340,32 @@ def run_adjoint(args, steps, warm):
                                             "NumPy restatement (oracle/adjoint.py)",
                                   "seconds": round(cpu_s, 3), "host_cpus": os.cpu_count()}
         log(f"oracle backward {cpu_s:.1f}s; rel err {result['rel_err_vs_oracle']}")
-    print(json.dumps(result), flush=True)
+    return result
+
+
+def form_legs(args, use_graph):
+    """The other two parts of the metric, measured beside the plain path in the default
+    run: the uneven batch against the plain path on the same 64 problems, and the
+    adjoint's cost against one forward solve, each with its parity number."""
+    import copy
+    legs = {}
+    a = copy.copy(args)
+    a.problems = 64
+    r = run_uneven(a, 20, 3, use_graph)
+    legs["batched_form"] = {
+        "workload": r["config"]["workload"], "value": r["value"], "unit": r["unit"],
+        "plain_same_problems": r["plain_same_problems"]["value"],
+        "batched_vs_plain": r["batched_vs_plain"], "steps": r["steps"],
+        "ms_per_step": r["ms_per_step"], "total_rows": r["config"]["total_rows"],
+        "rel_err_vs_oracle": r.get("rel_err_vs_oracle")}
+    r = run_adjoint(a, 10, 2)
+    legs["adjoint_form"] = {
+        "workload": r["config"]["workload"], "value": r["value"], "unit": r["unit"],
+        "forward_ms": r["forward_ms"], "forward_backward_ms": r["ms_per_step"],
+        "adjoint_cost_vs_forward": r["adjoint_cost_vs_forward"],
+        "recompute_basis_cost_vs_forward": r["recompute_basis"]["adjoint_cost_vs_forward"],
+        "rel_err_vs_oracle": r.get("rel_err_vs_oracle")}
+    return legs
 
 
 def build_ms512(args):
@@ -800,6 +828,8 @@ def main():
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="fem1m: skip the batched (uneven64) and adjoint (adjoint256) legs")
     ap.add_argument("--setup-cache", default=None,
                     help="directory to keep the FEM set-up (mesh, operator, eigenpairs) between runs")
     ap.add_argument("--schedule", default="auto", choices=["auto", "rows", "blocks", "blocks16"],
@@ -851,7 +881,7 @@ def main():
         run_ms512(args, steps, warm, use_graph)
         return
     if args.config == "adjoint256":
-        run_adjoint(args, steps, warm)
+        print(json.dumps(run_adjoint(args, steps, warm)), flush=True)
         return
     if args.config == "eigs512":
         run_eigs512(args)
@@ -862,7 +892,7 @@ def main():
     if args.config == "uneven64":
         if args.problems == 256:
             args.problems = 64
-        run_uneven(args, steps, warm, use_graph)
+        print(json.dumps(run_uneven(args, steps, warm, use_graph)), flush=True)
         return
 
     inp, cfg, setup_s = (build_fem1m if args.config == "fem1m" else build_dirichlet128)(args)
@@ -925,6 +955,8 @@ def main():
     if not args.no_cpu and not args.profile_only:
         cpu_leg(inp, inp.U[:, 0], inp.U[:, 1], U1, result, study=args.parity_study)
     eng.delete()
+    if args.config == "fem1m" and not args.no_forms and not args.profile_only:
+        result.update(form_legs(args, use_graph))
     print(json.dumps(result), flush=True)
 
 
