@@ -162,6 +162,16 @@ def build_batch3d(args):
     t0 = time.time()
     side = args.side3d
     A = problems.laplacian_3d_dirichlet(side)
+    order = None
+    if args.node_order == "pencil":
+        # the same operator with the nodes in 16 x 16 pencils (P A P^T, and P u for
+        # every right-hand side): f(P A P^T) P u = P f(A) u
+        import scipy.sparse as sp
+        order = problems.pencil_order(side, 16)
+        A = sp.csr_matrix(A[order][:, order])
+        A.sort_indices()
+        A.indptr = A.indptr.astype(np.int32)
+        A.indices = A.indices.astype(np.int32)
     h = 1.0 / (side + 1)
     l1d = 4.0 * np.sin(np.arange(1, side + 1) * np.pi * h / 2) ** 2 / h ** 2
     lmin, lmax = 3 * l1d.min(), 3 * l1d.max()
@@ -170,6 +180,8 @@ def build_batch3d(args):
     U = np.empty((N, 2 * P))
     for p in range(P):
         u = np.random.default_rng(p).standard_normal(N)
+        if order is not None:
+            u = u[order]
         U[:, 2 * p] = u
         U[:, 2 * p + 1] = u * (1 - u)
     inp = q.make_inputs(A, U[:, 0], U[:, 1], m=50, n=64, eta=1e-3, alpha=1.5, dt=1e-3,
@@ -177,7 +189,7 @@ def build_batch3d(args):
     cfg = {"workload": f"batch_laplace3d_{side}_{P}rhs", "n": int(N), "nnz": int(A.nnz),
            "problems": P, "m": 50, "poly_degree": 0, "shifts": 64, "deflation": 0,
            "alpha": 1.5, "tau": 1e-3, "kappa": 1.0, "columns_per_step": 2 * P,
-           "rhs_seeds": f"0..{P - 1}"}
+           "rhs_seeds": f"0..{P - 1}", "node_order": args.node_order}
     return inp, U, cfg, time.time() - t0
 
 
@@ -211,7 +223,7 @@ def run_uneven(args, steps, warm, use_graph):
     inps, cfg, setup_s = build_uneven64(args)
     P = len(inps)
     log(f"setup {setup_s:.1f}s, {P} problems, {cfg['total_rows']} rows")
-    ub = UnevenBatch.from_inputs(inps, clamp=False)
+    ub = UnevenBatch.from_inputs(inps, clamp=False, autotune=True)
     ub.step(1, use_graph=False)
     ub.synchronize()
     U1 = [ub.get_U(p) for p in range(P)]
@@ -244,6 +256,7 @@ def run_uneven(args, steps, warm, use_graph):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded right-hand sides)", "config": cfg, "graph": use_graph,
         "row_blocks": info["row_blocks"],
+        "spmm_schedule": {k: info[k] for k in ("spmm_tile", "index_bytes", "early_loads")},
         "plain_same_problems": {"value": round(plain, 3), "unit": "solves/s",
                                 "how": "one plain engine per problem, stepped in turn"},
         "batched_vs_plain": round(value / plain, 3),
@@ -828,13 +841,18 @@ def main():
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--node-order", default="natural", choices=["natural", "pencil"],
+                    help="batch3d: grid nodes in natural (i + n j + n^2 k) or 16 x 16-pencil order")
+    ap.add_argument("--chunk-columns", type=int, default=0,
+                    help="batch3d: columns per pass (0: chosen by the create-time autotuner)")
     ap.add_argument("--no-forms", action="store_true",
                     help="fem1m: skip the batched (uneven64) and adjoint (adjoint256) legs")
     ap.add_argument("--setup-cache", default=None,
                     help="directory to keep the FEM set-up (mesh, operator, eigenpairs) between runs")
-    ap.add_argument("--schedule", default="auto", choices=["auto", "rows", "blocks", "blocks16"],
+    ap.add_argument("--schedule", default="auto", choices=["auto", "rows", "blocks", "blocks16", "blocks16e"],
                     help="SpMM schedule of the plain path: autotuned at create (default), "
-                         "or forced (fixed kernels for counter runs)")
+                         "or forced (fixed kernels for counter runs; blocks16e: 16-bit "
+                         "column offsets with the epilogue operands loaded early)")
     ap.add_argument("--adjoint-engine", default="batch", choices=["batch", "streams"],
                     help="msadj512: all stimuli in one multi-column engine, or one engine "
                          "per stimulus on concurrent host threads")
@@ -902,7 +920,8 @@ def main():
         eng = LanczosEngine.from_inputs(inp, clamp=False, autotune=True)
     else:
         eng = LanczosEngine.from_inputs(inp, clamp=False, schedule=args.schedule[:6],
-                                        index16=args.schedule == "blocks16")
+                                        index16=args.schedule.startswith("blocks16"),
+                                        early_loads=args.schedule == "blocks16e")
     info = eng.info()
     log(f"create {time.time() - t0:.2f}s, schedule {info['spmm_schedule']}")
 
@@ -925,6 +944,11 @@ def main():
     log(f"spmm mode {mode}: {spmm_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
     copy_bytes = 2 * 16 * n * 8   # read + write of 8 column pairs
     copy_ms = eng.profile_copy(copy_bytes // 2, reps=5 if args.profile_only else 50)
+    # the same launches with L2 and the Infinity Cache flushed before each (a 1 GiB
+    # streaming copy), so nothing of the working set can be served from cache
+    cold_ms = eng.profile_spmm_flushed(mode=mode, reps=5 if args.profile_only else 50)
+    cold = bytes_per / (cold_ms * 1e-3) / 1e9
+    log(f"spmm mode {mode} after a 1 GiB flush: {cold_ms * 1e3:.2f} us/launch, {cold:.1f} GB/s")
 
     result = {
         "metric": "f(A)b solves/sec (FP64, plain path)",
@@ -948,7 +972,12 @@ def main():
                      "traffic": measured_traffic(cfg["workload"], kernel_key(info, 1 if mode == 0 else 2)),
                      "kernel": kernel_symbol(info, 1 if mode == 0 else 2),
                      "avg_launch_us": round(spmm_ms * 1e3, 3),
-                     "algorithmic_bytes_per_launch": int(bytes_per)},
+                     "algorithmic_bytes_per_launch": int(bytes_per),
+                     "cache_flushed": {
+                         "what": "each launch after a 1 GiB streaming copy (cold L2 and "
+                                 "Infinity Cache), timed alone with its launch latency",
+                         "avg_launch_us": round(cold_ms * 1e3, 3),
+                         "achieved": round(cold, 1), "frac": round(cold / HBM_PEAK_GBS, 4)}},
         "copy_kernel_gbs": round(copy_bytes / (copy_ms * 1e-3) / 1e9, 1),
         "cpu_baseline": None,
     }
@@ -967,7 +996,10 @@ def run_batch(args, steps, warm, use_graph):
     log(f"setup {setup_s:.1f}s, U {U.shape}")
     P = U.shape[1] // 2
     t0 = time.time()
-    b = BatchEngine.from_inputs(inp, U, clamp=False, autotune=True)
+    if args.chunk_columns:
+        b = BatchEngine.from_inputs(inp, U, clamp=False, chunk_columns=args.chunk_columns)
+    else:
+        b = BatchEngine.from_inputs(inp, U, clamp=False, autotune=True)
     info = b.info()
     log(f"batch create {time.time() - t0:.1f}s, chunk columns {info['chunk_columns']}")
     # parity leg: problem 0 after one step
@@ -978,16 +1010,23 @@ def run_batch(args, steps, warm, use_graph):
     value = 2.0 * P * steps / elapsed
     log(f"batched: {steps} steps in {elapsed:.3f}s -> {value:.1f} solves/s")
     b.delete()
+    u0_cpu, g0_cpu = U[:, 0].copy(), U[:, 1].copy()   # problem 0, for the CPU leg
     del U
 
     # the plain path on the same matrix (one problem, 2 columns per step)
-    e = LanczosEngine.from_inputs(inp, clamp=False, autotune=True)
+    if args.schedule == "auto":
+        e = LanczosEngine.from_inputs(inp, clamp=False, autotune=True)
+    else:
+        e = LanczosEngine.from_inputs(inp, clamp=False, schedule=args.schedule[:6],
+                                      index16=args.schedule.startswith("blocks16"),
+                                      early_loads=args.schedule == "blocks16e")
     einfo = e.info()
     psteps = max(5, steps * 4)
     pel = timed(lambda k: e.step(k, use_graph=use_graph), e.synchronize, psteps, 2)
     plain = 2.0 * psteps / pel
     mode = 1
     spmm_ms = e.profile_spmm(mode=mode, reps=10 if args.profile_only else 100)
+    cold_ms = e.profile_spmm_flushed(mode=mode, reps=5 if args.profile_only else 50)
     n, nnz = einfo["n"], einfo["nnz"]
     bytes_per = spmm_bytes(einfo, 2)
     e.delete()
@@ -1017,12 +1056,14 @@ def run_batch(args, steps, warm, use_graph):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_per / (spmm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("laplace3d_128_plain", kernel_key(einfo, 2)),
-                     "avg_launch_us": round(spmm_ms * 1e3, 3)},
+                     "avg_launch_us": round(spmm_ms * 1e3, 3),
+                     "cache_flushed": {
+                         "avg_launch_us": round(cold_ms * 1e3, 3),
+                         "frac": round(bytes_per / (cold_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
         "cpu_baseline": None,
     }
     if not args.no_cpu and not args.profile_only:
-        r0 = np.random.default_rng(0).standard_normal(inp.A.shape[0])
-        cpu_leg(inp, r0, r0 * (1 - r0), U1, result)
+        cpu_leg(inp, u0_cpu, g0_cpu, U1, result)
     print(json.dumps(result), flush=True)
 
 

@@ -34,7 +34,7 @@ EXPORTS = (
     "gm_get_U", "gm_get_xsol", "gm_get_aabb", "gm_set_y", "gm_get_y", "gm_get_state",
     "gm_set_state", "gm_step", "gm_synchronize", "gm_get_info", "gm_get_schedule",
     "gm_get_stream",
-    "gm_profile_spmm", "gm_profile_copy", "gm_set_ms_params", "gm_update_U_ms",
+    "gm_profile_spmm", "gm_profile_spmm_flushed", "gm_profile_copy", "gm_set_ms_params", "gm_update_U_ms",
     "gm_batch_last_error", "gm_batch_create", "gm_batch_delete", "gm_batch_step",
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
     "gm_batch_get_ode",
@@ -43,7 +43,7 @@ EXPORTS = (
     "gm_fab_backward_param",
     "gm_fab_set_param_derivs",
     "gm_ubatch_last_error", "gm_ubatch_create", "gm_ubatch_delete", "gm_ubatch_step",
-    "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info",
+    "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info", "gm_ubatch_get_schedule",
     "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
     "gm_subspace_size",
     "gm_eigs_last_error", "gm_eigs_smallest", "gm_sym_eig",
@@ -277,6 +277,7 @@ def load():
         "gm_get_schedule": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                       C.POINTER(C.c_int32)]),
         "gm_profile_spmm": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
+        "gm_profile_spmm_flushed": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int64, _dp]),
         "gm_profile_copy": (C.c_int, [vp, C.c_int64, C.c_int32, _dp]),
         "gm_set_ms_params": (C.c_int, [vp, C.POINTER(GmMsParams)]),
         "gm_update_U_ms": (C.c_int, [vp]),
@@ -309,6 +310,8 @@ def load():
         "gm_ubatch_synchronize": (C.c_int, [vp]),
         "gm_ubatch_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int32)]),
+        "gm_ubatch_get_schedule": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_int32)]),
         "gm_polynomial_coeffs": (C.c_int, [C.c_int32, C.c_double, _dp]),
         "gm_contour_setup": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_double,
                                        C.c_double, _dp, _dp, _dp, _dp, _dp]),

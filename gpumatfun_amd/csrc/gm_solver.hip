@@ -1096,6 +1096,60 @@ gm_status gm_profile_spmm(gm_solver* s, int32_t mode, int32_t reps, double* avg_
   return st;
 }
 
+gm_status gm_profile_spmm_flushed(gm_solver* s, int32_t mode, int32_t reps, int64_t flush_bytes,
+                                  double* avg_ms) {
+  GM_CHECK_HANDLE(s);
+  if (!avg_ms || reps < 1 || flush_bytes < 32)
+    return fail(GM_EINVAL, "avg_ms must be non-null, reps >= 1 and flush_bytes >= 32");
+  if (mode != 0 && mode != 1) return fail(GM_EINVAL, "mode must be 0 or 1");
+  if (!s->built) return fail(GM_ESTATE, "profile_spmm_flushed needs a built subspace (run a step first)");
+  const size_t N = (size_t)s->n;
+  SpmmArgs a = spmm_base(s);
+  a.g0 = s->gamma[0];
+  a.gi = s->nd > 1 ? s->gamma[1] : 0.0;
+  const int k = s->m > 1 ? 1 : 0;   // as in gm_profile_spmm
+  a.k = k;
+  a.vk = s->V + (size_t)k * N;
+  a.vkm1 = k > 0 ? s->V : nullptr;
+  const long long n = flush_bytes / 32;   // half read, half written
+  double2 *src = nullptr, *dst = nullptr;
+  GM_TRY(dalloc(&src, (size_t)n));
+  gm_status st = dalloc(&dst, (size_t)n);
+  if (st != GM_OK) {
+    (void)hipFree(src);
+    return st;
+  }
+  hipEvent_t e0, e1;
+  GM_HIP(hipEventCreate(&e0));
+  GM_HIP(hipEventCreate(&e1));
+  const int grid = (int)std::min<long long>(8192, (n + kBlock - 1) / kBlock);
+  double total = 0.0;
+  for (int i = 0; i <= reps && st == GM_OK; ++i) {   // i = 0: warm-up, not timed
+    hipLaunchKernelGGL(gm::k_copy16<0>, dim3(grid), dim3(kBlock), 0, s->stream, src, dst, n);
+    GM_HIP(hipEventRecord(e0, s->stream));
+    if (mode == 0) {
+      a.x = (i & 1) ? s->T[1] : s->V;
+      a.y = (i & 1) ? s->T[0] : s->T[1];
+      st = launch_spmm_mode<gm::kMid>(s, a);
+    } else {
+      a.x = s->V + (size_t)k * N;
+      a.y = s->W[i & 1];
+      st = launch_spmm_mode<gm::kLast>(s, a);
+    }
+    GM_HIP(hipEventRecord(e1, s->stream));
+    GM_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (i > 0) total += ms;
+  }
+  *avg_ms = total / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return st;
+}
+
 gm_status gm_profile_copy(gm_solver* s, int64_t bytes, int32_t reps, double* avg_ms) {
   GM_CHECK_HANDLE(s);
   if (!avg_ms || reps < 1 || bytes < 16) return fail(GM_EINVAL, "bad profile_copy arguments");

@@ -7,10 +7,13 @@
 // The matrices are concatenated into one block-diagonal CSR and the vectors
 // into one N = sum n_p row range, laid out exactly as the plain path lays out
 // one problem (double2 rows [exp column, phi column], V[k][i]).  Rows are cut
-// into blocks of at most kBlock rows and 1024 nonzeros that never straddle two
+// into blocks of at most kBlock rows and TILE nonzeros that never straddle two
 // problems, so every kernel of a step covers all problems in one launch:
-//   * SpMM: k_spmm_stream<1024, MODE, SEG> (each block reads its problem's
-//     alpha/beta, writes its partial at its row-block index);
+//   * SpMM: k_spmm_stream<TILE, MODE, SEG> (each block reads its problem's
+//     alpha/beta, writes its partial at its row-block index); TILE (512, 1024
+//     or 2048), 16-bit column offsets and early epilogue loads are picked per
+//     batch by timing whole steps at create (GM_FLAG_AUTOTUNE), else 1024
+//     nonzeros, 16-bit offsets where they fit, early loads;
 //   * reductions: k_reduce in segmented form, one block per problem;
 //   * finish, V*C, update: row-block kernels below (one thread per row);
 //   * shifted solves and collapse: the plain kernels with blockIdx.y = problem.
@@ -34,7 +37,7 @@
 namespace gmu {
 using namespace gm;
 
-constexpr int kTile = 1024;   // nonzeros per row block
+constexpr int kTile = 1024;   // nonzeros per row block without the autotuner
 
 // w <- w - alpha_k v_k with the problem's alpha; partial ||w||^2 per block.
 // build_subspace.cu:63-76 (per problem).
@@ -189,6 +192,8 @@ struct gm_ubatch {
   int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr, *blk_prob = nullptr,
       *prob_blk = nullptr;
   short* col16 = nullptr;        // 16-bit column offsets from each block's first row
+  int tile = gmu::kTile;         // nonzeros per row block (SpMM LDS tile)
+  bool use16 = false, pre = true;  // SpMM: 16-bit offsets; epilogue loads before gathers
   double *val = nullptr, *mass = nullptr, *aabb = nullptr, *wts1 = nullptr, *wts2 = nullptr,
          *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *yr = nullptr, *yi = nullptr,
          *partials = nullptr;
@@ -251,14 +256,30 @@ gm_status ureduce(gm_ubatch* u, double* out, int stride, int do_sqrt, int flag_z
   return GM_OK;
 }
 
+template <int TILE, int MODE, bool C16, bool PRE>
+void uspmm_launch(gm_ubatch* u, const gm::SpmmArgs& a) {
+  hipLaunchKernelGGL((gm::k_spmm_stream<TILE, MODE, true, C16, PRE>), dim3(u->G),
+                     dim3(gm::kBlock), 0, u->stream, a);
+}
+
+// The schedules the autotuner chooses from: every tile with 16-bit offsets or
+// not and early epilogue loads, and the 1024-nonzero tile with late loads.
 template <int MODE>
 gm_status uspmm(gm_ubatch* u, const gm::SpmmArgs& a) {
-  if (a.col16)
-    hipLaunchKernelGGL((gm::k_spmm_stream<gmu::kTile, MODE, true, true>), dim3(u->G),
-                       dim3(gm::kBlock), 0, u->stream, a);
-  else
-    hipLaunchKernelGGL((gm::k_spmm_stream<gmu::kTile, MODE, true, false>), dim3(u->G),
-                       dim3(gm::kBlock), 0, u->stream, a);
+  const bool c16 = a.col16 != nullptr;
+  if (u->tile == 512) {
+    if (c16) uspmm_launch<512, MODE, true, true>(u, a);
+    else uspmm_launch<512, MODE, false, true>(u, a);
+  } else if (u->tile == 2048) {
+    if (c16) uspmm_launch<2048, MODE, true, true>(u, a);
+    else uspmm_launch<2048, MODE, false, true>(u, a);
+  } else if (u->pre) {
+    if (c16) uspmm_launch<1024, MODE, true, true>(u, a);
+    else uspmm_launch<1024, MODE, false, true>(u, a);
+  } else {
+    if (c16) uspmm_launch<1024, MODE, true, false>(u, a);
+    else uspmm_launch<1024, MODE, false, false>(u, a);
+  }
   U_HIP(hipGetLastError());
   return GM_OK;
 }
@@ -269,7 +290,7 @@ gm::SpmmArgs uspmm_base(gm_ubatch* u) {
   a.col = u->col;
   a.val = u->val;
   a.rowblk = u->rowblk;
-  a.col16 = u->col16;
+  a.col16 = u->use16 ? u->col16 : nullptr;
   a.blk_prob = u->blk_prob;
   a.aabb_stride = (int)aabb_stride(u);
   a.nrows = u->N;
@@ -422,18 +443,114 @@ gm_status enqueue_ustep(gm_ubatch* u) {
 
 // Rows of problem q cut into blocks of <= kBlock rows and <= kTile nonzeros
 // (a longer row alone), appended to blk with global row numbers.
-void problem_blocks(const std::vector<int>& rp, int r0, int r1, std::vector<int>& blk) {
+void problem_blocks(const std::vector<int>& rp, int r0, int r1, int tile, std::vector<int>& blk) {
   int r = r0;
   while (r < r1) {
     const int start = r;
     const long long base = rp[r];
-    if (rp[r + 1] - base > gmu::kTile) {
+    if (rp[r + 1] - base > tile) {
       ++r;
     } else {
-      while (r < r1 && r - start < gm::kBlock && rp[r + 1] - base <= gmu::kTile) ++r;
+      while (r < r1 && r - start < gm::kBlock && rp[r + 1] - base <= tile) ++r;
     }
     blk.push_back(r);
   }
+}
+
+// Cuts the concatenated rows into row blocks of at most `tile` nonzeros that never
+// straddle two problems and uploads them (row-block starts, each block's problem,
+// each problem's block range, and the 16-bit column offsets where they all fit).
+gm_status set_blocks(gm_ubatch* u, const std::vector<int>& rp, const std::vector<int>& ci,
+                     int tile) {
+  std::vector<int> blk{0}, bprob, pblk{0};
+  for (int q = 0; q < u->P; ++q) {
+    const size_t before = blk.size();
+    problem_blocks(rp, u->off[q], u->off[q + 1], tile, blk);
+    for (size_t t = before; t < blk.size(); ++t) bprob.push_back(q);
+    pblk.push_back((int)blk.size() - 1);
+  }
+  for (void* p : {(void*)u->rowblk, (void*)u->blk_prob, (void*)u->prob_blk, (void*)u->col16})
+    if (p) U_HIP(hipFree(p));
+  u->rowblk = u->blk_prob = u->prob_blk = nullptr;
+  u->col16 = nullptr;
+  u->G = (int)bprob.size();
+  u->tile = tile;
+  hipStream_t st = u->stream;
+  U_TRY(uupload(&u->rowblk, blk.data(), blk.size(), st));
+  U_TRY(uupload(&u->blk_prob, bprob.data(), bprob.size(), st));
+  U_TRY(uupload(&u->prob_blk, pblk.data(), pblk.size(), st));
+  std::vector<short> c16(ci.size());
+  bool fits = true;
+  for (int b = 0; b < u->G && fits; ++b)
+    for (int j = rp[blk[b]]; j < rp[blk[b + 1]]; ++j) {
+      const int off = ci[j] - blk[b];
+      if (off < -32768 || off > 32767) {
+        fits = false;
+        break;
+      }
+      c16[j] = (short)off;
+    }
+  if (fits) U_TRY(uupload(&u->col16, c16.data(), c16.size(), st));
+  U_HIP(hipStreamSynchronize(st));   // the host vectors go out of scope
+  return GM_OK;
+}
+
+
+// Times two whole steps (after one untimed) under every candidate schedule and
+// keeps the fastest; the state the steps advance is restored afterwards.
+gm_status autotune(gm_ubatch* u, const std::vector<int>& rp, const std::vector<int>& ci) {
+  const size_t N = (size_t)u->N;
+  double2* keep = nullptr;
+  U_TRY(ualloc(&keep, N));
+  struct Cand { int tile; bool c16, pre; };
+  std::vector<Cand> cands;
+  for (int tile : {512, 1024, 2048})
+    for (int c16 = 1; c16 >= 0; --c16) cands.push_back({tile, c16 != 0, true});
+  for (int c16 = 1; c16 >= 0; --c16) cands.push_back({1024, c16 != 0, false});
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  gm_status st = GM_OK;
+  float best = 1e30f;
+  Cand pick{gmu::kTile, u->use16, true};
+  if (hipMemcpyAsync(keep, u->Usol, N * sizeof(double2), hipMemcpyDeviceToDevice, u->stream) !=
+          hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    st = ufail(GM_EDEVICE, "autotune set-up");
+  for (const Cand& c : cands) {
+    if (st != GM_OK) break;
+    if (c.tile != u->tile) st = set_blocks(u, rp, ci, c.tile);
+    if (st != GM_OK) break;
+    if (c.c16 && !u->col16) continue;   // offsets do not fit this cut
+    u->use16 = c.c16;
+    u->pre = c.pre;
+    st = enqueue_ustep(u);
+    if (st == GM_OK && hipEventRecord(e0, u->stream) != hipSuccess) st = ufail(GM_EDEVICE, "event");
+    for (int i = 0; i < 2 && st == GM_OK; ++i) st = enqueue_ustep(u);
+    if (st == GM_OK && (hipEventRecord(e1, u->stream) != hipSuccess ||
+                        hipEventSynchronize(e1) != hipSuccess))
+      st = ufail(GM_EDEVICE, "event");
+    float ms = 0.f;
+    if (st == GM_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = ufail(GM_EDEVICE, "event");
+    if (st == GM_OK && ms < best) {
+      best = ms;
+      pick = c;
+    }
+  }
+  if (st == GM_OK && pick.tile != u->tile) st = set_blocks(u, rp, ci, pick.tile);
+  if (st == GM_OK) {
+    u->use16 = pick.c16 && u->col16 != nullptr;
+    u->pre = pick.pre;
+    if (hipMemcpyAsync(u->Usol, keep, N * sizeof(double2), hipMemcpyDeviceToDevice, u->stream) !=
+        hipSuccess)
+      st = ufail(GM_EDEVICE, "restore state");
+  }
+  if (st == GM_OK) st = enqueue_update(u, 1);   // U, beta_0 and X from the restored state
+  if (st == GM_OK && hipMemsetAsync(u->status, 0, sizeof(unsigned), u->stream) != hipSuccess)
+    st = ufail(GM_EDEVICE, "status");
+  if (st == GM_OK && hipStreamSynchronize(u->stream) != hipSuccess) st = ufail(GM_EDEVICE, "sync");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(keep);
+  return st;
 }
 
 gm_status create_impl(const gm_ubatch_desc* d, gm_ubatch* u) {
@@ -458,7 +575,7 @@ gm_status create_impl(const gm_ubatch_desc* d, gm_ubatch* u) {
   u->nnz = nnz;
   const size_t N = (size_t)u->N;
   // block-diagonal CSR, row blocks that never straddle two problems
-  std::vector<int> rp(N + 1), ci((size_t)nnz), blk{0}, bprob, pblk{0};
+  std::vector<int> rp(N + 1), ci((size_t)nnz);
   std::vector<double> vv((size_t)nnz), mass(N, 1.0);
   std::vector<double2> u2(N);
   long long e = 0;
@@ -475,30 +592,18 @@ gm_status create_impl(const gm_ubatch_desc* d, gm_ubatch* u) {
     }
     e += d->nnz[q];
     rp[o + n] = (int)e;
-    const size_t before = blk.size();
-    problem_blocks(rp, o, o + n, blk);
-    for (size_t t = before; t < blk.size(); ++t) bprob.push_back(q);
-    pblk.push_back((int)blk.size() - 1);
   }
-  u->G = (int)bprob.size();
   U_TRY(uupload(&u->rowptr, rp.data(), N + 1, st));
   U_TRY(uupload(&u->col, ci.data(), (size_t)nnz, st));
   U_TRY(uupload(&u->val, vv.data(), (size_t)nnz, st));
-  U_TRY(uupload(&u->rowblk, blk.data(), blk.size(), st));
-  std::vector<short> c16((size_t)nnz);
-  bool fits = true;
-  for (int b = 0; b < u->G && fits; ++b)
-    for (int j = rp[blk[b]]; j < rp[blk[b + 1]]; ++j) {
-      const int off = ci[j] - blk[b];
-      if (off < -32768 || off > 32767) {
-        fits = false;
-        break;
-      }
-      c16[j] = (short)off;
-    }
-  if (fits) U_TRY(uupload(&u->col16, c16.data(), c16.size(), st));
-  U_TRY(uupload(&u->blk_prob, bprob.data(), bprob.size(), st));
-  U_TRY(uupload(&u->prob_blk, pblk.data(), pblk.size(), st));
+  // the most row blocks any candidate tile cuts (partials are per row block)
+  int maxG = 0;
+  for (int tile : {512, 1024, 2048}) {
+    U_TRY(set_blocks(u, rp, ci, tile));
+    maxG = std::max(maxG, u->G);
+  }
+  U_TRY(set_blocks(u, rp, ci, gmu::kTile));
+  u->use16 = u->col16 != nullptr;
   U_TRY(uupload(&u->mass, mass.data(), N, st));
   U_TRY(uupload(&u->Usol, u2.data(), N, st));
   const size_t P = (size_t)u->P, m = (size_t)u->m, ns = (size_t)u->ns, nd = (size_t)u->nd;
@@ -528,11 +633,12 @@ gm_status create_impl(const gm_ubatch_desc* d, gm_ubatch* u) {
   U_TRY(ualloc(&u->wDU, P * 2 * m * ns));
   U_TRY(ualloc(&u->wDU2, P * 2 * m * ns));
   U_TRY(ualloc(&u->wX, P * 2 * m * ns));
-  U_TRY(ualloc(&u->partials, 2 * (size_t)u->G));
+  U_TRY(ualloc(&u->partials, 2 * (size_t)maxG));
   U_TRY(ualloc(&u->status, 1));
   U_HIP(hipMemsetAsync(u->status, 0, sizeof(unsigned), st));
   U_TRY(enqueue_update(u, 1));   // U = Usol and beta_0 (lanczos_impl.cu:79-91, K = 0)
   U_HIP(hipStreamSynchronize(st));
+  if (d->flags & GM_FLAG_AUTOTUNE) U_TRY(autotune(u, rp, ci));
   return GM_OK;
 }
 
@@ -674,6 +780,15 @@ gm_status gm_ubatch_info(gm_ubatch* u, int32_t* num_problems, int64_t* total_row
   if (num_problems) *num_problems = u->P;
   if (total_rows) *total_rows = u->N;
   if (row_blocks) *row_blocks = u->G;
+  return GM_OK;
+}
+
+gm_status gm_ubatch_get_schedule(gm_ubatch* u, int32_t* tile, int32_t* index_bytes,
+                                 int32_t* early_loads) {
+  if (!ulive(u)) return ufail(GM_EBADHANDLE, "invalid uneven-batch handle");
+  if (tile) *tile = u->tile;
+  if (index_bytes) *index_bytes = u->use16 ? 2 : 4;
+  if (early_loads) *early_loads = u->pre ? 1 : 0;
   return GM_OK;
 }
 

@@ -295,6 +295,14 @@ class LanczosEngine:
         self._call(self._lib.gm_profile_spmm, int(mode), int(reps), C.byref(out))
         return out.value
 
+    def profile_spmm_flushed(self, mode=0, reps=20, flush_bytes=1 << 30):
+        """As profile_spmm, with a streaming copy of ``flush_bytes`` before every launch
+        (cold L2 and Infinity Cache); each launch timed alone, latency included."""
+        out = C.c_double()
+        self._call(self._lib.gm_profile_spmm_flushed, int(mode), int(reps), int(flush_bytes),
+                   C.byref(out))
+        return out.value
+
     def profile_copy(self, nbytes, reps=20):
         """Average duration (ms) of a plain device copy of ``nbytes`` bytes."""
         out = C.c_double()

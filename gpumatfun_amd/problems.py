@@ -70,6 +70,23 @@ def laplacian_3d_dirichlet(n, scaled=True):
     return _csr(A)
 
 
+def pencil_order(n, tile=16):
+    """Node order of an n^3 grid in pencils: the (i, j) plane is cut into tile x tile
+    squares, each square's column of n planes is one contiguous range (k outer,
+    (i, j) inner), so a node's k +- 1 neighbours are tile^2 positions away instead
+    of n^2.  Returns old_of_new: new position q holds natural node i + n j + n^2 k.
+    ``tile`` must divide n."""
+    if n % tile:
+        raise ValueError("tile must divide n")
+    t = n // tile
+    q = np.arange(n ** 3)
+    tid, rem = np.divmod(q, tile * tile * n)
+    k, w = np.divmod(rem, tile * tile)
+    jw, iw = np.divmod(w, tile)
+    tj, ti = np.divmod(tid, t)
+    return (ti * tile + iw) + n * (tj * tile + jw) + n * n * k
+
+
 def gaussian_random_field(n, corr_len=0.1, seed=1):
     """Zero-mean, unit-variance stationary GRF on the n x n interior grid of the unit square.
 

@@ -210,6 +210,13 @@ gm_status gm_get_stream(gm_solver* s, void** stream);
  * copy kernel over `bytes` bytes (read once, written once), the reference
  * point for the HBM counters.                                               */
 gm_status gm_profile_spmm(gm_solver* s, int32_t mode, int32_t reps, double* avg_ms);
+/* As gm_profile_spmm, but every timed launch follows a streaming copy of
+ * flush_bytes (half read, half written) and is timed alone between its own two
+ * events, so L2 and the Infinity Cache hold nothing of the SpMM's working set
+ * when it starts (flush_bytes well above 256 MB); the average includes one
+ * launch latency per call.                                                  */
+gm_status gm_profile_spmm_flushed(gm_solver* s, int32_t mode, int32_t reps, int64_t flush_bytes,
+                                  double* avg_ms);
 gm_status gm_profile_copy(gm_solver* s, int64_t bytes, int32_t reps, double* avg_ms);
 
 /* ---------------------------------------------------------------------------
@@ -401,7 +408,12 @@ typedef struct gm_ubatch_desc {
     const double* wts2;
     const double* shifts;               /* P blocks of 4 m n_s                */
     double dt;
-    uint32_t flags;                     /* GM_FLAG_NO_CLAMP                   */
+    uint32_t flags;                     /* GM_FLAG_NO_CLAMP; GM_FLAG_AUTOTUNE
+                                           times whole steps at create under each
+                                           SpMM schedule (512/1024/2048-nonzero
+                                           row blocks, 16- or 32-bit column
+                                           offsets, early or late epilogue
+                                           loads) and keeps the fastest       */
     int32_t device;
 } gm_ubatch_desc;
 
@@ -415,6 +427,10 @@ gm_status gm_ubatch_get_U(gm_ubatch* u, int32_t problem, double* dst);
 gm_status gm_ubatch_synchronize(gm_ubatch* u);
 gm_status gm_ubatch_info(gm_ubatch* u, int32_t* num_problems, int64_t* total_rows,
                          int32_t* row_blocks);
+/* The SpMM schedule in use: nonzeros per row block, bytes per stored column
+ * index (2 or 4), and whether the epilogue operands are loaded early.      */
+gm_status gm_ubatch_get_schedule(gm_ubatch* u, int32_t* tile, int32_t* index_bytes,
+                                 int32_t* early_loads);
 
 /* ---------------------------------------------------------------------------
  * Host set-up of the create inputs (host code only; usable without a GPU).

@@ -91,14 +91,19 @@ def test_uneven_batch_log_uniform_sides_vs_oracle():
     """The uneven workload's shape at a test size: 16 2D Dirichlet Laplacians with
     sides drawn log-uniformly in [32, 448] (seed 3: sides 40 to 399, 1.6k to 159k
     rows, several problems spanning hundreds of row blocks), m = 40, 32 shifts, no
-    preconditioner.  Every problem against the oracle's C restatement to 1e-10."""
+    preconditioner, SpMM schedule chosen by the create-time autotuner (whose
+    timing steps must leave the state as it found it).  Every problem against the
+    oracle's C restatement to 1e-10."""
     from oracle import cfab
     m, n, steps = 40, 32, 2
     sides = problems.uneven_batch_sides(16, 32, 448, seed=3)
     inps = [_dirichlet(int(s), 100 + i, m, n, 0, 1.0) for i, s in enumerate(sides)]
     print(f"case uneven batch 16 problems sides={sorted(int(x) for x in sides)}", flush=True)
-    ub = UnevenBatch.from_inputs(inps, clamp=False)
-    assert ub.info()["row_blocks"] > len(inps)
+    ub = UnevenBatch.from_inputs(inps, clamp=False, autotune=True)
+    info = ub.info()
+    print(f"autotuned schedule: {info['spmm_tile']}-nonzero row blocks, "
+          f"{info['index_bytes']}-byte indices, early loads {info['early_loads']}", flush=True)
+    assert info["row_blocks"] > len(inps) and info["spmm_tile"] in (512, 1024, 2048)
     ub.step(steps)
     ub.synchronize()
     for i, inp in enumerate(inps):
