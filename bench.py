@@ -24,6 +24,9 @@ Workloads (``--config``):
   (Chebyshev-filtered subspace iteration, block 32, degree 200, relative
   residual 1e-10), timed beside scipy's ARPACK shift-invert on the host.
 
+* ``bfab512`` -- the multi-column Krylov engine alone: forward passes of 32
+  columns on the ``ms512`` operator (m = 40, no deflation).
+
 * ``msadj512`` -- the gradient of the ``ms512`` run's activation-map mismatch
   with respect to the conductivity ratio, through 1000 deflated time steps of
   its 16 stimuli (the adjoint of the monodomain time stepping, deflation reused).
@@ -678,6 +681,46 @@ def run_msadj512(args, steps):
     print(json.dumps(result), flush=True)
 
 
+def run_bfab512(args, passes):
+    """The multi-column Krylov engine alone (gm_bfab_forward): u = f(A) b for R = 32
+    columns on the 512^2 anisotropic 9-point operator (ratio 9, fibre 0 -> pi/3),
+    m = 40, 32 contour nodes on the operator's spectrum, no deflation; `passes`
+    forward passes on one seeded 262,144 x 32 block.  The workload of the
+    32-column SpMM's counter runs."""
+    import scipy.sparse.linalg as sla
+    import torch
+    from gpumatfun_amd import problems, quadrature as q
+    from gpumatfun_amd.adjoint import BatchFabEngine
+    t0 = time.time()
+    n, R, m, ns, tau, alpha = 512, 32, 40, 32, 0.05e-3, 1.7
+    A = problems.anisotropic_9pt(n, ratio=9.0)
+    lo = float(sla.eigsh(A, k=1, sigma=0, which="LM", return_eigenvectors=False)[0])
+    hi = float(sla.eigsh(A, k=1, which="LA", return_eigenvectors=False)[0])
+    c = q.contour_nodes(0.98 * lo, 1.02 * hi, ns, tau, alpha)
+    eng = BatchFabEngine(A, R=R, m=m, xi=c["xi"], wts1=c["wts1"], wts2=c["wts2"])
+    B = torch.as_tensor(np.random.default_rng(0).standard_normal((n * n, R)), device="cuda")
+    setup_s = time.time() - t0
+    eng.forward(B)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(passes):
+        U = eng.forward(B)
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t1) / passes
+    eng.delete()
+    result = {
+        "metric": "multi-column f(A)b forward passes/sec (FP64, 32 columns)",
+        "value": round(1.0 / per, 3), "unit": "passes/s", "n_gpus": 1, "steps": passes,
+        "warmup": 1, "ms_per_step": round(1e3 * per, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded block of right-hand sides)",
+        "config": {"workload": "bfab_aniso512_32col", "n": n * n, "nnz": int(A.nnz), "columns": R,
+                   "m": m, "shifts": ns, "deflation": 0},
+        "setup_s": round(setup_s, 1), "roofline": None, "cpu_baseline": None,
+        "output_norm": float(torch.linalg.norm(U))}
+    print(json.dumps(result), flush=True)
+
+
 def oracle_setup_dict(inp, A=None, perm=None):
     S = {"A": inp.A if A is None else A, "Q": inp.Q if inp.Q.shape[1] else None,
          "exp_lambda": inp.exp_lambda, "phi_lambda": inp.phi_lambda, "dt": inp.dt,
@@ -835,7 +878,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--config", default="fem1m",
                     choices=["fem1m", "dirichlet128", "batch3d", "uneven64", "adjoint256",
-                             "ms512", "eigs512", "msadj512"])
+                             "ms512", "eigs512", "msadj512", "bfab512"])
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--side3d", type=int, default=128)
     ap.add_argument("--problems", type=int, default=256)
@@ -906,6 +949,9 @@ def main():
         return
     if args.config == "msadj512":
         run_msadj512(args, args.steps or 1000)
+        return
+    if args.config == "bfab512":
+        run_bfab512(args, args.steps or 6)
         return
     if args.config == "uneven64":
         if args.problems == 256:
