@@ -721,6 +721,28 @@ def run_bfab512(args, passes):
     print(json.dumps(result), flush=True)
 
 
+def run_barrier(args):
+    """The cost of one cooperative grid barrier against a kernel boundary, on 64, 128
+    and 256 resident workgroups (gm_profile_barrier): what a persistent Lanczos
+    kernel would pay per phase in place of the launches of a captured step."""
+    import ctypes as C
+    import torch
+    from gpumatfun_amd import _lib
+    torch.cuda.init()
+    lib = _lib.load()
+    rows = []
+    for blocks in (64, 128, 256):
+        g, k = C.c_double(), C.c_double()
+        _lib.check(lib.gm_profile_barrier(0, blocks, 1000, C.byref(g), C.byref(k)))
+        rows.append({"blocks": blocks, "grid_sync_us": round(g.value, 3),
+                     "kernel_boundary_us": round(k.value, 3)})
+        log(f"{blocks} workgroups: grid barrier {g.value:.2f} us, kernel boundary {k.value:.2f} us")
+    print(json.dumps({"metric": "grid barrier vs kernel boundary (us)", "value": rows[0]["grid_sync_us"],
+                      "unit": "us", "n_gpus": 1, "higher_is_better": False,
+                      "config": {"workload": "cooperative_grid_barrier", "iterations": 1000},
+                      "rows": rows}), flush=True)
+
+
 def oracle_setup_dict(inp, A=None, perm=None):
     S = {"A": inp.A if A is None else A, "Q": inp.Q if inp.Q.shape[1] else None,
          "exp_lambda": inp.exp_lambda, "phi_lambda": inp.phi_lambda, "dt": inp.dt,
@@ -878,7 +900,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--config", default="fem1m",
                     choices=["fem1m", "dirichlet128", "batch3d", "uneven64", "adjoint256",
-                             "ms512", "eigs512", "msadj512", "bfab512"])
+                             "ms512", "eigs512", "msadj512", "bfab512", "barrier"])
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--side3d", type=int, default=128)
     ap.add_argument("--problems", type=int, default=256)
@@ -949,6 +971,9 @@ def main():
         return
     if args.config == "msadj512":
         run_msadj512(args, args.steps or 1000)
+        return
+    if args.config == "barrier":
+        run_barrier(args)
         return
     if args.config == "bfab512":
         run_bfab512(args, args.steps or 6)

@@ -34,7 +34,7 @@ EXPORTS = (
     "gm_get_U", "gm_get_xsol", "gm_get_aabb", "gm_set_y", "gm_get_y", "gm_get_state",
     "gm_set_state", "gm_step", "gm_synchronize", "gm_get_info", "gm_get_schedule",
     "gm_get_stream",
-    "gm_profile_spmm", "gm_profile_spmm_flushed", "gm_profile_copy", "gm_set_ms_params", "gm_update_U_ms",
+    "gm_profile_spmm", "gm_profile_spmm_flushed", "gm_profile_copy", "gm_profile_barrier", "gm_set_ms_params", "gm_update_U_ms",
     "gm_batch_last_error", "gm_batch_create", "gm_batch_delete", "gm_batch_step",
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
     "gm_batch_get_ode",
@@ -278,6 +278,7 @@ def load():
                                       C.POINTER(C.c_int32)]),
         "gm_profile_spmm": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "gm_profile_spmm_flushed": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int64, _dp]),
+        "gm_profile_barrier": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _dp, _dp]),
         "gm_profile_copy": (C.c_int, [vp, C.c_int64, C.c_int32, _dp]),
         "gm_set_ms_params": (C.c_int, [vp, C.POINTER(GmMsParams)]),
         "gm_update_U_ms": (C.c_int, [vp]),

@@ -355,7 +355,10 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_finish(FinishArgs p) {
 // nicksmethodcpu.m:19-20).  System s < n_s uses column 0 (exp), s >= n_s
 // column 1 (phi); (T - shift_s I) y = -beta_0 w_s e1 as in
 // lin_sys_solves.cu:5-21.  Workspaces are [row][system] so a wavefront's
-// accesses are contiguous.
+// accesses are contiguous.  LDS = 1: a block solves p.sb systems (lanes
+// 0..sb-1) and keeps the elimination's factors in LDS (4 m sb double2, sized
+// by the launch) instead of the global workspaces, so the back substitution
+// reads them at LDS latency; the arithmetic is the same either way.
 // ---------------------------------------------------------------------------
 
 struct SolveArgs {
@@ -374,13 +377,25 @@ struct SolveArgs {
   // independent problems (blockIdx.y = problem): element strides between
   // consecutive problems' aabb, wts1/wts2, shifts, yr/yi and workspaces
   size_t aabb_ps, wts_ps, shifts_ps, y_ps, work_ps;
+  int sb;               // LDS form: systems per block
 };
 
-template <int kUnused = 0>
+// Systems per block for the LDS form of k_shifted_solves (factors 64 m bytes a
+// system, within kSolveLds bytes), or 0 when one system's factors do not fit.
+constexpr int kSolveLds = 48 * 1024;
+inline int solve_systems_per_block(int m) {
+  const int sb = kSolveLds / (64 * (m > 0 ? m : 1));
+  return sb < 1 ? 0 : (sb > 64 ? 64 : sb);
+}
+
+template <int LDS>
 __global__ __launch_bounds__(64) void k_shifted_solves(SolveArgs p) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  extern __shared__ double2 sfac[];
+  constexpr bool lds = LDS != 0;
+  const int t = threadIdx.x;
+  const int s = lds ? blockIdx.x * p.sb + t : blockIdx.x * blockDim.x + t;
   const int S = 2 * p.ns;
-  if (s >= S) return;
+  if (s >= S || (lds && t >= p.sb)) return;
   const size_t q = blockIdx.y;
   p.aabb += q * p.aabb_ps;
   p.wts1 += q * p.wts_ps;
@@ -392,6 +407,13 @@ __global__ __launch_bounds__(64) void k_shifted_solves(SolveArgs p) {
   p.wDU += q * p.work_ps;
   p.wDU2 += q * p.work_ps;
   p.wX += q * p.work_ps;
+  // factor k of this system: LDS [f][k][t] (f = D, DU, DU2, X), or global [k][s]
+  const size_t fs = lds ? (size_t)p.m * p.sb : 0;
+  double2* const fD = lds ? sfac : p.wD;
+  double2* const fDU = lds ? sfac + fs : p.wDU;
+  double2* const fDU2 = lds ? sfac + 2 * fs : p.wDU2;
+  double2* const fX = lds ? sfac + 3 * fs : p.wX;
+  auto fo = [&](int k) { return lds ? (size_t)k * p.sb + t : (size_t)k * S + s; };
   const int c = s < p.ns ? 0 : 1;
   const int node = s - c * p.ns;
   const double* w = c == 0 ? p.wts1 : p.wts2;
@@ -449,8 +471,8 @@ __global__ __launch_bounds__(64) void k_shifted_solves(SolveArgs p) {
         sX = next_x;
         next_x = csub(cur_x, cmul(mult, next_x));
       }
-      const size_t o = (size_t)k * S + s;
-      p.wD[o] = sD; p.wDU[o] = sDU; p.wDU2[o] = sDU2; p.wX[o] = sX;
+      const size_t o = fo(k);
+      fD[o] = sD; fDU[o] = sDU; fDU2[o] = sDU2; fX[o] = sX;
       cur_d = next_d; cur_x = next_x; cur_du = next_du;
     }
   }
@@ -465,26 +487,26 @@ __global__ __launch_bounds__(64) void k_shifted_solves(SolveArgs p) {
   p.yr[(m - 1) + (size_t)m * s] = x2.x;
   p.yi[(m - 1) + (size_t)m * s] = x2.y;
   if (m > 1) {
-    const size_t o = (size_t)(m - 2) * S + s;
-    double2 x1 = cdiv(csub(p.wX[o], cmul(p.wDU[o], x2)), p.wD[o]);
+    const size_t o = fo(m - 2);
+    double2 x1 = cdiv(csub(fX[o], cmul(fDU[o], x2)), fD[o]);
     p.yr[(m - 2) + (size_t)m * s] = x1.x;
     p.yi[(m - 2) + (size_t)m * s] = x1.y;
     for (int k1 = m - 3; k1 >= 0; k1 -= U) {
-      double2 fX[U], fDU[U], fDU2[U], fD[U];   // step k = k1 - u
+      double2 bX[U], bDU[U], bDU2[U], bD[U];   // step k = k1 - u
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (k1 - u >= 0) {
-          const size_t q = (size_t)(k1 - u) * S + s;
-          fX[u] = p.wX[q]; fDU[u] = p.wDU[q]; fDU2[u] = p.wDU2[q]; fD[u] = p.wD[q];
+          const size_t o2 = fo(k1 - u);
+          bX[u] = fX[o2]; bDU[u] = fDU[o2]; bDU2[u] = fDU2[o2]; bD[u] = fD[o2];
         }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = k1 - u;
         if (k < 0) break;
-        double2 t = csub(fX[u], cmul(fDU[u], x1));
-        t = csub(t, cmul(fDU2[u], x2));
-        const double2 xk = cdiv(t, fD[u]);
+        double2 tt = csub(bX[u], cmul(bDU[u], x1));
+        tt = csub(tt, cmul(bDU2[u], x2));
+        const double2 xk = cdiv(tt, bD[u]);
         p.yr[k + (size_t)m * s] = xk.x;
         p.yi[k + (size_t)m * s] = xk.y;
         x2 = x1;
@@ -606,7 +628,12 @@ struct UpdateArgs {
   unsigned* status;
 };
 
-template <int MODE>
+// KT deflation vectors per register tile of pass 2: kDeflTile, or kDeflTileWide
+// when K fits in one wide tile (one pass over the rows and Q instead of
+// ceil(K / 8)); each Z partial is summed in the same order either way.
+constexpr int kDeflTileWide = 24;
+
+template <int MODE, int KT = kDeflTile>
 __global__ __launch_bounds__(kBlock) void k_update1(UpdateArgs p) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int row0 = b * p.chunk;
@@ -653,18 +680,18 @@ __global__ __launch_bounds__(kBlock) void k_update1(UpdateArgs p) {
     block_partial2(nrm, p.partials);   // beta_0 = sqrt(sum), formed by k_reduce
     return;
   }
-  // pass 2: block partials of Z = Q^T Usol, kDeflTile vectors at a time.
-  __shared__ double red[kWaves][2 * kDeflTile];
+  // pass 2: block partials of Z = Q^T Usol, KT vectors at a time.
+  __shared__ double red[kWaves][2 * KT];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int k0 = 0; k0 < p.K; k0 += kDeflTile) {
-    double acc[2 * kDeflTile];
+  for (int k0 = 0; k0 < p.K; k0 += KT) {
+    double acc[2 * KT];
 #pragma unroll
-    for (int t = 0; t < 2 * kDeflTile; ++t) acc[t] = 0.0;
+    for (int t = 0; t < 2 * KT; ++t) acc[t] = 0.0;
     for (int r = row0 + threadIdx.x; r < row1; r += kBlock) {
       const double2 us = p.Usol[r];
       const double* q = p.Q + (size_t)r * p.K;
 #pragma unroll
-      for (int t = 0; t < kDeflTile; ++t) {
+      for (int t = 0; t < KT; ++t) {
         if (k0 + t < p.K) {
           const double qv = q[k0 + t];
           acc[2 * t] = fma(qv, us.x, acc[2 * t]);
@@ -673,14 +700,14 @@ __global__ __launch_bounds__(kBlock) void k_update1(UpdateArgs p) {
       }
     }
 #pragma unroll
-    for (int t = 0; t < 2 * kDeflTile; ++t) acc[t] = wave_sum(acc[t]);
+    for (int t = 0; t < 2 * KT; ++t) acc[t] = wave_sum(acc[t]);
     __syncthreads();
     if (lane == 0) {
 #pragma unroll
-      for (int t = 0; t < 2 * kDeflTile; ++t) red[wv][t] = acc[t];
+      for (int t = 0; t < 2 * KT; ++t) red[wv][t] = acc[t];
     }
     __syncthreads();
-    if (threadIdx.x < 2 * kDeflTile && k0 * 2 + (int)threadIdx.x < 2 * p.K) {
+    if (threadIdx.x < 2 * KT && k0 * 2 + (int)threadIdx.x < 2 * p.K) {
       double s = 0.0;
 #pragma unroll
       for (int i = 0; i < kWaves; ++i) s += red[i][threadIdx.x];

@@ -5,6 +5,7 @@
 // points only enqueue kernels; nothing inside a step waits on the host, so a
 // whole step can be captured into a HIP graph (gm_step).
 #include <hip/hip_runtime.h>
+#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 #include <cmath>
@@ -95,6 +96,8 @@ struct gm_solver {
   double2* Z = nullptr;
   double2* C = nullptr;
   double *Q = nullptr, *el = nullptr, *pl = nullptr, *mass = nullptr, *ode = nullptr;
+  int solve_sb = 0;   // shifted solves: 0 = factors in global memory, > 0 = systems per
+                      // block with the factors in LDS (chosen by the autotuner)
   double *aabb = nullptr, *Yr = nullptr, *Yi = nullptr, *wts1 = nullptr, *wts2 = nullptr,
          *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *Zdefl = nullptr;
   double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
@@ -207,6 +210,22 @@ gm_status launch_reduce(gm_solver* s, int G, int nv, double* out, int stride, in
   hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, s->stream, r);
   GM_HIP(hipGetLastError());
   return GM_OK;
+}
+
+// Measurement kernels of gm_profile_barrier: `iters` cooperative grid barriers,
+// each after every thread stores one value; and the same store as a kernel of
+// its own (the kernel boundary the barrier would replace).
+__global__ __launch_bounds__(kBlock) void k_grid_barriers(int iters, double* buf) {
+  cooperative_groups::grid_group g = cooperative_groups::this_grid();
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  for (int it = 0; it < iters; ++it) {
+    buf[i] = (double)it;
+    g.sync();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_touch(double* buf) {
+  buf[(size_t)blockIdx.x * kBlock + threadIdx.x] += 1.0;
 }
 
 SpmmArgs spmm_base(gm_solver* s) {
@@ -358,7 +377,13 @@ gm_status enqueue_solves(gm_solver* s) {
   a.yr = s->Yr;
   a.yi = s->Yi;
   a.status = s->status;
-  hipLaunchKernelGGL(gm::k_shifted_solves<0>, dim3(cdiv_i(2 * s->ns, 64)), dim3(64), 0, s->stream, a);
+  a.sb = s->solve_sb;
+  if (a.sb > 0)
+    hipLaunchKernelGGL(gm::k_shifted_solves<1>, dim3(cdiv_i(2 * s->ns, a.sb)), dim3(64),
+                       4 * (size_t)s->m * a.sb * sizeof(double2), s->stream, a);
+  else
+    hipLaunchKernelGGL(gm::k_shifted_solves<0>, dim3(cdiv_i(2 * s->ns, 64)), dim3(64), 0,
+                       s->stream, a);
   GM_HIP(hipGetLastError());
   return GM_OK;
 }
@@ -453,20 +478,28 @@ gm_status enqueue_update(gm_solver* s, int mode) {
   a.status = s->status;
   const int G = s->K == 0 ? s->G_el : s->G_df;
   a.chunk = s->K == 0 ? s->chunk_el : s->chunk_df;
+  const bool wide = s->K > gm::kDeflTile && s->K <= gm::kDeflTileWide;
+#define GM_UPD1(M)                                                                        \
+  if (wide)                                                                               \
+    hipLaunchKernelGGL((gm::k_update1<M, gm::kDeflTileWide>), dim3(G), dim3(kBlock), 0,     \
+                       s->stream, a);                                                     \
+  else                                                                                    \
+    hipLaunchKernelGGL((gm::k_update1<M>), dim3(G), dim3(kBlock), 0, s->stream, a);
   switch (mode) {
     case gm::kUpdFisher:
-      hipLaunchKernelGGL(gm::k_update1<gm::kUpdFisher>, dim3(G), dim3(kBlock), 0, s->stream, a);
+      GM_UPD1(gm::kUpdFisher)
       break;
     case gm::kUpdMono:
-      hipLaunchKernelGGL(gm::k_update1<gm::kUpdMono>, dim3(G), dim3(kBlock), 0, s->stream, a);
+      GM_UPD1(gm::kUpdMono)
       break;
     case gm::kUpdMS:
-      hipLaunchKernelGGL(gm::k_update1<gm::kUpdMS>, dim3(G), dim3(kBlock), 0, s->stream, a);
+      GM_UPD1(gm::kUpdMS)
       break;
     default:
-      hipLaunchKernelGGL(gm::k_update1<gm::kUpdPrepare>, dim3(G), dim3(kBlock), 0, s->stream, a);
+      GM_UPD1(gm::kUpdPrepare)
       break;
   }
+#undef GM_UPD1
   GM_HIP(hipGetLastError());
   double* beta0 = s->aabb + 2 * (s->m + 1);
   if (s->K == 0) {
@@ -789,9 +822,35 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
       GM_HIP(hipEventElapsedTime(&ms, e0, e1));
       if (ms < best) { best = ms; best_c = l; }
     }
+    apply(best_c);
+    // The shifted solves with their factors in global memory or in LDS (a few
+    // systems a block), each timed together with apply_V: the LDS form spreads
+    // Y over several blocks (XCDs), which the collapse that reads Y pays for
+    // (m = 80, 64 nodes: LDS wins; m = 50, 32 nodes: global).  Timed on the
+    // last build's tridiagonals; apply_V's output X is cleared afterwards.
+    if (gm::solve_systems_per_block(s->m) > 0) {
+      float tbest = 1e30f;
+      int pick = 0;
+      for (int sb : {0, gm::solve_systems_per_block(s->m)}) {
+        s->solve_sb = sb;
+        GM_TRY(enqueue_solves(s));  // warm-up
+        GM_TRY(enqueue_apply(s));
+        GM_HIP(hipEventRecord(e0, st));
+        for (int i = 0; i < 3; ++i) {
+          GM_TRY(enqueue_solves(s));
+          GM_TRY(enqueue_apply(s));
+        }
+        GM_HIP(hipEventRecord(e1, st));
+        GM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < tbest) { tbest = ms; pick = sb; }
+      }
+      s->solve_sb = pick;
+      s->x_zero = true;   // X holds the timing runs' output: the next apply_V overwrites it
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    apply(best_c);
     GM_HIP(hipMemsetAsync(s->status, 0, sizeof(unsigned), st));
     GM_HIP(hipStreamSynchronize(st));
   }
@@ -1148,6 +1207,59 @@ gm_status gm_profile_spmm_flushed(gm_solver* s, int32_t mode, int32_t reps, int6
   (void)hipFree(src);
   (void)hipFree(dst);
   return st;
+}
+
+gm_status gm_profile_barrier(int32_t device, int32_t blocks, int32_t iters, double* grid_sync_us,
+                             double* launch_us) {
+  if (!grid_sync_us || !launch_us || blocks < 1 || iters < 1)
+    return fail(GM_EINVAL, "bad profile_barrier arguments");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return fail(GM_EDEVICE, "no HIP device %d", device);
+  GM_HIP(hipSetDevice(device));
+  int coop = 0, per_cu = 0, cus = 0;
+  GM_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
+  GM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  GM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_grid_barriers,
+                                                       kBlock, 0));
+  if (!coop) return fail(GM_EDEVICE, "device %d has no cooperative launch", device);
+  if (blocks > per_cu * cus)
+    return fail(GM_EINVAL, "%d blocks cannot all be resident (at most %d)", blocks, per_cu * cus);
+  hipStream_t st = nullptr;
+  GM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  double* buf = nullptr;
+  gm_status rc = dalloc(&buf, (size_t)blocks * kBlock);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == GM_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(GM_EDEVICE, "events");
+  auto timed = [&](bool coop_kernel, int n, float* ms) -> gm_status {
+    void* args[] = {&n, &buf};
+    if (hipEventRecord(e0, st) != hipSuccess) return fail(GM_EDEVICE, "event");
+    if (coop_kernel) {
+      const hipError_t e = hipLaunchCooperativeKernel((const void*)k_grid_barriers, dim3(blocks),
+                                                      dim3(kBlock), args, 0, st);
+      if (e != hipSuccess) return fail(GM_EDEVICE, "cooperative launch: %s", hipGetErrorString(e));
+    } else {
+      for (int i = 0; i < n; ++i)
+        hipLaunchKernelGGL(k_touch, dim3(blocks), dim3(kBlock), 0, st, buf);
+    }
+    if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+      return fail(GM_EDEVICE, "timing");
+    return GM_OK;
+  };
+  float ms = 0.f;
+  if (rc == GM_OK) rc = timed(true, 1, &ms);            // warm-up
+  if (rc == GM_OK) rc = timed(true, iters, &ms);
+  if (rc == GM_OK) *grid_sync_us = 1000.0 * ms / iters;
+  if (rc == GM_OK) rc = timed(false, 1, &ms);
+  if (rc == GM_OK) rc = timed(false, iters, &ms);
+  if (rc == GM_OK) *launch_us = 1000.0 * ms / iters;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (buf) (void)hipFree(buf);
+  (void)hipStreamDestroy(st);
+  return rc;
 }
 
 gm_status gm_profile_copy(gm_solver* s, int64_t bytes, int32_t reps, double* avg_ms) {

@@ -194,6 +194,7 @@ struct gm_ubatch {
   short* col16 = nullptr;        // 16-bit column offsets from each block's first row
   int tile = gmu::kTile;         // nonzeros per row block (SpMM LDS tile)
   bool use16 = false, pre = true;  // SpMM: 16-bit offsets; epilogue loads before gathers
+  int solve_sb = 0;              // shifted solves: factors in global memory (0) or LDS
   double *val = nullptr, *mass = nullptr, *aabb = nullptr, *wts1 = nullptr, *wts2 = nullptr,
          *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *yr = nullptr, *yi = nullptr,
          *partials = nullptr;
@@ -391,8 +392,13 @@ gm_status enqueue_ustep(gm_ubatch* u) {
   s.shifts_ps = 4 * (size_t)u->m * u->ns;
   s.y_ps = 2 * (size_t)u->m * u->ns;
   s.work_ps = 2 * (size_t)u->m * u->ns;
-  hipLaunchKernelGGL(gm::k_shifted_solves<0>, dim3((2 * u->ns + 63) / 64, u->P), dim3(64), 0,
-                     u->stream, s);
+  s.sb = u->solve_sb;
+  if (s.sb > 0)
+    hipLaunchKernelGGL(gm::k_shifted_solves<1>, dim3((2 * u->ns + s.sb - 1) / s.sb, u->P),
+                       dim3(64), 4 * (size_t)u->m * s.sb * sizeof(double2), u->stream, s);
+  else
+    hipLaunchKernelGGL(gm::k_shifted_solves<0>, dim3((2 * u->ns + 63) / 64, u->P), dim3(64), 0,
+                       u->stream, s);
   U_HIP(hipGetLastError());
   // collapse over the nodes, then V*C and the Horner SpMMs (apply_V)
   gm::CollapseArgs c{};
@@ -539,6 +545,30 @@ gm_status autotune(gm_ubatch* u, const std::vector<int>& rp, const std::vector<i
   if (st == GM_OK) {
     u->use16 = pick.c16 && u->col16 != nullptr;
     u->pre = pick.pre;
+  }
+  // then the shifted solves' two forms under the kept schedule
+  if (st == GM_OK && gm::solve_systems_per_block(u->m) > 0) {
+    float sbest = 1e30f;
+    int spick = 0;
+    for (int sb : {0, gm::solve_systems_per_block(u->m)}) {
+      u->solve_sb = sb;
+      st = enqueue_ustep(u);
+      if (st == GM_OK && hipEventRecord(e0, u->stream) != hipSuccess) st = ufail(GM_EDEVICE, "event");
+      for (int i = 0; i < 2 && st == GM_OK; ++i) st = enqueue_ustep(u);
+      float ms = 0.f;
+      if (st == GM_OK && (hipEventRecord(e1, u->stream) != hipSuccess ||
+                          hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = ufail(GM_EDEVICE, "event");
+      if (st != GM_OK) break;
+      if (ms < sbest) {
+        sbest = ms;
+        spick = sb;
+      }
+    }
+    u->solve_sb = spick;
+  }
+  if (st == GM_OK) {
     if (hipMemcpyAsync(u->Usol, keep, N * sizeof(double2), hipMemcpyDeviceToDevice, u->stream) !=
         hipSuccess)
       st = ufail(GM_EDEVICE, "restore state");

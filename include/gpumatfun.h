@@ -218,6 +218,12 @@ gm_status gm_profile_spmm(gm_solver* s, int32_t mode, int32_t reps, double* avg_
 gm_status gm_profile_spmm_flushed(gm_solver* s, int32_t mode, int32_t reps, int64_t flush_bytes,
                                   double* avg_ms);
 gm_status gm_profile_copy(gm_solver* s, int64_t bytes, int32_t reps, double* avg_ms);
+/* What a persistent kernel would pay instead of a kernel boundary: the average
+ * cost (us) of one cooperative grid barrier over `blocks` resident workgroups
+ * (each thread stores one value before it), and of launching the same store as
+ * back-to-back kernels of `blocks` workgroups.  No solver needed.           */
+gm_status gm_profile_barrier(int32_t device, int32_t blocks, int32_t iters, double* grid_sync_us,
+                             double* launch_us);
 
 /* ---------------------------------------------------------------------------
  * Batched form (no reference counterpart): P independent problems that share
