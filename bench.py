@@ -5,7 +5,8 @@ Workloads (``--config``):
 * ``fem1m`` (default) -- the plain method at full scale, FP64: P1 FEM
   stiffness K and lumped mass M on a seeded Delaunay triangulation of the unit
   disc with 1,000,000 nodes (uniform random interior points plus 4096
-  equispaced boundary points), A = M^{-1/2} K M^{-1/2} in CSR (symrcm order),
+  equispaced boundary points), A = M^{-1/2} K M^{-1/2} in CSR (reverse Cuthill-McKee
+  order from the library's gm_rcm_order, in place of symrcm),
   f1 = exp(-tau kappa A^{alpha/2}), f2 = phi(...), alpha = 1.7, tau = 1e-3,
   kappa = 1, degree-8 Jacobi polynomial preconditioner, Lanczos m = 80, 64
   contour shifts, 20 deflated eigenvectors.  Its line also carries the other
@@ -97,7 +98,7 @@ def build_fem1m(args):
     from gpumatfun_amd import problems, quadrature as q
     t0 = time.time()
     Kdefl = 20
-    cache = (os.path.join(args.setup_cache, f"fem1m_{args.nodes}.npz")
+    cache = (os.path.join(args.setup_cache, f"fem1m_{args.nodes}_{args.rcm}.npz")
              if args.setup_cache else None)
     if cache and os.path.exists(cache):
         import scipy.sparse as sp
@@ -117,7 +118,7 @@ def build_fem1m(args):
     cfg = {"workload": "fem_disc_1m_plain", "n": int(A.shape[0]), "nnz": int(A.nnz),
            "m": 80, "poly_degree": 8, "shifts": 64, "deflation": Kdefl, "alpha": 1.7,
            "tau": 1e-3, "kappa": 1.0, "columns_per_step": 2, "mesh_seed": 0,
-           "lambda_max": lN, "l1": float(lam[Kdefl])}
+           "lambda_max": lN, "l1": float(lam[Kdefl]), "rcm": args.rcm}
     return inp, cfg, time.time() - t0
 
 
@@ -128,7 +129,8 @@ def _fem1m_setup(args, t0, Kdefl):
     nodes, tri = problems.disc_mesh(args.nodes, 4096, seed=0)
     log(f"mesh {nodes.shape[0]} nodes, {tri.shape[0]} triangles ({time.time() - t0:.1f}s)")
     K, mdiag = problems.p1_stiffness_lumped_mass(nodes, tri)
-    A, mass, perm = problems.fem_operator(K, mdiag)
+    A, mass, perm = problems.fem_operator(K, mdiag,
+                                          reorder="native" if args.rcm == "native" else True)
     nodes = nodes[perm]
     log(f"A: n={A.shape[0]} nnz={A.nnz} ({time.time() - t0:.1f}s)")
     lam, Q = problems.smallest_eigenpairs(A, Kdefl + 1)
@@ -906,6 +908,9 @@ def main():
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--rcm", default="native", choices=["native", "scipy"],
+                    help="fem1m: reverse Cuthill-McKee order from the library's set-up code "
+                         "(gm_rcm_order, pseudo-peripheral start as symrcm) or SciPy's")
     ap.add_argument("--node-order", default="natural", choices=["natural", "pencil"],
                     help="batch3d: grid nodes in natural (i + n j + n^2 k) or 16 x 16-pencil order")
     ap.add_argument("--chunk-columns", type=int, default=0,

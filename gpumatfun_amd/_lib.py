@@ -45,7 +45,7 @@ EXPORTS = (
     "gm_ubatch_last_error", "gm_ubatch_create", "gm_ubatch_delete", "gm_ubatch_step",
     "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info", "gm_ubatch_get_schedule",
     "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
-    "gm_subspace_size",
+    "gm_subspace_size", "gm_rcm_order",
     "gm_eigs_last_error", "gm_eigs_smallest", "gm_sym_eig",
     "gm_bfab_last_error", "gm_bfab_create", "gm_bfab_delete", "gm_bfab_forward",
     "gm_bfab_set_param_derivs", "gm_bfab_backward_param", "gm_bfab_info",
@@ -321,6 +321,8 @@ def load():
         "gm_eig_weights": (C.c_int, [_dp, C.c_int32, C.c_double, C.c_double, _dp, _dp]),
         "gm_subspace_size": (C.c_int, [_dp, C.c_int32, C.c_double, C.c_double, C.c_double,
                                        C.POINTER(C.c_int32)]),
+        "gm_rcm_order": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32)]),
         "gm_eigs_last_error": (C.c_char_p, []),
         "gm_eigs_smallest": (C.c_int, [C.POINTER(GmEigsDesc), _dp, _dp, C.POINTER(C.c_int32),
                                        _dp]),

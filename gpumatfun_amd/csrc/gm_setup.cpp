@@ -7,10 +7,13 @@
 //   * gm_shift_setup         -- problem_startup.m:106-123, 140-144
 //   * gm_eig_weights         -- problem_startup.m:72-77
 //   * gm_subspace_size       -- problem_startup_nonuniform.m:101-113
+//   * gm_rcm_order           -- the symrcm reordering of process_nonuniform.m:15
+//                               (reverse Cuthill-McKee, George & Liu)
 // Jacobi elliptic functions of real argument come from the arithmetic-
 // geometric mean (A&S 16.4); at the complex points x + iK'/2 they follow from
 // the addition formulas A&S 16.21.1-3 with sn, cn, dn(iK'/2 | m) in closed form
 // (A&S 16.5 with the complementary parameter).  K(m) = pi / (2 AGM(1, sqrt(1-m))).
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -242,6 +245,95 @@ gm_status gm_subspace_size(const double* gamma, int32_t gamma_size, double l1, d
     return GM_OK;
   }
   *m = (int32_t)std::ceil(std::log(tol / 2.0) / std::log(kp));
+  return GM_OK;
+}
+
+// Reverse Cuthill-McKee ordering of a symmetric sparsity pattern (George & Liu,
+// "Computer Solution of Large Sparse Positive Definite Systems", 1981): per
+// connected component, a pseudo-peripheral start node (repeated breadth-first
+// searches from a minimum-degree node, moving to a minimum-degree node of the
+// last level while the eccentricity grows), then a breadth-first numbering that
+// takes each node's unnumbered neighbours in increasing degree (ties by index),
+// and the whole order reversed.  Diagonal entries are ignored.
+gm_status gm_rcm_order(int32_t n, const int32_t* ptr, const int32_t* ind, int32_t* perm) {
+  if (n < 0 || (n > 0 && (!ptr || !ind || !perm))) return GM_EINVAL;
+  if (n == 0) return GM_OK;
+  if (ptr[0] != 0) return GM_EINVAL;
+  for (int i = 0; i < n; ++i)
+    if (ptr[i + 1] < ptr[i]) return GM_EINVAL;
+  for (int j = 0; j < ptr[n]; ++j)
+    if (ind[j] < 0 || ind[j] >= n) return GM_EINVAL;
+  std::vector<int> deg(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int j = ptr[i]; j < ptr[i + 1]; ++j)
+      if (ind[j] != i) ++deg[i];
+  std::vector<int> level(n, -1), order;
+  order.reserve(n);
+  std::vector<char> numbered(n, 0);
+  // breadth-first levels from `root` over the unnumbered nodes; returns the last level
+  std::vector<int> queue(n), last;
+  auto bfs_levels = [&](int root, int* height) {
+    std::vector<int> touched;
+    int head = 0, tail = 0;
+    queue[tail++] = root;
+    level[root] = 0;
+    touched.push_back(root);
+    int h = 0;
+    while (head < tail) {
+      const int v = queue[head++];
+      h = level[v];
+      for (int j = ptr[v]; j < ptr[v + 1]; ++j) {
+        const int w = ind[j];
+        if (w == v || numbered[w] || level[w] >= 0) continue;
+        level[w] = level[v] + 1;
+        queue[tail++] = w;
+        touched.push_back(w);
+      }
+    }
+    last.clear();
+    for (int t = 0; t < tail; ++t)
+      if (level[queue[t]] == h) last.push_back(queue[t]);
+    for (int v : touched) level[v] = -1;
+    *height = h;
+  };
+  std::vector<int> byd(n);
+  for (int i = 0; i < n; ++i) byd[i] = i;
+  std::stable_sort(byd.begin(), byd.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+  std::vector<int> nbr;
+  for (int seed : byd) {
+    if (numbered[seed]) continue;
+    int root = seed, h = 0;
+    bfs_levels(root, &h);
+    for (int it = 0; it < 16; ++it) {   // pseudo-peripheral node
+      int cand = last[0];
+      for (int v : last)
+        if (deg[v] < deg[cand] || (deg[v] == deg[cand] && v < cand)) cand = v;
+      int h2 = 0;
+      bfs_levels(cand, &h2);
+      if (h2 <= h) break;
+      root = cand;
+      h = h2;
+    }
+    // Cuthill-McKee numbering of this component from root
+    size_t head = order.size();
+    order.push_back(root);
+    numbered[root] = 1;
+    while (head < order.size()) {
+      const int v = order[head++];
+      nbr.clear();
+      for (int j = ptr[v]; j < ptr[v + 1]; ++j) {
+        const int w = ind[j];
+        if (w != v && !numbered[w]) {
+          numbered[w] = 1;
+          nbr.push_back(w);
+        }
+      }
+      std::sort(nbr.begin(), nbr.end(),
+                [&](int a, int b) { return deg[a] < deg[b] || (deg[a] == deg[b] && a < b); });
+      for (int w : nbr) order.push_back(w);
+    }
+  }
+  for (int i = 0; i < n; ++i) perm[i] = order[n - 1 - i];
   return GM_OK;
 }
 

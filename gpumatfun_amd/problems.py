@@ -262,15 +262,35 @@ def disc_mesh(n_nodes, n_boundary=4096, seed=0):
     return nodes, tri
 
 
+def rcm_order(A):
+    """Reverse Cuthill-McKee order of A's (symmetric) pattern from the library's host
+    set-up code (``gm_rcm_order``, in place of symrcm at ``process_nonuniform.m:15``):
+    ``perm[q]`` is the old index placed at q."""
+    import ctypes as C
+    from gpumatfun_amd import _lib
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    ptr = np.ascontiguousarray(A.indptr, dtype=np.int32)
+    ind = np.ascontiguousarray(A.indices, dtype=np.int32)
+    perm = np.empty(n, dtype=np.int32)
+    ip = C.POINTER(C.c_int32)
+    _lib.check(_lib.load().gm_rcm_order(n, ptr.ctypes.data_as(ip), ind.ctypes.data_as(ip),
+                                        perm.ctypes.data_as(ip)))
+    return perm.astype(np.int64)
+
+
 def fem_operator(K, mdiag, reorder=True):
     """A = M^{-1/2} K M^{-1/2} with symrcm reordering, ``process_nonuniform.m:15-20,90-91``.
 
-    Returns (A, mass, perm) where mass = sqrt(diag(M)) in the new order and
-    ``perm`` maps new index -> old index.
+    ``reorder``: True (SciPy's reverse Cuthill-McKee), "native" (the library's
+    ``gm_rcm_order``) or False.  Returns (A, mass, perm) where mass = sqrt(diag(M))
+    in the new order and ``perm`` maps new index -> old index.
     """
     K = sp.csr_matrix(K)
     N = K.shape[0]
-    if reorder:
+    if reorder == "native":
+        perm = rcm_order(K)
+    elif reorder:
         perm = reverse_cuthill_mckee(K, symmetric_mode=True).astype(np.int64)
     else:
         perm = np.arange(N)

@@ -468,6 +468,10 @@ gm_status gm_eig_weights(const double* lambda, int32_t K, double eta, double alp
  * points only).                                                            */
 gm_status gm_subspace_size(const double* gamma, int32_t gamma_size, double l1, double lN,
                            double tol, int32_t* m);
+/* Reverse Cuthill-McKee order of the symmetric pattern (ptr n + 1, ind) of an
+ * n x n matrix, in place of symrcm (process_nonuniform.m:15): perm[q] is the
+ * old index of the node placed at q (A(perm, perm) is banded).  Host only.  */
+gm_status gm_rcm_order(int32_t n, const int32_t* ptr, const int32_t* ind, int32_t* perm);
 
 /* ---------------------------------------------------------------------------
  * f(A) on R columns at once with the parameter adjoint (no reference
