@@ -46,6 +46,7 @@ struct SpmmArgs {
   const int* __restrict__ col;
   const double* __restrict__ val;
   const int* __restrict__ rowblk;  // row-block bounds (k_spmm_stream only)
+  const int* __restrict__ blkj;    // rowptr at each row-block bound (k_spmm_stream only)
   const short* __restrict__ col16;  // C16: column - first row of the nonzero's row block
   const int* __restrict__ blk_prob;  // SEG: problem of each row block
   int aabb_stride;             // SEG: doubles between problems' aabb
@@ -240,9 +241,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
   constexpr int PER = TILE / kBlock;
   __shared__ double2 prod[TILE];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
+  // the block's row and nonzero ranges are four independent loads, so the
+  // nonzero stream starts one round trip after launch, not two
   const int r0 = p.rowblk[b], r1 = p.rowblk[b + 1];
+  const int j0 = p.blkj[b], j1 = p.blkj[b + 1];
   const int t = threadIdx.x;
-  const int j0 = p.rowptr[r0], j1 = p.rowptr[r1];
   const double* aabb = SEG ? p.aabb + (size_t)p.blk_prob[b] * p.aabb_stride : p.aabb;
   const FusedBeta fb{SEG ? nullptr : p.bpart, p.bG, p.beta_out, blockIdx.x == 0};
   const SpmmScalars c = spmm_scalars<MODE>(p, aabb, fb);

@@ -82,6 +82,7 @@ struct gm_solver {
   bool pre = false;     // row blocks: epilogue operands loaded before the gathers
   int G_sp = 1, chunk_sp = 1, G_el = 1, chunk_el = 1, G_df = 1, chunk_df = 1;
   int* rowblk[kNumTiles] = {};
+  int* blkj[kNumTiles] = {};    // rowptr at each row-block bound
   int nrowblk[kNumTiles] = {};
   // 16-bit column offsets from each nonzero's row-block start, per tile size
   // (null when some offset does not fit, or GM_FLAG_INDEX32)
@@ -143,6 +144,8 @@ void free_all(gm_solver* s) {
     if (g) hipGraphExecDestroy(g);
   for (int* rb : s->rowblk)
     if (rb) hipFree(rb);
+  for (int* bj : s->blkj)
+    if (bj) hipFree(bj);
   for (short* c : s->col16)
     if (c) hipFree(c);
   void* bufs[] = {s->rowptr, s->col, s->val, s->U, s->Usol, s->X, s->upd, s->W[0], s->W[1],
@@ -234,6 +237,7 @@ SpmmArgs spmm_base(gm_solver* s) {
   a.col = s->col;
   a.val = s->val;
   a.rowblk = s->tile > 0 ? s->rowblk[s->tile_idx] : nullptr;
+  a.blkj = s->tile > 0 ? s->blkj[s->tile_idx] : nullptr;
   a.col16 = (s->tile > 0 && s->use16) ? s->col16[s->tile_idx] : nullptr;
   a.nrows = s->n;
   a.chunk = s->chunk_sp;
@@ -682,6 +686,9 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
     s->nrowblk[i] = (int)blk.size() - 1;
     max_blocks = std::max(max_blocks, s->nrowblk[i]);
     GM_TRY(upload(&s->rowblk[i], blk.data(), blk.size(), st));
+    std::vector<int> bj(blk.size());
+    for (size_t b = 0; b < blk.size(); ++b) bj[b] = (int)rp[blk[b]];
+    GM_TRY(upload(&s->blkj[i], bj.data(), bj.size(), st));
     // column offsets from each block's first row, if every one fits 16 bits
     std::vector<short> c16;
     if (!(d->flags & GM_FLAG_INDEX32)) {

@@ -189,8 +189,8 @@ struct gm_ubatch {
   bool clamp = true;
   std::vector<double> gamma;
   std::vector<int> off;          // P + 1 row offsets
-  int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr, *blk_prob = nullptr,
-      *prob_blk = nullptr;
+  int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr, *blkj = nullptr,
+      *blk_prob = nullptr, *prob_blk = nullptr;
   short* col16 = nullptr;        // 16-bit column offsets from each block's first row
   int tile = gmu::kTile;         // nonzeros per row block (SpMM LDS tile)
   bool use16 = false, pre = true;  // SpMM: 16-bit offsets; epilogue loads before gathers
@@ -228,7 +228,8 @@ gm_status uupload(T** p, const T* src, size_t count, hipStream_t st) {
 
 void ufree_all(gm_ubatch* u) {
   if (u->graph) (void)hipGraphExecDestroy(u->graph);
-  void* bufs[] = {u->col16, u->rowptr, u->col, u->rowblk, u->blk_prob, u->prob_blk, u->val, u->mass,
+  void* bufs[] = {u->col16, u->rowptr, u->col, u->rowblk, u->blkj, u->blk_prob, u->prob_blk, u->val,
+                  u->mass,
                   u->aabb, u->wts1, u->wts2, u->shifts, u->gsr, u->gsi, u->yr, u->yi,
                   u->partials, u->U, u->Usol, u->X, u->W[0], u->W[1], u->T[0], u->T[1], u->V,
                   u->Z, u->C, u->wD, u->wDU, u->wDU2, u->wX, u->status};
@@ -291,6 +292,7 @@ gm::SpmmArgs uspmm_base(gm_ubatch* u) {
   a.col = u->col;
   a.val = u->val;
   a.rowblk = u->rowblk;
+  a.blkj = u->blkj;
   a.col16 = u->use16 ? u->col16 : nullptr;
   a.blk_prob = u->blk_prob;
   a.aabb_stride = (int)aabb_stride(u);
@@ -475,14 +477,18 @@ gm_status set_blocks(gm_ubatch* u, const std::vector<int>& rp, const std::vector
     for (size_t t = before; t < blk.size(); ++t) bprob.push_back(q);
     pblk.push_back((int)blk.size() - 1);
   }
-  for (void* p : {(void*)u->rowblk, (void*)u->blk_prob, (void*)u->prob_blk, (void*)u->col16})
+  for (void* p : {(void*)u->rowblk, (void*)u->blkj, (void*)u->blk_prob, (void*)u->prob_blk,
+                  (void*)u->col16})
     if (p) U_HIP(hipFree(p));
-  u->rowblk = u->blk_prob = u->prob_blk = nullptr;
+  u->rowblk = u->blkj = u->blk_prob = u->prob_blk = nullptr;
   u->col16 = nullptr;
   u->G = (int)bprob.size();
   u->tile = tile;
   hipStream_t st = u->stream;
   U_TRY(uupload(&u->rowblk, blk.data(), blk.size(), st));
+  std::vector<int> bj(blk.size());
+  for (size_t b = 0; b < blk.size(); ++b) bj[b] = rp[blk[b]];
+  U_TRY(uupload(&u->blkj, bj.data(), bj.size(), st));
   U_TRY(uupload(&u->blk_prob, bprob.data(), bprob.size(), st));
   U_TRY(uupload(&u->prob_blk, pblk.data(), pblk.size(), st));
   std::vector<short> c16(ci.size());
