@@ -9,7 +9,8 @@ Workloads (``--config``):
   order from the library's gm_rcm_order, in place of symrcm),
   f1 = exp(-tau kappa A^{alpha/2}), f2 = phi(...), alpha = 1.7, tau = 1e-3,
   kappa = 1, degree-8 Jacobi polynomial preconditioner, Lanczos m = 80, 64
-  contour shifts, 20 deflated eigenvectors.  Its line also carries the other
+  contour shifts, 20 deflated eigenvectors (from the library's sparse
+  Cholesky + shift-invert block Lanczos, in place of eigs(K, M, k, 'sm')).  Its line also carries the other
   two parts of the metric: ``batched_form`` (the ``uneven64`` batch against the
   plain path on the same problems) and ``adjoint_form`` (``adjoint256``: the
   value-and-gradient cost against one forward solve), each with its parity.
@@ -51,6 +52,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+EIGS_INFO = {}   # fem1m: which eigensolver produced the deflation pairs, and its figures
 
 
 def kernel_symbol(info, mode):
@@ -98,7 +100,7 @@ def build_fem1m(args):
     from gpumatfun_amd import problems, quadrature as q
     t0 = time.time()
     Kdefl = 20
-    cache = (os.path.join(args.setup_cache, f"fem1m_{args.nodes}_{args.rcm}.npz")
+    cache = (os.path.join(args.setup_cache, f"fem1m_{args.nodes}_{args.rcm}_{args.eigs}.npz")
              if args.setup_cache else None)
     if cache and os.path.exists(cache):
         import scipy.sparse as sp
@@ -119,6 +121,8 @@ def build_fem1m(args):
            "m": 80, "poly_degree": 8, "shifts": 64, "deflation": Kdefl, "alpha": 1.7,
            "tau": 1e-3, "kappa": 1.0, "columns_per_step": 2, "mesh_seed": 0,
            "lambda_max": lN, "l1": float(lam[Kdefl]), "rcm": args.rcm}
+    if EIGS_INFO:
+        cfg["deflation_setup"] = dict(EIGS_INFO)
     return inp, cfg, time.time() - t0
 
 
@@ -133,9 +137,25 @@ def _fem1m_setup(args, t0, Kdefl):
                                           reorder="native" if args.rcm == "native" else True)
     nodes = nodes[perm]
     log(f"A: n={A.shape[0]} nnz={A.nnz} ({time.time() - t0:.1f}s)")
-    lam, Q = problems.smallest_eigenpairs(A, Kdefl + 1)
-    lN = problems.largest_eigenvalue(A)
-    log(f"eigenpairs: l1={lam[Kdefl]:.6g} lN={lN:.6g} ({time.time() - t0:.1f}s)")
+    te = time.time()
+    if args.eigs == "arpack":
+        lam, Q = problems.smallest_eigenpairs(A, Kdefl + 1)
+        lN = problems.largest_eigenvalue(A)
+        EIGS_INFO.update(eigensolver="scipy eigsh (SuperLU + ARPACK shift-invert, 'LA')")
+    else:
+        from gpumatfun_amd import eigs as ge
+        sigma = ge.default_shift(A)
+        lam, Q, inf = ge.eigs_shift_invert(A, Kdefl + 1, sigma=sigma, info=True)
+        lN, linf = ge.lambda_max(A, info=True)
+        EIGS_INFO.update(eigensolver="gm_eigs_shift_invert + gm_lambda_max (host, library)",
+                         sigma=sigma, block_solves=inf["blocks"],
+                         shift_invert_residual=inf["max_residual"],
+                         lambda_max_iterations=linf["iterations"],
+                         max_residual_over_lambda_max=float(np.max(np.linalg.norm(
+                             A @ Q - Q * lam, axis=0)) / lN))
+    EIGS_INFO["seconds"] = round(time.time() - te, 2)
+    log(f"eigenpairs: l1={lam[Kdefl]:.6g} lN={lN:.6g} ({time.time() - t0:.1f}s; "
+        f"{EIGS_INFO})")
     # initial condition of problem_startup_nonuniform.m:38-52
     x, y = nodes[:, 0], nodes[:, 1]
     r = np.sqrt((x - 0.1) ** 2 + (y + 0.1) ** 2)
@@ -908,6 +928,10 @@ def main():
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--eigs", default="native", choices=["native", "arpack"],
+                    help="fem1m: deflation pairs and lambda_max from the library's "
+                         "factorisation-based eigensolver (gm_eigs_shift_invert, "
+                         "gm_lambda_max) or scipy's ARPACK")
     ap.add_argument("--rcm", default="native", choices=["native", "scipy"],
                     help="fem1m: reverse Cuthill-McKee order from the library's set-up code "
                          "(gm_rcm_order, pseudo-peripheral start as symrcm) or SciPy's")

@@ -47,6 +47,8 @@ EXPORTS = (
     "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
     "gm_subspace_size", "gm_rcm_order",
     "gm_eigs_last_error", "gm_eigs_smallest", "gm_sym_eig",
+    "gm_chol_last_error", "gm_nd_order", "gm_chol_factor", "gm_chol_solve", "gm_chol_info",
+    "gm_chol_perm", "gm_chol_delete", "gm_eigs_shift_invert", "gm_lambda_max",
     "gm_bfab_last_error", "gm_bfab_create", "gm_bfab_delete", "gm_bfab_forward",
     "gm_bfab_set_param_derivs", "gm_bfab_backward_param", "gm_bfab_info",
 )
@@ -204,6 +206,25 @@ class GmEigsDesc(C.Structure):
     ]
 
 
+class GmSieigsDesc(C.Structure):
+    """Mirror of ``gm_sieigs_desc``."""
+    _fields_ = [
+        ("n", C.c_int32),
+        ("nnz", C.c_int64),
+        ("A_values", _dp),
+        ("A_ptr", _ip),
+        ("A_ind", _ip),
+        ("num_eigs", C.c_int32),
+        ("block_size", C.c_int32),
+        ("max_blocks", C.c_int32),
+        ("max_restarts", C.c_int32),
+        ("tol", C.c_double),
+        ("sigma", C.c_double),
+        ("seed", C.c_uint64),
+        ("threads", C.c_int32),
+    ]
+
+
 class GmBfabDesc(C.Structure):
     """Mirror of ``gm_bfab_desc``."""
     _fields_ = [
@@ -327,6 +348,18 @@ def load():
         "gm_eigs_smallest": (C.c_int, [C.POINTER(GmEigsDesc), _dp, _dp, C.POINTER(C.c_int32),
                                        _dp]),
         "gm_sym_eig": (C.c_int, [C.c_int32, _dp, _dp, _dp]),
+        "gm_chol_last_error": (C.c_char_p, []),
+        "gm_nd_order": (C.c_int, [C.c_int32, _ip, _ip, _ip]),
+        "gm_chol_factor": (C.c_int, [C.c_int32, _ip, _ip, _dp, C.c_double, C.c_int32,
+                                     C.POINTER(vp)]),
+        "gm_chol_solve": (C.c_int, [vp, C.c_int32, _dp]),
+        "gm_chol_info": (C.c_int, [vp, C.POINTER(C.c_int64), _dp, C.POINTER(C.c_int32), _dp]),
+        "gm_chol_perm": (C.c_int, [vp, _ip]),
+        "gm_chol_delete": (C.c_int, [vp]),
+        "gm_eigs_shift_invert": (C.c_int, [C.POINTER(GmSieigsDesc), _dp, _dp,
+                                           C.POINTER(C.c_int32), _dp]),
+        "gm_lambda_max": (C.c_int, [C.c_int32, _ip, _ip, _dp, C.c_double, C.c_int32,
+                                    C.c_uint64, C.c_int32, _dp, C.POINTER(C.c_int32)]),
         "gm_bfab_last_error": (C.c_char_p, []),
         "gm_bfab_create": (C.c_int, [C.POINTER(GmBfabDesc), C.POINTER(vp)]),
         "gm_bfab_delete": (C.c_int, [vp]),
@@ -343,10 +376,12 @@ def load():
     return lib
 
 
-def check(status, batch=False, fab=False, ubatch=False, eigs=False, bfab=False):
+def check(status, batch=False, fab=False, ubatch=False, eigs=False, bfab=False, chol=False):
     if status != GM_OK:
         lib = load()
-        if bfab:
+        if chol:
+            raw = lib.gm_chol_last_error()
+        elif bfab:
             raw = lib.gm_bfab_last_error()
         elif eigs:
             raw = lib.gm_eigs_last_error()

@@ -559,6 +559,73 @@ gm_status gm_eigs_smallest(const gm_eigs_desc* desc, double* lambda, double* Q,
  * eigenvectors as columns.                                                */
 gm_status gm_sym_eig(int32_t n, const double* A, double* w, double* V);
 
+/* ---------------------------------------------------------------------------
+ * Deflation eigenpairs through a sparse factorisation (host code, threads;
+ * no GPU needed).  Replaces eigs(K, M, k, 'sm') of process_nonuniform.m:38
+ * and :106-110 -- MATLAB factors K - sigma M and runs shift-invert Lanczos --
+ * for operators whose lambda_max / lambda_K is too large for the Chebyshev
+ * filter of gm_eigs_smallest (sliver elements of an unstructured mesh), and
+ * eigs(K, M, 1, 'lm') of process_nonuniform.m:40 (gm_lambda_max).  A is the
+ * symmetric n x n CSR matrix with both triangles (int32 indices); with the
+ * lumped mass, A = M^(-1/2) K M^(-1/2) and its orthonormal eigenvectors are
+ * M^(1/2) times the M-orthonormal ones of (K, M) (process_nonuniform.m:107).
+ * Results do not depend on the thread count.
+ * ------------------------------------------------------------------------- */
+typedef struct gm_chol gm_chol;
+
+const char* gm_chol_last_error(void);
+/* Nested-dissection order of the symmetric pattern: perm[q] is the old index
+ * of the node placed at q.                                                  */
+gm_status gm_nd_order(int32_t n, const int32_t* ptr, const int32_t* ind, int32_t* perm);
+/* Cholesky factor of P (A - sigma I) P^T (multifrontal, supernodal) with P
+ * from gm_nd_order refined to a postorder of the elimination tree.
+ * threads <= 0: all hardware threads.  GM_EBREAKDOWN if A - sigma I is not
+ * positive definite.                                                        */
+gm_status gm_chol_factor(int32_t n, const int32_t* ptr, const int32_t* ind, const double* val,
+                         double sigma, int32_t threads, gm_chol** out);
+/* X <- (A - sigma I)^(-1) X for nrhs columns (n x nrhs column-major, host). */
+gm_status gm_chol_solve(gm_chol* f, int32_t nrhs, double* X);
+/* nnz(L) (stored, incl. the diagonal), factorisation flops, supernodes, and
+ * seconds[3] = ordering, symbolic, numeric (any may be NULL).               */
+gm_status gm_chol_info(gm_chol* f, int64_t* nnz_l, double* flops, int32_t* nsuper,
+                       double* seconds);
+/* The factor's final order (n): perm[q] = old index at q.                   */
+gm_status gm_chol_perm(gm_chol* f, int32_t* perm);
+gm_status gm_chol_delete(gm_chol* f);
+
+typedef struct gm_sieigs_desc {
+    int32_t n;
+    int64_t nnz;
+    const double* A_values;        /* CSR, both triangles, symmetric         */
+    const int32_t* A_ptr;
+    const int32_t* A_ind;
+    int32_t num_eigs;              /* K >= 1 smallest                        */
+    int32_t block_size;            /* K <= block_size <= min(512, n / 2)     */
+    int32_t max_blocks;            /* Lanczos blocks per restart (0: 12)     */
+    int32_t max_restarts;          /* < 0: 4                                 */
+    double tol;                    /* ||A q - l q||_2 / ||A||_inf per pair   */
+                                   /*  (<= 0: 1e-14)                         */
+    double sigma;                  /* shift below lambda_1 (A - sigma I SPD) */
+    uint64_t seed;                 /* start block: uniform [-1, 1) deviates  */
+    int32_t threads;               /* <= 0: all hardware threads             */
+} gm_sieigs_desc;
+
+/* lambda (K, ascending) and, when Q is not NULL, Q (n x K column-major,
+ * orthonormal columns) by block Lanczos on (A - sigma I)^(-1) with full
+ * reorthogonalisation; a restart cycle ends when the recurrence's residual
+ * estimate passes tol, and the pairs are accepted on their explicit
+ * residuals.  blocks (may be NULL) = block solves taken, max_residual (may
+ * be NULL) = max ||A q - l q||_2 / ||A||_inf.  GM_EBREAKDOWN, with the
+ * outputs filled, if tol is not met in max_restarts restarts.              */
+gm_status gm_eigs_shift_invert(const gm_sieigs_desc* desc, double* lambda, double* Q,
+                               int32_t* blocks, double* max_residual);
+/* lambda_max by Lanczos from a seeded start vector (the largest Ritz value,
+ * stopped when it changes by <= tol relative three times running; tol <= 0:
+ * 1e-12, max_iter <= 0: 300).  iters may be NULL.                          */
+gm_status gm_lambda_max(int32_t n, const int32_t* ptr, const int32_t* ind, const double* val,
+                        double tol, int32_t max_iter, uint64_t seed, int32_t threads,
+                        double* lmax, int32_t* iters);
+
 #ifdef __cplusplus
 }
 #endif

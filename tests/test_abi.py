@@ -37,7 +37,8 @@ def test_library_exports_every_declared_symbol():
                                            ("gm_fab_desc", "GmFabDesc"),
                                            ("gm_ubatch_desc", "GmUbatchDesc"),
                                            ("gm_eigs_desc", "GmEigsDesc"),
-                                           ("gm_bfab_desc", "GmBfabDesc")])
+                                           ("gm_bfab_desc", "GmBfabDesc"),
+                                           ("gm_sieigs_desc", "GmSieigsDesc")])
 def test_struct_layout_matches_header_field_order(cname, pyname):
     text = open(HEADER).read()
     body = text[text.index("typedef struct " + cname):text.index("} " + cname + ";")]
