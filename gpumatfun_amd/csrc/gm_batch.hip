@@ -143,14 +143,16 @@ gm_status breduce(gm_batch* b, int nv, double* out, int do_sqrt, int flag_zero, 
 
 template <int R>
 struct Launch {
+  // chunks of >= 128 columns stream v_k, v_{k-1} and y non-temporally
+  static constexpr bool kNT = R >= 128;
   static gm_status spmm(gm_batch* b, int mode, const gmb::BSpmm& a) {
     dim3 g(b->G_sp), t(gm::kBlock);
     switch (mode) {
-      case gmb::kFirst: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kFirst, true>), g, t, 0, b->stream, a); break;
-      case gmb::kMid: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kMid, true>), g, t, 0, b->stream, a); break;
-      case gmb::kLast: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kLast, true>), g, t, 0, b->stream, a); break;
-      case gmb::kOnly: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kOnly, true>), g, t, 0, b->stream, a); break;
-      default: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kHorner, true>), g, t, 0, b->stream, a); break;
+      case gmb::kFirst: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kFirst, true, kNT>), g, t, 0, b->stream, a); break;
+      case gmb::kMid: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kMid, true, kNT>), g, t, 0, b->stream, a); break;
+      case gmb::kLast: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kLast, true, kNT>), g, t, 0, b->stream, a); break;
+      case gmb::kOnly: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kOnly, true, kNT>), g, t, 0, b->stream, a); break;
+      default: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kHorner, true, kNT>), g, t, 0, b->stream, a); break;
     }
     B_HIP(hipGetLastError());
     return GM_OK;

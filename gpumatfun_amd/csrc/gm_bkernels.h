@@ -57,6 +57,21 @@ struct BSpmm {
   unsigned* status;
 };
 
+// Single-use vector streams of the SpMM (v_k and y written, v_{k-1} read,
+// once per launch) as non-temporal accesses when NT.  A compile-time choice:
+// a run-time branch between the two forms is merged by the compiler into one
+// plain access.
+template <bool NT>
+__device__ __forceinline__ void st_stream(double* ptr, double v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, ptr);
+  else *ptr = v;
+}
+template <bool NT>
+__device__ __forceinline__ double ld_stream(const double* ptr) {
+  if constexpr (NT) return __builtin_nontemporal_load(ptr);
+  else return *ptr;
+}
+
 // Column sums of G partials [G][R] (written by the previous kernel) into
 // out[0..R) in LDS, the same fixed order in every block: thread t sums
 // column t % W over g = t / W, t / W + ns, ... (ns = kBlock / W, W = min(R,
@@ -130,8 +145,11 @@ inline void bspmm_row_blocks(const int* rowptr, int n, std::vector<int>& rb) {
 // and row pointers are first staged in LDS by coalesced loads (one HBM
 // latency per row block instead of one per row).  Otherwise block b covers
 // p.chunk rows read straight from global memory.  Partials are per grid
-// block either way.
-template <int R, int MODE, bool kStaged = false>
+// block either way.  kNT: the single-use vector streams are non-temporal
+// (wide chunks, whose 2 KB rows overflow L2 anyway: 5.22 -> 5.04 ms per launch
+// at 256 columns on the 128^3 Laplacian, the kernels that read them next
+// unchanged).
+template <int R, int MODE, bool kStaged = false, bool kNT = false>
 __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
   constexpr int CPW = R < 64 ? R : 64;
   constexpr int RPW = 64 / CPW;
@@ -218,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
           o1[q] = 0.0;
           if (MODE == kFirst || MODE == kOnly) o0[q] = p.x[(size_t)r * p.ldx + c];
           if (MODE == kMid || MODE == kLast) o0[q] = p.vk[iv];
-          if ((MODE == kLast || MODE == kOnly) && p.k > 0) o1[q] = p.vkm1[iv];
+          if ((MODE == kLast || MODE == kOnly) && p.k > 0) o1[q] = ld_stream<kNT>(p.vkm1 + iv);
           if (MODE == kHorner) {
             o0[q] = p.z[(size_t)r * p.ldz + c];
             if (p.accumulate) o1[q] = p.y[(size_t)r * p.ldy + c];
@@ -264,22 +282,22 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
           const size_t iv = (size_t)r * R + c;
           if (MODE == kFirst) {
             const double v = safe_div(o0[q], bet[q]);
-            p.vk[iv] = v;
-            p.y[iy] = fma(p.gi, v, acc[q] * inv[q]);
+            st_stream<kNT>(p.vk + iv, v);
+            st_stream<kNT>(p.y + iy, fma(p.gi, v, acc[q] * inv[q]));
           } else if (MODE == kMid) {
-            p.y[iy] = fma(p.gi, o0[q], acc[q]);
+            st_stream<kNT>(p.y + iy, fma(p.gi, o0[q], acc[q]));
           } else if (MODE == kLast || MODE == kOnly) {
             double v, yv;
             if (MODE == kOnly) {
               v = safe_div(o0[q], bet[q]);
-              p.vk[iv] = v;
+              st_stream<kNT>(p.vk + iv, v);
               yv = acc[q] * inv[q];
             } else {
               v = o0[q];
               yv = acc[q];
             }
             if (p.k > 0) yv = fma(-bprev[q], o1[q], yv);
-            p.y[iy] = yv;
+            st_stream<kNT>(p.y + iy, yv);
             dot[q] = fma(yv, v, dot[q]);
           } else if (MODE == kCheb) {
             const double sh = p.shiftc ? p.shiftc[c] : p.cc;
