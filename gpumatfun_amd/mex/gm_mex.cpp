@@ -18,6 +18,11 @@
 //  12 eigs_smallest_mex       A, k [, opts] -> Q, lambda  (no reference MEX: replaces
 //                             [Q, lambda] = eigs(K, M, k, 'sm') and Q = M^(1/2) Q of
 //                             process_nonuniform.m:106-110, on A = M^(-1/2) K M^(-1/2))
+//  13 eigs_shift_invert_mex  A, k [, sigma [, opts]] -> Q, lambda  (the same replacement
+//                             through the host sparse factorisation, for meshes whose
+//                             lambda_max / lambda_k is too large for the GPU filter)
+//  14 lambda_max_mex         A -> lambda_max  (replaces eigs(K, M, 1, 'lm') of
+//                             process_nonuniform.m:40)
 //
 // Argument errors use the reference's message identifiers; a non-OK status
 // of the library becomes mexErrMsgIdAndTxt("gpumatfun:<status>", message).
@@ -31,7 +36,7 @@
 #include "gpumatfun.h"
 
 #ifndef GM_GATEWAY
-#error "compile with -DGM_GATEWAY=<1..12>"
+#error "compile with -DGM_GATEWAY=<1..14>"
 #endif
 
 namespace {
@@ -312,6 +317,99 @@ extern "C" void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
   check(gm_eigs_smallest(&d, mxGetPr(lam), mxGetPr(Q), nullptr, nullptr), gm_eigs_last_error);
   plhs[0] = Q;
   if (nlhs == 2) plhs[1] = lam;
+#elif GM_GATEWAY == 13
+  // [Q, lambda] = eigs_shift_invert_mex(A, k [, sigma [, [block max_blocks tol]]])
+  if (nrhs < 2 || nrhs > 4)
+    mexErrMsgIdAndTxt("MATLAB:eigs_shift_invert:invalidNumInputs",
+                      "Two to four input arguments are required.");
+  if (nlhs < 1 || nlhs > 2)
+    mexErrMsgIdAndTxt("MATLAB:eigs_shift_invert:maxlhs",
+                      "One or two output arguments are required.");
+  const mxArray* A = prhs[0];
+  if (!mxIsSparse(A) || !mxIsDouble(A))
+    mexErrMsgIdAndTxt("MATLAB:eigs_shift_invert:inputNotSparse",
+                      "A must be a sparse double matrix.");
+  const size_t n = mxGetM(A);
+  if (mxGetN(A) != n || n > 0x7fffffff)
+    mexErrMsgIdAndTxt("MATLAB:eigs_shift_invert:notSquare", "A must be square.");
+  if (!mxIsDouble(prhs[1]) || mxGetM(prhs[1]) * mxGetN(prhs[1]) != 1)
+    mexErrMsgIdAndTxt("MATLAB:eigs_shift_invert:bad_k", "k must be a scalar.");
+  const int k = static_cast<int>(mxGetPr(prhs[1])[0]);
+  if (k < 1 || static_cast<double>(k) != mxGetPr(prhs[1])[0])
+    mexErrMsgIdAndTxt("MATLAB:eigs_shift_invert:bad_k", "k must be a positive integer.");
+  const mwIndex* jc = mxGetJc(A);
+  const mwIndex* ir = mxGetIr(A);
+  const size_t nnz = jc[n];
+  const double* val = mxGetPr(A);
+  // default shift: -1e-10 times the largest absolute column sum (= row sum)
+  double sigma = 0.0;
+  for (size_t j = 0; j < n; ++j) {
+    double acc = 0.0;
+    for (mwIndex e = jc[j]; e < jc[j + 1]; ++e) acc += val[e] < 0 ? -val[e] : val[e];
+    sigma = acc > sigma ? acc : sigma;
+  }
+  sigma *= -1e-10;
+  if (nrhs >= 3 && !mxIsEmpty(prhs[2])) {
+    if (!mxIsDouble(prhs[2]) || mxGetM(prhs[2]) * mxGetN(prhs[2]) != 1)
+      mexErrMsgIdAndTxt("MATLAB:eigs_shift_invert:bad_sigma", "sigma must be a scalar.");
+    sigma = mxGetPr(prhs[2])[0];
+  }
+  double opt[3] = {0.0, 12.0, 1e-14};   // block (0: k + 4, at least 8), blocks per cycle, tol
+  if (nrhs == 4) {
+    if (!mxIsDouble(prhs[3]) || mxGetM(prhs[3]) * mxGetN(prhs[3]) != 3)
+      mexErrMsgIdAndTxt("MATLAB:eigs_shift_invert:bad_opts",
+                        "opts must be [block max_blocks tol].");
+    std::memcpy(opt, mxGetPr(prhs[3]), sizeof(opt));
+  }
+  int block = static_cast<int>(opt[0]);
+  if (block == 0) block = k + 4 < 8 ? 8 : k + 4;
+  if (2 * static_cast<size_t>(block) > n) block = static_cast<int>(n / 2);
+  // CSC of a symmetric matrix: the same arrays as its CSR
+  std::vector<int32_t> ptr(n + 1), ind(nnz);
+  for (size_t i = 0; i <= n; ++i) ptr[i] = static_cast<int32_t>(jc[i]);
+  for (size_t j = 0; j < nnz; ++j) ind[j] = static_cast<int32_t>(ir[j]);
+  gm_sieigs_desc d{};
+  d.n = static_cast<int32_t>(n);
+  d.nnz = static_cast<int64_t>(nnz);
+  d.A_values = val;
+  d.A_ptr = ptr.data();
+  d.A_ind = ind.data();
+  d.num_eigs = k;
+  d.block_size = block;
+  d.max_blocks = static_cast<int32_t>(opt[1]);
+  d.max_restarts = -1;
+  d.tol = opt[2];
+  d.sigma = sigma;
+  d.seed = 0;
+  d.threads = 0;
+  mxArray* Q = mxCreateDoubleMatrix(static_cast<mwSize>(n), static_cast<mwSize>(k), mxREAL);
+  mxArray* lam = mxCreateDoubleMatrix(static_cast<mwSize>(k), 1, mxREAL);
+  check(gm_eigs_shift_invert(&d, mxGetPr(lam), mxGetPr(Q), nullptr, nullptr), gm_chol_last_error);
+  plhs[0] = Q;
+  if (nlhs == 2) plhs[1] = lam;
+#elif GM_GATEWAY == 14
+  // lmax = lambda_max_mex(A)
+  if (nrhs != 1)
+    mexErrMsgIdAndTxt("MATLAB:lambda_max:invalidNumInputs", "One input argument is required.");
+  if (nlhs > 1)
+    mexErrMsgIdAndTxt("MATLAB:lambda_max:maxlhs", "Too many output arguments.");
+  const mxArray* A = prhs[0];
+  if (!mxIsSparse(A) || !mxIsDouble(A))
+    mexErrMsgIdAndTxt("MATLAB:lambda_max:inputNotSparse", "A must be a sparse double matrix.");
+  const size_t n = mxGetM(A);
+  if (mxGetN(A) != n || n > 0x7fffffff)
+    mexErrMsgIdAndTxt("MATLAB:lambda_max:notSquare", "A must be square.");
+  const mwIndex* jc = mxGetJc(A);
+  const mwIndex* ir = mxGetIr(A);
+  const size_t nnz = jc[n];
+  std::vector<int32_t> ptr(n + 1), ind(nnz);
+  for (size_t i = 0; i <= n; ++i) ptr[i] = static_cast<int32_t>(jc[i]);
+  for (size_t j = 0; j < nnz; ++j) ind[j] = static_cast<int32_t>(ir[j]);
+  double lmax = 0.0;
+  check(gm_lambda_max(static_cast<int32_t>(n), ptr.data(), ind.data(), mxGetPr(A), 0.0, 0, 0, 0,
+                      &lmax, nullptr),
+        gm_chol_last_error);
+  plhs[0] = mxCreateDoubleScalar(lmax);
 #else
 #error "unknown GM_GATEWAY"
 #endif

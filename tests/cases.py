@@ -29,10 +29,13 @@ def dirichlet_case(side=128, m=50, n=32, degree=0, seed=0, tau=1e-3, kappa=1.0, 
 
 def fem_case(n_nodes=3000, n_boundary=256, K=6, m=40, n=32, degree=2, seed=3,
              eta=1e-3, alpha=1.7, dt=1e-3):
-    """Small FEM disc with mass lumping, deflation of K eigenpairs, Jacobi polynomial."""
+    """Small FEM disc with mass lumping, deflation of K eigenpairs, Jacobi polynomial.
+    The deflation pairs and lambda_max come from the library's factorisation-based
+    eigensolver, as in the full-scale set-up."""
+    from gpumatfun_amd.eigs import eigs_shift_invert, lambda_max
     A, mass, nodes = problems.fem_disc(n_nodes, n_boundary, seed)
-    lam, Q = problems.smallest_eigenpairs(A, K + 1)
-    lN = problems.largest_eigenvalue(A)
+    lam, Q = eigs_shift_invert(A, K + 1)
+    lN = lambda_max(A)
     l1 = float(lam[K])
     x = nodes[:, 0]
     y = nodes[:, 1]

@@ -60,6 +60,14 @@ mxArray* mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity c) {
   return a;
 }
 
+mxArray* mxCreateDoubleScalar(double v) {
+  mxArray* a = mxCreateDoubleMatrix(1, 1, mxREAL);
+  a->pr[0] = v;
+  return a;
+}
+
+int mxIsEmpty(const mxArray* a) { return a->m == 0 || a->n == 0; }
+
 mxArray* mxCreateNumericArray(mwSize ndim, const mwSize* dims, mxClassID cls, mxComplexity) {
   mxArray* a = new mxArray_tag();
   a->cls = cls;

@@ -24,6 +24,7 @@ GATEWAYS = {
     "lin_sys_solves_mex": 4, "apply_V_mex": 5, "update_U_mex": 6,
     "update_U_monodomain_mex": 7, "get_U_mex": 8, "get_xsol_mex": 9,
     "get_aabb_mex": 10, "set_y_mex": 11, "eigs_smallest_mex": 12,
+    "eigs_shift_invert_mex": 13, "lambda_max_mex": 14,
 }
 
 

@@ -31,7 +31,9 @@ mwSize mxGetNzmax(const mxArray* a);
 int mxIsDouble(const mxArray* a);
 int mxIsSparse(const mxArray* a);
 int mxIsUint64(const mxArray* a);
+int mxIsEmpty(const mxArray* a);
 mxArray* mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity c);
+mxArray* mxCreateDoubleScalar(double v);
 mxArray* mxCreateNumericArray(mwSize ndim, const mwSize* dims, mxClassID cls, mxComplexity c);
 void mexErrMsgIdAndTxt(const char* id, const char* fmt, ...);
 

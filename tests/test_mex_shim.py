@@ -196,3 +196,46 @@ def test_eigs_gateway_matches_library(mex):
     assert np.abs(lam[:, 0] - exact).max() <= 1e-10 * exact[0]
     lam2, Q2 = eigs_smallest(A, 6, block_size=16, degree=40, max_passes=200, tol=1e-10)
     assert np.array_equal(lam[:, 0], lam2) and np.array_equal(Q, Q2)
+
+
+def test_shift_invert_gateways_match_library_and_arpack(mex):
+    """[Q, lambda] = eigs_shift_invert_mex(A, k) and lmax = lambda_max_mex(A) --
+    host code, no GPU: the same pairs as the Python wrapper (same default shift,
+    same arithmetic), eigenvalues as ARPACK's, and lambda_max as ARPACK's."""
+    from gpumatfun_amd import problems
+    from gpumatfun_amd.eigs import eigs_shift_invert, lambda_max
+    A, mass, nodes = problems.fem_disc(2000, 200, 4)
+    Q, lam = mex("eigs_shift_invert_mex", A.tocsc(), 7.0, nlhs=2)
+    assert Q.shape == (2000, 7) and lam.shape == (7, 1)
+    lam2, Q2 = eigs_shift_invert(A, 7)
+    assert np.array_equal(lam[:, 0], lam2) and np.array_equal(Q, Q2)
+    lam_a, _ = problems.smallest_eigenpairs(A, 7)
+    lmax = float(np.asarray(mex("lambda_max_mex", A.tocsc(), nlhs=1)).ravel()[0])
+    assert lmax == lambda_max(A)
+    assert lmax == pytest.approx(problems.largest_eigenvalue(A), rel=1e-11)
+    assert np.abs(lam[:, 0] - lam_a).max() <= 1e-12 * lmax
+    # an explicit shift and options reach the library
+    Q3, lam3 = mex("eigs_shift_invert_mex", A.tocsc(), 7.0, -0.5, np.array([16.0, 10.0, 1e-13]),
+                   nlhs=2)
+    lam4, _ = eigs_shift_invert(A, 7, sigma=-0.5, block_size=16, max_blocks=10, tol=1e-13)
+    assert np.array_equal(lam3[:, 0], lam4)
+
+
+def test_shift_invert_gateway_argument_errors(mex):
+    from gpumatfun_amd import problems
+    A = problems.laplacian_2d_dirichlet(8)
+    lam1 = float(problems.dirichlet_eigs_2d(8).min())
+    cases = [((A.toarray(), 4.0), "MATLAB:eigs_shift_invert:inputNotSparse"),
+             ((A, np.array([1.0, 2.0])), "MATLAB:eigs_shift_invert:bad_k"),
+             ((A, 2.5), "MATLAB:eigs_shift_invert:bad_k"),
+             ((A, 4.0, np.array([1.0, 2.0])), "MATLAB:eigs_shift_invert:bad_sigma"),
+             ((A, 4.0, -1.0, np.array([16.0, 40.0])), "MATLAB:eigs_shift_invert:bad_opts"),
+             ((A,), "MATLAB:eigs_shift_invert:invalidNumInputs"),
+             ((A, 4.0, 2.0 * lam1), "gpumatfun:GM_EBREAKDOWN")]
+    for args, ident in cases:
+        with pytest.raises(harness.MexError) as e:
+            mex("eigs_shift_invert_mex", *args, nlhs=2)
+        assert e.value.ident == ident, (args, e.value)
+    with pytest.raises(harness.MexError) as e:
+        mex("lambda_max_mex", A.toarray(), nlhs=1)
+    assert e.value.ident == "MATLAB:lambda_max:inputNotSparse"
