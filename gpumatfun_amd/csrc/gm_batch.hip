@@ -169,10 +169,16 @@ struct Launch {
   }
 };
 
+// Chunk widths are powers of two >= 2: four columns per thread from 4 up.
 template <int ND>
 void launch_bvc(gm_batch* b, const gmb::BVC& a) {
-  const long long e = (long long)b->n * b->Rc;
-  hipLaunchKernelGGL((gmb::k_bvc<ND>), dim3(cdivi(e, gm::kBlock)), dim3(gm::kBlock), 0, b->stream, a);
+  if (b->Rc >= 4) {
+    const long long e = (long long)b->n * b->Rc / 4;
+    hipLaunchKernelGGL((gmb::k_bvc<ND, 4>), dim3(cdivi(e, gm::kBlock)), dim3(gm::kBlock), 0, b->stream, a);
+  } else {
+    const long long e = (long long)b->n * b->Rc / 2;
+    hipLaunchKernelGGL((gmb::k_bvc<ND, 2>), dim3(cdivi(e, gm::kBlock)), dim3(gm::kBlock), 0, b->stream, a);
+  }
 }
 
 // One step of one chunk of Rc columns starting at column c0.

@@ -576,27 +576,86 @@ struct BVC {
   int accumulate;
 };
 
-template <int ND>
+// Z_j = sum_k V[k] C[j][k] per column.  One element per thread (W = 1), or
+// W adjacent columns per thread (W = 2 or 4: R, the column offset and the row
+// strides are multiples of W) with 16-byte loads, the Krylov slices' loads
+// issued eight at a time.
+template <int ND, int W = 1>
 __global__ __launch_bounds__(kBlock) void k_bvc(BVC p) {
-  const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (e >= (long long)p.nrows * p.R) return;
-  const int c = (int)(e % p.R);
-  const long long i = e / p.R;
-  const size_t slice = (size_t)p.nrows * p.R;
-  double acc[ND];
+  if constexpr (W == 1) {
+    const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= (long long)p.nrows * p.R) return;
+    const int c = (int)(e % p.R);
+    const long long i = e / p.R;
+    const size_t slice = (size_t)p.nrows * p.R;
+    double acc[ND];
 #pragma unroll
-  for (int j = 0; j < ND; ++j) acc[j] = 0.0;
-  for (int k = 0; k < p.m; ++k) {
-    const double v = p.V[(size_t)k * slice + e];
+    for (int j = 0; j < ND; ++j) acc[j] = 0.0;
+    for (int k = 0; k < p.m; ++k) {
+      const double v = p.V[(size_t)k * slice + e];
 #pragma unroll
-    for (int j = 0; j < ND; ++j) acc[j] = fma(v, p.C[((size_t)j * p.m + k) * p.R + c], acc[j]);
-  }
-  if (ND == 1) {
-    const size_t o = (size_t)i * p.ldz + c;
-    p.Z[o] = p.accumulate ? p.Z[o] + acc[0] : acc[0];
+      for (int j = 0; j < ND; ++j) acc[j] = fma(v, p.C[((size_t)j * p.m + k) * p.R + c], acc[j]);
+    }
+    if (ND == 1) {
+      const size_t o = (size_t)i * p.ldz + c;
+      p.Z[o] = p.accumulate ? p.Z[o] + acc[0] : acc[0];
+    } else {
+#pragma unroll
+      for (int j = 0; j < ND; ++j) p.Z[(size_t)j * slice + e] = acc[j];
+    }
   } else {
+    static_assert(W == 2 || W == 4, "W is 1, 2 or 4");
+    constexpr int H = W / 2;   // double2 loads per slice
+    const long long e = (long long)W * ((long long)blockIdx.x * kBlock + threadIdx.x);
+    if (e >= (long long)p.nrows * p.R) return;
+    const int c = (int)(e % p.R);
+    const long long i = e / p.R;
+    const size_t slice = (size_t)p.nrows * p.R;
+    double2 acc[ND][H];
 #pragma unroll
-    for (int j = 0; j < ND; ++j) p.Z[(size_t)j * slice + e] = acc[j];
+    for (int j = 0; j < ND; ++j)
+#pragma unroll
+      for (int h = 0; h < H; ++h) acc[j][h] = d2(0.0, 0.0);
+    constexpr int KB = 8;
+    for (int k0 = 0; k0 < p.m; k0 += KB) {
+      double2 v[KB][H];
+#pragma unroll
+      for (int t = 0; t < KB; ++t)
+#pragma unroll
+        for (int h = 0; h < H; ++h)
+          v[t][h] = k0 + t < p.m ? *reinterpret_cast<const double2*>(
+                                       p.V + (size_t)(k0 + t) * slice + e + 2 * h)
+                                 : d2(0.0, 0.0);
+#pragma unroll
+      for (int t = 0; t < KB; ++t) {
+        if (k0 + t >= p.m) break;
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+#pragma unroll
+          for (int h = 0; h < H; ++h) {
+            const double2 cc = *reinterpret_cast<const double2*>(
+                p.C + ((size_t)j * p.m + k0 + t) * p.R + c + 2 * h);
+            acc[j][h].x = fma(v[t][h].x, cc.x, acc[j][h].x);
+            acc[j][h].y = fma(v[t][h].y, cc.y, acc[j][h].y);
+          }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      if (ND == 1) {
+        double2* o = reinterpret_cast<double2*>(p.Z + (size_t)i * p.ldz + c + 2 * h);
+        if (p.accumulate) {
+          const double2 z = *o;
+          *o = d2(z.x + acc[0][h].x, z.y + acc[0][h].y);
+        } else {
+          *o = acc[0][h];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+          *reinterpret_cast<double2*>(p.Z + (size_t)j * slice + e + 2 * h) = acc[j][h];
+      }
+    }
   }
 }
 

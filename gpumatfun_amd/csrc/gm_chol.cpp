@@ -30,6 +30,8 @@
 // Every reduction runs in an order fixed by the matrix alone (the subtree cut
 // and the row chunks of the tall products do not depend on the thread count),
 // so the results are bitwise independent of the number of threads.
+#include <sched.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -64,10 +66,19 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// threads <= 0: the CPUs this process may run on (its affinity mask), capped
+// by OMP_NUM_THREADS when that is set and by 64.
 int resolve_threads(int t) {
   if (t > 0) return std::min(t, 256);
-  const unsigned h = std::thread::hardware_concurrency();
-  return h ? (int)std::min(h, 64u) : 1;
+  int n = 0;
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+  if (n <= 0) n = (int)std::thread::hardware_concurrency();
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int k = std::atoi(e);
+    if (k > 0) n = std::min(n > 0 ? n : k, k);
+  }
+  return std::max(1, std::min(n, 64));
 }
 
 // body(i, tid) for i in [0, count) on up to nthreads threads, dynamic order.
