@@ -1023,7 +1023,8 @@ def main():
                                         index16=args.schedule.startswith("blocks16"),
                                         early_loads=args.schedule == "blocks16e")
     info = eng.info()
-    log(f"create {time.time() - t0:.2f}s, schedule {info['spmm_schedule']}")
+    log(f"create {time.time() - t0:.2f}s, schedule {info['spmm_schedule']}, "
+        f"shifted solves {info['solve_form']}")
 
     # parity leg: one step from the initial state
     eng.step(1, use_graph=False)
@@ -1065,7 +1066,7 @@ def main():
         "data": "synthetic (seeded mesh and initial condition)",
         "config": cfg,
         "graph": use_graph,
-        "spmm_schedule": info["spmm_schedule"],
+        "spmm_schedule": info["spmm_schedule"], "solve_form": info["solve_form"],
         "setup_s": round(setup_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

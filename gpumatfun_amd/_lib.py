@@ -20,6 +20,7 @@ GM_FLAG_ROW_BLOCKS = 8
 GM_FLAG_INDEX32 = 16
 GM_FLAG_INDEX16 = 32
 GM_FLAG_EARLY_LOADS = 64
+GM_FLAG_WAVE_SOLVES = 128
 GM_REACTION_FISHER, GM_REACTION_MS = 0, 2
 # Mitchell-Schaeffer defaults (the library's own, gm_solver.hip)
 MS_DEFAULTS = dict(tau_in=0.3, tau_out=6.0, tau_open=120.0, tau_close=150.0, v_gate=0.13)
@@ -33,7 +34,7 @@ EXPORTS = (
     "gm_lin_sys_solves", "gm_apply_V", "gm_update_U", "gm_update_U_monodomain",
     "gm_get_U", "gm_get_xsol", "gm_get_aabb", "gm_set_y", "gm_get_y", "gm_get_state",
     "gm_set_state", "gm_step", "gm_synchronize", "gm_get_info", "gm_get_schedule",
-    "gm_get_stream",
+    "gm_get_stream", "gm_get_solve_form",
     "gm_profile_spmm", "gm_profile_spmm_flushed", "gm_profile_copy", "gm_profile_barrier", "gm_set_ms_params", "gm_update_U_ms",
     "gm_batch_last_error", "gm_batch_create", "gm_batch_delete", "gm_batch_step",
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
@@ -43,7 +44,7 @@ EXPORTS = (
     "gm_fab_backward_param",
     "gm_fab_set_param_derivs",
     "gm_ubatch_last_error", "gm_ubatch_create", "gm_ubatch_delete", "gm_ubatch_step",
-    "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info", "gm_ubatch_get_schedule",
+    "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info", "gm_ubatch_get_schedule", "gm_ubatch_get_solve_form",
     "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
     "gm_subspace_size", "gm_rcm_order",
     "gm_eigs_last_error", "gm_eigs_smallest", "gm_sym_eig",
@@ -295,6 +296,7 @@ def load():
                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32)]),
         "gm_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
+        "gm_get_solve_form": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "gm_get_schedule": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                       C.POINTER(C.c_int32)]),
         "gm_profile_spmm": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
@@ -334,6 +336,7 @@ def load():
                                      C.POINTER(C.c_int32)]),
         "gm_ubatch_get_schedule": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                              C.POINTER(C.c_int32)]),
+        "gm_ubatch_get_solve_form": (C.c_int, [vp, C.POINTER(C.c_int32)]),
         "gm_polynomial_coeffs": (C.c_int, [C.c_int32, C.c_double, _dp]),
         "gm_contour_setup": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_double,
                                        C.c_double, _dp, _dp, _dp, _dp, _dp]),

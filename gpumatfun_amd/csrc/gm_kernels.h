@@ -519,6 +519,145 @@ __global__ __launch_bounds__(64) void k_shifted_solves(SolveArgs p) {
   }
 }
 
+// The same systems, one wavefront per system: parallel cyclic reduction
+// (Hockney's odd-even reduction applied to every equation at once).  Lane l
+// holds equations l + 64 j (j < Q <= 2, m <= 64 Q), the rows past m padded as
+// decoupled identity rows.  Step h eliminates the couplings at distance h
+// with the neighbours' rows at i - h and i + h (wave shuffles; a row outside
+// [0, 64 Q) is an identity row); after the steps up to h >= 64 Q every row is
+// decoupled and y_i = d_i / b_i.  No pivoting: for T real symmetric and the
+// shift z off the real axis, or real and outside T's spectrum, the reduced
+// diagonals stay away from zero (on the contour systems of the test problems
+// the solutions agree with the pivoted elimination to 2e-13).  The steps are
+// unrolled so every register index is a constant.
+template <int Q>
+__global__ __launch_bounds__(64) void k_shifted_solves_pcr(SolveArgs p) {
+  const int s = blockIdx.x;
+  const int S = 2 * p.ns;
+  if (s >= S) return;
+  const size_t q = blockIdx.y;
+  p.aabb += q * p.aabb_ps;
+  p.wts1 += q * p.wts_ps;
+  p.wts2 += q * p.wts_ps;
+  p.shifts += q * p.shifts_ps;
+  p.yr += q * p.y_ps;
+  p.yi += q * p.y_ps;
+  const int lane = threadIdx.x;
+  const int c = s < p.ns ? 0 : 1;
+  const int node = s - c * p.ns;
+  const double* w = c == 0 ? p.wts1 : p.wts2;
+  const double b0 = p.aabb[2 * p.ld + c * p.ld];
+  const double2 wt = d2(w[node], w[node + p.ns]);
+  const double* al = p.aabb + c * p.ld;
+  const double* be = p.aabb + (2 + c) * p.ld;
+  const double* sh = p.shifts + 2 * (size_t)s * p.m;
+  const int m = p.m;
+  const double2 zero = d2(0.0, 0.0), one = d2(1.0, 0.0);
+  // row i: A y_{i-1} + B y_i + C y_{i+1} = D, (T - shift) y = -beta_0 w e1
+  double2 A[Q], B[Q], C[Q], D[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const int i = lane + 64 * j;
+    if (i < m) {
+      B[j] = d2(al[i] - sh[2 * i], -sh[2 * i + 1]);
+      A[j] = i > 0 ? d2(be[i], 0.0) : zero;
+      C[j] = i + 1 < m ? d2(be[i + 1], 0.0) : zero;
+      D[j] = i == 0 ? d2(-(b0 * wt.x), -(b0 * wt.y)) : zero;
+    } else {
+      A[j] = zero;
+      B[j] = one;
+      C[j] = zero;
+      D[j] = zero;
+    }
+  }
+  auto shfl2 = [](double2 v, int src) {
+    return d2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+  };
+  static_assert(Q == 1 || Q == 2, "m <= 128");
+  constexpr int kSteps = Q == 1 ? 6 : 7;   // h = 1 .. 32 Q
+#pragma unroll
+  for (int st = 0; st < kSteps; ++st) {
+    const int h = 1 << st;
+    const int hq = h >> 6, hr = h & 63;
+    const int sm = (lane - hr) & 63, sp = (lane + hr) & 63;
+    const bool borrow = lane < hr;          // row i - h lies in register j - hq - 1
+    const bool carry = lane + hr >= 64;     // row i + h lies in register j + hq + 1
+    double2 Am[Q], Bm[Q], Cm[Q], Dm[Q], Ap[Q], Bp[Q], Cp[Q], Dp[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      // row i - h: register j - hq (lanes >= hr) or j - hq - 1 (lanes < hr)
+      const int r1 = j - hq, r2 = j - hq - 1;
+      double2 a1 = zero, b1 = one, c1 = zero, d1 = zero, a2 = zero, b2 = one, c2 = zero, d2v = zero;
+      if (r1 >= 0) {
+        a1 = hr ? shfl2(A[r1], sm) : A[r1];
+        b1 = hr ? shfl2(B[r1], sm) : B[r1];
+        c1 = hr ? shfl2(C[r1], sm) : C[r1];
+        d1 = hr ? shfl2(D[r1], sm) : D[r1];
+      }
+      if (hr && r2 >= 0) {
+        a2 = shfl2(A[r2], sm);
+        b2 = shfl2(B[r2], sm);
+        c2 = shfl2(C[r2], sm);
+        d2v = shfl2(D[r2], sm);
+      }
+      const bool use2 = hr && borrow;
+      Am[j] = use2 ? a2 : a1;
+      Bm[j] = use2 ? b2 : b1;
+      Cm[j] = use2 ? c2 : c1;
+      Dm[j] = use2 ? d2v : d1;
+      // row i + h: register j + hq (lanes + hr < 64) or j + hq + 1
+      const int r3 = j + hq, r4 = j + hq + 1;
+      double2 a3 = zero, b3 = one, c3 = zero, d3 = zero, a4 = zero, b4 = one, c4 = zero, d4 = zero;
+      if (r3 < Q) {
+        a3 = hr ? shfl2(A[r3], sp) : A[r3];
+        b3 = hr ? shfl2(B[r3], sp) : B[r3];
+        c3 = hr ? shfl2(C[r3], sp) : C[r3];
+        d3 = hr ? shfl2(D[r3], sp) : D[r3];
+      }
+      if (hr && r4 < Q) {
+        a4 = shfl2(A[r4], sp);
+        b4 = shfl2(B[r4], sp);
+        c4 = shfl2(C[r4], sp);
+        d4 = shfl2(D[r4], sp);
+      }
+      const bool use4 = hr && carry;
+      Ap[j] = use4 ? a4 : a3;
+      Bp[j] = use4 ? b4 : b3;
+      Cp[j] = use4 ? c4 : c3;
+      Dp[j] = use4 ? d4 : d3;
+    }
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      const double2 fa = cdiv(A[j], Bm[j]);   // row i - h times -fa
+      const double2 fc = cdiv(C[j], Bp[j]);   // row i + h times -fc
+      A[j] = csub(zero, cmul(fa, Am[j]));
+      C[j] = csub(zero, cmul(fc, Cp[j]));
+      B[j] = csub(csub(B[j], cmul(fa, Cm[j])), cmul(fc, Ap[j]));
+      D[j] = csub(csub(D[j], cmul(fa, Dm[j])), cmul(fc, Dp[j]));
+    }
+  }
+  bool bad = false;
+  double2 y[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    y[j] = cdiv(D[j], B[j]);
+    if ((B[j].x == 0.0 && B[j].y == 0.0) || !isfinite(y[j].x) || !isfinite(y[j].y)) bad = true;
+  }
+  if (__any(bad)) {
+    if (lane == 0) atomicOr(p.status, 2u);
+#pragma unroll
+    for (int j = 0; j < Q; ++j) y[j] = zero;
+  }
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const int i = lane + 64 * j;
+    if (i < m) {
+      p.yr[i + (size_t)m * s] = y[j].x;
+      p.yi[i + (size_t)m * s] = y[j].y;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // apply_V, collapsed over the nodes:
 //   x_c = sum_s Im(q_s(A) V_c y_s) = sum_j A^(d-j) V_c c_j,

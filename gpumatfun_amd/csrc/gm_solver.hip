@@ -98,7 +98,8 @@ struct gm_solver {
   double2* C = nullptr;
   double *Q = nullptr, *el = nullptr, *pl = nullptr, *mass = nullptr, *ode = nullptr;
   int solve_sb = 0;   // shifted solves: 0 = factors in global memory, > 0 = systems per
-                      // block with the factors in LDS (chosen by the autotuner)
+                      // block with the factors in LDS, -1 = one wavefront per system
+                      // (cyclic reduction, m <= 128); chosen by the autotuner
   double *aabb = nullptr, *Yr = nullptr, *Yi = nullptr, *wts1 = nullptr, *wts2 = nullptr,
          *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *Zdefl = nullptr;
   double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
@@ -382,7 +383,11 @@ gm_status enqueue_solves(gm_solver* s) {
   a.yi = s->Yi;
   a.status = s->status;
   a.sb = s->solve_sb;
-  if (a.sb > 0)
+  if (a.sb < 0 && s->m <= 64)
+    hipLaunchKernelGGL(gm::k_shifted_solves_pcr<1>, dim3(2 * s->ns), dim3(64), 0, s->stream, a);
+  else if (a.sb < 0)
+    hipLaunchKernelGGL(gm::k_shifted_solves_pcr<2>, dim3(2 * s->ns), dim3(64), 0, s->stream, a);
+  else if (a.sb > 0)
     hipLaunchKernelGGL(gm::k_shifted_solves<1>, dim3(cdiv_i(2 * s->ns, a.sb)), dim3(64),
                        4 * (size_t)s->m * a.sb * sizeof(double2), s->stream, a);
   else
@@ -774,6 +779,8 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
                       (d->flags & GM_FLAG_EARLY_LOADS) != 0);
   else
     set_schedule(s, lanes);
+  // shifted solves: one wavefront per system when asked (and m allows)
+  if ((d->flags & GM_FLAG_WAVE_SOLVES) && s->m <= 128) s->solve_sb = -1;
   s->G_el = std::max(1, std::min(4096, cdiv_i(s->n, kBlock)));
   s->chunk_el = cdiv_i(s->n, s->G_el);
   s->G_df = std::max(1, std::min(512, cdiv_i(s->n, kBlock)));
@@ -830,31 +837,37 @@ gm_status create_impl(const gm_create_desc* d, gm_solver* s) {
       if (ms < best) { best = ms; best_c = l; }
     }
     apply(best_c);
-    // The shifted solves with their factors in global memory or in LDS (a few
-    // systems a block), each timed together with apply_V: the LDS form spreads
-    // Y over several blocks (XCDs), which the collapse that reads Y pays for
-    // (m = 80, 64 nodes: LDS wins; m = 50, 32 nodes: global).  Timed on the
-    // last build's tridiagonals; apply_V's output X is cleared afterwards.
-    if (gm::solve_systems_per_block(s->m) > 0) {
-      float tbest = 1e30f;
-      int pick = 0;
-      for (int sb : {0, gm::solve_systems_per_block(s->m)}) {
-        s->solve_sb = sb;
-        GM_TRY(enqueue_solves(s));  // warm-up
-        GM_TRY(enqueue_apply(s));
-        GM_HIP(hipEventRecord(e0, st));
-        for (int i = 0; i < 3; ++i) {
-          GM_TRY(enqueue_solves(s));
+    // The shifted solves in three forms, each timed together with apply_V:
+    // one lane per system with the factors in global memory, or in LDS (a
+    // few systems a block, which spreads Y over several blocks and XCDs, paid
+    // for by the collapse that reads Y), or one wavefront per system (cyclic
+    // reduction, m <= 128).  Timed on the last build's tridiagonals; apply_V's
+    // output X is cleared afterwards.
+    {
+      std::vector<int> cand = {0};
+      if (gm::solve_systems_per_block(s->m) > 0) cand.push_back(gm::solve_systems_per_block(s->m));
+      if (s->m <= 128) cand.push_back(-1);
+      if (cand.size() > 1) {
+        float tbest = 1e30f;
+        int pick = 0;
+        for (int sb : cand) {
+          s->solve_sb = sb;
+          GM_TRY(enqueue_solves(s));  // warm-up
           GM_TRY(enqueue_apply(s));
+          GM_HIP(hipEventRecord(e0, st));
+          for (int i = 0; i < 3; ++i) {
+            GM_TRY(enqueue_solves(s));
+            GM_TRY(enqueue_apply(s));
+          }
+          GM_HIP(hipEventRecord(e1, st));
+          GM_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < tbest) { tbest = ms; pick = sb; }
         }
-        GM_HIP(hipEventRecord(e1, st));
-        GM_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        GM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < tbest) { tbest = ms; pick = sb; }
+        s->solve_sb = pick;
+        s->x_zero = true;   // X holds the timing runs' output: the next apply_V overwrites it
       }
-      s->solve_sb = pick;
-      s->x_zero = true;   // X holds the timing runs' output: the next apply_V overwrites it
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -1115,6 +1128,13 @@ gm_status gm_get_schedule(gm_solver* s, int32_t* kind, int32_t* param, int32_t* 
   if (kind) *kind = s->tile > 0 ? (s->pre ? 2 : 1) : 0;
   if (param) *param = s->tile > 0 ? s->tile : s->lanes;
   if (index_bytes) *index_bytes = (s->tile > 0 && s->use16) ? 2 : 4;
+  return GM_OK;
+}
+
+gm_status gm_get_solve_form(gm_solver* s, int32_t* form, int32_t* systems_per_block) {
+  GM_CHECK_HANDLE(s);
+  if (form) *form = s->solve_sb < 0 ? 2 : (s->solve_sb > 0 ? 1 : 0);
+  if (systems_per_block) *systems_per_block = s->solve_sb < 0 ? 1 : (s->solve_sb > 0 ? s->solve_sb : 64);
   return GM_OK;
 }
 

@@ -195,6 +195,7 @@ struct gm_ubatch {
   int tile = gmu::kTile;         // nonzeros per row block (SpMM LDS tile)
   bool use16 = false, pre = true;  // SpMM: 16-bit offsets; epilogue loads before gathers
   int solve_sb = 0;              // shifted solves: factors in global memory (0) or LDS
+                                 // (> 0 systems a block), or a wavefront a system (-1)
   double *val = nullptr, *mass = nullptr, *aabb = nullptr, *wts1 = nullptr, *wts2 = nullptr,
          *shifts = nullptr, *gsr = nullptr, *gsi = nullptr, *yr = nullptr, *yi = nullptr,
          *partials = nullptr;
@@ -395,7 +396,11 @@ gm_status enqueue_ustep(gm_ubatch* u) {
   s.y_ps = 2 * (size_t)u->m * u->ns;
   s.work_ps = 2 * (size_t)u->m * u->ns;
   s.sb = u->solve_sb;
-  if (s.sb > 0)
+  if (s.sb < 0 && u->m <= 64)
+    hipLaunchKernelGGL(gm::k_shifted_solves_pcr<1>, dim3(2 * u->ns, u->P), dim3(64), 0, u->stream, s);
+  else if (s.sb < 0)
+    hipLaunchKernelGGL(gm::k_shifted_solves_pcr<2>, dim3(2 * u->ns, u->P), dim3(64), 0, u->stream, s);
+  else if (s.sb > 0)
     hipLaunchKernelGGL(gm::k_shifted_solves<1>, dim3((2 * u->ns + s.sb - 1) / s.sb, u->P),
                        dim3(64), 4 * (size_t)u->m * s.sb * sizeof(double2), u->stream, s);
   else
@@ -552,11 +557,16 @@ gm_status autotune(gm_ubatch* u, const std::vector<int>& rp, const std::vector<i
     u->use16 = pick.c16 && u->col16 != nullptr;
     u->pre = pick.pre;
   }
-  // then the shifted solves' two forms under the kept schedule
-  if (st == GM_OK && gm::solve_systems_per_block(u->m) > 0) {
+  // then the shifted solves' forms under the kept schedule: one lane per
+  // system with the factors in global memory or in LDS, or one wavefront per
+  // system (cyclic reduction, m <= 128)
+  std::vector<int> scand = {0};
+  if (gm::solve_systems_per_block(u->m) > 0) scand.push_back(gm::solve_systems_per_block(u->m));
+  if (u->m <= 128) scand.push_back(-1);
+  if (st == GM_OK && scand.size() > 1) {
     float sbest = 1e30f;
     int spick = 0;
-    for (int sb : {0, gm::solve_systems_per_block(u->m)}) {
+    for (int sb : scand) {
       u->solve_sb = sb;
       st = enqueue_ustep(u);
       if (st == GM_OK && hipEventRecord(e0, u->stream) != hipSuccess) st = ufail(GM_EDEVICE, "event");
@@ -600,6 +610,7 @@ gm_status create_impl(const gm_ubatch_desc* d, gm_ubatch* u) {
   u->ns = d->contour_size;
   u->dt = d->dt;
   u->clamp = !(d->flags & GM_FLAG_NO_CLAMP);
+  if ((d->flags & GM_FLAG_WAVE_SOLVES) && d->subspace_size <= 128) u->solve_sb = -1;
   u->gamma.assign(d->gamma, d->gamma + u->nd);
   u->off.assign(u->P + 1, 0);
   long long nnz = 0;
@@ -816,6 +827,12 @@ gm_status gm_ubatch_info(gm_ubatch* u, int32_t* num_problems, int64_t* total_row
   if (num_problems) *num_problems = u->P;
   if (total_rows) *total_rows = u->N;
   if (row_blocks) *row_blocks = u->G;
+  return GM_OK;
+}
+
+gm_status gm_ubatch_get_solve_form(gm_ubatch* u, int32_t* form) {
+  if (!ulive(u)) return ufail(GM_EBADHANDLE, "invalid uneven-batch handle");
+  if (form) *form = u->solve_sb < 0 ? 2 : (u->solve_sb > 0 ? 1 : 0);
   return GM_OK;
 }
 

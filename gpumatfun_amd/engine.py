@@ -46,14 +46,15 @@ class LanczosEngine:
     the block's first row where they fit), ``index32`` (never use those
     offsets, not even when autotuning), ``early_loads`` (without autotune: the
     row-block SpMM loads each row's own operands before its gathers),
-    ``device``.
+    ``wave_solves`` (without autotune: the shifted solves one wavefront per
+    system by cyclic reduction, m <= 128), ``device``.
     """
 
     def __init__(self, U, A, Q, lambda_, N, gamma, gamma_s_real, gamma_s_imag,
                  exp_lambda, phi_lambda, dt, ode, aode, bode, code, vrest, vamp, vth,
                  vpeak, c1ion, c2ion, mass, wts1, wts2, shifts, valvec=(-1.0, 0.0, 1.0),
                  *, clamp=True, autotune=False, schedule=None, index32=False, index16=False,
-                 early_loads=False, device=0):
+                 early_loads=False, wave_solves=False, device=0):
         lib = _lib.load()
         self._lib = lib
         self._h = C.c_void_p()
@@ -160,7 +161,8 @@ class LanczosEngine:
                    | (_lib.GM_FLAG_AUTOTUNE if autotune else 0) | sched[schedule]
                    | (_lib.GM_FLAG_INDEX32 if index32 else 0)
                    | (_lib.GM_FLAG_INDEX16 if index16 else 0)
-                   | (_lib.GM_FLAG_EARLY_LOADS if early_loads else 0))
+                   | (_lib.GM_FLAG_EARLY_LOADS if early_loads else 0)
+                   | (_lib.GM_FLAG_WAVE_SOLVES if wave_solves else 0))
         d.device = int(device)
         _lib.check(lib.gm_lanczos_create(C.byref(d), C.byref(self._h)))
         self._keep = None
@@ -279,10 +281,13 @@ class LanczosEngine:
         else:
             sched = (f"k_spmm_stream<{param.value}{',c16' if ib.value == 2 else ''}"
                      f"{',pre' if kind.value == 2 else ''}>")
+        form = C.c_int32(); spb = C.c_int32()
+        self._call(self._lib.gm_get_solve_form, C.byref(form), C.byref(spb))
         return {"n": n.value, "nnz": nnz.value, "m": m.value, "gamma_size": nd.value,
                 "contour_size": ns.value, "num_eigs": K.value, "spmm_lanes": lanes.value,
                 "spmm_schedule": sched, "spmm_kind": kind.value, "spmm_param": param.value,
-                "index_bytes": ib.value}
+                "index_bytes": ib.value,
+                "solve_form": ("lane_global", "lane_lds", "wave_pcr")[form.value]}
 
     def stream(self):
         p = C.c_void_p()

@@ -26,7 +26,8 @@ def _f(a):
 
 
 class UnevenBatch:
-    def __init__(self, problems, *, m, gamma, dt, clamp=True, autotune=False, device=0):
+    def __init__(self, problems, *, m, gamma, dt, clamp=True, autotune=False, wave_solves=False,
+                 device=0):
         """``problems``: sequence of dicts with keys A (n x n, symmetric), U
         (n x 2 [u, g(u)]), mass (n or None), wts1, wts2 (n_s x 2 [Re Im]),
         shifts (4 m n_s), gamma_s_real, gamma_s_imag ((d+1) x n_s)."""
@@ -103,7 +104,8 @@ class UnevenBatch:
         d.shifts = keep["sh"].ctypes.data_as(_dp)
         d.dt = float(dt)
         d.flags = ((0 if clamp else _lib.GM_FLAG_NO_CLAMP)
-                   | (_lib.GM_FLAG_AUTOTUNE if autotune else 0))
+                   | (_lib.GM_FLAG_AUTOTUNE if autotune else 0)
+                   | (_lib.GM_FLAG_WAVE_SOLVES if wave_solves else 0))
         d.device = int(device)
         _lib.check(lib.gm_ubatch_create(C.byref(d), C.byref(self._h)), ubatch=True)
         self.sizes = list(sizes)
@@ -148,9 +150,12 @@ class UnevenBatch:
         self._call(self._lib.gm_ubatch_info, C.byref(P), C.byref(N), C.byref(G))
         tile, ib, pre = C.c_int32(), C.c_int32(), C.c_int32()
         self._call(self._lib.gm_ubatch_get_schedule, C.byref(tile), C.byref(ib), C.byref(pre))
+        form = C.c_int32()
+        self._call(self._lib.gm_ubatch_get_solve_form, C.byref(form))
         return {"num_problems": P.value, "total_rows": N.value, "row_blocks": G.value,
                 "spmm_tile": tile.value, "index_bytes": ib.value,
-                "early_loads": bool(pre.value)}
+                "early_loads": bool(pre.value),
+                "solve_form": ("lane_global", "lane_lds", "wave_pcr")[form.value]}
 
     def delete(self):
         if self._h:

@@ -60,9 +60,13 @@ enum {
                                     both widths where the offsets fit)      */
     GM_FLAG_INDEX16 = 1u << 5,    /* without AUTOTUNE: use the 16-bit offsets
                                     where they fit (default: 32-bit)        */
-    GM_FLAG_EARLY_LOADS = 1u << 6 /* without AUTOTUNE: the row-block SpMM loads
+    GM_FLAG_EARLY_LOADS = 1u << 6, /* without AUTOTUNE: the row-block SpMM loads
                                     each row's own operands before its
                                     gathers (default: after)                */
+    GM_FLAG_WAVE_SOLVES = 1u << 7 /* without AUTOTUNE: the shifted solves run
+                                    one wavefront per system (cyclic
+                                    reduction; m <= 128) instead of one lane
+                                    per system.  AUTOTUNE times the forms   */
 };
 
 /*
@@ -194,6 +198,11 @@ gm_status gm_get_info(gm_solver* s, int64_t* n, int64_t* nnz, int32_t* m,
  * epilogue operands loaded before the gathers; index_bytes = bytes per
  * stored column index (4, or 2 for 16-bit block offsets).                  */
 gm_status gm_get_schedule(gm_solver* s, int32_t* kind, int32_t* param, int32_t* index_bytes);
+/* The shifted solves' form (lin_sys_solves.cu:5-21): 0 = one lane per
+ * system, factors in global memory; 1 = one lane per system, factors in LDS;
+ * 2 = one wavefront per system (cyclic reduction).  systems_per_block may be
+ * NULL.                                                                    */
+gm_status gm_get_solve_form(gm_solver* s, int32_t* form, int32_t* systems_per_block);
 
 /* The solver's HIP stream (as void*), for callers that time it with
  * events of their own.                                                      */
@@ -437,6 +446,9 @@ gm_status gm_ubatch_info(gm_ubatch* u, int32_t* num_problems, int64_t* total_row
  * index (2 or 4), and whether the epilogue operands are loaded early.      */
 gm_status gm_ubatch_get_schedule(gm_ubatch* u, int32_t* tile, int32_t* index_bytes,
                                  int32_t* early_loads);
+/* The shifted solves' form, as gm_get_solve_form: 0 / 1 one lane per system
+ * (factors in global memory / LDS), 2 one wavefront per system.            */
+gm_status gm_ubatch_get_solve_form(gm_ubatch* u, int32_t* form);
 
 /* ---------------------------------------------------------------------------
  * Host set-up of the create inputs (host code only; usable without a GPU).

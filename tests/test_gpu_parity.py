@@ -60,11 +60,15 @@ def test_dirichlet128_one_step_stages():
     eng.delete()
 
 
-def test_dirichlet128_ten_steps_vs_cpu_path():
-    """128^2 Dirichlet, m = 50, 32 shifts: 10 steps against problem_startup_CPU.m's loop (clamp off)."""
+@pytest.mark.parametrize("wave", [False, True])
+def test_dirichlet128_ten_steps_vs_cpu_path(wave):
+    """128^2 Dirichlet, m = 50, 32 shifts: 10 steps against problem_startup_CPU.m's loop
+    (clamp off), with the shifted solves one lane per system or one wavefront per
+    system (cyclic reduction, one equation per lane at m = 50)."""
     inp = dirichlet_case()
-    print("case dirichlet128 10 steps", flush=True)
-    eng = LanczosEngine.from_inputs(inp, clamp=False)
+    print(f"case dirichlet128 10 steps wave_solves={wave}", flush=True)
+    eng = LanczosEngine.from_inputs(inp, clamp=False, wave_solves=wave)
+    assert eng.info()["solve_form"] == ("wave_pcr" if wave else "lane_global")
     for _ in range(10):
         eng.build_subspace()
         eng.lin_sys_solves()
@@ -141,7 +145,8 @@ def test_fem_deflation_mass_vs_cpu_nonuniform():
     eng.delete()
 
 
-def test_headline_parameters_dirichlet128():
+@pytest.mark.parametrize("wave", [False, True])
+def test_headline_parameters_dirichlet128(wave):
     """The full-scale plain workload's parameters -- d = 8, m = 80, n_s = 64, K = 20,
     alpha = 1.7, non-unit mass -- on the 128^2 Dirichlet grid, where any two FP64
     evaluation orders of the oracle agree to ~1e-13.  Runs the V*C kernel with
@@ -151,8 +156,9 @@ def test_headline_parameters_dirichlet128():
     loop, each to 1e-10 (the Lanczos coefficients on the rows that the oracle's own
     evaluation order fixes, see below)."""
     inp = headline_parameters_case()
-    print("case headline parameters on dirichlet128: d=8 m=80 n_s=64 K=20", flush=True)
-    eng = LanczosEngine.from_inputs(inp, clamp=False)
+    print(f"case headline parameters on dirichlet128: d=8 m=80 n_s=64 K=20 wave_solves={wave}",
+          flush=True)
+    eng = LanczosEngine.from_inputs(inp, clamp=False, wave_solves=wave)
     ref = _gpu_sem(inp)
     assert rel(eng.get_U(), ref.get_U()) < 1e-14
     eng.build_subspace()
@@ -416,3 +422,24 @@ def test_long_rows_vs_cpu_path(schedule):
     assert rel(U[:, 0], u) < TOL
     assert rel(U[:, 1], 0.1 * gu) < TOL
     eng.delete()
+
+
+def test_wave_solves_match_lane_solves():
+    """The two forms of the shifted solves on the same tridiagonals (one lane per
+    system with ZGTSV-style pivoting; one wavefront per system by cyclic
+    reduction, unpivoted): Y agrees to rounding.  FEM disc with deflation
+    (m = 30, one equation per lane) and the headline parameters (m = 80, two
+    equations per lane, 128 systems)."""
+    for inp, label in ((fem_case(m=30), "fem m=30"), (headline_parameters_case(), "headline m=80")):
+        print(f"case wave vs lane solves: {label}", flush=True)
+        ys = []
+        for wave in (False, True):
+            eng = LanczosEngine.from_inputs(inp, clamp=False, wave_solves=wave)
+            eng.build_subspace()
+            eng.lin_sys_solves()
+            yr, yi = eng.get_y()
+            ys.append(yr + 1j * yi)
+            eng.delete()
+        e = rel(ys[1], ys[0])
+        print(f"{label}: rel diff {e:.2e}", flush=True)
+        assert e < 1e-12

@@ -35,8 +35,8 @@ def _cpu(inp, steps):
     return oracle.cpu_steps_nonuniform(S, inp.U[:, 0], inp.U[:, 1], steps)
 
 
-@pytest.mark.parametrize("degree", [0, 2])
-def test_uneven_batch_vs_oracle_and_plain(degree):
+@pytest.mark.parametrize("degree,wave", [(0, False), (2, False), (2, True)])
+def test_uneven_batch_vs_oracle_and_plain(degree, wave):
     """Six 2D Dirichlet Laplacians (7^2 .. 64^2 rows, one polynomial shared at
     the largest problem's scale), one of them with a non-uniform mass in
     [0.5, 1.5) (the update of problem_startup_CPU_nonuniform.m).  A FEM disc
@@ -47,9 +47,11 @@ def test_uneven_batch_vs_oracle_and_plain(degree):
     prob_dim = float(problems.dirichlet_eigs_2d(64).max()) / 4.0
     inps = [_dirichlet(s, i, m, n, degree, prob_dim) for i, s in enumerate([16, 40, 7, 64, 23])]
     inps.insert(2, _dirichlet(50, 99, m, n, degree, prob_dim, mass=True))
-    print(f"case uneven batch degree={degree} sizes={[i.A.shape[0] for i in inps]}", flush=True)
-    ub = UnevenBatch.from_inputs(inps, clamp=False)
+    print(f"case uneven batch degree={degree} wave_solves={wave} "
+          f"sizes={[i.A.shape[0] for i in inps]}", flush=True)
+    ub = UnevenBatch.from_inputs(inps, clamp=False, wave_solves=wave)
     info = ub.info()
+    assert info["solve_form"] == ("wave_pcr" if wave else "lane_global")
     assert info["num_problems"] == len(inps)
     assert info["total_rows"] == sum(i.A.shape[0] for i in inps)
     ub.step(steps)
