@@ -73,6 +73,28 @@ def spmm_bytes(info, mode):
     return (8 + info["index_bytes"]) * nnz + 4 * (n + 1) + (64 if mode == 2 else 48) * n
 
 
+def fab_step_bytes(n, nnz, m, d, K, columns, R):
+    """Algorithmic bytes of one time step of `columns` f(A)b columns whose SpMMs
+    are shared by R columns at a time: per column m(d+1)(S_A/R + 16n) (the
+    Lanczos SpMMs: matrix share, x gathered and y written) + 32nm (recurrence:
+    v_(k-1), v_k read, v_(k+1) written; one read of V for V*C) + d(S_A/R + 16n)
+    (the collapsed Horner apply) + 24n (output) + 16nK (deflation), with
+    S_A = 12 nnz + 4(n+1)."""
+    SA = 12 * nnz + 4 * (n + 1)
+    per_col = (m * (d + 1) * (SA / R + 16 * n) + 32 * n * m + d * (SA / R + 16 * n)
+               + 24 * n + 16 * n * K)
+    return columns * per_col
+
+
+def step_roofline(bytes_per_step, seconds_per_step, what):
+    """Whole-step roofline: algorithmic bytes of one step over the measured step
+    time, against the HBM peak (for forms without a per-kernel timing entry)."""
+    gbs = bytes_per_step / seconds_per_step / 1e9
+    return {"bound": "hbm", "scope": "whole step: " + what, "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": None, "algorithmic_bytes_per_step": int(bytes_per_step)}
+
+
 def kernel_key(info, mode):
     """Schedule/mode key of the committed counter summaries."""
     return f"{info['spmm_schedule']}/mode{mode}"
@@ -285,7 +307,10 @@ def run_uneven(args, steps, warm, use_graph):
         "plain_same_problems": {"value": round(plain, 3), "unit": "solves/s",
                                 "how": "one plain engine per problem, stepped in turn"},
         "batched_vs_plain": round(value / plain, 3),
-        "roofline": None, "cpu_baseline": None,
+        "roofline": step_roofline(
+            sum(fab_step_bytes(i.A.shape[0], i.A.nnz, i.m, 0, 0, 2, 2) for i in inps),
+            elapsed / steps, "every kernel of a batched step over all problems"),
+        "cpu_baseline": None,
     }
     if not args.no_cpu and not args.profile_only:
         threads = min(16, os.cpu_count() or 1)
@@ -496,7 +521,11 @@ def run_ms512(args, steps, warm, use_graph):
                                 "how": "one plain engine per stimulus, stepped in turn"},
         "batched_vs_plain": round(value / plain, 3),
         "batched_vs_plain_rel_err_step1": rel(U1[:, 0:2], Up1),
-        "roofline": None, "cpu_baseline": None,
+        "roofline": step_roofline(
+            fab_step_bytes(N, int(inp.A.nnz), inp.m, 0, cfg["deflation"], 2 * P,
+                           info["chunk_columns"]),
+            elapsed / steps, "every kernel of a batched step, all stimuli"),
+        "cpu_baseline": None,
     }
     if not args.no_cpu and not args.profile_only:
         import oracle
@@ -1079,6 +1108,10 @@ def main():
                                  "Infinity Cache), timed alone with its launch latency",
                          "avg_launch_us": round(cold_ms * 1e3, 3),
                          "achieved": round(cold, 1), "frac": round(cold / HBM_PEAK_GBS, 4)}},
+        "roofline_step": step_roofline(
+            fab_step_bytes(n, info["nnz"], info["m"], info["gamma_size"] - 1, info["num_eigs"],
+                           2, 2),
+            ms_per_step * 1e-3, "every kernel of one plain step (the SpMM's share is above)"),
         "copy_kernel_gbs": round(copy_bytes / (copy_ms * 1e-3) / 1e9, 1),
         "cpu_baseline": None,
     }
