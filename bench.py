@@ -473,7 +473,8 @@ def run_ms512(args, steps, warm, use_graph):
     inp, U, stims, prm, cfg, setup_s = build_ms512(args)
     N, P = inp.A.shape[0], len(stims)
     log(f"setup {setup_s:.1f}s")
-    b = BatchEngine.from_inputs(inp, U, clamp=False, reaction="ms", ode=np.ones((N, P)), ms=prm)
+    b = BatchEngine.from_inputs(inp, U, clamp=False, reaction="ms", ode=np.ones((N, P)), ms=prm,
+                                column_slabs=args.column_slabs)
     b.step(1, use_graph=False)
     b.synchronize()
     U1 = b.get_U()
@@ -515,7 +516,8 @@ def run_ms512(args, steps, warm, use_graph):
         "warmup": warm, "ms_per_step": round(1000.0 * elapsed / steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded-free stimulus patches)", "config": cfg, "graph": use_graph,
-        "chunk_columns": info["chunk_columns"], "setup_s": round(setup_s, 1),
+        "chunk_columns": info["chunk_columns"], "column_slabs": info["column_slabs"],
+        "setup_s": round(setup_s, 1),
         "plain_same_problems": {"value": round(plain, 3), "unit": "solves/s",
                                 "steps": psteps,
                                 "how": "one plain engine per stimulus, stepped in turn"},
@@ -966,6 +968,8 @@ def main():
                          "(gm_rcm_order, pseudo-peripheral start as symrcm) or SciPy's")
     ap.add_argument("--node-order", default="natural", choices=["natural", "pencil"],
                     help="batch3d: grid nodes in natural (i + n j + n^2 k) or 16 x 16-pencil order")
+    ap.add_argument("--column-slabs", action="store_true",
+                    help="batch3d with --chunk-columns, ms512: the SpMM in column slabs of 32")
     ap.add_argument("--chunk-columns", type=int, default=0,
                     help="batch3d: columns per pass (0: chosen by the create-time autotuner)")
     ap.add_argument("--no-forms", action="store_true",
@@ -1131,11 +1135,13 @@ def run_batch(args, steps, warm, use_graph):
     P = U.shape[1] // 2
     t0 = time.time()
     if args.chunk_columns:
-        b = BatchEngine.from_inputs(inp, U, clamp=False, chunk_columns=args.chunk_columns)
+        b = BatchEngine.from_inputs(inp, U, clamp=False, chunk_columns=args.chunk_columns,
+                                    column_slabs=args.column_slabs)
     else:
         b = BatchEngine.from_inputs(inp, U, clamp=False, autotune=True)
     info = b.info()
-    log(f"batch create {time.time() - t0:.1f}s, chunk columns {info['chunk_columns']}")
+    log(f"batch create {time.time() - t0:.1f}s, chunk columns {info['chunk_columns']}, "
+        f"column slabs {info['column_slabs']}")
     # parity leg: problem 0 after one step
     b.step(1, use_graph=False)
     b.synchronize()
@@ -1180,7 +1186,7 @@ def run_batch(args, steps, warm, use_graph):
         "data": "synthetic (seeded right-hand sides)",
         "config": cfg,
         "graph": use_graph,
-        "chunk_columns": info["chunk_columns"],
+        "chunk_columns": info["chunk_columns"], "column_slabs": info["column_slabs"],
         "setup_s": round(setup_s, 1),
         "plain_same_matrix": {"value": round(plain, 3), "unit": "solves/s",
                               "spmm_schedule": einfo["spmm_schedule"]},

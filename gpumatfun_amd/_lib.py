@@ -21,6 +21,7 @@ GM_FLAG_INDEX32 = 16
 GM_FLAG_INDEX16 = 32
 GM_FLAG_EARLY_LOADS = 64
 GM_FLAG_WAVE_SOLVES = 128
+GM_FLAG_COLUMN_SLABS = 256
 GM_REACTION_FISHER, GM_REACTION_MS = 0, 2
 # Mitchell-Schaeffer defaults (the library's own, gm_solver.hip)
 MS_DEFAULTS = dict(tau_in=0.3, tau_out=6.0, tau_open=120.0, tau_close=150.0, v_gate=0.13)
@@ -38,7 +39,7 @@ EXPORTS = (
     "gm_profile_spmm", "gm_profile_spmm_flushed", "gm_profile_copy", "gm_profile_barrier", "gm_set_ms_params", "gm_update_U_ms",
     "gm_batch_last_error", "gm_batch_create", "gm_batch_delete", "gm_batch_step",
     "gm_batch_get_U", "gm_batch_synchronize", "gm_batch_info", "gm_batch_get_stream",
-    "gm_batch_get_ode",
+    "gm_batch_get_ode", "gm_batch_get_column_slabs",
     "gm_fab_last_error", "gm_fab_create", "gm_fab_delete", "gm_fab_set_matrix",
     "gm_fab_set_weights", "gm_fab_forward", "gm_fab_forward_noalloc", "gm_fab_backward",
     "gm_fab_backward_param",
@@ -46,7 +47,7 @@ EXPORTS = (
     "gm_ubatch_last_error", "gm_ubatch_create", "gm_ubatch_delete", "gm_ubatch_step",
     "gm_ubatch_get_U", "gm_ubatch_synchronize", "gm_ubatch_info", "gm_ubatch_get_schedule", "gm_ubatch_get_solve_form",
     "gm_polynomial_coeffs", "gm_contour_setup", "gm_shift_setup", "gm_eig_weights",
-    "gm_subspace_size", "gm_rcm_order",
+    "gm_subspace_size", "gm_rcm_order", "gm_locality_order",
     "gm_eigs_last_error", "gm_eigs_smallest", "gm_sym_eig",
     "gm_chol_last_error", "gm_nd_order", "gm_chol_factor", "gm_chol_solve", "gm_chol_info",
     "gm_chol_perm", "gm_chol_delete", "gm_eigs_shift_invert", "gm_lambda_max",
@@ -314,6 +315,7 @@ def load():
         "gm_batch_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "gm_batch_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
         "gm_batch_get_ode": (C.c_int, [vp, _dp]),
+        "gm_batch_get_column_slabs": (C.c_int, [vp, C.POINTER(C.c_int32)]),
         "gm_fab_last_error": (C.c_char_p, []),
         "gm_fab_create": (C.c_int, [C.POINTER(GmFabDesc), C.POINTER(vp)]),
         "gm_fab_delete": (C.c_int, [vp]),
@@ -347,6 +349,8 @@ def load():
                                        C.POINTER(C.c_int32)]),
         "gm_rcm_order": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32)]),
+        "gm_locality_order": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                        C.c_int32, C.POINTER(C.c_int32)]),
         "gm_eigs_last_error": (C.c_char_p, []),
         "gm_eigs_smallest": (C.c_int, [C.POINTER(GmEigsDesc), _dp, _dp, C.POINTER(C.c_int32),
                                        _dp]),

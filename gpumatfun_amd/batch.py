@@ -29,7 +29,7 @@ class BatchEngine:
     def __init__(self, A, U, *, m, gamma, gamma_s_real, gamma_s_imag, wts1, wts2, etaxi, dt,
                  mass=None, chunk_columns=0, clamp=True, autotune=False, device=0,
                  Q=None, exp_lambda=None, phi_lambda=None, reaction="fisher", ode=None,
-                 ms=None):
+                 ms=None, column_slabs=False):
         lib = _lib.load()
         self._lib = lib
         self._h = C.c_void_p()
@@ -83,7 +83,8 @@ class BatchEngine:
         d.dt = float(dt)
         d.mass = _p(keep["mass"]) if keep["mass"] is not None else None
         d.chunk_columns = int(chunk_columns)
-        d.flags = (0 if clamp else _lib.GM_FLAG_NO_CLAMP) | (_lib.GM_FLAG_AUTOTUNE if autotune else 0)
+        d.flags = ((0 if clamp else _lib.GM_FLAG_NO_CLAMP) | (_lib.GM_FLAG_AUTOTUNE if autotune else 0)
+                   | (_lib.GM_FLAG_COLUMN_SLABS if column_slabs else 0))
         d.device = int(device)
         K = 0 if Q is None else np.asarray(Q).reshape(n, -1).shape[1]
         if K:
@@ -148,7 +149,9 @@ class BatchEngine:
         P = C.c_int32()
         rc = C.c_int32()
         self._call(self._lib.gm_batch_info, C.byref(P), C.byref(rc))
-        return {"num_problems": P.value, "chunk_columns": rc.value}
+        xs = C.c_int32()
+        self._call(self._lib.gm_batch_get_column_slabs, C.byref(xs))
+        return {"num_problems": P.value, "chunk_columns": rc.value, "column_slabs": bool(xs.value)}
 
     def stream(self):
         p = C.c_void_p()

@@ -82,6 +82,12 @@ struct gm_batch {
   std::vector<double> gamma;
   int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr;
   int nrowblk = 0;                  // LDS-staged SpMM row blocks
+  // XCD-slab SpMM of 256-column chunks (gmb::k_bspmm_xs): its row-block
+  // descriptors (4 ints each), grid (a multiple of 8) and whether the step
+  // uses it
+  int* rowblk_xs = nullptr;
+  int nrowblk_xs = 0, G_xs = 8;
+  bool xs = false;
   double *val = nullptr, *Usol = nullptr, *X = nullptr, *W0 = nullptr, *W1 = nullptr, *T0 = nullptr,
          *T1 = nullptr, *V = nullptr, *Z = nullptr, *C = nullptr, *al = nullptr, *be = nullptr,
          *yr = nullptr, *yi = nullptr, *gsr = nullptr, *gsi = nullptr, *wts1 = nullptr,
@@ -117,7 +123,7 @@ gm_status balloc(T** p, size_t count) {
 
 void bfree_all(gm_batch* b) {
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
-  void* bufs[] = {b->rowptr, b->col, b->rowblk, b->val, b->Usol, b->X, b->W0, b->W1, b->T0, b->T1, b->V,
+  void* bufs[] = {b->rowptr, b->col, b->rowblk, b->rowblk_xs, b->val, b->Usol, b->X, b->W0, b->W1, b->T0, b->T1, b->V,
                   b->Z, b->C, b->al, b->be, b->yr, b->yi, b->gsr, b->gsi, b->wts1, b->wts2,
                   b->etaxi, b->mass, b->partials, b->stage, b->wD, b->wDU, b->wDU2, b->wX,
                   b->status, b->Q, b->el, b->pl, b->Zd, b->Ud, b->upd, b->ode};
@@ -141,11 +147,66 @@ gm_status breduce(gm_batch* b, int nv, double* out, int do_sqrt, int flag_zero, 
   return GM_OK;
 }
 
+// Workgroups of the slab SpMM: as many as are resident at once (the fewest
+// that any of its instantiations allows per CU), so that each slab's
+// workgroups sweep the rows together as one front; a multiple of 8.
+template <int R>
+void xs_occupancy(int* per_cu) {
+  auto occ = [&](const void* k) {
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k, gm::kBlock, 0) == hipSuccess && v > 0)
+      *per_cu = std::min(*per_cu, v);
+  };
+  constexpr bool nt = R >= 128;
+  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kFirst, nt>));
+  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kMid, nt>));
+  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kLast, nt>));
+  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kOnly, nt>));
+  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kHorner, nt>));
+}
+
+int xs_grid(int dev, int nrowblk) {
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    ncu = 8;
+  int per_cu = 1 << 30;
+  xs_occupancy<32>(&per_cu);
+  xs_occupancy<64>(&per_cu);
+  xs_occupancy<128>(&per_cu);
+  xs_occupancy<256>(&per_cu);
+  if (per_cu == (1 << 30)) per_cu = 1;
+  return 8 * std::max(1, std::min(nrowblk, std::max(1, ncu / 8) * per_cu));
+}
+
+// Rows of block partials that a kLast/kOnly SpMM of a chunk of R columns
+// leaves for k_reduce.
+int spmm_partial_rows(const gm_batch* b, int R) {
+  return (R >= 32 && b->xs) ? b->G_xs / (R / 32) : b->G_sp;
+}
+
 template <int R>
 struct Launch {
   // chunks of >= 128 columns stream v_k, v_{k-1} and y non-temporally
   static constexpr bool kNT = R >= 128;
-  static gm_status spmm(gm_batch* b, int mode, const gmb::BSpmm& a) {
+  static gm_status spmm(gm_batch* b, int mode, const gmb::BSpmm& a0) {
+    if constexpr (R >= 32) {
+      if (b->xs) {
+        gmb::BSpmm a = a0;
+        a.rowblk = b->rowblk_xs;
+        a.nrowblk = b->nrowblk_xs;
+        dim3 g(b->G_xs), t(gm::kBlock);
+        switch (mode) {
+          case gmb::kFirst: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kFirst, kNT>), g, t, 0, b->stream, a); break;
+          case gmb::kMid: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kMid, kNT>), g, t, 0, b->stream, a); break;
+          case gmb::kLast: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kLast, kNT>), g, t, 0, b->stream, a); break;
+          case gmb::kOnly: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kOnly, kNT>), g, t, 0, b->stream, a); break;
+          default: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kHorner, kNT>), g, t, 0, b->stream, a); break;
+        }
+        B_HIP(hipGetLastError());
+        return GM_OK;
+      }
+    }
+    const gmb::BSpmm& a = a0;
     dim3 g(b->G_sp), t(gm::kBlock);
     switch (mode) {
       case gmb::kFirst: hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kFirst, true, kNT>), g, t, 0, b->stream, a); break;
@@ -230,7 +291,7 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
       a.y = wnext;
       a.ldy = R;
       B_TRY(Launch<R>::spmm(b, gmb::kOnly, a));
-      B_TRY(breduce(b, R, b->al + (size_t)k * R, 0, 0));                // alpha_k
+      B_TRY(breduce(b, R, b->al + (size_t)k * R, 0, 0, spmm_partial_rows(b, R)));   // alpha_k
     } else {
       a.y = Tb[0];
       a.ldy = R;
@@ -247,7 +308,7 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
       a.ldx = R;
       a.y = wnext;
       B_TRY(Launch<R>::spmm(b, gmb::kLast, a));
-      B_TRY(breduce(b, R, b->al + (size_t)k * R, 0, 0));                // alpha_k
+      B_TRY(breduce(b, R, b->al + (size_t)k * R, 0, 0, spmm_partial_rows(b, R)));   // alpha_k
     }
     gmb::BFinish f{};
     f.w = wnext;
@@ -590,7 +651,15 @@ gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
     if (hipMemcpyAsync(b->rowblk, rb.data(), rb.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       return fail_clean(bfail(GM_EDEVICE, "upload row blocks"));
+    gmb::bspmm_xs_blocks(rp.data(), b->n, rb);
+    b->nrowblk_xs = (int)rb.size() / 4;
+    b->G_xs = xs_grid(b->dev, b->nrowblk_xs);
+    B_CREATE(balloc(&b->rowblk_xs, rb.size()));
+    if (hipMemcpyAsync(b->rowblk_xs, rb.data(), rb.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail_clean(bfail(GM_EDEVICE, "upload row blocks"));
   }
+  b->xs = (d->flags & GM_FLAG_COLUMN_SLABS) != 0;
   // U (n x 2P column-major) -> row-major Usol with dt on the g columns
   std::vector<double> u((size_t)N * b->R);
   for (int c = 0; c < b->R; ++c)
@@ -658,33 +727,40 @@ gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
     // Time one chunk of the step for each chunk width that the allocated
     // buffers hold (every width <= Rc) and keep the cheapest per column.  The
     // runs overwrite only X and scratch; Usol is untouched.
+    // From 32 columns up both SpMM forms (whole rows, column slabs) are timed.
     const int rc_max = b->Rc;
     double best = 1e300;
     int best_rc = rc_max;
+    bool best_xs = false;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
       return fail_clean(bfail(GM_EDEVICE, "event create"));
     for (int rc = rc_max; rc >= 8 || rc == rc_max; rc /= 2) {
       b->Rc = rc;
       set_bschedule(b);
-      gm_status s1 = dispatch_chunk(b, 0);                       // warm-up
-      (void)hipEventRecord(e0, st);
-      gm_status s2 = dispatch_chunk(b, 0);
-      (void)hipEventRecord(e1, st);
-      if (s1 != GM_OK || s2 != GM_OK || hipEventSynchronize(e1) != hipSuccess)
-        return fail_clean(s1 != GM_OK ? s1 : (s2 != GM_OK ? s2 : bfail(GM_EDEVICE, "autotune")));
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      const double per_col = (double)ms / rc;
-      if (per_col < best) {
-        best = per_col;
-        best_rc = rc;
+      for (int xs = 0; xs <= (rc >= 32 ? 1 : 0); ++xs) {
+        b->xs = xs != 0;
+        gm_status s1 = dispatch_chunk(b, 0);                       // warm-up
+        (void)hipEventRecord(e0, st);
+        gm_status s2 = dispatch_chunk(b, 0);
+        (void)hipEventRecord(e1, st);
+        if (s1 != GM_OK || s2 != GM_OK || hipEventSynchronize(e1) != hipSuccess)
+          return fail_clean(s1 != GM_OK ? s1 : (s2 != GM_OK ? s2 : bfail(GM_EDEVICE, "autotune")));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double per_col = (double)ms / rc;
+        if (per_col < best) {
+          best = per_col;
+          best_rc = rc;
+          best_xs = b->xs;
+        }
       }
       if (rc <= 2) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->Rc = best_rc;
+    b->xs = best_xs;
     set_bschedule(b);
     if (hipMemsetAsync(b->status, 0, sizeof(unsigned), st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -769,6 +845,13 @@ gm_status gm_batch_info(gm_batch* b, int32_t* num_problems, int32_t* chunk_colum
   if (!blive(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
   if (num_problems) *num_problems = b->P;
   if (chunk_columns) *chunk_columns = b->Rc;
+  return GM_OK;
+}
+
+gm_status gm_batch_get_column_slabs(gm_batch* b, int32_t* on) {
+  if (!blive(b)) return bfail(GM_EBADHANDLE, "invalid batch handle");
+  if (!on) return bfail(GM_EINVAL, "null argument");
+  *on = (b->Rc >= 32 && b->xs) ? 1 : 0;
   return GM_OK;
 }
 

@@ -127,14 +127,15 @@ __device__ void block_partial_cols(const double (&v)[(R < 64 ? 1 : R / 64)], dou
 constexpr int kBTile = 512;
 constexpr int kBRows = 256;
 
-inline void bspmm_row_blocks(const int* rowptr, int n, std::vector<int>& rb) {
+inline void bspmm_row_blocks(const int* rowptr, int n, std::vector<int>& rb,
+                             int max_rows = kBRows) {
   rb.clear();
   int r = 0;
   while (r < n) {
     rb.push_back(r);
     const int j0 = rowptr[r];
     int e = r + 1;
-    while (e < n && e - r < kBRows && rowptr[e + 1] - j0 <= kBTile) ++e;
+    while (e < n && e - r < max_rows && rowptr[e + 1] - j0 <= kBTile) ++e;
     r = e;
   }
   rb.push_back(n);
@@ -318,6 +319,227 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BSpmm p) {
   }
   if (MODE == kLast || MODE == kOnly)
     block_partial_cols<R>(dot, p.partials);
+}
+
+// Two adjacent doubles as one 16-byte access (plain or non-temporal).
+typedef double d2v __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ void st_stream2(double* ptr, double a, double b) {
+  const d2v v = {a, b};
+  if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<d2v*>(ptr));
+  else *reinterpret_cast<d2v*>(ptr) = v;
+}
+template <bool NT>
+__device__ __forceinline__ d2v ld_stream2(const double* ptr) {
+  if constexpr (NT) return __builtin_nontemporal_load(reinterpret_cast<const d2v*>(ptr));
+  else return *reinterpret_cast<const d2v*>(ptr);
+}
+
+// Row blocks of the XCD-slab form: at most kBTile nonzeros and kXsRows rows
+// (four passes of a workgroup's 16 row slots), described to the kernel by
+// {first row, end row, first nonzero, nonzeros} (bspmm_xs_blocks).
+constexpr int kXsRows = 64;
+static_assert(kBTile == 2 * kBlock && kXsRows < kBlock, "two staged nonzeros per thread");
+
+inline void bspmm_xs_blocks(const int* rowptr, int n, std::vector<int>& desc) {
+  std::vector<int> rb;
+  bspmm_row_blocks(rowptr, n, rb, kXsRows);
+  desc.clear();
+  for (size_t i = 0; i + 1 < rb.size(); ++i) {
+    desc.push_back(rb[i]);
+    desc.push_back(rb[i + 1]);
+    desc.push_back(rowptr[rb[i]]);
+    desc.push_back(rowptr[rb[i + 1]] - rowptr[rb[i]]);
+  }
+}
+
+// Chunks of R = 32 .. 256 columns in column slabs of 32 (the kXs form of
+// k_bspmm).
+//  - Workgroup b runs on XCD b % 8 (the dispatcher deals workgroups to the
+//    eight XCDs in turn) and takes slab s = b % S (S = R / 32 slabs), so at
+//    256 columns XCD s owns columns 32s .. 32s + 31 of every row: each XCD's
+//    L2 then holds only its own 256-byte slice of an x row, and no line is
+//    fetched by two XCDs (a whole 2 KB row leaves a 4 MB L2 before the
+//    neighbours that gather it again come by).
+//  - The G/S workgroups of a slab (all resident: the host sizes G by
+//    occupancy) take row blocks in turn (w, w + G/S, ...), so that they sweep
+//    the rows as one front of G/S row blocks.
+//  - Row blocks are double-buffered in LDS: the next block's values, column
+//    indices and row pointers are loaded into registers before this block's
+//    rows and stored to the other buffer after them; the block descriptors
+//    are read two blocks ahead.
+//  - A lane holds two adjacent columns of its slab: one 16-byte access per
+//    row operand, 16 lanes (256 contiguous bytes) a row, 4 rows a wavefront.
+//  - Per row the sums are k_bspmm's (the same nonzeros in the same order, the
+//    same epilogue).  Column sums go to partials [G/S][R] (row w = b / S),
+//    reduced after the kernel.
+//  - p.rowblk holds p.nrowblk descriptors (bspmm_xs_blocks); a block that does
+//    not fit the LDS tile (one row of more than kBTile nonzeros) is read from
+//    global memory.  Requires: G a multiple of S; x, y, vk, vkm1, z 16-byte
+//    aligned and every row stride even; no fused beta (bpart); MODE != kCheb.
+template <int R, int MODE, bool kNT>
+__global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
+  static_assert(MODE != kCheb, "the slab form serves the Lanczos and Horner modes");
+  constexpr int RS = 32, S = R / RS, LPR = RS / 2, RPW = 64 / LPR, RPB = RPW * kWaves;
+  static_assert(R % RS == 0 && S >= 1 && S <= 8, "R = 32 .. 256 in slabs of 32");
+  constexpr int U = kUnroll;
+  const int slab = (int)(blockIdx.x % S), w0 = (int)(blockIdx.x / S);
+  const int W = (int)(gridDim.x / S);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rslot = wave * RPW + lane / LPR;
+  const int c = slab * RS + 2 * (lane % LPR);   // columns c, c + 1
+  const int4* desc = reinterpret_cast<const int4*>(p.rowblk);
+  double bet[2], inv[2], bprev[2], dot[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    bet[q] = (MODE == kFirst || MODE == kOnly) ? p.be[(size_t)p.k * R + c + q] : 0.0;
+    inv[q] = safe_inv(bet[q]) * p.g0;
+    bprev[q] = ((MODE == kLast || MODE == kOnly) && p.k > 0) ? p.be[(size_t)p.k * R + c + q] : 0.0;
+    dot[q] = 0.0;
+  }
+  __shared__ double sval[2][kBTile];
+  __shared__ int scol[2][kBTile];
+  __shared__ int srp[2][kXsRows + 1];
+  auto fits = [](const int4 d) { return d.w <= kBTile && d.y - d.x <= kXsRows; };
+  // a row block's LDS tile, held in registers between its loads and stores:
+  // thread t has nonzeros t and t + kBlock and row pointer t
+  double pv[2];
+  int pc[2], pr;
+  auto fetch = [&](const int4 d) {
+    const bool st = fits(d);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int t = (int)threadIdx.x + u * kBlock;
+      pv[u] = (st && t < d.w) ? p.val[d.z + t] : 0.0;
+      pc[u] = (st && t < d.w) ? p.col[d.z + t] : 0;
+    }
+    pr = (st && (int)threadIdx.x <= d.y - d.x) ? p.rowptr[d.x + threadIdx.x] - d.z : 0;
+  };
+  auto commit = [&](const int4 d, int buf) {
+    if (!fits(d)) return;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int t = (int)threadIdx.x + u * kBlock;
+      if (t < d.w) {
+        sval[buf][t] = pv[u];
+        scol[buf][t] = pc[u];
+      }
+    }
+    if ((int)threadIdx.x <= d.y - d.x) srp[buf][threadIdx.x] = pr;
+  };
+  const int4 none = {0, 0, 0, 0};
+  int4 dcur = w0 < p.nrowblk ? desc[w0] : none;
+  int4 dnx = w0 + W < p.nrowblk ? desc[w0 + W] : none;
+  fetch(dcur);
+  commit(dcur, 0);
+  __syncthreads();
+  int buf = 0;
+  for (int it = w0; it < p.nrowblk; it += W) {
+    const int4 dnx2 = it + 2 * W < p.nrowblk ? desc[it + 2 * W] : none;
+    fetch(dnx);   // in flight during this block's rows
+    const int row0 = dcur.x, row1 = dcur.y;
+    auto rows = [&](auto from_lds) {
+      constexpr bool L = decltype(from_lds)::value;
+      for (int r = row0 + rslot; r < row1; r += RPB) {
+        const int j0 = L ? srp[buf][r - row0] : p.rowptr[r];
+        const int j1 = L ? srp[buf][r - row0 + 1] : p.rowptr[r + 1];
+        const size_t iv = (size_t)r * R + c;
+        // the epilogue's own-row operands first, so that their latency
+        // overlaps the gathers
+        d2v o0 = {0.0, 0.0}, o1 = {0.0, 0.0};
+        if (MODE == kFirst || MODE == kOnly) o0 = ld_stream2<false>(p.x + (size_t)r * p.ldx + c);
+        if (MODE == kMid || MODE == kLast) o0 = ld_stream2<false>(p.vk + iv);
+        if ((MODE == kLast || MODE == kOnly) && p.k > 0) o1 = ld_stream2<kNT>(p.vkm1 + iv);
+        if (MODE == kHorner) {
+          o0 = ld_stream2<false>(p.z + (size_t)r * p.ldz + c);
+          if (p.accumulate) o1 = ld_stream2<false>(p.y + (size_t)r * p.ldy + c);
+        }
+        double acc[2] = {0.0, 0.0};
+        for (int jb = j0; jb < j1; jb += U) {
+          double a[U];
+          int ci[U];
+#pragma unroll
+          for (int t = 0; t < U; ++t) {
+            const bool ok = jb + t < j1;
+            a[t] = ok ? (L ? sval[buf][jb + t] : p.val[jb + t]) : 0.0;
+            ci[t] = ok ? (L ? scol[buf][jb + t] : p.col[jb + t]) : -1;
+          }
+          d2v xv[U];
+#pragma unroll
+          for (int t = 0; t < U; ++t) {
+            const d2v zero = {0.0, 0.0};
+            xv[t] = ci[t] < 0 ? zero : ld_stream2<false>(p.x + (size_t)ci[t] * p.ldx + c);
+          }
+#pragma unroll
+          for (int t = 0; t < U; ++t) {
+            if (ci[t] >= 0) {
+              acc[0] = fma(a[t], xv[t].x, acc[0]);
+              acc[1] = fma(a[t], xv[t].y, acc[1]);
+            }
+          }
+        }
+        const double own0[2] = {o0.x, o0.y}, own1[2] = {o1.x, o1.y};
+        double vo[2] = {0.0, 0.0}, yo[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          if (MODE == kFirst) {
+            vo[q] = safe_div(own0[q], bet[q]);
+            yo[q] = fma(p.gi, vo[q], acc[q] * inv[q]);
+          } else if (MODE == kMid) {
+            yo[q] = fma(p.gi, own0[q], acc[q]);
+          } else if (MODE == kLast || MODE == kOnly) {
+            double v, yv;
+            if (MODE == kOnly) {
+              v = safe_div(own0[q], bet[q]);
+              vo[q] = v;
+              yv = acc[q] * inv[q];
+            } else {
+              v = own0[q];
+              yv = acc[q];
+            }
+            if (p.k > 0) yv = fma(-bprev[q], own1[q], yv);
+            yo[q] = yv;
+            dot[q] = fma(yv, v, dot[q]);
+          } else {
+            double yv = acc[q] + own0[q];
+            if (p.accumulate) yv += own1[q];
+            yo[q] = yv;
+          }
+        }
+        if (MODE == kFirst || MODE == kOnly) st_stream2<kNT>(p.vk + iv, vo[0], vo[1]);
+        if (MODE == kHorner) {
+          *reinterpret_cast<d2v*>(p.y + (size_t)r * p.ldy + c) = d2v{yo[0], yo[1]};
+        } else {
+          st_stream2<kNT>(p.y + (size_t)r * p.ldy + c, yo[0], yo[1]);
+        }
+      }
+    };
+    if (fits(dcur)) rows(std::true_type{});
+    else rows(std::false_type{});
+    // the other buffer's last readers were this block's predecessor's rows,
+    // which ended before the previous barrier
+    commit(dnx, buf ^ 1);
+    __syncthreads();
+    dcur = dnx;
+    dnx = dnx2;
+    buf ^= 1;
+  }
+  if (MODE == kLast || MODE == kOnly) {
+    // fixed-order block sums per column: row slots of a wavefront in order,
+    // then the wavefronts in order
+    __shared__ double sh[kBlock * 2];
+    sh[threadIdx.x * 2] = dot[0];
+    sh[threadIdx.x * 2 + 1] = dot[1];
+    __syncthreads();
+    if (threadIdx.x < RS) {
+      const int j = threadIdx.x;                       // slab column
+      const int l = j >> 1, q = j & 1;
+      double s = 0.0;
+      for (int wv = 0; wv < kWaves; ++wv)
+        for (int sl = 0; sl < RPW; ++sl) s += sh[(wv * 64 + sl * LPR + l) * 2 + q];
+      p.partials[(size_t)w0 * R + slab * RS + j] = s;
+    }
+  }
 }
 
 struct BFinish {

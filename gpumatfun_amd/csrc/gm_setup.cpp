@@ -337,4 +337,97 @@ gm_status gm_rcm_order(int32_t n, const int32_t* ptr, const int32_t* ind, int32_
   return GM_OK;
 }
 
+// Cache-locality order of a symmetric sparsity pattern by recursive
+// bisection: a part of more than `leaf` nodes is numbered breadth-first from a
+// pseudo-peripheral node (found as in gm_rcm_order, within the part; each
+// further component of the part from its first node) and cut into the first
+// and the second half of that numbering; the halves are ordered depth-first,
+// first before second, and a part of at most `leaf` nodes keeps its nodes in
+// increasing index.  On a mesh every part is a compact piece of it and parts
+// consecutive in the order are neighbours, so that rows numbered close
+// together gather nearby rows (the bisection tree of a space-filling order).
+gm_status gm_locality_order(int32_t n, const int32_t* ptr, const int32_t* ind, int32_t leaf,
+                            int32_t* perm) {
+  if (n < 0 || leaf < 1 || (n > 0 && (!ptr || !ind || !perm))) return GM_EINVAL;
+  if (n == 0) return GM_OK;
+  if (ptr[0] != 0) return GM_EINVAL;
+  for (int i = 0; i < n; ++i)
+    if (ptr[i + 1] < ptr[i]) return GM_EINVAL;
+  for (int j = 0; j < ptr[n]; ++j)
+    if (ind[j] < 0 || ind[j] >= n) return GM_EINVAL;
+  std::vector<int> deg(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int j = ptr[i]; j < ptr[i + 1]; ++j)
+      if (ind[j] != i) ++deg[i];
+  std::vector<int> vs(n), part(n, 0), level(n, -1), queue(n), last;
+  for (int i = 0; i < n; ++i) vs[i] = i;
+  int stamp = 0;
+  // breadth-first numbering of the part's nodes (part[v] == id) from root,
+  // into queue[0..); returns the count and the height, last level in `last`
+  auto bfs = [&](int root, int id, int* height) {
+    int head = 0, tail = 0;
+    queue[tail++] = root;
+    level[root] = 0;
+    int h = 0;
+    while (head < tail) {
+      const int v = queue[head++];
+      h = level[v];
+      for (int j = ptr[v]; j < ptr[v + 1]; ++j) {
+        const int w = ind[j];
+        if (part[w] != id || level[w] >= 0) continue;
+        level[w] = level[v] + 1;
+        queue[tail++] = w;
+      }
+    }
+    last.clear();
+    for (int t = tail - 1; t >= 0 && level[queue[t]] == h; --t) last.push_back(queue[t]);
+    for (int t = 0; t < tail; ++t) level[queue[t]] = -1;
+    *height = h;
+    return tail;
+  };
+  std::vector<std::pair<int, int>> stack{{0, n}};
+  std::vector<int> numbering;
+  int out = 0;
+  while (!stack.empty()) {
+    const auto [lo, hi] = stack.back();
+    stack.pop_back();
+    const int sz = hi - lo;
+    if (sz <= leaf) {
+      std::sort(vs.begin() + lo, vs.begin() + hi);
+      for (int t = lo; t < hi; ++t) perm[out++] = vs[t];
+      continue;
+    }
+    const int id = ++stamp;
+    for (int t = lo; t < hi; ++t) part[vs[t]] = id;
+    numbering.clear();
+    // every component of the part, from its first node in vs order
+    for (int t = lo; t < hi && (int)numbering.size() < sz; ++t) {
+      const int seed = vs[t];
+      if (part[seed] != id) continue;   // numbered already
+      int root = seed, h = 0;
+      bfs(root, id, &h);
+      for (int it = 0; it < 16; ++it) {
+        int cand = last[0];
+        for (int v : last)
+          if (deg[v] < deg[cand] || (deg[v] == deg[cand] && v < cand)) cand = v;
+        int h2 = 0;
+        bfs(cand, id, &h2);
+        if (h2 <= h) break;
+        root = cand;
+        h = h2;
+      }
+      const int cnt = bfs(root, id, &h);
+      for (int q = 0; q < cnt; ++q) {
+        numbering.push_back(queue[q]);
+        part[queue[q]] = -id;   // out of the part for the next components
+      }
+    }
+    std::copy(numbering.begin(), numbering.end(), vs.begin() + lo);
+    const int mid = lo + sz / 2;
+    stack.push_back({mid, hi});
+    stack.push_back({lo, mid});
+  }
+  return GM_OK;
+}
+
 }  // extern "C"

@@ -279,6 +279,23 @@ def rcm_order(A):
     return perm.astype(np.int64)
 
 
+def locality_order(A, leaf=4096):
+    """Cache-locality order of A's (symmetric) pattern from the library's host code
+    (``gm_locality_order``: recursive breadth-first bisection down to parts of at
+    most ``leaf`` nodes): ``perm[q]`` is the old index placed at q."""
+    import ctypes as C
+    from gpumatfun_amd import _lib
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    ptr = np.ascontiguousarray(A.indptr, dtype=np.int32)
+    ind = np.ascontiguousarray(A.indices, dtype=np.int32)
+    perm = np.empty(n, dtype=np.int32)
+    ip = C.POINTER(C.c_int32)
+    _lib.check(_lib.load().gm_locality_order(n, ptr.ctypes.data_as(ip), ind.ctypes.data_as(ip),
+                                             int(leaf), perm.ctypes.data_as(ip)))
+    return perm.astype(np.int64)
+
+
 def fem_operator(K, mdiag, reorder=True):
     """A = M^{-1/2} K M^{-1/2} with symrcm reordering, ``process_nonuniform.m:15-20,90-91``.
 

@@ -63,10 +63,14 @@ enum {
     GM_FLAG_EARLY_LOADS = 1u << 6, /* without AUTOTUNE: the row-block SpMM loads
                                     each row's own operands before its
                                     gathers (default: after)                */
-    GM_FLAG_WAVE_SOLVES = 1u << 7 /* without AUTOTUNE: the shifted solves run
+    GM_FLAG_WAVE_SOLVES = 1u << 7, /* without AUTOTUNE: the shifted solves run
                                     one wavefront per system (cyclic
                                     reduction; m <= 128) instead of one lane
                                     per system.  AUTOTUNE times the forms   */
+    GM_FLAG_COLUMN_SLABS = 1u << 8 /* gm_batch: chunks of >= 32 columns run
+                                    their SpMM in slabs of 32 columns (at 256
+                                    columns one slab per XCD) instead of
+                                    whole rows.  AUTOTUNE times both forms  */
 };
 
 /*
@@ -292,6 +296,8 @@ gm_status gm_batch_step(gm_batch* b, int32_t nsteps, int32_t use_graph);
 gm_status gm_batch_get_U(gm_batch* b, double* dst);
 gm_status gm_batch_synchronize(gm_batch* b);
 gm_status gm_batch_info(gm_batch* b, int32_t* num_problems, int32_t* chunk_columns);
+/* on = 1 when the chunks run their SpMM in column slabs.                  */
+gm_status gm_batch_get_column_slabs(gm_batch* b, int32_t* on);
 gm_status gm_batch_get_stream(gm_batch* b, void** stream);
 /* dst: n x P column-major MS gate h of every problem.                      */
 gm_status gm_batch_get_ode(gm_batch* b, double* dst);
@@ -484,6 +490,12 @@ gm_status gm_subspace_size(const double* gamma, int32_t gamma_size, double l1, d
  * n x n matrix, in place of symrcm (process_nonuniform.m:15): perm[q] is the
  * old index of the node placed at q (A(perm, perm) is banded).  Host only.  */
 gm_status gm_rcm_order(int32_t n, const int32_t* ptr, const int32_t* ind, int32_t* perm);
+/* Cache-locality order (no reference counterpart): recursive breadth-first
+ * bisection of the symmetric pattern down to parts of at most `leaf` nodes,
+ * the parts depth-first and each part's nodes in increasing index; perm as
+ * for gm_rcm_order.  Used by gm_batch with GM_FLAG_LOCALITY_ORDER.  Host only. */
+gm_status gm_locality_order(int32_t n, const int32_t* ptr, const int32_t* ind, int32_t leaf,
+                            int32_t* perm);
 
 /* ---------------------------------------------------------------------------
  * f(A) on R columns at once with the parameter adjoint (no reference

@@ -81,6 +81,71 @@ def test_batch_laplace3d_chunk_widths(chunk):
         assert max(errs) < 1e-13
 
 
+def _hub_laplace3d_case(side=20, hub=600, w=1.0):
+    """The 20^3 Laplacian case plus a hub: node 0 coupled to `hub` other nodes
+    with weight w (a weighted graph Laplacian added, so still SPD), which puts
+    a row of hub + 7 nonzeros (longer than one LDS tile) into the matrix."""
+    import scipy.sparse as sp
+    from cases import laplace3d_case
+    from gpumatfun_amd import quadrature as q
+    base = laplace3d_case(side=side)
+    A = sp.lil_matrix(base.A)
+    nodes = np.random.default_rng(5).choice(np.arange(1, A.shape[0]), hub, replace=False)
+    for j in nodes:
+        A[0, j] -= w
+        A[j, 0] -= w
+        A[j, j] += w
+    A[0, 0] += hub * w
+    A = sp.csr_matrix(A)
+    h = 1.0 / (side + 1)
+    lo = 3 * 4.0 * np.sin(np.pi * h / 2) ** 2 / h ** 2
+    hi = float(abs(A).sum(axis=1).max())            # Gershgorin
+    u0 = np.random.default_rng(0).standard_normal(A.shape[0])
+    return q.make_inputs(A, u0, u0 * (1 - u0), m=30, n=64, eta=1e-3, alpha=1.5, dt=1e-3,
+                         l1=lo, lN=hi, degree=0)
+
+
+@pytest.mark.parametrize("chunk,degree,hub", [(256, 0, False), (256, 2, False), (256, 0, True),
+                                              (32, 0, False), (64, 2, False), (128, 0, True)])
+def test_batch_column_slabs_match_whole_rows(chunk, degree, hub):
+    """Chunks with the SpMM in column slabs of 32 against the whole-row SpMM: 128
+    problems on the 20^3 Laplacian (with the Jacobi polynomial at degree 2: the
+    kFirst/kMid/kLast and Horner modes; with a 607-nonzero hub row: the row read
+    from global memory), chunks of 32 to 256 columns.  Every column agrees to
+    1e-13 (the per-row sums are the same; only the column sums' block order
+    differs), and problems 0 and 127 match the oracle to 1e-10."""
+    from cases import laplace3d_case
+    from gpumatfun_amd import quadrature as q
+    if hub:
+        inp = _hub_laplace3d_case()
+    else:
+        inp = laplace3d_case()
+        if degree:
+            inp = q.make_inputs(inp.A, inp.U[:, 0], inp.U[:, 1], m=30, n=64, eta=1e-3,
+                                alpha=1.5, dt=1e-3, l1=inp.extra["l1"], lN=inp.extra["lN"],
+                                degree=degree)
+    P = 128
+    U = batch_U(inp, P, seed0=0)
+    outs = {}
+    for xs in (False, True):
+        print(f"case batch chunk={chunk} column_slabs={xs} degree={degree} hub={hub}", flush=True)
+        b = BatchEngine.from_inputs(inp, U, chunk_columns=chunk, clamp=False, column_slabs=xs)
+        info = b.info()
+        assert info["chunk_columns"] == chunk and info["column_slabs"] == xs
+        b.step(2)
+        outs[xs] = b.get_U()
+        b.synchronize()
+        b.delete()
+    errs = [rel(outs[True][:, c], outs[False][:, c]) for c in range(2 * P)]
+    print("max column difference slabs vs whole rows:", max(errs), flush=True)
+    assert max(errs) < 1e-13
+    for p in (0, P - 1):
+        u, gu = oracle_pair(inp, U[:, 2 * p], U[:, 2 * p + 1], 2)
+        e = (rel(outs[True][:, 2 * p], u), rel(outs[True][:, 2 * p + 1], inp.dt * gu))
+        print(p, e, flush=True)
+        assert max(e) < TOL
+
+
 def test_batch_graph_matches_eager_bitwise():
     inp = dirichlet_case(side=32, m=24, n=16)
     U = batch_U(inp, 8, seed0=10)
