@@ -474,7 +474,7 @@ def run_ms512(args, steps, warm, use_graph):
     N, P = inp.A.shape[0], len(stims)
     log(f"setup {setup_s:.1f}s")
     b = BatchEngine.from_inputs(inp, U, clamp=False, reaction="ms", ode=np.ones((N, P)), ms=prm,
-                                column_slabs=args.column_slabs)
+                                autotune=not args.column_slabs, column_slabs=args.column_slabs)
     b.step(1, use_graph=False)
     b.synchronize()
     U1 = b.get_U()

@@ -147,37 +147,6 @@ gm_status breduce(gm_batch* b, int nv, double* out, int do_sqrt, int flag_zero, 
   return GM_OK;
 }
 
-// Workgroups of the slab SpMM: as many as are resident at once (the fewest
-// that any of its instantiations allows per CU), so that each slab's
-// workgroups sweep the rows together as one front; a multiple of 8.
-template <int R>
-void xs_occupancy(int* per_cu) {
-  auto occ = [&](const void* k) {
-    int v = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k, gm::kBlock, 0) == hipSuccess && v > 0)
-      *per_cu = std::min(*per_cu, v);
-  };
-  constexpr bool nt = R >= 128;
-  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kFirst, nt>));
-  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kMid, nt>));
-  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kLast, nt>));
-  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kOnly, nt>));
-  occ(reinterpret_cast<const void*>(&gmb::k_bspmm_xs<R, gmb::kHorner, nt>));
-}
-
-int xs_grid(int dev, int nrowblk) {
-  int ncu = 0;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    ncu = 8;
-  int per_cu = 1 << 30;
-  xs_occupancy<32>(&per_cu);
-  xs_occupancy<64>(&per_cu);
-  xs_occupancy<128>(&per_cu);
-  xs_occupancy<256>(&per_cu);
-  if (per_cu == (1 << 30)) per_cu = 1;
-  return 8 * std::max(1, std::min(nrowblk, std::max(1, ncu / 8) * per_cu));
-}
-
 // Rows of block partials that a kLast/kOnly SpMM of a chunk of R columns
 // leaves for k_reduce.
 int spmm_partial_rows(const gm_batch* b, int R) {
@@ -653,7 +622,7 @@ gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
       return fail_clean(bfail(GM_EDEVICE, "upload row blocks"));
     gmb::bspmm_xs_blocks(rp.data(), b->n, rb);
     b->nrowblk_xs = (int)rb.size() / 4;
-    b->G_xs = xs_grid(b->dev, b->nrowblk_xs);
+    b->G_xs = gmb::bspmm_xs_grid(b->dev, b->nrowblk_xs);
     B_CREATE(balloc(&b->rowblk_xs, rb.size()));
     if (hipMemcpyAsync(b->rowblk_xs, rb.data(), rb.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)

@@ -7,6 +7,7 @@
 // on one row at a time, with R < 64 on 64/R rows.  All kernels are templates
 // so that several translation units may instantiate them.
 #pragma once
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -376,10 +377,9 @@ inline void bspmm_xs_blocks(const int* rowptr, int n, std::vector<int>& desc) {
 //  - p.rowblk holds p.nrowblk descriptors (bspmm_xs_blocks); a block that does
 //    not fit the LDS tile (one row of more than kBTile nonzeros) is read from
 //    global memory.  Requires: G a multiple of S; x, y, vk, vkm1, z 16-byte
-//    aligned and every row stride even; no fused beta (bpart); MODE != kCheb.
+//    aligned and every row stride even; no fused beta (bpart).
 template <int R, int MODE, bool kNT>
 __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
-  static_assert(MODE != kCheb, "the slab form serves the Lanczos and Horner modes");
   constexpr int RS = 32, S = R / RS, LPR = RS / 2, RPW = 64 / LPR, RPB = RPW * kWaves;
   static_assert(R % RS == 0 && S >= 1 && S <= 8, "R = 32 .. 256 in slabs of 32");
   constexpr int U = kUnroll;
@@ -454,6 +454,10 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
           o0 = ld_stream2<false>(p.z + (size_t)r * p.ldz + c);
           if (p.accumulate) o1 = ld_stream2<false>(p.y + (size_t)r * p.ldy + c);
         }
+        if (MODE == kCheb) {
+          o0 = ld_stream2<false>(p.x + (size_t)r * p.ldx + c);
+          if (p.z) o1 = ld_stream2<false>(p.z + (size_t)r * p.ldz + c);
+        }
         double acc[2] = {0.0, 0.0};
         for (int jb = j0; jb < j1; jb += U) {
           double a[U];
@@ -500,6 +504,9 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
             if (p.k > 0) yv = fma(-bprev[q], own1[q], yv);
             yo[q] = yv;
             dot[q] = fma(yv, v, dot[q]);
+          } else if (MODE == kCheb) {
+            const double sh = p.shiftc ? p.shiftc[c + q] : p.cc;
+            yo[q] = fma(p.cb, own1[q], p.ca * fma(-sh, own0[q], acc[q]));
           } else {
             double yv = acc[q] + own0[q];
             if (p.accumulate) yv += own1[q];
@@ -507,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
           }
         }
         if (MODE == kFirst || MODE == kOnly) st_stream2<kNT>(p.vk + iv, vo[0], vo[1]);
-        if (MODE == kHorner) {
+        if (MODE == kHorner || MODE == kCheb) {
           *reinterpret_cast<d2v*>(p.y + (size_t)r * p.ldy + c) = d2v{yo[0], yo[1]};
         } else {
           st_stream2<kNT>(p.y + (size_t)r * p.ldy + c, yo[0], yo[1]);
@@ -540,6 +547,42 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
       p.partials[(size_t)w0 * R + slab * RS + j] = s;
     }
   }
+}
+
+// Workgroups of the slab SpMM: as many as are resident at once (the fewest
+// that any of its instantiations allows per CU), so that each slab's
+// workgroups sweep the rows together as one front; a multiple of 8.
+template <int R>
+void bspmm_xs_occupancy(int* per_cu) {
+  auto occ = [&](const void* k) {
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k, kBlock, 0) == hipSuccess && v > 0)
+      *per_cu = std::min(*per_cu, v);
+  };
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kFirst, false>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kMid, false>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kLast, false>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kOnly, false>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kHorner, false>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kCheb, false>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kFirst, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kMid, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kLast, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kOnly, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kHorner, true>));
+}
+
+inline int bspmm_xs_grid(int dev, int nrowblk) {
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    ncu = 8;
+  int per_cu = 1 << 30;
+  bspmm_xs_occupancy<32>(&per_cu);
+  bspmm_xs_occupancy<64>(&per_cu);
+  bspmm_xs_occupancy<128>(&per_cu);
+  bspmm_xs_occupancy<256>(&per_cu);
+  if (per_cu == (1 << 30)) per_cu = 1;
+  return 8 * std::max(1, std::min(nrowblk, std::max(1, ncu / 8) * per_cu));
 }
 
 struct BFinish {

@@ -191,10 +191,13 @@ struct Eigs {
   hipStream_t st = nullptr;
   int n = 0, s = 0, G_sp = 1, chunk_sp = 1, G_df = 1, chunk_df = 1, G_rb = 1;
   int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr;
+  // slab SpMM at >= 32 columns (gmb::k_bspmm_xs): row-block descriptors, grid
+  int* rowblk_xs = nullptr;
+  int nrb_xs = 0, G_xs = 8;
   double *val = nullptr, *X = nullptr, *Y = nullptr, *Z = nullptr, *AX = nullptr,
          *partials = nullptr, *small = nullptr, *wmat = nullptr, *shift = nullptr;
   ~Eigs() {
-    void* bufs[] = {rowptr, col, rowblk, val, X, Y, Z, AX, partials, small, wmat, shift};
+    void* bufs[] = {rowptr, col, rowblk, rowblk_xs, val, X, Y, Z, AX, partials, small, wmat, shift};
     for (void* p : bufs)
       if (p) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
@@ -282,8 +285,16 @@ gm_status cheb_step(Eigs& e, const double* x, const double* z, double* y, double
   a.cb = cb;
   a.cc = cc;
   a.shiftc = shiftc;
-  hipLaunchKernelGGL((gmb::k_bspmm<S, gmb::kCheb, true>), dim3(std::min(e.G_rb, 8192)),
-                     dim3(gm::kBlock), 0, e.st, a);
+  if constexpr (S >= 32) {
+    // column slabs of 32 (16-byte accesses, double-buffered row blocks)
+    a.rowblk = e.rowblk_xs;
+    a.nrowblk = e.nrb_xs;
+    hipLaunchKernelGGL((gmb::k_bspmm_xs<S, gmb::kCheb, false>), dim3(e.G_xs), dim3(gm::kBlock), 0,
+                       e.st, a);
+  } else {
+    hipLaunchKernelGGL((gmb::k_bspmm<S, gmb::kCheb, true>), dim3(std::min(e.G_rb, 8192)),
+                       dim3(gm::kBlock), 0, e.st, a);
+  }
   E_HIP(hipGetLastError());
   return GM_OK;
 }
@@ -504,6 +515,12 @@ gm_status gm_eigs_smallest(const gm_eigs_desc* d, double* lambda, double* Q, int
     E_TRY(ealloc(&e.rowblk, rb.size()));
     E_HIP(hipMemcpyAsync(e.rowblk, rb.data(), rb.size() * sizeof(int), hipMemcpyHostToDevice, e.st));
     E_HIP(hipStreamSynchronize(e.st));   // rb is freed at the end of the scope
+    gmb::bspmm_xs_blocks(d->A_ptr, e.n, rb);
+    e.nrb_xs = (int)rb.size() / 4;
+    e.G_xs = gmb::bspmm_xs_grid(d->device, e.nrb_xs);
+    E_TRY(ealloc(&e.rowblk_xs, rb.size()));
+    E_HIP(hipMemcpyAsync(e.rowblk_xs, rb.data(), rb.size() * sizeof(int), hipMemcpyHostToDevice, e.st));
+    E_HIP(hipStreamSynchronize(e.st));
   }
   E_TRY(ealloc(&e.X, N * s));
   E_TRY(ealloc(&e.Y, N * s));
