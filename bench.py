@@ -210,11 +210,15 @@ def build_batch3d(args):
     side = args.side3d
     A = problems.laplacian_3d_dirichlet(side)
     order = None
-    if args.node_order == "pencil":
-        # the same operator with the nodes in 16 x 16 pencils (P A P^T, and P u for
-        # every right-hand side): f(P A P^T) P u = P f(A) u
+    if args.node_order in ("pencil", "locality"):
+        # the same operator with the nodes in 16 x 16 pencils, or in the library's
+        # graph-only locality order (P A P^T, and P u for every right-hand side):
+        # f(P A P^T) P u = P f(A) u
         import scipy.sparse as sp
-        order = problems.pencil_order(side, 16)
+        if args.node_order == "pencil":
+            order = problems.pencil_order(side, 16)
+        else:
+            order = problems.locality_order(A, 4096)
         A = sp.csr_matrix(A[order][:, order])
         A.sort_indices()
         A.indptr = A.indptr.astype(np.int32)
@@ -966,8 +970,12 @@ def main():
     ap.add_argument("--rcm", default="native", choices=["native", "scipy"],
                     help="fem1m: reverse Cuthill-McKee order from the library's set-up code "
                          "(gm_rcm_order, pseudo-peripheral start as symrcm) or SciPy's")
-    ap.add_argument("--node-order", default="natural", choices=["natural", "pencil"],
-                    help="batch3d: grid nodes in natural (i + n j + n^2 k) or 16 x 16-pencil order")
+    ap.add_argument("--node-order", default="natural", choices=["natural", "pencil", "locality"],
+                    help="batch3d: grid nodes in natural (i + n j + n^2 k) or 16 x 16-pencil order, "
+                         "or the library's locality order (gm_locality_order, leaves of 4096)")
+    ap.add_argument("--locality-order", default="on", choices=["on", "off"],
+                    help="batch3d: the batch engine renumbers the nodes internally "
+                         "(GM_FLAG_LOCALITY_ORDER); inputs and results in the caller's order")
     ap.add_argument("--column-slabs", action="store_true",
                     help="batch3d with --chunk-columns, ms512: the SpMM in column slabs of 32")
     ap.add_argument("--chunk-columns", type=int, default=0,
@@ -1134,14 +1142,15 @@ def run_batch(args, steps, warm, use_graph):
     log(f"setup {setup_s:.1f}s, U {U.shape}")
     P = U.shape[1] // 2
     t0 = time.time()
+    loc = args.locality_order == "on"
     if args.chunk_columns:
         b = BatchEngine.from_inputs(inp, U, clamp=False, chunk_columns=args.chunk_columns,
-                                    column_slabs=args.column_slabs)
+                                    column_slabs=args.column_slabs, locality_order=loc)
     else:
-        b = BatchEngine.from_inputs(inp, U, clamp=False, autotune=True)
+        b = BatchEngine.from_inputs(inp, U, clamp=False, autotune=True, locality_order=loc)
     info = b.info()
     log(f"batch create {time.time() - t0:.1f}s, chunk columns {info['chunk_columns']}, "
-        f"column slabs {info['column_slabs']}")
+        f"column slabs {info['column_slabs']}, locality order {loc}")
     # parity leg: problem 0 after one step
     b.step(1, use_graph=False)
     b.synchronize()
@@ -1187,6 +1196,7 @@ def run_batch(args, steps, warm, use_graph):
         "config": cfg,
         "graph": use_graph,
         "chunk_columns": info["chunk_columns"], "column_slabs": info["column_slabs"],
+        "locality_order": info["locality_order"],
         "setup_s": round(setup_s, 1),
         "plain_same_matrix": {"value": round(plain, 3), "unit": "solves/s",
                               "spmm_schedule": einfo["spmm_schedule"]},

@@ -22,6 +22,7 @@ GM_FLAG_INDEX16 = 32
 GM_FLAG_EARLY_LOADS = 64
 GM_FLAG_WAVE_SOLVES = 128
 GM_FLAG_COLUMN_SLABS = 256
+GM_FLAG_LOCALITY_ORDER = 512
 GM_REACTION_FISHER, GM_REACTION_MS = 0, 2
 # Mitchell-Schaeffer defaults (the library's own, gm_solver.hip)
 MS_DEFAULTS = dict(tau_in=0.3, tau_out=6.0, tau_open=120.0, tau_close=150.0, v_gate=0.13)

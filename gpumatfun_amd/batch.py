@@ -24,14 +24,21 @@ class BatchEngine:
     ``phi_lambda``), and ``reaction="ms"`` (Mitchell-Schaeffer, gate ``ode``
     n x P, parameters ``ms`` as for ``LanczosEngine.set_ms_params``) instead
     of Fisher-KPP.
+
+    Schedule: ``chunk_columns`` (columns per pass, 0: the widest that fits),
+    ``autotune`` (time the chunk widths and the two SpMM forms at create),
+    ``column_slabs`` (without autotune: the SpMM in slabs of 32 columns),
+    ``locality_order`` (renumber the nodes inside the engine by the library's
+    locality order; inputs and results stay in the caller's order).
     """
 
     def __init__(self, A, U, *, m, gamma, gamma_s_real, gamma_s_imag, wts1, wts2, etaxi, dt,
                  mass=None, chunk_columns=0, clamp=True, autotune=False, device=0,
                  Q=None, exp_lambda=None, phi_lambda=None, reaction="fisher", ode=None,
-                 ms=None, column_slabs=False):
+                 ms=None, column_slabs=False, locality_order=False):
         lib = _lib.load()
         self._lib = lib
+        self._locality = bool(locality_order)
         self._h = C.c_void_p()
         A = sp.csr_matrix(A)
         A.sort_indices()
@@ -84,7 +91,8 @@ class BatchEngine:
         d.mass = _p(keep["mass"]) if keep["mass"] is not None else None
         d.chunk_columns = int(chunk_columns)
         d.flags = ((0 if clamp else _lib.GM_FLAG_NO_CLAMP) | (_lib.GM_FLAG_AUTOTUNE if autotune else 0)
-                   | (_lib.GM_FLAG_COLUMN_SLABS if column_slabs else 0))
+                   | (_lib.GM_FLAG_COLUMN_SLABS if column_slabs else 0)
+                   | (_lib.GM_FLAG_LOCALITY_ORDER if locality_order else 0))
         d.device = int(device)
         K = 0 if Q is None else np.asarray(Q).reshape(n, -1).shape[1]
         if K:
@@ -151,7 +159,8 @@ class BatchEngine:
         self._call(self._lib.gm_batch_info, C.byref(P), C.byref(rc))
         xs = C.c_int32()
         self._call(self._lib.gm_batch_get_column_slabs, C.byref(xs))
-        return {"num_problems": P.value, "chunk_columns": rc.value, "column_slabs": bool(xs.value)}
+        return {"num_problems": P.value, "chunk_columns": rc.value, "column_slabs": bool(xs.value),
+                "locality_order": self._locality}
 
     def stream(self):
         p = C.c_void_p()

@@ -67,10 +67,15 @@ enum {
                                     one wavefront per system (cyclic
                                     reduction; m <= 128) instead of one lane
                                     per system.  AUTOTUNE times the forms   */
-    GM_FLAG_COLUMN_SLABS = 1u << 8 /* gm_batch: chunks of >= 32 columns run
+    GM_FLAG_COLUMN_SLABS = 1u << 8, /* gm_batch: chunks of >= 32 columns run
                                     their SpMM in slabs of 32 columns (at 256
                                     columns one slab per XCD) instead of
                                     whole rows.  AUTOTUNE times both forms  */
+    GM_FLAG_LOCALITY_ORDER = 1u << 9 /* gm_batch: renumber the nodes inside the
+                                    engine by gm_locality_order (leaves of
+                                    4096 nodes) so that rows gathered together
+                                    lie close; inputs and results stay in the
+                                    caller's order                          */
 };
 
 /*

@@ -146,6 +146,47 @@ def test_batch_column_slabs_match_whole_rows(chunk, degree, hub):
         assert max(e) < TOL
 
 
+@pytest.mark.parametrize("case", ["fisher_deflated", "ms"])
+def test_batch_locality_order_matches_caller_order(case):
+    """The engine renumbering its nodes (GM_FLAG_LOCALITY_ORDER) against the same
+    batch in the caller's order: results come back in the caller's order and
+    agree to 1e-13 per column (each row's sums are unchanged; the column sums
+    run over another row partition).  Fisher with K = 6 shared eigenpairs and a
+    lumped mass on the FEM disc, and Mitchell-Schaeffer (gate state) on the
+    20^3 Laplacian."""
+    from gpumatfun_amd._lib import MS_DEFAULTS as prm
+    if case == "ms":
+        from cases import laplace3d_case
+        inp = laplace3d_case()
+        P = 16
+        U = batch_U(inp, P, seed0=3)
+        N = U.shape[0]
+        ode = np.random.default_rng(4).uniform(0.5, 1.0, (N, P))
+        kw = dict(reaction="ms", ode=ode, ms=prm)
+    else:
+        from cases import fem_case
+        inp = fem_case(n_nodes=3000, K=6, m=30, degree=2)
+        P = 8
+        U = batch_U(inp, P, seed0=5)
+        kw = {}
+    outs, odes = {}, {}
+    for loc in (False, True):
+        print(f"case batch locality_order={loc} {case}", flush=True)
+        b = BatchEngine.from_inputs(inp, U, clamp=False, locality_order=loc, **kw)
+        assert b.info()["locality_order"] == loc
+        b.step(3)
+        outs[loc] = b.get_U()
+        if case == "ms":
+            odes[loc] = b.get_ode()
+        b.synchronize()
+        b.delete()
+    errs = [rel(outs[True][:, c], outs[False][:, c]) for c in range(2 * P)]
+    print("max column difference, locality vs caller order:", max(errs), flush=True)
+    assert max(errs) < 1e-13
+    if case == "ms":
+        assert rel(odes[True], odes[False]) < 1e-13
+
+
 def test_batch_graph_matches_eager_bitwise():
     inp = dirichlet_case(side=32, m=24, n=16)
     U = batch_U(inp, 8, seed0=10)
