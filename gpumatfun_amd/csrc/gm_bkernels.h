@@ -357,14 +357,15 @@ inline void bspmm_xs_blocks(const int* rowptr, int n, std::vector<int>& desc) {
 // Chunks of R = 32 .. 256 columns in column slabs of 32 (the kXs form of
 // k_bspmm).
 //  - Workgroup b runs on XCD b % 8 (the dispatcher deals workgroups to the
-//    eight XCDs in turn) and takes slab s = b % S (S = R / 32 slabs), so at
+//    eight XCDs in turn) and takes slab (b % 8) % S (S = R / 32 slabs), so at
 //    256 columns XCD s owns columns 32s .. 32s + 31 of every row: each XCD's
 //    L2 then holds only its own 256-byte slice of an x row, and no line is
 //    fetched by two XCDs (a whole 2 KB row leaves a 4 MB L2 before the
-//    neighbours that gather it again come by).
-//  - The G/S workgroups of a slab (all resident: the host sizes G by
-//    occupancy) take row blocks in turn (w, w + G/S, ...), so that they sweep
-//    the rows as one front of G/S row blocks.
+//    neighbours that gather it again come by).  With S < 8 each of a slab's
+//    8 / S XCDs owns a contiguous part of the row blocks.
+//  - The G/8 workgroups of an XCD (all resident: the host sizes G by
+//    occupancy) take its row blocks in turn (w, w + G/8, ...), so that they
+//    sweep the rows as one front of G/8 row blocks.
 //  - Row blocks are double-buffered in LDS: the next block's values, column
 //    indices and row pointers are loaded into registers before this block's
 //    rows and stored to the other buffer after them; the block descriptors
@@ -372,8 +373,8 @@ inline void bspmm_xs_blocks(const int* rowptr, int n, std::vector<int>& desc) {
 //  - A lane holds two adjacent columns of its slab: one 16-byte access per
 //    row operand, 16 lanes (256 contiguous bytes) a row, 4 rows a wavefront.
 //  - Per row the sums are k_bspmm's (the same nonzeros in the same order, the
-//    same epilogue).  Column sums go to partials [G/S][R] (row w = b / S),
-//    reduced after the kernel.
+//    same epilogue).  Column sums go to partials [G/S][R] (row
+//    part * G/8 + b / 8 of the slab's columns), reduced after the kernel.
 //  - p.rowblk holds p.nrowblk descriptors (bspmm_xs_blocks); a block that does
 //    not fit the LDS tile (one row of more than kBTile nonzeros) is read from
 //    global memory.  Requires: G a multiple of S; x, y, vk, vkm1, z 16-byte
@@ -383,8 +384,15 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
   constexpr int RS = 32, S = R / RS, LPR = RS / 2, RPW = 64 / LPR, RPB = RPW * kWaves;
   static_assert(R % RS == 0 && S >= 1 && S <= 8, "R = 32 .. 256 in slabs of 32");
   constexpr int U = kUnroll;
-  const int slab = (int)(blockIdx.x % S), w0 = (int)(blockIdx.x / S);
-  const int W = (int)(gridDim.x / S);
+  // workgroup b runs on XCD b % 8; it takes slab (b % 8) % S, and the XCD's
+  // share of that slab is the contiguous part (b % 8) / S of its row blocks,
+  // swept by the XCD's G/8 workgroups in turn
+  constexpr int PARTS = 8 / S;
+  const int xcd = (int)(blockIdx.x & 7);
+  const int slab = xcd % S, part = xcd / S;
+  const int W = (int)(gridDim.x >> 3), lw = (int)(blockIdx.x >> 3);
+  const int it_end = (int)((long long)p.nrowblk * (part + 1) / PARTS);
+  const int w0 = (int)((long long)p.nrowblk * part / PARTS) + lw;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rslot = wave * RPW + lane / LPR;
   const int c = slab * RS + 2 * (lane % LPR);   // columns c, c + 1
@@ -428,14 +436,14 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
     if ((int)threadIdx.x <= d.y - d.x) srp[buf][threadIdx.x] = pr;
   };
   const int4 none = {0, 0, 0, 0};
-  int4 dcur = w0 < p.nrowblk ? desc[w0] : none;
-  int4 dnx = w0 + W < p.nrowblk ? desc[w0 + W] : none;
+  int4 dcur = w0 < it_end ? desc[w0] : none;
+  int4 dnx = w0 + W < it_end ? desc[w0 + W] : none;
   fetch(dcur);
   commit(dcur, 0);
   __syncthreads();
   int buf = 0;
-  for (int it = w0; it < p.nrowblk; it += W) {
-    const int4 dnx2 = it + 2 * W < p.nrowblk ? desc[it + 2 * W] : none;
+  for (int it = w0; it < it_end; it += W) {
+    const int4 dnx2 = it + 2 * W < it_end ? desc[it + 2 * W] : none;
     fetch(dnx);   // in flight during this block's rows
     const int row0 = dcur.x, row1 = dcur.y;
     auto rows = [&](auto from_lds) {
@@ -544,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
       double s = 0.0;
       for (int wv = 0; wv < kWaves; ++wv)
         for (int sl = 0; sl < RPW; ++sl) s += sh[(wv * 64 + sl * LPR + l) * 2 + q];
-      p.partials[(size_t)w0 * R + slab * RS + j] = s;
+      p.partials[(size_t)(part * W + lw) * R + slab * RS + j] = s;
     }
   }
 }
