@@ -194,7 +194,11 @@ struct Launch {
     return GM_OK;
   }
   static gm_status finish(gm_batch* b, const gmb::BFinish& a) {
-    hipLaunchKernelGGL((gmb::k_bfinish<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
+    if constexpr (R >= 4) {
+      hipLaunchKernelGGL((gmb::k_bfinish2<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
+    } else {
+      hipLaunchKernelGGL((gmb::k_bfinish<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
+    }
     B_HIP(hipGetLastError());
     return GM_OK;
   }

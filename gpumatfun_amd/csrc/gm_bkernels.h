@@ -661,6 +661,50 @@ __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
   block_partial_cols<R>(nrm, p.partials);
 }
 
+// k_bfinish with two adjacent columns a thread (16-byte accesses; R / 2
+// threads a row, kBlock / (R / 2) rows a pass) and four rows' loads in flight
+// per thread.  The same update w = w - alpha_k v_k per element; the column
+// sums of w^2 run over another partition (fixed order).  R >= 4, no apart.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_bfinish2(BFinish p) {
+  static_assert(R >= 4 && R <= 2 * kBlock, "two columns a thread");
+  constexpr int P2 = R / 2, RPP = kBlock / P2, NR = 4;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int row0 = b * p.chunk;
+  const int row1 = min(row0 + p.chunk, p.nrows);
+  const int cp = (int)threadIdx.x % P2, rs = (int)threadIdx.x / P2;
+  const int c = 2 * cp;
+  const double al0 = p.al[(size_t)p.k * R + c], al1 = p.al[(size_t)p.k * R + c + 1];
+  double n0 = 0.0, n1 = 0.0;
+  for (int r = row0 + rs; r < row1; r += NR * RPP) {
+    d2v wv[NR], vv[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+      const int rh = r + h * RPP < row1 ? r + h * RPP : r;
+      wv[h] = ld_stream2<false>(p.w + (size_t)rh * R + c);
+      vv[h] = ld_stream2<false>(p.vk + (size_t)rh * R + c);
+    }
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+      if (r + h * RPP >= row1) break;
+      const double w0 = fma(-al0, vv[h].x, wv[h].x), w1 = fma(-al1, vv[h].y, wv[h].y);
+      st_stream2<false>(p.w + (size_t)(r + h * RPP) * R + c, w0, w1);
+      n0 = fma(w0, w0, n0);
+      n1 = fma(w1, w1, n1);
+    }
+  }
+  __shared__ double sh[2 * kBlock];
+  sh[2 * threadIdx.x] = n0;
+  sh[2 * threadIdx.x + 1] = n1;
+  __syncthreads();
+  for (int col = threadIdx.x; col < R; col += kBlock) {
+    const int pc = col >> 1, q = col & 1;
+    double sum = 0.0;
+    for (int k = 0; k < RPP; ++k) sum += sh[2 * (k * P2 + pc) + q];
+    p.partials[(size_t)blockIdx.x * R + col] = sum;
+  }
+}
+
 // Column norms of U (ld ldu, column offset applied) -> be[0][c].
 struct BNorm {
   const double* u;
