@@ -194,7 +194,7 @@ struct Launch {
     return GM_OK;
   }
   static gm_status finish(gm_batch* b, const gmb::BFinish& a) {
-    if constexpr (R >= 4) {
+    if constexpr (R >= 4 && R <= 64) {   // measured: 34.3 against 35.9 us at 32 columns; 2.43 against 2.40 ms at 256
       hipLaunchKernelGGL((gmb::k_bfinish2<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
     } else {
       hipLaunchKernelGGL((gmb::k_bfinish<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
@@ -746,14 +746,21 @@ gm_status gm_batch_create(const gm_batch_desc* d, gm_batch** out) {
       for (int xs = 0; xs <= (rc >= 32 ? 1 : 0); ++xs) {
         b->xs = xs != 0;
         gm_status s1 = dispatch_chunk(b, 0);                       // warm-up
-        (void)hipEventRecord(e0, st);
-        gm_status s2 = dispatch_chunk(b, 0);
-        (void)hipEventRecord(e1, st);
-        if (s1 != GM_OK || s2 != GM_OK || hipEventSynchronize(e1) != hipSuccess)
-          return fail_clean(s1 != GM_OK ? s1 : (s2 != GM_OK ? s2 : bfail(GM_EDEVICE, "autotune")));
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        const double per_col = (double)ms / rc;
+        if (s1 != GM_OK) return fail_clean(s1);
+        // the faster of two timed runs (the widths' costs per column lie
+        // within a few per cent of each other on large operators)
+        double ms_best = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+          (void)hipEventRecord(e0, st);
+          gm_status s2 = dispatch_chunk(b, 0);
+          (void)hipEventRecord(e1, st);
+          if (s2 != GM_OK || hipEventSynchronize(e1) != hipSuccess)
+            return fail_clean(s2 != GM_OK ? s2 : bfail(GM_EDEVICE, "autotune"));
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          ms_best = std::min(ms_best, (double)ms);
+        }
+        const double per_col = ms_best / rc;
         if (per_col < best) {
           best = per_col;
           best_rc = rc;
