@@ -256,17 +256,27 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
     B_HIP(hipGetLastError());
     B_TRY(breduce(b, R, b->be, 1, 0));                                   // beta_0
   }
+  // With the slab SpMM the basis V holds the unnormalised w_k (the finish
+  // writes w_{k+1} straight into V[k+1], the SpMM gathers it from there and
+  // forms v_k = w_k / beta_k where it needs it), so v_k is never stored: one
+  // vector pass less per Lanczos step.  V C then takes C / beta.
+  const bool ub = R >= 32 && b->xs;
+  auto Vk = [&](int k) { return b->V + (size_t)k * N * R; };
   for (int k = 0; k < b->m; ++k) {
     gmb::BSpmm a = base;
-    const double* wsrc = k == 0 ? (b->K ? b->Ud : b->Usol) + c0 : Wb[(k - 1) & 1];
+    const double* wsrc = k == 0 ? (b->K ? b->Ud : b->Usol) + c0 : (ub ? Vk(k) : Wb[(k - 1) & 1]);
     const size_t ldw = k == 0 ? ldu : (size_t)R;
     double* wnext = Wb[k & 1];
     a.k = k;
-    a.vk = b->V + (size_t)k * N * R;
-    a.vkm1 = k > 0 ? b->V + (size_t)(k - 1) * N * R : nullptr;
+    a.vk = Vk(k);
+    a.vkm1 = k > 0 ? Vk(k - 1) : nullptr;
     a.x = wsrc;
     a.ldx = ldw;
+    a.vw = ub ? 1 : 0;
+    // kFirst/kOnly: with ub only k = 0 stores (the raw w_0)
+    double* vk_first = (ub && k > 0) ? nullptr : Vk(k);
     if (d == 0) {
+      a.vk = vk_first;
       a.y = wnext;
       a.ldy = R;
       B_TRY(Launch<R>::spmm(b, gmb::kOnly, a));
@@ -275,7 +285,9 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
       a.y = Tb[0];
       a.ldy = R;
       a.gi = b->gamma[1];
+      a.vk = vk_first;
       B_TRY(Launch<R>::spmm(b, gmb::kFirst, a));
+      a.vk = Vk(k);
       for (int i = 2; i <= d; ++i) {
         a.x = Tb[(i - 2) & 1];
         a.ldx = R;
@@ -291,7 +303,9 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
     }
     gmb::BFinish f{};
     f.w = wnext;
-    f.vk = a.vk;
+    f.vk = Vk(k);
+    f.vw = ub ? 1 : 0;
+    f.wout = (ub && k + 1 < b->m) ? Vk(k + 1) : nullptr;
     f.al = b->al;
     f.be = b->be;
     f.k = k;
@@ -334,6 +348,11 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
   hipLaunchKernelGGL(gmb::k_bcollapse<0>, dim3(std::min(1024, cdivi((long long)b->nd * b->m * R, gm::kBlock))),
                      dim3(gm::kBlock), 0, b->stream, cc);
   B_HIP(hipGetLastError());
+  if (ub) {
+    hipLaunchKernelGGL(gmb::k_bscale_c<0>, dim3(cdivi((long long)b->nd * b->m * R, gm::kBlock)),
+                       dim3(gm::kBlock), 0, b->stream, b->C, b->be, b->nd, b->m, R);
+    B_HIP(hipGetLastError());
+  }
   gmb::BVC v{};
   v.V = b->V;
   v.C = b->C;

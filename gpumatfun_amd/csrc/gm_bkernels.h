@@ -56,6 +56,10 @@ struct BSpmm {
   const double* bpart;
   int bG;
   unsigned* status;
+  // k_bspmm_xs only: the basis V holds the unnormalised w_j (v_j = w_j / beta_j
+  // formed where read); v_k is then not stored (vk = null), or at k = 0 stored
+  // as the raw w_0
+  int vw;
 };
 
 // Single-use vector streams of the SpMM (v_k and y written, v_{k-1} read,
@@ -397,13 +401,17 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
   const int rslot = wave * RPW + lane / LPR;
   const int c = slab * RS + 2 * (lane % LPR);   // columns c, c + 1
   const int4* desc = reinterpret_cast<const int4*>(p.rowblk);
-  double bet[2], inv[2], bprev[2], dot[2];
+  double bet[2], inv[2], bprev[2], dot[2], bk[2], bkm1[2];
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     bet[q] = (MODE == kFirst || MODE == kOnly) ? p.be[(size_t)p.k * R + c + q] : 0.0;
     inv[q] = safe_inv(bet[q]) * p.g0;
     bprev[q] = ((MODE == kLast || MODE == kOnly) && p.k > 0) ? p.be[(size_t)p.k * R + c + q] : 0.0;
     dot[q] = 0.0;
+    // beta_k and beta_{k-1} that turn the stored w_k, w_{k-1} into v_k, v_{k-1}
+    bk[q] = (p.vw && (MODE == kMid || MODE == kLast)) ? p.be[(size_t)p.k * R + c + q] : 1.0;
+    bkm1[q] = (p.vw && (MODE == kLast || MODE == kOnly) && p.k > 0)
+                  ? p.be[(size_t)(p.k - 1) * R + c + q] : 1.0;
   }
   __shared__ double sval[2][kBTile];
   __shared__ int scol[2][kBTile];
@@ -490,7 +498,16 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
             }
           }
         }
-        const double own0[2] = {o0.x, o0.y}, own1[2] = {o1.x, o1.y};
+        double own0[2] = {o0.x, o0.y}, own1[2] = {o1.x, o1.y};
+        if (p.vw) {
+          // the same v_k = w_k / beta_k and v_{k-1} that the normalised basis
+          // would hold, bit for bit
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            if (MODE == kMid || MODE == kLast) own0[q] = safe_div(own0[q], bk[q]);
+            if ((MODE == kLast || MODE == kOnly) && p.k > 0) own1[q] = safe_div(own1[q], bkm1[q]);
+          }
+        }
         double vo[2] = {0.0, 0.0}, yo[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -521,7 +538,10 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
             yo[q] = yv;
           }
         }
-        if (MODE == kFirst || MODE == kOnly) st_stream2<kNT>(p.vk + iv, vo[0], vo[1]);
+        if ((MODE == kFirst || MODE == kOnly) && p.vk) {
+          if (p.vw) st_stream2<kNT>(p.vk + iv, o0.x, o0.y);   // the raw w_0
+          else st_stream2<kNT>(p.vk + iv, vo[0], vo[1]);
+        }
         if (MODE == kHorner || MODE == kCheb) {
           *reinterpret_cast<d2v*>(p.y + (size_t)r * p.ldy + c) = d2v{yo[0], yo[1]};
         } else {
@@ -606,6 +626,10 @@ struct BFinish {
   const double* apart;
   int aG;
   double* al_out;
+  // vw: vk holds the unnormalised w_k (v_k = w_k / beta_k formed here, bit for
+  // bit the normalised basis' value); wout: where w_{k+1} goes (null: over w)
+  int vw;
+  double* wout;
 };
 
 template <int R>
@@ -625,11 +649,14 @@ __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
     if (blockIdx.x == 0)
       for (int c = threadIdx.x; c < R; c += kBlock) p.al_out[(size_t)p.k * R + c] = fused[c];
   }
+  double bk[CL];
 #pragma unroll
   for (int q = 0; q < CL; ++q) {
     al[q] = p.apart ? fused[c0 + q * CPW] : p.al[(size_t)p.k * R + c0 + q * CPW];
     nrm[q] = 0.0;
+    bk[q] = p.vw ? p.be[(size_t)p.k * R + c0 + q * CPW] : 1.0;
   }
+  double* wo = p.wout ? p.wout : p.w;
   // NR rows per pass, all loads before the stores (w and vk are distinct
   // buffers, which the compiler cannot assume); a lane's rows are added into
   // nrm in the same order whatever NR is
@@ -652,8 +679,9 @@ __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
       if (r + h * STEP >= row1) break;
 #pragma unroll
       for (int q = 0; q < CL; ++q) {
-        const double w = fma(-al[q], vv[h][q], wv[h][q]);
-        p.w[(size_t)(r + h * STEP) * R + c0 + q * CPW] = w;
+        const double v = p.vw ? safe_div(vv[h][q], bk[q]) : vv[h][q];
+        const double w = fma(-al[q], v, wv[h][q]);
+        wo[(size_t)(r + h * STEP) * R + c0 + q * CPW] = w;
         nrm[q] = fma(w, w, nrm[q]);
       }
     }
@@ -675,6 +703,9 @@ __global__ __launch_bounds__(kBlock) void k_bfinish2(BFinish p) {
   const int cp = (int)threadIdx.x % P2, rs = (int)threadIdx.x / P2;
   const int c = 2 * cp;
   const double al0 = p.al[(size_t)p.k * R + c], al1 = p.al[(size_t)p.k * R + c + 1];
+  const double bk0 = p.vw ? p.be[(size_t)p.k * R + c] : 1.0;
+  const double bk1 = p.vw ? p.be[(size_t)p.k * R + c + 1] : 1.0;
+  double* wo = p.wout ? p.wout : p.w;
   double n0 = 0.0, n1 = 0.0;
   for (int r = row0 + rs; r < row1; r += NR * RPP) {
     d2v wv[NR], vv[NR];
@@ -687,8 +718,10 @@ __global__ __launch_bounds__(kBlock) void k_bfinish2(BFinish p) {
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
       if (r + h * RPP >= row1) break;
-      const double w0 = fma(-al0, vv[h].x, wv[h].x), w1 = fma(-al1, vv[h].y, wv[h].y);
-      st_stream2<false>(p.w + (size_t)(r + h * RPP) * R + c, w0, w1);
+      const double v0 = p.vw ? safe_div(vv[h].x, bk0) : vv[h].x;
+      const double v1 = p.vw ? safe_div(vv[h].y, bk1) : vv[h].y;
+      const double w0 = fma(-al0, v0, wv[h].x), w1 = fma(-al1, v1, wv[h].y);
+      st_stream2<false>(wo + (size_t)(r + h * RPP) * R + c, w0, w1);
       n0 = fma(w0, w0, n0);
       n1 = fma(w1, w1, n1);
     }
@@ -703,6 +736,17 @@ __global__ __launch_bounds__(kBlock) void k_bfinish2(BFinish p) {
     for (int k = 0; k < RPP; ++k) sum += sh[2 * (k * P2 + pc) + q];
     p.partials[(size_t)blockIdx.x * R + col] = sum;
   }
+}
+
+// C[j][k][c] /= beta_k[c] (nd x m x R): the V C product over a basis that holds
+// the unnormalised w_k.
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_bscale_c(double* C, const double* be, int nd, int m,
+                                                     int R) {
+  const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (long long)nd * m * R) return;
+  const int c = (int)(e % R), k = (int)((e / R) % m);
+  C[e] = safe_div(C[e], be[(size_t)k * R + c]);
 }
 
 // Column norms of U (ld ldu, column offset applied) -> be[0][c].
