@@ -171,11 +171,11 @@ struct Launch {
         a.nrowblk = b->nrowblk_xs;
         dim3 g(b->G_xs), t(gm::kBlock);
         switch (mode) {
-          case gmb::kFirst: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kFirst, kNT>), g, t, 0, b->stream, a); break;
-          case gmb::kMid: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kMid, kNT>), g, t, 0, b->stream, a); break;
-          case gmb::kLast: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kLast, kNT>), g, t, 0, b->stream, a); break;
-          case gmb::kOnly: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kOnly, kNT>), g, t, 0, b->stream, a); break;
-          default: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kHorner, kNT>), g, t, 0, b->stream, a); break;
+          case gmb::kFirst: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kFirst, kNT, true>), g, t, 0, b->stream, a); break;
+          case gmb::kMid: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kMid, kNT, true>), g, t, 0, b->stream, a); break;
+          case gmb::kLast: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kLast, kNT, true>), g, t, 0, b->stream, a); break;
+          case gmb::kOnly: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kOnly, kNT, true>), g, t, 0, b->stream, a); break;
+          default: hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kHorner, kNT, true>), g, t, 0, b->stream, a); break;
         }
         B_HIP(hipGetLastError());
         return GM_OK;
@@ -193,12 +193,18 @@ struct Launch {
     B_HIP(hipGetLastError());
     return GM_OK;
   }
-  static gm_status finish(gm_batch* b, const gmb::BFinish& a) {
+  // vw: the basis holds the unnormalised w_k (the slab SpMM's dataflow)
+  template <bool VW>
+  static void finish_launch(gm_batch* b, const gmb::BFinish& a) {
     if constexpr (R >= 4 && R <= 64) {   // measured: 34.3 against 35.9 us at 32 columns; 2.43 against 2.40 ms at 256
-      hipLaunchKernelGGL((gmb::k_bfinish2<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
+      hipLaunchKernelGGL((gmb::k_bfinish2<R, VW>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
     } else {
-      hipLaunchKernelGGL((gmb::k_bfinish<R>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
+      hipLaunchKernelGGL((gmb::k_bfinish<R, VW>), dim3(b->G_sp), dim3(gm::kBlock), 0, b->stream, a);
     }
+  }
+  static gm_status finish(gm_batch* b, const gmb::BFinish& a, bool vw) {
+    if (vw) finish_launch<true>(b, a);
+    else finish_launch<false>(b, a);
     B_HIP(hipGetLastError());
     return GM_OK;
   }
@@ -272,7 +278,6 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
     a.vkm1 = k > 0 ? Vk(k - 1) : nullptr;
     a.x = wsrc;
     a.ldx = ldw;
-    a.vw = ub ? 1 : 0;
     // kFirst/kOnly: with ub only k = 0 stores (the raw w_0)
     double* vk_first = (ub && k > 0) ? nullptr : Vk(k);
     if (d == 0) {
@@ -304,7 +309,6 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
     gmb::BFinish f{};
     f.w = wnext;
     f.vk = Vk(k);
-    f.vw = ub ? 1 : 0;
     f.wout = (ub && k + 1 < b->m) ? Vk(k + 1) : nullptr;
     f.al = b->al;
     f.be = b->be;
@@ -313,7 +317,7 @@ gm_status enqueue_chunk(gm_batch* b, int c0) {
     f.chunk = b->chunk_sp;
     f.partials = b->partials;
     f.status = b->status;
-    B_TRY(Launch<R>::finish(b, f));
+    B_TRY(Launch<R>::finish(b, f, ub));
     B_TRY(breduce(b, R, b->be + (size_t)(k + 1) * R, 1, 1));          // beta_{k+1}
   }
   // shifted solves

@@ -56,10 +56,6 @@ struct BSpmm {
   const double* bpart;
   int bG;
   unsigned* status;
-  // k_bspmm_xs only: the basis V holds the unnormalised w_j (v_j = w_j / beta_j
-  // formed where read); v_k is then not stored (vk = null), or at k = 0 stored
-  // as the raw w_0
-  int vw;
 };
 
 // Single-use vector streams of the SpMM (v_k and y written, v_{k-1} read,
@@ -379,11 +375,14 @@ inline void bspmm_xs_blocks(const int* rowptr, int n, std::vector<int>& desc) {
 //  - Per row the sums are k_bspmm's (the same nonzeros in the same order, the
 //    same epilogue).  Column sums go to partials [G/S][R] (row
 //    part * G/8 + b / 8 of the slab's columns), reduced after the kernel.
+//  - VW: the basis holds the unnormalised w_j; v_k = w_k / beta_k and v_{k-1}
+//    are formed where read (the bits the stored v_k would have), and kFirst /
+//    kOnly store nothing (vk = null) or, at k = 0, the raw w_0.
 //  - p.rowblk holds p.nrowblk descriptors (bspmm_xs_blocks); a block that does
 //    not fit the LDS tile (one row of more than kBTile nonzeros) is read from
 //    global memory.  Requires: G a multiple of S; x, y, vk, vkm1, z 16-byte
 //    aligned and every row stride even; no fused beta (bpart).
-template <int R, int MODE, bool kNT>
+template <int R, int MODE, bool kNT, bool VW = false>
 __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
   constexpr int RS = 32, S = R / RS, LPR = RS / 2, RPW = 64 / LPR, RPB = RPW * kWaves;
   static_assert(R % RS == 0 && S >= 1 && S <= 8, "R = 32 .. 256 in slabs of 32");
@@ -409,8 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
     bprev[q] = ((MODE == kLast || MODE == kOnly) && p.k > 0) ? p.be[(size_t)p.k * R + c + q] : 0.0;
     dot[q] = 0.0;
     // beta_k and beta_{k-1} that turn the stored w_k, w_{k-1} into v_k, v_{k-1}
-    bk[q] = (p.vw && (MODE == kMid || MODE == kLast)) ? p.be[(size_t)p.k * R + c + q] : 1.0;
-    bkm1[q] = (p.vw && (MODE == kLast || MODE == kOnly) && p.k > 0)
+    bk[q] = (VW && (MODE == kMid || MODE == kLast)) ? p.be[(size_t)p.k * R + c + q] : 1.0;
+    bkm1[q] = (VW && (MODE == kLast || MODE == kOnly) && p.k > 0)
                   ? p.be[(size_t)(p.k - 1) * R + c + q] : 1.0;
   }
   __shared__ double sval[2][kBTile];
@@ -499,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
           }
         }
         double own0[2] = {o0.x, o0.y}, own1[2] = {o1.x, o1.y};
-        if (p.vw) {
+        if (VW) {
           // the same v_k = w_k / beta_k and v_{k-1} that the normalised basis
           // would hold, bit for bit
 #pragma unroll
@@ -538,10 +537,9 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
             yo[q] = yv;
           }
         }
-        if ((MODE == kFirst || MODE == kOnly) && p.vk) {
-          if (p.vw) st_stream2<kNT>(p.vk + iv, o0.x, o0.y);   // the raw w_0
-          else st_stream2<kNT>(p.vk + iv, vo[0], vo[1]);
-        }
+        if ((MODE == kFirst || MODE == kOnly) && !VW) st_stream2<kNT>(p.vk + iv, vo[0], vo[1]);
+        if ((MODE == kFirst || MODE == kOnly) && VW && p.vk)
+          st_stream2<kNT>(p.vk + iv, o0.x, o0.y);   // the raw w_0
         if (MODE == kHorner || MODE == kCheb) {
           *reinterpret_cast<d2v*>(p.y + (size_t)r * p.ldy + c) = d2v{yo[0], yo[1]};
         } else {
@@ -598,6 +596,16 @@ void bspmm_xs_occupancy(int* per_cu) {
   occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kLast, true>));
   occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kOnly, true>));
   occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kHorner, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kFirst, true, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kMid, true, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kLast, true, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kOnly, true, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kFirst, false, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kMid, false, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kLast, false, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kOnly, false, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kHorner, true, true>));
+  occ(reinterpret_cast<const void*>(&k_bspmm_xs<R, kHorner, false, true>));
 }
 
 inline int bspmm_xs_grid(int dev, int nrowblk) {
@@ -626,13 +634,13 @@ struct BFinish {
   const double* apart;
   int aG;
   double* al_out;
-  // vw: vk holds the unnormalised w_k (v_k = w_k / beta_k formed here, bit for
-  // bit the normalised basis' value); wout: where w_{k+1} goes (null: over w)
-  int vw;
+  // wout: where w_{k+1} goes (null: over w).  With VW (the kernels' template
+  // flag) vk holds the unnormalised w_k and v_k = w_k / beta_k is formed here,
+  // bit for bit the normalised basis' value.
   double* wout;
 };
 
-template <int R>
+template <int R, bool VW = false>
 __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
   constexpr int CPW = R < 64 ? R : 64;
   constexpr int RPW = 64 / CPW;
@@ -654,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
   for (int q = 0; q < CL; ++q) {
     al[q] = p.apart ? fused[c0 + q * CPW] : p.al[(size_t)p.k * R + c0 + q * CPW];
     nrm[q] = 0.0;
-    bk[q] = p.vw ? p.be[(size_t)p.k * R + c0 + q * CPW] : 1.0;
+    bk[q] = VW ? p.be[(size_t)p.k * R + c0 + q * CPW] : 1.0;
   }
   double* wo = p.wout ? p.wout : p.w;
   // NR rows per pass, all loads before the stores (w and vk are distinct
@@ -679,7 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
       if (r + h * STEP >= row1) break;
 #pragma unroll
       for (int q = 0; q < CL; ++q) {
-        const double v = p.vw ? safe_div(vv[h][q], bk[q]) : vv[h][q];
+        const double v = VW ? safe_div(vv[h][q], bk[q]) : vv[h][q];
         const double w = fma(-al[q], v, wv[h][q]);
         wo[(size_t)(r + h * STEP) * R + c0 + q * CPW] = w;
         nrm[q] = fma(w, w, nrm[q]);
@@ -693,7 +701,7 @@ __global__ __launch_bounds__(kBlock) void k_bfinish(BFinish p) {
 // threads a row, kBlock / (R / 2) rows a pass) and four rows' loads in flight
 // per thread.  The same update w = w - alpha_k v_k per element; the column
 // sums of w^2 run over another partition (fixed order).  R >= 4, no apart.
-template <int R>
+template <int R, bool VW = false>
 __global__ __launch_bounds__(kBlock) void k_bfinish2(BFinish p) {
   static_assert(R >= 4 && R <= 2 * kBlock, "two columns a thread");
   constexpr int P2 = R / 2, RPP = kBlock / P2, NR = 4;
@@ -703,8 +711,8 @@ __global__ __launch_bounds__(kBlock) void k_bfinish2(BFinish p) {
   const int cp = (int)threadIdx.x % P2, rs = (int)threadIdx.x / P2;
   const int c = 2 * cp;
   const double al0 = p.al[(size_t)p.k * R + c], al1 = p.al[(size_t)p.k * R + c + 1];
-  const double bk0 = p.vw ? p.be[(size_t)p.k * R + c] : 1.0;
-  const double bk1 = p.vw ? p.be[(size_t)p.k * R + c + 1] : 1.0;
+  const double bk0 = VW ? p.be[(size_t)p.k * R + c] : 1.0;
+  const double bk1 = VW ? p.be[(size_t)p.k * R + c + 1] : 1.0;
   double* wo = p.wout ? p.wout : p.w;
   double n0 = 0.0, n1 = 0.0;
   for (int r = row0 + rs; r < row1; r += NR * RPP) {
@@ -718,8 +726,8 @@ __global__ __launch_bounds__(kBlock) void k_bfinish2(BFinish p) {
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
       if (r + h * RPP >= row1) break;
-      const double v0 = p.vw ? safe_div(vv[h].x, bk0) : vv[h].x;
-      const double v1 = p.vw ? safe_div(vv[h].y, bk1) : vv[h].y;
+      const double v0 = VW ? safe_div(vv[h].x, bk0) : vv[h].x;
+      const double v1 = VW ? safe_div(vv[h].y, bk1) : vv[h].y;
       const double w0 = fma(-al0, v0, wv[h].x), w1 = fma(-al1, v1, wv[h].y);
       st_stream2<false>(wo + (size_t)(r + h * RPP) * R + c, w0, w1);
       n0 = fma(w0, w0, n0);

@@ -82,11 +82,15 @@ struct FusedBeta {
 
 // Per-launch constants of the row epilogue: 1/beta_k, beta_k (kFirst, kOnly),
 // beta_k for the -beta_k v_{k-1} term (kLast, kOnly), and the scale of A x.
+// VW: the basis holds the unnormalised w_j (the segmented form): v_k = w_k /
+// beta_k and v_{k-1} are formed where read, the bits the stored v_k would
+// have, and kFirst / kOnly store nothing (vk = null) or, at k = 0, the raw w_0.
 struct SpmmScalars {
   double2 inv, bet, bprev, sx;
+  double2 bk, bkm1;   // VW: beta_k (kMid, kLast) and beta_{k-1} (kLast, kOnly)
 };
 
-template <int MODE>
+template <int MODE, bool VW = false>
 __device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p, const double* aabb,
                                                     const FusedBeta& fb) {
   SpmmScalars c;
@@ -116,6 +120,11 @@ __device__ __forceinline__ SpmmScalars spmm_scalars(const SpmmArgs& p, const dou
     c.bprev = (MODE == kOnly && fb.part) ? c.bet
                                          : d2(aabb[p.k + 2 * p.ld], aabb[p.k + 3 * p.ld]);
   c.sx = (MODE == kFirst || MODE == kOnly) ? d2(p.g0 * c.inv.x, p.g0 * c.inv.y) : d2(1.0, 1.0);
+  c.bk = d2(1.0, 1.0);
+  c.bkm1 = d2(1.0, 1.0);
+  if (VW && (MODE == kMid || MODE == kLast)) c.bk = d2(aabb[p.k + 2 * p.ld], aabb[p.k + 3 * p.ld]);
+  if (VW && (MODE == kLast || MODE == kOnly) && p.k > 0)
+    c.bkm1 = d2(aabb[p.k - 1 + 2 * p.ld], aabb[p.k - 1 + 3 * p.ld]);
   return c;
 }
 
@@ -136,12 +145,18 @@ __device__ __forceinline__ void spmm_row_load(const SpmmArgs& p, int r, double2&
 
 // Row r's output from acc = (A x)_r and the operands of spmm_row_load;
 // kLast/kOnly add y_r v_r to dot.
-template <int MODE>
+template <int MODE, bool VW = false>
 __device__ __forceinline__ void spmm_row_store(const SpmmArgs& p, const SpmmScalars& c, int r,
                                                double2 acc, double2 o0, double2 o1, double2& dot) {
+  if (VW) {
+    if (MODE == kMid || MODE == kLast) o0 = d2(safe_div(o0.x, c.bk.x), safe_div(o0.y, c.bk.y));
+    if ((MODE == kLast || MODE == kOnly) && p.k > 0)
+      o1 = d2(safe_div(o1.x, c.bkm1.x), safe_div(o1.y, c.bkm1.y));
+  }
   if (MODE == kFirst) {
     const double2 v = d2(safe_div(o0.x, c.bet.x), safe_div(o0.y, c.bet.y));
-    p.vk[r] = v;
+    if (!VW) p.vk[r] = v;
+    else if (p.vk) p.vk[r] = o0;
     p.y[r] = d2(fma(p.gi, v.x, acc.x * c.sx.x), fma(p.gi, v.y, acc.y * c.sx.y));
   } else if (MODE == kMid) {
     p.y[r] = d2(fma(p.gi, o0.x, acc.x), fma(p.gi, o0.y, acc.y));
@@ -149,7 +164,8 @@ __device__ __forceinline__ void spmm_row_store(const SpmmArgs& p, const SpmmScal
     double2 v;
     if (MODE == kOnly) {
       v = d2(safe_div(o0.x, c.bet.x), safe_div(o0.y, c.bet.y));
-      p.vk[r] = v;
+      if (!VW) p.vk[r] = v;
+      else if (p.vk) p.vk[r] = o0;
       acc.x *= c.sx.x;
       acc.y *= c.sx.y;
     } else {
@@ -173,12 +189,12 @@ __device__ __forceinline__ void spmm_row_store(const SpmmArgs& p, const SpmmScal
   }
 }
 
-template <int MODE>
+template <int MODE, bool VW = false>
 __device__ __forceinline__ void spmm_row_out(const SpmmArgs& p, const SpmmScalars& c, int r,
                                              double2 acc, double2& dot) {
   double2 o0, o1;
   spmm_row_load<MODE>(p, r, o0, o1);
-  spmm_row_store<MODE>(p, c, r, acc, o0, o1, dot);
+  spmm_row_store<MODE, VW>(p, c, r, acc, o0, o1, dot);
 }
 
 // Row-group schedule: LANES consecutive lanes share one row (coalesced
@@ -229,7 +245,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs p) {
 // instead (strided FMAs, then a fixed-order block tree).
 // SEG: the rows belong to independent problems, no block straddles two, and
 // each block takes its Lanczos scalars from its problem's aabb and writes its
-// partial at its row-block index (so a problem's partials are contiguous).
+// partial at its row-block index (so a problem's partials are contiguous);
+// the basis holds the unnormalised w_k (VW of spmm_row_store).
 // C16: column indices are stored as 16-bit offsets from the block's first row
 // (banded orderings such as symrcm keep them small), 2 bytes per nonzero
 // instead of 4.  PRE: the epilogue's own-row operands are loaded before the
@@ -248,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
   const int t = threadIdx.x;
   const double* aabb = SEG ? p.aabb + (size_t)p.blk_prob[b] * p.aabb_stride : p.aabb;
   const FusedBeta fb{SEG ? nullptr : p.bpart, p.bG, p.beta_out, blockIdx.x == 0};
-  const SpmmScalars c = spmm_scalars<MODE>(p, aabb, fb);
+  const SpmmScalars c = spmm_scalars<MODE, SEG>(p, aabb, fb);
   double2 dot = d2(0.0, 0.0);
 
   if (j1 - j0 <= TILE) {
@@ -292,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
         acc.y += q.y;
       }
       if (!PRE) spmm_row_load<MODE>(p, r, o0, o1);
-      spmm_row_store<MODE>(p, c, r, acc, o0, o1, dot);
+      spmm_row_store<MODE, SEG>(p, c, r, acc, o0, o1, dot);
     }
   } else {
     double2 acc = d2(0.0, 0.0);
@@ -303,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
       acc.y = fma(av, xv.y, acc.y);
     }
     acc = block_sum2(acc);
-    if (t == 0) spmm_row_out<MODE>(p, c, r0, acc, dot);
+    if (t == 0) spmm_row_out<MODE, SEG>(p, c, r0, acc, dot);
   }
 
   if (MODE == kLast || MODE == kOnly) block_partial2_at(dot, p.partials, SEG ? b : (int)blockIdx.x);

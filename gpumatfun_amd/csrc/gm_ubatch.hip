@@ -41,6 +41,8 @@ constexpr int kTile = 1024;   // nonzeros per row block without the autotuner
 
 // w <- w - alpha_k v_k with the problem's alpha; partial ||w||^2 per block.
 // build_subspace.cu:63-76 (per problem).
+// vk holds the unnormalised w_k (v_k = w_k / beta_k formed here, the bits of
+// the normalised basis) and w_{k+1} goes to wout (null: over w).
 struct UFinish {
   double2* w;
   const double2* vk;
@@ -50,6 +52,7 @@ struct UFinish {
   const int* rowblk;
   const int* blk_prob;
   double* partials;
+  double2* wout;
 };
 
 template <int kUnused = 0>
@@ -60,11 +63,12 @@ __global__ __launch_bounds__(kBlock) void k_ufinish(UFinish p) {
   const double2 al = d2(aabb[p.k], aabb[p.k + p.ld]);
   double2 nrm = d2(0.0, 0.0);
   if (r < p.rowblk[b + 1]) {
-    const double2 v = p.vk[r];
+    double2 v = p.vk[r];
+    v = d2(safe_div(v.x, aabb[p.k + 2 * p.ld]), safe_div(v.y, aabb[p.k + 3 * p.ld]));
     double2 w = p.w[r];
     w.x = fma(-al.x, v.x, w.x);
     w.y = fma(-al.y, v.y, w.y);
-    p.w[r] = w;
+    (p.wout ? p.wout : p.w)[r] = w;
     nrm.x = w.x * w.x;
     nrm.y = w.y * w.y;
   }
@@ -105,6 +109,20 @@ __global__ __launch_bounds__(kBlock) void k_uvc(UVC p) {
   }
 #pragma unroll
   for (int j = 0; j < ND; ++j) p.Z[(size_t)j * p.ldv + r] = acc[j];
+}
+
+// C[p][j][k] /= beta_k of problem p (both columns): V C over a basis that
+// holds the unnormalised w_k.
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_uscale_c(double2* C, const double* aabb,
+                                                     size_t aabb_stride, int ld, int nd, int m,
+                                                     int P) {
+  const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (long long)P * nd * m) return;
+  const int k = (int)(e % m), q = (int)(e / ((long long)nd * m));
+  const double* ab = aabb + (size_t)q * aabb_stride;
+  const double2 cv = C[e];
+  C[e] = d2(safe_div(cv.x, ab[k + 2 * ld]), safe_div(cv.y, ab[k + 3 * ld]));
 }
 
 // Fisher-KPP update (update_U.cu:8-16, no deflation) or, when prepare, the
@@ -334,13 +352,17 @@ gm_status enqueue_ustep(gm_ubatch* u) {
   const int d = u->nd - 1;
   const size_t N = (size_t)u->N;
   const int ld = u->m + 1;
+  // The basis V holds the unnormalised w_k: the finish writes w_{k+1} straight
+  // into V[k+1], the SpMM gathers it there and forms v_k = w_k / beta_k where
+  // it needs it (the same bits), so v_k is never stored; V C takes C / beta.
+  auto Vk = [&](int k) { return u->V + (size_t)k * N; };
   for (int k = 0; k < u->m; ++k) {
-    const double2* wsrc = k == 0 ? u->U : u->W[(k - 1) & 1];
+    const double2* wsrc = k == 0 ? u->U : Vk(k);
     double2* wnext = u->W[k & 1];
     gm::SpmmArgs a = uspmm_base(u);
     a.k = k;
-    a.vk = u->V + (size_t)k * N;
-    a.vkm1 = k > 0 ? u->V + (size_t)(k - 1) * N : nullptr;
+    a.vk = k == 0 ? Vk(0) : nullptr;   // kFirst/kOnly: only the raw w_0 is stored
+    a.vkm1 = k > 0 ? Vk(k - 1) : nullptr;
     a.x = wsrc;
     if (d == 0) {
       a.y = wnext;
@@ -349,6 +371,7 @@ gm_status enqueue_ustep(gm_ubatch* u) {
       a.y = u->T[0];
       a.gi = u->gamma[1];
       U_TRY(uspmm<gm::kFirst>(u, a));
+      a.vk = Vk(k);
       for (int i = 2; i <= d; ++i) {
         a.x = u->T[(i - 2) & 1];
         a.y = u->T[(i - 1) & 1];
@@ -362,7 +385,8 @@ gm_status enqueue_ustep(gm_ubatch* u) {
     U_TRY(ureduce(u, u->aabb + k, ld, 0, 0));                          // alpha_k
     gmu::UFinish f{};
     f.w = wnext;
-    f.vk = a.vk;
+    f.vk = Vk(k);
+    f.wout = k + 1 < u->m ? Vk(k + 1) : nullptr;
     f.aabb = u->aabb;
     f.aabb_stride = aabb_stride(u);
     f.ld = ld;
@@ -422,6 +446,11 @@ gm_status enqueue_ustep(gm_ubatch* u) {
   c.c_ps = (size_t)u->nd * u->m;
   hipLaunchKernelGGL(gm::k_collapse<0>, dim3(std::max(1, std::min(64, (u->nd * u->m + 255) / 256)), u->P),
                      dim3(gm::kBlock), 0, u->stream, c);
+  U_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gmu::k_uscale_c<0>,
+                     dim3((int)(((long long)u->P * u->nd * u->m + gm::kBlock - 1) / gm::kBlock)),
+                     dim3(gm::kBlock), 0, u->stream, u->C, u->aabb, aabb_stride(u), ld, u->nd, u->m,
+                     u->P);
   U_HIP(hipGetLastError());
   gmu::UVC v{};
   v.V = u->V;
