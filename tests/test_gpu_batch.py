@@ -205,6 +205,31 @@ def test_batch_graph_matches_eager_bitwise():
     g.delete()
 
 
+def test_batch_graph_matches_eager_bitwise_slabs():
+    """Graph replays of the slab-SpMM dataflow (64-column chunks, unnormalised basis,
+    nodes renumbered by the locality order) against eager steps, bit for bit, with
+    eager work and synchronisations between the replays."""
+    from cases import laplace3d_case
+    inp = laplace3d_case(side=16, m=24, n=32)
+    U = batch_U(inp, 32, seed0=20)
+    kw = dict(clamp=False, chunk_columns=64, column_slabs=True, locality_order=True)
+    a = BatchEngine.from_inputs(inp, U, **kw)
+    g = BatchEngine.from_inputs(inp, U, **kw)
+    assert g.info()["column_slabs"]
+    print("case batch graph vs eager, slabs", flush=True)
+    a.step(2)
+    g.step(2, use_graph=True)
+    assert np.array_equal(a.get_U(), g.get_U())
+    for _ in range(3):
+        a.step(1)
+        g.step(1, use_graph=True)
+        a.synchronize()
+        g.synchronize()
+        assert np.array_equal(a.get_U(), g.get_U())
+    a.delete()
+    g.delete()
+
+
 def test_batch_agrees_with_plain_engine():
     inp = dirichlet_case(side=48, m=30, n=16)
     U = batch_U(inp, 2)
