@@ -336,7 +336,7 @@ __device__ __forceinline__ d2v ld_stream2(const double* ptr) {
   else return *reinterpret_cast<const d2v*>(ptr);
 }
 
-// Row blocks of the XCD-slab form: at most kBTile nonzeros and kXsRows rows
+// Row blocks of the slab form: at most kBTile nonzeros and kXsRows rows
 // (four passes of a workgroup's 16 row slots), described to the kernel by
 // {first row, end row, first nonzero, nonzeros} (bspmm_xs_blocks).
 constexpr int kXsRows = 64;
@@ -380,12 +380,12 @@ inline void bspmm_xs_blocks(const int* rowptr, int n, std::vector<int>& desc) {
 //    kOnly store nothing (vk = null) or, at k = 0, the raw w_0.
 //  - p.rowblk holds p.nrowblk descriptors (bspmm_xs_blocks); a block that does
 //    not fit the LDS tile (one row of more than kBTile nonzeros) is read from
-//    global memory.  Requires: G a multiple of S; x, y, vk, vkm1, z 16-byte
+//    global memory.  Requires: G a multiple of 8; x, y, vk, vkm1, z 16-byte
 //    aligned and every row stride even; no fused beta (bpart).
 template <int R, int MODE, bool kNT, bool VW = false>
 __global__ __launch_bounds__(kBlock) void k_bspmm_xs(BSpmm p) {
   constexpr int RS = 32, S = R / RS, LPR = RS / 2, RPW = 64 / LPR, RPB = RPW * kWaves;
-  static_assert(R % RS == 0 && S >= 1 && S <= 8, "R = 32 .. 256 in slabs of 32");
+  static_assert(R % RS == 0 && S >= 1 && 8 % S == 0, "R = 32, 64, 128 or 256 in slabs of 32");
   constexpr int U = kUnroll;
   // workgroup b runs on XCD b % 8; it takes slab (b % 8) % S, and the XCD's
   // share of that slab is the contiguous part (b % 8) / S of its row blocks,
