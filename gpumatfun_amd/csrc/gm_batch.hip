@@ -86,9 +86,9 @@ struct gm_batch {
   std::vector<double> gamma;
   int *rowptr = nullptr, *col = nullptr, *rowblk = nullptr;
   int nrowblk = 0;                  // LDS-staged SpMM row blocks
-  // XCD-slab SpMM of 256-column chunks (gmb::k_bspmm_xs): its row-block
+  // slab SpMM of chunks of >= 32 columns (gmb::k_bspmm_xs): its row-block
   // descriptors (4 ints each), grid (a multiple of 8) and whether the step
-  // uses it
+  // uses it (with it, the basis is kept unnormalised)
   int* rowblk_xs = nullptr;
   int nrowblk_xs = 0, G_xs = 8;
   bool xs = false;
