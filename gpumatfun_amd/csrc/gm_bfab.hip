@@ -217,6 +217,77 @@ __global__ __launch_bounds__(256) void k_bpp_dot(BPD p) {
   gmb::block_partial_cols<R>(dot, p.partials);
 }
 
+// k_bpp_dot with two adjacent columns a lane (16-byte accesses; R / 2 lanes a
+// row, 128 / R rows a wavefront, twice k_bpp_dot's rows in flight).  The same
+// terms: each (row, l, column) sum over the nonzeros in order; a lane adds its
+// terms row by row, l by l, and the block partials [G][R] take the lanes of a
+// column in (wavefront, row slot) order.  R = 4 .. 128.
+template <int R>
+__global__ __launch_bounds__(256) void k_bpp_dot2(BPD p) {
+  static_assert(R >= 4 && R <= 128 && (R & (R - 1)) == 0, "two columns a lane");
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  constexpr int LPR = R / 2, RPW = 64 / LPR, U = 12;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int slot = lane / LPR, c = 2 * (lane % LPR);
+  const size_t slice = (size_t)p.N * R;
+  double d0 = 0.0, d1 = 0.0;
+  for (int i = (blockIdx.x * kWaves + wave) * RPW + slot; i < p.N; i += gridDim.x * kWaves * RPW) {
+    const int e0 = p.rowptr[i], e1 = p.rowptr[i + 1];
+    if (e1 - e0 <= U) {
+      double ev[U];
+      int ci[U];
+#pragma unroll
+      for (int t = 0; t < U; ++t) {
+        const bool ok = e0 + t < e1;
+        ev[t] = ok ? p.E[e0 + t] : 0.0;
+        ci[t] = ok ? p.col[e0 + t] : i;
+      }
+      for (int l = 0; l < p.m; ++l) {
+        const double* Vl = p.Vb + (size_t)l * slice + c;
+        dv2 xv[U];
+#pragma unroll
+        for (int t = 0; t < U; ++t)
+          xv[t] = e0 + t < e1 ? *reinterpret_cast<const dv2*>(Vl + (size_t)ci[t] * R) : dv2{0.0, 0.0};
+        const dv2 tv = *reinterpret_cast<const dv2*>(p.T + (size_t)l * slice + (size_t)i * R + c);
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int t = 0; t < U; ++t)
+          if (e0 + t < e1) {
+            s0 = fma(ev[t], xv[t].x, s0);
+            s1 = fma(ev[t], xv[t].y, s1);
+          }
+        d0 = fma(s0, tv.x, d0);
+        d1 = fma(s1, tv.y, d1);
+      }
+    } else {
+      // a long row: nonzero by nonzero for each l
+      for (int l = 0; l < p.m; ++l) {
+        const double* Vl = p.Vb + (size_t)l * slice + c;
+        double s0 = 0.0, s1 = 0.0;
+        for (int e = e0; e < e1; ++e) {
+          const dv2 x = *reinterpret_cast<const dv2*>(Vl + (size_t)p.col[e] * R);
+          s0 = fma(p.E[e], x.x, s0);
+          s1 = fma(p.E[e], x.y, s1);
+        }
+        const dv2 tv = *reinterpret_cast<const dv2*>(p.T + (size_t)l * slice + (size_t)i * R + c);
+        d0 = fma(s0, tv.x, d0);
+        d1 = fma(s1, tv.y, d1);
+      }
+    }
+  }
+  __shared__ double sh[2 * 256];
+  sh[2 * threadIdx.x] = d0;
+  sh[2 * threadIdx.x + 1] = d1;
+  __syncthreads();
+  for (int col = threadIdx.x; col < R; col += 256) {
+    const int lc = col >> 1, q = col & 1;
+    double sum = 0.0;
+    for (int w = 0; w < kWaves; ++w)
+      for (int sl = 0; sl < RPW; ++sl) sum += sh[2 * (w * 64 + sl * LPR + lc) + q];
+    p.partials[(size_t)blockIdx.x * R + col] = sum;
+  }
+}
+
 // Block partials of M[j][k][c] = sum_i V[j][i][c] EQ[i][k] for every Krylov
 // vector j (block y) on FP64 MFMA: each wavefront holds all KT x CT 16 x 16
 // tiles of (k, c), so one EQ load feeds CT chains and one V load KT chains;
@@ -623,7 +694,10 @@ gm_status fppm(gm_bfab* f) {
 template <int R>
 gm_status fpp(gm_bfab* f, const double* E) {
   gmf::BPD d{f->rowptr, f->col, E, f->sb.V, f->Tpp, f->n, f->m, f->pdot};
-  hipLaunchKernelGGL((gmf::k_bpp_dot<R>), dim3(f->G_pd), dim3(256), 0, f->stream, d);
+  if constexpr (R >= 4)
+    hipLaunchKernelGGL((gmf::k_bpp_dot2<R>), dim3(f->G_pd), dim3(256), 0, f->stream, d);
+  else
+    hipLaunchKernelGGL((gmf::k_bpp_dot<R>), dim3(f->G_pd), dim3(256), 0, f->stream, d);
   F_HIP(hipGetLastError());
   gm::ReduceArgs r{};
   r.partials = f->pdot;
