@@ -17,8 +17,9 @@
 //   cross       sum_k (Q^T g)_k q_k^T E V_b H_b[:, k] + (b <-> g), with
 //               H[:, k] = (lambda_k - T)^{-1} (f(lambda_k) beta0 e1 - C).
 // The P-P block is contracted in the Krylov bases, (Vg^T E Vb) against an
-// m x m matrix M per column: T = Vg M on FP64 MFMA, then (E Vb) . T in one
-// pass over the rows.
+// m x m matrix M per column: from 16 columns with m <= 48 as S = Vg^T (E Vb)
+// on FP64 MFMA in one pass over the rows, else T = Vg M on FP64 MFMA, then
+// (E Vb) . T in one pass over the rows.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -285,6 +286,156 @@ __global__ __launch_bounds__(256) void k_bpp_dot2(BPD p) {
     for (int w = 0; w < kWaves; ++w)
       for (int sl = 0; sl < RPW; ++sl) sum += sh[2 * (w * 64 + sl * LPR + lc) + q];
     p.partials[(size_t)blockIdx.x * R + col] = sum;
+  }
+}
+
+// The P-P sums without T: S_c = Vg_c^T (E Vb_c), m x m per column, on FP64
+// MFMA, contracted with M_c once a block has run its rows:
+//   sum_{k,l} S_c[k][l] M_c[k][l] = sum_l (E Vb[l]) . T[l].
+// A block (8 wavefronts) takes 16 columns (one 128-byte line of a row) and a
+// group of consecutive 4-row steps.  Per step it stages, for k, l < 16 KT,
+//   sV[k][r][j] = Vg[k][i][cb + j],  sE[l][r][j] = (E Vb[l])[i][cb + j]
+// (i = 4 step + r; zero beyond m or N) in LDS, a thread a column pair with
+// the 8 threads of one (k or l, r) reading a whole line; each E Vb value is
+// summed over its row's nonzeros in order.  Wavefront w then runs the MFMA
+// chains of columns cb + 2w, cb + 2w + 1 from LDS: A = sV[kt 16 + lane % 16]
+// [lane / 16] (rows k of S), B = sE[lt 16 + lane % 16][lane / 16] (columns
+// l); D register r of tile (kt, lt) holds S[kt 16 + 4 r + lane / 16][lt 16 +
+// lane % 16].  Two LDS buffers alternate, so one barrier a step suffices.
+// The wavefront's sum goes to partials[row group][c] (lane order fixed by the
+// butterfly).
+struct BPS {
+  const int* rowptr;
+  const int* col;
+  const double* E;
+  const double* Vg;   // [m][N][R]
+  const double* Vb;   // [m][N][R]
+  const double* M;    // [R][mk][mp]
+  int N, R, m, mk, mp, NG;
+  double* partials;   // [NG][R]
+};
+
+constexpr int kPsStride = 4 * 16 + 2;   // doubles per k (or l) row of a staged tile
+
+template <int KT>
+__global__ __launch_bounds__(512) void k_bpp_s(BPS p) {
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  constexpr int U = 9;   // the 9-point stencil; longer rows take the nonzero-by-nonzero path
+  constexpr int TILE = KT * 16 * kPsStride;
+  constexpr int PASSES = KT * 16 * 4 / 64;   // (k, r) pairs over the block's 64 thread groups
+  __shared__ double sV[2][TILE];
+  __shared__ double sE[2][TILE];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int halves = p.R >> 4;
+  const int cb = (int)(blockIdx.x % halves) * 16, rg = (int)(blockIdx.x / halves);
+  const int t16 = lane & 15, r4 = lane >> 4;
+  const size_t slice = (size_t)p.N * p.R;
+  const int steps = (p.N + 3) >> 2;
+  const int sp = (steps + p.NG - 1) / p.NG;
+  const int s0 = min(steps, rg * sp), s1 = min(steps, s0 + sp);
+  // fill role: column pair jp (columns cb + 2 jp, + 1), row r of the step,
+  // and (k or l) = x0 + 16 pass for the passes below
+  const int jp = tid & 7, grp = tid >> 3, fr = grp & 3, x0 = grp >> 2;
+  d4 acc[KT][KT][2];
+#pragma unroll
+  for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+    for (int lt = 0; lt < KT; ++lt)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) acc[kt][lt][q] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int s = s0; s < s1; ++s) {
+    double* bV = sV[s & 1];
+    double* bE = sE[s & 1];
+    const int i = 4 * s + fr;
+    const bool rowok = i < p.N;
+    int e0 = 0, e1 = 0;
+    if (rowok) {
+      e0 = p.rowptr[i];
+      e1 = p.rowptr[i + 1];
+    }
+    const bool shortrow = e1 - e0 <= U;
+    // padding slots of a short row point at the row itself, so that the
+    // gathers below are issued together without branches; their terms are
+    // dropped by a select
+    double ev[U];
+    int ci[U];
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+      const bool ok = shortrow && e0 + t < e1;
+      ev[t] = ok ? p.E[e0 + t] : 0.0;
+      ci[t] = ok ? p.col[e0 + t] : i;
+    }
+#pragma unroll
+    for (int ps = 0; ps < PASSES; ++ps) {
+      const int x = x0 + 16 * ps;
+      dv2 v = dv2{0.0, 0.0}, e = dv2{0.0, 0.0};
+      if (x < p.m && rowok) {
+        const size_t off = (size_t)x * slice + cb + 2 * jp;
+        v = *reinterpret_cast<const dv2*>(p.Vg + off + (size_t)i * p.R);
+        const double* Vx = p.Vb + off;
+        if (shortrow) {
+          dv2 xv[U];
+#pragma unroll
+          for (int t = 0; t < U; ++t) xv[t] = *reinterpret_cast<const dv2*>(Vx + (size_t)ci[t] * p.R);
+#pragma unroll
+          for (int t = 0; t < U; ++t) {
+            const bool in = e0 + t < e1;
+            const double sx = fma(ev[t], xv[t].x, e.x), sy = fma(ev[t], xv[t].y, e.y);
+            e.x = in ? sx : e.x;
+            e.y = in ? sy : e.y;
+          }
+        } else {
+          for (int q = e0; q < e1; ++q) {
+            const dv2 xq = *reinterpret_cast<const dv2*>(Vx + (size_t)p.col[q] * p.R);
+            e.x = fma(p.E[q], xq.x, e.x);
+            e.y = fma(p.E[q], xq.y, e.y);
+          }
+        }
+      }
+      *reinterpret_cast<dv2*>(bV + x * kPsStride + fr * 16 + 2 * jp) = v;
+      *reinterpret_cast<dv2*>(bE + x * kPsStride + fr * 16 + 2 * jp) = e;
+    }
+    __syncthreads();
+    dv2 a[KT], b[KT];
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt) {
+      a[kt] = *reinterpret_cast<const dv2*>(bV + (kt * 16 + t16) * kPsStride + r4 * 16 + 2 * w);
+      b[kt] = *reinterpret_cast<const dv2*>(bE + (kt * 16 + t16) * kPsStride + r4 * 16 + 2 * w);
+    }
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+      for (int lt = 0; lt < KT; ++lt) {
+        acc[kt][lt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kt].x, b[lt].x, acc[kt][lt][0], 0, 0, 0);
+        acc[kt][lt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kt].y, b[lt].y, acc[kt][lt][1], 0, 0, 0);
+      }
+  }
+  const int c = cb + 2 * w;
+  double d0 = 0.0, d1 = 0.0;
+  const double* M0 = p.M + (size_t)c * p.mk * p.mp;
+  const double* M1 = M0 + (size_t)p.mk * p.mp;
+#pragma unroll
+  for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = kt * 16 + 4 * r + r4;
+#pragma unroll
+      for (int lt = 0; lt < KT; ++lt) {
+        const int l = lt * 16 + t16;
+        if (k < p.m && l < p.m) {
+          d0 = fma(acc[kt][lt][0][r], M0[(size_t)k * p.mp + l], d0);
+          d1 = fma(acc[kt][lt][1][r], M1[(size_t)k * p.mp + l], d1);
+        }
+      }
+    }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    d0 += __shfl_xor(d0, off);
+    d1 += __shfl_xor(d1, off);
+  }
+  if (lane == 0) {
+    p.partials[(size_t)rg * p.R + c] = d0;
+    p.partials[(size_t)rg * p.R + c + 1] = d1;
   }
 }
 
@@ -669,15 +820,24 @@ void launch_bpp_t(const gmf::BPPT& p, int LT, hipStream_t st) {
   }
 }
 
+// The P-P block without T (k_bpp_s) while all S tiles fit one wavefront's
+// accumulators (m <= 48: 3 x 3 tiles of 16 for each of two columns) and a
+// block's 16 columns fit R (measured on 512^2 at 32 columns, m = 40: 2.79 ms
+// against 1.99 + 1.38 ms for T and the dot; and no N R m buffer for T).
+bool fpp_fused(const gm_bfab* f) {
+  return f->m <= 48 && f->R >= 16;
+}
+
 gm_status fppm(gm_bfab* f) {
   const size_t N = (size_t)f->n;
   if (!f->Mpp) F_TRY(falloc(&f->Mpp, (size_t)f->R * f->mk * f->mp));
-  if (!f->Tpp) F_TRY(falloc(&f->Tpp, N * f->R * f->m));
   if (!f->pdot) F_TRY(falloc(&f->pdot, ((size_t)f->G_pd + 1) * f->R));
   gmf::BPPM a{f->sg.ur, f->sg.ui, f->sb.yr, f->sb.yi, f->R, f->m, f->ns, f->mk, f->mp, f->Mpp};
   hipLaunchKernelGGL(gmf::k_bpp_m<0>, dim3(std::min(4096, bdiv((long long)f->R * f->mk * f->mp, 256))),
                      dim3(256), 0, f->stream, a);
   F_HIP(hipGetLastError());
+  if (fpp_fused(f)) return GM_OK;
+  if (!f->Tpp) F_TRY(falloc(&f->Tpp, N * f->R * f->m));
   gmf::BPPT t{f->sg.V, f->Mpp, f->n, f->R, f->m, f->mk, f->mp, f->pp_items, f->Tpp};
   if (f->CW == 2)
     launch_bpp_t<2>(t, f->LT, f->stream);
@@ -688,13 +848,24 @@ gm_status fppm(gm_bfab* f) {
 }
 
 // The P-P sums of one parameter (E on A's pattern) for all R columns:
-// sum_l (E Vb[l]) . T[l] per column by k_bpp_dot, block partials [G_pd][R]
-// summed into pdot[G_pd R .. + R).  (m launches of the batched SpMM, one per
-// l, measured 3.1 against 2.25 ms on 512^2 at 32 columns, m = 40.)
+// sum_{k,l} (Vg^T E Vb)[k][l] M_c[k][l] by k_bpp_s, or sum_l (E Vb[l]) . T[l]
+// per column by k_bpp_dot; partials [G_pd][R] summed into pdot[G_pd R .. + R).
+// (m launches of the batched SpMM, one per l, measured 3.1 against 2.25 ms
+// for the one-pass dot on 512^2 at 32 columns, m = 40.)
 template <int R>
 gm_status fpp(gm_bfab* f, const double* E) {
   gmf::BPD d{f->rowptr, f->col, E, f->sb.V, f->Tpp, f->n, f->m, f->pdot};
-  if constexpr (R >= 4)
+  if (fpp_fused(f)) {
+    // G_pd row groups, R / 16 column groups of 16, 8 wavefronts a block
+    gmf::BPS b{f->rowptr, f->col, E, f->sg.V, f->sb.V, f->Mpp, f->n, R, f->m, f->mk, f->mp, f->G_pd, f->pdot};
+    const dim3 grid(f->G_pd * (R / 16));
+    if (f->m <= 16)
+      hipLaunchKernelGGL((gmf::k_bpp_s<1>), grid, dim3(512), 0, f->stream, b);
+    else if (f->m <= 32)
+      hipLaunchKernelGGL((gmf::k_bpp_s<2>), grid, dim3(512), 0, f->stream, b);
+    else
+      hipLaunchKernelGGL((gmf::k_bpp_s<3>), grid, dim3(512), 0, f->stream, b);
+  } else if constexpr (R >= 4)
     hipLaunchKernelGGL((gmf::k_bpp_dot2<R>), dim3(f->G_pd), dim3(256), 0, f->stream, d);
   else
     hipLaunchKernelGGL((gmf::k_bpp_dot<R>), dim3(f->G_pd), dim3(256), 0, f->stream, d);
@@ -896,7 +1067,7 @@ gm_status gm_bfab_create(const gm_bfab_desc* d, gm_bfab** out) {
   f->mk = bdiv(f->m, 4) * 4;
   f->mp = bdiv(f->m, 16 * f->LT) * 16 * f->LT;
   f->pp_items = bdiv(f->n, 16) * (R / f->CW);
-  f->G_pd = std::max(1, std::min(2048, bdiv(f->n, gm::kWaves * (R < 64 ? 64 / R : 1))));
+  f->G_pd = bdiv(std::max(1, std::min(2048, bdiv(f->n, gm::kWaves * (R < 64 ? 64 / R : 1)))), 8) * 8;
   hipStream_t st = f->stream;
   auto up = [&](auto** p, const auto* src, size_t cnt) -> gm_status {
     F_TRY(falloc(p, cnt));

@@ -258,14 +258,18 @@ def test_deflated_param_gradient_vs_exact_reference():
     assert abs(gr - ref) <= 1e-8 * abs(ref)
 
 
-@pytest.mark.parametrize("m,K,R", [(30, 0, 4), (70, 0, 4), (70, 4, 4), (40, 6, 32), (40, 0, 64)])
+@pytest.mark.parametrize("m,K,R", [(30, 0, 4), (70, 0, 4), (70, 4, 4), (40, 6, 32), (40, 0, 64),
+                                   (16, 0, 16), (30, 4, 16), (70, 0, 16)])
 def test_batch_param_gradient_matches_single_column(m, K, R):
-    """gm_bfab_backward_param (the P-P block contracted in the Krylov bases:
-    T = V_g M on MFMA, l in tiles of 16 up to 64 a pass, so m = 70 takes two
-    passes) against gm_fab_backward_param column by column (resolvent blocks,
-    pinned against exact resolvents above), with and without deflation.  Both
-    run the same Krylov approximation; they differ in reduction partitions, so
-    they agree to rounding amplified by m Lanczos steps (1e-9)."""
+    """gm_bfab_backward_param (the P-P block contracted in the Krylov bases)
+    against gm_fab_backward_param column by column (resolvent blocks, pinned
+    against exact resolvents above), with and without deflation.  The P-P
+    block takes two forms: S = V_g^T E V_b on MFMA contracted with M per
+    column (16 columns up, m <= 48: 1, 2 or 3 tiles of 16 a side), else
+    T = V_g M on MFMA (l in tiles of 16 up to 64 a pass, so m = 70 takes two
+    passes) and a one-pass dot.  Both engines run the same Krylov
+    approximation; they differ in reduction partitions, so they agree to
+    rounding amplified by m Lanczos steps (1e-9)."""
     from gpumatfun_amd.adjoint import BatchFabEngine
     n, r = 24, 4.0
     A = problems.anisotropic_9pt(n, ratio=r)
@@ -300,14 +304,16 @@ def test_batch_param_gradient_matches_single_column(m, K, R):
         assert abs(gt[0, col] - gr) <= 1e-9 * abs(gr)
 
 
-def test_batch_param_gradient_long_rows():
+@pytest.mark.parametrize("R", [4, 16])
+def test_batch_param_gradient_long_rows(R):
     """A 24^2 Laplacian with a hub row (node 0 coupled to 30 others, 31
-    nonzeros): the one-pass P-P dot takes its long-row path (rows over 12
-    nonzeros) there.  dL/dtheta for a random symmetric E on A's pattern, the
+    nonzeros): the P-P block takes its long-row path there (the one-pass dot
+    at 4 columns, rows over 12 nonzeros; the S = V_g^T E V_b form at 16, rows
+    over 9).  dL/dtheta for a random symmetric E on A's pattern, the
     multi-column engine against the single-column one, column by column."""
     import scipy.sparse as sp
     from gpumatfun_amd.adjoint import BatchFabEngine
-    n, R, m = 24, 4, 40
+    n, m = 24, 40
     A = problems.laplacian_2d_dirichlet(n).tolil()
     for j in range(1, 31):
         A[0, j] = A[j, 0] = -50.0
@@ -327,7 +333,7 @@ def test_batch_param_gradient_long_rows():
     cw, _ = contour_weights(c["xi"], c["dxidt"], c["s"], TAU, ALPHA)
     B = rng.standard_normal((n * n, R))
     G = rng.standard_normal((n * n, R))
-    print("case batch param gradient long rows", flush=True)
+    print(f"case batch param gradient long rows R={R}", flush=True)
     be = BatchFabEngine(A, R=R, m=m, xi=c["xi"], wts1=cw, wts2=cw)
     be.set_param_derivs([E])
     be.forward(B)
