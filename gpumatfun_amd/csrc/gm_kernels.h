@@ -277,23 +277,29 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(SpmmArgs p) {
       a1 = p.rowptr[r + 1] - j0;
       if (PRE) spmm_row_load<MODE>(p, r, o0, o1);
     }
+    // Slots past the block's last nonzero load that nonzero again (the same
+    // line for every such lane) and are dropped below, so the PER index loads
+    // and then the PER gathers each go out together, without a branch per
+    // slot (with one, each index load waited for its predecessor).  A block
+    // with no nonzeros loads nothing.
     double a[PER];
     int ci[PER];
+    double2 xv[PER];
+    if (j1 > j0) {
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int j = j0 + t + u * kBlock;
-      a[u] = 0.0;
-      ci[u] = 0;
-      if (j < j1) {
+      for (int u = 0; u < PER; ++u) {
+        const int j = min(j0 + t + u * kBlock, j1 - 1);
         a[u] = p.val[j];
         ci[u] = C16 ? r0 + (int)p.col16[j] : p.col[j];
       }
-    }
-    double2 xv[PER];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int j = j0 + t + u * kBlock;
-      xv[u] = j < j1 ? p.x[ci[u]] : d2(0.0, 0.0);
+      for (int u = 0; u < PER; ++u) xv[u] = p.x[ci[u]];
+    } else {
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        a[u] = 0.0;
+        xv[u] = d2(0.0, 0.0);
+      }
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
