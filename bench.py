@@ -218,7 +218,7 @@ def build_batch3d(args):
         if args.node_order == "pencil":
             order = problems.pencil_order(side, 16)
         else:
-            order = problems.locality_order(A, 4096)
+            order = problems.locality_order(A, 512)
         A = sp.csr_matrix(A[order][:, order])
         A.sort_indices()
         A.indptr = A.indptr.astype(np.int32)
@@ -972,7 +972,7 @@ def main():
                          "(gm_rcm_order, pseudo-peripheral start as symrcm) or SciPy's")
     ap.add_argument("--node-order", default="natural", choices=["natural", "pencil", "locality"],
                     help="batch3d: grid nodes in natural (i + n j + n^2 k) or 16 x 16-pencil order, "
-                         "or the library's locality order (gm_locality_order, leaves of 4096)")
+                         "or the library's locality order (gm_locality_order, leaves of 512)")
     ap.add_argument("--locality-order", default="on", choices=["on", "off"],
                     help="batch3d: the batch engine renumbers the nodes internally "
                          "(GM_FLAG_LOCALITY_ORDER); inputs and results in the caller's order")

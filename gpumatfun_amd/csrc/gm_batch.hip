@@ -41,8 +41,9 @@ namespace {
 constexpr int kBatchGrid = 1024;   // most blocks of the column kernels (set_bschedule)
 // leaf of the internal locality order: on the 128^3 Laplacian, leaves of 1024
 // to 65536 nodes leave the same fraction of gathers (~16%) with no reuse within
-// 2048 rows, against 43% in natural order
-constexpr int kLocalityLeaf = 4096;
+// 2048 rows, against 43% in natural order; timed with 256 columns, leaves of
+// 512, 1024, 4096 and 16384 nodes gave 730, 729, 725 and 721 solves/s
+constexpr int kLocalityLeaf = 512;
 
 
 thread_local std::string g_berr;

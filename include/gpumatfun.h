@@ -73,7 +73,7 @@ enum {
                                     whole rows.  AUTOTUNE times both forms  */
     GM_FLAG_LOCALITY_ORDER = 1u << 9 /* gm_batch: renumber the nodes inside the
                                     engine by gm_locality_order (leaves of
-                                    4096 nodes) so that rows gathered together
+                                    512 nodes) so that rows gathered together
                                     lie close; inputs and results stay in the
                                     caller's order                          */
 };
