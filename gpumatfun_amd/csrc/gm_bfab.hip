@@ -440,7 +440,7 @@ __global__ __launch_bounds__(512) void k_bpp_s(BPS p) {
 }
 
 // Block partials of M[j][k][c] = sum_i V[j][i][c] EQ[i][k] for every Krylov
-// vector j (block y) on FP64 MFMA: each wavefront holds all KT x CT 16 x 16
+// vector j (block x; the row chunk is block y) on FP64 MFMA: each wavefront holds all KT x CT 16 x 16
 // tiles of (k, c), so one EQ load feeds CT chains and one V load KT chains;
 // the 4 wavefronts take interleaved 4-row steps of the block's rows and are
 // summed in LDS in wavefront order.  partials[G][m][K][R].
@@ -454,8 +454,10 @@ struct BVtQ {
 template <int KT, int CT>
 __global__ __launch_bounds__(256) void k_bvtq(BVtQ p) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = blockIdx.y;
-  const int row0 = blockIdx.x * p.chunk, row1 = min(row0 + p.chunk, p.nrows);
+  // blockIdx.x = Krylov vector j, blockIdx.y = row chunk: the m blocks of one
+  // chunk are consecutive, so they run together and share its EQ rows in L2
+  const int j = blockIdx.x, cb = blockIdx.y;
+  const int row0 = cb * p.chunk, row1 = min(row0 + p.chunk, p.nrows);
   const int sub = lane >> 4;
   const double* Vj = p.V + (size_t)j * p.nrows * p.R;
   __shared__ double sh[KT * CT * 4 * 64];
@@ -464,25 +466,37 @@ __global__ __launch_bounds__(256) void k_bvtq(BVtQ p) {
   for (int kt = 0; kt < KT; ++kt)
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) acc[kt][ct] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int r = row0 + 4 * wave; r < row1; r += 16) {
-    const int row = r + sub;
-    const bool ok = row < row1;
-    double a[KT], b[CT];
+  // four 4-row steps a pass, all their loads issued together: out-of-range
+  // operands load a valid element (row0, column R - 1, k K - 1) and are
+  // replaced by zero afterwards, so no load sits behind a branch; the MFMA
+  // chains take the steps in the same order as one step a pass
+  constexpr int UR = 4;
+  for (int r = row0 + 4 * wave; r < row1; r += 16 * UR) {
+    double a[UR][KT], b[UR][CT];
 #pragma unroll
-    for (int kt = 0; kt < KT; ++kt) {
-      const int kk = kt * 16 + (lane & 15);
-      a[kt] = (ok && kk < p.K) ? p.EQ[(size_t)row * p.K + kk] : 0.0;
-    }
+    for (int u = 0; u < UR; ++u) {
+      const int row = r + 16 * u + sub;
+      const size_t rc = row < row1 ? row : row0;
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-      const int c = ct * 16 + (lane & 15);
-      b[ct] = (ok && c < p.R) ? Vj[(size_t)row * p.R + c] : 0.0;
-    }
-#pragma unroll
-    for (int kt = 0; kt < KT; ++kt)
+      for (int kt = 0; kt < KT; ++kt)
+        a[u][kt] = p.EQ[rc * p.K + min(kt * 16 + (lane & 15), p.K - 1)];
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct)
-        acc[kt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kt], b[ct], acc[kt][ct], 0, 0, 0);
+        b[u][ct] = Vj[rc * p.R + min(ct * 16 + (lane & 15), p.R - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < UR; ++u) {
+      const bool ok = r + 16 * u + sub < row1;
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt) a[u][kt] = (ok && kt * 16 + (lane & 15) < p.K) ? a[u][kt] : 0.0;
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) b[u][ct] = (ok && ct * 16 + (lane & 15) < p.R) ? b[u][ct] : 0.0;
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+          acc[kt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][kt], b[u][ct], acc[kt][ct], 0, 0, 0);
+    }
   }
   for (int w = 0; w < 4; ++w) {
     if (wave == w) {
@@ -503,7 +517,7 @@ __global__ __launch_bounds__(256) void k_bvtq(BVtQ p) {
     const int ln = t & 63, i = (t >> 6) & 3, tile = t >> 8;
     const int k2 = (tile / CT) * 16 + 4 * i + (ln >> 4), c = (tile % CT) * 16 + (ln & 15);
     if (k2 < p.K && c < p.R)
-      p.partials[(((size_t)blockIdx.x * p.m + j) * p.K + k2) * p.R + c] = sh[t];
+      p.partials[(((size_t)cb * p.m + j) * p.K + k2) * p.R + c] = sh[t];
   }
 }
 
@@ -974,7 +988,7 @@ void fshift_solve(int m, int R, int c, const double* al, const double* be, doubl
 
 template <int KT>
 void launch_bvtq_kt(const gmf::BVtQ& v, int G, hipStream_t st) {
-  const dim3 grid(G, v.m);
+  const dim3 grid(v.m, G);
   switch ((v.R + 15) / 16) {
     case 1: hipLaunchKernelGGL((gmf::k_bvtq<KT, 1>), grid, dim3(256), 0, st, v); break;
     case 2: hipLaunchKernelGGL((gmf::k_bvtq<KT, 2>), grid, dim3(256), 0, st, v); break;
