@@ -1140,25 +1140,37 @@ __global__ __launch_bounds__(kBlock) void k_bdefl_qtu(BDeflQtU p) {
   for (int kt = 0; kt < KT; ++kt)
 #pragma unroll
     for (int ct = 0; ct < CTW; ++ct) acc[kt][ct] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int r = row0 + 4 * wave; r < row1; r += 4 * kWaves) {
-    const int row = r + sub;
-    const bool ok = row < row1;
-    double a[KT], b[CTW];
+  // four 4-row steps a pass, all their loads issued together: out-of-range
+  // operands load a valid element (row0, column R - 1, k K - 1) and are
+  // replaced by zero afterwards, so no load sits behind a branch; the MFMA
+  // chains take the steps in the same order as one step a pass
+  constexpr int UR = 4;
+  for (int r = row0 + 4 * wave; r < row1; r += 4 * kWaves * UR) {
+    double a[UR][KT], b[UR][CTW];
 #pragma unroll
-    for (int kt = 0; kt < KT; ++kt) {
-      const int kk = kt * 16 + (lane & 15);
-      a[kt] = (ok && kk < p.K) ? p.Q[(size_t)row * p.K + kk] : 0.0;
-    }
+    for (int u = 0; u < UR; ++u) {
+      const int row = r + 4 * kWaves * u + sub;
+      const size_t rc = row < row1 ? row : row0;
 #pragma unroll
-    for (int ct = 0; ct < CTW; ++ct) {
-      const int c = (ct0 + ct) * 16 + (lane & 15);
-      b[ct] = (ok && c < p.R) ? p.U[(size_t)row * p.R + c] : 0.0;
-    }
-#pragma unroll
-    for (int kt = 0; kt < KT; ++kt)
+      for (int kt = 0; kt < KT; ++kt) a[u][kt] = p.Q[rc * p.K + min(kt * 16 + (lane & 15), p.K - 1)];
 #pragma unroll
       for (int ct = 0; ct < CTW; ++ct)
-        acc[kt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kt], b[ct], acc[kt][ct], 0, 0, 0);
+        b[u][ct] = p.U[rc * p.R + min((ct0 + ct) * 16 + (lane & 15), p.R - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < UR; ++u) {
+      const bool ok = r + 4 * kWaves * u + sub < row1;
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt) a[u][kt] = (ok && kt * 16 + (lane & 15) < p.K) ? a[u][kt] : 0.0;
+#pragma unroll
+      for (int ct = 0; ct < CTW; ++ct)
+        b[u][ct] = (ok && (ct0 + ct) * 16 + (lane & 15) < p.R) ? b[u][ct] : 0.0;
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+        for (int ct = 0; ct < CTW; ++ct)
+          acc[kt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][kt], b[u][ct], acc[kt][ct], 0, 0, 0);
+    }
   }
   for (int w = 0; w < kWaves; ++w) {
     if (wave == w) {
