@@ -565,6 +565,13 @@ struct FSide {
   double *U = nullptr, *V = nullptr, *W0 = nullptr, *W1 = nullptr, *al = nullptr, *be = nullptr,
          *yr = nullptr, *yi = nullptr, *ur = nullptr, *ui = nullptr, *C = nullptr,
          *out = nullptr, *Zd = nullptr, *upd = nullptr;
+  // the stream this side's deflation and Krylov run are enqueued on, and the
+  // block partials and solve scratch they use (the g side's are the engine's
+  // own; the b side has its stream and buffers, so that the b side's
+  // recomputed forward run can overlap the g side's run of a backward)
+  hipStream_t st = nullptr;
+  double* partials = nullptr;
+  double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
 };
 
 }  // namespace
@@ -572,6 +579,13 @@ struct FSide {
 struct gm_bfab {
   int dev = 0;
   hipStream_t stream = nullptr;
+  // the b side's stream; ev2 marks the end of its last enqueued run, and
+  // pend2 that a forward pass without output has not been joined yet (the
+  // next entry point makes the engine stream wait for it); ev1 orders the b
+  // side after the engine stream's earlier work
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev1 = nullptr, ev2 = nullptr;
+  bool pend2 = false;
   int n = 0, R = 0, m = 0, ns = 0, K = 0;
   long long nnz = 0;
   bool have_forward = false;
@@ -619,12 +633,36 @@ void free_bfab(gm_bfab* f) {
   for (void* p : bufs) ffree(p);
   for (double* p : f->pdA) ffree(p);
   for (double* p : f->pEQ) ffree(p);
+  {
+    // the b side's own partials and solve scratch (the g side's are the engine's)
+    void* bbufs[] = {f->sb.partials, f->sb.wD, f->sb.wDU, f->sb.wDU2, f->sb.wX};
+    for (void* p : bbufs) ffree(p);
+  }
   for (FSide* s : {&f->sb, &f->sg}) {
     void* sbufs[] = {s->U, s->V, s->W0, s->W1, s->al, s->be, s->yr, s->yi, s->ur, s->ui, s->C,
                      s->out, s->Zd, s->upd};
     for (void* p : sbufs) ffree(p);
   }
+  if (f->ev1) (void)hipEventDestroy(f->ev1);
+  if (f->ev2) (void)hipEventDestroy(f->ev2);
+  if (f->stream2) (void)hipStreamDestroy(f->stream2);
   if (f->stream) (void)hipStreamDestroy(f->stream);
+}
+
+gm_status freduce_on(gm_bfab* f, double* partials, hipStream_t st, int G, int nv, double* out,
+                     int do_sqrt, int flag_zero) {
+  gm::ReduceArgs r{};
+  r.partials = partials;
+  r.G = G;
+  r.nv = nv;
+  r.out = out;
+  r.stride = 1;
+  r.do_sqrt = do_sqrt;
+  r.flag_zero = flag_zero;
+  r.status = f->status;
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, st, r);
+  F_HIP(hipGetLastError());
+  return GM_OK;
 }
 
 gm_status freduce(gm_bfab* f, int G, int nv, double* out, int do_sqrt, int flag_zero) {
@@ -653,10 +691,10 @@ gm_status fdeflate(gm_bfab* f, FSide& s) {
   q.chunk = f->chunk_df;
   q.KT = bdiv(f->K, 16);
   q.CT = bdiv(f->R, 16);
-  q.partials = f->partials;
-  gmb::launch_bdefl_qtu(q, f->G_df, f->stream);
+  q.partials = s.partials;
+  gmb::launch_bdefl_qtu(q, f->G_df, s.st);
   F_HIP(hipGetLastError());
-  F_TRY(freduce(f, f->G_df, f->K * f->R, s.Zd, 0, 0));
+  F_TRY(freduce_on(f, s.partials, s.st, f->G_df, f->K * f->R, s.Zd, 0, 0));
   gmb::BDeflApply a{};
   a.Q = f->Q;
   a.Z = s.Zd;
@@ -669,7 +707,7 @@ gm_status fdeflate(gm_bfab* f, FSide& s) {
   a.R = f->R;
   a.nrows = f->n;
   hipLaunchKernelGGL(gmb::k_bdefl_apply<0>, dim3(std::min(4096, bdiv(f->n, gm::kWaves)), bdiv(f->R, 64)),
-                     dim3(gm::kBlock), 2 * (size_t)f->K * 64 * sizeof(double), f->stream, a);
+                     dim3(gm::kBlock), 2 * (size_t)f->K * 64 * sizeof(double), s.st, a);
   F_HIP(hipGetLastError());
   return GM_OK;
 }
@@ -685,11 +723,11 @@ gm_status fkrylov(gm_bfab* f, FSide& s, bool want_out, bool unit) {
   nn.ldu = R;
   nn.nrows = f->n;
   nn.chunk = f->chunk_sp;
-  nn.partials = f->partials;
+  nn.partials = s.partials;
   nn.status = f->status;
-  hipLaunchKernelGGL((gmb::k_bnorm<R>), dim3(f->G_sp), dim3(gm::kBlock), 0, f->stream, nn);
+  hipLaunchKernelGGL((gmb::k_bnorm<R>), dim3(f->G_sp), dim3(gm::kBlock), 0, s.st, nn);
   F_HIP(hipGetLastError());
-  F_TRY(freduce(f, f->G_sp, R, s.be, 1, 0));
+  F_TRY(freduce_on(f, s.partials, s.st, f->G_sp, R, s.be, 1, 0));
   gmb::BSpmm base{};
   base.rowptr = f->rowptr;
   base.col = f->col;
@@ -700,7 +738,7 @@ gm_status fkrylov(gm_bfab* f, FSide& s, bool want_out, bool unit) {
   base.nrowblk = f->nrowblk;
   base.al = s.al;
   base.be = s.be;
-  base.partials = f->partials;
+  base.partials = s.partials;
   base.g0 = 1.0;
   base.status = f->status;
   double* Wb[2] = {s.W0, s.W1};
@@ -720,14 +758,14 @@ gm_status fkrylov(gm_bfab* f, FSide& s, bool want_out, bool unit) {
       a.rowblk = f->rowblk_xs;
       a.nrowblk = f->nrowblk_xs;
       hipLaunchKernelGGL((gmb::k_bspmm_xs<R, gmb::kOnly, false>), dim3(f->G_xs), dim3(gm::kBlock),
-                         0, f->stream, a);
+                         0, s.st, a);
       gpart = f->G_xs / (R / 32);
     } else {
       hipLaunchKernelGGL((gmb::k_bspmm<R, gmb::kOnly, true>), dim3(f->G_sp), dim3(gm::kBlock), 0,
-                         f->stream, a);
+                         s.st, a);
     }
     F_HIP(hipGetLastError());
-    F_TRY(freduce(f, gpart, R, s.al + (size_t)k * R, 0, 0));
+    F_TRY(freduce_on(f, s.partials, s.st, gpart, R, s.al + (size_t)k * R, 0, 0));
     gmb::BFinish fn{};
     fn.w = Wb[k & 1];
     fn.vk = a.vk;
@@ -736,15 +774,15 @@ gm_status fkrylov(gm_bfab* f, FSide& s, bool want_out, bool unit) {
     fn.k = k;
     fn.nrows = f->n;
     fn.chunk = f->chunk_sp;
-    fn.partials = f->partials;
+    fn.partials = s.partials;
     fn.status = f->status;
     if constexpr (R >= 4 && R <= 64) {   // measured: 34.3 against 35.9 us at 32 columns; 2.43 against 2.40 ms at 256
-      hipLaunchKernelGGL((gmb::k_bfinish2<R>), dim3(f->G_sp), dim3(gm::kBlock), 0, f->stream, fn);
+      hipLaunchKernelGGL((gmb::k_bfinish2<R>), dim3(f->G_sp), dim3(gm::kBlock), 0, s.st, fn);
     } else {
-      hipLaunchKernelGGL((gmb::k_bfinish<R>), dim3(f->G_sp), dim3(gm::kBlock), 0, f->stream, fn);
+      hipLaunchKernelGGL((gmb::k_bfinish<R>), dim3(f->G_sp), dim3(gm::kBlock), 0, s.st, fn);
     }
     F_HIP(hipGetLastError());
-    F_TRY(freduce(f, f->G_sp, R, s.be + (size_t)(k + 1) * R, 1, 1));
+    F_TRY(freduce_on(f, s.partials, s.st, f->G_sp, R, s.be + (size_t)(k + 1) * R, 1, 1));
   }
   for (int pass = 0; pass < (unit ? 2 : 1); ++pass) {
     gmb::BSolve sv{};
@@ -756,15 +794,15 @@ gm_status fkrylov(gm_bfab* f, FSide& s, bool want_out, bool unit) {
     sv.wts1 = pass ? f->ones : f->wts1;
     sv.wts2 = pass ? f->ones : f->wts2;
     sv.etaxi = f->xi;
-    sv.wD = f->wD;
-    sv.wDU = f->wDU;
-    sv.wDU2 = f->wDU2;
-    sv.wX = f->wX;
+    sv.wD = s.wD;
+    sv.wDU = s.wDU;
+    sv.wDU2 = s.wDU2;
+    sv.wX = s.wX;
     sv.yr = pass ? s.ur : s.yr;
     sv.yi = pass ? s.ui : s.yi;
     sv.status = f->status;
     hipLaunchKernelGGL(gmb::k_bsolves<0>, dim3(bdiv((long long)R * f->ns, 64)), dim3(64), 0,
-                       f->stream, sv);
+                       s.st, sv);
     F_HIP(hipGetLastError());
   }
   gmb::BCollapse cc{};
@@ -778,7 +816,7 @@ gm_status fkrylov(gm_bfab* f, FSide& s, bool want_out, bool unit) {
   cc.nd = 1;
   cc.C = s.C;
   hipLaunchKernelGGL(gmb::k_bcollapse<0>, dim3(std::max(1, std::min(1024, bdiv((long long)f->m * R, gm::kBlock)))),
-                     dim3(gm::kBlock), 0, f->stream, cc);
+                     dim3(gm::kBlock), 0, s.st, cc);
   F_HIP(hipGetLastError());
   if (want_out) {
     gmb::BVC v{};
@@ -794,15 +832,15 @@ gm_status fkrylov(gm_bfab* f, FSide& s, bool want_out, bool unit) {
     if (R % 4 == 0 && reinterpret_cast<uintptr_t>(s.out) % 16 == 0) {
       // four adjacent columns a thread, 16-byte loads (7% faster at 256 columns)
       hipLaunchKernelGGL((gmb::k_bvc<1, 4>), dim3(bdiv((long long)f->n * R / 4, gm::kBlock)),
-                         dim3(gm::kBlock), 0, f->stream, v);
+                         dim3(gm::kBlock), 0, s.st, v);
     } else {
       hipLaunchKernelGGL((gmb::k_bvc<1>), dim3(bdiv((long long)f->n * R, gm::kBlock)), dim3(gm::kBlock), 0,
-                         f->stream, v);
+                         s.st, v);
     }
     F_HIP(hipGetLastError());
     if (f->K) {
       hipLaunchKernelGGL(gmf::k_add<0>, dim3(std::min(8192, bdiv((long long)N * R, 256))), dim3(256), 0,
-                         f->stream, s.upd, s.out, N * R);
+                         s.st, s.upd, s.out, N * R);
       F_HIP(hipGetLastError());
     }
   }
@@ -1005,6 +1043,16 @@ void launch_bvtq(const gmf::BVtQ& v, int G, hipStream_t st) {
   }
 }
 
+// The engine stream waits for a b-side run that a forward pass without
+// output left in flight.
+gm_status fjoin(gm_bfab* f) {
+  if (f->pend2) {
+    F_HIP(hipStreamWaitEvent(f->stream, f->ev2, 0));
+    f->pend2 = false;
+  }
+  return GM_OK;
+}
+
 gm_status fcheck_status(gm_bfab* f, const char* what) {
   unsigned stv = 0;
   F_HIP(hipMemcpyAsync(&stv, f->status, sizeof(unsigned), hipMemcpyDeviceToHost, f->stream));
@@ -1160,6 +1208,24 @@ gm_status gm_bfab_create(const gm_bfab_desc* d, gm_bfab** out) {
     F_CREATE(falloc(&s->Zd, (size_t)std::max(f->K, 1) * R));
     F_CREATE(falloc(&s->upd, f->K ? N * R : 1));
   }
+  // the b side runs on its own stream with its own partials and solve
+  // scratch; the g side uses the engine's
+  if (hipStreamCreateWithFlags(&f->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&f->ev1, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&f->ev2, hipEventDisableTiming) != hipSuccess)
+    return bail(ffail(GM_EDEVICE, "b-side stream/events"));
+  F_CREATE(falloc(&f->sb.partials, np));
+  F_CREATE(falloc(&f->sb.wD, ny));
+  F_CREATE(falloc(&f->sb.wDU, ny));
+  F_CREATE(falloc(&f->sb.wDU2, ny));
+  F_CREATE(falloc(&f->sb.wX, ny));
+  f->sb.st = f->stream2;
+  f->sg.st = f->stream;
+  f->sg.partials = f->partials;
+  f->sg.wD = f->wD;
+  f->sg.wDU = f->wDU;
+  f->sg.wDU2 = f->wDU2;
+  f->sg.wX = f->wX;
   if (hipStreamSynchronize(st) != hipSuccess) return bail(ffail(GM_EDEVICE, "sync"));
   {
     std::lock_guard<std::mutex> lk(g_fmu);
@@ -1177,6 +1243,7 @@ gm_status gm_bfab_delete(gm_bfab* f) {
     g_freg.erase(f);
   }
   (void)hipSetDevice(f->dev);
+  if (f->stream2) (void)hipStreamSynchronize(f->stream2);
   if (f->stream) (void)hipStreamSynchronize(f->stream);
   free_bfab(f);
   delete f;
@@ -1188,13 +1255,28 @@ gm_status gm_bfab_forward(gm_bfab* f, const double* B, double* U) {
   if (!B) return ffail(GM_EINVAL, "null input");
   F_HIP(hipSetDevice(f->dev));
   const size_t NR = (size_t)f->n * f->R;
-  F_HIP(hipMemcpyAsync(f->sb.U, B, NR * sizeof(double), hipMemcpyDefault, f->stream));
+  // the b side (its own stream) after the engine stream's earlier work
+  F_HIP(hipEventRecord(f->ev1, f->stream));
+  F_HIP(hipStreamWaitEvent(f->stream2, f->ev1, 0));
+  F_HIP(hipMemcpyAsync(f->sb.U, B, NR * sizeof(double), hipMemcpyDefault, f->stream2));
+  if (!U) {
+    // B is the caller's until this returns: wait for its copy only, and leave
+    // the run in flight; the next entry point joins it, so a backward's g-side
+    // run overlaps it.  Its breakdown flags are checked with that call's.
+    F_HIP(hipStreamSynchronize(f->stream2));
+  }
   if (f->K) F_TRY(fdeflate(f, f->sb));
   F_TRY(fkrylov_dispatch(f, f->sb, U != nullptr, false));
-  if (U) F_HIP(hipMemcpyAsync(U, f->sb.out, NR * sizeof(double), hipMemcpyDefault, f->stream));
-  F_TRY(fcheck_status(f, "forward pass"));
+  if (U) F_HIP(hipMemcpyAsync(U, f->sb.out, NR * sizeof(double), hipMemcpyDefault, f->stream2));
+  F_HIP(hipEventRecord(f->ev2, f->stream2));
   f->have_forward = true;
-  return GM_OK;
+  if (!U) {
+    f->pend2 = true;
+    return GM_OK;
+  }
+  F_HIP(hipStreamWaitEvent(f->stream, f->ev2, 0));
+  f->pend2 = false;   // the engine stream now waits for all b-side work
+  return fcheck_status(f, "forward pass");
 }
 
 gm_status gm_bfab_set_param_derivs(gm_bfab* f, int32_t nparams, const double* const* dA_values) {
@@ -1205,6 +1287,7 @@ gm_status gm_bfab_set_param_derivs(gm_bfab* f, int32_t nparams, const double* co
   if (f->K && nparams > 0 && (f->dd1.empty() || f->dd2.empty()))
     return ffail(GM_EINVAL, "the parameter gradient with deflation needs f1_divdiff and f2_divdiff");
   F_HIP(hipSetDevice(f->dev));
+  F_TRY(fjoin(f));
   F_HIP(hipStreamSynchronize(f->stream));
   for (double* p : f->pdA) ffree(p);
   for (double* p : f->pEQ) ffree(p);
@@ -1242,6 +1325,9 @@ gm_status gm_bfab_backward_param(gm_bfab* f, const double* G, double* grad_B, do
   F_HIP(hipMemcpyAsync(f->sg.U, G, NR * sizeof(double), hipMemcpyDefault, st));
   if (K) F_TRY(fdeflate(f, f->sg));
   F_TRY(fkrylov_dispatch(f, f->sg, grad_B != nullptr, P > 0));
+  // everything below reads the b side: wait for its run if a forward pass
+  // without output left it in flight (it overlapped the g side's run above)
+  F_TRY(fjoin(f));
   if (grad_B) F_HIP(hipMemcpyAsync(grad_B, f->sg.out, NR * sizeof(double), hipMemcpyDefault, st));
   if (P > 0) {
     F_TRY(fppm(f));

@@ -538,14 +538,19 @@ typedef struct gm_bfab_desc {
 const char* gm_bfab_last_error(void);
 gm_status gm_bfab_create(const gm_bfab_desc* desc, gm_bfab** out);
 gm_status gm_bfab_delete(gm_bfab* f);
-/* U = f(A) B column by column; U may be NULL to build only the forward state. */
+/* U = f(A) B column by column; U may be NULL to build only the forward state.
+ * With U NULL the call returns once B has been copied and leaves the run in
+ * flight on the engine's second stream; the next call on the engine waits for
+ * it (a gm_bfab_backward_param overlaps it with the adjoint's own run) and
+ * reports its breakdown flags.                                              */
 gm_status gm_bfab_forward(gm_bfab* f, const double* B, double* U);
 /* dA/dtheta_q (A's pattern, symmetric) kept on the device for the backward.  */
 gm_status gm_bfab_set_param_derivs(gm_bfab* f, int32_t nparams, const double* const* dA_values);
 /* After a forward: grad_B = f(A) G (may be NULL) and, per stored parameter q
  * and column c, grad_theta[q*R + c] = d(g_c^T f(A) b_c)/dtheta_q (host
- * memory).  The first call with stored parameters allocates one N x R x m
- * work block, kept until gm_bfab_delete.                                    */
+ * memory).  With m > 48 or fewer than 16 columns, the first call with
+ * stored parameters allocates one N x R x m work block, kept until
+ * gm_bfab_delete.                                                           */
 gm_status gm_bfab_backward_param(gm_bfab* f, const double* G, double* grad_B, double* grad_theta);
 gm_status gm_bfab_info(gm_bfab* f, int32_t* columns, int32_t* num_params);
 
