@@ -340,6 +340,12 @@ struct Side {
   double *U = nullptr, *V = nullptr, *W0 = nullptr, *W1 = nullptr, *al = nullptr, *be = nullptr,
          *yr = nullptr, *yi = nullptr, *Yg = nullptr, *Xc = nullptr, *out = nullptr,
          *C = nullptr, *a = nullptr;   // V-coefficients of f(A) rhs; Q^T rhs (deflation)
+  // the stream a Krylov run of this side is enqueued on and the block partials
+  // and solve scratch it uses: the engine's, except while the backward
+  // recomputes the b side's basis on the second stream with the b side's own
+  hipStream_t st = nullptr;
+  double *partials = nullptr, *partials2 = nullptr;
+  double2 *wD = nullptr, *wDU = nullptr, *wDU2 = nullptr, *wX = nullptr;
 };
 
 }  // namespace
@@ -347,6 +353,13 @@ struct Side {
 struct gm_fab {
   int dev = 0;
   hipStream_t stream = nullptr;
+  // the second stream of the recomputed b-side run (it overlaps the g side's
+  // run of a backward); ev1 orders it after the engine stream's earlier work,
+  // ev2 marks its end
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev1 = nullptr, ev2 = nullptr;
+  double *bpart = nullptr, *bpart2 = nullptr;   // the b side's partials while recomputing
+  double2 *bwD = nullptr, *bwDU = nullptr, *bwDU2 = nullptr, *bwX = nullptr;
   int n = 0, m = 0, ns = 0, ncol = 0;
   long long nnz = 0;
   bool recompute = false, have_forward = false;
@@ -401,7 +414,29 @@ void free_fab(gm_fab* f) {
                      s->C, s->a};
     for (void* p : sbufs) afree(p);
   }
+  void* bbufs[] = {f->bpart, f->bpart2, f->bwD, f->bwDU, f->bwDU2, f->bwX};
+  for (void* p : bbufs) afree(p);
+  if (f->ev1) (void)hipEventDestroy(f->ev1);
+  if (f->ev2) (void)hipEventDestroy(f->ev2);
+  if (f->stream2) (void)hipStreamDestroy(f->stream2);
   if (f->stream) (void)hipStreamDestroy(f->stream);
+}
+
+// k_reduce of a side's partials (or `part`) on the side's stream
+gm_status areduce_s(gm_fab* f, Side& s, int G, int nv, double* out, int do_sqrt, int flag_zero,
+                    const double* part = nullptr) {
+  gm::ReduceArgs r{};
+  r.partials = part ? part : s.partials;
+  r.G = G;
+  r.nv = nv;
+  r.out = out;
+  r.stride = 1;
+  r.do_sqrt = do_sqrt;
+  r.flag_zero = flag_zero;
+  r.status = f->status;
+  hipLaunchKernelGGL(gm::k_reduce<0>, dim3(gm::reduce_grid(r.nv)), dim3(gm::kReduceBlock), 0, s.st, r);
+  A_HIP(hipGetLastError());
+  return GM_OK;
 }
 
 gm_status areduce(gm_fab* f, int G, int nv, double* out, int do_sqrt, int flag_zero,
@@ -430,11 +465,11 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
   nn.be = s.be;
   nn.nrows = f->n;
   nn.chunk = f->chunk;
-  nn.partials = f->partials;
+  nn.partials = s.partials;
   nn.status = f->status;
-  hipLaunchKernelGGL((gmb::k_bnorm<1>), dim3(f->G), dim3(gm::kBlock), 0, f->stream, nn);
+  hipLaunchKernelGGL((gmb::k_bnorm<1>), dim3(f->G), dim3(gm::kBlock), 0, s.st, nn);
   A_HIP(hipGetLastError());
-  A_TRY(areduce(f, f->G, 1, s.be, 1, 0));
+  A_TRY(areduce_s(f, s, f->G, 1, s.be, 1, 0));
   gmb::BSpmm base{};
   base.rowptr = f->rowptr;
   base.col = f->col;
@@ -443,7 +478,7 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
   base.chunk = f->chunk;
   base.al = s.al;
   base.be = s.be;
-  base.partials = f->partials;
+  base.partials = s.partials;
   base.g0 = 1.0;
   double* Wb[2] = {s.W0, s.W1};
   // small grids: alpha_k and beta_{k+1} are summed inside the consuming
@@ -460,14 +495,14 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
     a.vkm1 = k > 0 ? s.V + (size_t)(k - 1) * N : nullptr;
     a.status = f->status;
     if (fuse && k > 0) {
-      a.bpart = f->partials2;
+      a.bpart = s.partials2;
       a.bG = f->G;
     }
     // one column: rows straight from global memory (the LDS-staged form measured
     // slower here: 18.1 vs 14.0 us on the 9-point 512^2 operator)
-    hipLaunchKernelGGL((gmb::k_bspmm<1, gmb::kOnly>), dim3(f->G), dim3(gm::kBlock), 0, f->stream, a);
+    hipLaunchKernelGGL((gmb::k_bspmm<1, gmb::kOnly>), dim3(f->G), dim3(gm::kBlock), 0, s.st, a);
     A_HIP(hipGetLastError());
-    if (!fuse) A_TRY(areduce(f, f->G, 1, s.al + k, 0, 0));
+    if (!fuse) A_TRY(areduce_s(f, s, f->G, 1, s.al + k, 0, 0));
     gmb::BFinish fn{};
     fn.w = Wb[k & 1];
     fn.vk = a.vk;
@@ -476,17 +511,17 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
     fn.k = k;
     fn.nrows = f->n;
     fn.chunk = f->chunk;
-    fn.partials = fuse ? f->partials2 : f->partials;
+    fn.partials = fuse ? s.partials2 : s.partials;
     fn.status = f->status;
     if (fuse) {
-      fn.apart = f->partials;
+      fn.apart = s.partials;
       fn.aG = f->G;
       fn.al_out = s.al;
     }
-    hipLaunchKernelGGL((gmb::k_bfinish<1>), dim3(f->G), dim3(gm::kBlock), 0, f->stream, fn);
+    hipLaunchKernelGGL((gmb::k_bfinish<1>), dim3(f->G), dim3(gm::kBlock), 0, s.st, fn);
     A_HIP(hipGetLastError());
     if (!fuse || k == f->m - 1)
-      A_TRY(areduce(f, f->G, 1, s.be + k + 1, 1, 1, fuse ? f->partials2 : nullptr));
+      A_TRY(areduce_s(f, s, f->G, 1, s.be + k + 1, 1, 1, fuse ? s.partials2 : nullptr));
   }
   gmb::BSolve sv{};
   sv.al = s.al;
@@ -497,14 +532,14 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
   sv.wts1 = f->ones;
   sv.wts2 = f->ones;
   sv.etaxi = f->xi;
-  sv.wD = f->wD;
-  sv.wDU = f->wDU;
-  sv.wDU2 = f->wDU2;
-  sv.wX = f->wX;
+  sv.wD = s.wD;
+  sv.wDU = s.wDU;
+  sv.wDU2 = s.wDU2;
+  sv.wX = s.wX;
   sv.yr = s.yr;
   sv.yi = s.yi;
   sv.status = f->status;
-  hipLaunchKernelGGL(gmb::k_bsolves<0>, dim3(adiv(f->ns, 64)), dim3(64), 0, f->stream, sv);
+  hipLaunchKernelGGL(gmb::k_bsolves<0>, dim3(adiv(f->ns, 64)), dim3(64), 0, s.st, sv);
   A_HIP(hipGetLastError());
   if (want_out) {
     gmb::BCollapse cc{};
@@ -517,7 +552,7 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
     cc.ns = f->ns;
     cc.nd = 1;
     cc.C = s.C;
-    hipLaunchKernelGGL(gmb::k_bcollapse<0>, dim3(adiv(f->m, 256)), dim3(256), 0, f->stream, cc);
+    hipLaunchKernelGGL(gmb::k_bcollapse<0>, dim3(adiv(f->m, 256)), dim3(256), 0, s.st, cc);
     A_HIP(hipGetLastError());
     gmb::BVC v{};
     v.V = s.V;
@@ -529,13 +564,45 @@ gm_status krylov(gm_fab* f, Side& s, bool want_out) {
     v.Z = s.out;
     v.ldz = 1;
     v.accumulate = 0;
-    hipLaunchKernelGGL((gmb::k_bvc<1>), dim3(adiv(f->n, 256)), dim3(256), 0, f->stream, v);
+    hipLaunchKernelGGL((gmb::k_bvc<1>), dim3(adiv(f->n, 256)), dim3(256), 0, s.st, v);
     A_HIP(hipGetLastError());
   }
   return GM_OK;
 }
 
 // side.Xc = V * Ytilde (N x ncol) on FP64 MFMA.
+// The backward's recomputed b-side run on the second stream, after the engine
+// stream's earlier work, with the b side's own partials and solve scratch;
+// join_b makes the engine stream wait for it.
+gm_status recompute_b(gm_fab* f) {
+  A_HIP(hipEventRecord(f->ev1, f->stream));
+  A_HIP(hipStreamWaitEvent(f->stream2, f->ev1, 0));
+  Side& s = f->sb;
+  s.st = f->stream2;
+  s.partials = f->bpart;
+  s.partials2 = f->bpart2;
+  s.wD = f->bwD;
+  s.wDU = f->bwDU;
+  s.wDU2 = f->bwDU2;
+  s.wX = f->bwX;
+  const gm_status rc = krylov(f, s, false);
+  s.st = f->stream;
+  s.partials = f->partials;
+  s.partials2 = f->partials2;
+  s.wD = f->wD;
+  s.wDU = f->wDU;
+  s.wDU2 = f->wDU2;
+  s.wX = f->wX;
+  if (rc != GM_OK) return rc;
+  A_HIP(hipEventRecord(f->ev2, f->stream2));
+  return GM_OK;
+}
+
+gm_status join_b(gm_fab* f) {
+  A_HIP(hipStreamWaitEvent(f->stream, f->ev2, 0));
+  return GM_OK;
+}
+
 gm_status resolvent_block(gm_fab* f, Side& s) {
   hipLaunchKernelGGL(gma::k_pack_y<0>, dim3(adiv((long long)f->m * f->ncol, 256)), dim3(256), 0,
                      f->stream, s.yr, s.yi, f->m, f->ns, f->ncol, s.Yg);
@@ -779,6 +846,25 @@ gm_status gm_fab_create(const gm_fab_desc* d, gm_fab** out) {
   }
   A_CREATE(alloc_side(f, f->sb));
   A_CREATE(alloc_side(f, f->sg));
+  for (Side* sd : {&f->sb, &f->sg}) {
+    sd->st = f->stream;
+    sd->partials = f->partials;
+    sd->partials2 = f->partials2;
+    sd->wD = f->wD;
+    sd->wDU = f->wDU;
+    sd->wDU2 = f->wDU2;
+    sd->wX = f->wX;
+  }
+  A_CREATE(aalloc(&f->bpart, (size_t)4096 * std::max(2, f->ncol)));
+  A_CREATE(aalloc(&f->bpart2, (size_t)4096));
+  A_CREATE(aalloc(&f->bwD, ny));
+  A_CREATE(aalloc(&f->bwDU, ny));
+  A_CREATE(aalloc(&f->bwDU2, ny));
+  A_CREATE(aalloc(&f->bwX, ny));
+  if (hipStreamCreateWithFlags(&f->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&f->ev1, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&f->ev2, hipEventDisableTiming) != hipSuccess)
+    return bail(afail(GM_EDEVICE, "second stream/events"));
   if (hipStreamSynchronize(st) != hipSuccess) return bail(afail(GM_EDEVICE, "sync"));
   {
     std::lock_guard<std::mutex> lk(g_amu);
@@ -796,6 +882,7 @@ gm_status gm_fab_delete(gm_fab* f) {
     g_areg.erase(f);
   }
   (void)hipSetDevice(f->dev);
+  if (f->stream2) (void)hipStreamSynchronize(f->stream2);
   if (f->stream) (void)hipStreamSynchronize(f->stream);
   free_fab(f);
   delete f;
@@ -862,6 +949,10 @@ gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, doubl
   A_HIP(hipSetDevice(f->dev));
   const size_t N = (size_t)f->n;
   hipStream_t st = f->stream;
+  // with recompute, V_b is rebuilt (deterministic) on the second stream while
+  // the z side runs on the engine stream
+  const bool rb = f->recompute && (grad_kappa || grad_alpha);
+  if (rb) A_TRY(recompute_b(f));
   A_HIP(hipMemcpyAsync(f->sg.U, g, N * sizeof(double), hipMemcpyDefault, st));
   // z side: f(A) g is dL/db
   if (f->K) A_TRY(deflate(f, f->sg));
@@ -870,7 +961,7 @@ gm_status gm_fab_backward(gm_fab* f, const double* g, const double* kappa, doubl
   if (grad_b)
     A_HIP(hipMemcpyAsync(grad_b, f->sg.out, N * sizeof(double), hipMemcpyDefault, st));
   if (grad_kappa || grad_alpha) {
-    if (f->recompute) A_TRY(krylov(f, f->sb, false));   // rebuild V_b (deterministic)
+    if (rb) A_TRY(join_b(f));
     A_TRY(resolvent_block(f, f->sb));
   }
   if (grad_kappa) {
